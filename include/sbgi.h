@@ -1,0 +1,305 @@
+/*
+ * sbgi.h — C ABI of the MI355X-native DDGI probe update.
+ *
+ * This is the boundary that replaces the reference's WebGPU submission path:
+ * three.js StorageBufferAttribute uploads plus renderer.computeAsync over
+ * [traceKernel, blendKernel, uploadKernel] (js/gi_probes.js:1484-1561,
+ * 3812-3836), the ranged uploads of markStorageDirty/addUpdateRange
+ * (js/gi_probes.js:1543-1561, 4239-4251), the CPU refit lanes
+ * (js/gi_refit.js:34-430, js/spectral_scene.js:1402-1514, 1831-1974) and the
+ * debug readback getArrayBufferAsync (js/gi_probes.js:4830-4841).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int32_t status: 0 on success, a negative SBGI_E_* code on
+ * failure (the message is available from sbgi_last_error). Host arrays are
+ * borrowed for the duration of the call only; device memory is owned by the
+ * sbgi_field handle.
+ *
+ * Buffers passed to sbgi_set_scene use the reference byte layouts unchanged:
+ *   bvhNodes     u32[8*N]  [f32 min.xyz, f32 max.xyz, u32 miss, u32 payload]
+ *                          payload 0xFFFFFFFF = interior, else
+ *                          (count << 24) | pool-absolute triangle offset
+ *                          (js/spectral_scene.js:1225-1234)
+ *   triIndex     u32[3*T]  pool-absolute vertex ids (js/spectral_scene.js:1218-1224)
+ *   vertexData   f32[8*V]  pos.xyz normal.xyz uv.xy, local space
+ *   triMaterial  u32[T]    uber record index
+ *   materials    f32       uber records x28 | instance records x28 at inst_base
+ *                          | TLAS nodes x12 at tlas_base (js/spectral_scene.js:1308-1366)
+ *   lights       f32[17*L] light records (js/spectral_scene.js:723-813)
+ */
+#ifndef SBGI_H
+#define SBGI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define SBGI_OK 0
+#define SBGI_E_INVALID -1   /* bad argument (null pointer, size out of range)      */
+#define SBGI_E_LAYOUT -2    /* threaded BVH layout violated; caller must rebuild   */
+#define SBGI_E_CAPACITY -3  /* scene exceeds a 24-bit index ceiling or MAX_LIGHTS  */
+#define SBGI_E_DEVICE -4    /* HIP runtime error                                   */
+#define SBGI_E_STATE -5     /* call out of order (no scene, tick in flight ...)    */
+#define SBGI_E_TIMEOUT -6   /* sbgi_wait timed out                                 */
+#define SBGI_E_NOMEM -7     /* device or host allocation failed                    */
+
+/* ---- reference constants (js/gi_probes.js:53-56, 97, 132; spectral_traverse.js:34-35) */
+#define SBGI_OCT_RES 6
+#define SBGI_BORDER 1
+#define SBGI_TILE 8
+#define SBGI_RAYS_MIN 32
+#define SBGI_RAYS_MAX 256
+#define SBGI_MAX_LIGHTS 64
+#define SBGI_LIGHT_STRIDE 17
+#define SBGI_MAT_STRIDE 28
+#define SBGI_TLAS_STRIDE 12
+#define SBGI_MAX_CASCADES 4
+
+/* blend arithmetic */
+#define SBGI_BLEND_FP32 0   /* plain FP32 vector ALU                                   */
+#define SBGI_BLEND_BF16 1   /* BF16 MFMA inputs, FP32 accumulate                         */
+#define SBGI_BLEND_BF16X3 2 /* BF16 hi/lo split on both operands (3 MFMAs), FP32 acc.  */
+#define SBGI_BLEND_FP16 3   /* FP16 MFMA inputs, FP32 accumulate                         */
+
+/* closest-hit traversal */
+#define SBGI_TRAV_MT_COMPAT 0   /* stackless left-first threaded walk + Moller-Trumbore,
+                                   the reference's visit order and epsilons */
+#define SBGI_TRAV_MT_ORDERED 1  /* nearest-child-first short-stack walk, same triangle test */
+#define SBGI_TRAV_WATERTIGHT 2  /* ordered walk + watertight triangle test (Woop et al. 2013) */
+
+/* buffer ids for sbgi_read / sbgi_write / sbgi_update_ranges */
+#define SBGI_BUF_BVH_NODES 0    /* u32[8*N]                 (update_ranges)           */
+#define SBGI_BUF_TRI_INDEX 1    /* u32[3*T]                 (update_ranges)           */
+#define SBGI_BUF_VERTEX_DATA 2  /* f32[8*V]                 (update_ranges)           */
+#define SBGI_BUF_TRI_MATERIAL 3 /* u32[T]                   (update_ranges)           */
+#define SBGI_BUF_MATERIALS 4    /* f32 materials|inst|tlas  (update_ranges, read)     */
+#define SBGI_BUF_LIGHTS 5       /* f32[17*L]                (update_ranges, read)     */
+#define SBGI_BUF_IRR_STATE 10   /* f32[4*P*64]    per cascade (read, write)            */
+#define SBGI_BUF_DEPTH_STATE 11 /* f32[2*P*TD*TD] per cascade (read, write)            */
+#define SBGI_BUF_META 12        /* f32[4*P]       per cascade (read, write)            */
+#define SBGI_BUF_IRR_ATLAS 13   /* u16 RGBA16F [(rx*8) x (ry*rz*8)]      (read)        */
+#define SBGI_BUF_DEPTH_ATLAS 14 /* u16 RGBA16F [(rx*TD) x (ry*rz*TD)]    (read)        */
+#define SBGI_BUF_RAY_DATA 15    /* f32[4*updated*R] reference order (slot*R+k)  (read) */
+#define SBGI_BUF_HIT_IDS 16     /* i32[2*updated*R] (tri, inst) per ray, same order    */
+#define SBGI_BUF_NODES_REFIT 17 /* bvhNodes after a device refit (read)               */
+
+/* ---- field description ---------------------------------------------------- */
+typedef struct sbgi_cascade_desc {
+    float grid_min[3];  /* world-space grid origin (U.gridMin)                       */
+    float grid_size[3]; /* world-space extent (U.gridSize)                          */
+    uint32_t res[3];    /* probes per axis (U.resX/Y/Z)                             */
+    uint32_t pad;
+} sbgi_cascade_desc;
+
+typedef struct sbgi_field_desc {
+    int32_t device;              /* HIP device ordinal                               */
+    uint32_t rays_per_probe;     /* R, multiple of 16 in [32, 256]                   */
+    uint32_t cascade_count;      /* 1..SBGI_MAX_CASCADES                              */
+    uint32_t depth_oct_res;      /* 6: depth shares the 8x8 tile (reference);
+                                    14: 16x16 depth tile                             */
+    uint32_t blend_precision;    /* SBGI_BLEND_*                                     */
+    uint32_t traversal;          /* SBGI_TRAV_*                                      */
+    uint32_t flags;              /* SBGI_FIELD_* bits                                */
+    uint32_t pad;
+    sbgi_cascade_desc cascades[SBGI_MAX_CASCADES];
+} sbgi_field_desc;
+
+#define SBGI_FIELD_KEEP_HIT_IDS 1u /* store per-ray (tri, inst) for parity checks */
+
+/* ---- scene view (reference layouts, borrowed host pointers) ---------------- */
+typedef struct sbgi_scene_view {
+    const uint32_t *bvh_nodes;
+    uint32_t node_count;
+    uint32_t tri_count;
+    const uint32_t *tri_index;
+    const float *vertex_data;
+    uint32_t vertex_count;
+    uint32_t materials_len;      /* floats in materials                              */
+    const uint32_t *tri_material;
+    const float *materials;
+    uint32_t inst_base;          /* float index of instance record 0                 */
+    uint32_t inst_count;
+    uint32_t tlas_base;          /* float index of TLAS node 0                       */
+    uint32_t tlas_node_count;
+    const float *lights;         /* 17 floats per light                              */
+    uint32_t light_count;
+    uint32_t pad;
+} sbgi_scene_view;
+
+/* ---- per-solve parameters: the uniform set U of js/gi_probes.js:1198-1255 ---- */
+typedef struct sbgi_tick_params {
+    uint32_t cascade;            /* which cascade to solve                           */
+    uint32_t probe_offset;       /* U.probeOffset (round-robin cursor)               */
+    uint32_t updated_count;      /* U.updatedCount; 0 means the whole cascade        */
+    uint32_t needs_clear;        /* clear atlases first (C.needsClear)               */
+    float frame_jitter;          /* U.frameJitter                                    */
+    float hysteresis;            /* U.hysteresis                                     */
+    float hysteresis_exponent;   /* U.hysteresisExponent                             */
+    float depth_sharpness;       /* U.depthSharpness (7)                             */
+    float radiance_clamp;        /* U.radianceClamp (8)                              */
+    float classify_strength;     /* U.classifyStrength (0)                           */
+    float filter_strength;       /* U.filterStrength (1)                             */
+    float filter_smooth;         /* U.filterSmooth (0.5)                             */
+    float temp_change_sigma1;    /* U.tempChangeSigma1 (2.5)                         */
+    float temp_change_h_drop;    /* U.tempChangeHDrop (0.30)                         */
+    float temp_clamp_sigma;      /* U.tempClampSigma (6)                             */
+    float nir_gate;              /* U.nirGate (0)                                    */
+    float nir_gain;              /* U.nirGain (1)                                    */
+    float sky_intensity;         /* U.skyIntensity (1)                               */
+    float sky_sh[27];            /* U.skySH[0..8] as xyz triples (zero = black sky)  */
+    float dbg_trace_bias;        /* U.debugTraceBiasScale (1)                        */
+    float dbg_ray_eps;           /* U.debugRayEpsScale (1)                           */
+    float dbg_direct;            /* U.debugDirectScale (1)                           */
+    float dbg_emissive;          /* U.debugEmissiveScale (1)                         */
+    float dbg_albedo;            /* U.debugAlbedoScale (1)                           */
+    float dbg_bounce;            /* U.debugBounceScale (1)                           */
+    float dbg_cosine_power;      /* U.debugCosinePower (1)                           */
+    float dbg_noise_h_boost;     /* U.debugTempNoiseHBoost (0.25)                    */
+    float dbg_change_sigma0;     /* U.debugTempChangeSigma0 (0.75)                   */
+    float dbg_min_change_h;      /* U.debugTempMinChangeH (0.55)                     */
+    float dbg_var_eps;           /* U.debugTempVarEps (2.5e-5)                       */
+    float dbg_var_rel;           /* U.debugTempVarRel (2.5e-3)                       */
+    float dbg_depth_history;     /* U.debugDepthHistoryScale (1)                     */
+    float dbg_filter_k;          /* U.debugFilterKScale (1)                          */
+    float dbg_filter_rel;        /* U.debugFilterRelScale (1)                        */
+    float dbg_filter_eps;        /* U.debugFilterEpsScale (1)                        */
+} sbgi_tick_params;
+
+/* per-stage device time of the last completed tick, from HIP events */
+typedef struct sbgi_stats {
+    float ms_raydir;
+    float ms_trace;
+    float ms_blend;
+    float ms_upload;
+    float ms_total;
+    uint32_t rays;               /* primary rays traced                               */
+    uint32_t updated;            /* probes updated                                    */
+    uint64_t shadow_rays;        /* any-hit rays launched (when counting is enabled)  */
+    uint64_t blas_nodes;         /* BLAS node visits (counting builds)                */
+    uint64_t tlas_nodes;         /* TLAS node visits                                  */
+    uint64_t tri_tests;          /* triangle tests                                    */
+    uint64_t inst_visits;        /* instance entries                                  */
+} sbgi_stats;
+
+typedef struct sbgi_field sbgi_field;
+
+/* Fill *p with the reference defaults (js/gi_probes.js:1198-1255, gated jitter mode). */
+void sbgi_tick_params_default(sbgi_tick_params *p);
+
+/* Library version string. */
+const char *sbgi_version(void);
+
+/* createProbeField + buildKernels resource allocation (js/gi_probes.js:945-1046,
+ * 1732-1867): allocates per-cascade state, atlases and ray scratch. */
+int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out);
+
+/* prepareSharedResources (js/gi_probes.js:1484-1561, 1594): copies the scene in the
+ * reference layouts to device memory and derives the packed traversal layouts.
+ * Fails with SBGI_E_CAPACITY for >=2^24 triangles/nodes/instances or >64 lights
+ * (js/spectral_scene.js:1186, 1196-1203; js/gi_probes.js:132). */
+int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *scene);
+
+/* copySceneStorageRanges / markStorageDirty (js/gi_probes.js:1543-1561, 4239-4251):
+ * ranges holds n_ranges (start, count) pairs in ELEMENTS of buffer buf_id; the data
+ * is taken from the same offsets of the full host array src. */
+int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges,
+                           uint32_t n_ranges, const void *src);
+
+/* refitFlatBlasRange (js/gi_refit.js:158-165) on the device for each [root, end)
+ * pair, then refitFlatTlasRange (js/gi_refit.js:245-292) over the instance world
+ * bounds when refit_tlas != 0. Returns SBGI_E_LAYOUT when the threaded layout is
+ * violated, as the reference's refit returns false. */
+int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges,
+                   int32_t refit_tlas);
+
+/* updateTransforms (js/spectral_scene.js:1402-1514): rewrite the instance records of
+ * the given TLAS slots from column-major world matrices (16 doubles each), recompute
+ * their world AABBs from the BLAS root bounds and refit the TLAS on the device. */
+int32_t sbgi_set_instance_transforms(sbgi_field *f, const uint32_t *slots, uint32_t n,
+                                     const double *world_matrices, const uint8_t *active);
+
+/* One solve of one cascade: trace -> blend -> upload, the ordered pass of
+ * renderer.computeAsync(gpu.solveKernels) (js/gi_probes.js:3729-3836). Enqueues on the
+ * field's stream and returns; *ticket identifies the submission for sbgi_wait/poll. */
+int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket);
+
+/* Block until the ticket has completed (timeout_ms < 0 waits forever). */
+int32_t sbgi_wait(sbgi_field *f, uint64_t ticket, int32_t timeout_ms);
+
+/* 1 when the ticket has completed, 0 when pending, negative on error. */
+int32_t sbgi_poll(sbgi_field *f, uint64_t ticket);
+
+/* _debugRead (js/gi_probes.js:4830-4841): synchronous device->host copy of a buffer.
+ * cascade selects the per-cascade buffers. bytes must equal the buffer size. */
+int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, size_t bytes);
+
+/* Restore per-cascade state (irr/depth/meta) from host memory (test fixtures). */
+int32_t sbgi_write(sbgi_field *f, int32_t buf_id, uint32_t cascade, const void *src,
+                   size_t bytes);
+
+/* Byte size of a buffer as sbgi_read expects it (0 when not allocated). */
+size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade);
+
+/* Device time of each stage of the last waited tick. */
+int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out);
+
+/* Counting mode (no reference counterpart; the measurement hook for the algorithmic
+ * byte accounting): while on, ticks run an instrumented trace that accumulates node
+ * visits, triangle tests, instance entries and shadow rays into sbgi_stats. Turning it
+ * on or off resets the counters. */
+int32_t sbgi_set_counting(sbgi_field *f, int32_t on);
+
+/* Ray queries against the resident scene: the closest-hit / any-hit walks that the
+ * trace uses (traverseClosest / traverseAny, js/spectral_traverse.js:441-474, 539-569),
+ * for arbitrary rays. rays8 holds n records {ro.x, ro.y, ro.z, tmax, rd.x, rd.y, rd.z,
+ * mode} with mode 0 = closest hit, 1 = any hit within (1e-3, tmax). out4 receives
+ * {t, tri (int32 bits), inst (int32 bits), blocked} per ray. Synchronous. */
+int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *out4);
+
+/* Last error message of this field (or of the last failed sbgi_create when f is NULL). */
+const char *sbgi_last_error(const sbgi_field *f);
+
+/* Release all device memory. Idempotent on NULL. */
+int32_t sbgi_destroy(sbgi_field *f);
+
+/* ---- scene builder: buildSpectralScene's BLAS/pool/TLAS assembly ---------------
+ * (js/spectral_scene.js:898-1003, 1188-1366). BLASes are built in local space by a
+ * binned-SAH builder of this library (in place of three-mesh-bvh's MeshBVH, leaf
+ * size <= 8) and flattened to the threaded layout; the TLAS is the reference's
+ * median split with stable centroid sort (buildTlasRecords), leaf size 2. */
+typedef struct sbgi_scene_builder sbgi_scene_builder;
+
+int32_t sbgi_sb_create(sbgi_scene_builder **out);
+/* Append one uber material record (28 floats); returns its index in *id. */
+int32_t sbgi_sb_add_material(sbgi_scene_builder *b, const float *rec28, uint32_t *id);
+/* Add one local-space mesh and build its BLAS. normals/uvs may be NULL.
+ * tri_material gives the uber index of every triangle (NULL = all 0). */
+int32_t sbgi_sb_add_mesh(sbgi_scene_builder *b, const float *positions, uint32_t vcount,
+                         const float *normals, const float *uvs, const uint32_t *indices,
+                         uint32_t tcount, const uint32_t *tri_material, uint32_t *mesh_id);
+/* Add one instance of a mesh; world is a column-major 4x4 (three.js Matrix4.elements). */
+int32_t sbgi_sb_add_instance(sbgi_scene_builder *b, uint32_t mesh_id, const double *world16,
+                             int32_t active, uint32_t *instance_id);
+int32_t sbgi_sb_add_light(sbgi_scene_builder *b, const float *rec17);
+/* Assemble pool, instance table and TLAS; *view points into builder-owned arrays
+ * that stay valid until the builder is changed or destroyed. */
+int32_t sbgi_sb_finalize(sbgi_scene_builder *b, sbgi_scene_view *view);
+/* slot -> source instance id (the TLAS order), n = instance count */
+int32_t sbgi_sb_tlas_order(const sbgi_scene_builder *b, uint32_t *order, uint32_t n);
+/* Replace the world matrix of a source instance and recompute the dynamic tail on the
+ * host (updateTransforms): instance rows, det sign, world AABB, sparse TLAS refit
+ * (refitFlatTlasDirty, js/gi_refit.js:352-430). Writes the TLAS slot of the instance. */
+int32_t sbgi_sb_set_transform(sbgi_scene_builder *b, uint32_t instance_id,
+                              const double *world16, uint32_t *slot);
+const char *sbgi_sb_last_error(const sbgi_scene_builder *b);
+int32_t sbgi_sb_destroy(sbgi_scene_builder *b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBGI_H */

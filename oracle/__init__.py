@@ -1,0 +1,197 @@
+"""TEST INFRASTRUCTURE ONLY — ctypes front end of the CPU oracle (oracle/_build/liboracle.so).
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker. The product (speedball_amd) never imports it.
+
+`prec` selects the FP64 oracle ("64", arrays of float64) or the FP32 restatement
+("32", arrays of float32) of the same functions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _lib.half_round.restype = C.c_float
+        _lib.half_round.argtypes = [C.c_float]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _dt(prec):
+    return np.float64 if prec == "64" else np.float32
+
+
+def _fn(prec, name):
+    return getattr(lib(), f"or{prec}_{name}")
+
+
+def half_round(x: float) -> float:
+    return lib().half_round(C.c_float(x))
+
+
+def ray_dir(k: int, rays: int, jitter: float, prec="64") -> np.ndarray:
+    out = np.zeros(3, _dt(prec))
+    f = _fn(prec, "ray_dir")
+    f(C.c_uint32(k), C.c_uint32(rays), C.c_float(jitter), _p(out))
+    return out
+
+
+def oct_encode(n, prec="64") -> np.ndarray:
+    n = np.ascontiguousarray(n, _dt(prec))
+    out = np.zeros(2, _dt(prec))
+    _fn(prec, "oct_encode")(_p(n), _p(out))
+    return out
+
+
+def oct_decode(u, v, prec="64") -> np.ndarray:
+    out = np.zeros(3, _dt(prec))
+    R = C.c_double if prec == "64" else C.c_float
+    f = _fn(prec, "oct_decode")
+    f.argtypes = [R, R, C.c_void_p]
+    f(R(u), R(v), _p(out))
+    return out
+
+
+def cascade_desc(grid_min, grid_size, res):
+    from speedball_amd._native import CascadeDesc
+    c = CascadeDesc()
+    for a in range(3):
+        c.grid_min[a] = grid_min[a]
+        c.grid_size[a] = grid_size[a]
+        c.res[a] = res[a]
+    return c
+
+
+def closest(scene, ro, rd, prec="64", count=False):
+    ro = np.ascontiguousarray(ro, _dt(prec))
+    rd = np.ascontiguousarray(rd, _dt(prec))
+    t = (C.c_double if prec == "64" else C.c_float)()
+    tri, inst = C.c_int32(), C.c_int32()
+    cnt = np.zeros(4, np.uint64) if count else None
+    _fn(prec, "closest")(C.byref(scene.view()), _p(ro), _p(rd), C.byref(t), C.byref(tri), C.byref(inst), _p(cnt))
+    return (t.value, tri.value, inst.value, cnt) if count else (t.value, tri.value, inst.value)
+
+
+def any_hit(scene, ro, rd, maxd, prec="64") -> bool:
+    ro = np.ascontiguousarray(ro, _dt(prec))
+    rd = np.ascontiguousarray(rd, _dt(prec))
+    R = C.c_double if prec == "64" else C.c_float
+    f = _fn(prec, "any")
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, R, C.c_void_p]
+    return bool(f(C.byref(scene.view()), _p(ro), _p(rd), R(maxd), None))
+
+
+def brute_closest(scene, ro, rd, prec="64"):
+    ro = np.ascontiguousarray(ro, _dt(prec))
+    rd = np.ascontiguousarray(rd, _dt(prec))
+    R = C.c_double if prec == "64" else C.c_float
+    t, t2 = R(), R()
+    tri, inst = C.c_int32(), C.c_int32()
+    _fn(prec, "brute_closest")(C.byref(scene.view()), _p(ro), _p(rd), C.byref(t), C.byref(tri), C.byref(inst),
+                               C.byref(t2))
+    return t.value, tri.value, inst.value, t2.value
+
+
+class OracleField:
+    """FP64 (or FP32) oracle state of one cascade, mirroring the device buffers."""
+
+    def __init__(self, scene, grid_min, grid_size, res, rays, depth_oct=6, prec="64"):
+        self.scene = scene
+        self.prec = prec
+        self.dt = _dt(prec)
+        self.c = cascade_desc(grid_min, grid_size, res)
+        self.res = tuple(res)
+        self.rays = rays
+        self.depth_oct = depth_oct
+        P = res[0] * res[1] * res[2]
+        self.P = P
+        td = 8 if depth_oct == 6 else depth_oct + 2
+        self.irr = np.zeros(P * 64 * 4, self.dt)
+        self.depth = np.zeros(P * (64 if depth_oct == 6 else td * td) * 2, self.dt)
+        self.meta = np.zeros(P * 4, np.float32)
+        aw, ah = res[0] * 8, res[1] * res[2] * 8
+        self.atlas = np.zeros(aw * ah * 4, self.dt)
+        self.atlas[3::4] = 1
+        dw, dh = (aw, ah) if depth_oct == 6 else (res[0] * td, res[1] * res[2] * td)
+        self.datlas = np.zeros(dw * dh * 4, self.dt)
+        self.datlas[3::4] = 1
+        self.ray_data = None
+        self.hit_ids = None
+
+    def params(self, **kw):
+        from speedball_amd._native import default_tick_params
+        return default_tick_params(**kw)
+
+    def trace(self, p, updated=None, count=False):
+        n = (updated or self.P) * self.rays
+        self.ray_data = np.zeros(n * 4, self.dt)
+        self.hit_ids = np.zeros(n * 2, np.int32)
+        cnt = np.zeros(5, np.uint64) if count else None
+        _fn(self.prec, "trace")(C.byref(self.scene.view()), C.byref(self.c), C.c_uint32(self.rays), C.byref(p),
+                                _p(self.atlas), _p(self.meta), _p(self.ray_data), _p(self.hit_ids), _p(cnt))
+        return cnt
+
+    def blend(self, p):
+        _fn(self.prec, "blend")(C.byref(self.c), C.c_uint32(self.rays), C.c_uint32(self.depth_oct), C.byref(p),
+                                _p(self.ray_data), _p(self.irr), _p(self.depth))
+
+    def upload(self, p):
+        _fn(self.prec, "upload")(C.byref(self.c), C.c_uint32(self.depth_oct), C.byref(p), _p(self.irr),
+                                 _p(self.depth), _p(self.atlas), _p(self.datlas))
+
+    def update(self, p):
+        self.trace(p, p.updated_count or None)
+        self.blend(p)
+        self.upload(p)
+
+    # reshaped views
+    def irr_view(self):
+        return self.irr.reshape(self.P, 64, 4)
+
+    def depth_view(self):
+        td = 64 if self.depth_oct == 6 else (self.depth_oct + 2) ** 2
+        return self.depth.reshape(self.P, td, 2)
+
+    def atlas_view(self, depth=False):
+        tile = 8 if (not depth or self.depth_oct == 6) else self.depth_oct + 2
+        a = self.datlas if depth else self.atlas
+        return a.reshape(self.res[1] * self.res[2] * tile, self.res[0] * tile, 4)
+
+    def rays_view(self):
+        return self.ray_data.reshape(-1, self.rays, 4)
+
+    def hits_view(self):
+        return self.hit_ids.reshape(-1, self.rays, 2)
+
+
+def refit_blas(nodes: np.ndarray, scene, root: int, end: int) -> bool:
+    f = lib().or_refit_blas
+    return bool(f(_p(nodes), C.c_uint32(nodes.size // 8), _p(scene.tri_index), C.c_uint32(scene.tri_count),
+                  _p(scene.vertex_data), C.c_uint32(scene.vertex_data.size // 8), C.c_uint32(root), C.c_uint32(end)))
+
+
+def refit_tlas(tlas: np.ndarray, inst_bounds: np.ndarray) -> bool:
+    f = lib().or_refit_tlas
+    return bool(f(_p(tlas), C.c_uint32(tlas.size // 12), _p(inst_bounds), C.c_uint32(inst_bounds.size // 6)))

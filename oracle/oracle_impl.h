@@ -1,0 +1,869 @@
+/*
+ * oracle_impl.h — CPU restatement of the reference DDGI probe update.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the parity oracle: it is compiled into
+ * oracle/_build/liboracle.so and loaded by tests/, by __graft_entry__.smoke() and by
+ * bench.py's cpu_baseline leg, only ever as the checker. Nothing in speedball_amd/
+ * links, loads or calls it.
+ *
+ * It is included twice by sbgi_oracle.c: once with REAL=double (the FP64 oracle) and
+ * once with REAL=float (the FP32 restatement, same operation order as the reference's
+ * TSL graphs, compiled with -ffp-contract=off). P(name) expands to or64_name / or32_name.
+ *
+ * Every function cites the reference lines it restates. Pinning of each function
+ * (golden vectors, closed forms, brute force) is listed in oracle/README.md.
+ */
+
+#define RCONST(x) ((REAL)(x))
+
+/* ---- octahedral map: js/gi_oct.js:22-57 (JS twin), 61-85 (TSL twin) ---- */
+void P(oct_encode)(const REAL n[3], REAL uv[2]) {
+    REAL s = RABS(n[0]) + RABS(n[1]) + RABS(n[2]);
+    /* TSL form: 1 / max(|x|+|y|+|z|, 1e-20) (gi_oct.js:64) */
+    REAL inv = RCONST(1) / (s > RCONST(1e-20) ? s : RCONST(1e-20));
+    REAL ox = n[0] * inv, oy = n[1] * inv;
+    if (n[2] < 0) {
+        REAL fx = (RCONST(1) - RABS(oy)) * (ox >= 0 ? RCONST(1) : RCONST(-1));
+        REAL fy = (RCONST(1) - RABS(ox)) * (oy >= 0 ? RCONST(1) : RCONST(-1));
+        ox = fx;
+        oy = fy;
+    }
+    uv[0] = ox * RCONST(0.5) + RCONST(0.5);
+    uv[1] = oy * RCONST(0.5) + RCONST(0.5);
+}
+
+void P(oct_decode)(REAL u, REAL v, REAL out[3]) {
+    REAL fx = u * RCONST(2) - RCONST(1);
+    REAL fy = v * RCONST(2) - RCONST(1);
+    REAL nz = RCONST(1) - RABS(fx) - RABS(fy);
+    REAL t = -nz > 0 ? -nz : RCONST(0);
+    REAL nx = fx + (fx >= 0 ? -t : t);
+    REAL ny = fy + (fy >= 0 ? -t : t);
+    REAL len = RSQRT(nx * nx + ny * ny + nz * nz);
+    out[0] = nx / len;
+    out[1] = ny / len;
+    out[2] = nz / len;
+}
+
+static void P(normalize3)(REAL v[3]) {
+    REAL len = RSQRT(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= len;
+    v[1] /= len;
+    v[2] /= len;
+}
+
+static REAL P(dot3)(const REAL a[3], const REAL b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void P(cross3)(const REAL a[3], const REAL b[3], REAL o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* ---- rayDir: js/gi_probes.js:1692-1704, normalized at :1984 and :2331 ---- */
+void P(ray_dir)(uint32_t k, uint32_t rays, float jitter, REAL out[3]) {
+    REAL j = (REAL)jitter;
+    REAL sk = ((REAL)k + RCONST(0.5)) / (REAL)rays;
+    REAL u = sk + j;
+    REAL uw = u - RFLOOR(u);
+    REAL z = RCONST(1) - uw * RCONST(2);
+    REAL rr = RCONST(1) - z * z;
+    REAL r = RSQRT(rr > 0 ? rr : RCONST(0));
+    REAL phi = (REAL)k * RCONST(GOLDEN_ANGLE) + j * RCONST(TWO_PI);
+    out[0] = r * RCOS(phi);
+    out[1] = r * RSIN(phi);
+    out[2] = z;
+    P(normalize3)(out);
+}
+
+/* ---- probeWorldPos: js/gi_probes.js:1714-1726; probeTraceOrigin :1876-1885 ---- */
+void P(probe_pos)(const sbgi_cascade_desc *c, uint32_t p, const float *meta, float classify,
+                  REAL out[3]) {
+    uint32_t ix = p % c->res[0];
+    uint32_t iy = (p / c->res[0]) % c->res[1];
+    uint32_t iz = p / (c->res[0] * c->res[1]);
+    REAL d[3];
+    uint32_t idx[3] = {ix, iy, iz};
+    for (int a = 0; a < 3; a++) {
+        REAL den = (REAL)c->res[a] - RCONST(1);
+        if (den < 1) den = 1;
+        d[a] = (REAL)idx[a] / den;
+        out[a] = (REAL)c->grid_min[a] + d[a] * (REAL)c->grid_size[a];
+    }
+    if (meta) {
+        for (int a = 0; a < 3; a++) out[a] += (REAL)meta[p * 4 + 1 + a] * (REAL)classify;
+    }
+}
+
+/* ---- scene accessors: js/spectral_traverse.js:116-165 ---- */
+#define VSTRIDE 8u
+static void P(vert)(const sbgi_scene_view *s, uint32_t vid, REAL o[3]) {
+    const float *v = s->vertex_data + (size_t)vid * VSTRIDE;
+    o[0] = v[0];
+    o[1] = v[1];
+    o[2] = v[2];
+}
+static float P(inst_f)(const sbgi_scene_view *s, uint32_t inst, uint32_t k) {
+    return s->materials[s->inst_base + inst * 28u + k];
+}
+static void P(inst_point)(const sbgi_scene_view *s, uint32_t inst, const REAL p[3], REAL o[3]) {
+    for (int r = 0; r < 3; r++) {
+        REAL row[3] = {P(inst_f)(s, inst, r * 4), P(inst_f)(s, inst, r * 4 + 1), P(inst_f)(s, inst, r * 4 + 2)};
+        o[r] = P(dot3)(row, p) + (REAL)P(inst_f)(s, inst, r * 4 + 3);
+    }
+}
+static void P(inst_dir)(const sbgi_scene_view *s, uint32_t inst, const REAL d[3], REAL o[3]) {
+    for (int r = 0; r < 3; r++) {
+        REAL row[3] = {P(inst_f)(s, inst, r * 4), P(inst_f)(s, inst, r * 4 + 1), P(inst_f)(s, inst, r * 4 + 2)};
+        o[r] = P(dot3)(row, d);
+    }
+}
+/* (M^-1)^T n = n.x row0 + n.y row1 + n.z row2, normalized (spectral_traverse.js:155-156) */
+static void P(inst_normal)(const sbgi_scene_view *s, uint32_t inst, const REAL n[3], REAL o[3]) {
+    for (int a = 0; a < 3; a++) {
+        o[a] = (REAL)P(inst_f)(s, inst, 0 + a) * n[0] + (REAL)P(inst_f)(s, inst, 4 + a) * n[1];
+        o[a] = o[a] + (REAL)P(inst_f)(s, inst, 8 + a) * n[2];
+    }
+    P(normalize3)(o);
+}
+static float P(tlas_f)(const sbgi_scene_view *s, uint32_t n, uint32_t k) {
+    return s->materials[s->tlas_base + n * 12u + k];
+}
+static float P(mat_f)(const sbgi_scene_view *s, uint32_t m, uint32_t k) { return s->materials[m * 28u + k]; }
+
+/* materialSideAccepts js/spectral_traverse.js:193-202 */
+static int P(side_accepts)(const sbgi_scene_view *s, uint32_t m, REAL det) {
+    float side = P(mat_f)(s, m, 22);
+    int front = det > 0;
+    int front_side = side < 0.5f;
+    int back_side = side >= 0.5f && side < 1.5f;
+    int dbl = side >= 1.5f;
+    return dbl || (front_side && front) || (back_side && det < 0);
+}
+/* materialAlphaAccepts js/spectral_traverse.js:204-220 with no map bound */
+static int P(alpha_accepts)(const sbgi_scene_view *s, uint32_t m) {
+    float alpha = P(mat_f)(s, m, 10);
+    float at = P(mat_f)(s, m, 23);
+    return alpha > 1.0e-4f && (at <= 0 || alpha >= at);
+}
+
+static REAL P(fmin)(REAL a, REAL b) { return RFMIN(a, b); }
+static REAL P(fmax)(REAL a, REAL b) { return RFMAX(a, b); }
+
+/* slab test shared by both levels: spectral_traverse.js:372-377, 446-451 */
+static void P(slab)(const REAL bmin[3], const REAL bmax[3], const REAL ro[3], const REAL invd[3], REAL *tn,
+                    REAL *tf) {
+    REAL t0[3], t1[3];
+    for (int a = 0; a < 3; a++) {
+        t0[a] = (bmin[a] - ro[a]) * invd[a];
+        t1[a] = (bmax[a] - ro[a]) * invd[a];
+    }
+    *tn = P(fmax)(P(fmax)(P(fmin)(t0[0], t1[0]), P(fmin)(t0[1], t1[1])), P(fmin)(t0[2], t1[2]));
+    *tf = P(fmin)(P(fmin)(P(fmax)(t0[0], t1[0]), P(fmax)(t0[1], t1[1])), P(fmax)(t0[2], t1[2]));
+}
+
+/* Moller-Trumbore as in spectral_traverse.js:388-409. Returns 1 and fills t/det when the
+ * barycentric tests pass. */
+static int P(mt)(const sbgi_scene_view *s, uint32_t tri, const REAL ro[3], const REAL rd[3], REAL *t_out,
+                 REAL *det_out) {
+    REAL p0[3], p1[3], p2[3], e1[3], e2[3], pv[3], tv[3], qv[3];
+    P(vert)(s, s->tri_index[tri * 3 + 0], p0);
+    P(vert)(s, s->tri_index[tri * 3 + 1], p1);
+    P(vert)(s, s->tri_index[tri * 3 + 2], p2);
+    for (int a = 0; a < 3; a++) {
+        e1[a] = p1[a] - p0[a];
+        e2[a] = p2[a] - p0[a];
+    }
+    P(cross3)(rd, e2, pv);
+    REAL det = P(dot3)(e1, pv);
+    if (!(RABS(det) > RCONST(1e-12))) return 0;
+    REAL inv_det = RCONST(1) / det;
+    for (int a = 0; a < 3; a++) tv[a] = ro[a] - p0[a];
+    REAL u = P(dot3)(tv, pv) * inv_det;
+    if (!(u >= 0 && u <= 1)) return 0;
+    P(cross3)(tv, e1, qv);
+    REAL v = P(dot3)(rd, qv) * inv_det;
+    if (!(v >= 0 && u + v <= 1)) return 0;
+    *t_out = P(dot3)(e2, qv) * inv_det;
+    *det_out = det;
+    return 1;
+}
+
+/* blasClosest js/spectral_traverse.js:354-435 */
+static void P(blas_closest)(const sbgi_scene_view *s, uint32_t inst, const REAL wro[3], const REAL wrd[3],
+                            REAL *best_t, int32_t *best_tri, int32_t *best_inst, uint64_t *cnt) {
+    REAL ro[3], rd[3], invd[3];
+    P(inst_point)(s, inst, wro, ro);
+    P(inst_dir)(s, inst, wrd, rd);
+    REAL dsign = P(inst_f)(s, inst, 14);
+    for (int a = 0; a < 3; a++) invd[a] = RCONST(1) / rd[a];
+    uint32_t end = (uint32_t)P(inst_f)(s, inst, 13);
+    uint32_t cur = (uint32_t)P(inst_f)(s, inst, 12);
+    for (uint32_t it = 0; it < s->node_count; it++) {
+        if (cur >= end) break;
+        const uint32_t *nd = s->bvh_nodes + (size_t)cur * 8;
+        REAL bmin[3], bmax[3], tn, tf;
+        for (int a = 0; a < 3; a++) {
+            float lo, hi;
+            memcpy(&lo, nd + a, 4);
+            memcpy(&hi, nd + 3 + a, 4);
+            bmin[a] = lo;
+            bmax[a] = hi;
+        }
+        uint32_t miss = nd[6], payload = nd[7];
+        if (cnt) cnt[0]++;
+        P(slab)(bmin, bmax, ro, invd, &tn, &tf);
+        if (tf >= P(fmax)(tn, 0) && tn < *best_t) {
+            if (payload == 0xFFFFFFFFu) {
+                cur = cur + 1;
+            } else {
+                uint32_t off = payload & 0x00FFFFFFu, count = payload >> 24;
+                for (uint32_t i = 0; i < count; i++) {
+                    uint32_t tri = off + i;
+                    REAL th, det;
+                    if (cnt) cnt[2]++;
+                    if (!P(mt)(s, tri, ro, rd, &th, &det)) continue;
+                    if (!(th > RCONST(RAY_EPS) && th < *best_t)) continue;
+                    uint32_t m = s->tri_material[tri];
+                    uint32_t flags = (uint32_t)P(mat_f)(s, m, 26);
+                    if (!P(side_accepts)(s, m, det * dsign)) continue;
+                    int ok = (flags & 1u) ? P(alpha_accepts)(s, m) : ((flags & 2u) != 0);
+                    if (ok) {
+                        *best_t = th;
+                        *best_tri = (int32_t)tri;
+                        *best_inst = (int32_t)inst;
+                    }
+                }
+                cur = miss;
+            }
+        } else {
+            cur = miss;
+        }
+    }
+}
+
+/* traverseClosest js/spectral_traverse.js:441-474. cnt (may be NULL) accumulates
+ * {blas nodes, tlas nodes, triangle tests, instance entries}. */
+void P(closest)(const sbgi_scene_view *s, const REAL ro[3], const REAL rd[3], REAL *best_t, int32_t *best_tri,
+                int32_t *best_inst, uint64_t *cnt) {
+    REAL invd[3];
+    for (int a = 0; a < 3; a++) invd[a] = RCONST(1) / rd[a];
+    *best_t = RCONST(T_MAX);
+    *best_tri = -1;
+    *best_inst = -1;
+    uint32_t cur = 0;
+    for (uint32_t it = 0; it < s->tlas_node_count; it++) {
+        if (cur >= s->tlas_node_count) break;
+        REAL bmin[3], bmax[3], tn, tf;
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = P(tlas_f)(s, cur, a);
+            bmax[a] = P(tlas_f)(s, cur, 3 + a);
+        }
+        uint32_t miss = (uint32_t)P(tlas_f)(s, cur, 6);
+        float cntf = P(tlas_f)(s, cur, 8);
+        if (cnt) cnt[1]++;
+        P(slab)(bmin, bmax, ro, invd, &tn, &tf);
+        if (tf >= P(fmax)(tn, 0) && tn < *best_t) {
+            if (cntf < 0.5f) {
+                cur = cur + 1;
+            } else {
+                uint32_t off = (uint32_t)P(tlas_f)(s, cur, 7);
+                uint32_t n = (uint32_t)cntf;
+                for (uint32_t i = 0; i < n; i++) {
+                    uint32_t inst = off + i;
+                    if (cnt) cnt[3]++;
+                    if (P(inst_f)(s, inst, 15) > 0.5f) P(blas_closest)(s, inst, ro, rd, best_t, best_tri, best_inst, cnt);
+                }
+                cur = miss;
+            }
+        } else {
+            cur = miss;
+        }
+    }
+}
+
+/* blasAny js/spectral_traverse.js:477-536 */
+static int P(blas_any)(const sbgi_scene_view *s, uint32_t inst, const REAL wro[3], const REAL wrd[3], REAL maxd,
+                       uint64_t *cnt) {
+    REAL ro[3], rd[3], invd[3];
+    P(inst_point)(s, inst, wro, ro);
+    P(inst_dir)(s, inst, wrd, rd);
+    REAL dsign = P(inst_f)(s, inst, 14);
+    for (int a = 0; a < 3; a++) invd[a] = RCONST(1) / rd[a];
+    uint32_t end = (uint32_t)P(inst_f)(s, inst, 13);
+    uint32_t cur = (uint32_t)P(inst_f)(s, inst, 12);
+    for (uint32_t it = 0; it < s->node_count; it++) {
+        if (cur >= end) break;
+        const uint32_t *nd = s->bvh_nodes + (size_t)cur * 8;
+        REAL bmin[3], bmax[3], tn, tf;
+        for (int a = 0; a < 3; a++) {
+            float lo, hi;
+            memcpy(&lo, nd + a, 4);
+            memcpy(&hi, nd + 3 + a, 4);
+            bmin[a] = lo;
+            bmax[a] = hi;
+        }
+        uint32_t miss = nd[6], payload = nd[7];
+        if (cnt) cnt[0]++;
+        P(slab)(bmin, bmax, ro, invd, &tn, &tf);
+        if (tf >= P(fmax)(tn, 0) && tn < maxd) {
+            if (payload == 0xFFFFFFFFu) {
+                cur = cur + 1;
+            } else {
+                uint32_t off = payload & 0x00FFFFFFu, count = payload >> 24;
+                for (uint32_t i = 0; i < count; i++) {
+                    uint32_t tri = off + i;
+                    REAL th, det;
+                    if (cnt) cnt[2]++;
+                    if (!P(mt)(s, tri, ro, rd, &th, &det)) continue;
+                    if (!(th > RCONST(RAY_EPS) && th < maxd)) continue;
+                    uint32_t m = s->tri_material[tri];
+                    uint32_t flags = (uint32_t)P(mat_f)(s, m, 26);
+                    if (!(P(side_accepts)(s, m, det * dsign) && (flags & 4u))) continue;
+                    int ok = (flags & 1u) ? P(alpha_accepts)(s, m) : ((flags & 2u) != 0);
+                    if (ok) return 1;
+                }
+                cur = miss;
+            }
+        } else {
+            cur = miss;
+        }
+    }
+    return 0;
+}
+
+/* traverseAny js/spectral_traverse.js:539-569 */
+int P(any)(const sbgi_scene_view *s, const REAL ro[3], const REAL rd[3], REAL maxd, uint64_t *cnt) {
+    REAL invd[3];
+    for (int a = 0; a < 3; a++) invd[a] = RCONST(1) / rd[a];
+    uint32_t cur = 0;
+    for (uint32_t it = 0; it < s->tlas_node_count; it++) {
+        if (cur >= s->tlas_node_count) break;
+        REAL bmin[3], bmax[3], tn, tf;
+        for (int a = 0; a < 3; a++) {
+            bmin[a] = P(tlas_f)(s, cur, a);
+            bmax[a] = P(tlas_f)(s, cur, 3 + a);
+        }
+        uint32_t miss = (uint32_t)P(tlas_f)(s, cur, 6);
+        float cntf = P(tlas_f)(s, cur, 8);
+        if (cnt) cnt[1]++;
+        P(slab)(bmin, bmax, ro, invd, &tn, &tf);
+        if (tf >= P(fmax)(tn, 0) && tn < maxd) {
+            if (cntf < 0.5f) {
+                cur = cur + 1;
+            } else {
+                uint32_t off = (uint32_t)P(tlas_f)(s, cur, 7);
+                uint32_t n = (uint32_t)cntf;
+                for (uint32_t i = 0; i < n; i++) {
+                    uint32_t inst = off + i;
+                    if (cnt) cnt[3]++;
+                    if (P(inst_f)(s, inst, 15) > 0.5f && P(blas_any)(s, inst, ro, rd, maxd, cnt)) return 1;
+                }
+                cur = miss;
+            }
+        } else {
+            cur = miss;
+        }
+    }
+    return 0;
+}
+
+/* Brute force over every triangle of every active instance, no bounding boxes: the
+ * closest accepted hit, first-visited on exact ties. Used to pin the BVH walk. */
+void P(brute_closest)(const sbgi_scene_view *s, const REAL wro[3], const REAL wrd[3], REAL *best_t,
+                      int32_t *best_tri, int32_t *best_inst, REAL *second_t) {
+    *best_t = RCONST(T_MAX);
+    *best_tri = -1;
+    *best_inst = -1;
+    if (second_t) *second_t = RCONST(T_MAX);
+    for (uint32_t inst = 0; inst < s->inst_count; inst++) {
+        if (!(P(inst_f)(s, inst, 15) > 0.5f)) continue;
+        REAL ro[3], rd[3];
+        P(inst_point)(s, inst, wro, ro);
+        P(inst_dir)(s, inst, wrd, rd);
+        REAL dsign = P(inst_f)(s, inst, 14);
+        uint32_t end = (uint32_t)P(inst_f)(s, inst, 13);
+        for (uint32_t n = (uint32_t)P(inst_f)(s, inst, 12); n < end; n++) {
+            uint32_t payload = s->bvh_nodes[(size_t)n * 8 + 7];
+            if (payload == 0xFFFFFFFFu) continue;
+            uint32_t off = payload & 0x00FFFFFFu, count = payload >> 24;
+            for (uint32_t i = 0; i < count; i++) {
+                uint32_t tri = off + i;
+                REAL th, det;
+                if (!P(mt)(s, tri, ro, rd, &th, &det)) continue;
+                if (!(th > RCONST(RAY_EPS))) continue;
+                uint32_t m = s->tri_material[tri];
+                uint32_t flags = (uint32_t)P(mat_f)(s, m, 26);
+                if (!P(side_accepts)(s, m, det * dsign)) continue;
+                int ok = (flags & 1u) ? P(alpha_accepts)(s, m) : ((flags & 2u) != 0);
+                if (!ok) continue;
+                if (th < *best_t) {
+                    if (second_t) *second_t = *best_t;
+                    *best_t = th;
+                    *best_tri = (int32_t)tri;
+                    *best_inst = (int32_t)inst;
+                } else if (second_t && th < *second_t) {
+                    *second_t = th;
+                }
+            }
+        }
+    }
+}
+
+/* Derived per-cascade uniforms: maxDist = |gridSize|, cellMin = max(1e-4, min cell)
+ * (js/gi_probes.js:3209-3211, 3361-3362). Computed in double as the JS host does. */
+void P(cascade_derived)(const sbgi_cascade_desc *c, float *max_dist, float *cell_min) {
+    double gx = c->grid_size[0], gy = c->grid_size[1], gz = c->grid_size[2];
+    *max_dist = (float)sqrt(gx * gx + gy * gy + gz * gz);
+    double cx = gx / (c->res[0] > 1 ? c->res[0] - 1 : 1);
+    double cy = gy / (c->res[1] > 1 ? c->res[1] - 1 : 1);
+    double cz = gz / (c->res[2] > 1 ? c->res[2] - 1 : 1);
+    double m = cx < cy ? cx : cy;
+    m = m < cz ? m : cz;
+    if (m < 1e-4) m = 1e-4;
+    *cell_min = (float)(m > 1e-4 ? m : 1e-4);
+}
+
+/* software bilinear, texel-centre convention, clamp-to-edge, of an RGBA atlas */
+static void P(bilinear)(const REAL *atlas, uint32_t w, uint32_t h, REAL u, REAL v, REAL out[3]) {
+    REAL x = u * (REAL)w - RCONST(0.5);
+    REAL y = v * (REAL)h - RCONST(0.5);
+    REAL x0f = RFLOOR(x), y0f = RFLOOR(y);
+    REAL fx = x - x0f, fy = y - y0f;
+    int32_t x0 = (int32_t)x0f, y0 = (int32_t)y0f;
+    int32_t x1 = x0 + 1, y1 = y0 + 1;
+#define CLAMPI(a, hi) ((a) < 0 ? 0 : ((a) > (int32_t)(hi) - 1 ? (int32_t)(hi) - 1 : (a)))
+    x0 = CLAMPI(x0, w);
+    x1 = CLAMPI(x1, w);
+    y0 = CLAMPI(y0, h);
+    y1 = CLAMPI(y1, h);
+#undef CLAMPI
+    REAL w00 = (RCONST(1) - fx) * (RCONST(1) - fy), w10 = fx * (RCONST(1) - fy);
+    REAL w01 = (RCONST(1) - fx) * fy, w11 = fx * fy;
+    for (int c = 0; c < 3; c++) {
+        REAL t00 = atlas[((size_t)y0 * w + x0) * 4 + c], t10 = atlas[((size_t)y0 * w + x1) * 4 + c];
+        REAL t01 = atlas[((size_t)y1 * w + x0) * 4 + c], t11 = atlas[((size_t)y1 * w + x1) * 4 + c];
+        out[c] = t00 * w00 + t10 * w10 + t01 * w01 + t11 * w11;
+    }
+}
+
+/* sampleAtlas js/gi_probes.js:1941-1974 over the previous FP16-rounded atlas */
+static void P(sample_atlas)(const sbgi_cascade_desc *c, const REAL *atlas, const REAL pos[3], const REAL nrm[3],
+                            REAL out[3]) {
+    uint32_t aw = c->res[0] * 8u, ah = c->res[1] * c->res[2] * 8u;
+    REAL cell[3], gf[3], base[3], fr[3];
+    for (int a = 0; a < 3; a++) {
+        REAL den = (REAL)c->res[a] - RCONST(1);
+        if (den < 1) den = 1;
+        cell[a] = (REAL)c->grid_size[a] / den;
+        REAL cm = cell[a] > RCONST(1e-6) ? cell[a] : RCONST(1e-6);
+        gf[a] = (pos[a] - (REAL)c->grid_min[a]) / cm;
+        REAL hi = (REAL)c->res[a] - RCONST(2);
+        if (hi < 0) hi = 0;
+        REAL b = RFLOOR(gf[a]);
+        b = b < 0 ? 0 : (b > hi ? hi : b);
+        base[a] = b;
+        REAL f = gf[a] - b;
+        fr[a] = f < 0 ? 0 : (f > 1 ? 1 : f);
+    }
+    REAL nn[3] = {nrm[0], nrm[1], nrm[2]}, uv[2];
+    P(normalize3)(nn);
+    P(oct_encode)(nn, uv);
+    REAL acc[3] = {0, 0, 0}, wsum = 0;
+    for (int i = 0; i < 8; i++) {
+        int dx = i & 1, dy = (i >> 1) & 1, dz = (i >> 2) & 1;
+        REAL px = base[0] + dx, py = base[1] + dy, pz = base[2] + dz;
+        REAL wx = dx ? fr[0] : RCONST(1) - fr[0];
+        REAL wy = dy ? fr[1] : RCONST(1) - fr[1];
+        REAL wz = dz ? fr[2] : RCONST(1) - fr[2];
+        REAL w = wx * wy * wz + RCONST(1e-4);
+        REAL col = px, row = pz * (REAL)c->res[1] + py;
+        REAL ox = col * RCONST(8) + RCONST(1) + uv[0] * RCONST(6);
+        REAL oy = row * RCONST(8) + RCONST(1) + uv[1] * RCONST(6);
+        REAL t[3];
+        P(bilinear)(atlas, aw, ah, ox / (REAL)aw, oy / (REAL)ah, t);
+        for (int a = 0; a < 3; a++) acc[a] += t[a] * w;
+        wsum += w;
+    }
+    REAL den = wsum > RCONST(1e-4) ? wsum : RCONST(1e-4);
+    for (int a = 0; a < 3; a++) out[a] = acc[a] / den;
+}
+
+/* One ray of traceKernel js/gi_probes.js:1977-2214 (non-clustered NEE, no texture maps,
+ * rough reflections off). out = (r, g, b, hitT). prev_atlas may be NULL (zero atlas). */
+void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays,
+                  const sbgi_tick_params *p, const REAL *prev_atlas, const float *meta, uint32_t probe,
+                  uint32_t k, REAL out[4], int32_t ids[2], uint64_t *cnt) {
+    float max_dist, cell_min;
+    P(cascade_derived)(c, &max_dist, &cell_min);
+    REAL ro[3], rd[3];
+    P(probe_pos)(c, probe, meta, p->classify_strength, ro);
+    P(ray_dir)(k, rays, p->frame_jitter, rd);
+    REAL best_t;
+    int32_t best_tri, best_inst;
+    P(closest)(s, ro, rd, &best_t, &best_tri, &best_inst, cnt);
+    ids[0] = best_tri;
+    ids[1] = best_inst;
+    if (best_tri >= 0) {
+        uint32_t tri = (uint32_t)best_tri, inst = (uint32_t)best_inst;
+        uint32_t m = s->tri_material[tri];
+        REAL p0[3], p1[3], p2[3], e1[3], e2[3], nl[3], ng[3];
+        P(vert)(s, s->tri_index[tri * 3 + 0], p0);
+        P(vert)(s, s->tri_index[tri * 3 + 1], p1);
+        P(vert)(s, s->tri_index[tri * 3 + 2], p2);
+        for (int a = 0; a < 3; a++) {
+            e1[a] = p1[a] - p0[a];
+            e2[a] = p2[a] - p0[a];
+        }
+        P(cross3)(e1, e2, nl);
+        P(normalize3)(nl);
+        P(inst_normal)(s, inst, nl, ng);
+        REAL ff = P(dot3)(ng, rd) < 0 ? RCONST(1) : RCONST(-1);
+        for (int a = 0; a < 3; a++) ng[a] *= ff;
+        REAL b1 = (REAL)cell_min * RCONST(0.005) * (REAL)p->dbg_trace_bias;
+        REAL b2 = RCONST(RAY_EPS) * (REAL)p->dbg_ray_eps;
+        REAL bias = b1 > b2 ? b1 : b2;
+        REAL hit[3];
+        for (int a = 0; a < 3; a++) hit[a] = (ro[a] + rd[a] * best_t) + ng[a] * bias;
+        REAL base[3], rad[3], kd[3];
+        REAL metal = P(mat_f)(s, m, 4), transm = P(mat_f)(s, m, 5);
+        for (int a = 0; a < 3; a++) {
+            base[a] = P(mat_f)(s, m, a);
+            rad[a] = (REAL)P(mat_f)(s, m, 7 + a) * (REAL)p->dbg_emissive;
+            REAL v = base[a] * (REAL)p->dbg_albedo * (RCONST(1) - metal) * (RCONST(1) - transm);
+            kd[a] = v < 0 ? 0 : (v > RCONST(0.95) ? RCONST(0.95) : v);
+        }
+        /* NEE over all lights: js/gi_probes.js:2129-2176 */
+        for (uint32_t li = 0; li < s->light_count; li++) {
+            const float *L = s->lights + li * 17u;
+            float ltype = L[0];
+            if (!(ltype < 2.5f)) continue;
+            int is_ir = fabsf(L[16] - 4.0f) < 0.25f;
+            REAL gate = is_ir ? (REAL)p->nir_gate * (REAL)p->nir_gain : RCONST(1);
+            REAL lcol[3] = {(REAL)L[7] * gate, (REAL)L[8] * gate, (REAL)L[9] * gate};
+            REAL lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
+            int is_dir = ltype < 0.5f;
+            int is_spot = fabsf(ltype - 2.0f) < 0.5f;
+            REAL ldir[3] = {L[4], L[5], L[6]};
+            REAL tl[3];
+            for (int a = 0; a < 3; a++) tl[a] = is_dir ? -ldir[a] : (REAL)L[1 + a] - hit[a];
+            REAL len = RSQRT(P(dot3)(tl, tl));
+            REAL dist = is_dir ? RCONST(1e4) : (len > RCONST(1e-4) ? len : RCONST(1e-4));
+            REAL wi[3] = {tl[0], tl[1], tl[2]};
+            P(normalize3)(wi);
+            REAL ndl = P(dot3)(ng, wi);
+            ndl = ndl > 0 ? ndl : 0;
+            REAL lpeak = P(fmax)(P(fmax)(lcol[0], lcol[1]), lcol[2]);
+            if (!(ndl * lpeak > 0 && (is_dir || lrange <= 0 || dist < lrange))) continue;
+            if (P(any)(s, hit, wi, dist - bias, cnt)) continue;
+            REAL pw = RPOW(dist, ldecay);
+            REAL falloff = RCONST(1) / (pw > RCONST(0.01) ? pw : RCONST(0.01));
+            REAL rr = dist / (lrange > RCONST(1e-4) ? lrange : RCONST(1e-4));
+            REAL rr2 = rr * rr;
+            REAL win = RCONST(1) - rr2 * rr2;
+            win = win < 0 ? 0 : (win > 1 ? 1 : win);
+            REAL ranged = falloff * (win * win);
+            REAL pos_att = lrange > 0 ? ranged : falloff;
+            REAL dist_att = is_dir ? RCONST(1) : pos_att;
+            REAL acos_ = P(dot3)(wi, ldir) * RCONST(-1);
+            REAL den = lcp - lca;
+            den = den > RCONST(1e-4) ? den : RCONST(1e-4);
+            REAL spot = (acos_ - lca) / den;
+            spot = spot < 0 ? 0 : (spot > 1 ? 1 : spot);
+            REAL att = dist_att * (is_spot ? spot : RCONST(1));
+            for (int a = 0; a < 3; a++) {
+                REAL diffuse = kd[a] * (RCONST(1) / RCONST(PI_D));
+                rad[a] += diffuse * ndl * lcol[a] * att * (REAL)p->dbg_direct;
+            }
+        }
+        /* multibounce js/gi_probes.js:2181-2188 */
+        REAL bounce[3] = {0, 0, 0};
+        if (prev_atlas) {
+            P(sample_atlas)(c, prev_atlas, hit, ng, bounce);
+        }
+        for (int a = 0; a < 3; a++) bounce[a] = bounce[a] * kd[a] * (RCONST(1) / RCONST(PI_D));
+        REAL bl = bounce[0] * RCONST(0.2126) + bounce[1] * RCONST(0.7152) + bounce[2] * RCONST(0.0722);
+        REAL rc = p->radiance_clamp;
+        REAL m1 = rc > bl ? rc : bl;
+        REAL roll = rc / (m1 > RCONST(1e-6) ? m1 : RCONST(1e-6));
+        for (int a = 0; a < 3; a++) rad[a] += bounce[a] * roll * (REAL)p->dbg_bounce;
+        out[0] = rad[0];
+        out[1] = rad[1];
+        out[2] = rad[2];
+        out[3] = best_t;
+    } else {
+        /* miss -> SH-9 sky js/gi_probes.js:2189-2207 */
+        REAL x = rd[0], y = rd[1], z = rd[2];
+        const float *sh = p->sky_sh;
+        for (int a = 0; a < 3; a++) {
+            REAL v = (REAL)sh[0 * 3 + a] * RCONST(0.282095) + (REAL)sh[1 * 3 + a] * (y * RCONST(0.488603)) +
+                     (REAL)sh[2 * 3 + a] * (z * RCONST(0.488603)) + (REAL)sh[3 * 3 + a] * (x * RCONST(0.488603)) +
+                     (REAL)sh[4 * 3 + a] * (x * y * RCONST(1.092548)) +
+                     (REAL)sh[5 * 3 + a] * (y * z * RCONST(1.092548)) +
+                     (REAL)sh[6 * 3 + a] * ((z * z * RCONST(3) - RCONST(1)) * RCONST(0.315392)) +
+                     (REAL)sh[7 * 3 + a] * (x * z * RCONST(1.092548)) +
+                     (REAL)sh[8 * 3 + a] * ((x * x - y * y) * RCONST(0.546274));
+            out[a] = (v > 0 ? v : 0) * (REAL)p->sky_intensity;
+        }
+        out[3] = RCONST(-1);
+    }
+}
+
+/* Trace every (slot, ray) of one solve into ray_data[(slot*R + k)*4] (reference order). */
+void P(trace)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays, const sbgi_tick_params *p,
+              const REAL *prev_atlas, const float *meta, REAL *ray_data, int32_t *hit_ids, uint64_t *cnt) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+#pragma omp parallel for schedule(dynamic, 4) if (cnt == NULL)
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        for (uint32_t k = 0; k < rays; k++) {
+            int32_t ids[2];
+            size_t o = (size_t)slot * rays + k;
+            P(trace_ray)(s, c, rays, p, prev_atlas, meta, probe, k, ray_data + o * 4, ids, cnt);
+            if (hit_ids) {
+                hit_ids[o * 2] = ids[0];
+                hit_ids[o * 2 + 1] = ids[1];
+            }
+        }
+    }
+}
+
+/* Texel direction of interior texel (tx, ty) of an n x n octahedral interior:
+ * octDecode((t + 0.5)/n) as in js/gi_probes.js:2345-2347. */
+void P(texel_dir)(uint32_t tx, uint32_t ty, uint32_t n, REAL out[3]) {
+    REAL u = ((REAL)tx + RCONST(0.5)) / (REAL)n;
+    REAL v = ((REAL)ty + RCONST(0.5)) / (REAL)n;
+    P(oct_decode)(u, v, out);
+}
+
+/* blendKernel js/gi_probes.js:2317-2532: cosine gather + adaptive temporal policy.
+ * depth_oct = 6 keeps depth in the 8x8 irradiance tile (reference); 14 uses a 16x16
+ * depth tile whose interior texels gather with their own directions. */
+void P(blend)(const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct, const sbgi_tick_params *p,
+              const REAL *ray_data, REAL *irr, REAL *depth) {
+    float max_dist, cell_min;
+    P(cascade_derived)(c, &max_dist, &cell_min);
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+    REAL *dirs = (REAL *)malloc(sizeof(REAL) * 3 * rays);
+    for (uint32_t k = 0; k < rays; k++) P(ray_dir)(k, rays, p->frame_jitter, dirs + 3 * k);
+    REAL cpow = (REAL)(p->dbg_cosine_power > 1e-4f ? p->dbg_cosine_power : 1e-4f);
+    REAL hyst = p->hysteresis, hexp = p->hysteresis_exponent;
+    const REAL LUMA[3] = {RCONST(0.2126), RCONST(0.7152), RCONST(0.0722)};
+    uint32_t tile_d = depth_oct + 2;
+#pragma omp parallel for schedule(static)
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        const REAL *rd = ray_data + (size_t)slot * rays * 4;
+        /* irradiance texels (and depth texels in the shared-tile mode) */
+        for (uint32_t lane = 0; lane < 36; lane++) {
+            uint32_t lx = lane % 6 + 1, ly = lane / 6 + 1;
+            uint32_t local = ly * 8 + lx;
+            REAL dir[3];
+            P(texel_dir)(lx - 1, ly - 1, 6, dir);
+            REAL acc[3] = {0, 0, 0}, wsum = 0, dacc = 0, dacc2 = 0, dwsum = 0, lacc = 0, lacc2 = 0;
+            for (uint32_t k = 0; k < rays; k++) {
+                const REAL *r = rd + k * 4;
+                REAL cd = P(dot3)(dir, dirs + 3 * k);
+                cd = cd > 0 ? cd : 0;
+                REAL cw = RPOW(cd, cpow);
+                for (int a = 0; a < 3; a++) acc[a] += r[a] * cw;
+                wsum += cw;
+                REAL rl = r[0] * LUMA[0] + r[1] * LUMA[1] + r[2] * LUMA[2];
+                lacc += rl * cw;
+                lacc2 += rl * rl * cw;
+                REAL ht = r[3];
+                REAL dec = ht < RCONST(-1.5) ? -ht - RCONST(2) : ht;
+                REAL rdist = ht == RCONST(-1) ? (REAL)max_dist : dec;
+                REAL dw = RPOW(cw, (REAL)p->depth_sharpness);
+                dacc += rdist * dw;
+                dacc2 += rdist * rdist * dw;
+                dwsum += dw;
+            }
+            REAL ws = wsum > RCONST(1e-4) ? wsum : RCONST(1e-4);
+            REAL mean[3] = {acc[0] / ws, acc[1] / ws, acc[2] / ws};
+            REAL cur_l = lacc / ws, cur_m2 = lacc2 / ws;
+            REAL *I = irr + ((size_t)probe * 64 + local) * 4;
+            REAL prev[3] = {I[0], I[1], I[2]};
+            int was_black = prev[0] + prev[1] + prev[2] < RCONST(1e-6);
+            REAL prev_l = P(dot3)(prev, LUMA);
+            REAL prev_m2 = I[3];
+            REAL prev_var = prev_m2 - prev_l * prev_l;
+            prev_var = prev_var > 0 ? prev_var : 0;
+            REAL cur_var = cur_m2 - cur_l * cur_l;
+            cur_var = cur_var > 0 ? cur_var : 0;
+            REAL lref = P(fmax)(P(fmax)(cur_l, prev_l), 0);
+            REAL var_floor = (REAL)p->dbg_var_eps + lref * lref * (REAL)p->dbg_var_rel;
+            REAL sigma = RSQRT(prev_var + cur_var + var_floor);
+            REAL delta = cur_l - prev_l;
+            REAL ad = RABS(delta);
+            REAL s0 = sigma * (REAL)p->dbg_change_sigma0;
+            REAL sg1 = (REAL)p->temp_change_sigma1;
+            REAL sg1m = (REAL)p->dbg_change_sigma0 + RCONST(0.01);
+            REAL s1 = sigma * (sg1 > sg1m ? sg1 : sg1m);
+            REAL dd = s1 - s0;
+            REAL cwt = (ad - s0) / (dd > RCONST(1e-6) ? dd : RCONST(1e-6));
+            cwt = cwt < 0 ? 0 : (cwt > 1 ? 1 : cwt);
+            REAL noise_h = hyst + (RCONST(1) - hyst) * (REAL)p->dbg_noise_h_boost;
+            REAL ch = hyst - (REAL)p->temp_change_h_drop;
+            REAL change_h = ch > (REAL)p->dbg_min_change_h ? ch : (REAL)p->dbg_min_change_h;
+            REAL raw = noise_h * (RCONST(1) - cwt) + change_h * cwt;
+            raw = raw < RCONST(1e-6) ? RCONST(1e-6) : (raw > 1 ? 1 : raw);
+            REAL heff = RPOW(raw, hexp);
+            REAL h = was_black ? 0 : heff;
+            REAL band = RSQRT(prev_var + var_floor) * (REAL)p->temp_clamp_sigma;
+            REAL cs = band / (ad > RCONST(1e-6) ? ad : RCONST(1e-6));
+            cs = cs < 1 ? cs : 1;
+            for (int a = 0; a < 3; a++) {
+                REAL clipped = prev[a] + (mean[a] - prev[a]) * cs;
+                REAL cand = was_black ? mean[a] : clipped;
+                I[a] = cand * (RCONST(1) - h) + prev[a] * h;
+            }
+            I[3] = cur_m2 * (RCONST(1) - h) + prev_m2 * h;
+            if (depth_oct == 6) {
+                REAL dws = dwsum > RCONST(1e-4) ? dwsum : RCONST(1e-4);
+                REAL mr = dacc / dws, mr2 = dacc2 / dws;
+                REAL *D = depth + ((size_t)probe * 64 + local) * 2;
+                int dzero = D[0] < RCONST(1e-6);
+                REAL rdh = hyst * (REAL)p->dbg_depth_history;
+                rdh = rdh < 0 ? 0 : (rdh > RCONST(0.999) ? RCONST(0.999) : rdh);
+                REAL dh = dzero ? 0 : RPOW(rdh > RCONST(1e-6) ? rdh : RCONST(1e-6), hexp);
+                D[0] = mr * (RCONST(1) - dh) + D[0] * dh;
+                D[1] = mr2 * (RCONST(1) - dh) + D[1] * dh;
+            }
+        }
+        if (depth_oct != 6) {
+            for (uint32_t t = 0; t < depth_oct * depth_oct; t++) {
+                uint32_t lx = t % depth_oct + 1, ly = t / depth_oct + 1;
+                REAL dir[3];
+                P(texel_dir)(lx - 1, ly - 1, depth_oct, dir);
+                REAL dacc = 0, dacc2 = 0, dwsum = 0;
+                for (uint32_t k = 0; k < rays; k++) {
+                    const REAL *r = rd + k * 4;
+                    REAL cd = P(dot3)(dir, dirs + 3 * k);
+                    cd = cd > 0 ? cd : 0;
+                    REAL cw = RPOW(cd, cpow);
+                    REAL ht = r[3];
+                    REAL dec = ht < RCONST(-1.5) ? -ht - RCONST(2) : ht;
+                    REAL rdist = ht == RCONST(-1) ? (REAL)max_dist : dec;
+                    REAL dw = RPOW(cw, (REAL)p->depth_sharpness);
+                    dacc += rdist * dw;
+                    dacc2 += rdist * rdist * dw;
+                    dwsum += dw;
+                }
+                REAL dws = dwsum > RCONST(1e-4) ? dwsum : RCONST(1e-4);
+                REAL mr = dacc / dws, mr2 = dacc2 / dws;
+                REAL *D = depth + ((size_t)probe * tile_d * tile_d + ly * tile_d + lx) * 2;
+                int dzero = D[0] < RCONST(1e-6);
+                REAL rdh = hyst * (REAL)p->dbg_depth_history;
+                rdh = rdh < 0 ? 0 : (rdh > RCONST(0.999) ? RCONST(0.999) : rdh);
+                REAL dh = dzero ? 0 : RPOW(rdh > RCONST(1e-6) ? rdh : RCONST(1e-6), hexp);
+                D[0] = mr * (RCONST(1) - dh) + D[0] * dh;
+                D[1] = mr2 * (RCONST(1) - dh) + D[1] * dh;
+            }
+        }
+    }
+    free(dirs);
+}
+
+/* Gutter source texel for tile texel (lx, ly): js/gi_probes.js:2810-2837 */
+void P(gutter_src)(uint32_t lx, uint32_t ly, uint32_t tile, uint32_t *sx, uint32_t *sy) {
+    uint32_t edge = tile - 1, lo = 1, hi = tile - 2;
+    int on_l = lx == 0, on_r = lx == edge, on_t = ly == 0, on_b = ly == edge;
+    int on_col = on_l || on_r, on_row = on_t || on_b, on_corner = on_col && on_row;
+    *sx = on_corner ? (on_l ? hi : lo) : (on_row ? edge - lx : (on_col ? (on_l ? lo : hi) : lx));
+    *sy = on_corner ? (on_t ? hi : lo) : (on_row ? (on_t ? lo : hi) : (on_col ? edge - ly : ly));
+}
+
+/* uploadKernel js/gi_probes.js:2737-2846: 3x3 bilateral filter of the interior, mirrored
+ * gutters, RGBA16F store (values rounded to FP16 here). Atlases are RGBA, row-major. */
+void P(upload)(const sbgi_cascade_desc *c, uint32_t depth_oct, const sbgi_tick_params *p, const REAL *irr,
+               const REAL *depth, REAL *atlas, REAL *datlas) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+    uint32_t aw = c->res[0] * 8u;
+    uint32_t tile_d = depth_oct + 2;
+    uint32_t dw_ = c->res[0] * tile_d;
+    const REAL LUMA[3] = {RCONST(0.2126), RCONST(0.7152), RCONST(0.0722)};
+    REAL sm_w = RCONST(1) + (REAL)p->filter_smooth * RCONST(6);
+    REAL k_eff = RCONST(8) * sm_w * (REAL)p->dbg_filter_k;
+    REAL rel_eff = RCONST(0.0225) * sm_w * (REAL)p->dbg_filter_rel;
+    REAL eps = RCONST(0.001) * (REAL)p->dbg_filter_eps;
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        const REAL *I = irr + (size_t)probe * 64 * 4;
+        REAL cache[36][4];
+        for (uint32_t lane = 0; lane < 36; lane++) {
+            int sx = (int)(lane % 6) + 1, sy = (int)(lane / 6) + 1;
+            const REAL *e = I + (sy * 8 + sx) * 4;
+            REAL lc = P(dot3)(e, LUMA);
+            REAL vc = e[3] - lc * lc;
+            vc = vc > 0 ? vc : 0;
+            REAL facc[3] = {0, 0, 0}, fw = 0;
+            for (int jy = -1; jy <= 1; jy++) {
+                for (int jx = -1; jx <= 1; jx++) {
+                    REAL gw = (REAL)(RGAUSS(jx, jy));
+                    int nx = sx + jx, ny = sy + jy;
+                    nx = nx < 1 ? 1 : (nx > 6 ? 6 : nx);
+                    ny = ny < 1 ? 1 : (ny > 6 ? 6 : ny);
+                    const REAL *en = I + (ny * 8 + nx) * 4;
+                    REAL dl = P(dot3)(en, LUMA) - lc;
+                    REAL fl = lc * lc * rel_eff;
+                    REAL den = vc * k_eff + (eps > fl ? eps : fl);
+                    den = den > RCONST(1e-8) ? den : RCONST(1e-8);
+                    REAL es = REXP(dl * dl / den * RCONST(-1));
+                    REAL w = gw * es;
+                    for (int a = 0; a < 3; a++) facc[a] += en[a] * w;
+                    fw += w;
+                }
+            }
+            REAL fd = fw > RCONST(1e-4) ? fw : RCONST(1e-4);
+            REAL fs = p->filter_strength;
+            for (int a = 0; a < 3; a++) {
+                REAL filt = facc[a] / fd;
+                cache[lane][a] = e[a] * (RCONST(1) - fs) + filt * fs;
+            }
+            cache[lane][3] = 1;
+        }
+        uint32_t col = probe % c->res[0];
+        uint32_t row = (probe / (c->res[0] * c->res[1])) * c->res[1] + (probe / c->res[0]) % c->res[1];
+        for (uint32_t local = 0; local < 64; local++) {
+            uint32_t lx = local % 8, ly = local / 8, sx, sy;
+            P(gutter_src)(lx, ly, 8, &sx, &sy);
+            uint32_t src = (sy - 1) * 6 + (sx - 1);
+            size_t o = ((size_t)(row * 8 + ly) * aw + col * 8 + lx) * 4;
+            for (int a = 0; a < 4; a++) atlas[o + a] = (REAL)half_round((float)cache[src][a]);
+            if (depth_oct == 6) {
+                const REAL *D = depth + ((size_t)probe * 64 + sy * 8 + sx) * 2;
+                datlas[o + 0] = (REAL)half_round((float)D[0]);
+                datlas[o + 1] = (REAL)half_round((float)D[1]);
+                datlas[o + 2] = 0;
+                datlas[o + 3] = 1;
+            }
+        }
+        if (depth_oct != 6) {
+            for (uint32_t local = 0; local < tile_d * tile_d; local++) {
+                uint32_t lx = local % tile_d, ly = local / tile_d, sx, sy;
+                P(gutter_src)(lx, ly, tile_d, &sx, &sy);
+                const REAL *D = depth + ((size_t)probe * tile_d * tile_d + sy * tile_d + sx) * 2;
+                size_t o = ((size_t)(row * tile_d + ly) * dw_ + col * tile_d + lx) * 4;
+                datlas[o + 0] = (REAL)half_round((float)D[0]);
+                datlas[o + 1] = (REAL)half_round((float)D[1]);
+                datlas[o + 2] = 0;
+                datlas[o + 3] = 1;
+            }
+        }
+    }
+}
+
+/* One full update of one cascade (trace -> blend -> upload), as one solve of tick()
+ * (js/gi_probes.js:3836). prev_atlas is read for the bounce, then atlas is rewritten:
+ * pass the same array for both to chain updates. */
+void P(update)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct,
+               const sbgi_tick_params *p, const float *meta, REAL *irr, REAL *depth, REAL *atlas, REAL *datlas,
+               REAL *ray_data, int32_t *hit_ids, uint64_t *cnt) {
+    P(trace)(s, c, rays, p, atlas, meta, ray_data, hit_ids, cnt);
+    P(blend)(c, rays, depth_oct, p, ray_data, irr, depth);
+    P(upload)(c, depth_oct, p, irr, depth, atlas, datlas);
+}

@@ -1,0 +1,179 @@
+"""ctypes binding of libsbgi.so (include/sbgi.h).
+
+The library is loaded from this package directory. There is no fallback: when the
+library is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsbgi.so")
+
+SBGI_OK = 0
+SBGI_E_INVALID = -1
+SBGI_E_LAYOUT = -2
+SBGI_E_CAPACITY = -3
+SBGI_E_DEVICE = -4
+SBGI_E_STATE = -5
+SBGI_E_TIMEOUT = -6
+SBGI_E_NOMEM = -7
+
+BLEND_FP32, BLEND_BF16, BLEND_BF16X3, BLEND_FP16 = 0, 1, 2, 3
+TRAV_MT_COMPAT, TRAV_MT_ORDERED, TRAV_WATERTIGHT = 0, 1, 2
+FIELD_KEEP_HIT_IDS = 1
+
+BUF_BVH_NODES, BUF_TRI_INDEX, BUF_VERTEX_DATA, BUF_TRI_MATERIAL, BUF_MATERIALS, BUF_LIGHTS = 0, 1, 2, 3, 4, 5
+BUF_IRR_STATE, BUF_DEPTH_STATE, BUF_META, BUF_IRR_ATLAS, BUF_DEPTH_ATLAS = 10, 11, 12, 13, 14
+BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT = 15, 16, 17
+
+# every symbol include/sbgi.h declares
+EXPORTS = [
+    "sbgi_tick_params_default", "sbgi_version", "sbgi_create", "sbgi_set_scene", "sbgi_update_ranges",
+    "sbgi_refit", "sbgi_set_instance_transforms", "sbgi_tick", "sbgi_wait", "sbgi_poll", "sbgi_read",
+    "sbgi_write", "sbgi_buffer_bytes", "sbgi_query_rays", "sbgi_get_stats", "sbgi_set_counting", "sbgi_last_error",
+    "sbgi_destroy", "sbgi_sb_create", "sbgi_sb_add_material", "sbgi_sb_add_mesh", "sbgi_sb_add_instance",
+    "sbgi_sb_add_light", "sbgi_sb_finalize", "sbgi_sb_tlas_order", "sbgi_sb_set_transform",
+    "sbgi_sb_last_error", "sbgi_sb_destroy",
+]
+
+
+class CascadeDesc(C.Structure):
+    _fields_ = [("grid_min", C.c_float * 3), ("grid_size", C.c_float * 3), ("res", C.c_uint32 * 3),
+                ("pad", C.c_uint32)]
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [("device", C.c_int32), ("rays_per_probe", C.c_uint32), ("cascade_count", C.c_uint32),
+                ("depth_oct_res", C.c_uint32), ("blend_precision", C.c_uint32), ("traversal", C.c_uint32),
+                ("flags", C.c_uint32), ("pad", C.c_uint32), ("cascades", CascadeDesc * 4)]
+
+
+class SceneView(C.Structure):
+    _fields_ = [("bvh_nodes", C.c_void_p), ("node_count", C.c_uint32), ("tri_count", C.c_uint32),
+                ("tri_index", C.c_void_p), ("vertex_data", C.c_void_p), ("vertex_count", C.c_uint32),
+                ("materials_len", C.c_uint32), ("tri_material", C.c_void_p), ("materials", C.c_void_p),
+                ("inst_base", C.c_uint32), ("inst_count", C.c_uint32), ("tlas_base", C.c_uint32),
+                ("tlas_node_count", C.c_uint32), ("lights", C.c_void_p), ("light_count", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+TICK_FLOATS = [
+    "frame_jitter", "hysteresis", "hysteresis_exponent", "depth_sharpness", "radiance_clamp",
+    "classify_strength", "filter_strength", "filter_smooth", "temp_change_sigma1", "temp_change_h_drop",
+    "temp_clamp_sigma", "nir_gate", "nir_gain", "sky_intensity",
+]
+TICK_DEBUG = [
+    "dbg_trace_bias", "dbg_ray_eps", "dbg_direct", "dbg_emissive", "dbg_albedo", "dbg_bounce",
+    "dbg_cosine_power", "dbg_noise_h_boost", "dbg_change_sigma0", "dbg_min_change_h", "dbg_var_eps",
+    "dbg_var_rel", "dbg_depth_history", "dbg_filter_k", "dbg_filter_rel", "dbg_filter_eps",
+]
+
+
+class TickParams(C.Structure):
+    _fields_ = ([("cascade", C.c_uint32), ("probe_offset", C.c_uint32), ("updated_count", C.c_uint32),
+                 ("needs_clear", C.c_uint32)]
+                + [(n, C.c_float) for n in TICK_FLOATS]
+                + [("sky_sh", C.c_float * 27)]
+                + [(n, C.c_float) for n in TICK_DEBUG])
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_raydir", C.c_float), ("ms_trace", C.c_float), ("ms_blend", C.c_float),
+                ("ms_upload", C.c_float), ("ms_total", C.c_float), ("rays", C.c_uint32),
+                ("updated", C.c_uint32), ("shadow_rays", C.c_uint64), ("blas_nodes", C.c_uint64),
+                ("tlas_nodes", C.c_uint64), ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64)]
+
+
+class SbgiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sbgi error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libsbgi.so; raises when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m speedball_amd.build` "
+                           "(the HIP extension is required; there is no fallback path)")
+    # PyTorch ships its own libamdhip64.so whose SONAME (libamdhip64.so.7) equals the
+    # system runtime's. Loading torch first makes its copy THE process's HIP runtime, which
+    # libsbgi.so then binds to by SONAME; loading libsbgi.so first would put two HIP
+    # runtimes in one process and torch would see no device.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32, f32p = C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p
+    sig = {
+        "sbgi_tick_params_default": (None, [C.POINTER(TickParams)]),
+        "sbgi_version": (C.c_char_p, []),
+        "sbgi_create": (i32, [C.POINTER(FieldDesc), C.POINTER(vp)]),
+        "sbgi_set_scene": (i32, [vp, C.POINTER(SceneView)]),
+        "sbgi_update_ranges": (i32, [vp, i32, vp, u32, vp]),
+        "sbgi_refit": (i32, [vp, vp, u32, i32]),
+        "sbgi_set_instance_transforms": (i32, [vp, vp, u32, vp, vp]),
+        "sbgi_tick": (i32, [vp, C.POINTER(TickParams), C.POINTER(C.c_uint64)]),
+        "sbgi_wait": (i32, [vp, C.c_uint64, i32]),
+        "sbgi_poll": (i32, [vp, C.c_uint64]),
+        "sbgi_read": (i32, [vp, i32, u32, vp, C.c_size_t]),
+        "sbgi_write": (i32, [vp, i32, u32, vp, C.c_size_t]),
+        "sbgi_buffer_bytes": (C.c_size_t, [vp, i32, u32]),
+        "sbgi_get_stats": (i32, [vp, C.POINTER(Stats)]),
+        "sbgi_set_counting": (i32, [vp, i32]),
+        "sbgi_query_rays": (i32, [vp, vp, u32, vp]),
+        "sbgi_last_error": (C.c_char_p, [vp]),
+        "sbgi_destroy": (i32, [vp]),
+        "sbgi_sb_create": (i32, [C.POINTER(vp)]),
+        "sbgi_sb_add_material": (i32, [vp, f32p, C.POINTER(u32)]),
+        "sbgi_sb_add_mesh": (i32, [vp, f32p, u32, f32p, f32p, vp, u32, vp, C.POINTER(u32)]),
+        "sbgi_sb_add_instance": (i32, [vp, u32, vp, i32, C.POINTER(u32)]),
+        "sbgi_sb_add_light": (i32, [vp, f32p]),
+        "sbgi_sb_finalize": (i32, [vp, C.POINTER(SceneView)]),
+        "sbgi_sb_tlas_order": (i32, [vp, vp, u32]),
+        "sbgi_sb_set_transform": (i32, [vp, u32, vp, C.POINTER(u32)]),
+        "sbgi_sb_last_error": (C.c_char_p, [vp]),
+        "sbgi_sb_destroy": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code: int, handle=None, builder: bool = False) -> int:
+    if code < 0:
+        L = lib()
+        msg = (L.sbgi_sb_last_error(handle) if builder else L.sbgi_last_error(handle)) or b""
+        raise SbgiError(code, msg.decode(errors="replace"))
+    return code
+
+
+def default_tick_params(**kw) -> TickParams:
+    p = TickParams()
+    lib().sbgi_tick_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "sky_sh":
+            for i, x in enumerate(v):
+                p.sky_sh[i] = float(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def ptr(a) -> C.c_void_p:
+    """Data pointer of a C-contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return C.c_void_p(a.ctypes.data)

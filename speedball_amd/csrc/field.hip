@@ -1,0 +1,840 @@
+// field.hip — the probe-field controller behind the C ABI of include/sbgi.h.
+//
+// One sbgi_field owns one HIP stream on one device, the per-cascade temporal state
+// (irradiance/depth moments, probe meta), the RGBA16F atlases, the ray scratch and
+// the scene in device memory. A solve (sbgi_tick) enqueues
+//     raydirs -> weights -> trace -> blend -> upload
+// on the field's stream, the ordered pass that js/gi_probes.js:3836 submits through
+// renderer.computeAsync, and returns a ticket; HIP events bracket every stage.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kernels.h"
+#include "sbgi_host.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct CascadeState {
+    sbgi_cascade_desc d{};
+    uint32_t total = 0, aw = 0, ah = 0, daw = 0, dah = 0, tile_d = 8;
+    float max_dist = 0.f, cell_min = 0.f;
+    float *irr = nullptr, *depth = nullptr, *meta = nullptr;
+    ushort4 *atlas = nullptr, *datlas = nullptr;
+};
+
+constexpr int kTicketRing = 16;
+
+}  // namespace
+
+struct sbgi_field {
+    sbgi_field_desc desc{};
+    hipStream_t stream = nullptr;
+    CascadeState casc[SBGI_MAX_CASCADES];
+    uint32_t max_total = 0;
+    // scene (device)
+    uint32_t *d_nodes = nullptr, *d_tri_index = nullptr, *d_tri_mat = nullptr;
+    float *d_vdata = nullptr, *d_mats = nullptr, *d_lights = nullptr, *d_inst_bounds = nullptr;
+    float4 *d_tri = nullptr, *d_inst = nullptr;
+    uint4 *d_tlas = nullptr;
+    int32_t *d_parent = nullptr, *d_tlas_parent = nullptr;
+    uint32_t *d_arrive = nullptr, *d_tlas_arrive = nullptr;
+    // scene (host mirror, reference layouts)
+    std::vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
+    std::vector<float> h_vdata, h_mats, h_lights, h_inst_bounds;
+    std::vector<double> h_world;  // per TLAS slot, column-major 4x4
+    sbgi_scene_view view{};
+    bool has_scene = false;
+    // solve scratch
+    float4 *rays = nullptr, *dirs = nullptr;
+    int2 *hit_ids = nullptr;
+    void *tables = nullptr;
+    BlendTables bt{};
+    Counters *counters = nullptr;
+    bool count_mode = false;
+    uint32_t last_updated = 0, last_cascade = 0;
+    // tickets and timing
+    hipEvent_t ring[kTicketRing] = {};
+    hipEvent_t ev[5] = {};
+    uint64_t ticket = 0;
+    bool stats_valid = false;
+    std::string err;
+};
+
+namespace {
+
+int32_t fail(sbgi_field *f, int32_t code, const std::string &msg) {
+    if (f)
+        f->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+int32_t hip_fail(sbgi_field *f, hipError_t e, const char *what) {
+    return fail(f, SBGI_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_OK(f, expr)                                          \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) return hip_fail((f), e_, #expr);   \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), bytes);
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_scene(sbgi_field *f) {
+    dfree(f->d_nodes);
+    dfree(f->d_tri_index);
+    dfree(f->d_tri_mat);
+    dfree(f->d_vdata);
+    dfree(f->d_mats);
+    dfree(f->d_lights);
+    dfree(f->d_inst_bounds);
+    dfree(f->d_tri);
+    dfree(f->d_inst);
+    dfree(f->d_tlas);
+    dfree(f->d_parent);
+    dfree(f->d_tlas_parent);
+    dfree(f->d_arrive);
+    dfree(f->d_tlas_arrive);
+    f->has_scene = false;
+}
+
+DevCascade make_cascade(const sbgi_field *f, const CascadeState &cs, uint32_t offset, uint32_t updated) {
+    DevCascade c{};
+    for (int a = 0; a < 3; a++) {
+        c.gmin[a] = cs.d.grid_min[a];
+        c.gsize[a] = cs.d.grid_size[a];
+        c.res[a] = cs.d.res[a];
+    }
+    c.total = cs.total;
+    c.offset = offset;
+    c.updated = updated;
+    c.rays = f->desc.rays_per_probe;
+    c.depth_oct = f->desc.depth_oct_res;
+    c.max_dist = cs.max_dist;
+    c.cell_min = cs.cell_min;
+    c.aw = cs.aw;
+    c.ah = cs.ah;
+    c.daw = cs.daw;
+    c.dah = cs.dah;
+    return c;
+}
+
+DevScene make_scene(const sbgi_field *f) {
+    DevScene s{};
+    s.blas = reinterpret_cast<const uint4 *>(f->d_nodes);
+    s.tri = f->d_tri;
+    s.tlas = f->d_tlas;
+    s.inst = f->d_inst;
+    s.mats = f->d_mats;
+    s.tri_mat = f->d_tri_mat;
+    s.lights = f->d_lights;
+    s.node_count = f->view.node_count;
+    s.tlas_count = f->view.tlas_node_count;
+    s.light_count = f->view.light_count;
+    s.tri_count = f->view.tri_count;
+    return s;
+}
+
+bool is_int(float v) { return std::isfinite(v) && v == std::floor(v); }
+
+// Validate the reference layouts so that every device access stays in bounds.
+int32_t validate_scene(sbgi_field *f) {
+    const sbgi_scene_view &v = f->view;
+    if (v.node_count >= (1u << 24) || v.tri_count >= (1u << 24) || v.inst_count >= (1u << 24))
+        return fail(f, SBGI_E_CAPACITY, "BVH pool or instance count exceeds the 24-bit index ceiling");
+    if (v.light_count > SBGI_MAX_LIGHTS) return fail(f, SBGI_E_CAPACITY, "more than MAX_LIGHTS (64) lights");
+    if (v.node_count == 0 || v.tri_count == 0 || v.inst_count == 0 || v.tlas_node_count == 0)
+        return fail(f, SBGI_E_INVALID, "empty scene");
+    if ((size_t)v.inst_base + (size_t)v.inst_count * 28 > v.tlas_base ||
+        (size_t)v.tlas_base + (size_t)v.tlas_node_count * 12 > v.materials_len || v.inst_base % 28 != 0)
+        return fail(f, SBGI_E_INVALID, "materials buffer too small for the instance/TLAS tail");
+    const uint32_t n_mat = v.inst_base / 28;
+    for (uint32_t i = 0; i < v.node_count; i++) {
+        const uint32_t *nd = f->h_nodes.data() + (size_t)i * 8;
+        if (!(nd[6] > i && nd[6] <= v.node_count)) return fail(f, SBGI_E_LAYOUT, "BLAS miss link out of order");
+        if (nd[7] != 0xFFFFFFFFu) {
+            uint64_t off = nd[7] & 0x00FFFFFFu, cnt = nd[7] >> 24;
+            if (off + cnt > v.tri_count) return fail(f, SBGI_E_LAYOUT, "BLAS leaf triangle range out of bounds");
+        }
+    }
+    for (uint32_t t = 0; t < v.tri_count; t++) {
+        for (int k = 0; k < 3; k++)
+            if (f->h_tri_index[(size_t)t * 3 + k] >= v.vertex_count)
+                return fail(f, SBGI_E_LAYOUT, "triangle vertex index out of range");
+        if (f->h_tri_mat[t] >= n_mat) return fail(f, SBGI_E_LAYOUT, "triangle material index out of range");
+    }
+    for (uint32_t i = 0; i < v.inst_count; i++) {
+        const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
+        if (!is_int(r[12]) || !is_int(r[13]) || !(r[12] >= 0.f && r[12] < r[13] && r[13] <= (float)v.node_count))
+            return fail(f, SBGI_E_LAYOUT, "instance BLAS range out of bounds");
+    }
+    for (uint32_t i = 0; i < v.tlas_node_count; i++) {
+        const float *r = f->h_mats.data() + v.tlas_base + (size_t)i * 12;
+        if (!is_int(r[6]) || !(r[6] > (float)i && r[6] <= (float)v.tlas_node_count))
+            return fail(f, SBGI_E_LAYOUT, "TLAS miss link out of order");
+        if (r[8] >= 0.5f) {
+            if (!is_int(r[7]) || !is_int(r[8]) || r[8] > 255.f || r[7] < 0.f ||
+                r[7] + r[8] > (float)v.inst_count)
+                return fail(f, SBGI_E_LAYOUT, "TLAS leaf instance range out of bounds");
+        }
+    }
+    return SBGI_OK;
+}
+
+void build_parents(const std::vector<uint32_t> &nodes, uint32_t n, std::vector<int32_t> &parent) {
+    parent.assign(n, -1);
+    for (uint32_t i = 0; i < n; i++) {
+        if (nodes[(size_t)i * 8 + 7] != 0xFFFFFFFFu) continue;
+        uint32_t l = i + 1;
+        if (l >= n) continue;
+        uint32_t r = nodes[(size_t)l * 8 + 6];
+        parent[l] = (int32_t)i;
+        if (r < n) parent[r] = (int32_t)i;
+    }
+}
+
+void build_tlas_parents(const float *tl, uint32_t n, std::vector<int32_t> &parent) {
+    parent.assign(n, -1);
+    for (uint32_t i = 0; i < n; i++) {
+        if (tl[(size_t)i * 12 + 8] >= 0.5f) continue;
+        uint32_t l = i + 1;
+        if (l >= n) continue;
+        uint32_t r = (uint32_t)tl[(size_t)l * 12 + 6];
+        parent[l] = (int32_t)i;
+        if (r < n) parent[r] = (int32_t)i;
+    }
+}
+
+// Box3.applyMatrix4 in double of the BLAS root bounds (js/spectral_scene.js:1301-1306)
+void inst_aabb(const float *root_node, const double *e, bool active, float *out6) {
+    if (!active) {
+        for (int a = 0; a < 3; a++) out6[a] = out6[3 + a] = (float)e[12 + a];
+        return;
+    }
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 8; c++) {
+        double x = (c & 1) ? root_node[3] : root_node[0];
+        double y = (c & 2) ? root_node[4] : root_node[1];
+        double z = (c & 4) ? root_node[5] : root_node[2];
+        double w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
+        double p[3] = {(e[0] * x + e[4] * y + e[8] * z + e[12]) * w, (e[1] * x + e[5] * y + e[9] * z + e[13]) * w,
+                       (e[2] * x + e[6] * y + e[10] * z + e[14]) * w};
+        for (int a = 0; a < 3; a++) {
+            mn[a] = std::min(mn[a], p[a]);
+            mx[a] = std::max(mx[a], p[a]);
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        out6[a] = (float)mn[a];
+        out6[3 + a] = (float)mx[a];
+    }
+}
+
+void invert_affine_rows(const float *rows12, double *world16) {
+    // world = inverse of [rows | 0 0 0 1], column-major output
+    double m[16] = {rows12[0], rows12[4], rows12[8], 0, rows12[1], rows12[5], rows12[9], 0,
+                    rows12[2], rows12[6], rows12[10], 0, rows12[3], rows12[7], rows12[11], 1};
+    double a = m[0], b = m[4], c = m[8], d = m[1], e = m[5], f = m[9], g = m[2], h = m[6], i = m[10];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    double det = a * A + b * B + c * C;
+    for (int k = 0; k < 16; k++) world16[k] = 0;
+    world16[15] = 1;
+    if (det == 0) return;
+    double id = 1 / det;
+    double inv[9] = {A * id, -(b * i - c * h) * id, (b * f - c * e) * id,
+                     B * id, (a * i - c * g) * id,  -(a * f - c * d) * id,
+                     C * id, -(a * h - b * g) * id, (a * e - b * d) * id};
+    // inv is row-major 3x3
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) world16[q * 4 + r] = inv[r * 3 + q];
+    for (int r = 0; r < 3; r++)
+        world16[12 + r] = -(inv[r * 3 + 0] * m[12] + inv[r * 3 + 1] * m[13] + inv[r * 3 + 2] * m[14]);
+}
+
+int32_t repack_all(sbgi_field *f) {
+    const sbgi_scene_view &v = f->view;
+    HIP_OK(f, launch_pack_tris(f->d_tri_index, f->d_vdata, f->d_tri_mat, f->d_mats, v.tri_count, f->d_tri, f->stream));
+    HIP_OK(f, launch_pack_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_tlas, f->stream));
+    HIP_OK(f, launch_pack_inst(f->d_mats, v.inst_base, v.inst_count, f->d_inst, f->stream));
+    return SBGI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sbgi_tick_params_default(sbgi_tick_params *p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->frame_jitter = 0.f;
+    p->hysteresis = 0.60f;  // JITTER_HYSTERESIS_DEFAULTS.gated (js/gi_probes.js:146-149)
+    p->hysteresis_exponent = 1.f;
+    p->depth_sharpness = 7.f;
+    p->radiance_clamp = 8.f;
+    p->classify_strength = 0.f;
+    p->filter_strength = 1.f;
+    p->filter_smooth = 0.5f;
+    p->temp_change_sigma1 = 2.5f;
+    p->temp_change_h_drop = 0.30f;
+    p->temp_clamp_sigma = 6.f;
+    p->nir_gate = 0.f;
+    p->nir_gain = 1.f;
+    p->sky_intensity = 1.f;
+    p->dbg_trace_bias = 1.f;
+    p->dbg_ray_eps = 1.f;
+    p->dbg_direct = 1.f;
+    p->dbg_emissive = 1.f;
+    p->dbg_albedo = 1.f;
+    p->dbg_bounce = 1.f;
+    p->dbg_cosine_power = 1.f;
+    p->dbg_noise_h_boost = 0.25f;
+    p->dbg_change_sigma0 = 0.75f;
+    p->dbg_min_change_h = 0.55f;
+    p->dbg_var_eps = 0.000025f;
+    p->dbg_var_rel = 0.0025f;
+    p->dbg_depth_history = 1.f;
+    p->dbg_filter_k = 1.f;
+    p->dbg_filter_rel = 1.f;
+    p->dbg_filter_eps = 1.f;
+}
+
+const char *sbgi_version(void) { return "sbgi 0.1.0 gfx950"; }
+
+int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
+    if (!desc || !out) return fail(nullptr, SBGI_E_INVALID, "null argument");
+    *out = nullptr;
+    const uint32_t R = desc->rays_per_probe;
+    if (R < SBGI_RAYS_MIN || R > SBGI_RAYS_MAX || R % 16 != 0)
+        return fail(nullptr, SBGI_E_INVALID, "rays_per_probe must be a multiple of 16 in [32, 256]");
+    if (desc->cascade_count < 1 || desc->cascade_count > SBGI_MAX_CASCADES)
+        return fail(nullptr, SBGI_E_INVALID, "cascade_count must be 1..4");
+    if (desc->depth_oct_res != 6 && desc->depth_oct_res != 14)
+        return fail(nullptr, SBGI_E_INVALID, "depth_oct_res must be 6 or 14");
+    if (desc->blend_precision > SBGI_BLEND_FP16) return fail(nullptr, SBGI_E_INVALID, "unknown blend precision");
+    if (desc->traversal != SBGI_TRAV_MT_COMPAT)
+        return fail(nullptr, SBGI_E_INVALID, "only the MT_COMPAT traversal is built in this version");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || desc->device < 0 || desc->device >= ndev)
+        return fail(nullptr, SBGI_E_DEVICE, "HIP device not available");
+    sbgi_field *f = new (std::nothrow) sbgi_field();
+    if (!f) return fail(nullptr, SBGI_E_NOMEM, "out of host memory");
+    f->desc = *desc;
+    auto bail = [&](int32_t code) {
+        g_create_error = f->err;
+        sbgi_destroy(f);
+        return code;
+    };
+    if (hipSetDevice(desc->device) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipSetDevice failed"));
+    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(f, SBGI_E_DEVICE, "hipStreamCreate failed"));
+    for (auto &e : f->ring)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
+    for (auto &e : f->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
+    const uint32_t tile_d = desc->depth_oct_res + 2;
+    for (uint32_t ci = 0; ci < desc->cascade_count; ci++) {
+        CascadeState &cs = f->casc[ci];
+        cs.d = desc->cascades[ci];
+        const uint64_t total = (uint64_t)cs.d.res[0] * cs.d.res[1] * cs.d.res[2];
+        if (cs.d.res[0] < 1 || cs.d.res[1] < 1 || cs.d.res[2] < 1 || total >= (1u << 24))
+            return bail(fail(f, SBGI_E_INVALID, "cascade resolution out of range"));
+        cs.total = (uint32_t)total;
+        cs.tile_d = tile_d;
+        cs.aw = cs.d.res[0] * 8;
+        cs.ah = cs.d.res[1] * cs.d.res[2] * 8;
+        cs.daw = desc->depth_oct_res == 6 ? cs.aw : cs.d.res[0] * tile_d;
+        cs.dah = desc->depth_oct_res == 6 ? cs.ah : cs.d.res[1] * cs.d.res[2] * tile_d;
+        // maxDist = |gridSize|, cellMin = max(1e-4, min cell) (js/gi_probes.js:3209-3211, 3361-3362)
+        double gx = cs.d.grid_size[0], gy = cs.d.grid_size[1], gz = cs.d.grid_size[2];
+        cs.max_dist = (float)std::sqrt(gx * gx + gy * gy + gz * gz);
+        double cm = std::min({gx / std::max(1u, cs.d.res[0] - 1), gy / std::max(1u, cs.d.res[1] - 1),
+                              gz / std::max(1u, cs.d.res[2] - 1)});
+        cs.cell_min = (float)std::max(1e-4, cm);
+        const size_t depth_texels = desc->depth_oct_res == 6 ? 64 : (size_t)tile_d * tile_d;
+        if (dalloc(&cs.irr, (size_t)cs.total * 64 * 16) != hipSuccess ||
+            dalloc(&cs.depth, (size_t)cs.total * depth_texels * 8) != hipSuccess ||
+            dalloc(&cs.meta, (size_t)cs.total * 16) != hipSuccess ||
+            dalloc(&cs.atlas, (size_t)cs.aw * cs.ah * 8) != hipSuccess ||
+            dalloc(&cs.datlas, (size_t)cs.daw * cs.dah * 8) != hipSuccess)
+            return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+        (void)hipMemsetAsync(cs.irr, 0, (size_t)cs.total * 64 * 16, f->stream);
+        (void)hipMemsetAsync(cs.depth, 0, (size_t)cs.total * depth_texels * 8, f->stream);
+        (void)hipMemsetAsync(cs.meta, 0, (size_t)cs.total * 16, f->stream);
+        if (launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, f->stream) != hipSuccess ||
+            launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, f->stream) != hipSuccess)
+            return bail(fail(f, SBGI_E_DEVICE, "atlas clear failed"));
+        f->max_total = std::max(f->max_total, cs.total);
+    }
+    const uint32_t tdn = desc->depth_oct_res == 6 ? 36 : desc->depth_oct_res * desc->depth_oct_res;
+    const size_t tb = blend_tables_bytes(R, tdn);
+    if (dalloc(&f->rays, (size_t)f->max_total * R * 16) != hipSuccess || dalloc(&f->dirs, 256 * 16) != hipSuccess ||
+        dalloc(&f->tables, tb) != hipSuccess || dalloc(&f->counters, sizeof(Counters)) != hipSuccess)
+        return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+    if (desc->flags & SBGI_FIELD_KEEP_HIT_IDS) {
+        if (dalloc(&f->hit_ids, (size_t)f->max_total * R * 8) != hipSuccess)
+            return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+    }
+    (void)hipMemsetAsync(f->tables, 0, tb, f->stream);
+    (void)hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream);
+    blend_tables_carve(f->tables, R, tdn, f->bt);
+    if (hipStreamSynchronize(f->stream) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "stream sync failed"));
+    *out = f;
+    return SBGI_OK;
+}
+
+int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
+    if (!f || !s) return SBGI_E_INVALID;
+    if (!s->bvh_nodes || !s->tri_index || !s->vertex_data || !s->tri_material || !s->materials ||
+        (s->light_count && !s->lights))
+        return fail(f, SBGI_E_INVALID, "scene view has a null buffer");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    f->view = *s;
+    f->h_nodes.assign(s->bvh_nodes, s->bvh_nodes + (size_t)s->node_count * 8);
+    f->h_tri_index.assign(s->tri_index, s->tri_index + (size_t)s->tri_count * 3);
+    f->h_tri_mat.assign(s->tri_material, s->tri_material + s->tri_count);
+    f->h_vdata.assign(s->vertex_data, s->vertex_data + (size_t)s->vertex_count * 8);
+    f->h_mats.assign(s->materials, s->materials + s->materials_len);
+    f->h_lights.assign(s->lights ? s->lights : nullptr, s->lights ? s->lights + (size_t)s->light_count * 17 : nullptr);
+    int32_t rc = validate_scene(f);
+    if (rc != SBGI_OK) {
+        f->has_scene = false;
+        return rc;
+    }
+    free_scene(f);
+    const sbgi_scene_view &v = f->view;
+    std::vector<int32_t> parent, tparent;
+    build_parents(f->h_nodes, v.node_count, parent);
+    build_tlas_parents(f->h_mats.data() + v.tlas_base, v.tlas_node_count, tparent);
+    // world matrices recovered from the inverse rows; exact ones arrive via sbgi_set_instance_transforms
+    f->h_world.assign((size_t)v.inst_count * 16, 0.0);
+    f->h_inst_bounds.assign((size_t)v.inst_count * 6, 0.f);
+    for (uint32_t i = 0; i < v.inst_count; i++) {
+        const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
+        invert_affine_rows(r, f->h_world.data() + (size_t)i * 16);
+        const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)r[12] * 8);
+        inst_aabb(root, f->h_world.data() + (size_t)i * 16, r[15] > 0.5f, f->h_inst_bounds.data() + (size_t)i * 6);
+    }
+    if (dalloc(&f->d_nodes, f->h_nodes.size() * 4) != hipSuccess ||
+        dalloc(&f->d_tri_index, f->h_tri_index.size() * 4) != hipSuccess ||
+        dalloc(&f->d_tri_mat, f->h_tri_mat.size() * 4) != hipSuccess ||
+        dalloc(&f->d_vdata, f->h_vdata.size() * 4) != hipSuccess || dalloc(&f->d_mats, f->h_mats.size() * 4) != hipSuccess ||
+        dalloc(&f->d_lights, std::max<size_t>(17, f->h_lights.size()) * 4) != hipSuccess ||
+        dalloc(&f->d_inst_bounds, f->h_inst_bounds.size() * 4) != hipSuccess ||
+        dalloc(&f->d_tri, (size_t)v.tri_count * 48) != hipSuccess ||
+        dalloc(&f->d_inst, (size_t)v.inst_count * 64) != hipSuccess ||
+        dalloc(&f->d_tlas, (size_t)v.tlas_node_count * 32) != hipSuccess ||
+        dalloc(&f->d_parent, (size_t)v.node_count * 4) != hipSuccess ||
+        dalloc(&f->d_arrive, (size_t)v.node_count * 4) != hipSuccess ||
+        dalloc(&f->d_tlas_parent, (size_t)v.tlas_node_count * 4) != hipSuccess ||
+        dalloc(&f->d_tlas_arrive, (size_t)v.tlas_node_count * 4) != hipSuccess) {
+        free_scene(f);
+        return fail(f, SBGI_E_NOMEM, "device allocation for the scene failed");
+    }
+    auto up = [&](void *d, const void *h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, f->stream) : hipSuccess;
+    };
+    HIP_OK(f, up(f->d_nodes, f->h_nodes.data(), f->h_nodes.size() * 4));
+    HIP_OK(f, up(f->d_tri_index, f->h_tri_index.data(), f->h_tri_index.size() * 4));
+    HIP_OK(f, up(f->d_tri_mat, f->h_tri_mat.data(), f->h_tri_mat.size() * 4));
+    HIP_OK(f, up(f->d_vdata, f->h_vdata.data(), f->h_vdata.size() * 4));
+    HIP_OK(f, up(f->d_mats, f->h_mats.data(), f->h_mats.size() * 4));
+    HIP_OK(f, up(f->d_lights, f->h_lights.data(), f->h_lights.size() * 4));
+    HIP_OK(f, up(f->d_inst_bounds, f->h_inst_bounds.data(), f->h_inst_bounds.size() * 4));
+    HIP_OK(f, up(f->d_parent, parent.data(), parent.size() * 4));
+    HIP_OK(f, up(f->d_tlas_parent, tparent.data(), tparent.size() * 4));
+    rc = repack_all(f);
+    if (rc != SBGI_OK) return rc;
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    f->has_scene = true;
+    return SBGI_OK;
+}
+
+int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges, uint32_t n_ranges, const void *src) {
+    if (!f || (n_ranges && (!ranges || !src))) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    void *dev = nullptr;
+    void *host = nullptr;
+    size_t elems = 0, esz = 4;
+    switch (buf_id) {
+        case SBGI_BUF_BVH_NODES: dev = f->d_nodes; host = f->h_nodes.data(); elems = f->h_nodes.size(); break;
+        case SBGI_BUF_TRI_INDEX: dev = f->d_tri_index; host = f->h_tri_index.data(); elems = f->h_tri_index.size(); break;
+        case SBGI_BUF_VERTEX_DATA: dev = f->d_vdata; host = f->h_vdata.data(); elems = f->h_vdata.size(); break;
+        case SBGI_BUF_TRI_MATERIAL: dev = f->d_tri_mat; host = f->h_tri_mat.data(); elems = f->h_tri_mat.size(); break;
+        case SBGI_BUF_MATERIALS: dev = f->d_mats; host = f->h_mats.data(); elems = f->h_mats.size(); break;
+        case SBGI_BUF_LIGHTS: dev = f->d_lights; host = f->h_lights.data(); elems = f->h_lights.size(); break;
+        default: return fail(f, SBGI_E_INVALID, "buffer id not updatable");
+    }
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    std::vector<char> backup(static_cast<char *>(host), static_cast<char *>(host) + elems * esz);
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        uint64_t st = ranges[2 * r], cnt = ranges[2 * r + 1];
+        if (st + cnt > elems) return fail(f, SBGI_E_INVALID, "update range out of bounds");
+        memcpy(static_cast<char *>(host) + st * esz, static_cast<const char *>(src) + st * esz, cnt * esz);
+    }
+    int32_t rc = validate_scene(f);
+    if (rc != SBGI_OK) {
+        memcpy(host, backup.data(), backup.size());
+        return rc;
+    }
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        uint64_t st = ranges[2 * r], cnt = ranges[2 * r + 1];
+        if (cnt)
+            HIP_OK(f, hipMemcpyAsync(static_cast<char *>(dev) + st * esz, static_cast<char *>(host) + st * esz,
+                                     cnt * esz, hipMemcpyHostToDevice, f->stream));
+    }
+    rc = repack_all(f);
+    if (rc != SBGI_OK) return rc;
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges, int32_t refit_tlas) {
+    if (!f || (n_ranges && !blas_ranges)) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    const sbgi_scene_view &v = f->view;
+    // createFlatBlasRefitStepper validation (js/gi_refit.js:37-43, 95-139)
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        uint32_t root = blas_ranges[2 * r], end = blas_ranges[2 * r + 1];
+        if (!(end > root) || end > v.node_count) return fail(f, SBGI_E_LAYOUT, "refit range out of bounds");
+        for (uint32_t i = root; i < end; i++) {
+            const uint32_t *nd = f->h_nodes.data() + (size_t)i * 8;
+            if (nd[7] == 0xFFFFFFFFu) {
+                uint32_t l = i + 1;
+                if (l >= end) return fail(f, SBGI_E_LAYOUT, "interior node without a left child in range");
+                uint32_t rr = f->h_nodes[(size_t)l * 8 + 6];
+                if (!(rr > l && rr < end)) return fail(f, SBGI_E_LAYOUT, "right child outside (left, end)");
+            }
+        }
+    }
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        uint32_t root = blas_ranges[2 * r], end = blas_ranges[2 * r + 1];
+        HIP_OK(f, hipMemsetAsync(f->d_arrive + root, 0, (size_t)(end - root) * 4, f->stream));
+        HIP_OK(f, launch_refit_blas(f->d_nodes, f->d_tri_index, f->d_vdata, f->d_parent, f->d_arrive, root, end,
+                                    f->stream));
+    }
+    if (n_ranges) {
+        HIP_OK(f, hipMemcpyAsync(f->h_nodes.data(), f->d_nodes, f->h_nodes.size() * 4, hipMemcpyDeviceToHost,
+                                 f->stream));
+        HIP_OK(f, hipStreamSynchronize(f->stream));
+    }
+    if (refit_tlas) {
+        for (uint32_t i = 0; i < v.inst_count; i++) {
+            const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
+            const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)r[12] * 8);
+            inst_aabb(root, f->h_world.data() + (size_t)i * 16, r[15] > 0.5f, f->h_inst_bounds.data() + (size_t)i * 6);
+        }
+        HIP_OK(f, hipMemcpyAsync(f->d_inst_bounds, f->h_inst_bounds.data(), f->h_inst_bounds.size() * 4,
+                                 hipMemcpyHostToDevice, f->stream));
+        HIP_OK(f, hipMemsetAsync(f->d_tlas_arrive, 0, (size_t)v.tlas_node_count * 4, f->stream));
+        HIP_OK(f, launch_refit_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_inst_bounds, f->d_tlas_parent,
+                                    f->d_tlas_arrive, f->stream));
+        HIP_OK(f, hipMemcpyAsync(f->h_mats.data() + v.tlas_base, f->d_mats + v.tlas_base,
+                                 (size_t)v.tlas_node_count * 48, hipMemcpyDeviceToHost, f->stream));
+    }
+    int32_t rc = repack_all(f);
+    if (rc != SBGI_OK) return rc;
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+int32_t sbgi_set_instance_transforms(sbgi_field *f, const uint32_t *slots, uint32_t n, const double *world,
+                                     const uint8_t *active) {
+    if (!f || (n && (!slots || !world))) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    const sbgi_scene_view &v = f->view;
+    for (uint32_t q = 0; q < n; q++)
+        if (slots[q] >= v.inst_count) return fail(f, SBGI_E_INVALID, "instance slot out of range");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    for (uint32_t q = 0; q < n; q++) {
+        const uint32_t s = slots[q];
+        const double *e = world + (size_t)q * 16;
+        memcpy(f->h_world.data() + (size_t)s * 16, e, 16 * sizeof(double));
+        float *rec = f->h_mats.data() + v.inst_base + (size_t)s * 28;
+        // inverse rows, det sign, active flag (js/spectral_scene.js:1285-1297, 1339-1351)
+        double inv[16];
+        sbgi::invert4(e, inv);
+        static const int src[12] = {0, 4, 8, 12, 1, 5, 9, 13, 2, 6, 10, 14};  // column-major -> rows
+        for (int i = 0; i < 12; i++) rec[i] = (float)inv[src[i]];
+        rec[14] = sbgi::det4(e) < 0 ? -1.f : 1.f;
+        if (active) rec[15] = active[q] ? 1.f : 0.f;
+        const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)rec[12] * 8);
+        inst_aabb(root, e, rec[15] > 0.5f, f->h_inst_bounds.data() + (size_t)s * 6);
+        HIP_OK(f, hipMemcpyAsync(f->d_mats + v.inst_base + (size_t)s * 28, rec, 28 * 4, hipMemcpyHostToDevice,
+                                 f->stream));
+    }
+    HIP_OK(f, hipMemcpyAsync(f->d_inst_bounds, f->h_inst_bounds.data(), f->h_inst_bounds.size() * 4,
+                             hipMemcpyHostToDevice, f->stream));
+    HIP_OK(f, hipMemsetAsync(f->d_tlas_arrive, 0, (size_t)v.tlas_node_count * 4, f->stream));
+    HIP_OK(f, launch_refit_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_inst_bounds, f->d_tlas_parent,
+                                f->d_tlas_arrive, f->stream));
+    HIP_OK(f, hipMemcpyAsync(f->h_mats.data() + v.tlas_base, f->d_mats + v.tlas_base, (size_t)v.tlas_node_count * 48,
+                             hipMemcpyDeviceToHost, f->stream));
+    HIP_OK(f, launch_pack_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_tlas, f->stream));
+    HIP_OK(f, launch_pack_inst(f->d_mats, v.inst_base, v.inst_count, f->d_inst, f->stream));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
+    if (!f || !p) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene: call sbgi_set_scene first");
+    if (p->cascade >= f->desc.cascade_count) return fail(f, SBGI_E_INVALID, "cascade index out of range");
+    CascadeState &cs = f->casc[p->cascade];
+    const uint32_t updated = p->updated_count ? std::min(p->updated_count, cs.total) : cs.total;
+    if (p->probe_offset >= cs.total) return fail(f, SBGI_E_INVALID, "probe_offset out of range");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    const DevCascade c = make_cascade(f, cs, p->probe_offset, updated);
+    const DevScene s = make_scene(f);
+    hipStream_t st = f->stream;
+    HIP_OK(f, hipEventRecord(f->ev[0], st));
+    if (p->needs_clear) {
+        HIP_OK(f, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
+        HIP_OK(f, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
+    }
+    HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
+    HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
+    HIP_OK(f, hipEventRecord(f->ev[1], st));
+    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, f->rays, f->hit_ids,
+                           f->count_mode ? f->counters : nullptr, st));
+    HIP_OK(f, hipEventRecord(f->ev[2], st));
+    HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, st));
+    HIP_OK(f, hipEventRecord(f->ev[3], st));
+    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, st));
+    HIP_OK(f, hipEventRecord(f->ev[4], st));
+    f->ticket++;
+    HIP_OK(f, hipEventRecord(f->ring[f->ticket % kTicketRing], st));
+    f->last_updated = updated;
+    f->last_cascade = p->cascade;
+    f->stats_valid = true;
+    if (ticket) *ticket = f->ticket;
+    return SBGI_OK;
+}
+
+int32_t sbgi_poll(sbgi_field *f, uint64_t ticket) {
+    if (!f) return SBGI_E_INVALID;
+    if (ticket > f->ticket) return fail(f, SBGI_E_INVALID, "unknown ticket");
+    if (ticket == 0) return 1;
+    hipEvent_t e = (f->ticket - ticket < kTicketRing) ? f->ring[ticket % kTicketRing] : nullptr;
+    if (!e) return 1;  // older than the ring: a later ticket's event already covers it
+    hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return 1;
+    if (q == hipErrorNotReady) return 0;
+    return hip_fail(f, q, "hipEventQuery");
+}
+
+int32_t sbgi_wait(sbgi_field *f, uint64_t ticket, int32_t timeout_ms) {
+    if (!f) return SBGI_E_INVALID;
+    if (ticket > f->ticket) return fail(f, SBGI_E_INVALID, "unknown ticket");
+    if (ticket == 0) return SBGI_OK;
+    hipEvent_t e = (f->ticket - ticket < kTicketRing) ? f->ring[ticket % kTicketRing] : f->ring[f->ticket % kTicketRing];
+    if (timeout_ms < 0) {
+        HIP_OK(f, hipEventSynchronize(e));
+        return SBGI_OK;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        hipError_t q = hipEventQuery(e);
+        if (q == hipSuccess) return SBGI_OK;
+        if (q != hipErrorNotReady) return hip_fail(f, q, "hipEventQuery");
+        auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now() - t0).count();
+        if (ms >= timeout_ms) return fail(f, SBGI_E_TIMEOUT, "sbgi_wait timed out");
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+}
+
+size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) {
+    if (!f) return 0;
+    const uint32_t R = f->desc.rays_per_probe;
+    if (buf_id >= SBGI_BUF_IRR_STATE && buf_id <= SBGI_BUF_DEPTH_ATLAS) {
+        if (cascade >= f->desc.cascade_count) return 0;
+        const CascadeState &cs = f->casc[cascade];
+        const size_t dt = f->desc.depth_oct_res == 6 ? 64 : (size_t)cs.tile_d * cs.tile_d;
+        switch (buf_id) {
+            case SBGI_BUF_IRR_STATE: return (size_t)cs.total * 64 * 16;
+            case SBGI_BUF_DEPTH_STATE: return (size_t)cs.total * dt * 8;
+            case SBGI_BUF_META: return (size_t)cs.total * 16;
+            case SBGI_BUF_IRR_ATLAS: return (size_t)cs.aw * cs.ah * 8;
+            case SBGI_BUF_DEPTH_ATLAS: return (size_t)cs.daw * cs.dah * 8;
+        }
+    }
+    switch (buf_id) {
+        case SBGI_BUF_RAY_DATA: return (size_t)f->last_updated * R * 16;
+        case SBGI_BUF_HIT_IDS: return f->hit_ids ? (size_t)f->last_updated * R * 8 : 0;
+        case SBGI_BUF_MATERIALS: return f->h_mats.size() * 4;
+        case SBGI_BUF_LIGHTS: return f->h_lights.size() * 4;
+        case SBGI_BUF_NODES_REFIT:
+        case SBGI_BUF_BVH_NODES: return f->h_nodes.size() * 4;
+    }
+    return 0;
+}
+
+int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, size_t bytes) {
+    if (!f || !dst) return SBGI_E_INVALID;
+    const size_t need = sbgi_buffer_bytes(f, buf_id, cascade);
+    if (need == 0 || bytes != need) return fail(f, SBGI_E_INVALID, "buffer id/cascade/size mismatch");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    const void *src = nullptr;
+    if (buf_id >= SBGI_BUF_IRR_STATE && buf_id <= SBGI_BUF_DEPTH_ATLAS) {
+        const CascadeState &cs = f->casc[cascade];
+        src = buf_id == SBGI_BUF_IRR_STATE     ? (const void *)cs.irr
+              : buf_id == SBGI_BUF_DEPTH_STATE ? (const void *)cs.depth
+              : buf_id == SBGI_BUF_META        ? (const void *)cs.meta
+              : buf_id == SBGI_BUF_IRR_ATLAS   ? (const void *)cs.atlas
+                                               : (const void *)cs.datlas;
+        HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
+    if (buf_id == SBGI_BUF_RAY_DATA || buf_id == SBGI_BUF_HIT_IDS) {
+        // device order is k-major (k * updated + slot); the reference order is slot * R + k
+        const uint32_t R = f->desc.rays_per_probe, U = f->last_updated;
+        const size_t es = buf_id == SBGI_BUF_RAY_DATA ? 16 : 8;
+        std::vector<char> tmp(bytes);
+        HIP_OK(f, hipMemcpy(tmp.data(), buf_id == SBGI_BUF_RAY_DATA ? (const void *)f->rays : (const void *)f->hit_ids,
+                            bytes, hipMemcpyDeviceToHost));
+        char *o = static_cast<char *>(dst);
+        for (uint32_t k = 0; k < R; k++)
+            for (uint32_t sl = 0; sl < U; sl++) memcpy(o + ((size_t)sl * R + k) * es, tmp.data() + ((size_t)k * U + sl) * es, es);
+        return SBGI_OK;
+    }
+    if (buf_id == SBGI_BUF_MATERIALS) {
+        HIP_OK(f, hipMemcpy(dst, f->d_mats, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
+    if (buf_id == SBGI_BUF_LIGHTS) {
+        HIP_OK(f, hipMemcpy(dst, f->d_lights, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
+    if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_NODES_REFIT) {
+        HIP_OK(f, hipMemcpy(dst, f->d_nodes, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
+    return fail(f, SBGI_E_INVALID, "buffer not readable");
+}
+
+int32_t sbgi_write(sbgi_field *f, int32_t buf_id, uint32_t cascade, const void *src, size_t bytes) {
+    if (!f || !src) return SBGI_E_INVALID;
+    if (buf_id != SBGI_BUF_IRR_STATE && buf_id != SBGI_BUF_DEPTH_STATE && buf_id != SBGI_BUF_META)
+        return fail(f, SBGI_E_INVALID, "only irr/depth/meta state is writable");
+    const size_t need = sbgi_buffer_bytes(f, buf_id, cascade);
+    if (need == 0 || bytes != need) return fail(f, SBGI_E_INVALID, "buffer id/cascade/size mismatch");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    const CascadeState &cs = f->casc[cascade];
+    void *dst = buf_id == SBGI_BUF_IRR_STATE ? (void *)cs.irr : buf_id == SBGI_BUF_DEPTH_STATE ? (void *)cs.depth
+                                                                                             : (void *)cs.meta;
+    HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SBGI_OK;
+}
+
+int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
+    if (!f || !out) return SBGI_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    if (!f->stats_valid) return fail(f, SBGI_E_STATE, "no tick yet");
+    HIP_OK(f, hipEventSynchronize(f->ev[4]));
+    float ms[4];
+    for (int i = 0; i < 4; i++) HIP_OK(f, hipEventElapsedTime(&ms[i], f->ev[i], f->ev[i + 1]));
+    out->ms_raydir = ms[0];
+    out->ms_trace = ms[1];
+    out->ms_blend = ms[2];
+    out->ms_upload = ms[3];
+    HIP_OK(f, hipEventElapsedTime(&out->ms_total, f->ev[0], f->ev[4]));
+    out->updated = f->last_updated;
+    out->rays = f->last_updated * f->desc.rays_per_probe;
+    Counters c{};
+    HIP_OK(f, hipMemcpy(&c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
+    out->blas_nodes = c.blas_nodes;
+    out->tlas_nodes = c.tlas_nodes;
+    out->tri_tests = c.tri_tests;
+    out->inst_visits = c.inst_visits;
+    out->shadow_rays = c.shadow_rays;
+    return SBGI_OK;
+}
+
+const char *sbgi_last_error(const sbgi_field *f) { return f ? f->err.c_str() : g_create_error.c_str(); }
+
+int32_t sbgi_destroy(sbgi_field *f) {
+    if (!f) return SBGI_OK;
+    (void)hipSetDevice(f->desc.device);
+    if (f->stream) (void)hipStreamSynchronize(f->stream);
+    free_scene(f);
+    for (auto &cs : f->casc) {
+        dfree(cs.irr);
+        dfree(cs.depth);
+        dfree(cs.meta);
+        dfree(cs.atlas);
+        dfree(cs.datlas);
+    }
+    dfree(f->rays);
+    dfree(f->dirs);
+    dfree(f->hit_ids);
+    dfree(f->tables);
+    dfree(f->counters);
+    for (auto &e : f->ring)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : f->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (f->stream) (void)hipStreamDestroy(f->stream);
+    delete f;
+    return SBGI_OK;
+}
+
+int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *out4) {
+    if (!f || (n && (!rays8 || !out4))) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    if (n == 0) return SBGI_OK;
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    float4 *din = nullptr, *dout = nullptr;
+    if (dalloc(&din, (size_t)n * 32) != hipSuccess || dalloc(&dout, (size_t)n * 16) != hipSuccess) {
+        dfree(din);
+        return fail(f, SBGI_E_NOMEM, "query allocation failed");
+    }
+    hipError_t e = hipMemcpyAsync(din, rays8, (size_t)n * 32, hipMemcpyHostToDevice, f->stream);
+    if (e == hipSuccess) e = launch_query(make_scene(f), din, dout, n, f->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out4, dout, (size_t)n * 16, hipMemcpyDeviceToHost, f->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+    dfree(din);
+    dfree(dout);
+    if (e != hipSuccess) return hip_fail(f, e, "sbgi_query_rays");
+    return SBGI_OK;
+}
+
+// Counting mode: the next ticks accumulate traversal counters (node visits, triangle
+// tests, instance entries, shadow rays) for the algorithmic-bytes accounting.
+int32_t sbgi_set_counting(sbgi_field *f, int32_t on) {
+    if (!f) return SBGI_E_INVALID;
+    f->count_mode = on != 0;
+    HIP_OK(f, hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+}  // extern "C"

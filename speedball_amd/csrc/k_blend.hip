@@ -1,0 +1,492 @@
+// k_blend.hip — the ray->texel blend as a batched contraction, plus the temporal policy.
+//
+// Restates blendKernel (js/gi_probes.js:2317-2532). The reference recomputes the cosine
+// weight cw = pow(max(dir_t . dir_k, 0), cosinePower) and the depth weight
+// dw = pow(cw, depthSharpness) per (texel, ray) inside every probe's workgroup. Both
+// depend only on the texel and the ray index — rayDir() ignores the probe
+// (js/gi_probes.js:1692-1704) — so the gather of ALL probes is one contraction with a
+// shared weight matrix:
+//     S[t, (p, c)] = sum_k W[t, k] * X[k, (p, c)]
+// with c = {r, g, b, L, L^2} against W_irr and c = {d, d^2} against W_d. The weights
+// and their normalisers (sum_k W) are built once per solve by k_weights.
+//
+//   * FP32 path: one lane per probe, 12 texels per lane, weights broadcast through
+//     scalar loads (they are wave-uniform), X read once per lane and ray.
+//   * MFMA path: v_mfma_f32_16x16x32_{bf16,f16}; a wave owns 16 probes, A = W tiles
+//     (16 texels x 32 rays, pre-swizzled into per-lane fragments), B = X tiles
+//     (32 rays x 16 probes, one channel per tile) built in registers from the k-major
+//     ray buffer, FP32 accumulation. BF16X3 splits both operands into hi + lo BF16
+//     parts and accumulates hi*hi + hi*lo + lo*hi.
+// Normalisation, the adaptive hysteresis policy and the depth blend stay FP32.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include "kernels.h"
+
+namespace {
+
+constexpr uint32_t kIrrTexels = 36;
+
+__device__ __forceinline__ uint32_t kb_count(uint32_t rays) { return (rays + 31) / 32; }
+__device__ __forceinline__ uint32_t mt_count(uint32_t texels) { return (texels + 15) / 16; }
+
+__device__ __forceinline__ float3 oct_decode(float u, float v) {
+    float fx = u * 2.f - 1.f, fy = v * 2.f - 1.f;
+    float nz = 1.f - fabsf(fx) - fabsf(fy);
+    float t = fmaxf(-nz, 0.f);
+    float nx = fx + (fx >= 0.f ? -t : t);
+    float ny = fy + (fy >= 0.f ? -t : t);
+    return normalize3(f3(nx, ny, nz));
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float x) { return __bfloat16_as_ushort(__float2bfloat16(x)); }
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __bfloat162float(__ushort_as_bfloat16(h)); }
+__device__ __forceinline__ uint16_t to_f16(float x) { return __half_as_ushort(__float2half_rn(x)); }
+__device__ __forceinline__ float from_f16(uint16_t h) { return __half2float(__ushort_as_half(h)); }
+
+// Weight tables. Block b < 36 owns irradiance texel b (and, with the shared 8x8 depth
+// tile, its depth weights); blocks 36.. own the 14x14 depth texels.
+__global__ void __launch_bounds__(256) k_weights(DevCascade c, sbgi_tick_params p, const float4 *__restrict__ dirs,
+                                                 BlendTables t, uint32_t precision) {
+    __shared__ float w_s[256];
+    __shared__ float d_s[256];
+    const uint32_t b = blockIdx.x;
+    const bool parity = c.depth_oct == 6;
+    const uint32_t td_count = parity ? kIrrTexels : c.depth_oct * c.depth_oct;
+    const bool is_irr = b < kIrrTexels;
+    uint32_t n, tx, ty, drow;
+    if (is_irr) {
+        n = 6;
+        tx = b % 6;
+        ty = b / 6;
+        drow = b;
+    } else {
+        n = c.depth_oct;
+        drow = b - kIrrTexels;
+        tx = drow % n;
+        ty = drow / n;
+    }
+    const float3 dir = oct_decode(((float)tx + 0.5f) / (float)n, ((float)ty + 0.5f) / (float)n);
+    const float cpow = fmaxf(p.dbg_cosine_power, 1e-4f);
+    const uint32_t R = c.rays, KB = kb_count(R);
+    const bool want_d = parity || !is_irr;
+    for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) {
+        float cd = fmaxf(dot3(dir, xyz(dirs[k])), 0.f);
+        float cw = powf(cd, cpow);
+        float dw = powf(cw, p.depth_sharpness);
+        w_s[k] = cw;
+        d_s[k] = dw;
+        if (is_irr) t.w_irr_t[(size_t)k * kIrrTexels + b] = cw;
+        if (want_d) t.w_d_t[(size_t)k * td_count + drow] = dw;
+    }
+    __syncthreads();
+    // MFMA fragments: element j of lane l of tile (mt, kb) is W[16 mt + l%16][32 kb + 8 (l/16) + j]
+    const uint32_t mt_irr = mt_count(kIrrTexels);
+    const size_t tile_elems = 64 * 8;
+    const size_t irr_frag = (size_t)mt_irr * KB * tile_elems;
+    const size_t d_frag = (size_t)mt_count(td_count) * KB * tile_elems;
+    uint16_t *f_irr_hi = t.frag, *f_irr_lo = t.frag + irr_frag;
+    uint16_t *f_d_hi = t.frag + 2 * irr_frag, *f_d_lo = f_d_hi + d_frag;
+    if (precision != SBGI_BLEND_FP32) {
+        for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) {
+            const uint32_t kb = k / 32, g = (k % 32) / 8, j = k % 8;
+            for (int which = 0; which < 2; which++) {
+                if (which == 0 && !is_irr) continue;
+                if (which == 1 && !want_d) continue;
+                const uint32_t row = which == 0 ? b : drow;
+                const float w = which == 0 ? w_s[k] : d_s[k];
+                const uint32_t mt = row / 16, lane = g * 16 + row % 16;
+                const size_t o = (((size_t)mt * KB + kb) * 64 + lane) * 8 + j;
+                uint16_t hi, lo;
+                if (precision == SBGI_BLEND_FP16) {
+                    hi = to_f16(w);
+                    lo = 0;
+                } else {
+                    hi = to_bf16(w);
+                    lo = precision == SBGI_BLEND_BF16X3 ? to_bf16(w - from_bf16(hi)) : (uint16_t)0;
+                }
+                if (which == 0) {
+                    f_irr_hi[o] = hi;
+                    f_irr_lo[o] = lo;
+                } else {
+                    f_d_hi[o] = hi;
+                    f_d_lo[o] = lo;
+                }
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        // normalisers, summed in ray order as the reference loop does (js/gi_probes.js:2368-2377)
+        float s = 0.f, sq = 0.f, ds = 0.f, dsq = 0.f;
+        for (uint32_t k = 0; k < R; k++) {
+            s += w_s[k];
+            ds += d_s[k];
+            float qw, qd;
+            if (precision == SBGI_BLEND_FP16) {
+                qw = from_f16(to_f16(w_s[k]));
+                qd = from_f16(to_f16(d_s[k]));
+            } else if (precision == SBGI_BLEND_BF16) {
+                qw = from_bf16(to_bf16(w_s[k]));
+                qd = from_bf16(to_bf16(d_s[k]));
+            } else if (precision == SBGI_BLEND_BF16X3) {
+                uint16_t h = to_bf16(w_s[k]);
+                qw = from_bf16(h) + from_bf16(to_bf16(w_s[k] - from_bf16(h)));
+                h = to_bf16(d_s[k]);
+                qd = from_bf16(h) + from_bf16(to_bf16(d_s[k] - from_bf16(h)));
+            } else {
+                qw = w_s[k];
+                qd = d_s[k];
+            }
+            sq += qw;
+            dsq += qd;
+        }
+        if (is_irr) {
+            t.norm_irr[b] = s;
+            t.norm_irr_q[b] = sq;
+        }
+        if (want_d) {
+            t.norm_d[drow] = ds;
+            t.norm_d_q[drow] = dsq;
+        }
+    }
+}
+
+// Adaptive temporal policy for one irradiance texel (js/gi_probes.js:2420-2493).
+__device__ __forceinline__ void irr_policy(const sbgi_tick_params &p, float *__restrict__ irr, uint32_t probe,
+                                           uint32_t local, float s_r, float s_g, float s_b, float s_l, float s_l2,
+                                           float wsum) {
+    const float ws = fmaxf(wsum, 1e-4f);
+    const float mr = s_r / ws, mg = s_g / ws, mb = s_b / ws;
+    const float cur_l = s_l / ws, cur_m2 = s_l2 / ws;
+    float4 *I = reinterpret_cast<float4 *>(irr + ((size_t)probe * 64 + local) * 4);
+    const float4 prev = *I;
+    const bool was_black = prev.x + prev.y + prev.z < 1e-6f;
+    const float prev_l = prev.x * 0.2126f + prev.y * 0.7152f + prev.z * 0.0722f;
+    const float prev_m2 = prev.w;
+    const float prev_var = fmaxf(prev_m2 - prev_l * prev_l, 0.f);
+    const float cur_var = fmaxf(cur_m2 - cur_l * cur_l, 0.f);
+    const float lref = fmaxf(fmaxf(cur_l, prev_l), 0.f);
+    const float var_floor = p.dbg_var_eps + lref * lref * p.dbg_var_rel;
+    const float sigma = sqrtf(prev_var + cur_var + var_floor);
+    const float ad = fabsf(cur_l - prev_l);
+    const float s0 = sigma * p.dbg_change_sigma0;
+    const float s1 = sigma * fmaxf(p.temp_change_sigma1, p.dbg_change_sigma0 + 0.01f);
+    const float cw = fminf(fmaxf((ad - s0) / fmaxf(s1 - s0, 1e-6f), 0.f), 1.f);
+    const float hy = p.hysteresis;
+    const float noise_h = hy + (1.f - hy) * p.dbg_noise_h_boost;
+    const float change_h = fmaxf(hy - p.temp_change_h_drop, p.dbg_min_change_h);
+    const float raw = fminf(fmaxf(noise_h * (1.f - cw) + change_h * cw, 1e-6f), 1.f);
+    const float h = was_black ? 0.f : powf(raw, p.hysteresis_exponent);
+    const float band = sqrtf(prev_var + var_floor) * p.temp_clamp_sigma;
+    const float cs = fminf(1.f, band / fmaxf(ad, 1e-6f));
+    const float cr = was_black ? mr : prev.x + (mr - prev.x) * cs;
+    const float cg = was_black ? mg : prev.y + (mg - prev.y) * cs;
+    const float cb = was_black ? mb : prev.z + (mb - prev.z) * cs;
+    *I = make_float4(cr * (1.f - h) + prev.x * h, cg * (1.f - h) + prev.y * h, cb * (1.f - h) + prev.z * h,
+                     cur_m2 * (1.f - h) + prev_m2 * h);
+}
+
+// Depth-moment blend for one depth texel (js/gi_probes.js:2500-2530).
+__device__ __forceinline__ void depth_policy(const sbgi_tick_params &p, float *__restrict__ depth, size_t idx,
+                                             float s_d, float s_d2, float dwsum) {
+    const float dws = fmaxf(dwsum, 1e-4f);
+    const float mr = s_d / dws, mr2 = s_d2 / dws;
+    float2 *D = reinterpret_cast<float2 *>(depth + idx * 2);
+    const float2 prev = *D;
+    const bool dzero = prev.x < 1e-6f;
+    const float rdh = fminf(fmaxf(p.hysteresis * p.dbg_depth_history, 0.f), 0.999f);
+    const float dh = dzero ? 0.f : powf(fmaxf(rdh, 1e-6f), p.hysteresis_exponent);
+    *D = make_float2(mr * (1.f - dh) + prev.x * dh, mr2 * (1.f - dh) + prev.y * dh);
+}
+
+__device__ __forceinline__ size_t depth_index(const DevCascade &c, uint32_t probe, uint32_t t) {
+    if (c.depth_oct == 6) return (size_t)probe * 64 + (t / 6 + 1) * 8 + (t % 6 + 1);
+    const uint32_t n = c.depth_oct, td = n + 2;
+    return (size_t)probe * td * td + (t / n + 1) * td + (t % n + 1);
+}
+
+__device__ __forceinline__ float ray_dist(float ht, float max_dist) {
+    const float dec = ht < -1.5f ? -ht - 2.f : ht;
+    return ht == -1.f ? max_dist : dec;
+}
+
+constexpr int kGroup = 12;
+
+// FP32 blend. blockIdx.y < 3: irradiance texel group; else depth texel group.
+__global__ void __launch_bounds__(64) k_blend_fp32(DevCascade c, sbgi_tick_params p, const float4 *__restrict__ rays,
+                                                   BlendTables t, float *__restrict__ irr,
+                                                   float *__restrict__ depth) {
+    const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
+    if (slot >= c.updated) return;
+    const uint32_t probe = (c.offset + slot) % c.total;
+    const uint32_t R = c.rays, U = c.updated;
+    const uint32_t gy = blockIdx.y;
+    if (gy < kIrrTexels / kGroup) {
+        float a[kGroup][5];
+#pragma unroll
+        for (int j = 0; j < kGroup; j++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) a[j][q] = 0.f;
+        const float *wbase = t.w_irr_t + gy * kGroup;
+        for (uint32_t k = 0; k < R; k++) {
+            const float4 x = rays[(size_t)k * U + slot];
+            const float L = x.x * 0.2126f + x.y * 0.7152f + x.z * 0.0722f;
+            const float L2 = L * L;
+            const float *w = wbase + (size_t)k * kIrrTexels;
+#pragma unroll
+            for (int j = 0; j < kGroup; j++) {
+                const float cw = w[j];
+                a[j][0] += x.x * cw;
+                a[j][1] += x.y * cw;
+                a[j][2] += x.z * cw;
+                a[j][3] += L * cw;
+                a[j][4] += L2 * cw;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) {
+            const uint32_t tt = gy * kGroup + j;
+            const uint32_t local = (tt / 6 + 1) * 8 + (tt % 6 + 1);
+            irr_policy(p, irr, probe, local, a[j][0], a[j][1], a[j][2], a[j][3], a[j][4], t.norm_irr[tt]);
+        }
+    } else {
+        const uint32_t g = gy - kIrrTexels / kGroup;
+        const uint32_t tdn = c.depth_oct == 6 ? kIrrTexels : c.depth_oct * c.depth_oct;
+        const uint32_t t0 = g * kGroup;
+        float a[kGroup][2];
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) a[j][0] = a[j][1] = 0.f;
+        const bool full = t0 + kGroup <= tdn;
+        for (uint32_t k = 0; k < R; k++) {
+            const float4 x = rays[(size_t)k * U + slot];
+            const float d = ray_dist(x.w, c.max_dist);
+            const float d2 = d * d;
+            const float *w = t.w_d_t + (size_t)k * tdn + t0;
+            if (full) {
+#pragma unroll
+                for (int j = 0; j < kGroup; j++) {
+                    a[j][0] += d * w[j];
+                    a[j][1] += d2 * w[j];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kGroup; j++) {
+                    if (t0 + j < tdn) {
+                        a[j][0] += d * w[j];
+                        a[j][1] += d2 * w[j];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) {
+            const uint32_t tt = t0 + j;
+            if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), a[j][0], a[j][1], t.norm_d[tt]);
+        }
+    }
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <uint32_t PREC>
+__device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 acc) {
+    if constexpr (PREC == SBGI_BLEND_FP16) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), acc,
+                                                      0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                       acc, 0, 0, 0);
+    }
+}
+
+template <uint32_t PREC>
+__device__ __forceinline__ void split(float v, uint16_t &hi, uint16_t &lo) {
+    if constexpr (PREC == SBGI_BLEND_FP16) {
+        hi = to_f16(v);
+        lo = 0;
+    } else {
+        hi = to_bf16(v);
+        lo = PREC == SBGI_BLEND_BF16X3 ? to_bf16(v - from_bf16(hi)) : (uint16_t)0;
+    }
+}
+
+// MFMA blend: one wave per 16 probes. MTD = depth M-tiles (3 for 36 texels, 13 for 196).
+template <uint32_t PREC, int MTD>
+__global__ void __launch_bounds__(64) k_blend_mfma(DevCascade c, sbgi_tick_params p,
+                                                   const float4 *__restrict__ rays, BlendTables t,
+                                                   float *__restrict__ irr, float *__restrict__ depth) {
+    constexpr int MTI = 3;
+    constexpr bool SPLIT = PREC == SBGI_BLEND_BF16X3;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane >> 4, n = lane & 15;
+    const uint32_t slot = blockIdx.x * 16 + n;
+    const bool valid = slot < c.updated;
+    const uint32_t R = c.rays, U = c.updated, KB = (R + 31) / 32;
+    const uint32_t tdn = c.depth_oct == 6 ? kIrrTexels : c.depth_oct * c.depth_oct;
+    const size_t tile_elems = 64 * 8;
+    const size_t irr_frag = (size_t)MTI * KB * tile_elems;
+    const size_t d_frag = (size_t)MTD * KB * tile_elems;
+    const u16x8 *f_irr_hi = reinterpret_cast<const u16x8 *>(t.frag);
+    const u16x8 *f_irr_lo = reinterpret_cast<const u16x8 *>(t.frag + irr_frag);
+    const u16x8 *f_d_hi = reinterpret_cast<const u16x8 *>(t.frag + 2 * irr_frag);
+    const u16x8 *f_d_lo = reinterpret_cast<const u16x8 *>(t.frag + 2 * irr_frag + d_frag);
+
+    f32x4 ai[MTI][5], ad[MTD][2];
+#pragma unroll
+    for (int m = 0; m < MTI; m++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) ai[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int m = 0; m < MTD; m++) ad[m][0] = ad[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (uint32_t kb = 0; kb < KB; kb++) {
+        u16x8 bh[7], bl[7];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t k = kb * 32 + g * 8 + j;
+            float v[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (valid && k < R) {
+                const float4 x = rays[(size_t)k * U + slot];
+                const float L = x.x * 0.2126f + x.y * 0.7152f + x.z * 0.0722f;
+                const float d = ray_dist(x.w, c.max_dist);
+                v[0] = x.x;
+                v[1] = x.y;
+                v[2] = x.z;
+                v[3] = L;
+                v[4] = L * L;
+                v[5] = d;
+                v[6] = d * d;
+            }
+#pragma unroll
+            for (int q = 0; q < 7; q++) {
+                uint16_t hi, lo;
+                split<PREC>(v[q], hi, lo);
+                bh[q][j] = hi;
+                bl[q][j] = lo;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MTI; m++) {
+            const size_t fo = ((size_t)m * KB + kb) * 64 + lane;
+            const u16x8 a_hi = f_irr_hi[fo];
+            u16x8 a_lo;
+            if (SPLIT) a_lo = f_irr_lo[fo];
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                ai[m][q] = mfma<PREC>(a_hi, bh[q], ai[m][q]);
+                if (SPLIT) {
+                    ai[m][q] = mfma<PREC>(a_hi, bl[q], ai[m][q]);
+                    ai[m][q] = mfma<PREC>(a_lo, bh[q], ai[m][q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MTD; m++) {
+            const size_t fo = ((size_t)m * KB + kb) * 64 + lane;
+            const u16x8 a_hi = f_d_hi[fo];
+            u16x8 a_lo;
+            if (SPLIT) a_lo = f_d_lo[fo];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                ad[m][q] = mfma<PREC>(a_hi, bh[5 + q], ad[m][q]);
+                if (SPLIT) {
+                    ad[m][q] = mfma<PREC>(a_hi, bl[5 + q], ad[m][q]);
+                    ad[m][q] = mfma<PREC>(a_lo, bh[5 + q], ad[m][q]);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const uint32_t probe = (c.offset + slot) % c.total;
+    // D layout: lane holds rows 4*(lane/16) + jj of column lane%16
+#pragma unroll
+    for (int m = 0; m < MTI; m++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const uint32_t tt = m * 16 + g * 4 + jj;
+            if (tt < kIrrTexels) {
+                const uint32_t local = (tt / 6 + 1) * 8 + (tt % 6 + 1);
+                irr_policy(p, irr, probe, local, ai[m][0][jj], ai[m][1][jj], ai[m][2][jj], ai[m][3][jj],
+                           ai[m][4][jj], t.norm_irr_q[tt]);
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MTD; m++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const uint32_t tt = m * 16 + g * 4 + jj;
+            if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), ad[m][0][jj], ad[m][1][jj], t.norm_d_q[tt]);
+        }
+    }
+}
+
+}  // namespace
+
+size_t blend_tables_bytes(uint32_t rays, uint32_t depth_texels) {
+    const size_t KB = (rays + 31) / 32;
+    const size_t f = 4 * ((size_t)rays * kIrrTexels + (size_t)rays * depth_texels + 2 * kIrrTexels + 2 * depth_texels);
+    const size_t frag = 2 * 2 * 64 * 8 * KB * (3 + (depth_texels + 15) / 16);
+    return f + frag + 1024;  // + alignment slack of blend_tables_carve
+}
+
+void blend_tables_carve(void *base, uint32_t rays, uint32_t depth_texels, BlendTables &t) {
+    char *p = static_cast<char *>(base);
+    auto take = [&](size_t bytes) {
+        char *r = p;
+        p += (bytes + 63) & ~(size_t)63;
+        return r;
+    };
+    t.w_irr_t = reinterpret_cast<float *>(take(4ull * rays * kIrrTexels));
+    t.w_d_t = reinterpret_cast<float *>(take(4ull * rays * depth_texels));
+    t.norm_irr = reinterpret_cast<float *>(take(4ull * kIrrTexels));
+    t.norm_d = reinterpret_cast<float *>(take(4ull * depth_texels));
+    t.norm_irr_q = reinterpret_cast<float *>(take(4ull * kIrrTexels));
+    t.norm_d_q = reinterpret_cast<float *>(take(4ull * depth_texels));
+    const size_t KB = (rays + 31) / 32;
+    t.frag = reinterpret_cast<uint16_t *>(take(2ull * 2 * 64 * 8 * KB * (3 + (depth_texels + 15) / 16)));
+}
+
+hipError_t launch_weights(const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs, const BlendTables &t,
+                          uint32_t precision, hipStream_t st) {
+    const uint32_t blocks = kIrrTexels + (c.depth_oct == 6 ? 0 : c.depth_oct * c.depth_oct);
+    hipLaunchKernelGGL(k_weights, dim3(blocks), dim3(256), 0, st, c, p, dirs, t, precision);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const BlendTables &t,
+                        uint32_t precision, float *irr, float *depth, hipStream_t st) {
+    if (c.updated == 0) return hipSuccess;
+    const uint32_t tdn = c.depth_oct == 6 ? kIrrTexels : c.depth_oct * c.depth_oct;
+    if (precision == SBGI_BLEND_FP32) {
+        dim3 grid((c.updated + 63) / 64, kIrrTexels / kGroup + (tdn + kGroup - 1) / kGroup);
+        hipLaunchKernelGGL(k_blend_fp32, grid, dim3(64), 0, st, c, p, rays, t, irr, depth);
+        return hipGetLastError();
+    }
+    dim3 grid((c.updated + 15) / 16);
+    const bool parity = c.depth_oct == 6;
+    if (!parity && c.depth_oct != 14) return hipErrorInvalidValue;
+#define SBGI_LAUNCH(PREC)                                                                                       \
+    if (parity)                                                                                                 \
+        hipLaunchKernelGGL((k_blend_mfma<PREC, 3>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth);          \
+    else                                                                                                        \
+        hipLaunchKernelGGL((k_blend_mfma<PREC, 13>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth);
+    switch (precision) {
+        case SBGI_BLEND_BF16:
+            SBGI_LAUNCH(SBGI_BLEND_BF16);
+            break;
+        case SBGI_BLEND_BF16X3:
+            SBGI_LAUNCH(SBGI_BLEND_BF16X3);
+            break;
+        case SBGI_BLEND_FP16:
+            SBGI_LAUNCH(SBGI_BLEND_FP16);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef SBGI_LAUNCH
+    return hipGetLastError();
+}

@@ -1,0 +1,201 @@
+// k_scene.hip — device-side derivation of the packed traversal layouts and the refits.
+//
+// pack_*: turn the reference buffers (bvhNodes/triIndex/vertexData/triMaterial/materials,
+// js/spectral_scene.js:1188-1366) into the device layouts described in sbgi_dev.h.
+// refit_*: refitFlatBlasRange / refitFlatTlasRange (js/gi_refit.js:34-292) as parallel
+// bottom-up sweeps. Each node's bounds depend only on its children, so instead of the
+// reference's reverse pre-order loop every leaf starts a thread that climbs toward the
+// root; the second child to arrive at a parent (per-node arrival counter) computes the
+// parent's union. min/max are exact, so the result equals the sequential sweep bit for bit.
+#include "kernels.h"
+
+namespace {
+
+// materialTraversalFlags + materialSideAccepts inputs folded into one word per triangle
+// (js/spectral_scene.js:285-290, js/spectral_traverse.js:193-220)
+__device__ __forceinline__ uint32_t tri_flags(const float *__restrict__ mats, uint32_t m) {
+    const float *r = mats + (size_t)m * SBGI_MAT_STRIDE;
+    const float side = r[22];
+    uint32_t sc = 3u;  // NaN side accepts nothing
+    if (side < 0.5f)
+        sc = 0u;
+    else if (side >= 0.5f && side < 1.5f)
+        sc = 1u;
+    else if (side >= 1.5f)
+        sc = 2u;
+    const uint32_t fl = (uint32_t)r[26];
+    const float alpha = r[10], at = r[23];
+    const bool alpha_ok = alpha > 1.0e-4f && (at <= 0.f || alpha >= at);
+    const bool acc_closest = (fl & 1u) ? alpha_ok : ((fl & 2u) != 0u);
+    const bool acc_any = (fl & 4u) && acc_closest;
+    return sc | (acc_closest ? 4u : 0u) | (acc_any ? 8u : 0u);
+}
+
+__global__ void k_pack_tris(const uint32_t *__restrict__ tri_index, const float *__restrict__ vdata,
+                            const uint32_t *__restrict__ tri_mat, const float *__restrict__ mats, uint32_t tri_count,
+                            float4 *__restrict__ tri) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tri_count) return;
+    const uint32_t i0 = tri_index[(size_t)t * 3], i1 = tri_index[(size_t)t * 3 + 1], i2 = tri_index[(size_t)t * 3 + 2];
+    const float *v0 = vdata + (size_t)i0 * 8, *v1 = vdata + (size_t)i1 * 8, *v2 = vdata + (size_t)i2 * 8;
+    const uint32_t m = tri_mat[t];
+    tri[(size_t)t * 3 + 0] = make_float4(v0[0], v0[1], v0[2], __uint_as_float(tri_flags(mats, m)));
+    tri[(size_t)t * 3 + 1] = make_float4(v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2], __uint_as_float(m));
+    tri[(size_t)t * 3 + 2] = make_float4(v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2], 0.f);
+}
+
+__global__ void k_pack_tlas(const float *__restrict__ mats, uint32_t tlas_base, uint32_t tlas_count,
+                            uint4 *__restrict__ tlas) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= tlas_count) return;
+    const float *f = mats + tlas_base + (size_t)n * SBGI_TLAS_STRIDE;
+    const uint32_t leaf = f[8] < 0.5f ? 0xFFFFFFFFu : (((uint32_t)f[8] << 24) | ((uint32_t)f[7] & 0x00FFFFFFu));
+    tlas[(size_t)n * 2] = make_uint4(__float_as_uint(f[0]), __float_as_uint(f[1]), __float_as_uint(f[2]), (uint32_t)f[6]);
+    tlas[(size_t)n * 2 + 1] = make_uint4(__float_as_uint(f[3]), __float_as_uint(f[4]), __float_as_uint(f[5]), leaf);
+}
+
+__global__ void k_pack_inst(const float *__restrict__ mats, uint32_t inst_base, uint32_t inst_count,
+                            float4 *__restrict__ inst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= inst_count) return;
+    const float *f = mats + inst_base + (size_t)i * SBGI_MAT_STRIDE;
+#pragma unroll
+    for (int q = 0; q < 4; q++) inst[(size_t)i * 4 + q] = make_float4(f[q * 4], f[q * 4 + 1], f[q * 4 + 2], f[q * 4 + 3]);
+}
+
+__device__ __forceinline__ float load_coherent(const float *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void union_children(float *__restrict__ nf, uint32_t i, const uint32_t *__restrict__ nu,
+                                               uint32_t stride, uint32_t miss_word, bool miss_is_float) {
+    const size_t l = (size_t)i + 1;
+    const uint32_t r = miss_is_float ? (uint32_t)load_coherent(nf + l * stride + miss_word)
+                                     : __hip_atomic_load(nu + l * stride + miss_word, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+    float mn[3], mx[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        mn[a] = fminf(load_coherent(nf + l * stride + a), load_coherent(nf + (size_t)r * stride + a));
+        mx[a] = fmaxf(load_coherent(nf + l * stride + 3 + a), load_coherent(nf + (size_t)r * stride + 3 + a));
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nf[(size_t)i * stride + a] = mn[a];
+        nf[(size_t)i * stride + 3 + a] = mx[a];
+    }
+}
+
+// Bottom-up BLAS refit over [root, end). parent[] holds each node's parent (or -1);
+// arrive[] must be zero for the range on entry. The host has validated the layout.
+__global__ void k_refit_blas(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
+                             const float *__restrict__ vdata, const int32_t *__restrict__ parent,
+                             uint32_t *__restrict__ arrive, uint32_t root, uint32_t end) {
+    const uint32_t i = root + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    const uint32_t info = nodes[(size_t)i * 8 + 7];
+    if (info == 0xFFFFFFFFu) return;  // interiors are reached from below
+    float *nf = reinterpret_cast<float *>(nodes);
+    const uint32_t off = info & 0x00FFFFFFu, cnt = info >> 24;
+    if (cnt > 0) {
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t t = off; t < off + cnt; t++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float *p = vdata + (size_t)tri_index[(size_t)t * 3 + k] * 8;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    mn[a] = fminf(mn[a], p[a]);
+                    mx[a] = fmaxf(mx[a], p[a]);
+                }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            nf[(size_t)i * 8 + a] = mn[a];
+            nf[(size_t)i * 8 + 3 + a] = mx[a];
+        }
+    }
+    int32_t p = parent[i];
+    while (p >= (int32_t)root) {
+        __threadfence();
+        if (atomicAdd(&arrive[p], 1u) == 0u) return;  // sibling not done yet: it will continue
+        __threadfence();
+        union_children(nf, (uint32_t)p, nodes, 8, 6, false);
+        p = parent[p];
+    }
+}
+
+// Bottom-up TLAS refit from per-slot instance world bounds (6 floats each).
+__global__ void k_refit_tlas(float *__restrict__ mats, uint32_t tlas_base, uint32_t tlas_count,
+                             const float *__restrict__ inst_bounds, const int32_t *__restrict__ parent,
+                             uint32_t *__restrict__ arrive) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tlas_count) return;
+    float *tl = mats + tlas_base;
+    float *nd = tl + (size_t)i * SBGI_TLAS_STRIDE;
+    const float cnt = nd[8];
+    if (cnt < 0.5f) return;
+    const uint32_t off = (uint32_t)nd[7], n = (uint32_t)cnt;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t s = off; s < off + n; s++) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(mn[a], inst_bounds[(size_t)s * 6 + a]);
+            mx[a] = fmaxf(mx[a], inst_bounds[(size_t)s * 6 + 3 + a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nd[a] = mn[a];
+        nd[3 + a] = mx[a];
+    }
+    int32_t p = parent[i];
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&arrive[p], 1u) == 0u) return;
+        __threadfence();
+        union_children(tl, (uint32_t)p, nullptr, SBGI_TLAS_STRIDE, 6, true);
+        p = parent[p];
+    }
+}
+
+}  // namespace
+
+hipError_t launch_pack_tris(const uint32_t *tri_index, const float *vdata, const uint32_t *tri_mat, const float *mats,
+                            uint32_t tri_count, float4 *tri, hipStream_t st) {
+    if (!tri_count) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_tris, dim3((tri_count + 255) / 256), dim3(256), 0, st, tri_index, vdata, tri_mat, mats,
+                       tri_count, tri);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_tlas(const float *mats, uint32_t tlas_base, uint32_t tlas_count, uint4 *tlas, hipStream_t st) {
+    if (!tlas_count) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_tlas, dim3((tlas_count + 255) / 256), dim3(256), 0, st, mats, tlas_base, tlas_count, tlas);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_inst(const float *mats, uint32_t inst_base, uint32_t inst_count, float4 *inst, hipStream_t st) {
+    if (!inst_count) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_inst, dim3((inst_count + 255) / 256), dim3(256), 0, st, mats, inst_base, inst_count,
+                       inst);
+    return hipGetLastError();
+}
+
+hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
+                             uint32_t *arrive, uint32_t root, uint32_t end, hipStream_t st) {
+    if (end <= root) return hipSuccess;
+    const uint32_t n = end - root;
+    hipLaunchKernelGGL(k_refit_blas, dim3((n + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata, parent, arrive,
+                       root, end);
+    return hipGetLastError();
+}
+
+hipError_t launch_refit_tlas(float *mats, uint32_t tlas_base, uint32_t tlas_count, const float *inst_bounds,
+                             const int32_t *parent, uint32_t *arrive, hipStream_t st) {
+    if (!tlas_count) return hipSuccess;
+    hipLaunchKernelGGL(k_refit_tlas, dim3((tlas_count + 255) / 256), dim3(256), 0, st, mats, tlas_base, tlas_count,
+                       inst_bounds, parent, arrive);
+    return hipGetLastError();
+}

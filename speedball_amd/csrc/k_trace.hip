@@ -1,0 +1,258 @@
+// k_trace.hip — the probe trace: ray generation, closest hit, NEE shadow rays, bounce.
+//
+// Restates traceKernel (js/gi_probes.js:1977-2214) for the configuration the reference
+// ships by default: non-clustered NEE over all lights, no texture maps, reflections off.
+//
+// Work mapping (MI355X-first): thread -> (ray k, probe slot) with the probe slot
+// fastest-varying, so the 64 lanes of a wave trace the SAME direction k from 64
+// neighbouring probes. rayDir() does not depend on the probe (js/gi_probes.js:1692-1704),
+// so a wave walks the BVH with parallel rays from nearby origins — the direction-binned
+// ordering that keeps stackless traversal coherent on 64-wide wavefronts. Output is
+// stored k-major (rays[k * updated + slot]) so the stores coalesce and the blend reads
+// it as the K x N operand of its contraction.
+#include <hip/hip_fp16.h>
+
+#include "kernels.h"
+#include "traverse.h"
+
+namespace {
+
+__device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
+
+// software bilinear of the previous RGBA16F atlas: texel-centre convention, clamp to edge
+__device__ __forceinline__ float3 bilinear(const ushort4 *__restrict__ atlas, uint32_t w, uint32_t h, float u, float v) {
+    float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    float x0f = floorf(x), y0f = floorf(y);
+    float fx = x - x0f, fy = y - y0f;
+    int x0 = (int)x0f, y0 = (int)y0f, x1 = x0 + 1, y1 = y0 + 1;
+    x0 = min(max(x0, 0), (int)w - 1);
+    x1 = min(max(x1, 0), (int)w - 1);
+    y0 = min(max(y0, 0), (int)h - 1);
+    y1 = min(max(y1, 0), (int)h - 1);
+    ushort4 t00 = atlas[(size_t)y0 * w + x0], t10 = atlas[(size_t)y0 * w + x1];
+    ushort4 t01 = atlas[(size_t)y1 * w + x0], t11 = atlas[(size_t)y1 * w + x1];
+    float w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy), w01 = (1.f - fx) * fy, w11 = fx * fy;
+    return f3(h2f(t00.x) * w00 + h2f(t10.x) * w10 + h2f(t01.x) * w01 + h2f(t11.x) * w11,
+              h2f(t00.y) * w00 + h2f(t10.y) * w10 + h2f(t01.y) * w01 + h2f(t11.y) * w11,
+              h2f(t00.z) * w00 + h2f(t10.z) * w10 + h2f(t01.z) * w01 + h2f(t11.z) * w11);
+}
+
+__device__ __forceinline__ float2 oct_encode(float3 n) {
+    float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    float inv = 1.0f / fmaxf(s, 1e-20f);
+    float ox = n.x * inv, oy = n.y * inv;
+    if (n.z < 0.f) {
+        float fx = (1.f - fabsf(oy)) * (ox >= 0.f ? 1.f : -1.f);
+        float fy = (1.f - fabsf(ox)) * (oy >= 0.f ? 1.f : -1.f);
+        ox = fx;
+        oy = fy;
+    }
+    return make_float2(ox * 0.5f + 0.5f, oy * 0.5f + 0.5f);
+}
+
+// sampleAtlas (js/gi_probes.js:1941-1974)
+__device__ float3 sample_atlas(const DevCascade &c, const ushort4 *__restrict__ atlas, float3 P, float3 N) {
+    float cell[3], gf[3], base[3], fr[3];
+    float pp[3] = {P.x, P.y, P.z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        cell[a] = c.gsize[a] / fmaxf(1.f, (float)c.res[a] - 1.f);
+        gf[a] = (pp[a] - c.gmin[a]) / fmaxf(cell[a], 1e-6f);
+        float hi = fmaxf((float)c.res[a] - 2.f, 0.f);
+        base[a] = fminf(fmaxf(floorf(gf[a]), 0.f), hi);
+        fr[a] = fminf(fmaxf(gf[a] - base[a], 0.f), 1.f);
+    }
+    float2 uv = oct_encode(normalize3(N));
+    float3 acc = f3(0.f, 0.f, 0.f);
+    float wsum = 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        int dx = i & 1, dy = (i >> 1) & 1, dz = (i >> 2) & 1;
+        float px = base[0] + dx, py = base[1] + dy, pz = base[2] + dz;
+        float wx = dx ? fr[0] : 1.f - fr[0];
+        float wy = dy ? fr[1] : 1.f - fr[1];
+        float wz = dz ? fr[2] : 1.f - fr[2];
+        float w = wx * wy * wz + 1e-4f;
+        float col = px, row = pz * (float)c.res[1] + py;
+        float ox = col * 8.f + 1.f + uv.x * 6.f;
+        float oy = row * 8.f + 1.f + uv.y * 6.f;
+        float3 t = bilinear(atlas, c.aw, c.ah, ox / (float)c.aw, oy / (float)c.ah);
+        acc = acc + t * w;
+        wsum += w;
+    }
+    return acc * (1.0f / fmaxf(wsum, 1e-4f));
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_trace(DevScene s, DevCascade c, sbgi_tick_params p,
+                                                const float4 *__restrict__ dirs, const float *__restrict__ meta,
+                                                const ushort4 *__restrict__ prev_atlas, float4 *__restrict__ rays,
+                                                int2 *__restrict__ hit_ids, Counters *__restrict__ counters) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = c.updated * c.rays;
+    if (gid >= n) return;
+    const uint32_t k = gid / c.updated;
+    const uint32_t slot = gid - k * c.updated;
+    const uint32_t probe = (c.offset + slot) % c.total;
+
+    // probeTraceOrigin (js/gi_probes.js:1714-1726, 1876-1885)
+    uint32_t ix = probe % c.res[0], iy = (probe / c.res[0]) % c.res[1], iz = probe / (c.res[0] * c.res[1]);
+    float3 ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
+                   c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
+                   c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
+    if (p.classify_strength != 0.f) {
+        const float *m = meta + (size_t)probe * 4;
+        ro = ro + f3(m[1], m[2], m[3]) * p.classify_strength;
+    }
+    float3 rd = xyz(dirs[k]);
+
+    TravCounters<COUNT> cnt;
+    float best_t;
+    int best_tri, best_inst;
+    trace_closest<COUNT>(s, ro, rd, best_t, best_tri, best_inst, cnt);
+
+    float3 out;
+    float hit_t = -1.f;
+    uint32_t shadow = 0;
+    if (best_tri >= 0) {
+        hit_t = best_t;
+        const uint32_t tri = (uint32_t)best_tri, inst = (uint32_t)best_inst;
+        const float4 *tp = s.tri + (size_t)tri * 3;
+        const float4 B = tp[1], C = tp[2];
+        const uint32_t m = __float_as_uint(B.w);
+        float4 r0, r1, r2, r3;
+        inst_rows(s, inst, r0, r1, r2, r3);
+        // geometric normal through (M^-1)^T, face-forwarded (js/gi_probes.js:2006-2011)
+        float3 nl = normalize3(cross3(xyz(B), xyz(C)));
+        float3 ng = normalize3(xyz(r0) * nl.x + xyz(r1) * nl.y + xyz(r2) * nl.z);
+        ng = ng * (dot3(ng, rd) < 0.f ? 1.f : -1.f);
+        const float bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
+        const float3 hit = (ro + rd * best_t) + ng * bias;
+        const float *mat = s.mats + (size_t)m * SBGI_MAT_STRIDE;
+        const float metal = mat[4], transm = mat[5];
+        float3 rad = f3(mat[7], mat[8], mat[9]) * p.dbg_emissive;
+        float3 kd;
+        kd.x = fminf(fmaxf(mat[0] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
+        kd.y = fminf(fmaxf(mat[1] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
+        kd.z = fminf(fmaxf(mat[2] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
+        const float3 diffuse = kd * (1.0f / SBGI_PI);
+        // NEE over all lights (js/gi_probes.js:2129-2176)
+        for (uint32_t li = 0; li < s.light_count; li++) {
+            const float *L = s.lights + li * SBGI_LIGHT_STRIDE;
+            const float ltype = L[0];
+            if (!(ltype < 2.5f)) continue;
+            const bool is_ir = fabsf(L[16] - 4.0f) < 0.25f;
+            const float gate = is_ir ? p.nir_gate * p.nir_gain : 1.f;
+            const float3 lcol = f3(L[7], L[8], L[9]) * gate;
+            const float lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
+            const bool is_dir = ltype < 0.5f;
+            const bool is_spot = fabsf(ltype - 2.0f) < 0.5f;
+            const float3 ldir = f3(L[4], L[5], L[6]);
+            const float3 tl = is_dir ? ldir * -1.f : f3(L[1], L[2], L[3]) - hit;
+            const float dist = is_dir ? 1e4f : fmaxf(sqrtf(dot3(tl, tl)), 1e-4f);
+            const float3 wi = normalize3(tl);
+            const float ndl = fmaxf(dot3(ng, wi), 0.f);
+            const float lpeak = fmaxf(fmaxf(lcol.x, lcol.y), lcol.z);
+            if (!(ndl * lpeak > 0.f && (is_dir || lrange <= 0.f || dist < lrange))) continue;
+            shadow++;
+            if (trace_any<COUNT>(s, hit, wi, dist - bias, cnt)) continue;
+            const float falloff = 1.0f / fmaxf(powf(dist, ldecay), 0.01f);
+            const float rr = dist / fmaxf(lrange, 1e-4f);
+            const float rr2 = rr * rr;
+            const float win = fminf(fmaxf(1.f - rr2 * rr2, 0.f), 1.f);
+            const float ranged = falloff * (win * win);
+            const float pos_att = lrange > 0.f ? ranged : falloff;
+            const float dist_att = is_dir ? 1.f : pos_att;
+            const float acos_ = dot3(wi, ldir) * -1.f;
+            const float spot = fminf(fmaxf((acos_ - lca) / fmaxf(lcp - lca, 1e-4f), 0.f), 1.f);
+            const float att = dist_att * (is_spot ? spot : 1.f);
+            rad = rad + diffuse * ndl * lcol * (att * p.dbg_direct);
+        }
+        // multibounce from the previous filtered atlas (js/gi_probes.js:2181-2188)
+        float3 bounce = sample_atlas(c, prev_atlas, hit, ng) * kd * (1.0f / SBGI_PI);
+        const float bl = bounce.x * 0.2126f + bounce.y * 0.7152f + bounce.z * 0.0722f;
+        const float roll = p.radiance_clamp / fmaxf(fmaxf(p.radiance_clamp, bl), 1e-6f);
+        out = rad + bounce * (roll * p.dbg_bounce);
+    } else {
+        // miss -> SH-9 sky (js/gi_probes.js:2189-2207)
+        const float x = rd.x, y = rd.y, z = rd.z;
+        const float *sh = p.sky_sh;
+        float o[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            float v = sh[0 + a] * 0.282095f + sh[3 + a] * (y * 0.488603f) + sh[6 + a] * (z * 0.488603f) +
+                      sh[9 + a] * (x * 0.488603f) + sh[12 + a] * (x * y * 1.092548f) +
+                      sh[15 + a] * (y * z * 1.092548f) + sh[18 + a] * ((z * z * 3.f - 1.f) * 0.315392f) +
+                      sh[21 + a] * (x * z * 1.092548f) + sh[24 + a] * ((x * x - y * y) * 0.546274f);
+            o[a] = fmaxf(v, 0.f) * p.sky_intensity;
+        }
+        out = f3(o[0], o[1], o[2]);
+    }
+    rays[gid] = make_float4(out.x, out.y, out.z, hit_t);
+    if (hit_ids) hit_ids[gid] = make_int2(best_tri, best_inst);
+    if (COUNT) {
+        atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
+        atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
+        atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
+        atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
+        atomicAdd(&counters->shadow_rays, (unsigned long long)shadow);
+    }
+}
+
+// Direction table: normalize(rayDir(k, jitter)) (js/gi_probes.js:1692-1704, 1984).
+__global__ void k_raydirs(float4 *__restrict__ dirs, uint32_t rays, float jitter) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= rays) return;
+    float sk = ((float)k + 0.5f) / (float)rays;
+    float u = sk + jitter;
+    float uw = u - floorf(u);
+    float z = 1.f - uw * 2.f;
+    float r = sqrtf(fmaxf(0.f, 1.f - z * z));
+    float phi = (float)k * SBGI_GOLDEN_ANGLE + jitter * SBGI_TWO_PI;
+    float3 d = normalize3(f3(r * cosf(phi), r * sinf(phi), z));
+    dirs[k] = make_float4(d.x, d.y, d.z, 0.f);
+}
+
+// Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit.
+// out[i] = {t, tri (bits), inst (bits), blocked}.
+__global__ void k_query(DevScene s, const float4 *__restrict__ in, float4 *__restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = in[(size_t)i * 2], b = in[(size_t)i * 2 + 1];
+    TravCounters<false> cnt;
+    if (b.w < 0.5f) {
+        float t;
+        int tri, inst;
+        trace_closest<false>(s, xyz(a), xyz(b), t, tri, inst, cnt);
+        out[i] = make_float4(t, __int_as_float(tri), __int_as_float(inst), 0.f);
+    } else {
+        const bool blocked = trace_any<false>(s, xyz(a), xyz(b), a.w, cnt);
+        out[i] = make_float4(0.f, 0.f, 0.f, blocked ? 1.f : 0.f);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_query, dim3((n + 255) / 256), dim3(256), 0, st, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st) {
+    hipLaunchKernelGGL(k_raydirs, dim3((rays + 63) / 64), dim3(64), 0, st, dirs, rays, jitter);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                        const float *meta, const ushort4 *prev_atlas, float4 *rays, int2 *hit_ids, Counters *counters,
+                        hipStream_t st) {
+    const uint32_t n = c.updated * c.rays;
+    if (n == 0) return hipSuccess;
+    dim3 grid((n + 255) / 256), block(256);
+    if (counters)
+        hipLaunchKernelGGL(k_trace<true>, grid, block, 0, st, s, c, p, dirs, meta, prev_atlas, rays, hit_ids, counters);
+    else
+        hipLaunchKernelGGL(k_trace<false>, grid, block, 0, st, s, c, p, dirs, meta, prev_atlas, rays, hit_ids, counters);
+    return hipGetLastError();
+}

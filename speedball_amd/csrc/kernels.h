@@ -1,0 +1,45 @@
+// kernels.h — host launchers of the HIP kernels (one translation unit each).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sbgi_dev.h"
+
+// k_trace.hip
+hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, hipStream_t st);
+hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
+hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                        const float *meta, const ushort4 *prev_atlas, float4 *rays, int2 *hit_ids,
+                        Counters *counters, hipStream_t st);
+
+// k_blend.hip
+struct BlendTables {
+    float *w_irr_t;     // [R][36]   cosine weights, k-major
+    float *w_d_t;       // [R][TD]   depth weights, k-major
+    float *norm_irr;    // [36]      sum_k cw
+    float *norm_d;      // [TD]      sum_k dw
+    uint16_t *frag;     // MFMA A fragments (bf16/fp16 hi and lo), see k_blend.hip
+    float *norm_irr_q;  // [36] normalisers of the quantized weights
+    float *norm_d_q;    // [TD]
+};
+size_t blend_tables_bytes(uint32_t rays, uint32_t depth_texels);
+void blend_tables_carve(void *base, uint32_t rays, uint32_t depth_texels, BlendTables &t);
+hipError_t launch_weights(const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs, const BlendTables &t,
+                          uint32_t precision, hipStream_t st);
+hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const BlendTables &t,
+                        uint32_t precision, float *irr, float *depth, hipStream_t st);
+
+// k_upload.hip
+hipError_t launch_upload(const DevCascade &c, const sbgi_tick_params &p, const float *irr, const float *depth,
+                         ushort4 *atlas, ushort4 *datlas, hipStream_t st);
+hipError_t launch_clear_atlas(ushort4 *atlas, size_t texels, hipStream_t st);
+
+// k_scene.hip
+hipError_t launch_pack_tris(const uint32_t *tri_index, const float *vdata, const uint32_t *tri_mat, const float *mats,
+                            uint32_t tri_count, float4 *tri, hipStream_t st);
+hipError_t launch_pack_tlas(const float *mats, uint32_t tlas_base, uint32_t tlas_count, uint4 *tlas, hipStream_t st);
+hipError_t launch_pack_inst(const float *mats, uint32_t inst_base, uint32_t inst_count, float4 *inst, hipStream_t st);
+// BLAS refit of the threaded range [root, end): bottom-up with per-node arrival counters.
+hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
+                             uint32_t *arrive, uint32_t root, uint32_t end, hipStream_t st);
+hipError_t launch_refit_tlas(float *mats, uint32_t tlas_base, uint32_t tlas_count, const float *inst_bounds,
+                             const int32_t *parent, uint32_t *arrive, hipStream_t st);

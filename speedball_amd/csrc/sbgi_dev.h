@@ -1,0 +1,158 @@
+// sbgi_dev.h — device layouts and traversal routines for gfx950.
+//
+// Device-side scene layouts (derived on the device from the reference buffers by the
+// pack kernels in k_scene.hip):
+//   blas  uint4[2*N]   the reference 32-byte node unchanged: {min.xyz, max.x}, {max.yz, miss,
+//                      payload} — two dwordx4 loads per node.
+//   tri   float4[3*T]  {p0.xyz, flags}, {e1.xyz, material}, {e2.xyz, 0}: one 48-byte record
+//                      replaces triIndex + 3 scattered vertices + triMaterial + material flags.
+//                      flags: bits 0-1 side (0 front, 1 back, 2 double), bit 2 closest-hit
+//                      accept, bit 3 shadow-ray accept (materialTraversalFlags,
+//                      js/spectral_traverse.js:193-220, 410-423, 512-516).
+//   tlas  uint4[2*M]   {min.xyz, miss}, {max.xyz, leaf} with leaf = 0xFFFFFFFF for interior,
+//                      else (count << 24) | instance slot offset.
+//   inst  float4[4*I]  inverse world rows 0..2 (w = translation), {blasRoot, blasEnd,
+//                      detSign, active} (js/spectral_traverse.js:140-158).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sbgi.h"
+
+#define SBGI_RAY_EPS 1e-3f
+#define SBGI_T_MAX 1e30f
+#define SBGI_PI 3.141592653589793f
+#define SBGI_GOLDEN_ANGLE 2.39996322972865332f
+#define SBGI_TWO_PI 6.283185307179586f
+
+struct DevScene {
+    const uint4 *__restrict__ blas;
+    const float4 *__restrict__ tri;
+    const uint4 *__restrict__ tlas;
+    const float4 *__restrict__ inst;
+    const float *__restrict__ mats;
+    const uint32_t *__restrict__ tri_mat;
+    const float *__restrict__ lights;
+    uint32_t node_count, tlas_count, light_count, tri_count;
+};
+
+// Per-solve cascade geometry (per-cascade uniforms of js/gi_probes.js:3200-3211).
+struct DevCascade {
+    float gmin[3], gsize[3];
+    uint32_t res[3];
+    uint32_t total;       // probeTotal
+    uint32_t offset;      // probeOffset
+    uint32_t updated;     // updatedCount
+    uint32_t rays;        // R
+    uint32_t depth_oct;   // 6 or 14
+    float max_dist, cell_min;
+    uint32_t aw, ah;      // irradiance atlas size in texels
+    uint32_t daw, dah;    // depth atlas size
+};
+
+struct Counters {
+    unsigned long long blas_nodes, tlas_nodes, tri_tests, inst_visits, shadow_rays;
+};
+
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 operator*(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross3(float3 a, float3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 normalize3(float3 v) {
+    float len = sqrtf(dot3(v, v));
+    return f3(v.x / len, v.y / len, v.z / len);
+}
+__device__ __forceinline__ float3 xyz(float4 v) { return f3(v.x, v.y, v.z); }
+
+// slab test (js/spectral_traverse.js:372-377)
+__device__ __forceinline__ void slab(float3 bmin, float3 bmax, float3 ro, float3 invd, float &tn, float &tf) {
+    float t0x = (bmin.x - ro.x) * invd.x, t1x = (bmax.x - ro.x) * invd.x;
+    float t0y = (bmin.y - ro.y) * invd.y, t1y = (bmax.y - ro.y) * invd.y;
+    float t0z = (bmin.z - ro.z) * invd.z, t1z = (bmax.z - ro.z) * invd.z;
+    tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+}
+
+__device__ __forceinline__ bool side_ok(uint32_t flags, float det) {
+    uint32_t side = flags & 3u;
+    return side == 2u || (side == 0u && det > 0.f) || (side == 1u && det < 0.f);
+}
+
+// Moller-Trumbore with the reference's tests and epsilon (js/spectral_traverse.js:388-409).
+__device__ __forceinline__ bool mt_test(float4 A, float4 B, float4 C, float3 ro, float3 rd, float &t, float &det) {
+    float3 p0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
+    float3 pv = cross3(rd, e2);
+    det = dot3(e1, pv);
+    if (!(fabsf(det) > 1e-12f)) return false;
+    float inv_det = __builtin_amdgcn_rcpf(det);
+    float3 tv = ro - p0;
+    float u = dot3(tv, pv) * inv_det;
+    if (!(u >= 0.f && u <= 1.f)) return false;
+    float3 qv = cross3(tv, e1);
+    float v = dot3(rd, qv) * inv_det;
+    if (!(v >= 0.f && u + v <= 1.f)) return false;
+    t = dot3(e2, qv) * inv_det;
+    return true;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald 2013, JCGT 2(1)): shear the
+// triangle into ray space, signed edge functions with a double-precision fallback
+// when one is exactly zero. Returns det with the sign convention of the
+// Moller-Trumbore det (positive for a counter-clockwise front face seen by the ray).
+struct WtRay {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+__device__ __forceinline__ WtRay wt_setup(float3 rd) {
+    float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float d[3] = {rd.x, rd.y, rd.z};
+    if (d[kz] < 0.f) {
+        int tmp = kx;
+        kx = ky;
+        ky = tmp;
+    }
+    WtRay w;
+    w.kx = kx;
+    w.ky = ky;
+    w.kz = kz;
+    w.sx = d[kx] / d[kz];
+    w.sy = d[ky] / d[kz];
+    w.sz = 1.0f / d[kz];
+    return w;
+}
+__device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ bool wt_test(float4 A, float4 B, float4 C, float3 ro, const WtRay &w, float &t,
+                                        float &det_mt) {
+    float3 p0 = xyz(A);
+    float3 p1 = p0 + xyz(B), p2 = p0 + xyz(C);
+    float3 a = p0 - ro, b = p1 - ro, c = p2 - ro;
+    float ax = comp(a, w.kx) - w.sx * comp(a, w.kz), ay = comp(a, w.ky) - w.sy * comp(a, w.kz);
+    float bx = comp(b, w.kx) - w.sx * comp(b, w.kz), by = comp(b, w.ky) - w.sy * comp(b, w.kz);
+    float cx = comp(c, w.kx) - w.sx * comp(c, w.kz), cy = comp(c, w.ky) - w.sy * comp(c, w.kz);
+    float U = cx * by - cy * bx, V = ax * cy - ay * cx, W = bx * ay - by * ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        double CxBy = (double)cx * by, CyBx = (double)cy * bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)ax * cy, AyCx = (double)ay * cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)bx * ay, ByAx = (double)by * ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    float det = U + V + W;
+    if (det == 0.f) return false;
+    float az = w.sz * comp(a, w.kz), bz = w.sz * comp(b, w.kz), cz = w.sz * comp(c, w.kz);
+    float T = U * az + V * bz + W * cz;
+    t = T / det;
+    // MT det = dot(e1, cross(rd, e2)) has the sign of -(edge-function sum) in this frame
+    det_mt = -det;
+    return true;
+}

@@ -1,0 +1,166 @@
+"""Host-side probe field over the C ABI (the Python mirror of createProbeField's
+solve path, js/gi_probes.js:945-4847, for tests and benchmarks).
+
+    f = ProbeField(res=(16,16,16), grid_min=..., grid_size=..., rays=64)
+    f.set_scene(scene)
+    f.tick(frame_jitter=j, hysteresis=0.97)   # one full-field update
+    irr = f.read_irr()
+
+Everything runs through libsbgi.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+def clamp_rays(n) -> int:
+    """setRays: round to a multiple of 16 within [32, 256] (js/gi_probes.js:4589-4594)."""
+    try:
+        v = int(round(float(n) / 16.0)) * 16
+    except (TypeError, ValueError):
+        v = 0
+    if v == 0:
+        v = 64
+    return max(32, min(256, v))
+
+
+class ProbeField:
+    def __init__(self, res, grid_min, grid_size, rays: int = 64, depth_oct: int = 6,
+                 precision: int = N.BLEND_FP32, keep_hit_ids: bool = False, device: int = 0, cascades=None):
+        self.L = N.lib()
+        d = N.FieldDesc()
+        d.device = device
+        d.rays_per_probe = rays
+        casc = cascades or [(grid_min, grid_size, res)]
+        d.cascade_count = len(casc)
+        d.depth_oct_res = depth_oct
+        d.blend_precision = precision
+        d.traversal = N.TRAV_MT_COMPAT
+        d.flags = N.FIELD_KEEP_HIT_IDS if keep_hit_ids else 0
+        for i, (gm, gs, rs) in enumerate(casc):
+            for a in range(3):
+                d.cascades[i].grid_min[a] = gm[a]
+                d.cascades[i].grid_size[a] = gs[a]
+                d.cascades[i].res[a] = rs[a]
+        h = C.c_void_p()
+        N.check(self.L.sbgi_create(C.byref(d), C.byref(h)), None)
+        self.h = h
+        self.desc = d
+        self.rays = rays
+        self.depth_oct = depth_oct
+        self.cascades = casc
+        self.needs_clear = [True] * len(casc)
+        self.last_ticket = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbgi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, code):
+        return N.check(code, self.h)
+
+    def probes(self, cascade: int = 0) -> int:
+        r = self.cascades[cascade][2]
+        return r[0] * r[1] * r[2]
+
+    def set_scene(self, scene):
+        self._scene = scene  # keep the arrays alive for the call
+        self._chk(self.L.sbgi_set_scene(self.h, C.byref(scene.view())))
+
+    def tick(self, cascade: int = 0, probe_offset: int = 0, updated: int = 0, wait: bool = True, **params) -> int:
+        p = N.default_tick_params(**params)
+        p.cascade = cascade
+        p.probe_offset = probe_offset
+        p.updated_count = updated
+        p.needs_clear = 1 if self.needs_clear[cascade] else 0
+        self.needs_clear[cascade] = False
+        t = C.c_uint64()
+        self._chk(self.L.sbgi_tick(self.h, C.byref(p), C.byref(t)))
+        self.last_ticket = t.value
+        if wait:
+            self.wait(t.value)
+        return t.value
+
+    def wait(self, ticket=None, timeout_ms: int = -1):
+        self._chk(self.L.sbgi_wait(self.h, self.last_ticket if ticket is None else ticket, timeout_ms))
+
+    def poll(self, ticket) -> bool:
+        return self._chk(self.L.sbgi_poll(self.h, ticket)) == 1
+
+    def stats(self) -> N.Stats:
+        s = N.Stats()
+        self._chk(self.L.sbgi_get_stats(self.h, C.byref(s)))
+        return s
+
+    def set_counting(self, on: bool):
+        self._chk(self.L.sbgi_set_counting(self.h, int(on)))
+
+    def nbytes(self, buf: int, cascade: int = 0) -> int:
+        return int(self.L.sbgi_buffer_bytes(self.h, buf, cascade))
+
+    def read(self, buf: int, dtype, cascade: int = 0) -> np.ndarray:
+        n = self.nbytes(buf, cascade)
+        out = np.empty(n // np.dtype(dtype).itemsize, dtype)
+        self._chk(self.L.sbgi_read(self.h, buf, cascade, N.ptr(out), n))
+        return out
+
+    def write(self, buf: int, data: np.ndarray, cascade: int = 0):
+        data = np.ascontiguousarray(data)
+        self._chk(self.L.sbgi_write(self.h, buf, cascade, N.ptr(data), data.nbytes))
+
+    def read_irr(self, cascade=0):
+        return self.read(N.BUF_IRR_STATE, np.float32, cascade).reshape(-1, 64, 4)
+
+    def read_depth(self, cascade=0):
+        t = 64 if self.depth_oct == 6 else (self.depth_oct + 2) ** 2
+        return self.read(N.BUF_DEPTH_STATE, np.float32, cascade).reshape(-1, t, 2)
+
+    def read_atlas(self, depth=False, cascade=0):
+        raw = self.read(N.BUF_DEPTH_ATLAS if depth else N.BUF_IRR_ATLAS, np.uint16, cascade)
+        res = self.cascades[cascade][2]
+        tile = 8 if (not depth or self.depth_oct == 6) else self.depth_oct + 2
+        return raw.view(np.float16).astype(np.float32).reshape(res[1] * res[2] * tile, res[0] * tile, 4)
+
+    def read_rays(self):
+        return self.read(N.BUF_RAY_DATA, np.float32).reshape(-1, self.rays, 4)
+
+    def read_hit_ids(self):
+        return self.read(N.BUF_HIT_IDS, np.int32).reshape(-1, self.rays, 2)
+
+    def query(self, ro, rd, tmax=None, mode=0):
+        """Closest-hit (mode 0) or any-hit (mode 1) queries of the resident scene.
+        Returns (t, tri, inst, blocked) arrays."""
+        ro = np.asarray(ro, np.float32).reshape(-1, 3)
+        rd = np.asarray(rd, np.float32).reshape(-1, 3)
+        n = ro.shape[0]
+        rec = np.zeros((n, 8), np.float32)
+        rec[:, 0:3] = ro
+        rec[:, 3] = 1e30 if tmax is None else tmax
+        rec[:, 4:7] = rd
+        rec[:, 7] = mode
+        out = np.zeros((n, 4), np.float32)
+        self._chk(self.L.sbgi_query_rays(self.h, N.ptr(rec), n, N.ptr(out)))
+        return out[:, 0], out[:, 1].view(np.int32), out[:, 2].view(np.int32), out[:, 3] > 0.5
+
+    def update_ranges(self, buf: int, ranges, src: np.ndarray):
+        r = np.ascontiguousarray(np.asarray(ranges, np.uint32).reshape(-1))
+        src = np.ascontiguousarray(src)
+        self._chk(self.L.sbgi_update_ranges(self.h, buf, N.ptr(r), r.size // 2, N.ptr(src)))
+
+    def refit(self, blas_ranges, refit_tlas: bool = True):
+        r = np.ascontiguousarray(np.asarray(blas_ranges, np.uint32).reshape(-1))
+        self._chk(self.L.sbgi_refit(self.h, N.ptr(r) if r.size else None, r.size // 2, int(refit_tlas)))
+
+    def set_instance_transforms(self, slots, worlds, active=None):
+        s = np.ascontiguousarray(np.asarray(slots, np.uint32))
+        w = np.ascontiguousarray(np.stack([np.asarray(m, np.float64).T.reshape(-1) for m in worlds]))
+        a = None if active is None else np.ascontiguousarray(np.asarray(active, np.uint8))
+        self._chk(self.L.sbgi_set_instance_transforms(self.h, N.ptr(s), s.size, N.ptr(w), N.ptr(a)))

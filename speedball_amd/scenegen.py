@@ -1,0 +1,269 @@
+"""Seeded procedural scenes for the probe-update workloads.
+
+Every stream is SplitMix64 with master seed 0x5BA11000 + workload id, so a scene is
+a pure function of (workload, size knobs). Geometry is generated with numpy and
+assembled by the native SceneBuilder into the reference layouts.
+
+  room_boxes()  10x10x10 m double-sided room (12 triangles) + 64 yawed boxes (12 each),
+                one merged BLAS, one identity instance; 1 point light (intensity 10,
+                decay 2) + 1 directional light; 16^3 probes inset 0.25 m, 64 rays.
+  instanced()   8 procedural meshes (UV sphere, torus, fbm-displaced grid) instanced
+                256 times with random TRS in a 40x20x25 m box, 262,144 triangles total,
+                16 point lights log-uniform [1, 100]; 32x16x24 probes, 256 rays.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .scene import SceneBuilder, Scene, uber_material, point_light, directional_light
+
+MASK64 = (1 << 64) - 1
+SEED_BASE = 0x5BA11000
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / (1 << 53)))
+
+    def log_uniform(self, lo: float, hi: float) -> float:
+        return math.exp(self.uniform(math.log(lo), math.log(hi)))
+
+    def fork(self, tag: int) -> "SplitMix64":
+        return SplitMix64(self.next_u64() ^ (tag * 0x632BE59BD9B4E019 & MASK64))
+
+
+@dataclass
+class Workload:
+    name: str
+    scene: Scene
+    grid_min: tuple
+    grid_size: tuple
+    res: tuple
+    rays: int
+    depth_oct: int
+    frame_jitter: float
+    hysteresis: float = 0.97
+    bounces: int = 1
+
+    @property
+    def probes(self) -> int:
+        return self.res[0] * self.res[1] * self.res[2]
+
+
+# ---------------------------------------------------------------- geometry helpers
+def box_mesh(center, half, yaw):
+    """12-triangle box, outward counter-clockwise winding."""
+    hx, hy, hz = half
+    c, s = math.cos(yaw), math.sin(yaw)
+    corners = []
+    for i in range(8):
+        x = hx if i & 1 else -hx
+        y = hy if i & 2 else -hy
+        z = hz if i & 4 else -hz
+        corners.append((center[0] + c * x + s * z, center[1] + y, center[2] - s * x + c * z))
+    v = np.array(corners, np.float32)
+    # faces as quads (a, b, c, d) counter-clockwise seen from outside
+    quads = [(0, 4, 6, 2), (1, 3, 7, 5), (0, 1, 5, 4), (2, 6, 7, 3), (0, 2, 3, 1), (4, 5, 7, 6)]
+    tris = []
+    for a, b, cc, d in quads:
+        tris += [(a, b, cc), (a, cc, d)]
+    return v, np.array(tris, np.uint32)
+
+
+def room_mesh(size=10.0):
+    """12 wall triangles of the [0,size]^3 room (double-sided material)."""
+    v, t = box_mesh((size / 2, size / 2, size / 2), (size / 2,) * 3, 0.0)
+    return v, t[:, ::-1].copy()  # inward-facing
+
+
+def uv_sphere(n_lat: int, n_lon: int, radius=1.0):
+    verts = [(0.0, radius, 0.0)]
+    for i in range(1, n_lat):
+        th = math.pi * i / n_lat
+        for j in range(n_lon):
+            ph = 2 * math.pi * j / n_lon
+            verts.append((radius * math.sin(th) * math.cos(ph), radius * math.cos(th), radius * math.sin(th) * math.sin(ph)))
+    verts.append((0.0, -radius, 0.0))
+    v = np.array(verts, np.float32)
+    tris = []
+    ring = lambda i, j: 1 + (i - 1) * n_lon + (j % n_lon)
+    for j in range(n_lon):
+        tris.append((0, ring(1, j + 1), ring(1, j)))
+    for i in range(1, n_lat - 1):
+        for j in range(n_lon):
+            a, b, c, d = ring(i, j), ring(i, j + 1), ring(i + 1, j + 1), ring(i + 1, j)
+            tris += [(a, b, c), (a, c, d)]
+    last = len(verts) - 1
+    for j in range(n_lon):
+        tris.append((last, ring(n_lat - 1, j), ring(n_lat - 1, j + 1)))
+    return v, np.array(tris, np.uint32)
+
+
+def torus(n_u: int, n_v: int, R=1.0, r=0.35):
+    u = np.arange(n_u) * (2 * math.pi / n_u)
+    w = np.arange(n_v) * (2 * math.pi / n_v)
+    uu, ww = np.meshgrid(u, w, indexing="ij")
+    x = (R + r * np.cos(ww)) * np.cos(uu)
+    y = r * np.sin(ww)
+    z = (R + r * np.cos(ww)) * np.sin(uu)
+    v = np.stack([x, y, z], -1).reshape(-1, 3).astype(np.float32)
+    i, j = np.meshgrid(np.arange(n_u), np.arange(n_v), indexing="ij")
+    a = i * n_v + j
+    b = ((i + 1) % n_u) * n_v + j
+    c = ((i + 1) % n_u) * n_v + (j + 1) % n_v
+    d = i * n_v + (j + 1) % n_v
+    t = np.concatenate([np.stack([a, d, c], -1).reshape(-1, 3), np.stack([a, c, b], -1).reshape(-1, 3)])
+    return v, t.astype(np.uint32)
+
+
+def _value_noise(x, y, seed: int):
+    xi, yi = np.floor(x).astype(np.int64), np.floor(y).astype(np.int64)
+    xf, yf = x - xi, y - yi
+
+    def h(ix, iy):
+        z = (ix * 0x27D4EB2D + iy * 0x165667B1 + seed * 0x9E3779B1) & 0xFFFFFFFF
+        z = ((z ^ (z >> 15)) * 0x85EBCA6B) & 0xFFFFFFFF
+        z = ((z ^ (z >> 13)) * 0xC2B2AE35) & 0xFFFFFFFF
+        z = z ^ (z >> 16)
+        return (z & 0xFFFFFF) / float(0xFFFFFF) * 2.0 - 1.0
+
+    sx, sy = xf * xf * (3 - 2 * xf), yf * yf * (3 - 2 * yf)
+    n0 = h(xi, yi) * (1 - sx) + h(xi + 1, yi) * sx
+    n1 = h(xi, yi + 1) * (1 - sx) + h(xi + 1, yi + 1) * sx
+    return n0 * (1 - sy) + n1 * sy
+
+
+def fbm_grid(n: int, size=2.0, amplitude=0.5, seed=1):
+    """n x n quads over [-size/2, size/2]^2 displaced in y by 4-octave fbm of amplitude 0.5 m."""
+    g = np.linspace(-size / 2, size / 2, n + 1)
+    xx, zz = np.meshgrid(g, g, indexing="ij")
+    hgt = np.zeros_like(xx)
+    amp, freq = 1.0, 1.5
+    norm = 0.0
+    for o in range(4):
+        hgt += amp * _value_noise(xx * freq + 17.0 * o, zz * freq - 11.0 * o, seed + o)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    hgt = hgt / norm * amplitude
+    v = np.stack([xx, hgt, zz], -1).reshape(-1, 3).astype(np.float32)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a = i * (n + 1) + j
+    b = (i + 1) * (n + 1) + j
+    c = (i + 1) * (n + 1) + j + 1
+    d = i * (n + 1) + j + 1
+    t = np.concatenate([np.stack([a, d, c], -1).reshape(-1, 3), np.stack([a, c, b], -1).reshape(-1, 3)])
+    return v, t.astype(np.uint32)
+
+
+def mesh_for(kind: int, tris: float, seed: int):
+    if kind == 0:
+        n_lat = max(3, int(round(math.sqrt(tris / 4.0))))
+        return uv_sphere(n_lat, 2 * n_lat)
+    if kind == 1:
+        n_v = max(3, int(round(math.sqrt(tris / 4.0))))
+        return torus(2 * n_v, n_v)
+    n = max(2, int(round(math.sqrt(tris / 2.0))))
+    return fbm_grid(n, seed=seed)
+
+
+def random_rotation(rng: SplitMix64) -> np.ndarray:
+    """Uniform random rotation (Shoemake's quaternion method)."""
+    u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
+    q = (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+         math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def trs(t, R, s) -> np.ndarray:
+    m = np.eye(4)
+    m[:3, :3] = R * s
+    m[:3, 3] = t
+    return m
+
+
+# ---------------------------------------------------------------- workloads
+def room_boxes(res=(16, 16, 16), rays: int = 64, n_boxes: int = 64, depth_oct: int = 6, seed_id: int = 1,
+               hysteresis: float = 0.97) -> Workload:
+    """Room + boxes: one BLAS, one identity instance, 1 point + 1 directional light."""
+    rng = SplitMix64(SEED_BASE + seed_id)
+    b = SceneBuilder()
+    wall = b.add_material(uber_material((0.5, 0.5, 0.5), side=2))
+    rv, rt = room_mesh(10.0)
+    verts, tris, tmat = [rv], [rt], [np.full(len(rt), wall, np.uint32)]
+    base = len(rv)
+    for _ in range(n_boxes):
+        half = (rng.uniform(0.3, 2.0) / 2, rng.uniform(0.3, 2.0) / 2, rng.uniform(0.3, 2.0) / 2)
+        yaw = rng.uniform(0.0, 2 * math.pi)
+        ex = abs(math.cos(yaw)) * half[0] + abs(math.sin(yaw)) * half[2]
+        ez = abs(math.sin(yaw)) * half[0] + abs(math.cos(yaw)) * half[2]
+        ctr = (rng.uniform(ex, 10 - ex), rng.uniform(half[1], 10 - half[1]), rng.uniform(ez, 10 - ez))
+        alb = (rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8))
+        m = b.add_material(uber_material(alb))
+        bv, bt = box_mesh(ctr, half, yaw)
+        verts.append(bv)
+        tris.append(bt + base)
+        tmat.append(np.full(len(bt), m, np.uint32))
+        base += len(bv)
+    mesh = b.add_mesh(np.concatenate(verts), np.concatenate(tris), tri_material=np.concatenate(tmat))
+    b.add_instance(mesh)
+    lp = (rng.uniform(1, 9), rng.uniform(6, 9), rng.uniform(1, 9))
+    b.add_light(point_light(lp, intensity=10.0, distance=0.0, decay=2.0))
+    d = np.array([rng.uniform(-1, 1), -rng.uniform(0.2, 1.0), rng.uniform(-1, 1)])
+    b.add_light(directional_light((5.0, 20.0, 5.0), np.array((5.0, 20.0, 5.0)) + d, intensity=1.0))
+    scene = b.finalize(workload="room_boxes")
+    seed = SEED_BASE + seed_id
+    jitter = (seed * 0.6180339887) % 1.0
+    return Workload("room_boxes", scene, (0.25, 0.25, 0.25), (9.5, 9.5, 9.5), tuple(res), rays, depth_oct,
+                    float(np.float32(jitter)), hysteresis)
+
+
+def instanced(res=(32, 16, 24), rays: int = 256, total_tris: int = 262144, n_instances: int = 256,
+              n_lights: int = 16, depth_oct: int = 14, seed_id: int = 2, hysteresis: float = 0.97) -> Workload:
+    """8 procedural meshes x 256 random-TRS instances in a 40x20x25 m box, 16 point lights."""
+    rng = SplitMix64(SEED_BASE + seed_id)
+    box = np.array([40.0, 20.0, 25.0])
+    n_mesh = 8
+    target = [rng.log_uniform(1000, 64000) for _ in range(n_mesh)]
+    assign = [int(rng.uniform(0, n_mesh)) % n_mesh for _ in range(n_instances)]
+    counts = np.bincount(assign, minlength=n_mesh)
+    scale = total_tris / float(np.dot(counts, target))
+    b = SceneBuilder()
+    meshes, mesh_tris = [], []
+    for m in range(n_mesh):
+        want = target[m] * scale
+        v, t = mesh_for(m % 3, want, seed=m + 7)
+        alb = (rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8))
+        mat = b.add_material(uber_material(alb, side=2 if m % 3 == 2 else 0))
+        meshes.append(b.add_mesh(v, t, tri_material=np.full(len(t), mat, np.uint32)))
+        mesh_tris.append(len(t))
+    actual = int(np.dot(counts, mesh_tris))
+    for i in range(n_instances):
+        R = random_rotation(rng)
+        s = rng.uniform(0.5, 2.0)
+        t = (rng.uniform(0, box[0]), rng.uniform(0, box[1]), rng.uniform(0, box[2]))
+        b.add_instance(meshes[assign[i]], trs(t, R, s))
+    for _ in range(n_lights):
+        p = (rng.uniform(0, box[0]), rng.uniform(0, box[1]), rng.uniform(0, box[2]))
+        b.add_light(point_light(p, intensity=rng.log_uniform(1.0, 100.0), distance=0.0, decay=2.0))
+    scene = b.finalize(workload="instanced", instanced_triangles=actual, unique_triangles=int(sum(mesh_tris)))
+    seed = SEED_BASE + seed_id
+    jitter = (seed * 0.6180339887) % 1.0
+    return Workload("instanced", scene, (0.0, 0.0, 0.0), tuple(box.tolist()), tuple(res), rays, depth_oct,
+                    float(np.float32(jitter)), hysteresis, bounces=2)

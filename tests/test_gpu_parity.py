@@ -1,0 +1,166 @@
+"""GPU parity of the probe update against the CPU oracle (run on an MI355X with -m gpu).
+
+Bars (stated per test): hit triangle/instance ids identical except near-tie rays (two
+nearest accepted hits within 1e-6 relative, counted and reported); hit t within 1e-5
+relative; plain-FP32 radiance/irradiance within 1e-4 relative of the FP32 restatement;
+MFMA blends within 1e-3 relative of the FP64 oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from speedball_amd import _native as N
+from speedball_amd.field import ProbeField
+from tests.parity import rel_err, summary, interior_irr, near_tie_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def make_field(w, precision=N.BLEND_FP32, keep=True, res=None):
+    f = ProbeField(res or w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=precision,
+                   keep_hit_ids=keep)
+    f.set_scene(w.scene)
+    return f
+
+
+def params(w, **kw):
+    return dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis, **kw)
+
+
+@pytest.fixture(scope="module")
+def room_run(room):
+    """Two chained updates of the full 16^3 x 64 field on the GPU (FP32 blend) and in the
+    FP32 restatement and FP64 oracle."""
+    w = room
+    f = make_field(w)
+    gpu = []
+    for _ in range(2):
+        f.tick(**params(w))
+        gpu.append(dict(rays=f.read_rays(), hits=f.read_hit_ids(), irr=f.read_irr(), depth=f.read_depth(),
+                        atlas=f.read_atlas(), datlas=f.read_atlas(depth=True)))
+    stats = f.stats()
+    f.close()
+    refs = {}
+    for prec in ("32", "64"):
+        o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec=prec)
+        p = o.params(**params(w))
+        steps = []
+        for _ in range(2):
+            o.update(p)
+            steps.append(dict(rays=o.rays_view().copy(), hits=o.hits_view().copy(), irr=o.irr_view().copy(),
+                              depth=o.depth_view().copy(), atlas=o.atlas_view().copy(),
+                              datlas=o.atlas_view(depth=True).copy()))
+        refs[prec] = steps
+    return w, gpu, refs, stats
+
+
+def test_room_hit_ids_and_t(room_run):
+    w, gpu, refs, _ = room_run
+    g, r = gpu[0], refs["64"][0]
+    mism = np.any(g["hits"] != r["hits"], axis=-1)
+    slots, ks = np.nonzero(mism)
+    ties = near_tie_mask(oracle, w.scene, w, slots, ks) if slots.size else np.zeros(0, bool)
+    print(f"\nrays {mism.size}, id mismatches {int(mism.sum())}, of which near ties {int(ties.sum())}")
+    assert int((~ties).sum()) == 0, "hit ids differ on rays that are not near ties"
+    hit = (r["hits"][..., 0] >= 0) & ~mism
+    et = rel_err(g["rays"][..., 3][hit], r["rays"][..., 3][hit], floor_frac=0)
+    print("hit t rel err", summary(et))
+    assert et.max() <= 1e-5
+    assert np.all(g["rays"][..., 3][r["hits"][..., 0] < 0] == -1.0)
+
+
+def test_room_radiance_fp32(room_run):
+    w, gpu, refs, _ = room_run
+    for step in range(2):
+        g, r = gpu[step], refs["32"][step]
+        same = np.all(g["hits"] == r["hits"], axis=-1)
+        e = rel_err(g["rays"][..., :3][same], r["rays"][..., :3][same])
+        print(f"\nstep {step} ray radiance vs FP32 restatement", summary(e))
+        assert np.percentile(e, 99.9) <= 1e-4
+
+
+def test_room_irradiance_fp32(room_run):
+    w, gpu, refs, _ = room_run
+    for step in range(2):
+        g = interior_irr(gpu[step]["irr"])
+        r32 = interior_irr(refs["32"][step]["irr"])
+        r64 = interior_irr(refs["64"][step]["irr"])
+        e32 = rel_err(g[..., :3], r32[..., :3])
+        e64 = rel_err(g[..., :3], r64[..., :3])
+        print(f"\nstep {step} irradiance rel err vs FP32 {summary(e32)} vs FP64 {summary(e64)}")
+        assert np.percentile(e32, 99) <= 1e-4
+        assert np.percentile(e64, 99) <= 1e-3
+
+
+def test_room_depth_and_atlas(room_run):
+    w, gpu, refs, _ = room_run
+    for step in range(2):
+        g, r = gpu[step], refs["32"][step]
+        gd = g["depth"].reshape(-1, 64, 2)
+        rd = r["depth"].reshape(-1, 64, 2)
+        ed = rel_err(gd, rd)
+        print(f"\nstep {step} depth moments rel err {summary(ed)}")
+        assert np.percentile(ed, 99) <= 1e-4
+        # atlas: FP16 values; allow one half-ulp step of difference
+        ea = rel_err(g["atlas"][..., :3], r["atlas"][..., :3])
+        print(f"step {step} atlas rel err {summary(ea)}")
+        assert np.percentile(ea, 99) <= 2e-3
+        assert np.all(g["atlas"][..., 3] == 1.0)
+
+
+def test_room_stats(room_run):
+    _, _, _, s = room_run
+    print(f"\ntrace {s.ms_trace:.3f} ms blend {s.ms_blend:.3f} ms upload {s.ms_upload:.3f} ms total {s.ms_total:.3f}")
+    assert s.updated == 4096 and s.rays == 4096 * 64
+
+
+@pytest.mark.parametrize("precision,bar", [(N.BLEND_BF16X3, 1e-4), (N.BLEND_FP16, 1e-3), (N.BLEND_BF16, 4e-3)])
+def test_room_mfma_blend(room, precision, bar):
+    """MFMA blend against the FP64 oracle blend of the SAME GPU ray data (isolates the blend)."""
+    w = room
+    f = make_field(w, precision=precision)
+    f.tick(**params(w))
+    rays = f.read_rays()
+    irr = f.read_irr()
+    depth = f.read_depth()
+    f.close()
+    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec="64")
+    o.ray_data = rays.astype(np.float64).reshape(-1).copy()
+    o.blend(o.params(**params(w)))
+    e = rel_err(interior_irr(irr)[..., :3], interior_irr(o.irr_view())[..., :3])
+    ed = rel_err(depth.reshape(-1, 64, 2), o.depth_view())
+    print(f"\nprecision {precision}: irradiance rel err {summary(e)} depth {summary(ed)}")
+    assert np.percentile(e, 99) <= bar
+    assert np.percentile(ed, 99) <= bar
+
+
+def test_instanced_subset(instanced_small):
+    """Two-level instanced scene (reduced size): hit ids, t and radiance."""
+    w = instanced_small
+    f = make_field(w, res=w.res)
+    f.tick(**params(w))
+    g_rays, g_hits = f.read_rays(), f.read_hit_ids()
+    f.tick(**params(w))
+    g_irr2 = f.read_irr()
+    f.close()
+    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec="32")
+    p = o.params(**params(w))
+    o.update(p)
+    r_rays, r_hits = o.rays_view().copy(), o.hits_view().copy()
+    o.update(p)
+    mism = np.any(g_hits != r_hits, axis=-1)
+    slots, ks = np.nonzero(mism)
+    ties = near_tie_mask(oracle, w.scene, w, slots, ks) if slots.size else np.zeros(0, bool)
+    print(f"\ninstanced: rays {mism.size} id mismatches {int(mism.sum())} near ties {int(ties.sum())}")
+    assert int((~ties).sum()) <= max(1, mism.size // 20000)
+    same = ~mism
+    et = rel_err(g_rays[..., 3][same & (r_hits[..., 0] >= 0)], r_rays[..., 3][same & (r_hits[..., 0] >= 0)], 0)
+    assert et.max() <= 1e-5
+    e = rel_err(g_rays[..., :3][same], r_rays[..., :3][same])
+    print("instanced radiance", summary(e))
+    assert np.percentile(e, 99.9) <= 1e-4
+    ei = rel_err(interior_irr(g_irr2)[..., :3], interior_irr(o.irr_view())[..., :3])
+    print("instanced irradiance after 2 updates", summary(ei))
+    assert np.percentile(ei, 99) <= 1e-3

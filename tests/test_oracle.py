@@ -1,0 +1,302 @@
+"""Pin the CPU oracle before trusting it (CPU only).
+
+The reference ships no tests or golden vectors for this path (its test scripts are
+absent from the repository), so each oracle function is pinned by mathematics that a
+plausible mistake would break: closed forms, invariants, brute force, and numpy's own
+IEEE half rounding. The committed fixture tests/golden/room_4x4x4.npz (written by
+tests/golden/make_golden.py from the FP64 oracle) guards against silent drift.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from speedball_amd import scenegen as sg
+
+
+# ---------------------------------------------------------------- octahedral map
+@pytest.mark.parametrize("prec", ["64", "32"])
+def test_oct_round_trip_and_seams(prec):
+    rng = np.random.default_rng(0)
+    tol = 1e-12 if prec == "64" else 2e-6
+    for _ in range(500):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        uv = oracle.oct_encode(d, prec)
+        assert 0 <= uv[0] <= 1 and 0 <= uv[1] <= 1
+        back = oracle.oct_decode(uv[0], uv[1], prec)
+        assert np.abs(back - d).max() < tol
+    # +Z at the centre, -Z at the corners, equator on the diamond (gi_oct.js:15-19)
+    assert np.allclose(oracle.oct_encode([0, 0, 1], prec), [0.5, 0.5])
+    assert np.allclose(oracle.oct_decode(0.0, 0.0, prec), [0, 0, -1], atol=1e-7)
+    assert np.allclose(oracle.oct_decode(1.0, 1.0, prec), [0, 0, -1], atol=1e-7)
+    assert np.allclose(oracle.oct_encode([1, 0, 0], prec), [1.0, 0.5])
+    # opposite tile edges map to the same direction (the gutter copy relies on it)
+    for t in np.linspace(0.05, 0.95, 7):
+        a = oracle.oct_decode(0.0, t, prec)
+        b = oracle.oct_decode(0.0, 1 - t, prec)
+        assert np.allclose(a, b, atol=1e-6)
+
+
+# ---------------------------------------------------------------- rayDir
+def test_ray_dir_closed_form():
+    """Spherical Fibonacci with Cranley-Patterson rotation (gi_probes.js:1692-1704)."""
+    R, j = 64, 0.3
+    ga = math.pi * (3 - math.sqrt(5))
+    zs = []
+    for k in range(R):
+        d = oracle.ray_dir(k, R, j)
+        u = ((k + 0.5) / R + j) % 1.0
+        z = 1 - 2 * u
+        r = math.sqrt(max(0.0, 1 - z * z))
+        phi = k * ga + 2 * math.pi * j
+        assert np.allclose(d, [r * math.cos(phi), r * math.sin(phi), z], atol=1e-12)
+        zs.append(d[2])
+    # stratified: the z values are exactly the R cell centres of [-1, 1] (rotated)
+    assert np.allclose(np.sort(zs), np.sort([1 - 2 * (((k + 0.5) / R + j) % 1.0) for k in range(R)]))
+    # jitter 0 gives the unrotated set with mean direction ~ 0
+    m = np.mean([oracle.ray_dir(k, 256, 0.0) for k in range(256)], axis=0)
+    assert np.linalg.norm(m) < 0.02
+
+
+def test_ray_dir_fp32_matches_fp64():
+    for k in (0, 7, 63, 255):
+        a = oracle.ray_dir(k, 256, 0.618, "32")
+        b = oracle.ray_dir(k, 256, 0.618, "64")
+        assert np.abs(a - b).max() < 5e-5
+
+
+# ---------------------------------------------------------------- traversal
+def test_closest_equals_brute_force_room(room):
+    """Closest hit is BVH-topology invariant: the threaded walk equals brute force."""
+    rng = np.random.default_rng(1)
+    s = room.scene
+    for _ in range(1500):
+        ro = rng.uniform(0.3, 9.7, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        t, tri, inst = oracle.closest(s, ro, d)
+        tb, trb, ib, _ = oracle.brute_closest(s, ro, d)
+        assert (tri, inst) == (trb, ib) and (tri < 0 or t == tb)
+
+
+def test_closest_equals_brute_force_instanced(instanced_small):
+    rng = np.random.default_rng(2)
+    s = instanced_small.scene
+    hits = 0
+    for _ in range(600):
+        ro = rng.uniform([0, 0, 0], [40, 20, 25])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        t, tri, inst = oracle.closest(s, ro, d)
+        tb, trb, ib, _ = oracle.brute_closest(s, ro, d)
+        assert (tri, inst) == (trb, ib)
+        hits += tri >= 0
+    assert hits > 20
+
+
+def test_any_hit_consistent_with_closest(instanced_small):
+    """any(tmax) == (closest accepted hit < tmax) when every material blocks shadows."""
+    rng = np.random.default_rng(3)
+    s = instanced_small.scene
+    for _ in range(600):
+        ro = rng.uniform([0, 0, 0], [40, 20, 25])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        tmax = rng.uniform(0.5, 40)
+        tb, trb, _, _ = oracle.brute_closest(s, ro, d)
+        assert oracle.any_hit(s, ro, d, tmax) == (trb >= 0 and tb < tmax)
+
+
+def test_moller_trumbore_analytic_plane():
+    """A ray straight down onto the room floor hits at the exact height."""
+    w = sg.room_boxes(n_boxes=0)
+    for x, z in [(2.5, 3.5), (7.1, 1.2), (5.0, 5.0)]:
+        t, tri, inst = oracle.closest(w.scene, [x, 4.0, z], [0, -1, 0])
+        assert tri >= 0 and abs(t - 4.0) < 1e-12
+
+
+def test_side_rules():
+    """Front-side boxes are invisible from inside; the double-sided room is not."""
+    w = sg.room_boxes(n_boxes=1)
+    s = w.scene
+    # find the box: its triangles are 12..23; shoot from its centre outwards
+    v = s.vertex_data.reshape(-1, 8)[:, :3]
+    ti = s.tri_index.reshape(-1, 3)
+    box_tris = [t for t in range(s.tri_count) if s.tri_material[t] != 0]
+    c = v[ti[box_tris].reshape(-1)].mean(0).astype(np.float64)
+    t, tri, _ = oracle.closest(s, c, [1, 0, 0])
+    assert s.tri_material[tri] == 0  # passes through the box's back face, hits a wall
+
+
+# ---------------------------------------------------------------- blend / upload
+def _uniform_radiance_field(value, prec):
+    w = sg.room_boxes(res=(3, 3, 3), rays=64)
+    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, 64, 6, prec=prec)
+    p = o.params(frame_jitter=0.2, hysteresis=0.97)
+    o.ray_data = np.zeros(27 * 64 * 4, o.dt)
+    o.ray_data[0::4], o.ray_data[1::4], o.ray_data[2::4] = value
+    o.ray_data[3::4] = 2.5
+    return o, p
+
+
+@pytest.mark.parametrize("prec", ["64", "32"])
+def test_blend_uniform_radiance_is_exact(prec):
+    """Uniform radiance + first update (h = 0) -> every interior texel equals it,
+    depth moments equal (d, d^2), and E[L^2] = L^2."""
+    val = (0.25, 0.5, 1.0)
+    o, p = _uniform_radiance_field(val, prec)
+    o.blend(p)
+    irr = o.irr_view()
+    idx = [(ly + 1) * 8 + lx + 1 for ly in range(6) for lx in range(6)]
+    tol = 1e-12 if prec == "64" else 2e-6
+    assert np.allclose(irr[:, idx, :3], val, rtol=tol)
+    L = 0.2126 * val[0] + 0.7152 * val[1] + 0.0722 * val[2]
+    assert np.allclose(irr[:, idx, 3], L * L, rtol=tol * 4)
+    d = o.depth_view()
+    assert np.allclose(d[:, idx, 0], 2.5, rtol=tol) and np.allclose(d[:, idx, 1], 6.25, rtol=tol)
+    # gutter texels are never written by the blend
+    assert np.all(irr[:, 0, :] == 0)
+
+
+def test_blend_cosine_weighted_mean_closed_form():
+    """First update = sum_k cw_k x_k / sum_k cw_k with cw = max(dir.r_k, 0)."""
+    o, p = _uniform_radiance_field((0, 0, 0), "64")
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0, 1, (27, 64))
+    o.ray_data[0::4] = x.reshape(-1)
+    o.blend(p)
+    dirs = np.array([oracle.ray_dir(k, 64, p.frame_jitter) for k in range(64)])
+    for t in (0, 14, 35):
+        tx, ty = t % 6, t // 6
+        dt = oracle.oct_decode((tx + 0.5) / 6, (ty + 0.5) / 6)
+        cw = np.maximum(dirs @ dt, 0)
+        want = (x * cw).sum(1) / max(cw.sum(), 1e-4)
+        got = o.irr_view()[:, (ty + 1) * 8 + tx + 1, 0]
+        assert np.allclose(got, want, rtol=1e-12)
+
+
+def test_blend_hysteresis_steady_state():
+    """A second update with identical input keeps the value (no change, no clamp)."""
+    o, p = _uniform_radiance_field((0.3, 0.3, 0.3), "64")
+    o.blend(p)
+    a = o.irr_view().copy()
+    o.blend(p)
+    assert np.allclose(o.irr_view(), a, rtol=1e-12)
+
+
+def test_blend_hysteresis_mixes_with_history():
+    """Small change: h = pow(h_noise, exp) with h_noise = hyst + (1-hyst)*0.25."""
+    o, p = _uniform_radiance_field((0.3, 0.3, 0.3), "64")
+    o.blend(p)
+    o.ray_data[0::4] *= 1.001
+    o.ray_data[1::4] *= 1.001
+    o.ray_data[2::4] *= 1.001
+    o.blend(p)
+    h = 0.97 + 0.03 * 0.25
+    want = 0.3 * 1.001 * (1 - h) + 0.3 * h
+    assert np.allclose(o.irr_view()[:, 9, 0], want, rtol=1e-9)
+
+
+@pytest.mark.parametrize("prec", ["64", "32"])
+def test_upload_gutters_and_fp16(prec):
+    o, p = _uniform_radiance_field((0, 0, 0), prec)
+    rng = np.random.default_rng(5)
+    o.ray_data[0::4] = rng.uniform(0, 2, 27 * 64)
+    o.ray_data[1::4] = rng.uniform(0, 2, 27 * 64)
+    o.blend(p)
+    o.upload(p)
+    a = o.atlas_view()
+    tile = a[0:8, 0:8]  # probe 0
+    # gutters mirror the interior (gi_probes.js:2810-2837)
+    for i in range(1, 7):
+        assert np.array_equal(tile[0, i], tile[1, 7 - i]) and np.array_equal(tile[7, i], tile[6, 7 - i])
+        assert np.array_equal(tile[i, 0], tile[7 - i, 1]) and np.array_equal(tile[i, 7], tile[7 - i, 6])
+    assert np.array_equal(tile[0, 0], tile[6, 6]) and np.array_equal(tile[7, 7], tile[1, 1])
+    assert np.all(a[..., 3] == 1)
+    # values are exactly representable in binary16
+    assert np.array_equal(a.astype(np.float16).astype(a.dtype), a)
+
+
+def test_half_round_matches_numpy():
+    rng = np.random.default_rng(6)
+    xs = np.concatenate([rng.normal(0, 10, 3000), rng.normal(0, 1e-5, 1000), [65504, 65519, 65520, 1e-8, 0.0]])
+    for x in xs.astype(np.float32):
+        assert oracle.half_round(float(x)) == float(np.float16(x)), x
+
+
+def test_filter_strength_zero_is_identity():
+    o, p = _uniform_radiance_field((0, 0, 0), "64")
+    rng = np.random.default_rng(7)
+    o.ray_data[0::4] = rng.uniform(0, 1, 27 * 64)
+    o.blend(p)
+    p.filter_strength = 0.0
+    o.upload(p)
+    a = o.atlas_view()
+    want = o.irr_view()[0, 1 * 8 + 1, 0]
+    assert a[1, 1, 0] == np.float16(want)
+
+
+# ---------------------------------------------------------------- refit
+def test_refit_blas_equals_fresh_build(instanced_small):
+    """Perturb every vertex, refit, and compare with bounds recomputed from scratch."""
+    s = instanced_small.scene
+    rng = np.random.default_rng(8)
+    scene2 = sg.Scene(**{k: getattr(s, k) for k in ("bvh_nodes", "tri_index", "vertex_data", "tri_material",
+                                                      "materials", "lights", "inst_base", "inst_count",
+                                                      "tlas_base", "tlas_node_count", "tlas_order")})
+    vd = s.vertex_data.copy().reshape(-1, 8)
+    vd[:, :3] += rng.normal(0, 0.05, (vd.shape[0], 3)).astype(np.float32)
+    scene2.vertex_data = np.ascontiguousarray(vd.reshape(-1))
+    nodes = s.bvh_nodes.copy()
+    assert oracle.refit_blas(nodes, scene2, 0, s.node_count)
+    nf = nodes.view(np.float32).reshape(-1, 8)
+    nu = nodes.reshape(-1, 8)
+    tri = s.tri_index.reshape(-1, 3)
+    # brute: every node's bounds = bounds of the triangles in its subtree
+    for i in rng.choice(s.node_count, 300, replace=False):
+        lo, hi = np.full(3, np.inf, np.float32), np.full(3, -np.inf, np.float32)
+        for j in range(i, nu[i, 6]):
+            if nu[j, 7] != 0xFFFFFFFF:
+                off, cnt = nu[j, 7] & 0xFFFFFF, nu[j, 7] >> 24
+                p = vd[tri[off:off + cnt].reshape(-1), :3]
+                lo = np.minimum(lo, p.min(0))
+                hi = np.maximum(hi, p.max(0))
+        assert np.array_equal(nf[i, 0:3], lo) and np.array_equal(nf[i, 3:6], hi)
+
+
+def test_refit_blas_fails_closed_on_corrupt_layout(room):
+    s = room.scene
+    nodes = s.bvh_nodes.copy()
+    nu = nodes.reshape(-1, 8)
+    interior = np.nonzero(nu[:, 7] == 0xFFFFFFFF)[0][0]
+    nu[interior + 1, 6] = interior  # left child's miss points backwards
+    assert not oracle.refit_blas(nodes, s, 0, s.node_count)
+
+
+def test_refit_tlas_matches_bounds(instanced_small):
+    s = instanced_small.scene
+    tl = s.materials[s.tlas_base:s.tlas_base + s.tlas_node_count * 12].copy()
+    rng = np.random.default_rng(9)
+    ib = rng.uniform(-5, 5, (s.inst_count, 6)).astype(np.float32)
+    ib[:, 3:] = ib[:, :3] + np.abs(ib[:, 3:])
+    assert oracle.refit_tlas(tl, ib.reshape(-1).copy())
+    t = tl.reshape(-1, 12)
+    assert np.array_equal(t[0, 0:3], ib[:, :3].min(0)) and np.array_equal(t[0, 3:6], ib[:, 3:].max(0))
+
+
+# ---------------------------------------------------------------- golden fixture
+def test_golden_room_fixture():
+    path = os.path.join(os.path.dirname(__file__), "golden", "room_4x4x4.npz")
+    g = np.load(path, allow_pickle=False)
+    w = sg.room_boxes(res=(4, 4, 4), rays=64)
+    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, 64, 6, prec="64")
+    p = o.params(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+    o.update(p)
+    assert np.array_equal(o.hit_ids.reshape(g["hit_ids"].shape), g["hit_ids"])
+    assert np.allclose(o.irr_view(), g["irr"], rtol=1e-12, atol=0)
+    assert np.allclose(o.atlas_view(), g["atlas"], rtol=0, atol=0)

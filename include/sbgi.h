@@ -86,6 +86,7 @@ extern "C" {
 #define SBGI_BUF_RAY_DATA 15    /* f32[4*updated*R] reference order (slot*R+k)  (read) */
 #define SBGI_BUF_HIT_IDS 16     /* i32[2*updated*R] (tri, inst) per ray, same order    */
 #define SBGI_BUF_NODES_REFIT 17 /* bvhNodes after a device refit (read)               */
+#define SBGI_BUF_TRI_PACKED 18  /* f32[12*T] packed triangle records (read, debug)    */
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {
@@ -256,7 +257,8 @@ int32_t sbgi_set_counting(sbgi_field *f, int32_t on);
 /* Ray queries against the resident scene: the closest-hit / any-hit walks that the
  * trace uses (traverseClosest / traverseAny, js/spectral_traverse.js:441-474, 539-569),
  * for arbitrary rays. rays8 holds n records {ro.x, ro.y, ro.z, tmax, rd.x, rd.y, rd.z,
- * mode} with mode 0 = closest hit, 1 = any hit within (1e-3, tmax). out4 receives
+ * mode} with mode 0 = closest hit, 1 = any hit within (1e-3, tmax), 2 = closest hit
+ * reported as blocked when it lies before tmax. out4 receives
  * {t, tri (int32 bits), inst (int32 bits), blocked} per ray. Synchronous. */
 int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *out4);
 

@@ -26,7 +26,7 @@ FIELD_KEEP_HIT_IDS = 1
 
 BUF_BVH_NODES, BUF_TRI_INDEX, BUF_VERTEX_DATA, BUF_TRI_MATERIAL, BUF_MATERIALS, BUF_LIGHTS = 0, 1, 2, 3, 4, 5
 BUF_IRR_STATE, BUF_DEPTH_STATE, BUF_META, BUF_IRR_ATLAS, BUF_DEPTH_ATLAS = 10, 11, 12, 13, 14
-BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT = 15, 16, 17
+BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT, BUF_TRI_PACKED = 15, 16, 17, 18
 
 # every symbol include/sbgi.h declares
 EXPORTS = [

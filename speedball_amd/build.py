@@ -73,7 +73,28 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     shutil.move(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_addon(verbose)
     return LIB
+
+
+NAPI_DIR = os.path.join(PKG, "napi")
+ADDON = os.path.join(NAPI_DIR, "sbgi.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+
+
+def build_addon(verbose: bool = False):
+    """The N-API addon over the C ABI (skipped when the Node headers are absent)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print("node_api.h not found; addon skipped")
+        return None
+    tmp = ADDON + ".tmp"
+    _run(["gcc", "-O2", "-shared", "-fPIC", "-std=c11", "-Wall", "-DNODE_GYP_MODULE_NAME=sbgi", "-I", NODE_INCLUDE,
+          os.path.join(NAPI_DIR, "sbgi_napi.c"), "-L", PKG, "-lsbgi", "-Wl,-rpath,$ORIGIN/..", "-o", tmp])
+    shutil.move(tmp, ADDON)
+    if verbose:
+        print("built", ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":

@@ -58,6 +58,8 @@ struct sbgi_field {
     // solve scratch
     float4 *rays = nullptr, *dirs = nullptr;
     int2 *hit_ids = nullptr;
+    TraceScratch tw{};
+    size_t queue_capacity = 0;
     void *tables = nullptr;
     BlendTables bt{};
     Counters *counters = nullptr;
@@ -119,6 +121,8 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_tlas_parent);
     dfree(f->d_arrive);
     dfree(f->d_tlas_arrive);
+    dfree(f->tw.queue.rec);
+    f->tw.queue.capacity = 0;
     f->has_scene = false;
 }
 
@@ -156,6 +160,7 @@ DevScene make_scene(const sbgi_field *f) {
     s.tlas_count = f->view.tlas_node_count;
     s.light_count = f->view.light_count;
     s.tri_count = f->view.tri_count;
+    s.inst_count = f->view.inst_count;
     return s;
 }
 
@@ -394,6 +399,12 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
     if (dalloc(&f->rays, (size_t)f->max_total * R * 16) != hipSuccess || dalloc(&f->dirs, 256 * 16) != hipSuccess ||
         dalloc(&f->tables, tb) != hipSuccess || dalloc(&f->counters, sizeof(Counters)) != hipSuccess)
         return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+    if (dalloc(&f->tw.hits, (size_t)f->max_total * R * 16) != hipSuccess ||
+        dalloc(&f->tw.vis, (size_t)f->max_total * R * 8) != hipSuccess ||
+        dalloc(&f->tw.queue.count, 64) != hipSuccess)
+        return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+    if ((uint64_t)f->max_total * R >= (1ull << 26))
+        return bail(fail(f, SBGI_E_CAPACITY, "probes x rays per solve must stay below 2^26"));
     if (desc->flags & SBGI_FIELD_KEEP_HIT_IDS) {
         if (dalloc(&f->hit_ids, (size_t)f->max_total * R * 8) != hipSuccess)
             return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
@@ -455,6 +466,13 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
         free_scene(f);
         return fail(f, SBGI_E_NOMEM, "device allocation for the scene failed");
     }
+    // shadow queue: at most one record per (ray, light)
+    const size_t cap = (size_t)f->max_total * f->desc.rays_per_probe * std::max(1u, v.light_count);
+    if (cap >= (1ull << 32) || dalloc(&f->tw.queue.rec, cap * 32) != hipSuccess) {
+        free_scene(f);
+        return fail(f, SBGI_E_NOMEM, "device allocation for the shadow-ray queue failed");
+    }
+    f->tw.queue.capacity = (uint32_t)cap;
     auto up = [&](void *d, const void *h, size_t bytes) {
         return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, f->stream) : hipSuccess;
     };
@@ -622,7 +640,7 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
     HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
     HIP_OK(f, hipEventRecord(f->ev[1], st));
-    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, f->rays, f->hit_ids,
+    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, f->tw, f->rays, f->hit_ids,
                            f->count_mode ? f->counters : nullptr, st));
     HIP_OK(f, hipEventRecord(f->ev[2], st));
     HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, st));
@@ -692,6 +710,7 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
         case SBGI_BUF_LIGHTS: return f->h_lights.size() * 4;
         case SBGI_BUF_NODES_REFIT:
         case SBGI_BUF_BVH_NODES: return f->h_nodes.size() * 4;
+        case SBGI_BUF_TRI_PACKED: return f->has_scene ? (size_t)f->view.tri_count * 48 : 0;
     }
     return 0;
 }
@@ -731,6 +750,10 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
     }
     if (buf_id == SBGI_BUF_LIGHTS) {
         HIP_OK(f, hipMemcpy(dst, f->d_lights, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
+    if (buf_id == SBGI_BUF_TRI_PACKED) {
+        HIP_OK(f, hipMemcpy(dst, f->d_tri, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
     if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_NODES_REFIT) {
@@ -795,6 +818,9 @@ int32_t sbgi_destroy(sbgi_field *f) {
     }
     dfree(f->rays);
     dfree(f->dirs);
+    dfree(f->tw.hits);
+    dfree(f->tw.vis);
+    dfree(f->tw.queue.count);
     dfree(f->hit_ids);
     dfree(f->tables);
     dfree(f->counters);

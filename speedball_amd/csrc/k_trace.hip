@@ -3,12 +3,22 @@
 // Restates traceKernel (js/gi_probes.js:1977-2214) for the configuration the reference
 // ships by default: non-clustered NEE over all lights, no texture maps, reflections off.
 //
-// Work mapping (MI355X-first): thread -> (ray k, probe slot) with the probe slot
-// fastest-varying, so the 64 lanes of a wave trace the SAME direction k from 64
-// neighbouring probes. rayDir() does not depend on the probe (js/gi_probes.js:1692-1704),
-// so a wave walks the BVH with parallel rays from nearby origins — the direction-binned
-// ordering that keeps stackless traversal coherent on 64-wide wavefronts. Output is
-// stored k-major (rays[k * updated + slot]) so the stores coalesce and the blend reads
+// MI355X layout of the work, three launches per solve instead of one monolithic kernel:
+//   1. k_primary  persistent blocks; thread -> (ray k, probe slot) with the slot
+//                 fastest-varying, so a wave traces ONE direction k (rayDir ignores the
+//                 probe, js/gi_probes.js:1692-1704) from 64 neighbouring probes. Closest hit
+//                 with the TLAS and instance table staged in LDS. For every hit and every
+//                 light that passes the reference's gate (n.l > 0, in range, not band-gated)
+//                 a shadow ray is appended to a queue; appends are aggregated per wave with
+//                 a ballot + one atomic, so the queue is dense and each wave-chunk holds one
+//                 light (coherent shadow rays from neighbouring probes).
+//   2. k_shadow   persistent any-hit over the dense queue: every lane has a shadow ray, no
+//                 lane idles on a primary miss. Unblocked rays set bit `light` of the ray's
+//                 visibility mask.
+//   3. k_resolve  per ray: emissive + sum over lights, in light order, of the visible NEE
+//                 terms + the bounce from the previous atlas — the reference's summation
+//                 order, independent of the queue order, so the result is deterministic.
+// Output is stored k-major (rays[k * updated + slot]): stores coalesce and the blend reads
 // it as the K x N operand of its contraction.
 #include <hip/hip_fp16.h>
 
@@ -83,93 +93,211 @@ __device__ float3 sample_atlas(const DevCascade &c, const ushort4 *__restrict__ 
     return acc * (1.0f / fmaxf(wsum, 1e-4f));
 }
 
-template <bool COUNT>
-__global__ void __launch_bounds__(256) k_trace(DevScene s, DevCascade c, sbgi_tick_params p,
-                                                const float4 *__restrict__ dirs, const float *__restrict__ meta,
-                                                const ushort4 *__restrict__ prev_atlas, float4 *__restrict__ rays,
-                                                int2 *__restrict__ hit_ids, Counters *__restrict__ counters) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = c.updated * c.rays;
-    if (gid >= n) return;
+// ray origin and direction of ray gid (probeTraceOrigin js/gi_probes.js:1714-1726, 1876-1885)
+__device__ __forceinline__ void make_ray(const DevCascade &c, const sbgi_tick_params &p, const float4 *__restrict__ dirs,
+                                         const float *__restrict__ meta, uint32_t gid, float3 &ro, float3 &rd) {
     const uint32_t k = gid / c.updated;
     const uint32_t slot = gid - k * c.updated;
     const uint32_t probe = (c.offset + slot) % c.total;
-
-    // probeTraceOrigin (js/gi_probes.js:1714-1726, 1876-1885)
-    uint32_t ix = probe % c.res[0], iy = (probe / c.res[0]) % c.res[1], iz = probe / (c.res[0] * c.res[1]);
-    float3 ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
-                   c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
-                   c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
+    const uint32_t ix = probe % c.res[0], iy = (probe / c.res[0]) % c.res[1], iz = probe / (c.res[0] * c.res[1]);
+    ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
+            c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
+            c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
     if (p.classify_strength != 0.f) {
         const float *m = meta + (size_t)probe * 4;
         ro = ro + f3(m[1], m[2], m[3]) * p.classify_strength;
     }
-    float3 rd = xyz(dirs[k]);
+    rd = xyz(dirs[k]);
+}
 
+// Geometric shading frame of a hit (js/gi_probes.js:1993-2054).
+struct HitFrame {
+    float3 hit, ng, kd, emis;
+    float bias;
+};
+
+__device__ __forceinline__ HitFrame hit_frame(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p,
+                                              float3 ro, float3 rd, float t, uint32_t tri, uint32_t inst) {
+    HitFrame h;
+    const float4 *tp = s.tri + (size_t)tri * 3;
+    const float4 B = tp[1], C = tp[2];
+    const uint32_t m = __float_as_uint(B.w);
+    const float4 *ip = s.inst + (size_t)inst * 4;
+    const float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
+    float3 nl = normalize3(cross3(xyz(B), xyz(C)));
+    float3 ng = normalize3(xyz(r0) * nl.x + xyz(r1) * nl.y + xyz(r2) * nl.z);
+    h.ng = ng * (dot3(ng, rd) < 0.f ? 1.f : -1.f);
+    h.bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
+    h.hit = (ro + rd * t) + h.ng * h.bias;
+    const float *mat = s.mats + (size_t)m * SBGI_MAT_STRIDE;
+    const float metal = mat[4], transm = mat[5];
+    h.emis = f3(mat[7], mat[8], mat[9]) * p.dbg_emissive;
+    h.kd.x = fminf(fmaxf(mat[0] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
+    h.kd.y = fminf(fmaxf(mat[1] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
+    h.kd.z = fminf(fmaxf(mat[2] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
+    return h;
+}
+
+// One light of the NEE loop (js/gi_probes.js:2129-2176): the gate before the shadow ray,
+// and the contribution when it is not blocked.
+struct LightSample {
+    float3 wi, lcol;
+    float dist, ndl;
+    bool pass;
+};
+
+__device__ __forceinline__ LightSample light_gate(const float *__restrict__ L, const HitFrame &h,
+                                                  const sbgi_tick_params &p) {
+    LightSample ls;
+    const float ltype = L[0];
+    ls.pass = false;
+    if (!(ltype < 2.5f)) return ls;
+    const bool is_ir = fabsf(L[16] - 4.0f) < 0.25f;
+    const float gate = is_ir ? p.nir_gate * p.nir_gain : 1.f;
+    ls.lcol = f3(L[7], L[8], L[9]) * gate;
+    const float lrange = L[10];
+    const bool is_dir = ltype < 0.5f;
+    const float3 ldir = f3(L[4], L[5], L[6]);
+    const float3 tl = is_dir ? ldir * -1.f : f3(L[1], L[2], L[3]) - h.hit;
+    ls.dist = is_dir ? 1e4f : fmaxf(sqrtf(dot3(tl, tl)), 1e-4f);
+    ls.wi = normalize3(tl);
+    ls.ndl = fmaxf(dot3(h.ng, ls.wi), 0.f);
+    const float lpeak = fmaxf(fmaxf(ls.lcol.x, ls.lcol.y), ls.lcol.z);
+    ls.pass = ls.ndl * lpeak > 0.f && (is_dir || lrange <= 0.f || ls.dist < lrange);
+    return ls;
+}
+
+__device__ __forceinline__ float3 light_contrib(const float *__restrict__ L, const HitFrame &h, const LightSample &ls,
+                                                const sbgi_tick_params &p) {
+    const float ltype = L[0];
+    const float lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
+    const bool is_dir = ltype < 0.5f;
+    const bool is_spot = fabsf(ltype - 2.0f) < 0.5f;
+    const float3 ldir = f3(L[4], L[5], L[6]);
+    const float falloff = 1.0f / fmaxf(powf(ls.dist, ldecay), 0.01f);
+    const float rr = ls.dist / fmaxf(lrange, 1e-4f);
+    const float rr2 = rr * rr;
+    const float win = fminf(fmaxf(1.f - rr2 * rr2, 0.f), 1.f);
+    const float ranged = falloff * (win * win);
+    const float pos_att = lrange > 0.f ? ranged : falloff;
+    const float dist_att = is_dir ? 1.f : pos_att;
+    const float acos_ = dot3(ls.wi, ldir) * -1.f;
+    const float spot = fminf(fmaxf((acos_ - lca) / fmaxf(lcp - lca, 1e-4f), 0.f), 1.f);
+    const float att = dist_att * (is_spot ? spot : 1.f);
+    const float3 diffuse = h.kd * (1.0f / SBGI_PI);
+    return ((diffuse * ls.ndl) * ls.lcol) * (att * p.dbg_direct);
+}
+
+constexpr uint32_t kBlock = 256;
+
+// Stage 1: closest hit + shadow-ray emission. hits[gid] = {t, tri, inst, -}.
+template <bool COUNT, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_primary(DevScene s, DevCascade c, sbgi_tick_params p,
+                                                     const float4 *__restrict__ dirs, const float *__restrict__ meta,
+                                                     float4 *__restrict__ hits, ShadowQueue q,
+                                                     Counters *__restrict__ counters) {
+    if (LDS) stage_tlas_lds(s);
+    const uint32_t n = c.updated * c.rays;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
     TravCounters<COUNT> cnt;
-    float best_t;
-    int best_tri, best_inst;
-    trace_closest<COUNT>(s, ro, rd, best_t, best_tri, best_inst, cnt);
+    uint32_t shadow = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t gid = base + threadIdx.x;
+        const bool active = gid < n;
+        float3 ro = f3(0.f, 0.f, 0.f), rd = f3(0.f, 0.f, 1.f);
+        float best_t = SBGI_T_MAX;
+        int best_tri = -1, best_inst = -1;
+        if (active) {
+            make_ray(c, p, dirs, meta, gid, ro, rd);
+            trace_closest<COUNT, LDS>(s, ro, rd, best_t, best_tri, best_inst, cnt);
+            hits[gid] = make_float4(best_t, __int_as_float(best_tri), __int_as_float(best_inst), 0.f);
+        }
+        const bool hit = active && best_tri >= 0;
+        // every wave runs the light loop together (ballots), non-hit lanes emit nothing
+        const bool wave_hit = __ballot(hit) != 0ull;
+        if (!wave_hit) continue;
+        HitFrame h;
+        if (hit) h = hit_frame(s, c, p, ro, rd, best_t, (uint32_t)best_tri, (uint32_t)best_inst);
+        for (uint32_t li = 0; li < s.light_count; li++) {
+            LightSample ls;
+            ls.pass = false;
+            if (hit) ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p);
+            const uint64_t m = __ballot(ls.pass);
+            if (m == 0ull) continue;
+            uint32_t first = 0;
+            if (lane == (uint32_t)__ffsll((unsigned long long)m) - 1u) first = atomicAdd(q.count, (uint32_t)__popcll(m));
+            first = __shfl(first, __ffsll((unsigned long long)m) - 1);
+            if (ls.pass) {
+                const uint32_t idx = first + (uint32_t)__popcll(m & lt_mask);
+                if (idx < q.capacity) {
+                    q.rec[(size_t)idx * 2] = make_float4(h.hit.x, h.hit.y, h.hit.z, ls.dist - h.bias);
+                    q.rec[(size_t)idx * 2 + 1] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, __uint_as_float((gid << 6) | li));
+                }
+                shadow++;
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
+        atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
+        atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
+        atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
+        atomicAdd(&counters->shadow_rays, (unsigned long long)shadow);
+    }
+}
 
+// Stage 2: any-hit over the dense shadow queue; unblocked -> visibility bit.
+template <bool COUNT, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_shadow(DevScene s, ShadowQueue q, unsigned long long *__restrict__ vis,
+                                                    Counters *__restrict__ counters) {
+    if (LDS) stage_tlas_lds(s);
+    const uint32_t n = min(*q.count, q.capacity);
+    TravCounters<COUNT> cnt;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const float4 a = q.rec[(size_t)i * 2], b = q.rec[(size_t)i * 2 + 1];
+        if (!trace_any<COUNT, LDS>(s, xyz(a), xyz(b), a.w, cnt)) {
+            const uint32_t w = __float_as_uint(b.w);
+            atomicOr(&vis[w >> 6], 1ull << (w & 63u));
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
+        atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
+        atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
+        atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
+    }
+}
+
+// Stage 3: shading in the reference's summation order.
+__global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sbgi_tick_params p,
+                                                     const float4 *__restrict__ dirs, const float *__restrict__ meta,
+                                                     const ushort4 *__restrict__ prev_atlas,
+                                                     const float4 *__restrict__ hits,
+                                                     const unsigned long long *__restrict__ vis,
+                                                     float4 *__restrict__ rays, int2 *__restrict__ hit_ids) {
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t n = c.updated * c.rays;
+    if (gid >= n) return;
+    float3 ro, rd;
+    make_ray(c, p, dirs, meta, gid, ro, rd);
+    const float4 hr = hits[gid];
+    const int tri = __float_as_int(hr.y), inst = __float_as_int(hr.z);
     float3 out;
     float hit_t = -1.f;
-    uint32_t shadow = 0;
-    if (best_tri >= 0) {
-        hit_t = best_t;
-        const uint32_t tri = (uint32_t)best_tri, inst = (uint32_t)best_inst;
-        const float4 *tp = s.tri + (size_t)tri * 3;
-        const float4 B = tp[1], C = tp[2];
-        const uint32_t m = __float_as_uint(B.w);
-        float4 r0, r1, r2, r3;
-        inst_rows(s, inst, r0, r1, r2, r3);
-        // geometric normal through (M^-1)^T, face-forwarded (js/gi_probes.js:2006-2011)
-        float3 nl = normalize3(cross3(xyz(B), xyz(C)));
-        float3 ng = normalize3(xyz(r0) * nl.x + xyz(r1) * nl.y + xyz(r2) * nl.z);
-        ng = ng * (dot3(ng, rd) < 0.f ? 1.f : -1.f);
-        const float bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
-        const float3 hit = (ro + rd * best_t) + ng * bias;
-        const float *mat = s.mats + (size_t)m * SBGI_MAT_STRIDE;
-        const float metal = mat[4], transm = mat[5];
-        float3 rad = f3(mat[7], mat[8], mat[9]) * p.dbg_emissive;
-        float3 kd;
-        kd.x = fminf(fmaxf(mat[0] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
-        kd.y = fminf(fmaxf(mat[1] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
-        kd.z = fminf(fmaxf(mat[2] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
-        const float3 diffuse = kd * (1.0f / SBGI_PI);
-        // NEE over all lights (js/gi_probes.js:2129-2176)
+    if (tri >= 0) {
+        hit_t = hr.x;
+        const HitFrame h = hit_frame(s, c, p, ro, rd, hr.x, (uint32_t)tri, (uint32_t)inst);
+        float3 rad = h.emis;
+        const unsigned long long vm = vis[gid];
         for (uint32_t li = 0; li < s.light_count; li++) {
+            if (!((vm >> li) & 1ull)) continue;
             const float *L = s.lights + li * SBGI_LIGHT_STRIDE;
-            const float ltype = L[0];
-            if (!(ltype < 2.5f)) continue;
-            const bool is_ir = fabsf(L[16] - 4.0f) < 0.25f;
-            const float gate = is_ir ? p.nir_gate * p.nir_gain : 1.f;
-            const float3 lcol = f3(L[7], L[8], L[9]) * gate;
-            const float lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
-            const bool is_dir = ltype < 0.5f;
-            const bool is_spot = fabsf(ltype - 2.0f) < 0.5f;
-            const float3 ldir = f3(L[4], L[5], L[6]);
-            const float3 tl = is_dir ? ldir * -1.f : f3(L[1], L[2], L[3]) - hit;
-            const float dist = is_dir ? 1e4f : fmaxf(sqrtf(dot3(tl, tl)), 1e-4f);
-            const float3 wi = normalize3(tl);
-            const float ndl = fmaxf(dot3(ng, wi), 0.f);
-            const float lpeak = fmaxf(fmaxf(lcol.x, lcol.y), lcol.z);
-            if (!(ndl * lpeak > 0.f && (is_dir || lrange <= 0.f || dist < lrange))) continue;
-            shadow++;
-            if (trace_any<COUNT>(s, hit, wi, dist - bias, cnt)) continue;
-            const float falloff = 1.0f / fmaxf(powf(dist, ldecay), 0.01f);
-            const float rr = dist / fmaxf(lrange, 1e-4f);
-            const float rr2 = rr * rr;
-            const float win = fminf(fmaxf(1.f - rr2 * rr2, 0.f), 1.f);
-            const float ranged = falloff * (win * win);
-            const float pos_att = lrange > 0.f ? ranged : falloff;
-            const float dist_att = is_dir ? 1.f : pos_att;
-            const float acos_ = dot3(wi, ldir) * -1.f;
-            const float spot = fminf(fmaxf((acos_ - lca) / fmaxf(lcp - lca, 1e-4f), 0.f), 1.f);
-            const float att = dist_att * (is_spot ? spot : 1.f);
-            rad = rad + diffuse * ndl * lcol * (att * p.dbg_direct);
+            const LightSample ls = light_gate(L, h, p);
+            if (ls.pass) rad = rad + light_contrib(L, h, ls, p);
         }
         // multibounce from the previous filtered atlas (js/gi_probes.js:2181-2188)
-        float3 bounce = sample_atlas(c, prev_atlas, hit, ng) * kd * (1.0f / SBGI_PI);
+        const float3 bounce = sample_atlas(c, prev_atlas, h.hit, h.ng) * h.kd * (1.0f / SBGI_PI);
         const float bl = bounce.x * 0.2126f + bounce.y * 0.7152f + bounce.z * 0.0722f;
         const float roll = p.radiance_clamp / fmaxf(fmaxf(p.radiance_clamp, bl), 1e-6f);
         out = rad + bounce * (roll * p.dbg_bounce);
@@ -189,13 +317,31 @@ __global__ void __launch_bounds__(256) k_trace(DevScene s, DevCascade c, sbgi_ti
         out = f3(o[0], o[1], o[2]);
     }
     rays[gid] = make_float4(out.x, out.y, out.z, hit_t);
-    if (hit_ids) hit_ids[gid] = make_int2(best_tri, best_inst);
-    if (COUNT) {
-        atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
-        atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
-        atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
-        atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
-        atomicAdd(&counters->shadow_rays, (unsigned long long)shadow);
+    if (hit_ids) hit_ids[gid] = make_int2(tri, inst);
+}
+
+__global__ void k_clear_u64(unsigned long long *__restrict__ p, uint32_t n, uint32_t *__restrict__ counter) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+    if (i == 0 && counter) *counter = 0u;
+}
+
+// Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit,
+// 2 closest hit reported as blocked when before tmax. out[i] = {t, tri, inst, blocked}.
+__global__ void k_query(DevScene s, const float4 *__restrict__ in, float4 *__restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = in[(size_t)i * 2], b = in[(size_t)i * 2 + 1];
+    TravCounters<false> cnt;
+    if (b.w < 0.5f || b.w >= 1.5f) {
+        float t;
+        int tri, inst;
+        trace_closest<false, false>(s, xyz(a), xyz(b), t, tri, inst, cnt);
+        const float blocked = (b.w >= 1.5f && tri >= 0 && t < a.w) ? 1.f : 0.f;
+        out[i] = make_float4(t, __int_as_float(tri), __int_as_float(inst), blocked);
+    } else {
+        const bool blocked = trace_any<false, false>(s, xyz(a), xyz(b), a.w, cnt);
+        out[i] = make_float4(0.f, 0.f, 0.f, blocked ? 1.f : 0.f);
     }
 }
 
@@ -213,25 +359,34 @@ __global__ void k_raydirs(float4 *__restrict__ dirs, uint32_t rays, float jitter
     dirs[k] = make_float4(d.x, d.y, d.z, 0.f);
 }
 
-// Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit.
-// out[i] = {t, tri (bits), inst (bits), blocked}.
-__global__ void k_query(DevScene s, const float4 *__restrict__ in, float4 *__restrict__ out, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = in[(size_t)i * 2], b = in[(size_t)i * 2 + 1];
-    TravCounters<false> cnt;
-    if (b.w < 0.5f) {
-        float t;
-        int tri, inst;
-        trace_closest<false>(s, xyz(a), xyz(b), t, tri, inst, cnt);
-        out[i] = make_float4(t, __int_as_float(tri), __int_as_float(inst), 0.f);
-    } else {
-        const bool blocked = trace_any<false>(s, xyz(a), xyz(b), a.w, cnt);
-        out[i] = make_float4(0.f, 0.f, 0.f, blocked ? 1.f : 0.f);
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
     }
+    return cus;
+}
+
+// Persistent grid: as many blocks as can be resident at once (occupancy API), capped by
+// the work. Every block then strides over the work, so no block waits for a slot.
+template <typename K>
+uint32_t persistent_blocks(K kernel, size_t lds, uint32_t work_items) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)kBlock, lds) != hipSuccess || per_cu <= 0)
+        per_cu = 2;
+    const uint32_t cap = (uint32_t)device_cus() * (uint32_t)per_cu;
+    const uint32_t want = (work_items + kBlock - 1) / kBlock;
+    return want < cap ? (want ? want : 1u) : cap;
 }
 
 }  // namespace
+
+size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count) {
+    return ((size_t)tlas_count * 2 + (size_t)inst_count * 4) * sizeof(uint4);
+}
 
 hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, hipStream_t st) {
     if (!n) return hipSuccess;
@@ -245,14 +400,35 @@ hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t
 }
 
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
-                        const float *meta, const ushort4 *prev_atlas, float4 *rays, int2 *hit_ids, Counters *counters,
-                        hipStream_t st) {
+                        const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
+                        int2 *hit_ids, Counters *counters, hipStream_t st) {
     const uint32_t n = c.updated * c.rays;
     if (n == 0) return hipSuccess;
-    dim3 grid((n + 255) / 256), block(256);
-    if (counters)
-        hipLaunchKernelGGL(k_trace<true>, grid, block, 0, st, s, c, p, dirs, meta, prev_atlas, rays, hit_ids, counters);
-    else
-        hipLaunchKernelGGL(k_trace<false>, grid, block, 0, st, s, c, p, dirs, meta, prev_atlas, rays, hit_ids, counters);
+    hipLaunchKernelGGL(k_clear_u64, dim3((n + 255) / 256), dim3(256), 0, st, w.vis, n, w.queue.count);
+    const size_t lds = trace_lds_bytes(s.tlas_count, s.inst_count);
+    const bool use_lds = lds <= kTraceLdsBudget;
+    const dim3 block(kBlock);
+#define SBGI_PRIMARY(CNT, L)                                                                              \
+    hipLaunchKernelGGL((k_primary<CNT, L>), dim3(persistent_blocks(k_primary<CNT, L>, L ? lds : 0, n)), block,  \
+                       L ? lds : 0, st, s, c, p, dirs, meta, w.hits, w.queue, counters)
+    if (counters) {
+        if (use_lds) SBGI_PRIMARY(true, true); else SBGI_PRIMARY(true, false);
+    } else {
+        if (use_lds) SBGI_PRIMARY(false, true); else SBGI_PRIMARY(false, false);
+    }
+#undef SBGI_PRIMARY
+    // the queue length is only known on the device: size the persistent grid for the
+    // capacity and let every block read the live count
+#define SBGI_SHADOW(CNT, L)                                                                                \
+    hipLaunchKernelGGL((k_shadow<CNT, L>), dim3(persistent_blocks(k_shadow<CNT, L>, L ? lds : 0, w.queue.capacity)), \
+                       block, L ? lds : 0, st, s, w.queue, w.vis, counters)
+    if (counters) {
+        if (use_lds) SBGI_SHADOW(true, true); else SBGI_SHADOW(true, false);
+    } else {
+        if (use_lds) SBGI_SHADOW(false, true); else SBGI_SHADOW(false, false);
+    }
+#undef SBGI_SHADOW
+    hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), block, 0, st, s, c, p, dirs, meta, prev_atlas,
+                       w.hits, w.vis, rays, hit_ids);
     return hipGetLastError();
 }

@@ -5,11 +5,27 @@
 #include "sbgi_dev.h"
 
 // k_trace.hip
+// Dense queue of shadow rays: rec[2i] = {origin.xyz, tmax}, rec[2i+1] = {dir.xyz,
+// bits((ray << 6) | light)}; *count is the number of records appended this solve.
+struct ShadowQueue {
+    float4 *rec;
+    uint32_t *count;
+    uint32_t capacity;
+};
+// Per-solve trace scratch: closest-hit records {t, tri, inst, -} and per-ray light
+// visibility masks (bit l set when the shadow ray toward light l is unblocked).
+struct TraceScratch {
+    float4 *hits;
+    unsigned long long *vis;
+    ShadowQueue queue;
+};
+constexpr size_t kTraceLdsBudget = 64 * 1024;  // TLAS + instances staged in LDS up to this size
+size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count);
 hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, hipStream_t st);
 hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
-                        const float *meta, const ushort4 *prev_atlas, float4 *rays, int2 *hit_ids,
-                        Counters *counters, hipStream_t st);
+                        const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
+                        int2 *hit_ids, Counters *counters, hipStream_t st);
 
 // k_blend.hip
 struct BlendTables {

@@ -33,7 +33,7 @@ struct DevScene {
     const float *__restrict__ mats;
     const uint32_t *__restrict__ tri_mat;
     const float *__restrict__ lights;
-    uint32_t node_count, tlas_count, light_count, tri_count;
+    uint32_t node_count, tlas_count, light_count, tri_count, inst_count;
 };
 
 // Per-solve cascade geometry (per-cascade uniforms of js/gi_probes.js:3200-3211).
@@ -84,12 +84,15 @@ __device__ __forceinline__ bool side_ok(uint32_t flags, float det) {
 }
 
 // Moller-Trumbore with the reference's tests and epsilon (js/spectral_traverse.js:388-409).
+// Evaluated without FMA contraction and with a correctly rounded 1/det, the operation
+// order of the reference's graph, so edge decisions match the FP32 restatement.
 __device__ __forceinline__ bool mt_test(float4 A, float4 B, float4 C, float3 ro, float3 rd, float &t, float &det) {
+#pragma clang fp contract(off)
     float3 p0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
     float3 pv = cross3(rd, e2);
     det = dot3(e1, pv);
     if (!(fabsf(det) > 1e-12f)) return false;
-    float inv_det = __builtin_amdgcn_rcpf(det);
+    float inv_det = 1.0f / det;
     float3 tv = ro - p0;
     float u = dot3(tv, pv) * inv_det;
     if (!(u >= 0.f && u <= 1.f)) return false;

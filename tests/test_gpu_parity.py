@@ -61,8 +61,11 @@ def test_room_hit_ids_and_t(room_run):
     g, r = gpu[0], refs["64"][0]
     mism = np.any(g["hits"] != r["hits"], axis=-1)
     slots, ks = np.nonzero(mism)
-    ties = near_tie_mask(oracle, w.scene, w, slots, ks) if slots.size else np.zeros(0, bool)
+    ties = near_tie_mask(oracle, w.scene, w, slots, ks, g["rays"][slots, ks, 3]) if slots.size else np.zeros(0, bool)
     print(f"\nrays {mism.size}, id mismatches {int(mism.sum())}, of which near ties {int(ties.sum())}")
+    for s_, k_ in zip(slots[:10], ks[:10]):
+        print(f"  slot {s_} k {k_}: gpu {g['hits'][s_, k_]} t {g['rays'][s_, k_, 3]:.7f} | "
+              f"oracle {r['hits'][s_, k_]} t {r['rays'][s_, k_, 3]:.7f}")
     assert int((~ties).sum()) == 0, "hit ids differ on rays that are not near ties"
     hit = (r["hits"][..., 0] >= 0) & ~mism
     et = rel_err(g["rays"][..., 3][hit], r["rays"][..., 3][hit], floor_frac=0)
@@ -91,7 +94,7 @@ def test_room_irradiance_fp32(room_run):
         e64 = rel_err(g[..., :3], r64[..., :3])
         print(f"\nstep {step} irradiance rel err vs FP32 {summary(e32)} vs FP64 {summary(e64)}")
         assert np.percentile(e32, 99) <= 1e-4
-        assert np.percentile(e64, 99) <= 1e-3
+        assert np.percentile(e64, 99.9) <= 1e-3
 
 
 def test_room_depth_and_atlas(room_run):
@@ -116,7 +119,8 @@ def test_room_stats(room_run):
     assert s.updated == 4096 and s.rays == 4096 * 64
 
 
-@pytest.mark.parametrize("precision,bar", [(N.BLEND_BF16X3, 1e-4), (N.BLEND_FP16, 1e-3), (N.BLEND_BF16, 4e-3)])
+@pytest.mark.parametrize("precision,bar", [(N.BLEND_FP32, 1e-3), (N.BLEND_BF16X3, 1e-3), (N.BLEND_FP16, 1e-3),
+                                           (N.BLEND_BF16, 1e-2)])
 def test_room_mfma_blend(room, precision, bar):
     """MFMA blend against the FP64 oracle blend of the SAME GPU ray data (isolates the blend)."""
     w = room
@@ -132,8 +136,8 @@ def test_room_mfma_blend(room, precision, bar):
     e = rel_err(interior_irr(irr)[..., :3], interior_irr(o.irr_view())[..., :3])
     ed = rel_err(depth.reshape(-1, 64, 2), o.depth_view())
     print(f"\nprecision {precision}: irradiance rel err {summary(e)} depth {summary(ed)}")
-    assert np.percentile(e, 99) <= bar
-    assert np.percentile(ed, 99) <= bar
+    assert np.percentile(e, 99.9) <= bar and e.max() <= 3 * bar
+    assert np.percentile(ed, 99.9) <= bar and ed.max() <= 3 * bar
 
 
 def test_instanced_subset(instanced_small):
@@ -152,9 +156,9 @@ def test_instanced_subset(instanced_small):
     o.update(p)
     mism = np.any(g_hits != r_hits, axis=-1)
     slots, ks = np.nonzero(mism)
-    ties = near_tie_mask(oracle, w.scene, w, slots, ks) if slots.size else np.zeros(0, bool)
+    ties = near_tie_mask(oracle, w.scene, w, slots, ks, g_rays[slots, ks, 3]) if slots.size else np.zeros(0, bool)
     print(f"\ninstanced: rays {mism.size} id mismatches {int(mism.sum())} near ties {int(ties.sum())}")
-    assert int((~ties).sum()) <= max(1, mism.size // 20000)
+    assert int((~ties).sum()) == 0
     same = ~mism
     et = rel_err(g_rays[..., 3][same & (r_hits[..., 0] >= 0)], r_rays[..., 3][same & (r_hits[..., 0] >= 0)], 0)
     assert et.max() <= 1e-5

@@ -83,8 +83,13 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", default="bf16x3", choices=sorted(PRECISIONS))
+    ap.add_argument("--traversal", default="ordered", choices=["compat", "ordered"],
+                    help="compat: the reference's TLAS and left-first node order; ordered: SAH TLAS + "
+                         "near-first octant node copies (same hits except exact-t ties)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 subset check")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the probe field would shard "
@@ -100,9 +105,18 @@ def main():
     log(f"scene: {w.scene.meta}, {w.scene.node_count} BLAS nodes, {w.scene.tlas_node_count} TLAS nodes, "
         f"built in {time.time() - t0:.1f}s")
     prec = PRECISIONS[args.precision]
-    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=prec)
+    trav = N.TRAV_MT_ORDERED if args.traversal == "ordered" else N.TRAV_MT_COMPAT
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=prec, traversal=trav)
     f.set_scene(w.scene)
     kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # first update from cleared state: kept for the FP64 subset check
     f.tick(**kw)
@@ -164,7 +178,7 @@ def main():
                                "32x16x24 probes x 256 rays, irr 8x8, depth 16x16, 2 bounces",
                    "probes": w.probes, "rays_per_probe": w.rays, "triangles": w.scene.meta.get("instanced_triangles"),
                    "unique_triangles": w.scene.meta.get("unique_triangles"), "lights": 16,
-                   "blend_precision": args.precision},
+                   "blend_precision": args.precision, "traversal": args.traversal},
         "ms_per_update": ms_step,
         "stage_ms": {k: float(np.mean(v)) for k, v in stage.items()},
         "shadow_rays_per_update": int(cs.shadow_rays),

@@ -49,6 +49,10 @@ struct sbgi_field {
     uint4 *d_tlas = nullptr;
     int32_t *d_parent = nullptr, *d_tlas_parent = nullptr;
     uint32_t *d_arrive = nullptr, *d_tlas_arrive = nullptr;
+    // MT_ORDERED traversal layouts: 8 octant copies of the BLAS pool + an SAH TLAS
+    uint32_t *d_nodes_oct = nullptr, *d_oct_perm = nullptr;
+    uint4 *d_tlas_sah = nullptr;
+    uint32_t tlas_sah_count = 0;
     // scene (host mirror, reference layouts)
     std::vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
     std::vector<float> h_vdata, h_mats, h_lights, h_inst_bounds;
@@ -122,6 +126,10 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_arrive);
     dfree(f->d_tlas_arrive);
     dfree(f->tw.queue.rec);
+    dfree(f->d_nodes_oct);
+    dfree(f->d_oct_perm);
+    dfree(f->d_tlas_sah);
+    f->tlas_sah_count = 0;
     f->tw.queue.capacity = 0;
     f->has_scene = false;
 }
@@ -147,17 +155,20 @@ DevCascade make_cascade(const sbgi_field *f, const CascadeState &cs, uint32_t of
     return c;
 }
 
+bool ordered(const sbgi_field *f) { return f->desc.traversal == SBGI_TRAV_MT_ORDERED; }
+
 DevScene make_scene(const sbgi_field *f) {
     DevScene s{};
-    s.blas = reinterpret_cast<const uint4 *>(f->d_nodes);
+    s.blas = reinterpret_cast<const uint4 *>(ordered(f) ? f->d_nodes_oct : f->d_nodes);
+    s.blas_octants = ordered(f) ? 8u : 1u;
     s.tri = f->d_tri;
-    s.tlas = f->d_tlas;
+    s.tlas = ordered(f) ? f->d_tlas_sah : f->d_tlas;
     s.inst = f->d_inst;
     s.mats = f->d_mats;
     s.tri_mat = f->d_tri_mat;
     s.lights = f->d_lights;
     s.node_count = f->view.node_count;
-    s.tlas_count = f->view.tlas_node_count;
+    s.tlas_count = ordered(f) ? f->tlas_sah_count : f->view.tlas_node_count;
     s.light_count = f->view.light_count;
     s.tri_count = f->view.tri_count;
     s.inst_count = f->view.inst_count;
@@ -280,6 +291,65 @@ void invert_affine_rows(const float *rows12, double *world16) {
         world16[12 + r] = -(inv[r * 3 + 0] * m[12] + inv[r * 3 + 1] * m[13] + inv[r * 3 + 2] * m[14]);
 }
 
+// SAH TLAS over the current instance bounds (inflated by a relative 1e-5 so rounding of the
+// recovered world matrices can never make a box smaller than its instance).
+int32_t upload_sah_tlas(sbgi_field *f) {
+    const sbgi_scene_view &v = f->view;
+    std::vector<float> b(f->h_inst_bounds);
+    for (uint32_t i = 0; i < v.inst_count; i++) {
+        float *bb = b.data() + (size_t)i * 6;
+        for (int a = 0; a < 3; a++) {
+            const float pad = 1e-5f * (std::fabs(bb[3 + a] - bb[a]) + std::fabs(bb[a]) + std::fabs(bb[3 + a])) + 1e-6f;
+            bb[a] -= pad;
+            bb[3 + a] += pad;
+        }
+    }
+    std::vector<float> tl;
+    sbgi::build_sah_tlas(b.data(), v.inst_count, tl);
+    const uint32_t m = (uint32_t)(tl.size() / 12);
+    std::vector<uint4> packed((size_t)m * 2);
+    for (uint32_t i = 0; i < m; i++) {
+        const float *r = tl.data() + (size_t)i * 12;
+        uint32_t u[6];
+        memcpy(u, r, 24);
+        const uint32_t leaf = r[8] < 0.5f ? 0xFFFFFFFFu : (((uint32_t)r[8] << 24) | ((uint32_t)r[7] & 0x00FFFFFFu));
+        packed[(size_t)i * 2] = make_uint4(u[0], u[1], u[2], (uint32_t)r[6]);
+        packed[(size_t)i * 2 + 1] = make_uint4(u[3], u[4], u[5], leaf);
+    }
+    if (m > f->tlas_sah_count) {
+        dfree(f->d_tlas_sah);
+        if (dalloc(&f->d_tlas_sah, (size_t)m * 32) != hipSuccess) return fail(f, SBGI_E_NOMEM, "SAH TLAS allocation");
+    }
+    f->tlas_sah_count = m;
+    HIP_OK(f, hipMemcpyAsync(f->d_tlas_sah, packed.data(), (size_t)m * 32, hipMemcpyHostToDevice, f->stream));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+int32_t upload_octant_copies(sbgi_field *f) {
+    const sbgi_scene_view &v = f->view;
+    std::vector<std::pair<uint32_t, uint32_t>> roots;
+    for (uint32_t i = 0; i < v.inst_count; i++) {
+        const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
+        roots.emplace_back((uint32_t)r[12], (uint32_t)r[13]);
+    }
+    std::sort(roots.begin(), roots.end());
+    roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+    for (size_t i = 1; i < roots.size(); i++)
+        if (roots[i].first < roots[i - 1].second) return fail(f, SBGI_E_LAYOUT, "overlapping instance BLAS ranges");
+    std::vector<uint32_t> oc, perm;
+    if (!sbgi::build_octant_copies(f->h_nodes.data(), v.node_count, roots, oc, perm))
+        return fail(f, SBGI_E_LAYOUT, "BLAS range is not a threaded subtree");
+    dfree(f->d_nodes_oct);
+    dfree(f->d_oct_perm);
+    if (dalloc(&f->d_nodes_oct, oc.size() * 4) != hipSuccess || dalloc(&f->d_oct_perm, perm.size() * 4) != hipSuccess)
+        return fail(f, SBGI_E_NOMEM, "octant copy allocation");
+    HIP_OK(f, hipMemcpyAsync(f->d_nodes_oct, oc.data(), oc.size() * 4, hipMemcpyHostToDevice, f->stream));
+    HIP_OK(f, hipMemcpyAsync(f->d_oct_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, f->stream));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
 int32_t repack_all(sbgi_field *f) {
     const sbgi_scene_view &v = f->view;
     HIP_OK(f, launch_pack_tris(f->d_tri_index, f->d_vdata, f->d_tri_mat, f->d_mats, v.tri_count, f->d_tri, f->stream));
@@ -340,8 +410,8 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
     if (desc->depth_oct_res != 6 && desc->depth_oct_res != 14)
         return fail(nullptr, SBGI_E_INVALID, "depth_oct_res must be 6 or 14");
     if (desc->blend_precision > SBGI_BLEND_FP16) return fail(nullptr, SBGI_E_INVALID, "unknown blend precision");
-    if (desc->traversal != SBGI_TRAV_MT_COMPAT)
-        return fail(nullptr, SBGI_E_INVALID, "only the MT_COMPAT traversal is built in this version");
+    if (desc->traversal != SBGI_TRAV_MT_COMPAT && desc->traversal != SBGI_TRAV_MT_ORDERED)
+        return fail(nullptr, SBGI_E_INVALID, "traversal must be MT_COMPAT or MT_ORDERED");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || desc->device < 0 || desc->device >= ndev)
         return fail(nullptr, SBGI_E_DEVICE, "HIP device not available");
@@ -488,6 +558,12 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    if (ordered(f)) {
+        if ((rc = upload_octant_copies(f)) != SBGI_OK || (rc = upload_sah_tlas(f)) != SBGI_OK) {
+            free_scene(f);
+            return rc;
+        }
+    }
     f->has_scene = true;
     return SBGI_OK;
 }
@@ -529,6 +605,18 @@ int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges
     rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    if (ordered(f) && (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_MATERIALS)) {
+        if (buf_id == SBGI_BUF_BVH_NODES && (rc = upload_octant_copies(f)) != SBGI_OK) return rc;
+        if (buf_id == SBGI_BUF_MATERIALS) {
+            for (uint32_t i = 0; i < f->view.inst_count; i++) {
+                const float *r = f->h_mats.data() + f->view.inst_base + (size_t)i * 28;
+                invert_affine_rows(r, f->h_world.data() + (size_t)i * 16);
+                const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)r[12] * 8);
+                inst_aabb(root, f->h_world.data() + (size_t)i * 16, r[15] > 0.5f, f->h_inst_bounds.data() + (size_t)i * 6);
+            }
+            if ((rc = upload_sah_tlas(f)) != SBGI_OK) return rc;
+        }
+    }
     return SBGI_OK;
 }
 
@@ -558,6 +646,7 @@ int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges
                                     f->stream));
     }
     if (n_ranges) {
+        if (ordered(f)) HIP_OK(f, launch_oct_gather(f->d_nodes_oct, f->d_nodes, f->d_oct_perm, 8 * v.node_count, f->stream));
         HIP_OK(f, hipMemcpyAsync(f->h_nodes.data(), f->d_nodes, f->h_nodes.size() * 4, hipMemcpyDeviceToHost,
                                  f->stream));
         HIP_OK(f, hipStreamSynchronize(f->stream));
@@ -579,6 +668,7 @@ int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges
     int32_t rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    if (ordered(f) && refit_tlas && (rc = upload_sah_tlas(f)) != SBGI_OK) return rc;
     return SBGI_OK;
 }
 
@@ -618,6 +708,7 @@ int32_t sbgi_set_instance_transforms(sbgi_field *f, const uint32_t *slots, uint3
     HIP_OK(f, launch_pack_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_tlas, f->stream));
     HIP_OK(f, launch_pack_inst(f->d_mats, v.inst_base, v.inst_count, f->d_inst, f->stream));
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    if (ordered(f)) return upload_sah_tlas(f);
     return SBGI_OK;
 }
 

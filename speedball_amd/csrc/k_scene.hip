@@ -160,7 +160,23 @@ __global__ void k_refit_tlas(float *__restrict__ mats, uint32_t tlas_base, uint3
     }
 }
 
+__global__ void k_oct_gather(uint32_t *__restrict__ oct, const uint32_t *__restrict__ nodes,
+                             const uint32_t *__restrict__ perm, uint32_t n_total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_total) return;
+    const uint32_t src = perm[i];
+#pragma unroll
+    for (int w = 0; w < 6; w++) oct[(size_t)i * 8 + w] = nodes[(size_t)src * 8 + w];
+}
+
 }  // namespace
+
+hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_total,
+                             hipStream_t st) {
+    if (!n_total) return hipSuccess;
+    hipLaunchKernelGGL(k_oct_gather, dim3((n_total + 255) / 256), dim3(256), 0, st, oct_nodes, nodes, perm, n_total);
+    return hipGetLastError();
+}
 
 hipError_t launch_pack_tris(const uint32_t *tri_index, const float *vdata, const uint32_t *tri_mat, const float *mats,
                             uint32_t tri_count, float4 *tri, hipStream_t st) {

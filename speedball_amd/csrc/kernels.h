@@ -57,5 +57,8 @@ hipError_t launch_pack_inst(const float *mats, uint32_t inst_base, uint32_t inst
 // BLAS refit of the threaded range [root, end): bottom-up with per-node arrival counters.
 hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
                              uint32_t *arrive, uint32_t root, uint32_t end, hipStream_t st);
+// octant copies: bounds of copy node i <- canonical node perm[i] (after a refit)
+hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_total,
+                             hipStream_t st);
 hipError_t launch_refit_tlas(float *mats, uint32_t tlas_base, uint32_t tlas_count, const float *inst_bounds,
                              const int32_t *parent, uint32_t *arrive, hipStream_t st);

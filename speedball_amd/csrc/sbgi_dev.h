@@ -34,6 +34,7 @@ struct DevScene {
     const uint32_t *__restrict__ tri_mat;
     const float *__restrict__ lights;
     uint32_t node_count, tlas_count, light_count, tri_count, inst_count;
+    uint32_t blas_octants;  // 1: reference node order only; 8: + near-first copy per ray octant
 };
 
 // Per-solve cascade geometry (per-cascade uniforms of js/gi_probes.js:3200-3211).

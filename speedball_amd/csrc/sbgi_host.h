@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sbgi {
@@ -30,6 +31,11 @@ struct TlasRec {
 void invert4(const double *e, double *o);
 double det4(const double *e);
 void box_apply(const float *mn, const float *mx, const double *e, double *omn, double *omx);
+
+// traversal-only layouts (accel.cpp)
+void build_sah_tlas(const float *inst_bounds, uint32_t n, std::vector<float> &out);
+bool build_octant_copies(const uint32_t *nodes, uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>> &roots,
+                         std::vector<uint32_t> &out, std::vector<uint32_t> &perm);
 
 bool build_blas(const float *pos, uint32_t vcount, const uint32_t *idx, uint32_t tcount,
                 std::vector<FlatNode> &out, std::vector<uint32_t> &perm, std::string &err);

@@ -80,6 +80,13 @@ __device__ __forceinline__ float3 inst_dir(float4 r0, float4 r1, float4 r2, floa
 
 __device__ __forceinline__ float3 safe_inv(float3 d) { return f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
 
+// Which node copy a local ray walks: with 8 octant copies, copy o has every interior
+// node's children in near-first order for directions whose sign bits are o.
+__device__ __forceinline__ uint32_t octant_of(float3 rd, uint32_t octants) {
+    if (octants < 8) return 0u;
+    return (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
+}
+
 __device__ __forceinline__ uint64_t walk_budget(const DevScene &s) {
     return (uint64_t)s.tlas_count * 2 + (uint64_t)s.node_count * 256 + 1024;
 }
@@ -93,12 +100,13 @@ __device__ __forceinline__ void trace_closest(const DevScene &s, float3 wro, flo
     best_tri = -1;
     best_inst = -1;
     uint32_t tcur = 0, bcur = 0, bend = 0, inext = 0, iend = 0, cur_inst = 0;
+    const uint4 *blas = s.blas;
     float3 ro = wro, rd = wrd, invd = winvd;
     float dsign = 1.f;
     const uint64_t budget = walk_budget(s);
     for (uint64_t it = 0; it < budget; ++it) {
         if (bcur < bend) {
-            const uint4 a = s.blas[(size_t)bcur * 2], b = s.blas[(size_t)bcur * 2 + 1];
+            const uint4 a = blas[(size_t)bcur * 2], b = blas[(size_t)bcur * 2 + 1];
             const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
             const float3 bmax = f3(__uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y));
             cnt.add_blas();
@@ -142,6 +150,7 @@ __device__ __forceinline__ void trace_closest(const DevScene &s, float3 wro, flo
                 bcur = (uint32_t)r3.x;
                 bend = (uint32_t)r3.y;
                 cur_inst = inst;
+                blas = s.blas + (size_t)octant_of(rd, s.blas_octants) * s.node_count * 2;
             }
             continue;
         }
@@ -173,13 +182,14 @@ __device__ __forceinline__ bool trace_any(const DevScene &s, float3 wro, float3 
                                           TravCounters<COUNT> &cnt) {
     const float3 winvd = safe_inv(wrd);
     uint32_t tcur = 0, bcur = 0, bend = 0, inext = 0, iend = 0;
+    const uint4 *blas = s.blas;
     float3 ro = wro, rd = wrd, invd = winvd;
     float dsign = 1.f;
     bool blocked = false;
     const uint64_t budget = walk_budget(s);
     for (uint64_t it = 0; it < budget && !blocked; ++it) {
         if (bcur < bend) {
-            const uint4 a = s.blas[(size_t)bcur * 2], b = s.blas[(size_t)bcur * 2 + 1];
+            const uint4 a = blas[(size_t)bcur * 2], b = blas[(size_t)bcur * 2 + 1];
             const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
             const float3 bmax = f3(__uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y));
             cnt.add_blas();
@@ -218,6 +228,7 @@ __device__ __forceinline__ bool trace_any(const DevScene &s, float3 wro, float3 
                 dsign = r3.z;
                 bcur = (uint32_t)r3.x;
                 bend = (uint32_t)r3.y;
+                blas = s.blas + (size_t)octant_of(rd, s.blas_octants) * s.node_count * 2;
             }
             continue;
         }

@@ -30,7 +30,8 @@ def clamp_rays(n) -> int:
 
 class ProbeField:
     def __init__(self, res, grid_min, grid_size, rays: int = 64, depth_oct: int = 6,
-                 precision: int = N.BLEND_FP32, keep_hit_ids: bool = False, device: int = 0, cascades=None):
+                 precision: int = N.BLEND_FP32, keep_hit_ids: bool = False, device: int = 0, cascades=None,
+                 traversal: int = N.TRAV_MT_COMPAT):
         self.L = N.lib()
         d = N.FieldDesc()
         d.device = device
@@ -39,7 +40,7 @@ class ProbeField:
         d.cascade_count = len(casc)
         d.depth_oct_res = depth_oct
         d.blend_precision = precision
-        d.traversal = N.TRAV_MT_COMPAT
+        d.traversal = traversal
         d.flags = N.FIELD_KEEP_HIT_IDS if keep_hit_ids else 0
         for i, (gm, gs, rs) in enumerate(casc):
             for a in range(3):

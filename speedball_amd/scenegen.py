@@ -244,11 +244,18 @@ def instanced(res=(32, 16, 24), rays: int = 256, total_tris: int = 262144, n_ins
     assign = [int(rng.uniform(0, n_mesh)) % n_mesh for _ in range(n_instances)]
     counts = np.bincount(assign, minlength=n_mesh)
     scale = total_tris / float(np.dot(counts, target))
+    # quantized mesh resolutions: rescale until the instanced total is within +-1% of the target
+    geo = None
+    for _ in range(8):
+        geo = [mesh_for(m % 3, target[m] * scale, seed=m + 7) for m in range(n_mesh)]
+        actual = int(np.dot(counts, [len(t) for _, t in geo]))
+        if abs(actual - total_tris) <= 0.01 * total_tris:
+            break
+        scale *= total_tris / float(actual)
     b = SceneBuilder()
     meshes, mesh_tris = [], []
     for m in range(n_mesh):
-        want = target[m] * scale
-        v, t = mesh_for(m % 3, want, seed=m + 7)
+        v, t = geo[m]
         alb = (rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8))
         mat = b.add_material(uber_material(alb, side=2 if m % 3 == 2 else 0))
         meshes.append(b.add_mesh(v, t, tri_material=np.full(len(t), mat, np.uint32)))

@@ -49,3 +49,44 @@ def probe_pos(w, p):
     ix, iy, iz = p % rx, (p // rx) % ry, p // (rx * ry)
     f = np.array([ix / max(1, rx - 1), iy / max(1, ry - 1), iz / max(1, rz - 1)], np.float64)
     return np.asarray(w.grid_min, np.float64) + f * np.asarray(w.grid_size, np.float64)
+
+
+def edge_distance(scene, ro, rd, tri, inst):
+    """FP64 barycentric distance of the ray to the nearest edge of triangle `tri` of
+    instance slot `inst`: min(u, v, 1 - u - v) of the ray/plane intersection in the
+    instance's local frame (negative when the ray passes outside)."""
+    if tri < 0:
+        return np.inf
+    rows = scene.materials[scene.inst_base + inst * 28: scene.inst_base + inst * 28 + 12].astype(np.float64)
+    A = rows.reshape(3, 4)
+    lro = A[:, :3] @ np.asarray(ro, np.float64) + A[:, 3]
+    lrd = A[:, :3] @ np.asarray(rd, np.float64)
+    v = scene.vertex_data.reshape(-1, 8)[:, :3].astype(np.float64)
+    i0, i1, i2 = scene.tri_index.reshape(-1, 3)[tri]
+    p0, e1, e2 = v[i0], v[i1] - v[i0], v[i2] - v[i0]
+    pv = np.cross(lrd, e2)
+    det = e1 @ pv
+    if abs(det) < 1e-300:
+        return 0.0
+    tv = lro - p0
+    u = (tv @ pv) / det
+    qv = np.cross(tv, e1)
+    w = (lrd @ qv) / det
+    return min(u, w, 1.0 - u - w)
+
+
+def classify_mismatches(oracle_mod, scene, field_w, slots, ks, gpu_hits, ref_hits, t_gpu, tol=1e-6, edge=1e-5):
+    """Split hit-id mismatches into near ties, edge grazes (the ray passes within `edge`
+    barycentric units of an edge of either triangle: Moller-Trumbore's edge decisions there
+    depend on the last bit of the arithmetic) and real disagreements."""
+    ties = near_tie_mask(oracle_mod, scene, field_w, slots, ks, t_gpu, tol)
+    grazes = np.zeros(len(slots), bool)
+    for n, (probe, k) in enumerate(zip(slots, ks)):
+        if ties[n]:
+            continue
+        ro = probe_pos(field_w, probe)
+        rd = oracle_mod.ray_dir(int(k), field_w.rays, field_w.frame_jitter)
+        dg = edge_distance(scene, ro, rd, gpu_hits[n][0], gpu_hits[n][1])
+        dr = edge_distance(scene, ro, rd, ref_hits[n][0], ref_hits[n][1])
+        grazes[n] = abs(dg) < edge or abs(dr) < edge
+    return ties, grazes

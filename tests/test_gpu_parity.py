@@ -13,14 +13,14 @@ import pytest
 import oracle
 from speedball_amd import _native as N
 from speedball_amd.field import ProbeField
-from tests.parity import rel_err, summary, interior_irr, near_tie_mask
+from tests.parity import rel_err, summary, interior_irr, near_tie_mask, classify_mismatches
 
 pytestmark = pytest.mark.gpu
 
 
-def make_field(w, precision=N.BLEND_FP32, keep=True, res=None):
+def make_field(w, precision=N.BLEND_FP32, keep=True, res=None, traversal=N.TRAV_MT_COMPAT):
     f = ProbeField(res or w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=precision,
-                   keep_hit_ids=keep)
+                   keep_hit_ids=keep, traversal=traversal)
     f.set_scene(w.scene)
     return f
 
@@ -29,12 +29,12 @@ def params(w, **kw):
     return dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis, **kw)
 
 
-@pytest.fixture(scope="module")
-def room_run(room):
+@pytest.fixture(scope="module", params=[N.TRAV_MT_COMPAT, N.TRAV_MT_ORDERED], ids=["compat", "ordered"])
+def room_run(room, request):
     """Two chained updates of the full 16^3 x 64 field on the GPU (FP32 blend) and in the
-    FP32 restatement and FP64 oracle."""
+    FP32 restatement and FP64 oracle, for both traversal modes."""
     w = room
-    f = make_field(w)
+    f = make_field(w, traversal=request.param)
     gpu = []
     for _ in range(2):
         f.tick(**params(w))
@@ -57,16 +57,31 @@ def room_run(room):
 
 
 def test_room_hit_ids_and_t(room_run):
+    """Hit ids: identical except near ties (two nearest hits within 1e-6 relative) and edge
+    grazes, both counted and reported.
+
+    Edge grazes are rays within `edge` barycentric units of an edge of either triangle,
+    where Moller-Trumbore's accept decision turns on the last bits. Against the FP32
+    restatement (the same f32 ray directions) the band is 1e-5. Against the FP64 oracle it
+    is 5e-4: the reference evaluates phi = k * GA in f32 (js/gi_probes.js:1702), and at
+    phi ~ 150 rad one f32 ulp is ~8e-6 rad, so the f32 ray itself differs from the exact
+    one by that angle, i.e. up to ~3e-4 of a box face at the field's hit distances."""
     w, gpu, refs, _ = room_run
+    for prec, edge in (("32", 1e-5), ("64", 5e-4)):
+        g, r = gpu[0], refs[prec][0]
+        mism = np.any(g["hits"] != r["hits"], axis=-1)
+        slots, ks = np.nonzero(mism)
+        ties, grazes = classify_mismatches(oracle, w.scene, w, slots, ks, g["hits"][slots, ks], r["hits"][slots, ks],
+                                           g["rays"][slots, ks, 3], edge=edge)
+        print(f"\nvs FP{prec}: rays {mism.size}, id mismatches {int(mism.sum())}: near ties {int(ties.sum())}, "
+              f"edge grazes {int(grazes.sum())}")
+        for s_, k_ in zip(slots[:6], ks[:6]):
+            print(f"  slot {s_} k {k_}: gpu {g['hits'][s_, k_]} t {g['rays'][s_, k_, 3]:.7f} | "
+                  f"ref {r['hits'][s_, k_]} t {r['rays'][s_, k_, 3]:.7f}")
+        assert int((~(ties | grazes)).sum()) == 0, "hit ids differ on rays that are neither near ties nor edge grazes"
+        assert mism.sum() <= max(8, mism.size // 20000)
     g, r = gpu[0], refs["64"][0]
     mism = np.any(g["hits"] != r["hits"], axis=-1)
-    slots, ks = np.nonzero(mism)
-    ties = near_tie_mask(oracle, w.scene, w, slots, ks, g["rays"][slots, ks, 3]) if slots.size else np.zeros(0, bool)
-    print(f"\nrays {mism.size}, id mismatches {int(mism.sum())}, of which near ties {int(ties.sum())}")
-    for s_, k_ in zip(slots[:10], ks[:10]):
-        print(f"  slot {s_} k {k_}: gpu {g['hits'][s_, k_]} t {g['rays'][s_, k_, 3]:.7f} | "
-              f"oracle {r['hits'][s_, k_]} t {r['rays'][s_, k_, 3]:.7f}")
-    assert int((~ties).sum()) == 0, "hit ids differ on rays that are not near ties"
     hit = (r["hits"][..., 0] >= 0) & ~mism
     et = rel_err(g["rays"][..., 3][hit], r["rays"][..., 3][hit], floor_frac=0)
     print("hit t rel err", summary(et))
@@ -140,10 +155,11 @@ def test_room_mfma_blend(room, precision, bar):
     assert np.percentile(ed, 99.9) <= bar and ed.max() <= 3 * bar
 
 
-def test_instanced_subset(instanced_small):
+@pytest.mark.parametrize("traversal", [N.TRAV_MT_COMPAT, N.TRAV_MT_ORDERED], ids=["compat", "ordered"])
+def test_instanced_subset(instanced_small, traversal):
     """Two-level instanced scene (reduced size): hit ids, t and radiance."""
     w = instanced_small
-    f = make_field(w, res=w.res)
+    f = make_field(w, res=w.res, traversal=traversal)
     f.tick(**params(w))
     g_rays, g_hits = f.read_rays(), f.read_hit_ids()
     f.tick(**params(w))
@@ -156,9 +172,11 @@ def test_instanced_subset(instanced_small):
     o.update(p)
     mism = np.any(g_hits != r_hits, axis=-1)
     slots, ks = np.nonzero(mism)
-    ties = near_tie_mask(oracle, w.scene, w, slots, ks, g_rays[slots, ks, 3]) if slots.size else np.zeros(0, bool)
-    print(f"\ninstanced: rays {mism.size} id mismatches {int(mism.sum())} near ties {int(ties.sum())}")
-    assert int((~ties).sum()) == 0
+    ties, grazes = classify_mismatches(oracle, w.scene, w, slots, ks, g_hits[slots, ks], r_hits[slots, ks],
+                                       g_rays[slots, ks, 3], edge=1e-5)
+    print(f"\ninstanced: rays {mism.size} id mismatches {int(mism.sum())} near ties {int(ties.sum())} "
+          f"edge grazes {int(grazes.sum())}")
+    assert int((~(ties | grazes)).sum()) == 0
     same = ~mism
     et = rel_err(g_rays[..., 3][same & (r_hits[..., 0] >= 0)], r_rays[..., 3][same & (r_hits[..., 0] >= 0)], 0)
     assert et.max() <= 1e-5

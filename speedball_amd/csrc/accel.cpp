@@ -7,11 +7,14 @@
 //     12-float node format. The reference TLAS is a median split on the longest axis
 //     (js/spectral_scene.js:973-1003); with large overlapping instances the SAH tree
 //     enters ~2.5x fewer instances per ray.
-//   * build_octant_copies: eight copies of the threaded BLAS pool; in copy o every
-//     interior node's children are emitted near-first for ray directions whose sign bits
-//     are o (near = smaller projection of the child box centre on the octant direction).
-//     Each BLAS keeps its [root, end) range in every copy, so an instance record works
-//     unchanged; the walk stays stackless and picks the copy from the local ray direction.
+//   * build_node_copies: the threaded BLAS pool in the walk's node format
+//     {min.xyz, miss}, {max.xyz, payload}, as one copy in the reference's left-first order
+//     or as eight copies where, in copy o, every interior node's children are emitted
+//     near-first for ray directions whose sign bits are o (near = smaller projection of
+//     the child box centre on the octant direction). Each BLAS keeps its [root, end)
+//     range in every copy, so an instance record works unchanged; the walk stays
+//     stackless and picks the copy from the local ray direction. perm[i] is the pool node
+//     whose bounds copy node i carries (refits re-gather the bounds through it).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -142,13 +145,29 @@ void build_sah_tlas(const float *inst_bounds, uint32_t n, std::vector<float> &ou
     emit_sah(inst_bounds, ids, 0, n, out);
 }
 
-bool build_octant_copies(const uint32_t *nodes, uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>> &roots,
-                         std::vector<uint32_t> &out, std::vector<uint32_t> &perm) {
-    out.assign((size_t)8 * n * 8, 0u);
-    perm.assign((size_t)8 * n, 0u);
+bool build_node_copies(const uint32_t *nodes, uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>> &roots,
+                       uint32_t copies, std::vector<uint32_t> &out, std::vector<uint32_t> &perm) {
+    if (copies != 1 && copies != 8) return false;
+    out.assign((size_t)copies * n * 8, 0u);
+    perm.assign((size_t)copies * n, 0u);
+    if (copies == 1) {  // the pool itself, node for node
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t *src = nodes + (size_t)i * 8;
+            uint32_t *dst = out.data() + (size_t)i * 8;
+            dst[0] = src[0];
+            dst[1] = src[1];
+            dst[2] = src[2];
+            dst[3] = src[6];
+            dst[4] = src[3];
+            dst[5] = src[4];
+            dst[6] = src[5];
+            dst[7] = src[7];
+            perm[i] = i;
+        }
+        return true;
+    }
     const float *nf = reinterpret_cast<const float *>(nodes);
-    std::vector<uint32_t> stack;
-    for (uint32_t o = 0; o < 8; o++) {
+    for (uint32_t o = 0; o < copies; o++) {
         const float d[3] = {(o & 1) ? -1.f : 1.f, (o & 2) ? -1.f : 1.f, (o & 4) ? -1.f : 1.f};
         uint32_t *oc = out.data() + (size_t)o * n * 8;
         uint32_t *pm = perm.data() + (size_t)o * n;
@@ -165,16 +184,24 @@ bool build_octant_copies(const uint32_t *nodes, uint32_t n, const std::vector<st
                 Item it = work.back();
                 work.pop_back();
                 if (it.finish_of >= 0) {
-                    oc[(size_t)it.finish_of * 8 + 6] = pos;
+                    oc[(size_t)it.finish_of * 8 + 3] = pos;
                     continue;
                 }
                 const uint32_t i = it.src;
                 if (i >= n || pos >= re.second) return false;
                 const uint32_t me = pos++;
-                memcpy(oc + (size_t)me * 8, nodes + (size_t)i * 8, 32);
+                const uint32_t *src = nodes + (size_t)i * 8;
+                uint32_t *dst = oc + (size_t)me * 8;
+                dst[0] = src[0];
+                dst[1] = src[1];
+                dst[2] = src[2];
+                dst[4] = src[3];
+                dst[5] = src[4];
+                dst[6] = src[5];
+                dst[7] = src[7];
                 pm[me] = i;
-                if (nodes[(size_t)i * 8 + 7] != 0xFFFFFFFFu) {
-                    oc[(size_t)me * 8 + 6] = pos;
+                if (src[7] != 0xFFFFFFFFu) {
+                    dst[3] = pos;
                     continue;
                 }
                 const uint32_t l = i + 1;

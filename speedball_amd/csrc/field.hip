@@ -50,7 +50,7 @@ struct sbgi_field {
     int32_t *d_parent = nullptr, *d_tlas_parent = nullptr;
     uint32_t *d_arrive = nullptr, *d_tlas_arrive = nullptr;
     // MT_ORDERED traversal layouts: 8 octant copies of the BLAS pool + an SAH TLAS
-    uint32_t *d_nodes_oct = nullptr, *d_oct_perm = nullptr;
+    uint32_t *d_walk_nodes = nullptr, *d_walk_perm = nullptr;
     uint4 *d_tlas_sah = nullptr;
     uint32_t tlas_sah_count = 0;
     // scene (host mirror, reference layouts)
@@ -126,8 +126,8 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_arrive);
     dfree(f->d_tlas_arrive);
     dfree(f->tw.queue.rec);
-    dfree(f->d_nodes_oct);
-    dfree(f->d_oct_perm);
+    dfree(f->d_walk_nodes);
+    dfree(f->d_walk_perm);
     dfree(f->d_tlas_sah);
     f->tlas_sah_count = 0;
     f->tw.queue.capacity = 0;
@@ -159,7 +159,8 @@ bool ordered(const sbgi_field *f) { return f->desc.traversal == SBGI_TRAV_MT_ORD
 
 DevScene make_scene(const sbgi_field *f) {
     DevScene s{};
-    s.blas = reinterpret_cast<const uint4 *>(ordered(f) ? f->d_nodes_oct : f->d_nodes);
+    s.blas = reinterpret_cast<const uint4 *>(f->d_nodes);
+    s.nodes = reinterpret_cast<const uint4 *>(f->d_walk_nodes);
     s.blas_octants = ordered(f) ? 8u : 1u;
     s.tri = f->d_tri;
     s.tlas = ordered(f) ? f->d_tlas_sah : f->d_tlas;
@@ -326,7 +327,7 @@ int32_t upload_sah_tlas(sbgi_field *f) {
     return SBGI_OK;
 }
 
-int32_t upload_octant_copies(sbgi_field *f) {
+int32_t upload_walk_nodes(sbgi_field *f) {
     const sbgi_scene_view &v = f->view;
     std::vector<std::pair<uint32_t, uint32_t>> roots;
     for (uint32_t i = 0; i < v.inst_count; i++) {
@@ -335,17 +336,17 @@ int32_t upload_octant_copies(sbgi_field *f) {
     }
     std::sort(roots.begin(), roots.end());
     roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
-    for (size_t i = 1; i < roots.size(); i++)
+    for (size_t i = 1; ordered(f) && i < roots.size(); i++)
         if (roots[i].first < roots[i - 1].second) return fail(f, SBGI_E_LAYOUT, "overlapping instance BLAS ranges");
     std::vector<uint32_t> oc, perm;
-    if (!sbgi::build_octant_copies(f->h_nodes.data(), v.node_count, roots, oc, perm))
+    if (!sbgi::build_node_copies(f->h_nodes.data(), v.node_count, roots, ordered(f) ? 8u : 1u, oc, perm))
         return fail(f, SBGI_E_LAYOUT, "BLAS range is not a threaded subtree");
-    dfree(f->d_nodes_oct);
-    dfree(f->d_oct_perm);
-    if (dalloc(&f->d_nodes_oct, oc.size() * 4) != hipSuccess || dalloc(&f->d_oct_perm, perm.size() * 4) != hipSuccess)
-        return fail(f, SBGI_E_NOMEM, "octant copy allocation");
-    HIP_OK(f, hipMemcpyAsync(f->d_nodes_oct, oc.data(), oc.size() * 4, hipMemcpyHostToDevice, f->stream));
-    HIP_OK(f, hipMemcpyAsync(f->d_oct_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, f->stream));
+    dfree(f->d_walk_nodes);
+    dfree(f->d_walk_perm);
+    if (dalloc(&f->d_walk_nodes, oc.size() * 4) != hipSuccess || dalloc(&f->d_walk_perm, perm.size() * 4) != hipSuccess)
+        return fail(f, SBGI_E_NOMEM, "walk node allocation");
+    HIP_OK(f, hipMemcpyAsync(f->d_walk_nodes, oc.data(), oc.size() * 4, hipMemcpyHostToDevice, f->stream));
+    HIP_OK(f, hipMemcpyAsync(f->d_walk_perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice, f->stream));
     HIP_OK(f, hipStreamSynchronize(f->stream));
     return SBGI_OK;
 }
@@ -471,7 +472,7 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
     if (dalloc(&f->tw.hits, (size_t)f->max_total * R * 16) != hipSuccess ||
         dalloc(&f->tw.vis, (size_t)f->max_total * R * 8) != hipSuccess ||
-        dalloc(&f->tw.queue.count, 64) != hipSuccess)
+        dalloc(&f->tw.queue.count, kTraceCounterBytes) != hipSuccess)
         return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
     if ((uint64_t)f->max_total * R >= (1ull << 26))
         return bail(fail(f, SBGI_E_CAPACITY, "probes x rays per solve must stay below 2^26"));
@@ -558,11 +559,9 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
-    if (ordered(f)) {
-        if ((rc = upload_octant_copies(f)) != SBGI_OK || (rc = upload_sah_tlas(f)) != SBGI_OK) {
-            free_scene(f);
-            return rc;
-        }
+    if ((rc = upload_walk_nodes(f)) != SBGI_OK || (ordered(f) && (rc = upload_sah_tlas(f)) != SBGI_OK)) {
+        free_scene(f);
+        return rc;
     }
     f->has_scene = true;
     return SBGI_OK;
@@ -605,9 +604,10 @@ int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges
     rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
-    if (ordered(f) && (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_MATERIALS)) {
-        if (buf_id == SBGI_BUF_BVH_NODES && (rc = upload_octant_copies(f)) != SBGI_OK) return rc;
-        if (buf_id == SBGI_BUF_MATERIALS) {
+    if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_MATERIALS) {
+        // walk-format node copies depend on the pool and, in ordered mode, on the root set
+        if ((rc = upload_walk_nodes(f)) != SBGI_OK) return rc;
+        if (ordered(f) && buf_id == SBGI_BUF_MATERIALS) {
             for (uint32_t i = 0; i < f->view.inst_count; i++) {
                 const float *r = f->h_mats.data() + f->view.inst_base + (size_t)i * 28;
                 invert_affine_rows(r, f->h_world.data() + (size_t)i * 16);
@@ -646,7 +646,8 @@ int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges
                                     f->stream));
     }
     if (n_ranges) {
-        if (ordered(f)) HIP_OK(f, launch_oct_gather(f->d_nodes_oct, f->d_nodes, f->d_oct_perm, 8 * v.node_count, f->stream));
+        HIP_OK(f, launch_oct_gather(f->d_walk_nodes, f->d_nodes, f->d_walk_perm, (ordered(f) ? 8u : 1u) * v.node_count,
+                                    f->stream));
         HIP_OK(f, hipMemcpyAsync(f->h_nodes.data(), f->d_nodes, f->h_nodes.size() * 4, hipMemcpyDeviceToHost,
                                  f->stream));
         HIP_OK(f, hipStreamSynchronize(f->stream));
@@ -935,7 +936,7 @@ int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *ou
         return fail(f, SBGI_E_NOMEM, "query allocation failed");
     }
     hipError_t e = hipMemcpyAsync(din, rays8, (size_t)n * 32, hipMemcpyHostToDevice, f->stream);
-    if (e == hipSuccess) e = launch_query(make_scene(f), din, dout, n, f->stream);
+    if (e == hipSuccess) e = launch_query(make_scene(f), din, dout, n, f->tw.queue.count + trace_heads(2), f->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out4, dout, (size_t)n * 16, hipMemcpyDeviceToHost, f->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
     dfree(din);

@@ -160,13 +160,20 @@ __global__ void k_refit_tlas(float *__restrict__ mats, uint32_t tlas_base, uint3
     }
 }
 
+// Walk-format node copies {min.xyz, miss}, {max.xyz, payload}: re-gather the bounds of
+// copy node i from pool node perm[i] after a refit (structure words are unchanged).
 __global__ void k_oct_gather(uint32_t *__restrict__ oct, const uint32_t *__restrict__ nodes,
                              const uint32_t *__restrict__ perm, uint32_t n_total) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_total) return;
-    const uint32_t src = perm[i];
-#pragma unroll
-    for (int w = 0; w < 6; w++) oct[(size_t)i * 8 + w] = nodes[(size_t)src * 8 + w];
+    const uint32_t *src = nodes + (size_t)perm[i] * 8;
+    uint32_t *dst = oct + (size_t)i * 8;
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+    dst[4] = src[3];
+    dst[5] = src[4];
+    dst[6] = src[5];
 }
 
 }  // namespace

@@ -3,27 +3,35 @@
 // Restates traceKernel (js/gi_probes.js:1977-2214) for the configuration the reference
 // ships by default: non-clustered NEE over all lights, no texture maps, reflections off.
 //
-// MI355X layout of the work, three launches per solve instead of one monolithic kernel:
-//   1. k_primary  persistent blocks; thread -> (ray k, probe slot) with the slot
-//                 fastest-varying, so a wave traces ONE direction k (rayDir ignores the
-//                 probe, js/gi_probes.js:1692-1704) from 64 neighbouring probes. Closest hit
-//                 with the TLAS and instance table staged in LDS. For every hit and every
-//                 light that passes the reference's gate (n.l > 0, in range, not band-gated)
-//                 a shadow ray is appended to a queue; appends are aggregated per wave with
-//                 a ballot + one atomic, so the queue is dense and each wave-chunk holds one
-//                 light (coherent shadow rays from neighbouring probes).
-//   2. k_shadow   persistent any-hit over the dense queue: every lane has a shadow ray, no
-//                 lane idles on a primary miss. Unblocked rays set bit `light` of the ray's
-//                 visibility mask.
-//   3. k_resolve  per ray: emissive + sum over lights, in light order, of the visible NEE
+// MI355X layout of the work, four launches per solve instead of one monolithic kernel:
+//   1. k_walk<CLOSEST>  persistent waves pull probe rays from a fetch counter (index
+//                 k * updated + slot, slot fastest-varying, so neighbouring lanes start on
+//                 one direction k from neighbouring probes — rayDir ignores the probe,
+//                 js/gi_probes.js:1692-1704) and walk them with the wave-scheduled
+//                 two-level walk (traverse2.h): finished lanes are refilled, the expensive
+//                 triangle and instance phases run only with enough lanes waiting, and the
+//                 TLAS and instance table sit in LDS. Result: hits[i] = {t, tri, inst}.
+//   2. k_emit     per hit and per light that passes the reference's gate (n.l > 0, in
+//                 range, not band-gated) a shadow ray is appended to a dense queue; appends
+//                 are aggregated per wave with a ballot + one atomic, so each 64-ray chunk
+//                 holds one light's rays from neighbouring probes.
+//   3. k_walk<ANY>  the same persistent walk over the queue, any-hit; an unblocked ray
+//                 sets bit `light` of its ray's visibility mask.
+//   4. k_resolve  per ray: emissive + sum over lights, in light order, of the visible NEE
 //                 terms + the bounce from the previous atlas — the reference's summation
 //                 order, independent of the queue order, so the result is deterministic.
 // Output is stored k-major (rays[k * updated + slot]): stores coalesce and the blend reads
 // it as the K x N operand of its contraction.
 #include <hip/hip_fp16.h>
 
+// No FMA contraction anywhere in this translation unit (including the inlined helpers of
+// the headers below): ray origins, instance transforms and the triangle test are
+// evaluated in the operation order of the reference's graph, as the FP32 restatement
+// does, so hits and t agree bit for bit with it.
+#pragma clang fp contract(off)
+
 #include "kernels.h"
-#include "traverse.h"
+#include "traverse2.h"
 
 namespace {
 
@@ -94,7 +102,7 @@ __device__ float3 sample_atlas(const DevCascade &c, const ushort4 *__restrict__ 
 }
 
 // ray origin and direction of ray gid (probeTraceOrigin js/gi_probes.js:1714-1726, 1876-1885)
-__device__ __forceinline__ void make_ray(const DevCascade &c, const sbgi_tick_params &p, const float4 *__restrict__ dirs,
+__device__ __forceinline__ void make_ray(const DevCascade &c, float classify_strength, const float4 *__restrict__ dirs,
                                          const float *__restrict__ meta, uint32_t gid, float3 &ro, float3 &rd) {
     const uint32_t k = gid / c.updated;
     const uint32_t slot = gid - k * c.updated;
@@ -103,9 +111,9 @@ __device__ __forceinline__ void make_ray(const DevCascade &c, const sbgi_tick_pa
     ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
             c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
             c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
-    if (p.classify_strength != 0.f) {
+    if (classify_strength != 0.f) {
         const float *m = meta + (size_t)probe * 4;
-        ro = ro + f3(m[1], m[2], m[3]) * p.classify_strength;
+        ro = ro + f3(m[1], m[2], m[3]) * classify_strength;
     }
     rd = xyz(dirs[k]);
 }
@@ -189,83 +197,150 @@ __device__ __forceinline__ float3 light_contrib(const float *__restrict__ L, con
 }
 
 constexpr uint32_t kBlock = 256;
+// Walk blocks of 8 waves share one LDS copy of the TLAS, so the copy does not limit how
+// many waves a CU holds.
+constexpr uint32_t kWalkBlock = 512;
 
-// Stage 1: closest hit + shadow-ray emission. hits[gid] = {t, tri, inst, -}.
-template <bool COUNT, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_primary(DevScene s, DevCascade c, sbgi_tick_params p,
-                                                     const float4 *__restrict__ dirs, const float *__restrict__ meta,
-                                                     float4 *__restrict__ hits, ShadowQueue q,
-                                                     Counters *__restrict__ counters) {
-    if (LDS) stage_tlas_lds(s);
-    const uint32_t n = c.updated * c.rays;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    TravCounters<COUNT> cnt;
-    uint32_t shadow = 0;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        const uint32_t gid = base + threadIdx.x;
-        const bool active = gid < n;
-        float3 ro = f3(0.f, 0.f, 0.f), rd = f3(0.f, 0.f, 1.f);
-        float best_t = SBGI_T_MAX;
-        int best_tri = -1, best_inst = -1;
-        if (active) {
-            make_ray(c, p, dirs, meta, gid, ro, rd);
-            trace_closest<COUNT, LDS>(s, ro, rd, best_t, best_tri, best_inst, cnt);
-            hits[gid] = make_float4(best_t, __int_as_float(best_tri), __int_as_float(best_inst), 0.f);
-        }
-        const bool hit = active && best_tri >= 0;
-        // every wave runs the light loop together (ballots), non-hit lanes emit nothing
-        const bool wave_hit = __ballot(hit) != 0ull;
-        if (!wave_hit) continue;
-        HitFrame h;
-        if (hit) h = hit_frame(s, c, p, ro, rd, best_t, (uint32_t)best_tri, (uint32_t)best_inst);
-        for (uint32_t li = 0; li < s.light_count; li++) {
-            LightSample ls;
-            ls.pass = false;
-            if (hit) ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p);
-            const uint64_t m = __ballot(ls.pass);
-            if (m == 0ull) continue;
-            uint32_t first = 0;
-            if (lane == (uint32_t)__ffsll((unsigned long long)m) - 1u) first = atomicAdd(q.count, (uint32_t)__popcll(m));
-            first = __shfl(first, __ffsll((unsigned long long)m) - 1);
-            if (ls.pass) {
-                const uint32_t idx = first + (uint32_t)__popcll(m & lt_mask);
-                if (idx < q.capacity) {
-                    q.rec[(size_t)idx * 2] = make_float4(h.hit.x, h.hit.y, h.hit.z, ls.dist - h.bias);
-                    q.rec[(size_t)idx * 2 + 1] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, __uint_as_float((gid << 6) | li));
-                }
-                shadow++;
-            }
+// Ray sources of the walk kernels (see walk_wave).
+
+// Probe rays of this solve, index = k * updated + slot; result hits[i] = {t, tri, inst, 0}.
+struct PrimarySrc {
+    DevCascade c;
+    float classify;
+    const float4 *dirs;
+    const float *meta;
+    float4 *hits;
+    __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
+        float3 ro, rd;
+        make_ray(c, classify, dirs, meta, i, ro, rd);
+        lane_begin(s, r, ro, rd, SBGI_T_MAX);
+    }
+    __device__ __forceinline__ void store(uint32_t i, const LaneRay &r) const {
+        hits[i] = make_float4(r.tmax, __int_as_float(r.hit_tri), __int_as_float(r.hit_inst), 0.f);
+    }
+};
+
+// Shadow rays of the dense queue; an unblocked ray sets bit `light` of its ray's mask.
+struct ShadowSrc {
+    const float4 *rec;
+    unsigned long long *vis;
+    __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
+        const float4 a = rec[(size_t)i * 2], b = rec[(size_t)i * 2 + 1];
+        lane_begin(s, r, xyz(a), xyz(b), a.w);
+        r.tag = __float_as_uint(b.w);
+    }
+    __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
+        if (!r.blocked) atomicOr(&vis[r.tag >> 6], 1ull << (r.tag & 63u));
+    }
+};
+
+// Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit
+// within tmax, 2 closest hit reported as blocked when before tmax.
+// out[i] = {t, tri, inst, blocked}.
+struct QuerySrc {
+    const float4 *in;
+    float4 *out;
+    __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
+        const float4 a = in[(size_t)i * 2], b = in[(size_t)i * 2 + 1];
+        const bool any = b.w >= 0.5f && b.w < 1.5f;
+        lane_begin(s, r, xyz(a), xyz(b), any ? a.w : SBGI_T_MAX);
+        r.any = any;
+        r.tag = b.w >= 1.5f ? __float_as_uint(a.w) : 0x7FC00000u;  // quiet NaN: never "before"
+    }
+    __device__ __forceinline__ void store(uint32_t i, const LaneRay &r) const {
+        if (r.any) {
+            out[i] = make_float4(0.f, 0.f, 0.f, r.blocked ? 1.f : 0.f);
+        } else {
+            const float blocked = (r.hit_tri >= 0 && r.tmax < __uint_as_float(r.tag)) ? 1.f : 0.f;
+            out[i] = make_float4(r.tmax, __int_as_float(r.hit_tri), __int_as_float(r.hit_inst), blocked);
         }
     }
+};
+
+// Persistent walk: every wave pulls rays from the source until it is drained. The number
+// of rays is n_host, or min(*n_dev, n_host) when the count is only known on the device.
+template <int MODE, bool COUNT, bool LDS, class Src>
+__global__ void __launch_bounds__(kWalkBlock) k_walk(DevScene s, Src src, const uint32_t *__restrict__ n_dev,
+                                                      uint32_t n_host, uint32_t *heads, Counters *__restrict__ counters) {
+    if (LDS) stage_tlas_lds(s);
+    const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
+    TravCounters<COUNT> cnt;
+    walk_wave<MODE, COUNT, LDS>(s, src, n, heads, blockIdx.x * (kWalkBlock / 64u) + (threadIdx.x >> 6), cnt);
     if (COUNT) {
         atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
         atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
         atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
         atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
-        atomicAdd(&counters->shadow_rays, (unsigned long long)shadow);
     }
 }
 
-// Stage 2: any-hit over the dense shadow queue; unblocked -> visibility bit.
-template <bool COUNT, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_shadow(DevScene s, ShadowQueue q, unsigned long long *__restrict__ vis,
-                                                    Counters *__restrict__ counters) {
-    if (LDS) stage_tlas_lds(s);
-    const uint32_t n = min(*q.count, q.capacity);
-    TravCounters<COUNT> cnt;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const float4 a = q.rec[(size_t)i * 2], b = q.rec[(size_t)i * 2 + 1];
-        if (!trace_any<COUNT, LDS>(s, xyz(a), xyz(b), a.w, cnt)) {
-            const uint32_t w = __float_as_uint(b.w);
-            atomicOr(&vis[w >> 6], 1ull << (w & 63u));
+// Shadow-ray emission: for every primary hit and every light that passes the reference's
+// gate (n.l > 0, in range, not band-gated), one record is appended to the queue. A block
+// counts its records (per wave, per light, by ballot), reserves them with ONE atomic, and
+// writes them wave by wave and light by light, so each run of the queue holds one light's
+// rays from neighbouring probes and the queue costs one atomic per kEmitBlock rays.
+constexpr uint32_t kEmitBlock = 512;
+__global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, sbgi_tick_params p,
+                                                      const float4 *__restrict__ dirs, const float *__restrict__ meta,
+                                                      const float4 *__restrict__ hits, ShadowQueue q,
+                                                      Counters *__restrict__ counters) {
+    __shared__ uint32_t wave_base[kEmitBlock / 64];
+    const uint32_t n = c.updated * c.rays;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    const uint32_t gid = blockIdx.x * kEmitBlock + threadIdx.x;
+    const bool active = gid < n;
+    const float4 hr = active ? hits[gid] : make_float4(0.f, __int_as_float(-1), __int_as_float(-1), 0.f);
+    const int tri = __float_as_int(hr.y), inst = __float_as_int(hr.z);
+    const bool hit = active && tri >= 0;
+    const bool wave_hit = __ballot(hit) != 0ull;
+    HitFrame h;
+    if (hit) {
+        float3 ro, rd;
+        make_ray(c, p.classify_strength, dirs, meta, gid, ro, rd);
+        h = hit_frame(s, c, p, ro, rd, hr.x, (uint32_t)tri, (uint32_t)inst);
+    }
+    // pass 1: which lights pass, and how many records this wave appends
+    uint64_t pass_bits = 0;
+    uint32_t count = 0;
+    if (wave_hit) {
+        for (uint32_t li = 0; li < s.light_count; li++) {
+            bool pass = false;
+            if (hit) pass = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p).pass;
+            if (pass) pass_bits |= 1ull << li;
+            count += (uint32_t)__popcll(__ballot(pass));
         }
     }
-    if (COUNT) {
-        atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
-        atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
-        atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
-        atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
+    if (lane == 0) wave_base[wave] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kEmitBlock / 64; w++) {
+            const uint32_t c_w = wave_base[w];
+            wave_base[w] = total;
+            total += c_w;
+        }
+        const uint32_t base = total ? atomicAdd(q.count, total) : 0u;
+        for (uint32_t w = 0; w < kEmitBlock / 64; w++) wave_base[w] += base;
+        if (counters && total) atomicAdd(&counters->shadow_rays, (unsigned long long)total);
+    }
+    __syncthreads();
+    if (!wave_hit) return;
+    // pass 2: write the records
+    uint32_t pos = wave_base[wave];
+    for (uint32_t li = 0; li < s.light_count; li++) {
+        const bool pass = (pass_bits >> li) & 1ull;
+        const uint64_t m = __ballot(pass);
+        if (m == 0ull) continue;
+        if (pass) {
+            const LightSample ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p);
+            const uint32_t idx = pos + (uint32_t)__popcll(m & lt_mask);
+            if (idx < q.capacity) {
+                q.rec[(size_t)idx * 2] = make_float4(h.hit.x, h.hit.y, h.hit.z, ls.dist - h.bias);
+                q.rec[(size_t)idx * 2 + 1] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, __uint_as_float((gid << 6) | li));
+            }
+        }
+        pos += (uint32_t)__popcll(m);
     }
 }
 
@@ -280,7 +355,7 @@ __global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sb
     const uint32_t n = c.updated * c.rays;
     if (gid >= n) return;
     float3 ro, rd;
-    make_ray(c, p, dirs, meta, gid, ro, rd);
+    make_ray(c, p.classify_strength, dirs, meta, gid, ro, rd);
     const float4 hr = hits[gid];
     const int tri = __float_as_int(hr.y), inst = __float_as_int(hr.z);
     float3 out;
@@ -320,29 +395,12 @@ __global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sb
     if (hit_ids) hit_ids[gid] = make_int2(tri, inst);
 }
 
-__global__ void k_clear_u64(unsigned long long *__restrict__ p, uint32_t n, uint32_t *__restrict__ counter) {
+// Clears the visibility masks, the queue length and the fetch counters of the primary and
+// shadow walks (the solve counters of kernels.h).
+__global__ void k_clear_u64(unsigned long long *__restrict__ p, uint32_t n, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
-    if (i == 0 && counter) *counter = 0u;
-}
-
-// Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit,
-// 2 closest hit reported as blocked when before tmax. out[i] = {t, tri, inst, blocked}.
-__global__ void k_query(DevScene s, const float4 *__restrict__ in, float4 *__restrict__ out, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 a = in[(size_t)i * 2], b = in[(size_t)i * 2 + 1];
-    TravCounters<false> cnt;
-    if (b.w < 0.5f || b.w >= 1.5f) {
-        float t;
-        int tri, inst;
-        trace_closest<false, false>(s, xyz(a), xyz(b), t, tri, inst, cnt);
-        const float blocked = (b.w >= 1.5f && tri >= 0 && t < a.w) ? 1.f : 0.f;
-        out[i] = make_float4(t, __int_as_float(tri), __int_as_float(inst), blocked);
-    } else {
-        const bool blocked = trace_any<false, false>(s, xyz(a), xyz(b), a.w, cnt);
-        out[i] = make_float4(0.f, 0.f, 0.f, blocked ? 1.f : 0.f);
-    }
+    if (i < 1u + 2u * kFetchHeads) counters[(size_t)i * kHeadStride] = 0u;
 }
 
 // Direction table: normalize(rayDir(k, jitter)) (js/gi_probes.js:1692-1704, 1984).
@@ -355,7 +413,10 @@ __global__ void k_raydirs(float4 *__restrict__ dirs, uint32_t rays, float jitter
     float z = 1.f - uw * 2.f;
     float r = sqrtf(fmaxf(0.f, 1.f - z * z));
     float phi = (float)k * SBGI_GOLDEN_ANGLE + jitter * SBGI_TWO_PI;
-    float3 d = normalize3(f3(r * cosf(phi), r * sinf(phi), z));
+    // cos/sin of the f32 phi, rounded once from double: the correctly rounded f32 values
+    // (R evaluations per solve, so the double path costs nothing)
+    const float cphi = (float)cos((double)phi), sphi = (float)sin((double)phi);
+    float3 d = normalize3(f3(r * cphi, r * sphi, z));
     dirs[k] = make_float4(d.x, d.y, d.z, 0.f);
 }
 
@@ -373,12 +434,12 @@ int device_cus() {
 // Persistent grid: as many blocks as can be resident at once (occupancy API), capped by
 // the work. Every block then strides over the work, so no block waits for a slot.
 template <typename K>
-uint32_t persistent_blocks(K kernel, size_t lds, uint32_t work_items) {
+uint32_t persistent_blocks(K kernel, uint32_t block, size_t lds, uint32_t work_items) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)kBlock, lds) != hipSuccess || per_cu <= 0)
-        per_cu = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, lds) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
     const uint32_t cap = (uint32_t)device_cus() * (uint32_t)per_cu;
-    const uint32_t want = (work_items + kBlock - 1) / kBlock;
+    const uint32_t want = (work_items + block - 1) / block;
     return want < cap ? (want ? want : 1u) : cap;
 }
 
@@ -388,9 +449,14 @@ size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count) {
     return ((size_t)tlas_count * 2 + (size_t)inst_count * 4) * sizeof(uint4);
 }
 
-hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, hipStream_t st) {
+hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, uint32_t *head, hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_query, dim3((n + 255) / 256), dim3(256), 0, st, s, in, out, n);
+    hipError_t e = hipMemsetAsync(head, 0, (size_t)kFetchHeads * kHeadStride * sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+    const QuerySrc src{in, out};
+    auto kern = k_walk<WALK_MIXED, false, false, QuerySrc>;
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, 0, n)), dim3(kWalkBlock), 0, st, s, src, nullptr,
+                       n, head, nullptr);
     return hipGetLastError();
 }
 
@@ -404,30 +470,36 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
                         int2 *hit_ids, Counters *counters, hipStream_t st) {
     const uint32_t n = c.updated * c.rays;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_clear_u64, dim3((n + 255) / 256), dim3(256), 0, st, w.vis, n, w.queue.count);
+    uint32_t *ctr = w.queue.count;
+    const uint32_t nclear = std::max(n, 1u + 2u * kFetchHeads);
+    hipLaunchKernelGGL(k_clear_u64, dim3((nclear + 255) / 256), dim3(256), 0, st, w.vis, n, ctr);
     const size_t lds = trace_lds_bytes(s.tlas_count, s.inst_count);
     const bool use_lds = lds <= kTraceLdsBudget;
     const dim3 block(kBlock);
-#define SBGI_PRIMARY(CNT, L)                                                                              \
-    hipLaunchKernelGGL((k_primary<CNT, L>), dim3(persistent_blocks(k_primary<CNT, L>, L ? lds : 0, n)), block,  \
-                       L ? lds : 0, st, s, c, p, dirs, meta, w.hits, w.queue, counters)
-    if (counters) {
-        if (use_lds) SBGI_PRIMARY(true, true); else SBGI_PRIMARY(true, false);
-    } else {
-        if (use_lds) SBGI_PRIMARY(false, true); else SBGI_PRIMARY(false, false);
+    const PrimarySrc prim{c, p.classify_strength, dirs, meta, w.hits};
+    const ShadowSrc shad{w.queue.rec, w.vis};
+#define SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, CNT, L)                                                  \
+    {                                                                                                        \
+        auto kern = k_walk<MODE, CNT, L, SRC>;                                                               \
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, L ? lds : 0, NHOST)), dim3(kWalkBlock), \
+                           L ? lds : 0, st, s,                                                                 \
+                           SRCV, NDEV, NHOST, HEAD, counters);                                               \
     }
-#undef SBGI_PRIMARY
-    // the queue length is only known on the device: size the persistent grid for the
-    // capacity and let every block read the live count
-#define SBGI_SHADOW(CNT, L)                                                                                \
-    hipLaunchKernelGGL((k_shadow<CNT, L>), dim3(persistent_blocks(k_shadow<CNT, L>, L ? lds : 0, w.queue.capacity)), \
-                       block, L ? lds : 0, st, s, w.queue, w.vis, counters)
-    if (counters) {
-        if (use_lds) SBGI_SHADOW(true, true); else SBGI_SHADOW(true, false);
-    } else {
-        if (use_lds) SBGI_SHADOW(false, true); else SBGI_SHADOW(false, false);
+#define SBGI_WALK4(MODE, SRC, SRCV, NDEV, NHOST, HEAD)                                   \
+    if (counters) {                                                                    \
+        if (use_lds) SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, true, true)         \
+        else SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, true, false)                \
+    } else {                                                                           \
+        if (use_lds) SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, false, true)        \
+        else SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, false, false)               \
     }
-#undef SBGI_SHADOW
+    SBGI_WALK4(WALK_CLOSEST, PrimarySrc, prim, nullptr, n, ctr + trace_heads(0))
+    hipLaunchKernelGGL(k_emit, dim3((n + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, st, s, c, p, dirs, meta,
+                       w.hits, w.queue, counters);
+    // the queue length is only known on the device: the kernel reads it, bounded by the capacity
+    SBGI_WALK4(WALK_ANY, ShadowSrc, shad, w.queue.count, w.queue.capacity, ctr + trace_heads(1))
+#undef SBGI_WALK4
+#undef SBGI_WALK
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), block, 0, st, s, c, p, dirs, meta, prev_atlas,
                        w.hits, w.vis, rays, hit_ids);
     return hipGetLastError();

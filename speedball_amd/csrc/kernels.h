@@ -6,7 +6,12 @@
 
 // k_trace.hip
 // Dense queue of shadow rays: rec[2i] = {origin.xyz, tmax}, rec[2i+1] = {dir.xyz,
-// bits((ray << 6) | light)}; *count is the number of records appended this solve.
+// bits((ray << 6) | light)}. count points at the solve counters (kTraceCounterBytes):
+// word 0 is the number of records appended this solve; the fetch counters of the primary
+// (0), shadow (1) and query (2) walks start at word trace_heads(walk), one per 128 bytes.
+constexpr size_t kTraceCounterBytes = 32 * 1024;
+constexpr size_t trace_heads(uint32_t walk) { return (1u + (size_t)walk * kFetchHeads) * kHeadStride; }
+static_assert((trace_heads(3)) * sizeof(uint32_t) <= kTraceCounterBytes, "solve counters overflow");
 struct ShadowQueue {
     float4 *rec;
     uint32_t *count;
@@ -21,7 +26,7 @@ struct TraceScratch {
 };
 constexpr size_t kTraceLdsBudget = 64 * 1024;  // TLAS + instances staged in LDS up to this size
 size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count);
-hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, hipStream_t st);
+hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, uint32_t *head, hipStream_t st);
 hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                         const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
@@ -57,7 +62,7 @@ hipError_t launch_pack_inst(const float *mats, uint32_t inst_base, uint32_t inst
 // BLAS refit of the threaded range [root, end): bottom-up with per-node arrival counters.
 hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
                              uint32_t *arrive, uint32_t root, uint32_t end, hipStream_t st);
-// octant copies: bounds of copy node i <- canonical node perm[i] (after a refit)
+// walk-format node copies: bounds of copy node i <- pool node perm[i] (after a refit)
 hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_total,
                              hipStream_t st);
 hipError_t launch_refit_tlas(float *mats, uint32_t tlas_base, uint32_t tlas_count, const float *inst_bounds,

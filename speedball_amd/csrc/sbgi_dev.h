@@ -27,6 +27,7 @@
 
 struct DevScene {
     const uint4 *__restrict__ blas;
+    const uint4 *__restrict__ nodes;  // BLAS in the TLAS node format {min.xyz, miss}, {max.xyz, payload}
     const float4 *__restrict__ tri;
     const uint4 *__restrict__ tlas;
     const float4 *__restrict__ inst;
@@ -50,6 +51,11 @@ struct DevCascade {
     uint32_t aw, ah;      // irradiance atlas size in texels
     uint32_t daw, dah;    // depth atlas size
 };
+
+// Fetch counters of the persistent walks: kFetchHeads per walk, kHeadStride uint32 words
+// (one 128-byte line) apart.
+constexpr uint32_t kFetchHeads = 64;
+constexpr uint32_t kHeadStride = 32;
 
 struct Counters {
     unsigned long long blas_nodes, tlas_nodes, tri_tests, inst_visits, shadow_rays;

@@ -34,8 +34,8 @@ void box_apply(const float *mn, const float *mx, const double *e, double *omn, d
 
 // traversal-only layouts (accel.cpp)
 void build_sah_tlas(const float *inst_bounds, uint32_t n, std::vector<float> &out);
-bool build_octant_copies(const uint32_t *nodes, uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>> &roots,
-                         std::vector<uint32_t> &out, std::vector<uint32_t> &perm);
+bool build_node_copies(const uint32_t *nodes, uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>> &roots,
+                       uint32_t copies, std::vector<uint32_t> &out, std::vector<uint32_t> &perm);
 
 bool build_blas(const float *pos, uint32_t vcount, const uint32_t *idx, uint32_t tcount,
                 std::vector<FlatNode> &out, std::vector<uint32_t> &perm, std::string &err);

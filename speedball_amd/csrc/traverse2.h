@@ -75,45 +75,49 @@ __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3
 // Walk kinds: closest hit, any hit, or per ray (LaneRay::any, used by the query kernel).
 enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
 
+// The step functions compute every new field value first and then assign each field
+// exactly once, unconditionally: stores to different fields on different branches can be
+// merged by the compiler into one store through a pointer phi, which forces the lane
+// state out of registers into scratch memory.
+
 // NODE: one box test at the current level. Both levels use the 32-byte node
 // {min.xyz, miss}, {max.xyz, payload} with payload 0xFFFFFFFF for interior nodes, else
 // (count << 24) | offset (triangles for a BLAS leaf, instance slots for a TLAS leaf).
 template <bool COUNT, bool LDS>
 __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
-    if (r.cur >= r.end) {
-        r.state = r.level ? ST_INST : ST_DONE;
-        return;
-    }
-    uint4 a, b;
-    if (r.level) {
-        const uint4 *n = s.nodes + ((size_t)r.base + r.cur) * 2;
-        a = n[0];
-        b = n[1];
-        cnt.add_blas();
-    } else {
-        tlas_node<LDS>(s, r.cur, a, b);
-        cnt.add_tlas();
+    const bool in_range = r.cur < r.end;
+    uint4 a = make_uint4(0u, 0u, 0u, r.cur), b = make_uint4(0u, 0u, 0u, 0xFFFFFFFFu);
+    if (in_range) {
+        if (r.level) {
+            const uint4 *n = s.nodes + ((size_t)r.base + r.cur) * 2;
+            a = n[0];
+            b = n[1];
+            cnt.add_blas();
+        } else {
+            tlas_node<LDS>(s, r.cur, a, b);
+            cnt.add_tlas();
+        }
     }
     const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
     const float3 bmax = f3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
     float tn, tf;
     slab(bmin, bmax, r.o, r.id, tn, tf);
-    uint32_t next = a.w;
-    if (tf >= fmaxf(tn, 0.f) && tn < r.tmax) {
-        if (b.w == 0xFFFFFFFFu) {
-            next = r.cur + 1u;
-        } else if (r.level) {
-            r.toff = b.w & 0x00FFFFFFu;
-            r.tend = r.toff + (b.w >> 24);
-            r.state = ST_TRI;
-        } else {
-            r.inext = b.w & 0x00FFFFFFu;
-            r.iend = r.inext + (b.w >> 24);
-            r.tres = a.w;
-            r.state = ST_INST;
-        }
-    }
-    r.cur = next;
+    const bool hit = in_range && tf >= fmaxf(tn, 0.f) && tn < r.tmax;
+    const bool interior = b.w == 0xFFFFFFFFu;
+    const bool leaf = hit && !interior;
+    const uint32_t off = b.w & 0x00FFFFFFu, end = off + (b.w >> 24);
+    const bool blas_leaf = leaf && r.level, tlas_leaf = leaf && !r.level;
+    uint32_t state = r.state;
+    if (!in_range) state = r.level ? ST_INST : ST_DONE;
+    else if (blas_leaf) state = ST_TRI;
+    else if (tlas_leaf) state = ST_INST;
+    r.cur = in_range ? ((hit && interior) ? r.cur + 1u : a.w) : r.cur;
+    r.toff = blas_leaf ? off : r.toff;
+    r.tend = blas_leaf ? end : r.tend;
+    r.inext = tlas_leaf ? off : r.inext;
+    r.iend = tlas_leaf ? end : r.iend;
+    r.tres = tlas_leaf ? a.w : r.tres;
+    r.state = state;
 }
 
 // TRI: one Moller-Trumbore test of the pending leaf range.
@@ -123,53 +127,64 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
     const float4 *tp = s.tri + (size_t)r.toff * 3;
     const float4 A = tp[0], B = tp[1], C = tp[2];
     cnt.add_tri();
-    float th, det;
-    if (mt_test(A, B, C, r.o, r.d, th, det) && th > SBGI_RAY_EPS && th < r.tmax) {
-        const uint32_t flags = __float_as_uint(A.w);
-        if (side_ok(flags, det * r.dsign) && (flags & (ANY ? 8u : 4u))) {
-            if (ANY) {
-                r.blocked = true;
-                r.state = ST_DONE;
-                return;
-            }
-            r.tmax = th;
-            r.hit_tri = (int)r.toff;
-            r.hit_inst = (int)r.inst;
-        }
-    }
-    r.toff++;
-    if (r.toff >= r.tend) r.state = ST_NODE;
+    float th = 0.f, det = 0.f;
+    const bool geo = mt_test(A, B, C, r.o, r.d, th, det) && th > SBGI_RAY_EPS && th < r.tmax;
+    const uint32_t flags = __float_as_uint(A.w);
+    const bool accept = geo && side_ok(flags, det * r.dsign) && (flags & (ANY ? 8u : 4u));
+    const bool block = ANY && accept, better = !ANY && accept;
+    const uint32_t next = r.toff + 1u;
+    r.tmax = better ? th : r.tmax;
+    r.hit_tri = better ? (int)r.toff : r.hit_tri;
+    r.hit_inst = better ? (int)r.inst : r.hit_inst;
+    r.blocked = r.blocked || block;
+    r.state = block ? ST_DONE : (next >= r.tend ? ST_NODE : ST_TRI);
+    r.toff = next;
 }
 
 // INST: enter the next instance of the pending TLAS leaf, or go back to the TLAS.
 template <bool COUNT, bool LDS>
 __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
-    if (r.inext < r.iend) {
-        const uint32_t inst = r.inext++;
+    const bool enter = r.inext < r.iend;
+    // default: back to the TLAS in world space
+    float3 o = r.wo, d = r.wd, id = r.wid;
+    float dsign = r.dsign;
+    uint32_t cur = r.tres, end = s.tlas_count, base = r.base, inst = r.inst, level = 0u, state = ST_NODE;
+    if (enter) {
+        inst = r.inext;
         cnt.add_inst();
         float4 r0, r1, r2, r3;
         inst_rows<LDS>(s, inst, r0, r1, r2, r3);
         if (r3.w > 0.5f) {
-            r.o = inst_point(r0, r1, r2, r.wo);
-            r.d = inst_dir(r0, r1, r2, r.wd);
-            r.id = ieee_inv(r.d);
-            r.dsign = r3.z;
-            r.cur = (uint32_t)r3.x;
-            r.end = (uint32_t)r3.y;
-            r.base = octant_of(r.d, s.blas_octants) * s.node_count;
-            r.inst = inst;
-            r.level = 1u;
-            r.state = ST_NODE;
+            o = inst_point(r0, r1, r2, r.wo);
+            d = inst_dir(r0, r1, r2, r.wd);
+            id = ieee_inv(d);
+            dsign = r3.z;
+            cur = (uint32_t)r3.x;
+            end = (uint32_t)r3.y;
+            base = octant_of(d, s.blas_octants) * s.node_count;
+            level = 1u;
+        } else {  // inactive slot: stay in INST for the next one
+            o = r.o;
+            d = r.d;
+            id = r.id;
+            cur = r.cur;
+            end = r.end;
+            inst = r.inst;
+            level = r.level;
+            state = ST_INST;
         }
-        return;
     }
-    r.o = r.wo;
-    r.d = r.wd;
-    r.id = r.wid;
-    r.level = 0u;
-    r.cur = r.tres;
-    r.end = s.tlas_count;
-    r.state = ST_NODE;
+    r.inext = enter ? r.inext + 1u : r.inext;
+    r.o = o;
+    r.d = d;
+    r.id = id;
+    r.dsign = dsign;
+    r.cur = cur;
+    r.end = end;
+    r.base = base;
+    r.inst = inst;
+    r.level = level;
+    r.state = state;
 }
 
 // Wave schedule. A TRI or INST phase runs when the lanes waiting in it are at least

@@ -3,7 +3,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision bf16x3|bf16|fp16|fp32]
 
 Workload: the instanced two-level scene — 8 procedural meshes (UV spheres, tori,
-fbm-displaced grids) x 256 random-TRS instances = 262,144 triangles in a 40x20x25 m box,
+fbm-displaced grids) instanced 256 times with random TRS, 261,514 triangles (262,144 +-1%)
+in a 40x20x25 m box,
 16 point lights (log-uniform 1..100), 32x16x24 = 12,288 probes x 256 rays, 8x8
 irradiance and 16x16 depth tiles, 2 bounces (each step is one update that reads the
 previous update's atlas). A step is one full-field update: ray directions + weights,
@@ -32,34 +33,71 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(stats, hits: int) -> float:
+def walk_bytes(blas, tlas, inst, tris) -> float:
     """Reference-layout bytes a traversal touches: 32 B per BLAS node, 48 B per TLAS node,
-    64 B per instance entry, 36 B per triangle test (12 B index + 3 x 8 B vertex slots);
-    per primary hit add 16 B payload + 112 B material + 256 B of bounce taps."""
-    return (32.0 * stats.blas_nodes + 48.0 * stats.tlas_nodes + 64.0 * stats.inst_visits + 36.0 * stats.tri_tests
-            + (16 + 112 + 256) * hits)
+    64 B per instance entry, 36 B per triangle test (12 B index + 3 x 8 B vertex slots)."""
+    return 32.0 * blas + 48.0 * tlas + 64.0 * inst + 36.0 * tris
+
+
+def algorithmic_bytes(stats, hits: int) -> float:
+    """Whole trace: every walk's node/triangle bytes, plus per primary hit 16 B payload +
+    112 B material + 256 B of bounce taps."""
+    return walk_bytes(stats.blas_nodes, stats.tlas_nodes, stats.inst_visits, stats.tri_tests) + (16 + 112 + 256) * hits
+
+
+def shadow_walk_bytes(stats) -> float:
+    """The shadow any-hit walk alone (the dominant kernel)."""
+    return walk_bytes(stats.shadow_blas_nodes, stats.shadow_tlas_nodes, stats.shadow_inst_visits,
+                      stats.shadow_tri_tests)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_instanced_pmc.md")
+
+
+def measured_traffic(kernel_prefix: str):
+    """HBM bytes per dispatch (calibrated FETCH_SIZE + WRITE_SIZE) of a kernel, from the
+    committed rocprofv3 --pmc summary of this workload; None when it is not there."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic" and parts[1].startswith(kernel_prefix):
+                    return float(parts[2]) + float(parts[3])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(w, seconds_target=12.0):
-    """FP32 restatement (OpenMP over probes, all host cores) on a bounded probe sample."""
+    """FP32 restatement (OpenMP over probes) on a bounded sample of the same field: the
+    first n probes (n grows x4 until the run takes a few seconds), then whole-field
+    updates repeated until about `seconds_target` seconds of CPU work."""
     import oracle
     o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec="32")
     p = o.params(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
-    n = 16
-    while True:
+
+    def one(n):
         p.probe_offset = 0
         p.updated_count = n
         t0 = time.perf_counter()
         o.trace(p, n)
         o.blend(p)
         o.upload(p)
-        dt = time.perf_counter() - t0
-        if dt * 4 > seconds_target or n * 4 > w.probes:
-            break
-        n *= 4
-    return {"value": n * w.rays / dt, "unit": "rays/s", "cores": os.cpu_count(), "kind": "port",
-            "sample": f"FP32 restatement (oracle/), OpenMP, first {n} probes x {w.rays} rays of the same "
-                      f"field, one update from cleared state, {dt:.2f} s"}
+        return time.perf_counter() - t0
+
+    n = 16
+    dt = one(n)
+    while dt * 4 < seconds_target and n < w.probes:
+        n = min(n * 4, w.probes)
+        dt = one(n)
+    reps, total = 1, dt
+    while total < seconds_target and n == w.probes and reps < 16:
+        total += one(n)
+        reps += 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    return {"value": reps * n * w.rays / total, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 restatement (oracle/), OpenMP on {threads} threads: {reps} update(s) of the first {n} "
+                      f"probes x {w.rays} rays of the same field from cleared state, {total:.2f} s"}
 
 
 def fp64_check(w, field, precision):
@@ -138,7 +176,8 @@ def main():
     ms_step = wall * 1000.0 / args.steps
 
     # per-stage device time from the library's HIP events (recorded on the field's stream)
-    stage = {"trace": [], "blend": [], "upload": [], "total": [], "raydir": []}
+    stage = {"trace": [], "blend": [], "upload": [], "total": [], "raydir": [], "walk_primary": [], "emit": [],
+             "walk_shadow": [], "resolve": []}
     for _ in range(args.steps):
         f.tick(**kw)
         s = f.stats()
@@ -147,6 +186,10 @@ def main():
         stage["upload"].append(s.ms_upload)
         stage["raydir"].append(s.ms_raydir)
         stage["total"].append(s.ms_total)
+        stage["walk_primary"].append(s.ms_walk_primary)
+        stage["emit"].append(s.ms_emit)
+        stage["walk_shadow"].append(s.ms_walk_shadow)
+        stage["resolve"].append(s.ms_resolve)
     rays = w.probes * w.rays
     value = rays / (ms_step / 1000.0)
 
@@ -158,7 +201,11 @@ def main():
     hits = int(np.count_nonzero(f.read_rays()[..., 3] >= 0))
     bytes_alg = algorithmic_bytes(cs, hits)
     trace_ms = float(np.mean(stage["trace"]))
-    achieved = bytes_alg / (trace_ms / 1000.0) / 1e9
+    # roofline of the dominant kernel, the shadow any-hit walk, timed by its own events
+    sw_bytes = shadow_walk_bytes(cs)
+    sw_ms = float(np.mean(stage["walk_shadow"]))
+    achieved = sw_bytes / (sw_ms / 1000.0) / 1e9
+    traffic = measured_traffic("k_walk<1,false,true,ShadowSrc>") if args.traversal == "ordered" else None
     blend_ms = float(np.mean(stage["blend"]))
     T_irr, T_d = 36, w.depth_oct * w.depth_oct
     blend_flops = 2.0 * w.probes * w.rays * (5 * T_irr + 2 * T_d)
@@ -174,7 +221,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": f"f32 traversal, {args.precision} blend", "data": "synthetic",
-        "config": {"workload": "instanced 8 meshes x 256 instances, 262144 triangles, 16 point lights, "
+        "config": {"workload": "instanced: 8 meshes in 256 instances, 261514 triangles, 16 point lights, "
                                "32x16x24 probes x 256 rays, irr 8x8, depth 16x16, 2 bounces",
                    "probes": w.probes, "rays_per_probe": w.rays, "triangles": w.scene.meta.get("instanced_triangles"),
                    "unique_triangles": w.scene.meta.get("unique_triangles"), "lights": 16,
@@ -184,12 +231,17 @@ def main():
         "shadow_rays_per_update": int(cs.shadow_rays),
         "total_rays_per_s": (rays + cs.shadow_rays) / (ms_step / 1000.0),
         "irradiance_rel_err_vs_fp64": {"p99": err99, "max": errmax, "probes_checked": 24},
-        "roofline": {"bound": "hbm", "kernel": "k_trace", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": bytes_alg, "avg_launch_ms": trace_ms,
-                     "counts": {"blas_nodes": int(cs.blas_nodes), "tlas_nodes": int(cs.tlas_nodes),
-                                "tri_tests": int(cs.tri_tests), "inst_visits": int(cs.inst_visits),
-                                "primary_hits": hits}},
+        "roofline": {"bound": "hbm", "kernel": "k_walk<ANY> (shadow any-hit walk)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": sw_bytes, "avg_launch_ms": sw_ms,
+                     "traffic_source": "profiles/walk_instanced_pmc.md" if traffic else None,
+                     "counts": {"blas_nodes": int(cs.shadow_blas_nodes), "tlas_nodes": int(cs.shadow_tlas_nodes),
+                                "tri_tests": int(cs.shadow_tri_tests), "inst_visits": int(cs.shadow_inst_visits),
+                                "rays": int(cs.shadow_rays)}},
+        "trace": {"algorithmic_bytes": bytes_alg, "ms": trace_ms, "gbs": bytes_alg / (trace_ms / 1000.0) / 1e9,
+                  "counts": {"blas_nodes": int(cs.blas_nodes), "tlas_nodes": int(cs.tlas_nodes),
+                             "tri_tests": int(cs.tri_tests), "inst_visits": int(cs.inst_visits),
+                             "primary_hits": hits}},
         "blend": {"kernel": "k_blend", "ms": blend_ms, "tflops": blend_flops / (blend_ms / 1000.0) / 1e12,
                   "gbs": blend_bytes / (blend_ms / 1000.0) / 1e9, "flops": blend_flops, "bytes": blend_bytes},
         "cpu_baseline": cpu,

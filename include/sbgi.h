@@ -184,6 +184,17 @@ typedef struct sbgi_stats {
     uint64_t tlas_nodes;         /* TLAS node visits                                  */
     uint64_t tri_tests;          /* triangle tests                                    */
     uint64_t inst_visits;        /* instance entries                                  */
+    /* trace sub-stages of the last tick (ms): closest-hit walk, shadow-ray emission,
+       shadow any-hit walk, shading; their sum is ms_trace minus the clears            */
+    float ms_walk_primary;
+    float ms_emit;
+    float ms_walk_shadow;
+    float ms_resolve;
+    /* the shadow walk's share of the counts above (counting enabled)                 */
+    uint64_t shadow_blas_nodes;
+    uint64_t shadow_tlas_nodes;
+    uint64_t shadow_tri_tests;
+    uint64_t shadow_inst_visits;
 } sbgi_stats;
 
 typedef struct sbgi_field sbgi_field;

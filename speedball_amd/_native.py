@@ -83,7 +83,10 @@ class Stats(C.Structure):
     _fields_ = [("ms_raydir", C.c_float), ("ms_trace", C.c_float), ("ms_blend", C.c_float),
                 ("ms_upload", C.c_float), ("ms_total", C.c_float), ("rays", C.c_uint32),
                 ("updated", C.c_uint32), ("shadow_rays", C.c_uint64), ("blas_nodes", C.c_uint64),
-                ("tlas_nodes", C.c_uint64), ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64)]
+                ("tlas_nodes", C.c_uint64), ("tri_tests", C.c_uint64), ("inst_visits", C.c_uint64),
+                ("ms_walk_primary", C.c_float), ("ms_emit", C.c_float), ("ms_walk_shadow", C.c_float),
+                ("ms_resolve", C.c_float), ("shadow_blas_nodes", C.c_uint64), ("shadow_tlas_nodes", C.c_uint64),
+                ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64)]
 
 
 class SbgiError(RuntimeError):

@@ -72,6 +72,7 @@ struct sbgi_field {
     // tickets and timing
     hipEvent_t ring[kTicketRing] = {};
     hipEvent_t ev[5] = {};
+    hipEvent_t tev[4] = {};  // trace sub-stages
     uint64_t ticket = 0;
     bool stats_valid = false;
     std::string err;
@@ -431,6 +432,8 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     for (auto &e : f->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
+    for (auto &e : f->tev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     const uint32_t tile_d = desc->depth_oct_res + 2;
     for (uint32_t ci = 0; ci < desc->cascade_count; ci++) {
         CascadeState &cs = f->casc[ci];
@@ -733,7 +736,7 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
     HIP_OK(f, hipEventRecord(f->ev[1], st));
     HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, f->tw, f->rays, f->hit_ids,
-                           f->count_mode ? f->counters : nullptr, st));
+                           f->count_mode ? f->counters : nullptr, f->tev, st));
     HIP_OK(f, hipEventRecord(f->ev[2], st));
     HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, st));
     HIP_OK(f, hipEventRecord(f->ev[3], st));
@@ -886,11 +889,24 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     out->rays = f->last_updated * f->desc.rays_per_probe;
     Counters c{};
     HIP_OK(f, hipMemcpy(&c, f->counters, sizeof(c), hipMemcpyDeviceToHost));
-    out->blas_nodes = c.blas_nodes;
-    out->tlas_nodes = c.tlas_nodes;
-    out->tri_tests = c.tri_tests;
-    out->inst_visits = c.inst_visits;
+    out->blas_nodes = c.walk[0] + c.shadow_walk[0];
+    out->tlas_nodes = c.walk[1] + c.shadow_walk[1];
+    out->tri_tests = c.walk[2] + c.shadow_walk[2];
+    out->inst_visits = c.walk[3] + c.shadow_walk[3];
     out->shadow_rays = c.shadow_rays;
+    out->shadow_blas_nodes = c.shadow_walk[0];
+    out->shadow_tlas_nodes = c.shadow_walk[1];
+    out->shadow_tri_tests = c.shadow_walk[2];
+    out->shadow_inst_visits = c.shadow_walk[3];
+    float sub[4] = {0.f, 0.f, 0.f, 0.f};
+    if (f->last_updated) {
+        for (int i = 0; i < 3; i++) HIP_OK(f, hipEventElapsedTime(&sub[i], f->tev[i], f->tev[i + 1]));
+        HIP_OK(f, hipEventElapsedTime(&sub[3], f->tev[3], f->ev[2]));
+    }
+    out->ms_walk_primary = sub[0];
+    out->ms_emit = sub[1];
+    out->ms_walk_shadow = sub[2];
+    out->ms_resolve = sub[3];
     return SBGI_OK;
 }
 
@@ -919,6 +935,8 @@ int32_t sbgi_destroy(sbgi_field *f) {
     for (auto &e : f->ring)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : f->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : f->tev)
         if (e) (void)hipEventDestroy(e);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;

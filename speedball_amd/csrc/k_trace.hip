@@ -261,16 +261,17 @@ struct QuerySrc {
 // of rays is n_host, or min(*n_dev, n_host) when the count is only known on the device.
 template <int MODE, bool COUNT, bool LDS, class Src>
 __global__ void __launch_bounds__(kWalkBlock) k_walk(DevScene s, Src src, const uint32_t *__restrict__ n_dev,
-                                                      uint32_t n_host, uint32_t *heads, Counters *__restrict__ counters) {
+                                                      uint32_t n_host, uint32_t *heads,
+                                                      unsigned long long *__restrict__ counters) {
     if (LDS) stage_tlas_lds(s);
     const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     TravCounters<COUNT> cnt;
     walk_wave<MODE, COUNT, LDS>(s, src, n, heads, blockIdx.x * (kWalkBlock / 64u) + (threadIdx.x >> 6), cnt);
     if (COUNT) {
-        atomicAdd(&counters->blas_nodes, (unsigned long long)cnt.blas);
-        atomicAdd(&counters->tlas_nodes, (unsigned long long)cnt.tlas);
-        atomicAdd(&counters->tri_tests, (unsigned long long)cnt.tris);
-        atomicAdd(&counters->inst_visits, (unsigned long long)cnt.insts);
+        atomicAdd(&counters[0], (unsigned long long)cnt.blas);
+        atomicAdd(&counters[1], (unsigned long long)cnt.tlas);
+        atomicAdd(&counters[2], (unsigned long long)cnt.tris);
+        atomicAdd(&counters[3], (unsigned long long)cnt.insts);
     }
 }
 
@@ -465,42 +466,47 @@ hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t
     return hipGetLastError();
 }
 
+// One persistent walk launch: counting and LDS variants picked at run time.
+template <int MODE, class Src>
+void launch_walk(const DevScene &s, const Src &src, const uint32_t *n_dev, uint32_t n_host, uint32_t *heads,
+                 unsigned long long *cnt, size_t lds, hipStream_t st) {
+    const bool use_lds = lds <= kTraceLdsBudget;
+    auto go = [&](auto kern, size_t l) {
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, l, n_host)), dim3(kWalkBlock), l, st, s, src,
+                           n_dev, n_host, heads, cnt);
+    };
+    if (cnt) {
+        if (use_lds) go(k_walk<MODE, true, true, Src>, lds);
+        else go(k_walk<MODE, true, false, Src>, 0);
+    } else {
+        if (use_lds) go(k_walk<MODE, false, true, Src>, lds);
+        else go(k_walk<MODE, false, false, Src>, 0);
+    }
+}
+
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                         const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
-                        int2 *hit_ids, Counters *counters, hipStream_t st) {
+                        int2 *hit_ids, Counters *counters, const hipEvent_t *ev, hipStream_t st) {
     const uint32_t n = c.updated * c.rays;
     if (n == 0) return hipSuccess;
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
     uint32_t *ctr = w.queue.count;
     const uint32_t nclear = std::max(n, 1u + 2u * kFetchHeads);
     hipLaunchKernelGGL(k_clear_u64, dim3((nclear + 255) / 256), dim3(256), 0, st, w.vis, n, ctr);
     const size_t lds = trace_lds_bytes(s.tlas_count, s.inst_count);
-    const bool use_lds = lds <= kTraceLdsBudget;
-    const dim3 block(kBlock);
-    const PrimarySrc prim{c, p.classify_strength, dirs, meta, w.hits};
-    const ShadowSrc shad{w.queue.rec, w.vis};
-#define SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, CNT, L)                                                  \
-    {                                                                                                        \
-        auto kern = k_walk<MODE, CNT, L, SRC>;                                                               \
-        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, L ? lds : 0, NHOST)), dim3(kWalkBlock), \
-                           L ? lds : 0, st, s,                                                                 \
-                           SRCV, NDEV, NHOST, HEAD, counters);                                               \
-    }
-#define SBGI_WALK4(MODE, SRC, SRCV, NDEV, NHOST, HEAD)                                   \
-    if (counters) {                                                                    \
-        if (use_lds) SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, true, true)         \
-        else SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, true, false)                \
-    } else {                                                                           \
-        if (use_lds) SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, false, true)        \
-        else SBGI_WALK(MODE, SRC, SRCV, NDEV, NHOST, HEAD, false, false)               \
-    }
-    SBGI_WALK4(WALK_CLOSEST, PrimarySrc, prim, nullptr, n, ctr + trace_heads(0))
+    hipError_t e = mark(0);
+    if (e != hipSuccess) return e;
+    launch_walk<WALK_CLOSEST>(s, PrimarySrc{c, p.classify_strength, dirs, meta, w.hits}, nullptr, n,
+                              ctr + trace_heads(0), counters ? counters->walk : nullptr, lds, st);
+    if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_emit, dim3((n + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, st, s, c, p, dirs, meta,
                        w.hits, w.queue, counters);
+    if ((e = mark(2)) != hipSuccess) return e;
     // the queue length is only known on the device: the kernel reads it, bounded by the capacity
-    SBGI_WALK4(WALK_ANY, ShadowSrc, shad, w.queue.count, w.queue.capacity, ctr + trace_heads(1))
-#undef SBGI_WALK4
-#undef SBGI_WALK
-    hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), block, 0, st, s, c, p, dirs, meta, prev_atlas,
-                       w.hits, w.vis, rays, hit_ids);
+    launch_walk<WALK_ANY>(s, ShadowSrc{w.queue.rec, w.vis}, w.queue.count, w.queue.capacity, ctr + trace_heads(1),
+                          counters ? counters->shadow_walk : nullptr, lds, st);
+    if ((e = mark(3)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
+                       prev_atlas, w.hits, w.vis, rays, hit_ids);
     return hipGetLastError();
 }

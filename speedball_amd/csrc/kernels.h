@@ -28,9 +28,11 @@ constexpr size_t kTraceLdsBudget = 64 * 1024;  // TLAS + instances staged in LDS
 size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count);
 hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, uint32_t *head, hipStream_t st);
 hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
+// ev (optional): 4 events recorded before the closest-hit walk, after it, after the
+// shadow-ray emission and after the shadow walk.
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                         const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
-                        int2 *hit_ids, Counters *counters, hipStream_t st);
+                        int2 *hit_ids, Counters *counters, const hipEvent_t *ev, hipStream_t st);
 
 // k_blend.hip
 struct BlendTables {

@@ -57,8 +57,11 @@ struct DevCascade {
 constexpr uint32_t kFetchHeads = 64;
 constexpr uint32_t kHeadStride = 32;
 
+// Traversal counters of counting ticks: [0..3] closest-hit walk {BLAS nodes, TLAS nodes,
+// triangle tests, instance entries}, [4] shadow rays emitted, [5..8] the shadow walk's
+// {BLAS nodes, TLAS nodes, triangle tests, instance entries}.
 struct Counters {
-    unsigned long long blas_nodes, tlas_nodes, tri_tests, inst_visits, shadow_rays;
+    unsigned long long walk[4], shadow_rays, shadow_walk[4];
 };
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

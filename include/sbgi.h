@@ -68,8 +68,12 @@ extern "C" {
 /* closest-hit traversal */
 #define SBGI_TRAV_MT_COMPAT 0   /* stackless left-first threaded walk + Moller-Trumbore,
                                    the reference's visit order and epsilons */
-#define SBGI_TRAV_MT_ORDERED 1  /* nearest-child-first short-stack walk, same triangle test */
-#define SBGI_TRAV_WATERTIGHT 2  /* ordered walk + watertight triangle test (Woop et al. 2013) */
+#define SBGI_TRAV_MT_ORDERED 1  /* same stackless walk over a binned-SAH TLAS and near-first
+                                   octant copies of the BLAS pool; same triangle test, so
+                                   results differ only in which of two exactly equal-t
+                                   hits wins */
+#define SBGI_TRAV_WATERTIGHT 2  /* reserved (watertight triangle test, Woop et al. 2013);
+                                   sbgi_create rejects it in this build */
 
 /* buffer ids for sbgi_read / sbgi_write / sbgi_update_ranges */
 #define SBGI_BUF_BVH_NODES 0    /* u32[8*N]                 (update_ranges)           */

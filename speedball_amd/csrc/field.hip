@@ -49,7 +49,9 @@ struct sbgi_field {
     uint4 *d_tlas = nullptr;
     int32_t *d_parent = nullptr, *d_tlas_parent = nullptr;
     uint32_t *d_arrive = nullptr, *d_tlas_arrive = nullptr;
-    // MT_ORDERED traversal layouts: 8 octant copies of the BLAS pool + an SAH TLAS
+    // walk layouts: the BLAS pool in the walk's node format (1 copy, or 8 near-first
+    // octant copies in MT_ORDERED mode) with the pool node behind each copy node, and
+    // the SAH TLAS of MT_ORDERED mode
     uint32_t *d_walk_nodes = nullptr, *d_walk_perm = nullptr;
     uint4 *d_tlas_sah = nullptr;
     uint32_t tlas_sah_count = 0;

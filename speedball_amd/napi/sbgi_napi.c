@@ -369,6 +369,10 @@ static napi_value js_field_stats(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, o, "msTotal", make_f64(env, s.ms_total));
     napi_set_named_property(env, o, "rays", make_u32(env, s.rays));
     napi_set_named_property(env, o, "updated", make_u32(env, s.updated));
+    napi_set_named_property(env, o, "msWalkPrimary", make_f64(env, s.ms_walk_primary));
+    napi_set_named_property(env, o, "msEmit", make_f64(env, s.ms_emit));
+    napi_set_named_property(env, o, "msWalkShadow", make_f64(env, s.ms_walk_shadow));
+    napi_set_named_property(env, o, "msResolve", make_f64(env, s.ms_resolve));
     return o;
 }
 

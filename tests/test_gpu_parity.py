@@ -7,15 +7,20 @@ MFMA blends within 1e-3 relative of the FP64 oracle.
 """
 from __future__ import annotations
 
+import os
+import shutil
+
 import numpy as np
 import pytest
 
 import oracle
 from speedball_amd import _native as N
+from speedball_amd import scenegen as sg
 from speedball_amd.field import ProbeField
 from tests.parity import rel_err, summary, interior_irr, near_tie_mask, classify_mismatches
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def make_field(w, precision=N.BLEND_FP32, keep=True, res=None, traversal=N.TRAV_MT_COMPAT):
@@ -192,3 +197,37 @@ def test_instanced_subset(instanced_small, traversal):
     ei = rel_err(interior_irr(g_irr2)[..., :3], interior_irr(o.irr_view())[..., :3])
     print("instanced irradiance after 2 updates", summary(ei))
     assert np.percentile(ei, 99) <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
+def test_js_shim_matches_ctypes_path(tmp_path):
+    """installSpeedballGI's field (JS shim -> N-API addon -> C ABI) and the ctypes path
+    (speedball_amd.field) give byte-identical irradiance state on the same scene, grid
+    and parameters after two updates: the JS host drives the same library."""
+    import json
+    import subprocess
+    w = sg.room_boxes(res=(6, 6, 6), rays=64)
+    s = w.scene
+    for name, arr in (("bvhNodes", s.bvh_nodes), ("triIndex", s.tri_index), ("vertexData", s.vertex_data),
+                      ("triMaterial", s.tri_material), ("materials", s.materials), ("lights", s.lights)):
+        arr.tofile(str(tmp_path / f"{name}.bin"))
+    gmax = [a + b for a, b in zip(w.grid_min, w.grid_size)]
+    meta = dict(instBase=s.inst_base, instCount=s.inst_count, tlasBase=s.tlas_base, tlasNodeCount=s.tlas_node_count,
+                rays=64, precision="bf16x3", traversal="ordered", gridMin=list(w.grid_min), gridMax=gmax,
+                res=list(w.res), hysteresis=0.6)
+    (tmp_path / "meta.json").write_text(json.dumps(meta))
+    print("case: js shim room 6^3 x 64, bf16x3, ordered", flush=True)
+    r = subprocess.run(["node", os.path.join(ROOT, "tests", "js", "gpu_room.js"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    js_irr = np.fromfile(str(tmp_path / "js_irr.bin"), np.float32)
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=64, depth_oct=6, precision=2, traversal=1)
+    f.set_scene(s)
+    for _ in range(2):
+        f.tick(frame_jitter=0.0, hysteresis=0.6)
+    py_irr = f.read_irr().reshape(-1)
+    f.close()
+    assert js_irr.shape == py_irr.shape
+    assert np.array_equal(js_irr, py_irr)

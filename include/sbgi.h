@@ -277,6 +277,31 @@ int32_t sbgi_set_counting(sbgi_field *f, int32_t on);
  * {t, tri (int32 bits), inst (int32 bits), blocked} per ray. Synchronous. */
 int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *out4);
 
+/* ---- differentiable mode ---------------------------------------------------------
+ * No counterpart in the reference (its probe update is not differentiable); the build
+ * adds it for gradient-based callers (inverse rendering, light/albedo fitting).
+ *
+ * The differentiable solve is two first-updates of cascade 0 from cleared state, so
+ * each update's irradiance is the cosine-weighted mean of its rays with no history
+ * term; the second update's bounce reads the first update's unrounded, unfiltered FP32
+ * atlas. Blend FP32, filter off. Its output is the second update's interior irradiance
+ * irr_out[probe][36][3]. Parameters: per-instance-triangle albedo (replacing the
+ * material colour in kd) albedo[n][3] with n from sbgi_diff_param_count, ordered by
+ * TLAS slot, then pool triangle within the slot's BLAS (sbgi_diff_albedo_map gives each
+ * slot's (offset, first pool triangle)); and one scale per light on its colour x
+ * intensity (light_scale[lights], NULL = all 1). */
+int32_t sbgi_diff_param_count(sbgi_field *f, uint32_t *albedo_entries, uint32_t *lights);
+int32_t sbgi_diff_albedo_map(sbgi_field *f, int32_t *map2, uint32_t n_slots);
+int32_t sbgi_diff_forward(sbgi_field *f, const sbgi_tick_params *p, const float *albedo, const float *light_scale,
+                          float *irr_out);
+/* Adjoint of the last sbgi_diff_forward (hit sets frozen from it; any sbgi_tick in
+ * between invalidates them: SBGI_E_STATE): gradients of sum(grad_irr * irr_out) with
+ * respect to albedo (n x 3) and the light scales (lights; may be NULL). */
+int32_t sbgi_diff_adjoint(sbgi_field *f, const float *grad_irr, float *grad_albedo, float *grad_light_scale);
+/* Device time (HIP events) of the last diff forward (both updates, before the host
+ * copy) and of the last adjoint's kernels. */
+int32_t sbgi_diff_timing(sbgi_field *f, float *ms_forward, float *ms_adjoint);
+
 /* Last error message of this field (or of the last failed sbgi_create when f is NULL). */
 const char *sbgi_last_error(const sbgi_field *f);
 

@@ -195,3 +195,54 @@ def refit_blas(nodes: np.ndarray, scene, root: int, end: int) -> bool:
 def refit_tlas(tlas: np.ndarray, inst_bounds: np.ndarray) -> bool:
     f = lib().or_refit_tlas
     return bool(f(_p(tlas), C.c_uint32(tlas.size // 12), _p(inst_bounds), C.c_uint32(inst_bounds.size // 6)))
+
+
+# ---------------------------------------------------------------- differentiable mode
+def albedo_map(scene) -> tuple[np.ndarray, int]:
+    """Per TLAS slot: (offset into the albedo array, first pool triangle of its BLAS).
+    Entries are slot-major, then pool-triangle order within the BLAS."""
+    nodes = scene.bvh_nodes.reshape(-1, 8)
+    inst = scene.materials[scene.inst_base:scene.inst_base + scene.inst_count * 28].reshape(-1, 28)
+    m = np.zeros((scene.inst_count, 2), np.int32)
+    off = 0
+    for slot in range(scene.inst_count):
+        root, end = int(inst[slot, 12]), int(inst[slot, 13])
+        pay = nodes[root:end, 7]
+        leaf = pay != 0xFFFFFFFF
+        first = int((pay[leaf] & 0xFFFFFF).min())
+        count = int((pay[leaf] >> 24).sum())
+        m[slot] = (off, first)
+        off += count
+    return m, off
+
+
+def diff_forward(scene, grid_min, grid_size, res, rays, p, albedo, amap, light_scale, prec="64"):
+    """Two first-updates from cleared state, the second bouncing off the first's unrounded
+    atlas; returns (E2, E1, A1) with E* of shape (P, 36, 3)."""
+    dt = _dt(prec)
+    c = cascade_desc(grid_min, grid_size, res)
+    P = res[0] * res[1] * res[2]
+    E2 = np.zeros(P * 36 * 3, dt)
+    E1 = np.zeros(P * 36 * 3, dt)
+    A1 = np.zeros(res[0] * 8 * res[1] * res[2] * 8 * 4, dt)
+    alb = np.ascontiguousarray(albedo, dt).reshape(-1)
+    ls = np.ascontiguousarray(light_scale, dt)
+    mp = np.ascontiguousarray(amap, np.int32).reshape(-1)
+    _fn(prec, "diff_forward")(C.byref(scene.view()), C.byref(c), C.c_uint32(rays), C.byref(p), _p(alb), _p(mp),
+                              _p(ls), _p(E2), _p(E1), _p(A1))
+    return E2.reshape(P, 36, 3), E1.reshape(P, 36, 3), A1
+
+
+def diff_adjoint(scene, grid_min, grid_size, res, rays, p, albedo, amap, light_scale, A1, gE2, prec="64"):
+    """Gradients (d/d albedo (n, 3), d/d light scale (L,)) of <gE2, E2>."""
+    dt = _dt(prec)
+    c = cascade_desc(grid_min, grid_size, res)
+    alb = np.ascontiguousarray(albedo, dt).reshape(-1)
+    ls = np.ascontiguousarray(light_scale, dt)
+    mp = np.ascontiguousarray(amap, np.int32).reshape(-1)
+    g = np.ascontiguousarray(gE2, dt).reshape(-1)
+    ga = np.zeros_like(alb)
+    gl = np.zeros(len(ls), dt)
+    _fn(prec, "diff_adjoint")(C.byref(scene.view()), C.byref(c), C.c_uint32(rays), C.byref(p), _p(alb), _p(mp),
+                              _p(ls), _p(np.ascontiguousarray(A1, dt)), _p(g), _p(ga), _p(gl))
+    return ga.reshape(-1, 3), gl

@@ -424,6 +424,17 @@ void P(cascade_derived)(const sbgi_cascade_desc *c, float *max_dist, float *cell
     *cell_min = (float)(m > 1e-4 ? m : 1e-4);
 }
 
+/* Differentiable mode (test infrastructure for the adjoint): per-instance-triangle albedo
+ * replacing the material colour, albedo[(map[2 inst] + tri - map[2 inst + 1]) * 3 + c],
+ * and a scale per light on its colour x intensity. Set only inside diff_forward /
+ * diff_adjoint. */
+typedef struct {
+    const REAL *albedo;
+    const int32_t *map;
+    const REAL *light_scale;
+} P(diff_ctx);
+static P(diff_ctx) P(g_diff) = {NULL, NULL, NULL};
+
 /* software bilinear, texel-centre convention, clamp-to-edge, of an RGBA atlas */
 static void P(bilinear)(const REAL *atlas, uint32_t w, uint32_t h, REAL u, REAL v, REAL out[3]) {
     REAL x = u * (REAL)w - RCONST(0.5);
@@ -529,6 +540,8 @@ void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t
         REAL metal = P(mat_f)(s, m, 4), transm = P(mat_f)(s, m, 5);
         for (int a = 0; a < 3; a++) {
             base[a] = P(mat_f)(s, m, a);
+            if (P(g_diff).albedo)
+                base[a] = P(g_diff).albedo[(size_t)(P(g_diff).map[2 * inst] + (int32_t)tri - P(g_diff).map[2 * inst + 1]) * 3 + a];
             rad[a] = (REAL)P(mat_f)(s, m, 7 + a) * (REAL)p->dbg_emissive;
             REAL v = base[a] * (REAL)p->dbg_albedo * (RCONST(1) - metal) * (RCONST(1) - transm);
             kd[a] = v < 0 ? 0 : (v > RCONST(0.95) ? RCONST(0.95) : v);
@@ -540,6 +553,7 @@ void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t
             if (!(ltype < 2.5f)) continue;
             int is_ir = fabsf(L[16] - 4.0f) < 0.25f;
             REAL gate = is_ir ? (REAL)p->nir_gate * (REAL)p->nir_gain : RCONST(1);
+            if (P(g_diff).light_scale) gate *= P(g_diff).light_scale[li];
             REAL lcol[3] = {(REAL)L[7] * gate, (REAL)L[8] * gate, (REAL)L[9] * gate};
             REAL lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
             int is_dir = ltype < 0.5f;
@@ -866,4 +880,329 @@ void P(update)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t ra
     P(trace)(s, c, rays, p, atlas, meta, ray_data, hit_ids, cnt);
     P(blend)(c, rays, depth_oct, p, ray_data, irr, depth);
     P(upload)(c, depth_oct, p, irr, depth, atlas, datlas);
+}
+
+/* ---- differentiable mode: two first-updates from cleared state (hysteresis-free), the
+ * second reading the first's unrounded, unfiltered atlas as its bounce; irradiance of
+ * the second is the output. No reference counterpart (the reference is not
+ * differentiable): this is the oracle of the build's adjoint, pinned by finite
+ * differences in tests/test_oracle.py. ---- */
+
+/* irradiance tiles -> unrounded RGBA atlas with mirrored gutters (upload with filter 0) */
+static void P(diff_atlas)(const sbgi_cascade_desc *c, const REAL *irr, REAL *atlas) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2], aw = c->res[0] * 8u;
+    for (uint32_t probe = 0; probe < total; probe++) {
+        uint32_t col = probe % c->res[0];
+        uint32_t row = (probe / (c->res[0] * c->res[1])) * c->res[1] + (probe / c->res[0]) % c->res[1];
+        for (uint32_t q = 0; q < 64; q++) {
+            uint32_t lx = q % 8, ly = q / 8, sx, sy;
+            P(gutter_src)(lx, ly, 8, &sx, &sy);
+            const REAL *e = irr + ((size_t)probe * 64 + sy * 8 + sx) * 4;
+            REAL *o = atlas + ((size_t)(row * 8 + ly) * aw + col * 8 + lx) * 4;
+            o[0] = e[0];
+            o[1] = e[1];
+            o[2] = e[2];
+            o[3] = 1;
+        }
+    }
+}
+
+static void P(diff_interior)(uint32_t total, const REAL *irr, REAL *out) {
+    for (uint32_t probe = 0; probe < total; probe++)
+        for (uint32_t t = 0; t < 36; t++)
+            for (int a = 0; a < 3; a++)
+                out[((size_t)probe * 36 + t) * 3 + a] = irr[((size_t)probe * 64 + (t / 6 + 1) * 8 + (t % 6 + 1)) * 4 + a];
+}
+
+/* E2 (P x 36 x 3) of the two-update differentiable solve; E1 and the bounce atlas A1
+ * (aw x ah x 4) of the first update are returned for the adjoint. p->updated_count must
+ * be 0 (the whole field). */
+void P(diff_forward)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays, const sbgi_tick_params *p,
+                     const REAL *albedo, const int32_t *map, const REAL *light_scale, REAL *E2, REAL *E1, REAL *A1) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t aw = c->res[0] * 8u, ah = c->res[1] * c->res[2] * 8u;
+    P(g_diff).albedo = albedo;
+    P(g_diff).map = map;
+    P(g_diff).light_scale = light_scale;
+    REAL *irr = (REAL *)calloc((size_t)total * 64 * 4, sizeof(REAL));
+    REAL *depth = (REAL *)calloc((size_t)total * 64 * 2, sizeof(REAL));
+    REAL *ray_data = (REAL *)malloc(sizeof(REAL) * (size_t)total * rays * 4);
+    float *meta = (float *)calloc((size_t)total * 4, sizeof(float));
+    P(trace)(s, c, rays, p, NULL, meta, ray_data, NULL, NULL);
+    P(blend)(c, rays, 6, p, ray_data, irr, depth);
+    P(diff_interior)(total, irr, E1);
+    memset(A1, 0, sizeof(REAL) * (size_t)aw * ah * 4);
+    P(diff_atlas)(c, irr, A1);
+    memset(irr, 0, sizeof(REAL) * (size_t)total * 64 * 4);
+    memset(depth, 0, sizeof(REAL) * (size_t)total * 64 * 2);
+    P(trace)(s, c, rays, p, A1, meta, ray_data, NULL, NULL);
+    P(blend)(c, rays, 6, p, ray_data, irr, depth);
+    P(diff_interior)(total, irr, E2);
+    free(irr);
+    free(depth);
+    free(ray_data);
+    free(meta);
+    P(g_diff).albedo = NULL;
+    P(g_diff).map = NULL;
+    P(g_diff).light_scale = NULL;
+}
+
+/* The 32 (atlas texel, weight) pairs of sample_atlas at (pos, nrm): weight = probe weight /
+ * wsum x bilinear weight, so sample = sum weight x atlas[texel]. */
+static void P(sample_weights)(const sbgi_cascade_desc *c, const REAL pos[3], const REAL nrm[3], size_t idx[32],
+                              REAL wt[32]) {
+    uint32_t aw = c->res[0] * 8u, ah = c->res[1] * c->res[2] * 8u;
+    REAL cell[3], gf[3], base[3], fr[3];
+    for (int a = 0; a < 3; a++) {
+        REAL den = (REAL)c->res[a] - RCONST(1);
+        if (den < 1) den = 1;
+        cell[a] = (REAL)c->grid_size[a] / den;
+        REAL cm = cell[a] > RCONST(1e-6) ? cell[a] : RCONST(1e-6);
+        gf[a] = (pos[a] - (REAL)c->grid_min[a]) / cm;
+        REAL hi = (REAL)c->res[a] - RCONST(2);
+        if (hi < 0) hi = 0;
+        REAL b = RFLOOR(gf[a]);
+        b = b < 0 ? 0 : (b > hi ? hi : b);
+        base[a] = b;
+        REAL f = gf[a] - b;
+        fr[a] = f < 0 ? 0 : (f > 1 ? 1 : f);
+    }
+    REAL nn[3] = {nrm[0], nrm[1], nrm[2]}, uv[2];
+    P(normalize3)(nn);
+    P(oct_encode)(nn, uv);
+    REAL wp[8], wsum = 0;
+    for (int i = 0; i < 8; i++) {
+        int dx = i & 1, dy = (i >> 1) & 1, dz = (i >> 2) & 1;
+        REAL wx = dx ? fr[0] : RCONST(1) - fr[0];
+        REAL wy = dy ? fr[1] : RCONST(1) - fr[1];
+        REAL wz = dz ? fr[2] : RCONST(1) - fr[2];
+        wp[i] = wx * wy * wz + RCONST(1e-4);
+        wsum += wp[i];
+    }
+    REAL den = wsum > RCONST(1e-4) ? wsum : RCONST(1e-4);
+    for (int i = 0; i < 8; i++) {
+        int dx = i & 1, dy = (i >> 1) & 1, dz = (i >> 2) & 1;
+        REAL col = base[0] + dx, row = (base[2] + dz) * (REAL)c->res[1] + base[1] + dy;
+        REAL ox = col * RCONST(8) + RCONST(1) + uv[0] * RCONST(6);
+        REAL oy = row * RCONST(8) + RCONST(1) + uv[1] * RCONST(6);
+        REAL x = (ox / (REAL)aw) * (REAL)aw - RCONST(0.5), y = (oy / (REAL)ah) * (REAL)ah - RCONST(0.5);
+        REAL x0f = RFLOOR(x), y0f = RFLOOR(y), fx = x - x0f, fy = y - y0f;
+        int32_t x0 = (int32_t)x0f, y0 = (int32_t)y0f, x1 = x0 + 1, y1 = y0 + 1;
+#define CLAMPI(a, hi) ((a) < 0 ? 0 : ((a) > (int32_t)(hi) - 1 ? (int32_t)(hi) - 1 : (a)))
+        x0 = CLAMPI(x0, aw);
+        x1 = CLAMPI(x1, aw);
+        y0 = CLAMPI(y0, ah);
+        y1 = CLAMPI(y1, ah);
+#undef CLAMPI
+        REAL pw = wp[i] / den;
+        idx[i * 4 + 0] = (size_t)y0 * aw + x0;
+        idx[i * 4 + 1] = (size_t)y0 * aw + x1;
+        idx[i * 4 + 2] = (size_t)y1 * aw + x0;
+        idx[i * 4 + 3] = (size_t)y1 * aw + x1;
+        wt[i * 4 + 0] = pw * (RCONST(1) - fx) * (RCONST(1) - fy);
+        wt[i * 4 + 1] = pw * fx * (RCONST(1) - fy);
+        wt[i * 4 + 2] = pw * (RCONST(1) - fx) * fy;
+        wt[i * 4 + 3] = pw * fx * fy;
+    }
+}
+
+/* Per ray of the differentiable solve: the hit, kd and d kd / d albedo, the per-light
+ * direct coefficient A (contribution = kd * A * scale, zero for gated or blocked lights)
+ * and the bounce sample weights. Returns 0 on a miss. */
+typedef struct {
+    int32_t tri, inst;
+    REAL kd[3], dkd[3], hit[3], ng[3];
+} P(diff_hit);
+
+static int P(diff_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays, const sbgi_tick_params *p,
+                       uint32_t probe, uint32_t k, P(diff_hit) * h, REAL *A /* light_count x 3 */) {
+    float max_dist, cell_min;
+    P(cascade_derived)(c, &max_dist, &cell_min);
+    REAL ro[3], rd[3];
+    P(probe_pos)(c, probe, NULL, 0.f, ro);
+    P(ray_dir)(k, rays, p->frame_jitter, rd);
+    REAL best_t;
+    P(closest)(s, ro, rd, &best_t, &h->tri, &h->inst, NULL);
+    if (h->tri < 0) return 0;
+    uint32_t tri = (uint32_t)h->tri, inst = (uint32_t)h->inst, m = s->tri_material[tri];
+    REAL p0[3], p1[3], p2[3], e1[3], e2[3], nl[3];
+    P(vert)(s, s->tri_index[tri * 3 + 0], p0);
+    P(vert)(s, s->tri_index[tri * 3 + 1], p1);
+    P(vert)(s, s->tri_index[tri * 3 + 2], p2);
+    for (int a = 0; a < 3; a++) {
+        e1[a] = p1[a] - p0[a];
+        e2[a] = p2[a] - p0[a];
+    }
+    P(cross3)(e1, e2, nl);
+    P(normalize3)(nl);
+    P(inst_normal)(s, inst, nl, h->ng);
+    REAL ff = P(dot3)(h->ng, rd) < 0 ? RCONST(1) : RCONST(-1);
+    for (int a = 0; a < 3; a++) h->ng[a] *= ff;
+    REAL b1 = (REAL)cell_min * RCONST(0.005) * (REAL)p->dbg_trace_bias, b2 = RCONST(RAY_EPS) * (REAL)p->dbg_ray_eps;
+    REAL bias = b1 > b2 ? b1 : b2;
+    for (int a = 0; a < 3; a++) h->hit[a] = (ro[a] + rd[a] * best_t) + h->ng[a] * bias;
+    REAL metal = P(mat_f)(s, m, 4), transm = P(mat_f)(s, m, 5);
+    REAL fac = (REAL)p->dbg_albedo * (RCONST(1) - metal) * (RCONST(1) - transm);
+    for (int a = 0; a < 3; a++) {
+        REAL base = P(g_diff).albedo[(size_t)(P(g_diff).map[2 * inst] + (int32_t)tri - P(g_diff).map[2 * inst + 1]) * 3 + a];
+        REAL v = base * fac;
+        h->kd[a] = v < 0 ? 0 : (v > RCONST(0.95) ? RCONST(0.95) : v);
+        h->dkd[a] = (v > 0 && v < RCONST(0.95)) ? fac : 0;
+    }
+    for (uint32_t li = 0; li < s->light_count; li++) {
+        const float *L = s->lights + li * 17u;
+        REAL *Al = A + li * 3;
+        Al[0] = Al[1] = Al[2] = 0;
+        float ltype = L[0];
+        if (!(ltype < 2.5f)) continue;
+        int is_ir = fabsf(L[16] - 4.0f) < 0.25f;
+        REAL gate = is_ir ? (REAL)p->nir_gate * (REAL)p->nir_gain : RCONST(1);
+        REAL scale = P(g_diff).light_scale ? P(g_diff).light_scale[li] : RCONST(1);
+        REAL lcol[3] = {(REAL)L[7] * gate, (REAL)L[8] * gate, (REAL)L[9] * gate};
+        REAL lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
+        int is_dir = ltype < 0.5f, is_spot = fabsf(ltype - 2.0f) < 0.5f;
+        REAL ldir[3] = {L[4], L[5], L[6]}, tl[3];
+        for (int a = 0; a < 3; a++) tl[a] = is_dir ? -ldir[a] : (REAL)L[1 + a] - h->hit[a];
+        REAL len = RSQRT(P(dot3)(tl, tl));
+        REAL dist = is_dir ? RCONST(1e4) : (len > RCONST(1e-4) ? len : RCONST(1e-4));
+        REAL wi[3] = {tl[0], tl[1], tl[2]};
+        P(normalize3)(wi);
+        REAL ndl = P(dot3)(h->ng, wi);
+        ndl = ndl > 0 ? ndl : 0;
+        REAL lpeak = P(fmax)(P(fmax)(lcol[0], lcol[1]), lcol[2]) * scale;
+        if (!(ndl * lpeak > 0 && (is_dir || lrange <= 0 || dist < lrange))) continue;
+        if (P(any)(s, h->hit, wi, dist - bias, NULL)) continue;
+        REAL pw = RPOW(dist, ldecay);
+        REAL falloff = RCONST(1) / (pw > RCONST(0.01) ? pw : RCONST(0.01));
+        REAL rr = dist / (lrange > RCONST(1e-4) ? lrange : RCONST(1e-4));
+        REAL rr2 = rr * rr, win = RCONST(1) - rr2 * rr2;
+        win = win < 0 ? 0 : (win > 1 ? 1 : win);
+        REAL dist_att = is_dir ? RCONST(1) : (lrange > 0 ? falloff * (win * win) : falloff);
+        REAL acos_ = P(dot3)(wi, ldir) * RCONST(-1);
+        REAL den = lcp - lca;
+        den = den > RCONST(1e-4) ? den : RCONST(1e-4);
+        REAL spot = (acos_ - lca) / den;
+        spot = spot < 0 ? 0 : (spot > 1 ? 1 : spot);
+        REAL att = dist_att * (is_spot ? spot : RCONST(1));
+        for (int a = 0; a < 3; a++) Al[a] = (RCONST(1) / RCONST(PI_D)) * ndl * lcol[a] * att * (REAL)p->dbg_direct;
+    }
+    return 1;
+}
+
+/* Adjoint of diff_forward: given dLoss/dE2 (P x 36 x 3), the gradients with respect to
+ * the albedo entries (n_albedo x 3, accumulated) and the light scales (accumulated). */
+void P(diff_adjoint)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays, const sbgi_tick_params *p,
+                     const REAL *albedo, const int32_t *map, const REAL *light_scale, const REAL *A1,
+                     const REAL *gE2, REAL *g_albedo, REAL *g_light) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t aw = c->res[0] * 8u, ah = c->res[1] * c->res[2] * 8u;
+    uint32_t L = s->light_count;
+    P(g_diff).albedo = albedo;
+    P(g_diff).map = map;
+    P(g_diff).light_scale = light_scale;
+    /* blend weights w[k][t] / max(sum_k w, 1e-4), as P(blend) forms them */
+    REAL *W = (REAL *)malloc(sizeof(REAL) * rays * 36);
+    REAL cpow = (REAL)(p->dbg_cosine_power > 1e-4f ? p->dbg_cosine_power : 1e-4f);
+    for (uint32_t t = 0; t < 36; t++) {
+        REAL dir[3], ws = 0;
+        P(texel_dir)(t % 6, t / 6, 6, dir);
+        for (uint32_t k = 0; k < rays; k++) {
+            REAL d[3];
+            P(ray_dir)(k, rays, p->frame_jitter, d);
+            REAL cd = P(dot3)(dir, d);
+            cd = cd > 0 ? cd : 0;
+            W[k * 36 + t] = RPOW(cd, cpow);
+            ws += W[k * 36 + t];
+        }
+        ws = ws > RCONST(1e-4) ? ws : RCONST(1e-4);
+        for (uint32_t k = 0; k < rays; k++) W[k * 36 + t] /= ws;
+    }
+    REAL *gL2 = (REAL *)calloc((size_t)total * rays * 3, sizeof(REAL));
+    REAL *gL1 = (REAL *)calloc((size_t)total * rays * 3, sizeof(REAL));
+    REAL *gA1 = (REAL *)calloc((size_t)aw * ah * 3, sizeof(REAL));
+    REAL *gE1 = (REAL *)calloc((size_t)total * 36 * 3, sizeof(REAL));
+    REAL *A = (REAL *)malloc(sizeof(REAL) * (L ? L : 1) * 3);
+    const REAL LUMA[3] = {RCONST(0.2126), RCONST(0.7152), RCONST(0.0722)};
+    for (uint32_t pr = 0; pr < total; pr++)
+        for (uint32_t k = 0; k < rays; k++)
+            for (uint32_t t = 0; t < 36; t++)
+                for (int a = 0; a < 3; a++)
+                    gL2[((size_t)pr * rays + k) * 3 + a] += W[k * 36 + t] * gE2[((size_t)pr * 36 + t) * 3 + a];
+    /* bounce term of the second update */
+    for (uint32_t pr = 0; pr < total; pr++) {
+        for (uint32_t k = 0; k < rays; k++) {
+            P(diff_hit) h;
+            if (!P(diff_ray)(s, c, rays, p, pr, k, &h, A)) continue;
+            size_t idx[32];
+            REAL wt[32], B[3] = {0, 0, 0};
+            P(sample_weights)(c, h.hit, h.ng, idx, wt);
+            for (int j = 0; j < 32; j++)
+                for (int a = 0; a < 3; a++) B[a] += wt[j] * A1[idx[j] * 4 + a];
+            REAL u[3], lum = 0;
+            for (int a = 0; a < 3; a++) {
+                u[a] = B[a] * h.kd[a] / RCONST(PI_D);
+                lum += u[a] * LUMA[a];
+            }
+            REAL rc = p->radiance_clamp, mx = rc > lum ? rc : lum;
+            mx = mx > RCONST(1e-6) ? mx : RCONST(1e-6);
+            REAL roll = rc / mx;
+            const REAL *g = gL2 + ((size_t)pr * rays + k) * 3;
+            REAL gob[3], gu[3], gdot = 0;
+            for (int a = 0; a < 3; a++) {
+                gob[a] = g[a] * (REAL)p->dbg_bounce;
+                gdot += gob[a] * u[a];
+            }
+            int clamped = lum > rc && lum > RCONST(1e-6);
+            for (int a = 0; a < 3; a++) gu[a] = roll * gob[a] - (clamped ? rc / (lum * lum) * gdot * LUMA[a] : 0);
+            size_t ai = (size_t)(map[2 * h.inst] + h.tri - map[2 * h.inst + 1]) * 3;
+            for (int a = 0; a < 3; a++) {
+                g_albedo[ai + a] += gu[a] * B[a] / RCONST(PI_D) * h.dkd[a];
+                REAL gB = gu[a] * h.kd[a] / RCONST(PI_D);
+                for (int j = 0; j < 32; j++) gA1[idx[j] * 3 + a] += wt[j] * gB;
+            }
+        }
+    }
+    /* atlas -> first-update irradiance (gutter copies fold into their interior source) */
+    for (uint32_t probe = 0; probe < total; probe++) {
+        uint32_t col = probe % c->res[0];
+        uint32_t row = (probe / (c->res[0] * c->res[1])) * c->res[1] + (probe / c->res[0]) % c->res[1];
+        for (uint32_t q = 0; q < 64; q++) {
+            uint32_t lx = q % 8, ly = q / 8, sx, sy;
+            P(gutter_src)(lx, ly, 8, &sx, &sy);
+            uint32_t t = (sy - 1) * 6 + (sx - 1);
+            size_t o = (size_t)(row * 8 + ly) * aw + col * 8 + lx;
+            for (int a = 0; a < 3; a++) gE1[((size_t)probe * 36 + t) * 3 + a] += gA1[o * 3 + a];
+        }
+    }
+    for (uint32_t pr = 0; pr < total; pr++)
+        for (uint32_t k = 0; k < rays; k++)
+            for (uint32_t t = 0; t < 36; t++)
+                for (int a = 0; a < 3; a++)
+                    gL1[((size_t)pr * rays + k) * 3 + a] += W[k * 36 + t] * gE1[((size_t)pr * 36 + t) * 3 + a];
+    /* direct light of both updates (identical in both) */
+    for (uint32_t pr = 0; pr < total; pr++) {
+        for (uint32_t k = 0; k < rays; k++) {
+            P(diff_hit) h;
+            if (!P(diff_ray)(s, c, rays, p, pr, k, &h, A)) continue;
+            size_t o = ((size_t)pr * rays + k) * 3;
+            size_t ai = (size_t)(map[2 * h.inst] + h.tri - map[2 * h.inst + 1]) * 3;
+            for (uint32_t li = 0; li < L; li++) {
+                REAL sc = light_scale ? light_scale[li] : RCONST(1);
+                for (int a = 0; a < 3; a++) {
+                    REAL g = gL1[o + a] + gL2[o + a];
+                    g_albedo[ai + a] += g * A[li * 3 + a] * sc * h.dkd[a];
+                    g_light[li] += g * h.kd[a] * A[li * 3 + a];
+                }
+            }
+        }
+    }
+    free(W);
+    free(gL2);
+    free(gL1);
+    free(gA1);
+    free(gE1);
+    free(A);
+    P(g_diff).albedo = NULL;
+    P(g_diff).map = NULL;
+    P(g_diff).light_scale = NULL;
 }

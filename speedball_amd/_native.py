@@ -35,7 +35,8 @@ EXPORTS = [
     "sbgi_write", "sbgi_buffer_bytes", "sbgi_query_rays", "sbgi_get_stats", "sbgi_set_counting", "sbgi_last_error",
     "sbgi_destroy", "sbgi_sb_create", "sbgi_sb_add_material", "sbgi_sb_add_mesh", "sbgi_sb_add_instance",
     "sbgi_sb_add_light", "sbgi_sb_finalize", "sbgi_sb_tlas_order", "sbgi_sb_set_transform",
-    "sbgi_sb_last_error", "sbgi_sb_destroy",
+    "sbgi_sb_last_error", "sbgi_sb_destroy", "sbgi_diff_param_count", "sbgi_diff_albedo_map", "sbgi_diff_forward",
+    "sbgi_diff_adjoint", "sbgi_diff_timing",
 ]
 
 
@@ -133,6 +134,11 @@ def lib() -> C.CDLL:
         "sbgi_get_stats": (i32, [vp, C.POINTER(Stats)]),
         "sbgi_set_counting": (i32, [vp, i32]),
         "sbgi_query_rays": (i32, [vp, vp, u32, vp]),
+        "sbgi_diff_param_count": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
+        "sbgi_diff_albedo_map": (i32, [vp, vp, u32]),
+        "sbgi_diff_forward": (i32, [vp, C.POINTER(TickParams), vp, vp, vp]),
+        "sbgi_diff_adjoint": (i32, [vp, vp, vp, vp]),
+        "sbgi_diff_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "sbgi_last_error": (C.c_char_p, [vp]),
         "sbgi_destroy": (i32, [vp]),
         "sbgi_sb_create": (i32, [C.POINTER(vp)]),

@@ -75,6 +75,19 @@ struct sbgi_field {
     hipEvent_t ring[kTicketRing] = {};
     hipEvent_t ev[5] = {};
     hipEvent_t tev[4] = {};  // trace sub-stages
+    // differentiable mode (sbgi_diff_*): parameters, the first update's FP32 atlas, the
+    // gradients and the adjoint's scratch; the frozen hits and visibility masks are the
+    // trace scratch tw of the last diff forward (diff_valid until any other solve)
+    float *d_albedo = nullptr, *d_light_scale = nullptr, *d_gE = nullptr, *d_galb = nullptr, *d_glight = nullptr;
+    int2 *d_albedo_map = nullptr;
+    float4 *d_atlas1 = nullptr;
+    void *d_adj_scratch = nullptr;
+    uint32_t diff_albedo_n = 0;
+    std::vector<int32_t> h_albedo_map;
+    bool diff_ready = false, diff_valid = false;
+    sbgi_tick_params diff_p{};
+    hipEvent_t dev[4] = {};
+    float ms_diff_fwd = 0.f, ms_diff_adj = 0.f;
     uint64_t ticket = 0;
     bool stats_valid = false;
     std::string err;
@@ -132,6 +145,15 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_walk_nodes);
     dfree(f->d_walk_perm);
     dfree(f->d_tlas_sah);
+    dfree(f->d_albedo);
+    dfree(f->d_light_scale);
+    dfree(f->d_gE);
+    dfree(f->d_galb);
+    dfree(f->d_glight);
+    dfree(f->d_albedo_map);
+    dfree(f->d_atlas1);
+    dfree(f->d_adj_scratch);
+    f->diff_ready = f->diff_valid = false;
     f->tlas_sah_count = 0;
     f->tw.queue.capacity = 0;
     f->has_scene = false;
@@ -436,6 +458,8 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     for (auto &e : f->tev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
+    for (auto &e : f->dev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     const uint32_t tile_d = desc->depth_oct_res + 2;
     for (uint32_t ci = 0; ci < desc->cascade_count; ci++) {
         CascadeState &cs = f->casc[ci];
@@ -729,6 +753,7 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     const DevCascade c = make_cascade(f, cs, p->probe_offset, updated);
     const DevScene s = make_scene(f);
     hipStream_t st = f->stream;
+    f->diff_valid = false;
     HIP_OK(f, hipEventRecord(f->ev[0], st));
     if (p->needs_clear) {
         HIP_OK(f, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
@@ -737,12 +762,12 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
     HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
     HIP_OK(f, hipEventRecord(f->ev[1], st));
-    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, f->tw, f->rays, f->hit_ids,
+    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
                            f->count_mode ? f->counters : nullptr, f->tev, st));
     HIP_OK(f, hipEventRecord(f->ev[2], st));
     HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, st));
     HIP_OK(f, hipEventRecord(f->ev[3], st));
-    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, st));
+    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st));
     HIP_OK(f, hipEventRecord(f->ev[4], st));
     f->ticket++;
     HIP_OK(f, hipEventRecord(f->ring[f->ticket % kTicketRing], st));
@@ -940,6 +965,8 @@ int32_t sbgi_destroy(sbgi_field *f) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : f->tev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : f->dev)
+        if (e) (void)hipEventDestroy(e);
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
     return SBGI_OK;
@@ -972,6 +999,178 @@ int32_t sbgi_set_counting(sbgi_field *f, int32_t on) {
     f->count_mode = on != 0;
     HIP_OK(f, hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream));
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+
+}  // extern "C"
+
+// ---- differentiable mode ---------------------------------------------------------
+
+namespace {
+
+// Per TLAS slot: (offset into the albedo array, first pool triangle of its BLAS); entries
+// are slot-major, then pool-triangle order within the BLAS.
+int32_t diff_prepare(sbgi_field *f) {
+    if (f->diff_ready) return SBGI_OK;
+    const sbgi_scene_view &v = f->view;
+    f->h_albedo_map.assign((size_t)v.inst_count * 2, 0);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < v.inst_count; i++) {
+        const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
+        const uint32_t root = (uint32_t)r[12], end = (uint32_t)r[13];
+        uint32_t first = 0xFFFFFFFFu;
+        uint64_t count = 0;
+        for (uint32_t nd = root; nd < end; nd++) {
+            const uint32_t pay = f->h_nodes[(size_t)nd * 8 + 7];
+            if (pay == 0xFFFFFFFFu) continue;
+            first = std::min(first, pay & 0x00FFFFFFu);
+            count += pay >> 24;
+        }
+        if (first == 0xFFFFFFFFu) first = 0;
+        f->h_albedo_map[2 * i] = (int32_t)off;
+        f->h_albedo_map[2 * i + 1] = (int32_t)first;
+        off += count;
+    }
+    if (off >= (1ull << 28)) return fail(f, SBGI_E_CAPACITY, "too many albedo entries");
+    f->diff_albedo_n = (uint32_t)off;
+    const CascadeState &cs = f->casc[0];
+    const uint32_t n = cs.total * f->desc.rays_per_probe;
+    const uint32_t cells = (std::max(cs.d.res[0], 2u) - 1u) * (std::max(cs.d.res[1], 2u) - 1u) *
+                           (std::max(cs.d.res[2], 2u) - 1u);
+    const uint32_t L = std::max(v.light_count, 1u);
+    if (dalloc(&f->d_albedo, (size_t)std::max(f->diff_albedo_n, 1u) * 12) != hipSuccess ||
+        dalloc(&f->d_galb, (size_t)std::max(f->diff_albedo_n, 1u) * 12) != hipSuccess ||
+        dalloc(&f->d_light_scale, (size_t)L * 4) != hipSuccess || dalloc(&f->d_glight, (size_t)L * 4) != hipSuccess ||
+        dalloc(&f->d_albedo_map, (size_t)std::max(v.inst_count, 1u) * 8) != hipSuccess ||
+        dalloc(&f->d_atlas1, (size_t)cs.aw * cs.ah * 16) != hipSuccess ||
+        dalloc(&f->d_gE, (size_t)cs.total * 108 * 4 * 2) != hipSuccess ||
+        dalloc(&f->d_adj_scratch, adjoint_scratch_bytes(n, cells, v.light_count)) != hipSuccess)
+        return fail(f, SBGI_E_NOMEM, "differentiable-mode allocation");
+    HIP_OK(f, hipMemcpyAsync(f->d_albedo_map, f->h_albedo_map.data(), f->h_albedo_map.size() * 4,
+                             hipMemcpyHostToDevice, f->stream));
+    f->diff_ready = true;
+    return SBGI_OK;
+}
+
+DevScene diff_scene(const sbgi_field *f) {
+    DevScene s = make_scene(f);
+    s.albedo = f->d_albedo;
+    s.albedo_map = f->d_albedo_map;
+    s.light_scale = f->d_light_scale;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t sbgi_diff_param_count(sbgi_field *f, uint32_t *albedo_entries, uint32_t *lights) {
+    if (!f || !albedo_entries || !lights) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    int32_t rc = diff_prepare(f);
+    if (rc != SBGI_OK) return rc;
+    *albedo_entries = f->diff_albedo_n;
+    *lights = f->view.light_count;
+    return SBGI_OK;
+}
+
+int32_t sbgi_diff_albedo_map(sbgi_field *f, int32_t *map2, uint32_t n_slots) {
+    if (!f || !map2) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    int32_t rc = diff_prepare(f);
+    if (rc != SBGI_OK) return rc;
+    if (n_slots != f->view.inst_count) return fail(f, SBGI_E_INVALID, "map size must be the instance count");
+    memcpy(map2, f->h_albedo_map.data(), f->h_albedo_map.size() * 4);
+    return SBGI_OK;
+}
+
+int32_t sbgi_diff_forward(sbgi_field *f, const sbgi_tick_params *p, const float *albedo, const float *light_scale,
+                          float *irr_out) {
+    if (!f || !p || !albedo || !irr_out) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    if (f->desc.depth_oct_res != 6) return fail(f, SBGI_E_INVALID, "the differentiable mode uses the 8x8 depth tile");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    int32_t rc = diff_prepare(f);
+    if (rc != SBGI_OK) return rc;
+    CascadeState &cs = f->casc[0];
+    hipStream_t st = f->stream;
+    sbgi_tick_params q = *p;
+    q.cascade = 0;
+    q.probe_offset = 0;
+    q.updated_count = 0;
+    q.needs_clear = 0;
+    q.classify_strength = 0.f;
+    q.filter_strength = 0.f;
+    const DevCascade c = make_cascade(f, cs, 0, cs.total);
+    const DevScene s = diff_scene(f);
+    const uint32_t L = f->view.light_count;
+    std::vector<float> ones(std::max(L, 1u), 1.f);
+    HIP_OK(f, hipMemcpyAsync(f->d_albedo, albedo, (size_t)f->diff_albedo_n * 12, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipMemcpyAsync(f->d_light_scale, light_scale ? light_scale : ones.data(), (size_t)std::max(L, 1u) * 4,
+                             hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipEventRecord(f->dev[0], st));
+    HIP_OK(f, launch_raydirs(f->dirs, c.rays, q.frame_jitter, st));
+    HIP_OK(f, launch_weights(c, q, f->dirs, f->bt, SBGI_BLEND_FP32, st));
+    const size_t irr_bytes = (size_t)cs.total * 64 * 16, dep_bytes = (size_t)cs.total * 64 * 8;
+    // update 1: cleared state, zero bounce source; its upload writes the FP32 atlas A1
+    HIP_OK(f, hipMemsetAsync(cs.irr, 0, irr_bytes, st));
+    HIP_OK(f, hipMemsetAsync(cs.depth, 0, dep_bytes, st));
+    HIP_OK(f, hipMemsetAsync(f->d_atlas1, 0, (size_t)cs.aw * cs.ah * 16, st));
+    HIP_OK(f, launch_trace(s, c, q, f->dirs, cs.meta, cs.atlas, f->d_atlas1, f->tw, f->rays, nullptr, nullptr, nullptr,
+                           st));
+    HIP_OK(f, launch_blend(c, q, f->rays, f->bt, SBGI_BLEND_FP32, cs.irr, cs.depth, st));
+    HIP_OK(f, launch_upload(c, q, cs.irr, cs.depth, cs.atlas, cs.datlas, f->d_atlas1, st));
+    // update 2: cleared state again, bouncing off A1
+    HIP_OK(f, hipMemsetAsync(cs.irr, 0, irr_bytes, st));
+    HIP_OK(f, hipMemsetAsync(cs.depth, 0, dep_bytes, st));
+    HIP_OK(f, launch_trace(s, c, q, f->dirs, cs.meta, cs.atlas, f->d_atlas1, f->tw, f->rays, nullptr, nullptr, nullptr,
+                           st));
+    HIP_OK(f, launch_blend(c, q, f->rays, f->bt, SBGI_BLEND_FP32, cs.irr, cs.depth, st));
+    HIP_OK(f, hipEventRecord(f->dev[1], st));
+    std::vector<float> irr((size_t)cs.total * 64 * 4);
+    HIP_OK(f, hipMemcpyAsync(irr.data(), cs.irr, irr_bytes, hipMemcpyDeviceToHost, st));
+    HIP_OK(f, hipStreamSynchronize(st));
+    for (uint32_t pr = 0; pr < cs.total; pr++)
+        for (uint32_t t = 0; t < 36; t++)
+            for (int a = 0; a < 3; a++)
+                irr_out[((size_t)pr * 36 + t) * 3 + a] = irr[((size_t)pr * 64 + (t / 6 + 1) * 8 + (t % 6 + 1)) * 4 + a];
+    HIP_OK(f, hipEventElapsedTime(&f->ms_diff_fwd, f->dev[0], f->dev[1]));
+    f->diff_p = q;
+    f->diff_valid = true;
+    return SBGI_OK;
+}
+
+int32_t sbgi_diff_adjoint(sbgi_field *f, const float *grad_irr, float *grad_albedo, float *grad_light_scale) {
+    if (!f || !grad_irr || !grad_albedo) return SBGI_E_INVALID;
+    if (!f->diff_valid) return fail(f, SBGI_E_STATE, "no differentiable forward to take the adjoint of");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    CascadeState &cs = f->casc[0];
+    hipStream_t st = f->stream;
+    const DevCascade c = make_cascade(f, cs, 0, cs.total);
+    const DevScene s = diff_scene(f);
+    const uint32_t L = f->view.light_count;
+    float *gE2 = f->d_gE, *gE1 = f->d_gE + (size_t)cs.total * 108;
+    HIP_OK(f, hipMemcpyAsync(gE2, grad_irr, (size_t)cs.total * 108 * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipEventRecord(f->dev[2], st));
+    HIP_OK(f, hipMemsetAsync(f->d_galb, 0, (size_t)f->diff_albedo_n * 12, st));
+    HIP_OK(f, hipMemsetAsync(f->d_glight, 0, (size_t)std::max(L, 1u) * 4, st));
+    HIP_OK(f, launch_adjoint(s, c, f->diff_p, f->dirs, f->bt, f->tw, f->d_atlas1, gE2, gE1, f->d_galb, f->d_glight,
+                             f->d_adj_scratch, st));
+    HIP_OK(f, hipEventRecord(f->dev[3], st));
+    HIP_OK(f, hipMemcpyAsync(grad_albedo, f->d_galb, (size_t)f->diff_albedo_n * 12, hipMemcpyDeviceToHost, st));
+    if (grad_light_scale && L)
+        HIP_OK(f, hipMemcpyAsync(grad_light_scale, f->d_glight, (size_t)L * 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(f, hipStreamSynchronize(st));
+    HIP_OK(f, hipEventElapsedTime(&f->ms_diff_adj, f->dev[2], f->dev[3]));
+    return SBGI_OK;
+}
+
+int32_t sbgi_diff_timing(sbgi_field *f, float *ms_forward, float *ms_adjoint) {
+    if (!f || !ms_forward || !ms_adjoint) return SBGI_E_INVALID;
+    *ms_forward = f->ms_diff_fwd;
+    *ms_adjoint = f->ms_diff_adj;
     return SBGI_OK;
 }
 

@@ -31,170 +31,10 @@
 #pragma clang fp contract(off)
 
 #include "kernels.h"
+#include "shade.h"
 #include "traverse2.h"
 
 namespace {
-
-__device__ __forceinline__ float h2f(unsigned short h) { return __half2float(__ushort_as_half(h)); }
-
-// software bilinear of the previous RGBA16F atlas: texel-centre convention, clamp to edge
-__device__ __forceinline__ float3 bilinear(const ushort4 *__restrict__ atlas, uint32_t w, uint32_t h, float u, float v) {
-    float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
-    float x0f = floorf(x), y0f = floorf(y);
-    float fx = x - x0f, fy = y - y0f;
-    int x0 = (int)x0f, y0 = (int)y0f, x1 = x0 + 1, y1 = y0 + 1;
-    x0 = min(max(x0, 0), (int)w - 1);
-    x1 = min(max(x1, 0), (int)w - 1);
-    y0 = min(max(y0, 0), (int)h - 1);
-    y1 = min(max(y1, 0), (int)h - 1);
-    ushort4 t00 = atlas[(size_t)y0 * w + x0], t10 = atlas[(size_t)y0 * w + x1];
-    ushort4 t01 = atlas[(size_t)y1 * w + x0], t11 = atlas[(size_t)y1 * w + x1];
-    float w00 = (1.f - fx) * (1.f - fy), w10 = fx * (1.f - fy), w01 = (1.f - fx) * fy, w11 = fx * fy;
-    return f3(h2f(t00.x) * w00 + h2f(t10.x) * w10 + h2f(t01.x) * w01 + h2f(t11.x) * w11,
-              h2f(t00.y) * w00 + h2f(t10.y) * w10 + h2f(t01.y) * w01 + h2f(t11.y) * w11,
-              h2f(t00.z) * w00 + h2f(t10.z) * w10 + h2f(t01.z) * w01 + h2f(t11.z) * w11);
-}
-
-__device__ __forceinline__ float2 oct_encode(float3 n) {
-    float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
-    float inv = 1.0f / fmaxf(s, 1e-20f);
-    float ox = n.x * inv, oy = n.y * inv;
-    if (n.z < 0.f) {
-        float fx = (1.f - fabsf(oy)) * (ox >= 0.f ? 1.f : -1.f);
-        float fy = (1.f - fabsf(ox)) * (oy >= 0.f ? 1.f : -1.f);
-        ox = fx;
-        oy = fy;
-    }
-    return make_float2(ox * 0.5f + 0.5f, oy * 0.5f + 0.5f);
-}
-
-// sampleAtlas (js/gi_probes.js:1941-1974)
-__device__ float3 sample_atlas(const DevCascade &c, const ushort4 *__restrict__ atlas, float3 P, float3 N) {
-    float cell[3], gf[3], base[3], fr[3];
-    float pp[3] = {P.x, P.y, P.z};
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        cell[a] = c.gsize[a] / fmaxf(1.f, (float)c.res[a] - 1.f);
-        gf[a] = (pp[a] - c.gmin[a]) / fmaxf(cell[a], 1e-6f);
-        float hi = fmaxf((float)c.res[a] - 2.f, 0.f);
-        base[a] = fminf(fmaxf(floorf(gf[a]), 0.f), hi);
-        fr[a] = fminf(fmaxf(gf[a] - base[a], 0.f), 1.f);
-    }
-    float2 uv = oct_encode(normalize3(N));
-    float3 acc = f3(0.f, 0.f, 0.f);
-    float wsum = 0.f;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        int dx = i & 1, dy = (i >> 1) & 1, dz = (i >> 2) & 1;
-        float px = base[0] + dx, py = base[1] + dy, pz = base[2] + dz;
-        float wx = dx ? fr[0] : 1.f - fr[0];
-        float wy = dy ? fr[1] : 1.f - fr[1];
-        float wz = dz ? fr[2] : 1.f - fr[2];
-        float w = wx * wy * wz + 1e-4f;
-        float col = px, row = pz * (float)c.res[1] + py;
-        float ox = col * 8.f + 1.f + uv.x * 6.f;
-        float oy = row * 8.f + 1.f + uv.y * 6.f;
-        float3 t = bilinear(atlas, c.aw, c.ah, ox / (float)c.aw, oy / (float)c.ah);
-        acc = acc + t * w;
-        wsum += w;
-    }
-    return acc * (1.0f / fmaxf(wsum, 1e-4f));
-}
-
-// ray origin and direction of ray gid (probeTraceOrigin js/gi_probes.js:1714-1726, 1876-1885)
-__device__ __forceinline__ void make_ray(const DevCascade &c, float classify_strength, const float4 *__restrict__ dirs,
-                                         const float *__restrict__ meta, uint32_t gid, float3 &ro, float3 &rd) {
-    const uint32_t k = gid / c.updated;
-    const uint32_t slot = gid - k * c.updated;
-    const uint32_t probe = (c.offset + slot) % c.total;
-    const uint32_t ix = probe % c.res[0], iy = (probe / c.res[0]) % c.res[1], iz = probe / (c.res[0] * c.res[1]);
-    ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
-            c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
-            c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
-    if (classify_strength != 0.f) {
-        const float *m = meta + (size_t)probe * 4;
-        ro = ro + f3(m[1], m[2], m[3]) * classify_strength;
-    }
-    rd = xyz(dirs[k]);
-}
-
-// Geometric shading frame of a hit (js/gi_probes.js:1993-2054).
-struct HitFrame {
-    float3 hit, ng, kd, emis;
-    float bias;
-};
-
-__device__ __forceinline__ HitFrame hit_frame(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p,
-                                              float3 ro, float3 rd, float t, uint32_t tri, uint32_t inst) {
-    HitFrame h;
-    const float4 *tp = s.tri + (size_t)tri * 3;
-    const float4 B = tp[1], C = tp[2];
-    const uint32_t m = __float_as_uint(B.w);
-    const float4 *ip = s.inst + (size_t)inst * 4;
-    const float4 r0 = ip[0], r1 = ip[1], r2 = ip[2];
-    float3 nl = normalize3(cross3(xyz(B), xyz(C)));
-    float3 ng = normalize3(xyz(r0) * nl.x + xyz(r1) * nl.y + xyz(r2) * nl.z);
-    h.ng = ng * (dot3(ng, rd) < 0.f ? 1.f : -1.f);
-    h.bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
-    h.hit = (ro + rd * t) + h.ng * h.bias;
-    const float *mat = s.mats + (size_t)m * SBGI_MAT_STRIDE;
-    const float metal = mat[4], transm = mat[5];
-    h.emis = f3(mat[7], mat[8], mat[9]) * p.dbg_emissive;
-    h.kd.x = fminf(fmaxf(mat[0] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
-    h.kd.y = fminf(fmaxf(mat[1] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
-    h.kd.z = fminf(fmaxf(mat[2] * p.dbg_albedo * (1.f - metal) * (1.f - transm), 0.f), 0.95f);
-    return h;
-}
-
-// One light of the NEE loop (js/gi_probes.js:2129-2176): the gate before the shadow ray,
-// and the contribution when it is not blocked.
-struct LightSample {
-    float3 wi, lcol;
-    float dist, ndl;
-    bool pass;
-};
-
-__device__ __forceinline__ LightSample light_gate(const float *__restrict__ L, const HitFrame &h,
-                                                  const sbgi_tick_params &p) {
-    LightSample ls;
-    const float ltype = L[0];
-    ls.pass = false;
-    if (!(ltype < 2.5f)) return ls;
-    const bool is_ir = fabsf(L[16] - 4.0f) < 0.25f;
-    const float gate = is_ir ? p.nir_gate * p.nir_gain : 1.f;
-    ls.lcol = f3(L[7], L[8], L[9]) * gate;
-    const float lrange = L[10];
-    const bool is_dir = ltype < 0.5f;
-    const float3 ldir = f3(L[4], L[5], L[6]);
-    const float3 tl = is_dir ? ldir * -1.f : f3(L[1], L[2], L[3]) - h.hit;
-    ls.dist = is_dir ? 1e4f : fmaxf(sqrtf(dot3(tl, tl)), 1e-4f);
-    ls.wi = normalize3(tl);
-    ls.ndl = fmaxf(dot3(h.ng, ls.wi), 0.f);
-    const float lpeak = fmaxf(fmaxf(ls.lcol.x, ls.lcol.y), ls.lcol.z);
-    ls.pass = ls.ndl * lpeak > 0.f && (is_dir || lrange <= 0.f || ls.dist < lrange);
-    return ls;
-}
-
-__device__ __forceinline__ float3 light_contrib(const float *__restrict__ L, const HitFrame &h, const LightSample &ls,
-                                                const sbgi_tick_params &p) {
-    const float ltype = L[0];
-    const float lrange = L[10], ldecay = L[11], lca = L[12], lcp = L[13];
-    const bool is_dir = ltype < 0.5f;
-    const bool is_spot = fabsf(ltype - 2.0f) < 0.5f;
-    const float3 ldir = f3(L[4], L[5], L[6]);
-    const float falloff = 1.0f / fmaxf(powf(ls.dist, ldecay), 0.01f);
-    const float rr = ls.dist / fmaxf(lrange, 1e-4f);
-    const float rr2 = rr * rr;
-    const float win = fminf(fmaxf(1.f - rr2 * rr2, 0.f), 1.f);
-    const float ranged = falloff * (win * win);
-    const float pos_att = lrange > 0.f ? ranged : falloff;
-    const float dist_att = is_dir ? 1.f : pos_att;
-    const float acos_ = dot3(ls.wi, ldir) * -1.f;
-    const float spot = fminf(fmaxf((acos_ - lca) / fmaxf(lcp - lca, 1e-4f), 0.f), 1.f);
-    const float att = dist_att * (is_spot ? spot : 1.f);
-    const float3 diffuse = h.kd * (1.0f / SBGI_PI);
-    return ((diffuse * ls.ndl) * ls.lcol) * (att * p.dbg_direct);
-}
 
 constexpr uint32_t kBlock = 256;
 // Walk blocks of 8 waves share one LDS copy of the TLAS, so the copy does not limit how
@@ -307,7 +147,7 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
     if (wave_hit) {
         for (uint32_t li = 0; li < s.light_count; li++) {
             bool pass = false;
-            if (hit) pass = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p).pass;
+            if (hit) pass = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p, light_scale(s, li)).pass;
             if (pass) pass_bits |= 1ull << li;
             count += (uint32_t)__popcll(__ballot(pass));
         }
@@ -334,7 +174,7 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
         const uint64_t m = __ballot(pass);
         if (m == 0ull) continue;
         if (pass) {
-            const LightSample ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p);
+            const LightSample ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p, light_scale(s, li));
             const uint32_t idx = pos + (uint32_t)__popcll(m & lt_mask);
             if (idx < q.capacity) {
                 q.rec[(size_t)idx * 2] = make_float4(h.hit.x, h.hit.y, h.hit.z, ls.dist - h.bias);
@@ -349,6 +189,7 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
 __global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sbgi_tick_params p,
                                                      const float4 *__restrict__ dirs, const float *__restrict__ meta,
                                                      const ushort4 *__restrict__ prev_atlas,
+                                                     const float4 *__restrict__ prev_atlas_f32,
                                                      const float4 *__restrict__ hits,
                                                      const unsigned long long *__restrict__ vis,
                                                      float4 *__restrict__ rays, int2 *__restrict__ hit_ids) {
@@ -369,11 +210,13 @@ __global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sb
         for (uint32_t li = 0; li < s.light_count; li++) {
             if (!((vm >> li) & 1ull)) continue;
             const float *L = s.lights + li * SBGI_LIGHT_STRIDE;
-            const LightSample ls = light_gate(L, h, p);
+            const LightSample ls = light_gate(L, h, p, light_scale(s, li));
             if (ls.pass) rad = rad + light_contrib(L, h, ls, p);
         }
         // multibounce from the previous filtered atlas (js/gi_probes.js:2181-2188)
-        const float3 bounce = sample_atlas(c, prev_atlas, h.hit, h.ng) * h.kd * (1.0f / SBGI_PI);
+        const float3 smp = prev_atlas_f32 ? sample_atlas(c, prev_atlas_f32, h.hit, h.ng)
+                                          : sample_atlas(c, prev_atlas, h.hit, h.ng);
+        const float3 bounce = smp * h.kd * (1.0f / SBGI_PI);
         const float bl = bounce.x * 0.2126f + bounce.y * 0.7152f + bounce.z * 0.0722f;
         const float roll = p.radiance_clamp / fmaxf(fmaxf(p.radiance_clamp, bl), 1e-6f);
         out = rad + bounce * (roll * p.dbg_bounce);
@@ -485,8 +328,9 @@ void launch_walk(const DevScene &s, const Src &src, const uint32_t *n_dev, uint3
 }
 
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
-                        const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
-                        int2 *hit_ids, Counters *counters, const hipEvent_t *ev, hipStream_t st) {
+                        const float *meta, const ushort4 *prev_atlas, const float4 *prev_atlas_f32,
+                        const TraceScratch &w, float4 *rays, int2 *hit_ids, Counters *counters, const hipEvent_t *ev,
+                        hipStream_t st) {
     const uint32_t n = c.updated * c.rays;
     if (n == 0) return hipSuccess;
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
@@ -507,6 +351,6 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
                           counters ? counters->shadow_walk : nullptr, lds, st);
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
-                       prev_atlas, w.hits, w.vis, rays, hit_ids);
+                       prev_atlas, prev_atlas_f32, w.hits, w.vis, rays, hit_ids);
     return hipGetLastError();
 }

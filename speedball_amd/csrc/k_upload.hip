@@ -26,7 +26,7 @@ __device__ __forceinline__ void gutter_src(uint32_t lx, uint32_t ly, uint32_t ti
 
 __global__ void __launch_bounds__(64) k_upload(DevCascade c, sbgi_tick_params p, const float *__restrict__ irr,
                                                const float *__restrict__ depth, ushort4 *__restrict__ atlas,
-                                               ushort4 *__restrict__ datlas) {
+                                               ushort4 *__restrict__ datlas, float4 *__restrict__ atlas_f32) {
     __shared__ float4 cache[36];
     __shared__ float2 dcache[36];
     const uint32_t slot = blockIdx.x;
@@ -78,6 +78,7 @@ __global__ void __launch_bounds__(64) k_upload(DevCascade c, sbgi_tick_params p,
         const size_t o = (size_t)(row * 8 + ly) * c.aw + col * 8 + lx;
         const float4 v = cache[src];
         atlas[o] = pack4(v.x, v.y, v.z, v.w);
+        if (atlas_f32) atlas_f32[o] = v;
         if (parity) {
             const float2 d = dcache[src];
             datlas[o] = pack4(d.x, d.y, 0.f, 1.f);
@@ -103,9 +104,9 @@ __global__ void k_clear_atlas(ushort4 *__restrict__ atlas, size_t texels) {
 }  // namespace
 
 hipError_t launch_upload(const DevCascade &c, const sbgi_tick_params &p, const float *irr, const float *depth,
-                         ushort4 *atlas, ushort4 *datlas, hipStream_t st) {
+                         ushort4 *atlas, ushort4 *datlas, float4 *atlas_f32, hipStream_t st) {
     if (c.updated == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_upload, dim3(c.updated), dim3(64), 0, st, c, p, irr, depth, atlas, datlas);
+    hipLaunchKernelGGL(k_upload, dim3(c.updated), dim3(64), 0, st, c, p, irr, depth, atlas, datlas, atlas_f32);
     return hipGetLastError();
 }
 

@@ -30,9 +30,12 @@ hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32
 hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
 // ev (optional): 4 events recorded before the closest-hit walk, after it, after the
 // shadow-ray emission and after the shadow walk.
+// prev_atlas_f32 (optional): an FP32 bounce source used instead of the RGBA16F prev_atlas
+// (the differentiable mode).
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
-                        const float *meta, const ushort4 *prev_atlas, const TraceScratch &w, float4 *rays,
-                        int2 *hit_ids, Counters *counters, const hipEvent_t *ev, hipStream_t st);
+                        const float *meta, const ushort4 *prev_atlas, const float4 *prev_atlas_f32,
+                        const TraceScratch &w, float4 *rays, int2 *hit_ids, Counters *counters, const hipEvent_t *ev,
+                        hipStream_t st);
 
 // k_blend.hip
 struct BlendTables {
@@ -52,8 +55,9 @@ hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const fl
                         uint32_t precision, float *irr, float *depth, hipStream_t st);
 
 // k_upload.hip
+// atlas_f32 (optional): also store the irradiance tiles unrounded (differentiable mode)
 hipError_t launch_upload(const DevCascade &c, const sbgi_tick_params &p, const float *irr, const float *depth,
-                         ushort4 *atlas, ushort4 *datlas, hipStream_t st);
+                         ushort4 *atlas, ushort4 *datlas, float4 *atlas_f32, hipStream_t st);
 hipError_t launch_clear_atlas(ushort4 *atlas, size_t texels, hipStream_t st);
 
 // k_scene.hip
@@ -69,3 +73,11 @@ hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const u
                              hipStream_t st);
 hipError_t launch_refit_tlas(float *mats, uint32_t tlas_base, uint32_t tlas_count, const float *inst_bounds,
                              const int32_t *parent, uint32_t *arrive, hipStream_t st);
+
+// k_adjoint.hip — adjoint of the differentiable two-update solve (see that file). w holds
+// the forward's frozen hits and visibility masks, atlas1 the first update's FP32 atlas;
+// gE2 and gE1 are P x 36 x 3; g_albedo (n x 3) and g_light (lights) accumulate.
+size_t adjoint_scratch_bytes(uint32_t rays_total, uint32_t cells, uint32_t lights);
+hipError_t launch_adjoint(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                          const BlendTables &t, const TraceScratch &w, const float4 *atlas1, const float *gE2,
+                          float *gE1, float *g_albedo, float *g_light, void *scratch, hipStream_t st);

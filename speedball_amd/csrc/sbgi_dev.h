@@ -36,7 +36,19 @@ struct DevScene {
     const float *__restrict__ lights;
     uint32_t node_count, tlas_count, light_count, tri_count, inst_count;
     uint32_t blas_octants;  // 1: reference node order only; 8: + near-first copy per ray octant
+    // differentiable mode (null otherwise): per-instance-triangle albedo overriding the
+    // material colour, albedo[(map[inst].x + tri - map[inst].y) * 3 + c], and a scale per
+    // light on its colour x intensity
+    const float *__restrict__ albedo;
+    const int2 *__restrict__ albedo_map;
+    const float *__restrict__ light_scale;
 };
+
+// albedo entry of (instance slot, pool triangle) in the differentiable mode
+__device__ __forceinline__ uint32_t albedo_index(const DevScene &s, uint32_t inst, uint32_t tri) {
+    const int2 m = s.albedo_map[inst];
+    return (uint32_t)(m.x + ((int)tri - m.y));
+}
 
 // Per-solve cascade geometry (per-cascade uniforms of js/gi_probes.js:3200-3211).
 struct DevCascade {

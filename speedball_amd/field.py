@@ -136,6 +136,45 @@ class ProbeField:
     def read_hit_ids(self):
         return self.read(N.BUF_HIT_IDS, np.int32).reshape(-1, self.rays, 2)
 
+    # ---- differentiable mode (sbgi_diff_*) ----
+    def diff_param_count(self):
+        """(albedo entries, lights) of the differentiable mode."""
+        n, L = C.c_uint32(), C.c_uint32()
+        self._chk(self.L.sbgi_diff_param_count(self.h, C.byref(n), C.byref(L)))
+        return n.value, L.value
+
+    def diff_albedo_map(self):
+        """Per TLAS slot: (offset into the albedo array, first pool triangle)."""
+        m = np.zeros((self._scene.inst_count, 2), np.int32)
+        self._chk(self.L.sbgi_diff_albedo_map(self.h, N.ptr(m), self._scene.inst_count))
+        return m
+
+    def diff_forward(self, albedo, light_scale=None, **params):
+        """Second-update interior irradiance (P, 36, 3) of the two-update differentiable solve."""
+        p = N.default_tick_params(**params)
+        alb = np.ascontiguousarray(albedo, np.float32)
+        ls = None if light_scale is None else np.ascontiguousarray(light_scale, np.float32)
+        out = np.zeros((self.probes(0), 36, 3), np.float32)
+        self._keep = (alb, ls)
+        self._chk(self.L.sbgi_diff_forward(self.h, C.byref(p), N.ptr(alb), None if ls is None else N.ptr(ls),
+                                           N.ptr(out)))
+        return out
+
+    def diff_adjoint(self, grad_irr):
+        """(d/d albedo (n, 3), d/d light scale (L,)) of sum(grad_irr * irr) at the last forward."""
+        n, L = self.diff_param_count()
+        g = np.ascontiguousarray(grad_irr, np.float32)
+        ga = np.zeros((n, 3), np.float32)
+        gl = np.zeros(max(L, 1), np.float32)
+        self._chk(self.L.sbgi_diff_adjoint(self.h, N.ptr(g), N.ptr(ga), N.ptr(gl)))
+        return ga, gl[:L]
+
+    def diff_timing(self):
+        """(forward ms, adjoint ms) of the last differentiable forward / adjoint (HIP events)."""
+        a, b = C.c_float(), C.c_float()
+        self._chk(self.L.sbgi_diff_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def query(self, ro, rd, tmax=None, mode=0):
         """Closest-hit (mode 0) or any-hit (mode 1) queries of the resident scene.
         Returns (t, tri, inst, blocked) arrays."""

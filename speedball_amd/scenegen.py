@@ -274,3 +274,40 @@ def instanced(res=(32, 16, 24), rays: int = 256, total_tris: int = 262144, n_ins
     jitter = (seed * 0.6180339887) % 1.0
     return Workload("instanced", scene, (0.0, 0.0, 0.0), tuple(box.tolist()), tuple(res), rays, depth_oct,
                     float(np.float32(jitter)), hysteresis, bounces=2)
+
+
+def differentiable(res=(16, 16, 16), rays: int = 128, n_mesh: int = 4, n_instances: int = 64, torus_n: int = 16,
+                   n_lights: int = 16, box: float = 20.0, seed_id: int = 4):
+    """Differentiable workload: n_mesh tori of 4 * torus_n^2 triangles (1024 at the
+    default), n_instances random-TRS instances in a box, n_lights point lights with
+    log-uniform intensity [1, 100], per-instance-triangle albedo U[0.1, 0.9] and a
+    target albedo perturbed by U[-0.2, 0.2] (multiplicative, next seed).
+
+    Returns (workload, albedo (n, 3) float32, target_albedo (n, 3) float32)."""
+    rng = SplitMix64(SEED_BASE + seed_id)
+    b = SceneBuilder()
+    mat = b.add_material(uber_material((0.5, 0.5, 0.5)))
+    meshes = []
+    for m in range(n_mesh):
+        R_ = 1.0
+        r_ = 0.2 + 0.15 * m
+        v, t = torus(2 * torus_n, torus_n, R=R_, r=r_)
+        meshes.append(b.add_mesh(v, t, tri_material=np.full(len(t), mat, np.uint32)))
+    for i in range(n_instances):
+        R = random_rotation(rng)
+        s = rng.uniform(0.75, 2.0)
+        tr = (rng.uniform(0, box), rng.uniform(0, box), rng.uniform(0, box))
+        b.add_instance(meshes[i % n_mesh], trs(tr, R, s))
+    for _ in range(n_lights):
+        p = (rng.uniform(0, box), rng.uniform(0, box), rng.uniform(0, box))
+        b.add_light(point_light(p, intensity=rng.log_uniform(1.0, 100.0), distance=0.0, decay=2.0))
+    scene = b.finalize(workload="differentiable")
+    n_alb = n_instances * 4 * torus_n * torus_n
+    gen = np.random.default_rng(SEED_BASE + seed_id)
+    albedo = gen.uniform(0.1, 0.9, (n_alb, 3)).astype(np.float32)
+    gen2 = np.random.default_rng(SEED_BASE + seed_id + 1)
+    target = (albedo * (1.0 + gen2.uniform(-0.2, 0.2, albedo.shape))).astype(np.float32)
+    seed = SEED_BASE + seed_id
+    jitter = float(np.float32((seed * 0.6180339887) % 1.0))
+    w = Workload("differentiable", scene, (0.0, 0.0, 0.0), (box, box, box), tuple(res), rays, 6, jitter, 0.0, bounces=2)
+    return w, albedo, target

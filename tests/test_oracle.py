@@ -397,3 +397,42 @@ def test_bounce_weights_are_a_partition_of_unity():
     hit = o.hits_view()[..., 0] >= 0
     assert hit.sum() > 100
     np.testing.assert_allclose(o.rays_view()[hit][:, :3], c * 0.625 / np.pi, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- differentiable mode
+def test_diff_adjoint_matches_central_differences():
+    """FP64 adjoint of the two-update differentiable solve against central finite
+    differences (eps 1e-3) of the FP64 forward, on albedo entries and light scales.
+    The loss is sum (E2 - T)^2 with a fixed random target."""
+    w, albedo, _ = sg.differentiable(res=(3, 3, 3), rays=32, n_mesh=2, n_instances=3, torus_n=4, n_lights=2, box=6.0)
+    amap, n = oracle.albedo_map(w.scene)
+    alb = albedo.astype(np.float64) * 0.8  # keeps kd inside the clamp
+    ls = np.array([1.0, 1.3])
+    p = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, 6, prec="64").params(
+        frame_jitter=w.frame_jitter, hysteresis=0.0)
+    args = (w.scene, w.grid_min, w.grid_size, w.res, w.rays, p)
+    E2, E1, A1 = oracle.diff_forward(*args, alb, amap, ls)
+    rng = np.random.default_rng(11)
+    T = E2 * (1 + rng.uniform(-0.3, 0.3, E2.shape))
+
+    def loss(a, l):
+        return float(((oracle.diff_forward(*args, a, amap, l)[0] - T) ** 2).sum())
+
+    ga, gl = oracle.diff_adjoint(*args, alb, amap, ls, A1, 2 * (E2 - T))
+    assert np.abs(ga).max() > 0 and np.abs(gl).max() > 0
+    eps = 1e-3
+    # the entries with the largest gradients (the rest are dominated by FD round-off)
+    flat = np.argsort(-np.abs(ga).reshape(-1))[:12]
+    for f in flat:
+        i, c = divmod(int(f), 3)
+        ap, am = alb.copy(), alb.copy()
+        ap[i, c] += eps
+        am[i, c] -= eps
+        fd = (loss(ap, ls) - loss(am, ls)) / (2 * eps)
+        assert abs(fd - ga[i, c]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(ga).max()), (i, c, fd, ga[i, c])
+    for j in range(2):
+        lp, lm = ls.copy(), ls.copy()
+        lp[j] += eps
+        lm[j] -= eps
+        fd = (loss(alb, lp) - loss(alb, lm)) / (2 * eps)
+        assert abs(fd - gl[j]) <= 1e-5 * max(abs(fd), 1e-6), (j, fd, gl[j])

@@ -115,6 +115,32 @@ def fp64_check(w, field, precision):
     return float(np.percentile(e, 99)), float(e.max())
 
 
+def adjoint_leg(reps=5):
+    """Forward + hand-written adjoint of the differentiable two-update solve on its own
+    workload (4 tori x 1024 triangles x 64 instances, 16 lights, 16^3 probes x 128 rays):
+    device time from the library's HIP events, median of `reps`."""
+    from speedball_amd import scenegen
+    from speedball_amd.field import ProbeField
+    w, albedo, target = scenegen.differentiable()
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=6, precision=0, traversal=1)
+    f.set_scene(w.scene)
+    params = dict(frame_jitter=w.frame_jitter, hysteresis=0.0)
+    g = 2 * (f.diff_forward(albedo, **params) - f.diff_forward(target, **params))
+    fwd, adj = [], []
+    for _ in range(reps):
+        f.diff_forward(albedo, **params)
+        f.diff_adjoint(g)
+        a, b = f.diff_timing()
+        fwd.append(a)
+        adj.append(b)
+    n_alb, n_l = f.diff_param_count()
+    f.close()
+    fm, am = float(np.median(fwd)), float(np.median(adj))
+    return {"workload": "differentiable: 4 tori x 1024 triangles in 64 instances (65536 albedo entries), 16 point "
+                        "lights, 16^3 probes x 128 rays, 2 updates", "albedo_params": n_alb, "light_params": n_l,
+            "forward_ms": fm, "adjoint_ms": am, "fwd_plus_adjoint_ms": fm + am, "ratio_to_forward": (fm + am) / fm}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +152,7 @@ def main():
                          "near-first octant node copies (same hits except exact-t ties)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 subset check")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
@@ -214,8 +241,9 @@ def main():
     err99 = errmax = None
     if not args.no_check:
         err99, errmax = fp64_check(w, first_irr, prec)
-    cpu = None if args.no_cpu else cpu_baseline(w)
     f.close()
+    adj = None if args.no_adjoint else adjoint_leg()
+    cpu = None if args.no_cpu else cpu_baseline(w)
 
     out = {
         "metric": METRIC, "value": value, "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
@@ -244,6 +272,8 @@ def main():
                              "primary_hits": hits}},
         "blend": {"kernel": "k_blend", "ms": blend_ms, "tflops": blend_flops / (blend_ms / 1000.0) / 1e12,
                   "gbs": blend_bytes / (blend_ms / 1000.0) / 1e9, "flops": blend_flops, "bytes": blend_bytes},
+        "fwd_plus_adjoint_ms": adj["fwd_plus_adjoint_ms"] if adj else None,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -201,6 +201,18 @@ typedef struct sbgi_stats {
     uint64_t shadow_inst_visits;
 } sbgi_stats;
 
+/* Wave schedule of the persistent walks (no reference counterpart: it is how this build
+ * schedules the reference's walk on 64-lane waves). A triangle / instance phase runs when
+ * the lanes waiting in it are >= 1/tri_ratio (1/inst_ratio) of the lanes still walking
+ * nodes; finished lanes are refilled in batches of >= refill_min; rays are fetched
+ * fetch_chunk at a time. Results do not depend on it. */
+typedef struct sbgi_walk_tuning {
+    uint32_t tri_ratio;   /* 1..64, default 4 */
+    uint32_t inst_ratio;  /* 1..64, default 4 */
+    uint32_t refill_min;  /* 1..64, default 8 */
+    uint32_t fetch_chunk; /* 1..1024, default 64 */
+} sbgi_walk_tuning;
+
 typedef struct sbgi_field sbgi_field;
 
 /* Fill *p with the reference defaults (js/gi_probes.js:1198-1255, gated jitter mode). */
@@ -301,6 +313,15 @@ int32_t sbgi_diff_adjoint(sbgi_field *f, const float *grad_irr, float *grad_albe
 /* Device time (HIP events) of the last diff forward (both updates, before the host
  * copy) and of the last adjoint's kernels. */
 int32_t sbgi_diff_timing(sbgi_field *f, float *ms_forward, float *ms_adjoint);
+
+/* Walk schedule of this field (see sbgi_walk_tuning). */
+int32_t sbgi_get_walk_tuning(sbgi_field *f, sbgi_walk_tuning *out);
+int32_t sbgi_set_walk_tuning(sbgi_field *f, const sbgi_walk_tuning *t);
+/* Autotuner: times the trace of cascade p->cascade (probe range of p) under candidate
+ * schedules by coordinate descent over the four parameters, keeps the fastest in the
+ * field and returns it with its trace time. Traces into the ray scratch only: the
+ * temporal state and atlases are not touched. */
+int32_t sbgi_autotune(sbgi_field *f, const sbgi_tick_params *p, sbgi_walk_tuning *best, float *best_ms);
 
 /* Last error message of this field (or of the last failed sbgi_create when f is NULL). */
 const char *sbgi_last_error(const sbgi_field *f);

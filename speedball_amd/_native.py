@@ -36,7 +36,7 @@ EXPORTS = [
     "sbgi_destroy", "sbgi_sb_create", "sbgi_sb_add_material", "sbgi_sb_add_mesh", "sbgi_sb_add_instance",
     "sbgi_sb_add_light", "sbgi_sb_finalize", "sbgi_sb_tlas_order", "sbgi_sb_set_transform",
     "sbgi_sb_last_error", "sbgi_sb_destroy", "sbgi_diff_param_count", "sbgi_diff_albedo_map", "sbgi_diff_forward",
-    "sbgi_diff_adjoint", "sbgi_diff_timing",
+    "sbgi_diff_adjoint", "sbgi_diff_timing", "sbgi_get_walk_tuning", "sbgi_set_walk_tuning", "sbgi_autotune",
 ]
 
 
@@ -90,6 +90,11 @@ class Stats(C.Structure):
                 ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64)]
 
 
+class WalkTuning(C.Structure):
+    _fields_ = [("tri_ratio", C.c_uint32), ("inst_ratio", C.c_uint32), ("refill_min", C.c_uint32),
+                ("fetch_chunk", C.c_uint32)]
+
+
 class SbgiError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"sbgi error {code}: {msg}")
@@ -139,6 +144,9 @@ def lib() -> C.CDLL:
         "sbgi_diff_forward": (i32, [vp, C.POINTER(TickParams), vp, vp, vp]),
         "sbgi_diff_adjoint": (i32, [vp, vp, vp, vp]),
         "sbgi_diff_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "sbgi_get_walk_tuning": (i32, [vp, C.POINTER(WalkTuning)]),
+        "sbgi_set_walk_tuning": (i32, [vp, C.POINTER(WalkTuning)]),
+        "sbgi_autotune": (i32, [vp, C.POINTER(TickParams), C.POINTER(WalkTuning), C.POINTER(C.c_float)]),
         "sbgi_last_error": (C.c_char_p, [vp]),
         "sbgi_destroy": (i32, [vp]),
         "sbgi_sb_create": (i32, [C.POINTER(vp)]),

@@ -75,6 +75,7 @@ struct sbgi_field {
     hipEvent_t ring[kTicketRing] = {};
     hipEvent_t ev[5] = {};
     hipEvent_t tev[4] = {};  // trace sub-stages
+    sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk};
     // differentiable mode (sbgi_diff_*): parameters, the first update's FP32 atlas, the
     // gradients and the adjoint's scratch; the frozen hits and visibility masks are the
     // trace scratch tw of the last diff forward (diff_valid until any other solve)
@@ -198,6 +199,10 @@ DevScene make_scene(const sbgi_field *f) {
     s.light_count = f->view.light_count;
     s.tri_count = f->view.tri_count;
     s.inst_count = f->view.inst_count;
+    s.tri_ratio = f->tuning.tri_ratio;
+    s.inst_ratio = f->tuning.inst_ratio;
+    s.refill_min = f->tuning.refill_min;
+    s.fetch_chunk = f->tuning.fetch_chunk;
     return s;
 }
 
@@ -1171,6 +1176,83 @@ int32_t sbgi_diff_timing(sbgi_field *f, float *ms_forward, float *ms_adjoint) {
     if (!f || !ms_forward || !ms_adjoint) return SBGI_E_INVALID;
     *ms_forward = f->ms_diff_fwd;
     *ms_adjoint = f->ms_diff_adj;
+    return SBGI_OK;
+}
+
+
+// ---- walk schedule and autotuner --------------------------------------------------
+
+int32_t sbgi_get_walk_tuning(sbgi_field *f, sbgi_walk_tuning *out) {
+    if (!f || !out) return SBGI_E_INVALID;
+    *out = f->tuning;
+    return SBGI_OK;
+}
+
+int32_t sbgi_set_walk_tuning(sbgi_field *f, const sbgi_walk_tuning *t) {
+    if (!f || !t) return SBGI_E_INVALID;
+    if (t->tri_ratio < 1 || t->tri_ratio > 64 || t->inst_ratio < 1 || t->inst_ratio > 64 || t->refill_min < 1 ||
+        t->refill_min > 64 || t->fetch_chunk < 1 || t->fetch_chunk > 1024)
+        return fail(f, SBGI_E_INVALID, "walk tuning out of range");
+    f->tuning = *t;
+    return SBGI_OK;
+}
+
+int32_t sbgi_autotune(sbgi_field *f, const sbgi_tick_params *p, sbgi_walk_tuning *best, float *best_ms) {
+    if (!f || !p) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    if (p->cascade >= f->desc.cascade_count) return fail(f, SBGI_E_INVALID, "cascade index out of range");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    CascadeState &cs = f->casc[p->cascade];
+    const uint32_t updated = p->updated_count ? std::min(p->updated_count, cs.total) : cs.total;
+    const DevCascade c = make_cascade(f, cs, p->probe_offset % cs.total, updated);
+    hipStream_t st = f->stream;
+    HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
+    f->diff_valid = false;
+    auto time_trace = [&](const sbgi_walk_tuning &t, float &ms) -> int32_t {
+        const sbgi_walk_tuning keep = f->tuning;
+        f->tuning = t;
+        const DevScene s = make_scene(f);
+        f->tuning = keep;
+        float best_run = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            HIP_OK(f, hipEventRecord(f->dev[0], st));
+            HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, nullptr, nullptr,
+                                   nullptr, st));
+            HIP_OK(f, hipEventRecord(f->dev[1], st));
+            HIP_OK(f, hipEventSynchronize(f->dev[1]));
+            float m = 0.f;
+            HIP_OK(f, hipEventElapsedTime(&m, f->dev[0], f->dev[1]));
+            best_run = std::min(best_run, m);
+        }
+        ms = best_run;
+        return SBGI_OK;
+    };
+    sbgi_walk_tuning cur = f->tuning;
+    float cur_ms = 0.f;
+    int32_t rc = time_trace(cur, cur_ms);
+    if (rc != SBGI_OK) return rc;
+    static const uint32_t ratios[] = {2, 4, 8, 16}, refills[] = {4, 8, 16, 32}, chunks[] = {32, 64, 128, 256};
+    for (int pass = 0; pass < 2; pass++) {
+        for (int param = 0; param < 4; param++) {
+            const uint32_t *cand = param < 2 ? ratios : (param == 2 ? refills : chunks);
+            for (int i = 0; i < 4; i++) {
+                sbgi_walk_tuning t = cur;
+                uint32_t *slot = param == 0 ? &t.tri_ratio : param == 1 ? &t.inst_ratio : param == 2 ? &t.refill_min
+                                                                                                  : &t.fetch_chunk;
+                if (*slot == cand[i]) continue;
+                *slot = cand[i];
+                float ms = 0.f;
+                if ((rc = time_trace(t, ms)) != SBGI_OK) return rc;
+                if (ms < cur_ms * 0.99f) {
+                    cur = t;
+                    cur_ms = ms;
+                }
+            }
+        }
+    }
+    f->tuning = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
     return SBGI_OK;
 }
 

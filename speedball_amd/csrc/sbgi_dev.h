@@ -42,6 +42,10 @@ struct DevScene {
     const float *__restrict__ albedo;
     const int2 *__restrict__ albedo_map;
     const float *__restrict__ light_scale;
+    // wave schedule of the walk (traverse2.h): a TRI / INST phase runs when the lanes
+    // waiting in it are >= 1/tri_ratio (1/inst_ratio) of those still in NODE; retired
+    // lanes are refilled in batches of >= refill_min; fetch_chunk rays per fetch atomic
+    uint32_t tri_ratio, inst_ratio, refill_min, fetch_chunk;
 };
 
 // albedo entry of (instance slot, pool triangle) in the differentiable mode
@@ -68,6 +72,8 @@ struct DevCascade {
 // (one 128-byte line) apart.
 constexpr uint32_t kFetchHeads = 64;
 constexpr uint32_t kHeadStride = 32;
+// default wave schedule of the walks (DevScene::tri_ratio ... fetch_chunk)
+constexpr uint32_t kDefaultTriRatio = 4, kDefaultInstRatio = 4, kDefaultRefillMin = 8, kDefaultFetchChunk = 64;
 
 // Traversal counters of counting ticks: [0..3] closest-hit walk {BLAS nodes, TLAS nodes,
 // triangle tests, instance entries}, [4] shadow rays emitted, [5..8] the shadow walk's

@@ -106,7 +106,8 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     const bool interior = b.w == 0xFFFFFFFFu;
     const bool leaf = hit && !interior;
     const uint32_t off = b.w & 0x00FFFFFFu, end = off + (b.w >> 24);
-    const bool blas_leaf = leaf && r.level, tlas_leaf = leaf && !r.level;
+    // an empty leaf (count 0) is passed over like a miss: TRI / INST never see an empty range
+    const bool blas_leaf = leaf && r.level && end > off, tlas_leaf = leaf && !r.level && end > off;
     uint32_t state = r.state;
     if (!in_range) state = r.level ? ST_INST : ST_DONE;
     else if (blas_leaf) state = ST_TRI;
@@ -187,17 +188,16 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
     r.state = state;
 }
 
-// Wave schedule. A TRI or INST phase runs when the lanes waiting in it are at least
-// 1/kPhaseRatio of the lanes still walking nodes (so always once no lane is in NODE);
-// retired lanes are refilled when at least kRefillMin are waiting, or nothing else is left.
-constexpr int kPhaseRatio = 4;
-constexpr int kRefillMin = 8;
-
+// Wave schedule (DevScene::tri_ratio, inst_ratio, refill_min, fetch_chunk; the defaults
+// came from a lockstep simulation of the instanced workload, and sbgi_autotune refines them
+// per scene). A TRI or INST phase runs when the lanes waiting in it are at least 1/ratio of
+// the lanes still walking nodes (so always once no lane is in NODE); retired lanes are
+// refilled when at least refill_min are waiting, or nothing else is left.
+//
 // Ray fetching. One counter serialises every atomic on it (~10 ns each on MI355X), so the
 // ray range is cut into kFetchHeads slices, each with its own counter kHeadStride words
-// (128 bytes) apart, and a wave takes kFetchChunk rays per atomic. A wave starts on slice
+// (128 bytes) apart, and a wave takes fetch_chunk rays per atomic. A wave starts on slice
 // (wave id % kFetchHeads) and moves to the next slice when its slice is drained.
-constexpr uint32_t kFetchChunk = 64;
 
 // Ray sources of walk_wave: load(scene, index, lane ray), store(index, lane ray).
 // walk_wave runs one wave until all n rays of the source are taken and walked; `heads`
@@ -215,10 +215,14 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
     uint32_t idx = 0;
     bool have = false, drained = false;
     uint32_t h = wave_id % kFetchHeads, tried = 0, cnext = 0, cend = 0;
+    // clamped so that a zero-initialised schedule still makes progress
+    const int tri_ratio = (int)max(s.tri_ratio, 1u), inst_ratio = (int)max(s.inst_ratio, 1u);
+    const int refill_min = (int)max(s.refill_min, 1u);
+    const uint32_t chunk = max(s.fetch_chunk, 1u);
     for (;;) {
         const uint64_t done = __ballot(r.state == ST_DONE);
         uint64_t live = __ballot(r.state < ST_DONE);
-        if (done && (__popcll(done) >= kRefillMin || !live)) {
+        if (done && (__popcll(done) >= refill_min || !live)) {
             if (r.state == ST_DONE) {
                 if (have) src.store(idx, r);
                 have = false;
@@ -231,7 +235,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
                     const uint32_t s0 = (uint32_t)(((uint64_t)n * h) / kFetchHeads);
                     const uint32_t s1 = (uint32_t)(((uint64_t)n * (h + 1u)) / kFetchHeads);
                     uint32_t off = 0;
-                    if ((int)lane == leader) off = atomicAdd(heads + (size_t)h * kHeadStride, kFetchChunk);
+                    if ((int)lane == leader) off = atomicAdd(heads + (size_t)h * kHeadStride, chunk);
                     off = (uint32_t)__shfl((int)off, leader);
                     if (off >= s1 - s0) {
                         h = h + 1u == kFetchHeads ? 0u : h + 1u;
@@ -239,7 +243,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
                         continue;
                     }
                     cnext = s0 + off;
-                    cend = min(cnext + kFetchChunk, s1);
+                    cend = min(cnext + chunk, s1);
                 }
                 const uint32_t take = min(want - got, cend - cnext);
                 if (r.state == ST_DONE && rank >= got && rank < got + take) {
@@ -260,12 +264,12 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         }
         uint64_t mn = __ballot(r.state == ST_NODE);
         const uint64_t mt = __ballot(r.state == ST_TRI);
-        if (mt && __popcll(mt) * kPhaseRatio >= __popcll(mn)) {
+        if (mt && __popcll(mt) * tri_ratio >= __popcll(mn)) {
             if (r.state == ST_TRI) tri_step<MODE, COUNT>(s, r, cnt);
         }
         mn = __ballot(r.state == ST_NODE);
         const uint64_t mi = __ballot(r.state == ST_INST);
-        if (mi && __popcll(mi) * kPhaseRatio >= __popcll(mn)) {
+        if (mi && __popcll(mi) * inst_ratio >= __popcll(mn)) {
             if (r.state == ST_INST) inst_step<COUNT, LDS>(s, r, cnt);
         }
     }

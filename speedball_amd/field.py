@@ -136,6 +136,28 @@ class ProbeField:
     def read_hit_ids(self):
         return self.read(N.BUF_HIT_IDS, np.int32).reshape(-1, self.rays, 2)
 
+    # ---- walk schedule ----
+    def walk_tuning(self) -> dict:
+        t = N.WalkTuning()
+        self._chk(self.L.sbgi_get_walk_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in N.WalkTuning._fields_}
+
+    def set_walk_tuning(self, **kw):
+        t = N.WalkTuning()
+        self._chk(self.L.sbgi_get_walk_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._chk(self.L.sbgi_set_walk_tuning(self.h, C.byref(t)))
+
+    def autotune(self, cascade: int = 0, **params):
+        """Coordinate-descent search of the walk schedule on this scene; keeps the best.
+        Returns (tuning dict, trace ms)."""
+        p = N.default_tick_params(**params)
+        p.cascade = cascade
+        t, ms = N.WalkTuning(), C.c_float()
+        self._chk(self.L.sbgi_autotune(self.h, C.byref(p), C.byref(t), C.byref(ms)))
+        return {k: getattr(t, k) for k, _ in N.WalkTuning._fields_}, ms.value
+
     # ---- differentiable mode (sbgi_diff_*) ----
     def diff_param_count(self):
         """(albedo entries, lights) of the differentiable mode."""

@@ -231,3 +231,42 @@ def test_js_shim_matches_ctypes_path(tmp_path):
     f.close()
     assert js_irr.shape == py_irr.shape
     assert np.array_equal(js_irr, py_irr)
+
+
+def test_walk_schedule_does_not_change_results(instanced_small):
+    """The wave schedule (sbgi_walk_tuning) changes only which lanes run together: every
+    ray still walks its own nodes in its own order, so state after two updates is
+    bit-identical under extreme schedules, and the autotuner (which traces into the ray
+    scratch only) leaves the temporal state as it found it."""
+    w = instanced_small
+    out = []
+    for tuning in (None, dict(tri_ratio=1, inst_ratio=64, refill_min=1, fetch_chunk=1),
+                   dict(tri_ratio=64, inst_ratio=1, refill_min=64, fetch_chunk=1024), "auto"):
+        f = make_field(w, traversal=N.TRAV_MT_ORDERED)
+        print(f"case: walk tuning {tuning}", flush=True)
+        f.tick(**params(w))
+        if tuning == "auto":
+            t, ms = f.autotune(**params(w))
+            print("autotuned", t, f"{ms:.3f} ms")
+            assert all(1 <= v <= 1024 for v in t.values()) and ms > 0
+            assert f.walk_tuning() == t
+        elif tuning:
+            f.set_walk_tuning(**tuning)
+            assert f.walk_tuning() == tuning
+        f.tick(**params(w))
+        out.append((f.read_hit_ids(), f.read_rays(), f.read_irr(), f.read_atlas()))
+        f.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
+
+
+def test_walk_tuning_rejects_out_of_range(instanced_small):
+    f = make_field(instanced_small, traversal=N.TRAV_MT_ORDERED)
+    try:
+        for bad in (dict(tri_ratio=0), dict(inst_ratio=65), dict(refill_min=0), dict(fetch_chunk=2048)):
+            with pytest.raises(RuntimeError):
+                f.set_walk_tuning(**bad)
+        assert f.walk_tuning() == dict(tri_ratio=4, inst_ratio=4, refill_min=8, fetch_chunk=64)
+    finally:
+        f.close()

@@ -78,7 +78,7 @@ extern "C" {
 /* buffer ids for sbgi_read / sbgi_write / sbgi_update_ranges */
 #define SBGI_BUF_BVH_NODES 0    /* u32[8*N]                 (update_ranges)           */
 #define SBGI_BUF_TRI_INDEX 1    /* u32[3*T]                 (update_ranges)           */
-#define SBGI_BUF_VERTEX_DATA 2  /* f32[8*V]                 (update_ranges)           */
+#define SBGI_BUF_VERTEX_DATA 2  /* f32[8*V]                 (update_ranges, read)     */
 #define SBGI_BUF_TRI_MATERIAL 3 /* u32[T]                   (update_ranges)           */
 #define SBGI_BUF_MATERIALS 4    /* f32 materials|inst|tlas  (update_ranges, read)     */
 #define SBGI_BUF_LIGHTS 5       /* f32[17*L]                (update_ranges, read)     */
@@ -91,6 +91,9 @@ extern "C" {
 #define SBGI_BUF_HIT_IDS 16     /* i32[2*updated*R] (tri, inst) per ray, same order    */
 #define SBGI_BUF_NODES_REFIT 17 /* bvhNodes after a device refit (read)               */
 #define SBGI_BUF_TRI_PACKED 18  /* f32[12*T] packed triangle records (read, debug)    */
+#define SBGI_BUF_INST_BOUNDS 19 /* f32[6*I] instance world AABBs per TLAS slot (read)  */
+#define SBGI_BUF_TLAS_SAH 20    /* u32[8*(2I-1)] MT_ORDERED SAH TLAS, walk format {min.xyz,
+                                   miss},{max.xyz, payload} (read)                     */
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {
@@ -233,22 +236,52 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *scene);
 
 /* copySceneStorageRanges / markStorageDirty (js/gi_probes.js:1543-1561, 4239-4251):
  * ranges holds n_ranges (start, count) pairs in ELEMENTS of buffer buf_id; the data
- * is taken from the same offsets of the full host array src. */
+ * is taken from the same offsets of the full host array src. Vertex and light ranges
+ * are enqueued without waiting for the device; structural buffers are validated first. */
 int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges,
                            uint32_t n_ranges, const void *src);
 
 /* refitFlatBlasRange (js/gi_refit.js:158-165) on the device for each [root, end)
- * pair, then refitFlatTlasRange (js/gi_refit.js:245-292) over the instance world
- * bounds when refit_tlas != 0. Returns SBGI_E_LAYOUT when the threaded layout is
- * violated, as the reference's refit returns false. */
+ * pair, then, when refit_tlas != 0, the instance world bounds from the refit roots and
+ * the TLAS: refitFlatTlasRange (js/gi_refit.js:245-292) in SBGI_TRAV_MT_COMPAT mode, a
+ * fresh SAH build in SBGI_TRAV_MT_ORDERED mode. Returns SBGI_E_LAYOUT when the threaded
+ * layout is violated, as the reference's refit returns false. Enqueued on the field's
+ * stream; nothing is copied back. */
 int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges,
                    int32_t refit_tlas);
 
 /* updateTransforms (js/spectral_scene.js:1402-1514): rewrite the instance records of
  * the given TLAS slots from column-major world matrices (16 doubles each), recompute
- * their world AABBs from the BLAS root bounds and refit the TLAS on the device. */
+ * every slot's world AABB from its BLAS root bounds on the device and update the TLAS
+ * (refit in SBGI_TRAV_MT_COMPAT mode, SAH rebuild in SBGI_TRAV_MT_ORDERED mode). One
+ * staged upload; enqueued on the field's stream. */
 int32_t sbgi_set_instance_transforms(sbgi_field *f, const uint32_t *slots, uint32_t n,
                                      const double *world_matrices, const uint8_t *active);
+
+/* Device skinning (no reference counterpart: the reference receives skinned positions
+ * from its host and re-gathers them into vertexData in updateDeforms,
+ * js/spectral_scene.js:1831-1927, then refits, js/gi_refit.js:158-165). A skin set is a
+ * pooled vertex range and the BLAS over it; its rest pose is that range of the
+ * vertexData last provided by the host (sbgi_set_scene / sbgi_update_ranges). Ranges
+ * must be disjoint; joints index one bone palette of n_bones. n_sets = 0 clears. */
+typedef struct sbgi_skin_set {
+    uint32_t vertex_start, vertex_count; /* pooled vertex rows                          */
+    uint32_t blas_root, blas_end;        /* threaded BLAS node range over those vertices */
+    const uint16_t *joints;              /* 4 per vertex                                */
+    const float *weights;                /* 4 per vertex                                */
+} sbgi_skin_set;
+int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets, uint32_t n_bones);
+
+/* One skinning frame: uploads the palette (n_bones 3x4 row-major matrices, 12 floats
+ * each), writes p' = sum_q w_q (B_q p) and n' = normalize(sum_q w_q B_q n) into
+ * vertexData for every registered vertex, refits the registered BLASes, updates their
+ * walk copies and packed triangles, and, when update_tlas != 0, the instance bounds and
+ * TLAS as sbgi_refit does. Enqueued on the field's stream. */
+int32_t sbgi_skin(sbgi_field *f, const float *bones, uint32_t n_bones, int32_t update_tlas);
+
+/* Device time (ms) of the last scene edits: skinning (or 0), BLAS refit with walk-copy
+ * and triangle updates, and instance records + bounds + TLAS. Waits for them. */
+int32_t sbgi_edit_timing(sbgi_field *f, float *ms_skin, float *ms_blas_refit, float *ms_tlas);
 
 /* One solve of one cascade: trace -> blend -> upload, the ordered pass of
  * renderer.computeAsync(gpu.solveKernels) (js/gi_probes.js:3729-3836). Enqueues on the
@@ -353,6 +386,10 @@ int32_t sbgi_sb_add_light(sbgi_scene_builder *b, const float *rec17);
 int32_t sbgi_sb_finalize(sbgi_scene_builder *b, sbgi_scene_view *view);
 /* slot -> source instance id (the TLAS order), n = instance count */
 int32_t sbgi_sb_tlas_order(const sbgi_scene_builder *b, uint32_t *order, uint32_t n);
+/* Pool placement of a mesh after finalize (the blasList entry's vertBase / vertCount,
+ * triangle base / count and blasRoot / blasEnd, js/spectral_scene.js:1140-1150):
+ * out6 = {vertex_start, vertex_count, tri_start, tri_count, blas_root, blas_end}. */
+int32_t sbgi_sb_mesh_ranges(const sbgi_scene_builder *b, uint32_t mesh_id, uint32_t *out6);
 /* Replace the world matrix of a source instance and recompute the dynamic tail on the
  * host (updateTransforms): instance rows, det sign, world AABB, sparse TLAS refit
  * (refitFlatTlasDirty, js/gi_refit.js:352-430). Writes the TLAS slot of the instance. */

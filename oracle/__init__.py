@@ -197,6 +197,42 @@ def refit_tlas(tlas: np.ndarray, inst_bounds: np.ndarray) -> bool:
     return bool(f(_p(tlas), C.c_uint32(tlas.size // 12), _p(inst_bounds), C.c_uint32(inst_bounds.size // 6)))
 
 
+# ---------------------------------------------------------------- per-frame TLAS, skinning
+def inst_bounds(scene_materials: np.ndarray, inst_base: int, inst_count: int, nodes: np.ndarray,
+                world: np.ndarray):
+    """World AABBs (float32 [n, 6]) and the inflated copy the SAH build reads, from
+    column-major double world matrices [n, 16] and the BLAS root bounds in nodes."""
+    mats = np.ascontiguousarray(scene_materials, np.float32)
+    nodes = np.ascontiguousarray(nodes, np.uint32)
+    world = np.ascontiguousarray(world, np.float64).reshape(-1, 16)
+    b = np.zeros((inst_count, 6), np.float32)
+    pad = np.zeros((inst_count, 6), np.float32)
+    lib().or_inst_bounds(_p(mats), C.c_uint32(inst_base), C.c_uint32(inst_count), _p(nodes), _p(world), _p(b), _p(pad))
+    return b, pad
+
+
+def build_sah_tlas(padded: np.ndarray) -> np.ndarray:
+    """The device SAH TLAS restated: u32 [2n-1, 8] walk-format nodes."""
+    padded = np.ascontiguousarray(padded, np.float32).reshape(-1, 6)
+    n = padded.shape[0]
+    out = np.zeros((2 * n - 1, 8), np.uint32)
+    assert lib().or_build_sah_tlas(_p(padded), C.c_uint32(n), _p(out))
+    return out
+
+
+def skin(rest6: np.ndarray, joints: np.ndarray, weights: np.ndarray, bones: np.ndarray) -> np.ndarray:
+    """Linear-blend skinning in float32: rest6 [V, 6] (position, normal), joints u16 [V, 4],
+    weights [V, 4], bones [B, 3, 4] -> [V, 6]."""
+    rest6 = np.ascontiguousarray(rest6, np.float32).reshape(-1, 6)
+    j = np.ascontiguousarray(joints, np.uint16).reshape(-1, 4)
+    w = np.ascontiguousarray(weights, np.float32).reshape(-1, 4)
+    b = np.ascontiguousarray(bones, np.float32).reshape(-1, 12)
+    assert int(j.max(initial=0)) < b.shape[0]
+    out = np.zeros_like(rest6)
+    lib().or_skin(_p(rest6), _p(j), _p(w), C.c_uint32(rest6.shape[0]), _p(b), _p(out))
+    return out
+
+
 # ---------------------------------------------------------------- differentiable mode
 def albedo_map(scene) -> tuple[np.ndarray, int]:
     """Per TLAS slot: (offset into the albedo array, first pool triangle of its BLAS).

@@ -181,3 +181,210 @@ int or_refit_tlas(float *tlas, uint32_t node_count, const float *inst_bounds, ui
     }
     return 1;
 }
+
+/* ---------------- per-frame TLAS (exact; precision-independent) ---------------- */
+
+/* World AABB of every TLAS slot: Box3.applyMatrix4 of the instance's BLAS root box in
+ * double, each corner divided by w (js/spectral_scene.js:1266-1285, three.js
+ * Vector3.applyMatrix4); an inactive slot is the point at its translation
+ * (js/spectral_scene.js:1279-1284). world is column-major 4x4 per slot. padded gets the
+ * box inflated per axis by 1e-5 * (|extent| + |min| + |max|) + 1e-6 in float, the input
+ * of the SAH build below. */
+void or_inst_bounds(const float *mats, uint32_t inst_base, uint32_t n, const uint32_t *nodes, const double *world,
+                    float *bounds, float *padded) {
+    for (uint32_t i = 0; i < n; i++) {
+        const float *rec = mats + inst_base + (size_t)i * 28;
+        const double *e = world + (size_t)i * 16;
+        float out[6];
+        if (!(rec[15] > 0.5f)) {
+            for (int a = 0; a < 3; a++) out[a] = out[3 + a] = (float)e[12 + a];
+        } else {
+            const float *rb = (const float *)(nodes + (size_t)(uint32_t)rec[12] * 8);
+            double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int c = 0; c < 8; c++) {
+                const double x = (c & 1) ? rb[3] : rb[0], y = (c & 2) ? rb[4] : rb[1], z = (c & 4) ? rb[5] : rb[2];
+                const double w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
+                const double p[3] = {(e[0] * x + e[4] * y + e[8] * z + e[12]) * w,
+                                     (e[1] * x + e[5] * y + e[9] * z + e[13]) * w,
+                                     (e[2] * x + e[6] * y + e[10] * z + e[14]) * w};
+                for (int a = 0; a < 3; a++) {
+                    if (p[a] < mn[a]) mn[a] = p[a];
+                    if (mx[a] < p[a]) mx[a] = p[a];
+                }
+            }
+            for (int a = 0; a < 3; a++) {
+                out[a] = (float)mn[a];
+                out[3 + a] = (float)mx[a];
+            }
+        }
+        for (int a = 0; a < 6; a++) bounds[(size_t)i * 6 + a] = out[a];
+        for (int a = 0; a < 3; a++) {
+            const float pad = 1e-5f * (fabsf(out[3 + a] - out[a]) + fabsf(out[a]) + fabsf(out[3 + a])) + 1e-6f;
+            padded[(size_t)i * 6 + a] = out[a] - pad;
+            padded[(size_t)i * 6 + 3 + a] = out[3 + a] + pad;
+        }
+    }
+}
+
+/* The SAH TLAS that the library builds on the device every frame (no reference
+ * counterpart: the reference TLAS is a frozen median split, js/spectral_scene.js:973-1003).
+ * Restated here top-down and recursively so the device's level-parallel build can be
+ * compared word for word: one slot per leaf, 32 centroid bins per axis with extent > 0,
+ * SAH cost area(L)·|L| + area(R)·|R| in double with the first minimum over (axis, bin),
+ * a stable partition, the middle split when no axis has extent; depth-first node order,
+ * miss = first node after the subtree. out is u32[8 (2n-1)] in the walk format
+ * {min.xyz, miss}, {max.xyz, payload = 1 << 24 | slot, or 0xFFFFFFFF for interiors}. */
+typedef struct {
+    float mn[3], mx[3];
+} or_box;
+
+static void or_box_empty(or_box *b) {
+    for (int a = 0; a < 3; a++) {
+        b->mn[a] = INFINITY;
+        b->mx[a] = -INFINITY;
+    }
+}
+static void or_box_grow(or_box *b, const float *mn, const float *mx) {
+    for (int a = 0; a < 3; a++) {
+        if (mn[a] < b->mn[a]) b->mn[a] = mn[a];
+        if (b->mx[a] < mx[a]) b->mx[a] = mx[a];
+    }
+}
+static double or_box_area(const or_box *b) {
+    double d[3];
+    for (int a = 0; a < 3; a++) {
+        d[a] = (double)b->mx[a] - (double)b->mn[a];
+        if (!(d[a] >= 0)) return 0.0;
+    }
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+static int or_bin(float c, float cmn, float ext) {
+    int k = (int)((c - cmn) / ext * 32);
+    return k < 0 ? 0 : (k > 31 ? 31 : k);
+}
+
+static uint32_t or_sah_emit(const float *bnd, uint32_t *ids, uint32_t *tmp, uint32_t lo, uint32_t hi, uint32_t pos,
+                            uint32_t *out) {
+    or_box b, cb;
+    or_box_empty(&b);
+    or_box_empty(&cb);
+    for (uint32_t i = lo; i < hi; i++) {
+        const float *bb = bnd + (size_t)ids[i] * 6;
+        float c[3];
+        for (int a = 0; a < 3; a++) c[a] = 0.5f * (bb[a] + bb[3 + a]);
+        or_box_grow(&b, bb, bb + 3);
+        or_box_grow(&cb, c, c);
+    }
+    uint32_t *node = out + (size_t)pos * 8;
+    memcpy(node, b.mn, 12);
+    memcpy(node + 4, b.mx, 12);
+    const uint32_t m = hi - lo;
+    if (m == 1) {
+        node[3] = pos + 1;
+        node[7] = (1u << 24) | (ids[lo] & 0x00FFFFFFu);
+        return pos + 1;
+    }
+    double best = INFINITY;
+    int best_axis = -1, best_bin = -1;
+    for (int axis = 0; axis < 3; axis++) {
+        const float ext = cb.mx[axis] - cb.mn[axis];
+        if (!(ext > 0)) continue;
+        or_box bins[32];
+        uint32_t cnt[32] = {0}, rc[32];
+        double ra[32];
+        for (int k = 0; k < 32; k++) or_box_empty(&bins[k]);
+        for (uint32_t i = lo; i < hi; i++) {
+            const float *bb = bnd + (size_t)ids[i] * 6;
+            const int k = or_bin(0.5f * (bb[axis] + bb[3 + axis]), cb.mn[axis], ext);
+            cnt[k]++;
+            or_box_grow(&bins[k], bb, bb + 3);
+        }
+        or_box acc;
+        or_box_empty(&acc);
+        uint32_t c = 0;
+        for (int k = 31; k > 0; k--) {
+            or_box_grow(&acc, bins[k].mn, bins[k].mx);
+            c += cnt[k];
+            ra[k] = or_box_area(&acc);
+            rc[k] = c;
+        }
+        or_box lacc;
+        or_box_empty(&lacc);
+        uint32_t lc = 0;
+        for (int k = 0; k < 31; k++) {
+            or_box_grow(&lacc, bins[k].mn, bins[k].mx);
+            lc += cnt[k];
+            if (!lc || !rc[k + 1]) continue;
+            const double cost = or_box_area(&lacc) * lc + ra[k + 1] * rc[k + 1];
+            if (cost < best) {
+                best = cost;
+                best_axis = axis;
+                best_bin = k;
+            }
+        }
+    }
+    uint32_t mid = lo + m / 2;
+    if (best_axis >= 0) {
+        const float ext = cb.mx[best_axis] - cb.mn[best_axis];
+        uint32_t nl = 0, nr = 0;
+        for (uint32_t i = lo; i < hi; i++) {
+            const float *bb = bnd + (size_t)ids[i] * 6;
+            if (or_bin(0.5f * (bb[best_axis] + bb[3 + best_axis]), cb.mn[best_axis], ext) <= best_bin)
+                ids[lo + nl++] = ids[i];
+            else
+                tmp[nr++] = ids[i];
+        }
+        memcpy(ids + lo + nl, tmp, nr * sizeof(uint32_t));
+        if (nl > 0 && nl < m) mid = lo + nl;
+    }
+    node[7] = 0xFFFFFFFFu;
+    uint32_t next = or_sah_emit(bnd, ids, tmp, lo, mid, pos + 1, out);
+    next = or_sah_emit(bnd, ids, tmp, mid, hi, next, out);
+    node[3] = next;
+    return next;
+}
+
+int or_build_sah_tlas(const float *padded, uint32_t n, uint32_t *out) {
+    if (n == 0) return 0;
+    uint32_t *ids = (uint32_t *)malloc((size_t)n * 4), *tmp = (uint32_t *)malloc((size_t)n * 4);
+    if (!ids || !tmp) {
+        free(ids);
+        free(tmp);
+        return 0;
+    }
+    for (uint32_t i = 0; i < n; i++) ids[i] = i;
+    const uint32_t end = or_sah_emit(padded, ids, tmp, 0, n, 0, out);
+    free(ids);
+    free(tmp);
+    return end == 2 * n - 1;
+}
+
+/* Linear-blend skinning of a vertex set (textbook LBS; the reference receives skinned
+ * positions from its host — CPU skinning — and re-gathers them in updateDeforms,
+ * js/spectral_scene.js:1831-1927). bones are 3x4 rows per bone; for each vertex
+ * p' = sum_q w_q (B_q p), n' = normalize(sum_q w_q (B_q[3x3] n)), evaluated in float in
+ * the order written (q = 0..3; row dot products left to right, translation last). rest
+ * is 6 floats (position, normal) per vertex; out is 6 floats per vertex. */
+void or_skin(const float *rest, const uint16_t *joints, const float *weights, uint32_t n, const float *bones,
+             float *out) {
+    for (uint32_t v = 0; v < n; v++) {
+        const float *p = rest + (size_t)v * 6, *nr = p + 3;
+        float acc[3] = {0.f, 0.f, 0.f}, nacc[3] = {0.f, 0.f, 0.f};
+        for (int q = 0; q < 4; q++) {
+            const float w = weights[(size_t)v * 4 + q];
+            const float *B = bones + (size_t)joints[(size_t)v * 4 + q] * 12;
+            for (int r = 0; r < 3; r++) {
+                const float *row = B + r * 4;
+                const float t = row[0] * p[0] + row[1] * p[1] + row[2] * p[2] + row[3];
+                const float tn = row[0] * nr[0] + row[1] * nr[1] + row[2] * nr[2];
+                acc[r] = acc[r] + w * t;
+                nacc[r] = nacc[r] + w * tn;
+            }
+        }
+        const float len = sqrtf(nacc[0] * nacc[0] + nacc[1] * nacc[1] + nacc[2] * nacc[2]);
+        for (int r = 0; r < 3; r++) {
+            out[(size_t)v * 6 + r] = acc[r];
+            out[(size_t)v * 6 + 3 + r] = len > 0.f ? nacc[r] / len : nacc[r];
+        }
+    }
+}

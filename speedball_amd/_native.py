@@ -26,7 +26,7 @@ FIELD_KEEP_HIT_IDS = 1
 
 BUF_BVH_NODES, BUF_TRI_INDEX, BUF_VERTEX_DATA, BUF_TRI_MATERIAL, BUF_MATERIALS, BUF_LIGHTS = 0, 1, 2, 3, 4, 5
 BUF_IRR_STATE, BUF_DEPTH_STATE, BUF_META, BUF_IRR_ATLAS, BUF_DEPTH_ATLAS = 10, 11, 12, 13, 14
-BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT, BUF_TRI_PACKED = 15, 16, 17, 18
+BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT, BUF_TRI_PACKED, BUF_INST_BOUNDS, BUF_TLAS_SAH = 15, 16, 17, 18, 19, 20
 
 # every symbol include/sbgi.h declares
 EXPORTS = [
@@ -37,6 +37,7 @@ EXPORTS = [
     "sbgi_sb_add_light", "sbgi_sb_finalize", "sbgi_sb_tlas_order", "sbgi_sb_set_transform",
     "sbgi_sb_last_error", "sbgi_sb_destroy", "sbgi_diff_param_count", "sbgi_diff_albedo_map", "sbgi_diff_forward",
     "sbgi_diff_adjoint", "sbgi_diff_timing", "sbgi_get_walk_tuning", "sbgi_set_walk_tuning", "sbgi_autotune",
+    "sbgi_set_skins", "sbgi_skin", "sbgi_edit_timing", "sbgi_sb_mesh_ranges",
 ]
 
 
@@ -88,6 +89,11 @@ class Stats(C.Structure):
                 ("ms_walk_primary", C.c_float), ("ms_emit", C.c_float), ("ms_walk_shadow", C.c_float),
                 ("ms_resolve", C.c_float), ("shadow_blas_nodes", C.c_uint64), ("shadow_tlas_nodes", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64)]
+
+
+class SkinSet(C.Structure):
+    _fields_ = [("vertex_start", C.c_uint32), ("vertex_count", C.c_uint32), ("blas_root", C.c_uint32),
+                ("blas_end", C.c_uint32), ("joints", C.c_void_p), ("weights", C.c_void_p)]
 
 
 class WalkTuning(C.Structure):
@@ -147,6 +153,9 @@ def lib() -> C.CDLL:
         "sbgi_get_walk_tuning": (i32, [vp, C.POINTER(WalkTuning)]),
         "sbgi_set_walk_tuning": (i32, [vp, C.POINTER(WalkTuning)]),
         "sbgi_autotune": (i32, [vp, C.POINTER(TickParams), C.POINTER(WalkTuning), C.POINTER(C.c_float)]),
+        "sbgi_set_skins": (i32, [vp, C.POINTER(SkinSet), u32, u32]),
+        "sbgi_skin": (i32, [vp, vp, u32, i32]),
+        "sbgi_edit_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "sbgi_last_error": (C.c_char_p, [vp]),
         "sbgi_destroy": (i32, [vp]),
         "sbgi_sb_create": (i32, [C.POINTER(vp)]),
@@ -157,6 +166,7 @@ def lib() -> C.CDLL:
         "sbgi_sb_finalize": (i32, [vp, C.POINTER(SceneView)]),
         "sbgi_sb_tlas_order": (i32, [vp, vp, u32]),
         "sbgi_sb_set_transform": (i32, [vp, u32, vp, C.POINTER(u32)]),
+        "sbgi_sb_mesh_ranges": (i32, [vp, u32, vp]),
         "sbgi_sb_last_error": (C.c_char_p, [vp]),
         "sbgi_sb_destroy": (i32, [vp]),
     }

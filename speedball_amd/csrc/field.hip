@@ -55,9 +55,39 @@ struct sbgi_field {
     uint32_t *d_walk_nodes = nullptr, *d_walk_perm = nullptr;
     uint4 *d_tlas_sah = nullptr;
     uint32_t tlas_sah_count = 0;
+    // per-frame TLAS on the device: world matrices (double, column-major 4x4 per slot),
+    // the inflated bounds the SAH build reads, its scratch, and the staging area of
+    // instance-record uploads {worlds f64[16 n], records f32[28 n], slots u32[n]}
+    double *d_world = nullptr;
+    float *d_inst_pad = nullptr;
+    void *d_tlas_scratch = nullptr;
+    void *d_inst_stage = nullptr;
+    // skinned vertex sets (sbgi_set_skins): pool vertex per skinned vertex, rest pose
+    // {pos, 0}, {normal, 0}, joints (4 x u16), weights, the bone palette, the leaves of the
+    // skinned BLASes and their node / triangle ranges
+    uint32_t *d_skin_vtx = nullptr, *d_skin_leaves = nullptr;
+    float4 *d_skin_rest = nullptr, *d_skin_w = nullptr, *d_bones = nullptr;
+    uint2 *d_skin_joints = nullptr, *d_skin_nranges = nullptr, *d_skin_tranges = nullptr;
+    uint32_t skin_vtx_n = 0, skin_bones_n = 0, skin_leaf_n = 0, skin_ranges_n = 0, skin_tranges_n = 0;
+    uint32_t skin_nmax = 0, skin_tmax = 0;
+    // bounds of d_nodes are newer than h_nodes after a device refit (structure is equal)
+    bool nodes_dirty = false;
+    // pinned upload staging ring (sbgi_set_instance_transforms, sbgi_skin)
+    static constexpr int kStageSlots = 4;
+    void *stage_host[kStageSlots] = {};
+    size_t stage_cap[kStageSlots] = {};
+    hipEvent_t stage_done[kStageSlots] = {};
+    int stage_next = 0;
+    // device timing of the last scene edits: kev[0..2] before skinning, after it, after the
+    // BLAS refit and walk-copy update (sbgi_skin, sbgi_refit); kev[3..4] around the
+    // instance-record scatter, instance bounds and TLAS (sbgi_set_instance_transforms, and
+    // sbgi_skin / sbgi_refit when they update the TLAS)
+    hipEvent_t kev[5] = {};
+    bool kev_blas = false, kev_tlas = false;
+    uint32_t skin_span_lo = 0, skin_span_hi = 0;
     // scene (host mirror, reference layouts)
     std::vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
-    std::vector<float> h_vdata, h_mats, h_lights, h_inst_bounds;
+    std::vector<float> h_vdata, h_mats, h_lights;
     std::vector<double> h_world;  // per TLAS slot, column-major 4x4
     sbgi_scene_view view{};
     bool has_scene = false;
@@ -127,6 +157,19 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+void free_skins(sbgi_field *f) {
+    dfree(f->d_skin_vtx);
+    dfree(f->d_skin_leaves);
+    dfree(f->d_skin_rest);
+    dfree(f->d_skin_w);
+    dfree(f->d_bones);
+    dfree(f->d_skin_joints);
+    dfree(f->d_skin_nranges);
+    dfree(f->d_skin_tranges);
+    f->skin_vtx_n = f->skin_bones_n = f->skin_leaf_n = f->skin_ranges_n = f->skin_tranges_n = 0;
+    f->skin_nmax = f->skin_tmax = 0;
+}
+
 void free_scene(sbgi_field *f) {
     dfree(f->d_nodes);
     dfree(f->d_tri_index);
@@ -146,6 +189,11 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_walk_nodes);
     dfree(f->d_walk_perm);
     dfree(f->d_tlas_sah);
+    dfree(f->d_world);
+    dfree(f->d_inst_pad);
+    dfree(f->d_tlas_scratch);
+    dfree(f->d_inst_stage);
+    free_skins(f);
     dfree(f->d_albedo);
     dfree(f->d_light_scale);
     dfree(f->d_gE);
@@ -157,6 +205,8 @@ void free_scene(sbgi_field *f) {
     f->diff_ready = f->diff_valid = false;
     f->tlas_sah_count = 0;
     f->tw.queue.capacity = 0;
+    f->nodes_dirty = false;
+    f->kev_blas = f->kev_tlas = false;
     f->has_scene = false;
 }
 
@@ -276,31 +326,6 @@ void build_tlas_parents(const float *tl, uint32_t n, std::vector<int32_t> &paren
     }
 }
 
-// Box3.applyMatrix4 in double of the BLAS root bounds (js/spectral_scene.js:1301-1306)
-void inst_aabb(const float *root_node, const double *e, bool active, float *out6) {
-    if (!active) {
-        for (int a = 0; a < 3; a++) out6[a] = out6[3 + a] = (float)e[12 + a];
-        return;
-    }
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int c = 0; c < 8; c++) {
-        double x = (c & 1) ? root_node[3] : root_node[0];
-        double y = (c & 2) ? root_node[4] : root_node[1];
-        double z = (c & 4) ? root_node[5] : root_node[2];
-        double w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
-        double p[3] = {(e[0] * x + e[4] * y + e[8] * z + e[12]) * w, (e[1] * x + e[5] * y + e[9] * z + e[13]) * w,
-                       (e[2] * x + e[6] * y + e[10] * z + e[14]) * w};
-        for (int a = 0; a < 3; a++) {
-            mn[a] = std::min(mn[a], p[a]);
-            mx[a] = std::max(mx[a], p[a]);
-        }
-    }
-    for (int a = 0; a < 3; a++) {
-        out6[a] = (float)mn[a];
-        out6[3 + a] = (float)mx[a];
-    }
-}
-
 void invert_affine_rows(const float *rows12, double *world16) {
     // world = inverse of [rows | 0 0 0 1], column-major output
     double m[16] = {rows12[0], rows12[4], rows12[8], 0, rows12[1], rows12[5], rows12[9], 0,
@@ -322,38 +347,64 @@ void invert_affine_rows(const float *rows12, double *world16) {
         world16[12 + r] = -(inv[r * 3 + 0] * m[12] + inv[r * 3 + 1] * m[13] + inv[r * 3 + 2] * m[14]);
 }
 
-// SAH TLAS over the current instance bounds (inflated by a relative 1e-5 so rounding of the
-// recovered world matrices can never make a box smaller than its instance).
-int32_t upload_sah_tlas(sbgi_field *f) {
+// Instance world bounds from the device node pool and world matrices, then the TLAS: in
+// MT_ORDERED mode the SAH TLAS is rebuilt; in MT_COMPAT mode the reference TLAS is refit
+// bottom-up when refit_compat is set (updateTransforms' refit, js/spectral_scene.js:
+// 1402-1514; refitFlatTlasRange, js/gi_refit.js:245-292). Stream-ordered, no host sync.
+int32_t device_tlas(sbgi_field *f, bool refit_compat) {
     const sbgi_scene_view &v = f->view;
-    std::vector<float> b(f->h_inst_bounds);
-    for (uint32_t i = 0; i < v.inst_count; i++) {
-        float *bb = b.data() + (size_t)i * 6;
-        for (int a = 0; a < 3; a++) {
-            const float pad = 1e-5f * (std::fabs(bb[3 + a] - bb[a]) + std::fabs(bb[a]) + std::fabs(bb[3 + a])) + 1e-6f;
-            bb[a] -= pad;
-            bb[3 + a] += pad;
-        }
+    HIP_OK(f, launch_inst_bounds(f->d_mats, v.inst_base, v.inst_count, f->d_nodes, f->d_world, f->d_inst_bounds,
+                                 f->d_inst_pad, f->stream));
+    if (ordered(f)) {
+        HIP_OK(f, launch_build_tlas(f->d_inst_pad, v.inst_count, f->d_tlas_scratch, f->d_tlas_sah, f->stream));
+    } else if (refit_compat) {
+        HIP_OK(f, launch_fill_u32(f->d_tlas_arrive, 0u, v.tlas_node_count, f->stream));
+        HIP_OK(f, launch_refit_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_inst_bounds, f->d_tlas_parent,
+                                    f->d_tlas_arrive, f->stream));
+        HIP_OK(f, launch_pack_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_tlas, f->stream));
     }
-    std::vector<float> tl;
-    sbgi::build_sah_tlas(b.data(), v.inst_count, tl);
-    const uint32_t m = (uint32_t)(tl.size() / 12);
-    std::vector<uint4> packed((size_t)m * 2);
-    for (uint32_t i = 0; i < m; i++) {
-        const float *r = tl.data() + (size_t)i * 12;
-        uint32_t u[6];
-        memcpy(u, r, 24);
-        const uint32_t leaf = r[8] < 0.5f ? 0xFFFFFFFFu : (((uint32_t)r[8] << 24) | ((uint32_t)r[7] & 0x00FFFFFFu));
-        packed[(size_t)i * 2] = make_uint4(u[0], u[1], u[2], (uint32_t)r[6]);
-        packed[(size_t)i * 2 + 1] = make_uint4(u[3], u[4], u[5], leaf);
-    }
-    if (m > f->tlas_sah_count) {
-        dfree(f->d_tlas_sah);
-        if (dalloc(&f->d_tlas_sah, (size_t)m * 32) != hipSuccess) return fail(f, SBGI_E_NOMEM, "SAH TLAS allocation");
-    }
-    f->tlas_sah_count = m;
-    HIP_OK(f, hipMemcpyAsync(f->d_tlas_sah, packed.data(), (size_t)m * 32, hipMemcpyHostToDevice, f->stream));
+    return SBGI_OK;
+}
+
+// Bring the host mirror of the node pool up to date after device refits (before host code
+// reads its bounds).
+int32_t sync_nodes(sbgi_field *f) {
+    if (!f->nodes_dirty) return SBGI_OK;
+    HIP_OK(f, hipMemcpyAsync(f->h_nodes.data(), f->d_nodes, f->h_nodes.size() * 4, hipMemcpyDeviceToHost, f->stream));
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    f->nodes_dirty = false;
+    return SBGI_OK;
+}
+
+// A pinned host buffer of at least `bytes` from the staging ring whose previous upload has
+// completed; stage_commit records that completion after the upload is enqueued.
+int32_t stage_acquire(sbgi_field *f, size_t bytes, void **out, int *slot) {
+    const int s = f->stage_next;
+    f->stage_next = (s + 1) % sbgi_field::kStageSlots;
+    if (!f->stage_done[s]) HIP_OK(f, hipEventCreateWithFlags(&f->stage_done[s], hipEventDisableTiming));
+    HIP_OK(f, hipEventSynchronize(f->stage_done[s]));
+    if (f->stage_cap[s] < bytes) {
+        if (f->stage_host[s]) HIP_OK(f, hipHostFree(f->stage_host[s]));
+        f->stage_host[s] = nullptr;
+        f->stage_cap[s] = 0;
+        const size_t cap = std::max<size_t>(bytes, 64 * 1024);
+        if (hipHostMalloc(&f->stage_host[s], cap, hipHostMallocDefault) != hipSuccess)
+            return fail(f, SBGI_E_NOMEM, "pinned staging allocation");
+        f->stage_cap[s] = cap;
+    }
+    *out = f->stage_host[s];
+    *slot = s;
+    return SBGI_OK;
+}
+
+int32_t stage_commit(sbgi_field *f, int slot) {
+    HIP_OK(f, hipEventRecord(f->stage_done[slot], f->stream));
+    return SBGI_OK;
+}
+
+int32_t record_kev(sbgi_field *f, int i) {
+    if (!f->kev[i]) HIP_OK(f, hipEventCreate(&f->kev[i]));
+    HIP_OK(f, hipEventRecord(f->kev[i], f->stream));
     return SBGI_OK;
 }
 
@@ -548,19 +599,20 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     build_tlas_parents(f->h_mats.data() + v.tlas_base, v.tlas_node_count, tparent);
     // world matrices recovered from the inverse rows; exact ones arrive via sbgi_set_instance_transforms
     f->h_world.assign((size_t)v.inst_count * 16, 0.0);
-    f->h_inst_bounds.assign((size_t)v.inst_count * 6, 0.f);
-    for (uint32_t i = 0; i < v.inst_count; i++) {
-        const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
-        invert_affine_rows(r, f->h_world.data() + (size_t)i * 16);
-        const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)r[12] * 8);
-        inst_aabb(root, f->h_world.data() + (size_t)i * 16, r[15] > 0.5f, f->h_inst_bounds.data() + (size_t)i * 6);
-    }
+    for (uint32_t i = 0; i < v.inst_count; i++)
+        invert_affine_rows(f->h_mats.data() + v.inst_base + (size_t)i * 28, f->h_world.data() + (size_t)i * 16);
+    const size_t stage_bytes = (size_t)v.inst_count * (16 * sizeof(double) + 28 * sizeof(float) + sizeof(uint32_t));
     if (dalloc(&f->d_nodes, f->h_nodes.size() * 4) != hipSuccess ||
         dalloc(&f->d_tri_index, f->h_tri_index.size() * 4) != hipSuccess ||
         dalloc(&f->d_tri_mat, f->h_tri_mat.size() * 4) != hipSuccess ||
         dalloc(&f->d_vdata, f->h_vdata.size() * 4) != hipSuccess || dalloc(&f->d_mats, f->h_mats.size() * 4) != hipSuccess ||
         dalloc(&f->d_lights, std::max<size_t>(17, f->h_lights.size()) * 4) != hipSuccess ||
-        dalloc(&f->d_inst_bounds, f->h_inst_bounds.size() * 4) != hipSuccess ||
+        dalloc(&f->d_inst_bounds, (size_t)v.inst_count * 24) != hipSuccess ||
+        dalloc(&f->d_inst_pad, (size_t)v.inst_count * 24) != hipSuccess ||
+        dalloc(&f->d_world, f->h_world.size() * sizeof(double)) != hipSuccess ||
+        dalloc(&f->d_inst_stage, stage_bytes) != hipSuccess ||
+        (ordered(f) && (dalloc(&f->d_tlas_sah, (2 * (size_t)v.inst_count - 1) * 32) != hipSuccess ||
+                        dalloc(&f->d_tlas_scratch, tlas_build_scratch_bytes(v.inst_count)) != hipSuccess)) ||
         dalloc(&f->d_tri, (size_t)v.tri_count * 48) != hipSuccess ||
         dalloc(&f->d_inst, (size_t)v.inst_count * 64) != hipSuccess ||
         dalloc(&f->d_tlas, (size_t)v.tlas_node_count * 32) != hipSuccess ||
@@ -587,13 +639,17 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     HIP_OK(f, up(f->d_vdata, f->h_vdata.data(), f->h_vdata.size() * 4));
     HIP_OK(f, up(f->d_mats, f->h_mats.data(), f->h_mats.size() * 4));
     HIP_OK(f, up(f->d_lights, f->h_lights.data(), f->h_lights.size() * 4));
-    HIP_OK(f, up(f->d_inst_bounds, f->h_inst_bounds.data(), f->h_inst_bounds.size() * 4));
+    HIP_OK(f, up(f->d_world, f->h_world.data(), f->h_world.size() * sizeof(double)));
     HIP_OK(f, up(f->d_parent, parent.data(), parent.size() * 4));
     HIP_OK(f, up(f->d_tlas_parent, tparent.data(), tparent.size() * 4));
     rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
+    // instance bounds for later refits; in MT_ORDERED mode also the first SAH TLAS. The
+    // reference TLAS of MT_COMPAT mode is used as given.
+    f->tlas_sah_count = ordered(f) ? 2 * v.inst_count - 1 : 0;
+    if ((rc = device_tlas(f, false)) != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
-    if ((rc = upload_walk_nodes(f)) != SBGI_OK || (ordered(f) && (rc = upload_sah_tlas(f)) != SBGI_OK)) {
+    if ((rc = upload_walk_nodes(f)) != SBGI_OK) {
         free_scene(f);
         return rc;
     }
@@ -616,16 +672,24 @@ int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges
         case SBGI_BUF_LIGHTS: dev = f->d_lights; host = f->h_lights.data(); elems = f->h_lights.size(); break;
         default: return fail(f, SBGI_E_INVALID, "buffer id not updatable");
     }
+    for (uint32_t r = 0; r < n_ranges; r++)
+        if ((uint64_t)ranges[2 * r] + ranges[2 * r + 1] > elems) return fail(f, SBGI_E_INVALID, "update range out of bounds");
     HIP_OK(f, hipSetDevice(f->desc.device));
-    HIP_OK(f, hipStreamSynchronize(f->stream));
-    std::vector<char> backup(static_cast<char *>(host), static_cast<char *>(host) + elems * esz);
+    // vertex positions and light records cannot break the layout: no validation, no host
+    // sync (pageable uploads are staged before the call returns)
+    const bool structural = buf_id != SBGI_BUF_VERTEX_DATA && buf_id != SBGI_BUF_LIGHTS;
+    int32_t rc = SBGI_OK;
+    std::vector<char> backup;
+    if (structural) {
+        if (buf_id == SBGI_BUF_BVH_NODES && (rc = sync_nodes(f)) != SBGI_OK) return rc;
+        HIP_OK(f, hipStreamSynchronize(f->stream));
+        backup.assign(static_cast<char *>(host), static_cast<char *>(host) + elems * esz);
+    }
     for (uint32_t r = 0; r < n_ranges; r++) {
         uint64_t st = ranges[2 * r], cnt = ranges[2 * r + 1];
-        if (st + cnt > elems) return fail(f, SBGI_E_INVALID, "update range out of bounds");
         memcpy(static_cast<char *>(host) + st * esz, static_cast<const char *>(src) + st * esz, cnt * esz);
     }
-    int32_t rc = validate_scene(f);
-    if (rc != SBGI_OK) {
+    if (structural && (rc = validate_scene(f)) != SBGI_OK) {
         memcpy(host, backup.data(), backup.size());
         return rc;
     }
@@ -635,21 +699,23 @@ int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges
             HIP_OK(f, hipMemcpyAsync(static_cast<char *>(dev) + st * esz, static_cast<char *>(host) + st * esz,
                                      cnt * esz, hipMemcpyHostToDevice, f->stream));
     }
-    rc = repack_all(f);
-    if (rc != SBGI_OK) return rc;
-    HIP_OK(f, hipStreamSynchronize(f->stream));
+    if (buf_id == SBGI_BUF_LIGHTS) return SBGI_OK;
+    if ((rc = repack_all(f)) != SBGI_OK) return rc;
     if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_MATERIALS) {
+        if ((rc = sync_nodes(f)) != SBGI_OK) return rc;
+        HIP_OK(f, hipStreamSynchronize(f->stream));
         // walk-format node copies depend on the pool and, in ordered mode, on the root set
         if ((rc = upload_walk_nodes(f)) != SBGI_OK) return rc;
-        if (ordered(f) && buf_id == SBGI_BUF_MATERIALS) {
-            for (uint32_t i = 0; i < f->view.inst_count; i++) {
-                const float *r = f->h_mats.data() + f->view.inst_base + (size_t)i * 28;
-                invert_affine_rows(r, f->h_world.data() + (size_t)i * 16);
-                const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)r[12] * 8);
-                inst_aabb(root, f->h_world.data() + (size_t)i * 16, r[15] > 0.5f, f->h_inst_bounds.data() + (size_t)i * 6);
-            }
-            if ((rc = upload_sah_tlas(f)) != SBGI_OK) return rc;
+        if (buf_id == SBGI_BUF_MATERIALS) {
+            for (uint32_t i = 0; i < f->view.inst_count; i++)
+                invert_affine_rows(f->h_mats.data() + f->view.inst_base + (size_t)i * 28,
+                                   f->h_world.data() + (size_t)i * 16);
+            HIP_OK(f, hipMemcpyAsync(f->d_world, f->h_world.data(), f->h_world.size() * sizeof(double),
+                                     hipMemcpyHostToDevice, f->stream));
         }
+        // instance bounds follow the new roots / records; the MT_ORDERED SAH TLAS is rebuilt
+        // (a caller-supplied reference TLAS stands as uploaded)
+        if ((rc = device_tlas(f, false)) != SBGI_OK) return rc;
     }
     return SBGI_OK;
 }
@@ -673,37 +739,27 @@ int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges
             }
         }
     }
+    int32_t rc = SBGI_OK;
+    if (n_ranges && ((rc = record_kev(f, 0)) != SBGI_OK || (rc = record_kev(f, 1)) != SBGI_OK)) return rc;
     for (uint32_t r = 0; r < n_ranges; r++) {
         uint32_t root = blas_ranges[2 * r], end = blas_ranges[2 * r + 1];
-        HIP_OK(f, hipMemsetAsync(f->d_arrive + root, 0, (size_t)(end - root) * 4, f->stream));
+        HIP_OK(f, launch_fill_u32(f->d_arrive + root, 0u, end - root, f->stream));
         HIP_OK(f, launch_refit_blas(f->d_nodes, f->d_tri_index, f->d_vdata, f->d_parent, f->d_arrive, root, end,
                                     f->stream));
     }
     if (n_ranges) {
         HIP_OK(f, launch_oct_gather(f->d_walk_nodes, f->d_nodes, f->d_walk_perm, (ordered(f) ? 8u : 1u) * v.node_count,
                                     f->stream));
-        HIP_OK(f, hipMemcpyAsync(f->h_nodes.data(), f->d_nodes, f->h_nodes.size() * 4, hipMemcpyDeviceToHost,
-                                 f->stream));
-        HIP_OK(f, hipStreamSynchronize(f->stream));
+        f->nodes_dirty = true;
+        if ((rc = record_kev(f, 2)) != SBGI_OK) return rc;
+        f->kev_blas = true;
     }
     if (refit_tlas) {
-        for (uint32_t i = 0; i < v.inst_count; i++) {
-            const float *r = f->h_mats.data() + v.inst_base + (size_t)i * 28;
-            const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)r[12] * 8);
-            inst_aabb(root, f->h_world.data() + (size_t)i * 16, r[15] > 0.5f, f->h_inst_bounds.data() + (size_t)i * 6);
-        }
-        HIP_OK(f, hipMemcpyAsync(f->d_inst_bounds, f->h_inst_bounds.data(), f->h_inst_bounds.size() * 4,
-                                 hipMemcpyHostToDevice, f->stream));
-        HIP_OK(f, hipMemsetAsync(f->d_tlas_arrive, 0, (size_t)v.tlas_node_count * 4, f->stream));
-        HIP_OK(f, launch_refit_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_inst_bounds, f->d_tlas_parent,
-                                    f->d_tlas_arrive, f->stream));
-        HIP_OK(f, hipMemcpyAsync(f->h_mats.data() + v.tlas_base, f->d_mats + v.tlas_base,
-                                 (size_t)v.tlas_node_count * 48, hipMemcpyDeviceToHost, f->stream));
+        if ((rc = record_kev(f, 3)) != SBGI_OK || (rc = device_tlas(f, true)) != SBGI_OK ||
+            (rc = record_kev(f, 4)) != SBGI_OK)
+            return rc;
+        f->kev_tlas = true;
     }
-    int32_t rc = repack_all(f);
-    if (rc != SBGI_OK) return rc;
-    HIP_OK(f, hipStreamSynchronize(f->stream));
-    if (ordered(f) && refit_tlas && (rc = upload_sah_tlas(f)) != SBGI_OK) return rc;
     return SBGI_OK;
 }
 
@@ -712,10 +768,19 @@ int32_t sbgi_set_instance_transforms(sbgi_field *f, const uint32_t *slots, uint3
     if (!f || (n && (!slots || !world))) return SBGI_E_INVALID;
     if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
     const sbgi_scene_view &v = f->view;
+    if (n > v.inst_count) return fail(f, SBGI_E_INVALID, "more slots than instances");
     for (uint32_t q = 0; q < n; q++)
         if (slots[q] >= v.inst_count) return fail(f, SBGI_E_INVALID, "instance slot out of range");
     HIP_OK(f, hipSetDevice(f->desc.device));
-    HIP_OK(f, hipStreamSynchronize(f->stream));
+    // records on the host mirror, then one staged upload {worlds, records, slots} and a scatter
+    void *stage = nullptr;
+    int slot = 0;
+    const size_t wb = (size_t)n * 16 * sizeof(double), rb = (size_t)n * 28 * sizeof(float);
+    int32_t rc = stage_acquire(f, wb + rb + (size_t)n * sizeof(uint32_t), &stage, &slot);
+    if (rc != SBGI_OK) return rc;
+    double *sw = static_cast<double *>(stage);
+    float *sr = reinterpret_cast<float *>(static_cast<char *>(stage) + wb);
+    uint32_t *ss = reinterpret_cast<uint32_t *>(static_cast<char *>(stage) + wb + rb);
     for (uint32_t q = 0; q < n; q++) {
         const uint32_t s = slots[q];
         const double *e = world + (size_t)q * 16;
@@ -728,22 +793,169 @@ int32_t sbgi_set_instance_transforms(sbgi_field *f, const uint32_t *slots, uint3
         for (int i = 0; i < 12; i++) rec[i] = (float)inv[src[i]];
         rec[14] = sbgi::det4(e) < 0 ? -1.f : 1.f;
         if (active) rec[15] = active[q] ? 1.f : 0.f;
-        const float *root = reinterpret_cast<const float *>(f->h_nodes.data() + (size_t)rec[12] * 8);
-        inst_aabb(root, e, rec[15] > 0.5f, f->h_inst_bounds.data() + (size_t)s * 6);
-        HIP_OK(f, hipMemcpyAsync(f->d_mats + v.inst_base + (size_t)s * 28, rec, 28 * 4, hipMemcpyHostToDevice,
-                                 f->stream));
+        memcpy(sw + (size_t)q * 16, e, 16 * sizeof(double));
+        memcpy(sr + (size_t)q * 28, rec, 28 * sizeof(float));
+        ss[q] = s;
     }
-    HIP_OK(f, hipMemcpyAsync(f->d_inst_bounds, f->h_inst_bounds.data(), f->h_inst_bounds.size() * 4,
-                             hipMemcpyHostToDevice, f->stream));
-    HIP_OK(f, hipMemsetAsync(f->d_tlas_arrive, 0, (size_t)v.tlas_node_count * 4, f->stream));
-    HIP_OK(f, launch_refit_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_inst_bounds, f->d_tlas_parent,
-                                f->d_tlas_arrive, f->stream));
-    HIP_OK(f, hipMemcpyAsync(f->h_mats.data() + v.tlas_base, f->d_mats + v.tlas_base, (size_t)v.tlas_node_count * 48,
-                             hipMemcpyDeviceToHost, f->stream));
-    HIP_OK(f, launch_pack_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_tlas, f->stream));
-    HIP_OK(f, launch_pack_inst(f->d_mats, v.inst_base, v.inst_count, f->d_inst, f->stream));
+    if ((rc = record_kev(f, 3)) != SBGI_OK) return rc;
+    if (n) {
+        char *dst = static_cast<char *>(f->d_inst_stage);
+        const size_t dwb = (size_t)v.inst_count * 16 * sizeof(double), drb = (size_t)v.inst_count * 28 * sizeof(float);
+        HIP_OK(f, hipMemcpyAsync(dst, sw, wb, hipMemcpyHostToDevice, f->stream));
+        HIP_OK(f, hipMemcpyAsync(dst + dwb, sr, rb, hipMemcpyHostToDevice, f->stream));
+        HIP_OK(f, hipMemcpyAsync(dst + dwb + drb, ss, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+        if ((rc = stage_commit(f, slot)) != SBGI_OK) return rc;
+        HIP_OK(f, launch_scatter_inst(reinterpret_cast<const uint32_t *>(dst + dwb + drb),
+                                      reinterpret_cast<const float *>(dst + dwb), reinterpret_cast<const double *>(dst),
+                                      n, f->d_mats, v.inst_base, f->d_world, f->d_inst, f->stream));
+    }
+    if ((rc = device_tlas(f, true)) != SBGI_OK || (rc = record_kev(f, 4)) != SBGI_OK) return rc;
+    f->kev_tlas = true;
+    return SBGI_OK;
+}
+
+int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets, uint32_t n_bones) {
+    if (!f || (n_sets && !sets)) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    const sbgi_scene_view &v = f->view;
+    if (n_sets && (n_bones == 0 || n_bones > 65536)) return fail(f, SBGI_E_INVALID, "bone count outside 1..65536");
+    // validate: vertex ranges and BLAS ranges in bounds and disjoint, threaded BLAS layout,
+    // joints inside the palette
+    std::vector<std::pair<uint32_t, uint32_t>> vr, nr;
+    uint64_t nv = 0;
+    for (uint32_t s = 0; s < n_sets; s++) {
+        const sbgi_skin_set &k = sets[s];
+        if ((uint64_t)k.vertex_start + k.vertex_count > v.vertex_count || !k.vertex_count)
+            return fail(f, SBGI_E_INVALID, "skin vertex range out of bounds");
+        if (!(k.blas_end > k.blas_root) || k.blas_end > v.node_count)
+            return fail(f, SBGI_E_LAYOUT, "skin BLAS range out of bounds");
+        if (!k.joints || !k.weights) return fail(f, SBGI_E_INVALID, "skin set without joints or weights");
+        for (uint64_t q = 0; q < 4ull * k.vertex_count; q++)
+            if (k.joints[q] >= n_bones) return fail(f, SBGI_E_INVALID, "joint index outside the bone palette");
+        for (uint32_t i = k.blas_root; i < k.blas_end; i++) {
+            const uint32_t *nd = f->h_nodes.data() + (size_t)i * 8;
+            if (nd[7] != 0xFFFFFFFFu) continue;
+            const uint32_t l = i + 1;
+            if (l >= k.blas_end) return fail(f, SBGI_E_LAYOUT, "interior node without a left child in range");
+            const uint32_t rr = f->h_nodes[(size_t)l * 8 + 6];
+            if (!(rr > l && rr < k.blas_end)) return fail(f, SBGI_E_LAYOUT, "right child outside (left, end)");
+        }
+        vr.emplace_back(k.vertex_start, k.vertex_start + k.vertex_count);
+        nr.emplace_back(k.blas_root, k.blas_end);
+        nv += k.vertex_count;
+    }
+    std::sort(vr.begin(), vr.end());
+    std::sort(nr.begin(), nr.end());
+    for (size_t i = 1; i < vr.size(); i++)
+        if (vr[i].first < vr[i - 1].second || nr[i].first < nr[i - 1].second)
+            return fail(f, SBGI_E_INVALID, "skin sets overlap");
+    if (nv >= (1ull << 32)) return fail(f, SBGI_E_CAPACITY, "too many skinned vertices");
+    HIP_OK(f, hipSetDevice(f->desc.device));
     HIP_OK(f, hipStreamSynchronize(f->stream));
-    if (ordered(f)) return upload_sah_tlas(f);
+    free_skins(f);
+    if (!n_sets) return SBGI_OK;
+    // rest pose = the vertexData last provided by the host
+    std::vector<uint32_t> vtx, leaves;
+    std::vector<float4> rest, w;
+    std::vector<uint2> joints, nranges, tranges;
+    vtx.reserve(nv);
+    rest.reserve(2 * nv);
+    w.reserve(nv);
+    joints.reserve(nv);
+    uint32_t nmax = 0, tmax = 0, span_lo = UINT32_MAX, span_hi = 0;
+    for (uint32_t s = 0; s < n_sets; s++) {
+        const sbgi_skin_set &k = sets[s];
+        for (uint32_t q = 0; q < k.vertex_count; q++) {
+            const uint32_t pv = k.vertex_start + q;
+            const float *d = f->h_vdata.data() + (size_t)pv * 8;
+            vtx.push_back(pv);
+            rest.push_back(make_float4(d[0], d[1], d[2], 0.f));
+            rest.push_back(make_float4(d[3], d[4], d[5], 0.f));
+            const uint16_t *j = k.joints + (size_t)q * 4;
+            joints.push_back(make_uint2(j[0] | ((uint32_t)j[1] << 16), j[2] | ((uint32_t)j[3] << 16)));
+            const float *ww = k.weights + (size_t)q * 4;
+            w.push_back(make_float4(ww[0], ww[1], ww[2], ww[3]));
+        }
+        uint32_t t0 = UINT32_MAX, t1 = 0;
+        for (uint32_t i = k.blas_root; i < k.blas_end; i++) {
+            const uint32_t info = f->h_nodes[(size_t)i * 8 + 7];
+            if (info == 0xFFFFFFFFu) continue;
+            leaves.push_back(i);
+            const uint32_t off = info & 0x00FFFFFFu, cnt = info >> 24;
+            if (cnt) {
+                t0 = std::min(t0, off);
+                t1 = std::max(t1, off + cnt);
+            }
+        }
+        nranges.push_back(make_uint2(k.blas_root, k.blas_end));
+        if (t1 > t0) tranges.push_back(make_uint2(t0, t1));
+        nmax = std::max(nmax, k.blas_end - k.blas_root);
+        if (t1 > t0) tmax = std::max(tmax, t1 - t0);
+        span_lo = std::min(span_lo, k.blas_root);
+        span_hi = std::max(span_hi, k.blas_end);
+    }
+    if (dalloc(&f->d_skin_vtx, vtx.size() * 4) != hipSuccess || dalloc(&f->d_skin_rest, rest.size() * 16) != hipSuccess ||
+        dalloc(&f->d_skin_joints, joints.size() * 8) != hipSuccess || dalloc(&f->d_skin_w, w.size() * 16) != hipSuccess ||
+        dalloc(&f->d_bones, (size_t)n_bones * 48) != hipSuccess || dalloc(&f->d_skin_leaves, leaves.size() * 4) != hipSuccess ||
+        dalloc(&f->d_skin_nranges, nranges.size() * 8) != hipSuccess ||
+        dalloc(&f->d_skin_tranges, std::max<size_t>(1, tranges.size()) * 8) != hipSuccess) {
+        free_skins(f);
+        return fail(f, SBGI_E_NOMEM, "skin allocation");
+    }
+    HIP_OK(f, hipMemcpy(f->d_skin_vtx, vtx.data(), vtx.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(f, hipMemcpy(f->d_skin_rest, rest.data(), rest.size() * 16, hipMemcpyHostToDevice));
+    HIP_OK(f, hipMemcpy(f->d_skin_joints, joints.data(), joints.size() * 8, hipMemcpyHostToDevice));
+    HIP_OK(f, hipMemcpy(f->d_skin_w, w.data(), w.size() * 16, hipMemcpyHostToDevice));
+    HIP_OK(f, hipMemcpy(f->d_skin_leaves, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice));
+    HIP_OK(f, hipMemcpy(f->d_skin_nranges, nranges.data(), nranges.size() * 8, hipMemcpyHostToDevice));
+    if (!tranges.empty()) HIP_OK(f, hipMemcpy(f->d_skin_tranges, tranges.data(), tranges.size() * 8, hipMemcpyHostToDevice));
+    f->skin_vtx_n = (uint32_t)vtx.size();
+    f->skin_bones_n = n_bones;
+    f->skin_leaf_n = (uint32_t)leaves.size();
+    f->skin_ranges_n = (uint32_t)nranges.size();
+    f->skin_tranges_n = (uint32_t)tranges.size();
+    f->skin_nmax = nmax;
+    f->skin_tmax = tmax;
+    f->skin_span_lo = span_lo;
+    f->skin_span_hi = span_hi;
+    return SBGI_OK;
+}
+
+int32_t sbgi_skin(sbgi_field *f, const float *bones, uint32_t n_bones, int32_t update_tlas) {
+    if (!f || !bones) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    if (!f->skin_vtx_n) return fail(f, SBGI_E_STATE, "no skin sets registered");
+    if (n_bones != f->skin_bones_n) return fail(f, SBGI_E_INVALID, "bone count differs from the registered palette");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    void *stage = nullptr;
+    int slot = 0;
+    const size_t bytes = (size_t)n_bones * 48;
+    int32_t rc = stage_acquire(f, bytes, &stage, &slot);
+    if (rc != SBGI_OK) return rc;
+    memcpy(stage, bones, bytes);
+    const sbgi_scene_view &v = f->view;
+    if ((rc = record_kev(f, 0)) != SBGI_OK) return rc;
+    HIP_OK(f, hipMemcpyAsync(f->d_bones, stage, bytes, hipMemcpyHostToDevice, f->stream));
+    if ((rc = stage_commit(f, slot)) != SBGI_OK) return rc;
+    HIP_OK(f, launch_skin(f->d_skin_vtx, f->d_skin_rest, f->d_skin_joints, f->d_skin_w, f->d_bones, f->skin_vtx_n,
+                          f->d_vdata, f->stream));
+    if ((rc = record_kev(f, 1)) != SBGI_OK) return rc;
+    HIP_OK(f, launch_fill_u32(f->d_arrive + f->skin_span_lo, 0u, f->skin_span_hi - f->skin_span_lo, f->stream));
+    HIP_OK(f, launch_refit_leaves(f->d_nodes, f->d_tri_index, f->d_vdata, f->d_parent, f->d_arrive, f->d_skin_leaves,
+                                  f->skin_leaf_n, f->stream));
+    HIP_OK(f, launch_oct_gather_ranges(f->d_walk_nodes, f->d_nodes, f->d_walk_perm, v.node_count, ordered(f) ? 8u : 1u,
+                                       f->d_skin_nranges, f->skin_ranges_n, f->skin_nmax, f->stream));
+    HIP_OK(f, launch_pack_tris_ranges(f->d_tri_index, f->d_vdata, f->d_tri_mat, f->d_mats, v.tri_count,
+                                      f->d_skin_tranges, f->skin_tranges_n, f->skin_tmax, f->d_tri, f->stream));
+    if ((rc = record_kev(f, 2)) != SBGI_OK) return rc;
+    f->nodes_dirty = true;
+    f->kev_blas = true;
+    if (update_tlas) {
+        if ((rc = record_kev(f, 3)) != SBGI_OK || (rc = device_tlas(f, true)) != SBGI_OK ||
+            (rc = record_kev(f, 4)) != SBGI_OK)
+            return rc;
+        f->kev_tlas = true;
+    }
     return SBGI_OK;
 }
 
@@ -838,6 +1050,9 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
         case SBGI_BUF_NODES_REFIT:
         case SBGI_BUF_BVH_NODES: return f->h_nodes.size() * 4;
         case SBGI_BUF_TRI_PACKED: return f->has_scene ? (size_t)f->view.tri_count * 48 : 0;
+        case SBGI_BUF_VERTEX_DATA: return f->h_vdata.size() * 4;
+        case SBGI_BUF_INST_BOUNDS: return f->has_scene ? (size_t)f->view.inst_count * 24 : 0;
+        case SBGI_BUF_TLAS_SAH: return f->has_scene ? (size_t)f->tlas_sah_count * 32 : 0;
     }
     return 0;
 }
@@ -879,8 +1094,13 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
         HIP_OK(f, hipMemcpy(dst, f->d_lights, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
-    if (buf_id == SBGI_BUF_TRI_PACKED) {
-        HIP_OK(f, hipMemcpy(dst, f->d_tri, bytes, hipMemcpyDeviceToHost));
+    if (buf_id == SBGI_BUF_TRI_PACKED || buf_id == SBGI_BUF_VERTEX_DATA || buf_id == SBGI_BUF_INST_BOUNDS ||
+        buf_id == SBGI_BUF_TLAS_SAH) {
+        src = buf_id == SBGI_BUF_TRI_PACKED    ? (const void *)f->d_tri
+              : buf_id == SBGI_BUF_VERTEX_DATA ? (const void *)f->d_vdata
+              : buf_id == SBGI_BUF_INST_BOUNDS ? (const void *)f->d_inst_bounds
+                                               : (const void *)f->d_tlas_sah;
+        HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
     if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_NODES_REFIT) {
@@ -942,6 +1162,25 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     return SBGI_OK;
 }
 
+int32_t sbgi_edit_timing(sbgi_field *f, float *ms_skin, float *ms_blas_refit, float *ms_tlas) {
+    if (!f) return SBGI_E_INVALID;
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    float s = 0.f, r = 0.f, t = 0.f;
+    if (f->kev_blas) {
+        HIP_OK(f, hipEventSynchronize(f->kev[2]));
+        HIP_OK(f, hipEventElapsedTime(&s, f->kev[0], f->kev[1]));
+        HIP_OK(f, hipEventElapsedTime(&r, f->kev[1], f->kev[2]));
+    }
+    if (f->kev_tlas) {
+        HIP_OK(f, hipEventSynchronize(f->kev[4]));
+        HIP_OK(f, hipEventElapsedTime(&t, f->kev[3], f->kev[4]));
+    }
+    if (ms_skin) *ms_skin = s;
+    if (ms_blas_refit) *ms_blas_refit = r;
+    if (ms_tlas) *ms_tlas = t;
+    return SBGI_OK;
+}
+
 const char *sbgi_last_error(const sbgi_field *f) { return f ? f->err.c_str() : g_create_error.c_str(); }
 
 int32_t sbgi_destroy(sbgi_field *f) {
@@ -972,6 +1211,12 @@ int32_t sbgi_destroy(sbgi_field *f) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : f->dev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : f->kev)
+        if (e) (void)hipEventDestroy(e);
+    for (int s = 0; s < sbgi_field::kStageSlots; s++) {
+        if (f->stage_done[s]) (void)hipEventDestroy(f->stage_done[s]);
+        if (f->stage_host[s]) (void)hipHostFree(f->stage_host[s]);
+    }
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
     return SBGI_OK;

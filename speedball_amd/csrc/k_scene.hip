@@ -31,10 +31,17 @@ __device__ __forceinline__ uint32_t tri_flags(const float *__restrict__ mats, ui
     return sc | (acc_closest ? 4u : 0u) | (acc_any ? 8u : 0u);
 }
 
+// All triangles, or (ranges != nullptr) the triangles [ranges[y].x, ranges[y].y) of range
+// y = blockIdx.y.
 __global__ void k_pack_tris(const uint32_t *__restrict__ tri_index, const float *__restrict__ vdata,
                             const uint32_t *__restrict__ tri_mat, const float *__restrict__ mats, uint32_t tri_count,
-                            float4 *__restrict__ tri) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+                            const uint2 *__restrict__ ranges, float4 *__restrict__ tri) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ranges) {
+        const uint2 r = ranges[blockIdx.y];
+        t += r.x;
+        if (t >= r.y) return;
+    }
     if (t >= tri_count) return;
     const uint32_t i0 = tri_index[(size_t)t * 3], i1 = tri_index[(size_t)t * 3 + 1], i2 = tri_index[(size_t)t * 3 + 2];
     const float *v0 = vdata + (size_t)i0 * 8, *v1 = vdata + (size_t)i1 * 8, *v2 = vdata + (size_t)i2 * 8;
@@ -86,13 +93,23 @@ __device__ __forceinline__ void union_children(float *__restrict__ nf, uint32_t 
     }
 }
 
-// Bottom-up BLAS refit over [root, end). parent[] holds each node's parent (or -1);
-// arrive[] must be zero for the range on entry. The host has validated the layout.
+// Bottom-up BLAS refit over [root, end), or (leaves != nullptr) from the n_leaves listed
+// leaf nodes up to their BLAS roots (root = 0: every BLAS root's parent is -1). parent[]
+// holds each node's parent (or -1); arrive[] must be zero on entry for every interior
+// node that will be reached. The host has validated the layout.
 __global__ void k_refit_blas(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
                              const float *__restrict__ vdata, const int32_t *__restrict__ parent,
-                             uint32_t *__restrict__ arrive, uint32_t root, uint32_t end) {
-    const uint32_t i = root + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= end) return;
+                             uint32_t *__restrict__ arrive, uint32_t root, uint32_t end,
+                             const uint32_t *__restrict__ leaves, uint32_t n_leaves) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i;
+    if (leaves) {
+        if (x >= n_leaves) return;
+        i = leaves[x];
+    } else {
+        i = root + x;
+        if (i >= end) return;
+    }
     const uint32_t info = nodes[(size_t)i * 8 + 7];
     if (info == 0xFFFFFFFFu) return;  // interiors are reached from below
     float *nf = reinterpret_cast<float *>(nodes);
@@ -161,10 +178,18 @@ __global__ void k_refit_tlas(float *__restrict__ mats, uint32_t tlas_base, uint3
 }
 
 // Walk-format node copies {min.xyz, miss}, {max.xyz, payload}: re-gather the bounds of
-// copy node i from pool node perm[i] after a refit (structure words are unchanged).
+// copy node i from pool node perm[i] after a refit (structure words are unchanged). All
+// n_total copy nodes, or (ranges != nullptr) the nodes [r.x, r.y) of range r =
+// ranges[blockIdx.z] in copy blockIdx.y (each BLAS keeps its pool range in every copy).
 __global__ void k_oct_gather(uint32_t *__restrict__ oct, const uint32_t *__restrict__ nodes,
-                             const uint32_t *__restrict__ perm, uint32_t n_total) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                             const uint32_t *__restrict__ perm, uint32_t n_total, const uint2 *__restrict__ ranges,
+                             uint32_t n_nodes) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ranges) {
+        const uint2 r = ranges[blockIdx.z];
+        if (r.x + i >= r.y) return;
+        i += blockIdx.y * n_nodes + r.x;
+    }
     if (i >= n_total) return;
     const uint32_t *src = nodes + (size_t)perm[i] * 8;
     uint32_t *dst = oct + (size_t)i * 8;
@@ -181,7 +206,17 @@ __global__ void k_oct_gather(uint32_t *__restrict__ oct, const uint32_t *__restr
 hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_total,
                              hipStream_t st) {
     if (!n_total) return hipSuccess;
-    hipLaunchKernelGGL(k_oct_gather, dim3((n_total + 255) / 256), dim3(256), 0, st, oct_nodes, nodes, perm, n_total);
+    hipLaunchKernelGGL(k_oct_gather, dim3((n_total + 255) / 256), dim3(256), 0, st, oct_nodes, nodes, perm, n_total,
+                       nullptr, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_oct_gather_ranges(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_nodes,
+                                    uint32_t copies, const uint2 *ranges, uint32_t n_ranges, uint32_t max_len,
+                                    hipStream_t st) {
+    if (!n_ranges || !max_len) return hipSuccess;
+    hipLaunchKernelGGL(k_oct_gather, dim3((max_len + 255) / 256, copies, n_ranges), dim3(256), 0, st, oct_nodes, nodes,
+                       perm, copies * n_nodes, ranges, n_nodes);
     return hipGetLastError();
 }
 
@@ -189,7 +224,16 @@ hipError_t launch_pack_tris(const uint32_t *tri_index, const float *vdata, const
                             uint32_t tri_count, float4 *tri, hipStream_t st) {
     if (!tri_count) return hipSuccess;
     hipLaunchKernelGGL(k_pack_tris, dim3((tri_count + 255) / 256), dim3(256), 0, st, tri_index, vdata, tri_mat, mats,
-                       tri_count, tri);
+                       tri_count, nullptr, tri);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_tris_ranges(const uint32_t *tri_index, const float *vdata, const uint32_t *tri_mat,
+                                   const float *mats, uint32_t tri_count, const uint2 *ranges, uint32_t n_ranges,
+                                   uint32_t max_len, float4 *tri, hipStream_t st) {
+    if (!n_ranges || !max_len) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_tris, dim3((max_len + 255) / 256, n_ranges), dim3(256), 0, st, tri_index, vdata, tri_mat,
+                       mats, tri_count, ranges, tri);
     return hipGetLastError();
 }
 
@@ -211,7 +255,15 @@ hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const f
     if (end <= root) return hipSuccess;
     const uint32_t n = end - root;
     hipLaunchKernelGGL(k_refit_blas, dim3((n + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata, parent, arrive,
-                       root, end);
+                       root, end, nullptr, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_refit_leaves(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
+                               uint32_t *arrive, const uint32_t *leaves, uint32_t n_leaves, hipStream_t st) {
+    if (!n_leaves) return hipSuccess;
+    hipLaunchKernelGGL(k_refit_blas, dim3((n_leaves + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata, parent,
+                       arrive, 0u, 0u, leaves, n_leaves);
     return hipGetLastError();
 }
 

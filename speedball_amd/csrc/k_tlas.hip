@@ -1,0 +1,318 @@
+// k_tlas.hip — the per-frame TLAS on the device.
+//
+// k_inst_bounds: world AABB of every TLAS slot, Box3.applyMatrix4 in double of the
+//   instance's BLAS root bounds (js/spectral_scene.js:1301-1306); an inactive slot is the
+//   point at its translation. It also writes the bounds inflated by a relative 1e-5 for
+//   the SAH build, so that rounding of a world matrix can never make a box smaller than
+//   its instance.
+// k_build_tlas: binned-SAH TLAS (32 bins per axis, one slot per leaf) over the inflated
+//   bounds, written in the walk's node format {min.xyz, miss}, {max.xyz, payload}. The
+//   reference keeps a median-split TLAS built once on the CPU and refits it
+//   (js/spectral_scene.js:973-1003); rebuilding with SAH every frame keeps the tree
+//   good as instances move.
+//
+// The build is one workgroup of 16 waves. Node positions are depth-first (left child at
+//   pos + 1, right child at pos + 2·|left items|, miss at pos + 2·|items| − 1: a subtree
+//   over m slots has 2m − 1 nodes), so every node of a level is independent and a level
+//   is processed in parallel, one wave per node, with the slot ids partitioned stably
+//   from one ping-pong array into the other. Bin counts and bounds are LDS atomics (min
+//   and max through an order-preserving float→u32 map); the SAH sweep runs one lane per
+//   axis in double. Every decision (bin index, sweep order, strict "<" across axes and
+//   bins, stable partition, middle split when no axis has extent) is deterministic, so
+//   the tree does not depend on the schedule of the waves.
+#pragma clang fp contract(off)
+#include "kernels.h"
+
+namespace {
+
+constexpr int kTlasBins = 32;
+constexpr int kTlasBlock = 1024;
+constexpr int kTlasWaves = kTlasBlock / 64;
+
+__global__ void k_inst_bounds(const float *__restrict__ mats, uint32_t inst_base, uint32_t n,
+                              const uint32_t *__restrict__ nodes, const double *__restrict__ world,
+                              float *__restrict__ bounds, float *__restrict__ padded) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *rec = mats + inst_base + (size_t)i * SBGI_MAT_STRIDE;
+    const double *e = world + (size_t)i * 16;
+    float out[6];
+    if (!(rec[15] > 0.5f)) {
+        for (int a = 0; a < 3; a++) out[a] = out[3 + a] = (float)e[12 + a];
+    } else {
+        const float *rb = reinterpret_cast<const float *>(nodes + (size_t)(uint32_t)rec[12] * 8);
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; c++) {
+            const double x = (c & 1) ? rb[3] : rb[0];
+            const double y = (c & 2) ? rb[4] : rb[1];
+            const double z = (c & 4) ? rb[5] : rb[2];
+            const double w = 1 / (e[3] * x + e[7] * y + e[11] * z + e[15]);
+            const double p[3] = {(e[0] * x + e[4] * y + e[8] * z + e[12]) * w,
+                                 (e[1] * x + e[5] * y + e[9] * z + e[13]) * w,
+                                 (e[2] * x + e[6] * y + e[10] * z + e[14]) * w};
+            for (int a = 0; a < 3; a++) {
+                mn[a] = p[a] < mn[a] ? p[a] : mn[a];
+                mx[a] = mx[a] < p[a] ? p[a] : mx[a];
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            out[a] = (float)mn[a];
+            out[3 + a] = (float)mx[a];
+        }
+    }
+    for (int a = 0; a < 6; a++) bounds[(size_t)i * 6 + a] = out[a];
+    for (int a = 0; a < 3; a++) {
+        const float pad = 1e-5f * (fabsf(out[3 + a] - out[a]) + fabsf(out[a]) + fabsf(out[3 + a])) + 1e-6f;
+        padded[(size_t)i * 6 + a] = out[a] - pad;
+        padded[(size_t)i * 6 + 3 + a] = out[3 + a] + pad;
+    }
+}
+
+// order-preserving map of finite floats and infinities onto u32 (for LDS atomic min/max)
+__device__ __forceinline__ uint32_t f2o(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+__device__ __forceinline__ float fmin_keep(float acc, float v) { return v < acc ? v : acc; }
+__device__ __forceinline__ float fmax_keep(float acc, float v) { return acc < v ? v : acc; }
+
+// surface area of a box in double; an empty or inverted box has area 0
+__device__ __forceinline__ double box_area(const float *mn, const float *mx) {
+    double d[3];
+    for (int a = 0; a < 3; a++) {
+        d[a] = (double)mx[a] - (double)mn[a];
+        if (!(d[a] >= 0)) return 0.0;
+    }
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+
+__device__ __forceinline__ int bin_of(float c, float cmn, float ext) {
+    int k = (int)((c - cmn) / ext * kTlasBins);
+    return min(max(k, 0), kTlasBins - 1);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// bins of one wave: per axis and bin {count, min.xyz, max.xyz} (min/max as f2o words)
+constexpr int kBinWords = 7;
+
+__global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restrict__ bnd, uint32_t n,
+                                                           uint32_t *__restrict__ ids0, uint32_t *__restrict__ ids1,
+                                                           uint4 *__restrict__ task0, uint4 *__restrict__ task1,
+                                                           uint4 *__restrict__ tlas) {
+    __shared__ uint32_t s_count[2];
+    __shared__ uint32_t s_bins[kTlasWaves][3 * kTlasBins * kBinWords];
+    __shared__ double s_ra[kTlasWaves][3 * kTlasBins];
+    __shared__ uint32_t s_rc[kTlasWaves][3 * kTlasBins];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t i = tid; i < n; i += kTlasBlock) ids0[i] = i;
+    if (tid == 0) {
+        task0[0] = make_uint4(0u, 0u, n, 0u);
+        s_count[0] = 1u;
+        s_count[1] = 0u;
+    }
+    __syncthreads();
+    uint32_t *bins = s_bins[wave];
+    double *ra = s_ra[wave];
+    uint32_t *rc = s_rc[wave];
+    for (uint32_t level = 0;; level++) {
+        const uint32_t cur = level & 1u, nxt = cur ^ 1u;
+        const uint32_t ntask = s_count[cur];
+        __syncthreads();  // every thread has read the task count; s_count[nxt] is zero
+        if (ntask == 0) break;
+        const uint4 *tasks = cur ? task1 : task0;
+        uint4 *next = cur ? task0 : task1;
+        const uint32_t *src = cur ? ids1 : ids0;
+        uint32_t *dst = cur ? ids0 : ids1;
+        for (uint32_t t = wave; t < ntask; t += kTlasWaves) {
+            const uint4 tk = tasks[t];
+            const uint32_t pos = tk.x, lo = tk.y, hi = tk.z, m = hi - lo;
+            float bmn[3] = {INFINITY, INFINITY, INFINITY}, bmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (uint32_t i = lo + lane; i < hi; i += 64) {
+                const float *bb = bnd + (size_t)src[i] * 6;
+                for (int a = 0; a < 3; a++) {
+                    bmn[a] = fmin_keep(bmn[a], bb[a]);
+                    bmx[a] = fmax_keep(bmx[a], bb[3 + a]);
+                    const float c = 0.5f * (bb[a] + bb[3 + a]);
+                    cmn[a] = fmin_keep(cmn[a], c);
+                    cmx[a] = fmax_keep(cmx[a], c);
+                }
+            }
+            for (int a = 0; a < 3; a++) {
+                bmn[a] = wave_min(bmn[a]);
+                bmx[a] = wave_max(bmx[a]);
+                cmn[a] = wave_min(cmn[a]);
+                cmx[a] = wave_max(cmx[a]);
+            }
+            if (m == 1) {
+                if (lane == 0) {
+                    tlas[(size_t)pos * 2] =
+                        make_uint4(__float_as_uint(bmn[0]), __float_as_uint(bmn[1]), __float_as_uint(bmn[2]), pos + 1);
+                    tlas[(size_t)pos * 2 + 1] = make_uint4(__float_as_uint(bmx[0]), __float_as_uint(bmx[1]),
+                                                           __float_as_uint(bmx[2]), (1u << 24) | (src[lo] & 0x00FFFFFFu));
+                }
+                continue;
+            }
+            const float ext[3] = {cmx[0] - cmn[0], cmx[1] - cmn[1], cmx[2] - cmn[2]};
+            for (uint32_t j = lane; j < 3 * kTlasBins; j += 64) {
+                uint32_t *b = bins + j * kBinWords;
+                b[0] = 0u;
+                b[1] = b[2] = b[3] = f2o(INFINITY);
+                b[4] = b[5] = b[6] = f2o(-INFINITY);
+            }
+            wave_sync();
+            for (uint32_t i = lo + lane; i < hi; i += 64) {
+                const float *bb = bnd + (size_t)src[i] * 6;
+                for (int a = 0; a < 3; a++) {
+                    if (!(ext[a] > 0)) continue;
+                    const int k = bin_of(0.5f * (bb[a] + bb[3 + a]), cmn[a], ext[a]);
+                    uint32_t *b = bins + (a * kTlasBins + k) * kBinWords;
+                    atomicAdd(b, 1u);
+                    for (int q = 0; q < 3; q++) {
+                        atomicMin(b + 1 + q, f2o(bb[q]));
+                        atomicMax(b + 4 + q, f2o(bb[3 + q]));
+                    }
+                }
+            }
+            wave_sync();
+            // SAH sweep, one lane per axis: right-to-left prefix areas and counts, then the
+            // left-to-right scan keeping the first minimum
+            double best = INFINITY;
+            int best_bin = -1;
+            if (lane < 3) {
+                const int a = (int)lane;
+                const float e = a == 0 ? ext[0] : (a == 1 ? ext[1] : ext[2]);
+                if (e > 0) {
+                    const uint32_t *ab = bins + a * kTlasBins * kBinWords;
+                    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+                    uint32_t c = 0;
+                    for (int k = kTlasBins - 1; k > 0; k--) {
+                        const uint32_t *b = ab + k * kBinWords;
+                        for (int q = 0; q < 3; q++) {
+                            mn[q] = fmin_keep(mn[q], o2f(b[1 + q]));
+                            mx[q] = fmax_keep(mx[q], o2f(b[4 + q]));
+                        }
+                        c += b[0];
+                        ra[a * kTlasBins + k] = box_area(mn, mx);
+                        rc[a * kTlasBins + k] = c;
+                    }
+                    float lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+                    uint32_t lc = 0;
+                    for (int k = 0; k < kTlasBins - 1; k++) {
+                        const uint32_t *b = ab + k * kBinWords;
+                        for (int q = 0; q < 3; q++) {
+                            lmn[q] = fmin_keep(lmn[q], o2f(b[1 + q]));
+                            lmx[q] = fmax_keep(lmx[q], o2f(b[4 + q]));
+                        }
+                        lc += b[0];
+                        const uint32_t r = rc[a * kTlasBins + k + 1];
+                        if (!lc || !r) continue;
+                        const double cost = box_area(lmn, lmx) * lc + ra[a * kTlasBins + k + 1] * r;
+                        if (cost < best) {
+                            best = cost;
+                            best_bin = k;
+                        }
+                    }
+                }
+            }
+            // axes in order, strict "<": ties keep the lower axis, then the lower bin
+            int best_axis = -1, bbin = -1;
+            double bcost = INFINITY;
+            for (int a = 0; a < 3; a++) {
+                const double ca = __shfl(best, a);
+                const int ka = __shfl(best_bin, a);
+                if (ka >= 0 && ca < bcost) {
+                    bcost = ca;
+                    best_axis = a;
+                    bbin = ka;
+                }
+            }
+            uint32_t mid;
+            if (best_axis < 0) {
+                mid = lo + m / 2;
+                for (uint32_t i = lo + lane; i < hi; i += 64) dst[i] = src[i];
+            } else {
+                const float e = best_axis == 0 ? ext[0] : (best_axis == 1 ? ext[1] : ext[2]);
+                const float cm = best_axis == 0 ? cmn[0] : (best_axis == 1 ? cmn[1] : cmn[2]);
+                auto goes_left = [&](uint32_t id) {
+                    const float *bb = bnd + (size_t)id * 6;
+                    return bin_of(0.5f * (bb[best_axis] + bb[3 + best_axis]), cm, e) <= bbin;
+                };
+                // count with the same predicate that scatters, so every write stays in [lo, hi)
+                uint32_t nleft = 0;
+                for (uint32_t base = lo; base < hi; base += 64) {
+                    const uint32_t i = base + lane;
+                    nleft += __popcll(__ballot(i < hi && goes_left(src[i])));
+                }
+                uint32_t lbase = 0, rbase = 0;
+                const uint64_t below = (1ull << lane) - 1ull;
+                for (uint32_t base = lo; base < hi; base += 64) {
+                    const uint32_t i = base + lane;
+                    const bool valid = i < hi;
+                    const uint32_t id = valid ? src[i] : 0u;
+                    const bool left = valid && goes_left(id);
+                    const uint64_t bl = __ballot(left), bv = __ballot(valid), br = bv & ~bl;
+                    if (valid) {
+                        if (left)
+                            dst[lo + lbase + __popcll(bl & below)] = id;
+                        else
+                            dst[lo + nleft + rbase + __popcll(br & below)] = id;
+                    }
+                    lbase += __popcll(bl);
+                    rbase += __popcll(br);
+                }
+                mid = lo + nleft;
+                if (mid == lo || mid == hi) mid = lo + m / 2;
+            }
+            if (lane == 0) {
+                tlas[(size_t)pos * 2] = make_uint4(__float_as_uint(bmn[0]), __float_as_uint(bmn[1]),
+                                                   __float_as_uint(bmn[2]), pos + 2 * m - 1);
+                tlas[(size_t)pos * 2 + 1] =
+                    make_uint4(__float_as_uint(bmx[0]), __float_as_uint(bmx[1]), __float_as_uint(bmx[2]), 0xFFFFFFFFu);
+                const uint32_t slot = atomicAdd(&s_count[nxt], 2u);
+                next[slot] = make_uint4(pos + 1, lo, mid, 0u);
+                next[slot + 1] = make_uint4(pos + 2 * (mid - lo), mid, hi, 0u);
+            }
+        }
+        __syncthreads();  // this level's nodes, partitions and appended tasks are complete
+        if (tid == 0) s_count[cur] = 0u;
+    }
+}
+
+}  // namespace
+
+size_t tlas_build_scratch_bytes(uint32_t n) { return (size_t)n * (2 * sizeof(uint32_t) + 2 * sizeof(uint4)); }
+
+hipError_t launch_inst_bounds(const float *mats, uint32_t inst_base, uint32_t n, const uint32_t *nodes,
+                              const double *world, float *bounds, float *padded, hipStream_t st) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_inst_bounds, dim3((n + 255) / 256), dim3(256), 0, st, mats, inst_base, n, nodes, world, bounds,
+                       padded);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_tlas(const float *padded, uint32_t n, void *scratch, uint4 *tlas, hipStream_t st) {
+    if (!n) return hipSuccess;
+    // tasks first: scratch is 256-byte aligned and the uint4 lists need 16
+    uint4 *task0 = static_cast<uint4 *>(scratch), *task1 = task0 + n;
+    uint32_t *ids0 = reinterpret_cast<uint32_t *>(task1 + n), *ids1 = ids0 + n;
+    hipLaunchKernelGGL(k_build_tlas, dim3(1), dim3(kTlasBlock), 0, st, padded, n, ids0, ids1, task0, task1, tlas);
+    return hipGetLastError();
+}

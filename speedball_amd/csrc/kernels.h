@@ -73,6 +73,30 @@ hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const u
                              hipStream_t st);
 hipError_t launch_refit_tlas(float *mats, uint32_t tlas_base, uint32_t tlas_count, const float *inst_bounds,
                              const int32_t *parent, uint32_t *arrive, hipStream_t st);
+// ranged variants: triangles / walk-copy nodes of the listed [x, y) ranges (max_len = the
+// longest range), and the refit climbing from a list of leaf nodes
+hipError_t launch_pack_tris_ranges(const uint32_t *tri_index, const float *vdata, const uint32_t *tri_mat,
+                                   const float *mats, uint32_t tri_count, const uint2 *ranges, uint32_t n_ranges,
+                                   uint32_t max_len, float4 *tri, hipStream_t st);
+hipError_t launch_oct_gather_ranges(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_nodes,
+                                    uint32_t copies, const uint2 *ranges, uint32_t n_ranges, uint32_t max_len,
+                                    hipStream_t st);
+hipError_t launch_refit_leaves(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
+                               uint32_t *arrive, const uint32_t *leaves, uint32_t n_leaves, hipStream_t st);
+
+// k_tlas.hip — instance world bounds (and the inflated copy the SAH build reads) and the
+// one-workgroup SAH TLAS build: tlas receives 2n - 1 walk-format nodes
+size_t tlas_build_scratch_bytes(uint32_t n);
+hipError_t launch_inst_bounds(const float *mats, uint32_t inst_base, uint32_t n, const uint32_t *nodes,
+                              const double *world, float *bounds, float *padded, hipStream_t st);
+hipError_t launch_build_tlas(const float *padded, uint32_t n, void *scratch, uint4 *tlas, hipStream_t st);
+
+// k_dynamic.hip — device skinning, staged instance-record scatter, fill
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);
+hipError_t launch_skin(const uint32_t *vtx, const float4 *rest, const uint2 *joints, const float4 *weights,
+                       const float4 *bones, uint32_t n, float *vdata, hipStream_t st);
+hipError_t launch_scatter_inst(const uint32_t *slots, const float *recs, const double *worlds, uint32_t n, float *mats,
+                               uint32_t inst_base, double *world, float4 *inst, hipStream_t st);
 
 // k_adjoint.hip — adjoint of the differentiable two-update solve (see that file). w holds
 // the forward's frozen hits and visibility masks, atlas1 the first update's FP32 atlas;

@@ -33,7 +33,6 @@ double det4(const double *e);
 void box_apply(const float *mn, const float *mx, const double *e, double *omn, double *omx);
 
 // traversal-only layouts (accel.cpp)
-void build_sah_tlas(const float *inst_bounds, uint32_t n, std::vector<float> &out);
 bool build_node_copies(const uint32_t *nodes, uint32_t n, const std::vector<std::pair<uint32_t, uint32_t>> &roots,
                        uint32_t copies, std::vector<uint32_t> &out, std::vector<uint32_t> &perm);
 

@@ -597,6 +597,18 @@ int32_t sbgi_sb_tlas_order(const sbgi_scene_builder *b, uint32_t *order, uint32_
     return SBGI_OK;
 }
 
+int32_t sbgi_sb_mesh_ranges(const sbgi_scene_builder *b, uint32_t mesh_id, uint32_t *out6) {
+    if (!b || !out6 || !b->final || mesh_id >= b->meshes.size()) return SBGI_E_INVALID;
+    const auto &m = b->meshes[mesh_id];
+    out6[0] = m.vert_base;
+    out6[1] = (uint32_t)(m.vdata.size() / 8);
+    out6[2] = m.tri_base;
+    out6[3] = (uint32_t)m.tri_mat.size();
+    out6[4] = m.node_base;
+    out6[5] = m.node_base + (uint32_t)m.nodes.size();
+    return SBGI_OK;
+}
+
 int32_t sbgi_sb_set_transform(sbgi_scene_builder *b, uint32_t instance_id, const double *world16, uint32_t *slot) {
     if (!b || !world16 || instance_id >= b->insts.size()) return SBGI_E_INVALID;
     if (!b->final) {

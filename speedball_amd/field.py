@@ -136,6 +136,29 @@ class ProbeField:
     def read_hit_ids(self):
         return self.read(N.BUF_HIT_IDS, np.int32).reshape(-1, self.rays, 2)
 
+    # ---- device skinning (sbgi_set_skins / sbgi_skin) ----
+    def set_skins(self, sets, n_bones: int):
+        """sets: iterable of (vertex_start, vertex_count, blas_root, blas_end, joints u16[V,4],
+        weights f32[V,4]); the rest pose is the vertexData last provided by the host."""
+        keep, arr = [], (N.SkinSet * max(1, len(sets)))()
+        for i, (vs, vc, br, be, j, w) in enumerate(sets):
+            j = np.ascontiguousarray(j, np.uint16).reshape(-1, 4)
+            w = np.ascontiguousarray(w, np.float32).reshape(-1, 4)
+            keep += [j, w]
+            arr[i] = N.SkinSet(vs, vc, br, be, j.ctypes.data, w.ctypes.data)
+        self._chk(self.L.sbgi_set_skins(self.h, arr, len(sets), n_bones))
+
+    def skin(self, bones, update_tlas: bool = True):
+        """One skinning frame; bones: [n_bones, 3, 4] row-major world-from-rest matrices."""
+        b = np.ascontiguousarray(bones, np.float32).reshape(-1, 12)
+        self._chk(self.L.sbgi_skin(self.h, N.ptr(b), b.shape[0], int(update_tlas)))
+
+    def edit_timing(self):
+        """(skin ms, BLAS refit ms, instance bounds + TLAS ms) of the last scene edits."""
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        self._chk(self.L.sbgi_edit_timing(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
     # ---- walk schedule ----
     def walk_tuning(self) -> dict:
         t = N.WalkTuning()

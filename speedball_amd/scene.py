@@ -126,6 +126,7 @@ class SceneBuilder:
         N.check(self.L.sbgi_sb_create(C.byref(h)))
         self.h = h
         self.n_instances = 0
+        self.n_meshes = 0
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -150,6 +151,7 @@ class SceneBuilder:
         i = C.c_uint32()
         self._chk(self.L.sbgi_sb_add_mesh(self.h, N.ptr(pos), pos.size // 3, N.ptr(nrm), N.ptr(uv), N.ptr(idx),
                                           idx.size // 3, N.ptr(tm), C.byref(i)))
+        self.n_meshes += 1
         return i.value
 
     def add_instance(self, mesh_id: int, world=None, active: bool = True) -> int:
@@ -176,6 +178,10 @@ class SceneBuilder:
 
         order = np.zeros(v.inst_count, np.uint32)
         self._chk(self.L.sbgi_sb_tlas_order(self.h, N.ptr(order), v.inst_count))
+        ranges = np.zeros((self.n_meshes, 6), np.uint32)
+        for m in range(self.n_meshes):
+            self._chk(self.L.sbgi_sb_mesh_ranges(self.h, m, N.ptr(ranges[m])))
+        meta = dict(meta, mesh_ranges=ranges)
         return Scene(
             bvh_nodes=grab(v.bvh_nodes, v.node_count * 8, np.uint32),
             tri_index=grab(v.tri_index, v.tri_count * 3, np.uint32),

@@ -311,3 +311,169 @@ def differentiable(res=(16, 16, 16), rays: int = 128, n_mesh: int = 4, n_instanc
     jitter = float(np.float32((seed * 0.6180339887) % 1.0))
     w = Workload("differentiable", scene, (0.0, 0.0, 0.0), (box, box, box), tuple(res), rays, 6, jitter, 0.0, bounces=2)
     return w, albedo, target
+
+
+# ---------------------------------------------------------------- dynamic workload
+def tube(n_around: int, n_along: int, length: float, radius: float):
+    """Open cylinder along +y from 0 to `length`: (n_along + 1) rings of n_around vertices,
+    radial normals, 2 * n_around * n_along triangles wound outward."""
+    ph = np.arange(n_around) * (2 * math.pi / n_around)
+    y = np.linspace(0.0, length, n_along + 1)
+    yy, pp = np.meshgrid(y, ph, indexing="ij")
+    nrm = np.stack([np.cos(pp), np.zeros_like(pp), np.sin(pp)], -1).reshape(-1, 3)
+    v = np.stack([radius * np.cos(pp), yy, radius * np.sin(pp)], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(n_along), np.arange(n_around), indexing="ij")
+    a = i * n_around + j
+    b = i * n_around + (j + 1) % n_around
+    c = (i + 1) * n_around + (j + 1) % n_around
+    d = (i + 1) * n_around + j
+    t = np.concatenate([np.stack([a, c, b], -1).reshape(-1, 3), np.stack([a, d, c], -1).reshape(-1, 3)])
+    return v.astype(np.float32), t.astype(np.uint32), nrm.astype(np.float32)
+
+
+def axis_angle(axis, angle: float) -> np.ndarray:
+    x, y, z = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    c, s = math.cos(angle), math.sin(angle)
+    C1 = 1 - c
+    return np.array([[c + x * x * C1, x * y * C1 - z * s, x * z * C1 + y * s],
+                     [y * x * C1 + z * s, c + y * y * C1, y * z * C1 - x * s],
+                     [z * x * C1 - y * s, z * y * C1 + x * s, c + z * z * C1]])
+
+
+@dataclass
+class DynamicWorkload:
+    """Skinned tubes + rigid instances. skin_sets / n_bones feed ProbeField.set_skins;
+    bones(t) is the palette at time t (s); rigid_step() advances the random walk one frame
+    and returns (TLAS slots, world matrices)."""
+    name: str
+    scene: Scene
+    cascades: list
+    rays: int
+    depth_oct: int
+    frame_jitter: float
+    hysteresis: float
+    skin_sets: list
+    n_bones: int
+    tube_len: np.ndarray
+    tube_axis: np.ndarray
+    bone_phase: np.ndarray
+    rigid_slots: np.ndarray
+    slot_worlds: np.ndarray
+    rigid_pos: np.ndarray
+    rigid_rot: np.ndarray
+    rigid_scale: np.ndarray
+    box: np.ndarray
+    walk_rng: np.random.Generator
+    amplitude: float = math.radians(30.0)
+    frequency: float = 0.5
+    fps: float = 60.0
+    bounces: int = 2
+
+    @property
+    def probes(self) -> int:
+        return sum(r[0] * r[1] * r[2] for _, _, r in self.cascades)
+
+    def bones(self, t: float) -> np.ndarray:
+        """[n_bones, 3, 4]: bone b of tube k swings about the tube's axis by
+        amplitude * sin(2 pi f t + phase) around its pivot at height b * L / 4, chained."""
+        out = np.zeros((self.n_bones, 3, 4), np.float64)
+        for k in range(len(self.tube_len)):
+            M = np.eye(4)
+            seg = self.tube_len[k] / 4.0
+            for b in range(4):
+                th = self.amplitude * math.sin(2 * math.pi * self.frequency * t + self.bone_phase[k, b])
+                L = np.eye(4)
+                L[:3, :3] = axis_angle(self.tube_axis[k], th)
+                piv = np.array([0.0, b * seg, 0.0])
+                L[:3, 3] = piv - L[:3, :3] @ piv
+                M = M @ L
+                out[k * 4 + b] = M[:3, :]
+        return out.astype(np.float32)
+
+    def rigid_step(self, sigma: float = 0.05, sigma_rot: float = math.radians(1.0)):
+        g = self.walk_rng
+        n = len(self.rigid_slots)
+        self.rigid_pos += g.normal(0.0, sigma, (n, 3))
+        lo, hi = np.zeros(3), self.box
+        self.rigid_pos = np.where(self.rigid_pos < lo, 2 * lo - self.rigid_pos, self.rigid_pos)
+        self.rigid_pos = np.where(self.rigid_pos > hi, 2 * hi - self.rigid_pos, self.rigid_pos)
+        worlds = []
+        for i in range(n):
+            ax = g.normal(size=3)
+            self.rigid_rot[i] = axis_angle(ax, g.normal(0.0, sigma_rot)) @ self.rigid_rot[i]
+            worlds.append(trs(self.rigid_pos[i], self.rigid_rot[i], self.rigid_scale[i]))
+        return self.rigid_slots, worlds
+
+
+def dynamic(res=(24, 24, 24), n_cascades: int = 4, rays: int = 128, n_tubes: int = 32, tube_seg=(128, 128),
+            n_rigid: int = 128, n_lights: int = 16, depth_oct: int = 14, seed_id: int = 5,
+            hysteresis: float = 0.97, box=(40.0, 20.0, 25.0)) -> DynamicWorkload:
+    """32 skinned tubes (4-bone chains, 128 x 128 segments = 32,768 triangles each, 1,048,576
+    in all) standing in a 40x20x25 m box with 128 rigid instances of a sphere, a torus and a
+    box doing a seeded random walk; 16 point lights log-uniform [1, 100]; 4 cascades of
+    24^3 probes centred on the box, each twice the extent of the previous (the largest is
+    the box), 128 rays."""
+    rng = SplitMix64(SEED_BASE + seed_id)
+    box = np.array(box, np.float64)
+    b = SceneBuilder()
+    tube_mat = b.add_material(uber_material((0.7, 0.6, 0.5), side=2))
+    tubes, lens, axes = [], [], []
+    for k in range(n_tubes):
+        L = rng.uniform(4.0, 10.0)
+        r = rng.uniform(0.2, 0.6)
+        v, t, nrm = tube(tube_seg[0], tube_seg[1], L, r)
+        tubes.append((b.add_mesh(v, t, normals=nrm, tri_material=np.full(len(t), tube_mat, np.uint32)), v))
+        lens.append(L)
+        a = rng.uniform(0, 2 * math.pi)
+        axes.append((math.cos(a), 0.0, math.sin(a)))
+    rigid_meshes = []
+    for kind, (v, t) in enumerate([uv_sphere(16, 32, 0.6), torus(32, 16, 0.6, 0.2), box_mesh((0, 0, 0), (0.5,) * 3, 0)]):
+        m = b.add_material(uber_material((rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8))))
+        rigid_meshes.append(b.add_mesh(v, t, tri_material=np.full(len(t), m, np.uint32)))
+    worlds = []
+    for k in range(n_tubes):
+        yaw = rng.uniform(0, 2 * math.pi)
+        R = axis_angle((0, 1, 0), yaw) @ axis_angle((1, 0, 0), rng.uniform(-0.2, 0.2))
+        pos = (rng.uniform(0.1, 0.9) * box[0], rng.uniform(0, 0.1) * box[1], rng.uniform(0.1, 0.9) * box[2])
+        worlds.append(trs(pos, R, 1.0))
+        b.add_instance(tubes[k][0], worlds[-1])
+    rigid_ids, rpos, rrot, rscale = [], [], [], []
+    for i in range(n_rigid):
+        R = random_rotation(rng)
+        s = rng.uniform(0.5, 2.0)
+        p = np.array([rng.uniform(0, box[0]), rng.uniform(0, box[1]), rng.uniform(0, box[2])])
+        worlds.append(trs(p, R, s))
+        rigid_ids.append(b.add_instance(rigid_meshes[i % 3], worlds[-1]))
+        rpos.append(p)
+        rrot.append(R)
+        rscale.append(s)
+    for _ in range(n_lights):
+        p = (rng.uniform(0, box[0]), rng.uniform(0, box[1]), rng.uniform(0, box[2]))
+        b.add_light(point_light(p, intensity=rng.log_uniform(1.0, 100.0), distance=0.0, decay=2.0))
+    scene = b.finalize(workload="dynamic", skinned_triangles=n_tubes * 2 * tube_seg[0] * tube_seg[1])
+    ranges = scene.meta["mesh_ranges"]
+    slot_of = np.argsort(scene.tlas_order)  # source instance id -> TLAS slot
+    # 4-bone weights: Gaussian in height around each bone's centre, normalised
+    sets = []
+    for k in range(n_tubes):
+        mid, v = tubes[k]
+        vs, vc, _, _, br, be = (int(x) for x in ranges[mid])
+        seg = lens[k] / 4.0
+        c = (np.arange(4) + 0.5) * seg
+        w = np.exp(-(((v[:, 1:2] - c[None, :]) / (0.6 * seg)) ** 2))
+        w = (w / w.sum(1, keepdims=True)).astype(np.float32)
+        j = (k * 4 + np.arange(4, dtype=np.uint16))[None, :].repeat(vc, 0)
+        sets.append((vs, vc, br, be, j, w))
+    gen = np.random.default_rng(SEED_BASE + seed_id)
+    phase = gen.uniform(0, 2 * math.pi, (n_tubes, 4))
+    centre = box / 2
+    cascades = []
+    for c in range(n_cascades):
+        size = box / (2 ** (n_cascades - 1 - c))
+        cascades.append((tuple((centre - size / 2).tolist()), tuple(size.tolist()), tuple(res)))
+    seed = SEED_BASE + seed_id
+    jitter = float(np.float32((seed * 0.6180339887) % 1.0))
+    return DynamicWorkload("dynamic", scene, cascades, rays, depth_oct, jitter, hysteresis, sets, n_tubes * 4,
+                           np.array(lens), np.array(axes), phase, slot_of[np.array(rigid_ids)].astype(np.uint32),
+                           np.array(worlds)[scene.tlas_order], np.array(rpos), np.array(rrot), np.array(rscale), box,
+                           np.random.default_rng(SEED_BASE + seed_id + 1))

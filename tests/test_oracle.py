@@ -289,6 +289,151 @@ def test_refit_tlas_matches_bounds(instanced_small):
     assert np.array_equal(t[0, 0:3], ib[:, :3].min(0)) and np.array_equal(t[0, 3:6], ib[:, 3:].max(0))
 
 
+# ---------------------------------------------------------------- per-frame TLAS, skinning
+def _mats_with_instances(n, active=None, root=0, end=1):
+    """A materials buffer holding n instance records (BLAS range [root, end), active flags)."""
+    m = np.zeros(28 + 28 * n, np.float32)
+    r = m[28:].reshape(n, 28)
+    r[:, 12], r[:, 13] = root, end
+    r[:, 15] = 1.0 if active is None else np.asarray(active, np.float32)
+    return m
+
+
+def _node_with_box(mn, mx):
+    nd = np.zeros(8, np.uint32)
+    nd[:6] = np.array(list(mn) + list(mx), np.float32).view(np.uint32)
+    nd[6], nd[7] = 1, (1 << 24)
+    return nd
+
+
+def test_inst_bounds_closed_forms():
+    """Identity -> the root box; translation -> shifted box; a 90 degree turn about y maps
+    x -> -z and z -> x exactly; a uniform scale scales; inactive -> the translation point;
+    the inflation is the stated 1e-5 relative + 1e-6."""
+    node = _node_with_box((-1.0, -2.0, -3.0), (1.0, 2.0, 4.0))
+    ident = np.eye(4)
+    tr = np.eye(4)
+    tr[:3, 3] = (10.0, 20.0, 30.0)
+    rot = np.array([[0, 0, 1, 0], [0, 1, 0, 0], [-1, 0, 0, 0], [0, 0, 0, 1]], np.float64)
+    sc = np.diag([2.0, 2.0, 2.0, 1.0])
+    worlds = np.stack([m.T.reshape(-1) for m in (ident, tr, rot, sc, tr)])
+    mats = _mats_with_instances(5, active=[1, 1, 1, 1, 0])
+    b, pad = oracle.inst_bounds(mats, 28, 5, node, worlds)
+    assert np.array_equal(b[0], [-1, -2, -3, 1, 2, 4])
+    assert np.array_equal(b[1], [9, 18, 27, 11, 22, 34])
+    assert np.array_equal(b[2], [-3, -2, -1, 4, 2, 1])  # x' = z, z' = -x
+    assert np.array_equal(b[3], [-2, -4, -6, 2, 4, 8])
+    assert np.array_equal(b[4], [10, 20, 30, 10, 20, 30])
+    ext = b[:, 3:] - b[:, :3]
+    want = (np.float32(1e-5) * (np.abs(ext) + np.abs(b[:, :3]) + np.abs(b[:, 3:])) + np.float32(1e-6)).astype(np.float32)
+    assert np.array_equal(pad[:, :3], b[:, :3] - want) and np.array_equal(pad[:, 3:], b[:, 3:] + want)
+
+
+def _check_threaded_tlas(out, boxes):
+    """Structural invariants of a walk-format TLAS over n slot boxes."""
+    n = len(boxes)
+    assert out.shape == (2 * n - 1, 8)
+    f = out.view(np.float32)
+    seen = np.zeros(n, np.int64)
+
+    def walk(i):  # returns (end, slots of the subtree)
+        if out[i, 7] != 0xFFFFFFFF:
+            assert out[i, 7] >> 24 == 1 and out[i, 3] == i + 1
+            s = out[i, 7] & 0xFFFFFF
+            seen[s] += 1
+            assert np.array_equal(f[i, [0, 1, 2, 4, 5, 6]], boxes[s])
+            return i + 1, [s]
+        l_end, ls = walk(i + 1)
+        r_end, rs = walk(l_end)
+        assert out[i, 3] == r_end and r_end == i + 2 * len(ls + rs) - 1
+        allb = boxes[ls + rs]
+        assert np.array_equal(f[i, 0:3], allb[:, :3].min(0)) and np.array_equal(f[i, 4:7], allb[:, 3:].max(0))
+        return r_end, ls + rs
+
+    end, _ = walk(0)
+    assert end == 2 * n - 1 and np.all(seen == 1)
+
+
+def test_sah_tlas_structure_random():
+    rng = np.random.default_rng(21)
+    for n in (1, 2, 3, 7, 64, 257):
+        lo = rng.uniform(-50, 50, (n, 3)).astype(np.float32)
+        boxes = np.concatenate([lo, lo + rng.uniform(0.1, 8, (n, 3)).astype(np.float32)], 1)
+        _check_threaded_tlas(oracle.build_sah_tlas(boxes), boxes)
+    # identical boxes: no axis has centroid extent -> middle splits all the way down
+    boxes = np.tile(np.array([[0, 0, 0, 1, 1, 1]], np.float32), (9, 1))
+    out = oracle.build_sah_tlas(boxes)
+    _check_threaded_tlas(out, boxes)
+    assert out[1, 3] == 1 + 2 * 4 - 1  # left subtree holds 9 // 2 = 4 slots
+
+
+def test_sah_tlas_root_split_is_the_exhaustive_sah_optimum():
+    """Small sets with centroids spread one per bin: the binned sweep must pick the split
+    of least area(L)|L| + area(R)|R| over all axis-sorted partitions (brute force)."""
+    rng = np.random.default_rng(22)
+
+    def area(b):
+        d = b[3:].astype(np.float64) - b[:3].astype(np.float64)
+        return 2 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0])
+
+    for trial in range(40):
+        n = int(rng.integers(2, 9))
+        c = np.stack([rng.permutation(n) for _ in range(3)], 1).astype(np.float32) * 4.0  # distinct per axis
+        h = rng.uniform(0.2, 3.0, (n, 3)).astype(np.float32)
+        boxes = np.concatenate([c - h, c + h], 1)
+        out = oracle.build_sah_tlas(boxes)
+        best = None
+        for a in range(3):
+            order = np.argsort(c[:, a])
+            for k in range(1, n):
+                L, R = boxes[order[:k]], boxes[order[k:]]
+                bl = np.concatenate([L[:, :3].min(0), L[:, 3:].max(0)])
+                br = np.concatenate([R[:, :3].min(0), R[:, 3:].max(0)])
+                cost = area(bl) * k + area(br) * (n - k)
+                if best is None or cost < best[0] - 1e-9:
+                    best = (cost, set(order[:k].tolist()))
+        # the root's left subtree is nodes [1, miss of node 1)
+        left = {int(out[i, 7] & 0xFFFFFF) for i in range(1, int(out[1, 3])) if out[i, 7] != 0xFFFFFFFF}
+        assert left == best[1] or set(range(n)) - left == best[1], (trial, left, best)
+
+
+def test_skin_closed_forms():
+    """Identity bones -> rest positions; one bone of weight 1 -> that rigid transform
+    (a 90 degree turn about y maps (x, y, z) to (z, y, -x) exactly, plus translation);
+    weights (w, 1 - w) over two translations -> the interpolated translation; joints
+    select the palette entry (swapping joints and weights gives the same result)."""
+    rng = np.random.default_rng(23)
+    V = 50
+    p = rng.uniform(-2, 2, (V, 3)).astype(np.float32)
+    nrm = rng.normal(size=(V, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    rest = np.concatenate([p, nrm], 1)
+    eye = np.tile(np.eye(3, 4, dtype=np.float32), (3, 1, 1))
+    j = np.zeros((V, 4), np.uint16)
+    w = np.zeros((V, 4), np.float32)
+    w[:, 0] = 1
+    out = oracle.skin(rest, j, w, eye)
+    assert np.array_equal(out[:, :3], p)
+    np.testing.assert_allclose(out[:, 3:], nrm, atol=2e-7)
+    bones = eye.copy()
+    bones[1] = np.array([[0, 0, 1, 5], [0, 1, 0, -1], [-1, 0, 0, 2]], np.float32)
+    j[:, 0] = 1
+    out = oracle.skin(rest, j, w, bones)
+    assert np.array_equal(out[:, 0], p[:, 2] + 5) and np.array_equal(out[:, 1], p[:, 1] - 1)
+    assert np.array_equal(out[:, 2], -p[:, 0] + 2)
+    np.testing.assert_allclose(out[:, 3:], np.stack([nrm[:, 2], nrm[:, 1], -nrm[:, 0]], 1), atol=2e-7)
+    bones = eye.copy()
+    bones[1, :, 3] = (4.0, 0.0, 0.0)
+    bones[2, :, 3] = (0.0, 0.0, 8.0)
+    j = np.tile(np.array([1, 2, 0, 0], np.uint16), (V, 1))
+    wt = rng.uniform(0, 1, V).astype(np.float32)
+    w = np.stack([wt, 1 - wt, np.zeros(V), np.zeros(V)], 1).astype(np.float32)
+    out = oracle.skin(rest, j, w, bones)
+    np.testing.assert_allclose(out[:, :3], p + np.stack([4 * wt, 0 * wt, 8 * (1 - wt)], 1), atol=1e-5)
+    out2 = oracle.skin(rest, j[:, [1, 0, 2, 3]], w[:, [1, 0, 2, 3]], bones)
+    np.testing.assert_allclose(out2, out, atol=1e-6)
+
+
 # ---------------------------------------------------------------- golden fixture
 def test_golden_room_fixture():
     path = os.path.join(os.path.dirname(__file__), "golden", "room_4x4x4.npz")

@@ -35,6 +35,14 @@ struct CascadeState {
 
 constexpr int kTicketRing = 16;
 
+// A level-synchronous refit of a set of threaded BLAS ranges: the ranges' nodes ordered by
+// height (leaves 0, interior = 1 + the taller child), level h = order[off[h], off[h + 1]).
+struct RefitPlan {
+    uint32_t *d_order = nullptr;
+    size_t cap = 0;
+    std::vector<uint32_t> off, key, h_order;
+};
+
 }  // namespace
 
 struct sbgi_field {
@@ -47,8 +55,8 @@ struct sbgi_field {
     float *d_vdata = nullptr, *d_mats = nullptr, *d_lights = nullptr, *d_inst_bounds = nullptr;
     float4 *d_tri = nullptr, *d_inst = nullptr;
     uint4 *d_tlas = nullptr;
-    int32_t *d_parent = nullptr, *d_tlas_parent = nullptr;
-    uint32_t *d_arrive = nullptr, *d_tlas_arrive = nullptr;
+    int32_t *d_tlas_parent = nullptr;
+    uint32_t *d_tlas_arrive = nullptr;
     // walk layouts: the BLAS pool in the walk's node format (1 copy, or 8 near-first
     // octant copies in MT_ORDERED mode) with the pool node behind each copy node, and
     // the SAH TLAS of MT_ORDERED mode
@@ -65,10 +73,10 @@ struct sbgi_field {
     // skinned vertex sets (sbgi_set_skins): pool vertex per skinned vertex, rest pose
     // {pos, 0}, {normal, 0}, joints (4 x u16), weights, the bone palette, the leaves of the
     // skinned BLASes and their node / triangle ranges
-    uint32_t *d_skin_vtx = nullptr, *d_skin_leaves = nullptr;
+    uint32_t *d_skin_vtx = nullptr;
     float4 *d_skin_rest = nullptr, *d_skin_w = nullptr, *d_bones = nullptr;
     uint2 *d_skin_joints = nullptr, *d_skin_nranges = nullptr, *d_skin_tranges = nullptr;
-    uint32_t skin_vtx_n = 0, skin_bones_n = 0, skin_leaf_n = 0, skin_ranges_n = 0, skin_tranges_n = 0;
+    uint32_t skin_vtx_n = 0, skin_bones_n = 0, skin_ranges_n = 0, skin_tranges_n = 0;
     uint32_t skin_nmax = 0, skin_tmax = 0;
     // bounds of d_nodes are newer than h_nodes after a device refit (structure is equal)
     bool nodes_dirty = false;
@@ -84,7 +92,8 @@ struct sbgi_field {
     // sbgi_skin / sbgi_refit when they update the TLAS)
     hipEvent_t kev[5] = {};
     bool kev_blas = false, kev_tlas = false;
-    uint32_t skin_span_lo = 0, skin_span_hi = 0;
+    // level-synchronous refit plans: the skinned BLASes, and the ranges of the last sbgi_refit
+    RefitPlan skin_plan, call_plan;
     // scene (host mirror, reference layouts)
     std::vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
     std::vector<float> h_vdata, h_mats, h_lights;
@@ -157,16 +166,68 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+void free_plan(RefitPlan &p) {
+    dfree(p.d_order);
+    p.cap = 0;
+    p.off.clear();
+    p.key.clear();
+    p.h_order.clear();
+}
+
+// Build (or reuse, when the ranges are the same as last time) the height-ordered node list
+// of n validated [root, end) ranges.
+int32_t build_plan(sbgi_field *f, const uint32_t *ranges, uint32_t n, RefitPlan &p) {
+    std::vector<uint32_t> key(ranges, ranges + 2 * (size_t)n);
+    if (key == p.key && p.d_order) return SBGI_OK;
+    std::vector<uint32_t> count;
+    size_t total = 0;
+    std::vector<std::vector<uint32_t>> heights(n);
+    for (uint32_t r = 0; r < n; r++) {
+        const uint32_t root = ranges[2 * r], end = ranges[2 * r + 1];
+        std::vector<uint32_t> &H = heights[r];
+        H.assign(end - root, 0u);
+        for (uint32_t i = end; i-- > root;) {
+            const uint32_t *nd = f->h_nodes.data() + (size_t)i * 8;
+            uint32_t hh = 0;
+            if (nd[7] == 0xFFFFFFFFu) {
+                const uint32_t l = i + 1, rr = f->h_nodes[(size_t)l * 8 + 6];
+                hh = std::max(H[l - root], H[rr - root]) + 1;
+            }
+            H[i - root] = hh;
+            if (hh >= count.size()) count.resize(hh + 1, 0u);
+            count[hh]++;
+        }
+        total += end - root;
+    }
+    p.off.assign(count.size() + 1, 0u);
+    for (size_t k = 0; k < count.size(); k++) p.off[k + 1] = p.off[k] + count[k];
+    p.h_order.assign(total, 0u);
+    std::vector<uint32_t> fill(p.off.begin(), p.off.end() - 1);
+    for (uint32_t r = 0; r < n; r++) {
+        const uint32_t root = ranges[2 * r];
+        for (size_t q = 0; q < heights[r].size(); q++) p.h_order[fill[heights[r][q]]++] = root + (uint32_t)q;
+    }
+    if (p.cap < total) {
+        dfree(p.d_order);
+        p.cap = 0;
+        if (dalloc(&p.d_order, total * 4) != hipSuccess) return fail(f, SBGI_E_NOMEM, "refit plan allocation");
+        p.cap = total;
+    }
+    HIP_OK(f, hipMemcpyAsync(p.d_order, p.h_order.data(), total * 4, hipMemcpyHostToDevice, f->stream));
+    p.key = key;
+    return SBGI_OK;
+}
+
 void free_skins(sbgi_field *f) {
     dfree(f->d_skin_vtx);
-    dfree(f->d_skin_leaves);
     dfree(f->d_skin_rest);
     dfree(f->d_skin_w);
     dfree(f->d_bones);
     dfree(f->d_skin_joints);
     dfree(f->d_skin_nranges);
     dfree(f->d_skin_tranges);
-    f->skin_vtx_n = f->skin_bones_n = f->skin_leaf_n = f->skin_ranges_n = f->skin_tranges_n = 0;
+    f->skin_vtx_n = f->skin_bones_n = f->skin_ranges_n = f->skin_tranges_n = 0;
+    free_plan(f->skin_plan);
     f->skin_nmax = f->skin_tmax = 0;
 }
 
@@ -181,9 +242,8 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_tri);
     dfree(f->d_inst);
     dfree(f->d_tlas);
-    dfree(f->d_parent);
     dfree(f->d_tlas_parent);
-    dfree(f->d_arrive);
+    free_plan(f->call_plan);
     dfree(f->d_tlas_arrive);
     dfree(f->tw.queue.rec);
     dfree(f->d_walk_nodes);
@@ -300,18 +360,6 @@ int32_t validate_scene(sbgi_field *f) {
         }
     }
     return SBGI_OK;
-}
-
-void build_parents(const std::vector<uint32_t> &nodes, uint32_t n, std::vector<int32_t> &parent) {
-    parent.assign(n, -1);
-    for (uint32_t i = 0; i < n; i++) {
-        if (nodes[(size_t)i * 8 + 7] != 0xFFFFFFFFu) continue;
-        uint32_t l = i + 1;
-        if (l >= n) continue;
-        uint32_t r = nodes[(size_t)l * 8 + 6];
-        parent[l] = (int32_t)i;
-        if (r < n) parent[r] = (int32_t)i;
-    }
 }
 
 void build_tlas_parents(const float *tl, uint32_t n, std::vector<int32_t> &parent) {
@@ -594,8 +642,7 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     }
     free_scene(f);
     const sbgi_scene_view &v = f->view;
-    std::vector<int32_t> parent, tparent;
-    build_parents(f->h_nodes, v.node_count, parent);
+    std::vector<int32_t> tparent;
     build_tlas_parents(f->h_mats.data() + v.tlas_base, v.tlas_node_count, tparent);
     // world matrices recovered from the inverse rows; exact ones arrive via sbgi_set_instance_transforms
     f->h_world.assign((size_t)v.inst_count * 16, 0.0);
@@ -616,8 +663,6 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
         dalloc(&f->d_tri, (size_t)v.tri_count * 48) != hipSuccess ||
         dalloc(&f->d_inst, (size_t)v.inst_count * 64) != hipSuccess ||
         dalloc(&f->d_tlas, (size_t)v.tlas_node_count * 32) != hipSuccess ||
-        dalloc(&f->d_parent, (size_t)v.node_count * 4) != hipSuccess ||
-        dalloc(&f->d_arrive, (size_t)v.node_count * 4) != hipSuccess ||
         dalloc(&f->d_tlas_parent, (size_t)v.tlas_node_count * 4) != hipSuccess ||
         dalloc(&f->d_tlas_arrive, (size_t)v.tlas_node_count * 4) != hipSuccess) {
         free_scene(f);
@@ -640,7 +685,6 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     HIP_OK(f, up(f->d_mats, f->h_mats.data(), f->h_mats.size() * 4));
     HIP_OK(f, up(f->d_lights, f->h_lights.data(), f->h_lights.size() * 4));
     HIP_OK(f, up(f->d_world, f->h_world.data(), f->h_world.size() * sizeof(double)));
-    HIP_OK(f, up(f->d_parent, parent.data(), parent.size() * 4));
     HIP_OK(f, up(f->d_tlas_parent, tparent.data(), tparent.size() * 4));
     rc = repack_all(f);
     if (rc != SBGI_OK) return rc;
@@ -741,13 +785,10 @@ int32_t sbgi_refit(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges
     }
     int32_t rc = SBGI_OK;
     if (n_ranges && ((rc = record_kev(f, 0)) != SBGI_OK || (rc = record_kev(f, 1)) != SBGI_OK)) return rc;
-    for (uint32_t r = 0; r < n_ranges; r++) {
-        uint32_t root = blas_ranges[2 * r], end = blas_ranges[2 * r + 1];
-        HIP_OK(f, launch_fill_u32(f->d_arrive + root, 0u, end - root, f->stream));
-        HIP_OK(f, launch_refit_blas(f->d_nodes, f->d_tri_index, f->d_vdata, f->d_parent, f->d_arrive, root, end,
-                                    f->stream));
-    }
     if (n_ranges) {
+        if ((rc = build_plan(f, blas_ranges, n_ranges, f->call_plan)) != SBGI_OK) return rc;
+        HIP_OK(f, launch_refit_levels(f->d_nodes, f->d_tri_index, f->d_vdata, f->call_plan.d_order,
+                                      f->call_plan.off.data(), (uint32_t)f->call_plan.off.size() - 1, f->stream));
         HIP_OK(f, launch_oct_gather(f->d_walk_nodes, f->d_nodes, f->d_walk_perm, (ordered(f) ? 8u : 1u) * v.node_count,
                                     f->stream));
         f->nodes_dirty = true;
@@ -855,14 +896,14 @@ int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets
     free_skins(f);
     if (!n_sets) return SBGI_OK;
     // rest pose = the vertexData last provided by the host
-    std::vector<uint32_t> vtx, leaves;
+    std::vector<uint32_t> vtx, plan_ranges;
     std::vector<float4> rest, w;
     std::vector<uint2> joints, nranges, tranges;
     vtx.reserve(nv);
     rest.reserve(2 * nv);
     w.reserve(nv);
     joints.reserve(nv);
-    uint32_t nmax = 0, tmax = 0, span_lo = UINT32_MAX, span_hi = 0;
+    uint32_t nmax = 0, tmax = 0;
     for (uint32_t s = 0; s < n_sets; s++) {
         const sbgi_skin_set &k = sets[s];
         for (uint32_t q = 0; q < k.vertex_count; q++) {
@@ -880,7 +921,6 @@ int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets
         for (uint32_t i = k.blas_root; i < k.blas_end; i++) {
             const uint32_t info = f->h_nodes[(size_t)i * 8 + 7];
             if (info == 0xFFFFFFFFu) continue;
-            leaves.push_back(i);
             const uint32_t off = info & 0x00FFFFFFu, cnt = info >> 24;
             if (cnt) {
                 t0 = std::min(t0, off);
@@ -888,15 +928,15 @@ int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets
             }
         }
         nranges.push_back(make_uint2(k.blas_root, k.blas_end));
+        plan_ranges.push_back(k.blas_root);
+        plan_ranges.push_back(k.blas_end);
         if (t1 > t0) tranges.push_back(make_uint2(t0, t1));
         nmax = std::max(nmax, k.blas_end - k.blas_root);
         if (t1 > t0) tmax = std::max(tmax, t1 - t0);
-        span_lo = std::min(span_lo, k.blas_root);
-        span_hi = std::max(span_hi, k.blas_end);
     }
     if (dalloc(&f->d_skin_vtx, vtx.size() * 4) != hipSuccess || dalloc(&f->d_skin_rest, rest.size() * 16) != hipSuccess ||
         dalloc(&f->d_skin_joints, joints.size() * 8) != hipSuccess || dalloc(&f->d_skin_w, w.size() * 16) != hipSuccess ||
-        dalloc(&f->d_bones, (size_t)n_bones * 48) != hipSuccess || dalloc(&f->d_skin_leaves, leaves.size() * 4) != hipSuccess ||
+        dalloc(&f->d_bones, (size_t)n_bones * 48) != hipSuccess ||
         dalloc(&f->d_skin_nranges, nranges.size() * 8) != hipSuccess ||
         dalloc(&f->d_skin_tranges, std::max<size_t>(1, tranges.size()) * 8) != hipSuccess) {
         free_skins(f);
@@ -906,18 +946,20 @@ int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets
     HIP_OK(f, hipMemcpy(f->d_skin_rest, rest.data(), rest.size() * 16, hipMemcpyHostToDevice));
     HIP_OK(f, hipMemcpy(f->d_skin_joints, joints.data(), joints.size() * 8, hipMemcpyHostToDevice));
     HIP_OK(f, hipMemcpy(f->d_skin_w, w.data(), w.size() * 16, hipMemcpyHostToDevice));
-    HIP_OK(f, hipMemcpy(f->d_skin_leaves, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice));
     HIP_OK(f, hipMemcpy(f->d_skin_nranges, nranges.data(), nranges.size() * 8, hipMemcpyHostToDevice));
     if (!tranges.empty()) HIP_OK(f, hipMemcpy(f->d_skin_tranges, tranges.data(), tranges.size() * 8, hipMemcpyHostToDevice));
     f->skin_vtx_n = (uint32_t)vtx.size();
     f->skin_bones_n = n_bones;
-    f->skin_leaf_n = (uint32_t)leaves.size();
     f->skin_ranges_n = (uint32_t)nranges.size();
     f->skin_tranges_n = (uint32_t)tranges.size();
     f->skin_nmax = nmax;
     f->skin_tmax = tmax;
-    f->skin_span_lo = span_lo;
-    f->skin_span_hi = span_hi;
+    int32_t rc = build_plan(f, plan_ranges.data(), n_sets, f->skin_plan);
+    if (rc != SBGI_OK) {
+        free_skins(f);
+        return rc;
+    }
+    HIP_OK(f, hipStreamSynchronize(f->stream));
     return SBGI_OK;
 }
 
@@ -940,9 +982,8 @@ int32_t sbgi_skin(sbgi_field *f, const float *bones, uint32_t n_bones, int32_t u
     HIP_OK(f, launch_skin(f->d_skin_vtx, f->d_skin_rest, f->d_skin_joints, f->d_skin_w, f->d_bones, f->skin_vtx_n,
                           f->d_vdata, f->stream));
     if ((rc = record_kev(f, 1)) != SBGI_OK) return rc;
-    HIP_OK(f, launch_fill_u32(f->d_arrive + f->skin_span_lo, 0u, f->skin_span_hi - f->skin_span_lo, f->stream));
-    HIP_OK(f, launch_refit_leaves(f->d_nodes, f->d_tri_index, f->d_vdata, f->d_parent, f->d_arrive, f->d_skin_leaves,
-                                  f->skin_leaf_n, f->stream));
+    HIP_OK(f, launch_refit_levels(f->d_nodes, f->d_tri_index, f->d_vdata, f->skin_plan.d_order,
+                                  f->skin_plan.off.data(), (uint32_t)f->skin_plan.off.size() - 1, f->stream));
     HIP_OK(f, launch_oct_gather_ranges(f->d_walk_nodes, f->d_nodes, f->d_walk_perm, v.node_count, ordered(f) ? 8u : 1u,
                                        f->d_skin_nranges, f->skin_ranges_n, f->skin_nmax, f->stream));
     HIP_OK(f, launch_pack_tris_ranges(f->d_tri_index, f->d_vdata, f->d_tri_mat, f->d_mats, v.tri_count,

@@ -2,11 +2,11 @@
 //
 // pack_*: turn the reference buffers (bvhNodes/triIndex/vertexData/triMaterial/materials,
 // js/spectral_scene.js:1188-1366) into the device layouts described in sbgi_dev.h.
-// refit_*: refitFlatBlasRange / refitFlatTlasRange (js/gi_refit.js:34-292) as parallel
-// bottom-up sweeps. Each node's bounds depend only on its children, so instead of the
-// reference's reverse pre-order loop every leaf starts a thread that climbs toward the
-// root; the second child to arrive at a parent (per-node arrival counter) computes the
-// parent's union. min/max are exact, so the result equals the sequential sweep bit for bit.
+// refit: refitFlatBlasRange (js/gi_refit.js:34-165) as a level-synchronous sweep, one
+// launch per node height, and refitFlatTlasRange (js/gi_refit.js:245-292) over the small
+// reference TLAS as a climb in which the second child to arrive at a parent (per-node
+// arrival counter) computes the union. min/max are exact, so both equal the reference's
+// sequential reverse pre-order sweep bit for bit.
 #include "kernels.h"
 
 namespace {
@@ -93,56 +93,6 @@ __device__ __forceinline__ void union_children(float *__restrict__ nf, uint32_t 
     }
 }
 
-// Bottom-up BLAS refit over [root, end), or (leaves != nullptr) from the n_leaves listed
-// leaf nodes up to their BLAS roots (root = 0: every BLAS root's parent is -1). parent[]
-// holds each node's parent (or -1); arrive[] must be zero on entry for every interior
-// node that will be reached. The host has validated the layout.
-__global__ void k_refit_blas(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
-                             const float *__restrict__ vdata, const int32_t *__restrict__ parent,
-                             uint32_t *__restrict__ arrive, uint32_t root, uint32_t end,
-                             const uint32_t *__restrict__ leaves, uint32_t n_leaves) {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t i;
-    if (leaves) {
-        if (x >= n_leaves) return;
-        i = leaves[x];
-    } else {
-        i = root + x;
-        if (i >= end) return;
-    }
-    const uint32_t info = nodes[(size_t)i * 8 + 7];
-    if (info == 0xFFFFFFFFu) return;  // interiors are reached from below
-    float *nf = reinterpret_cast<float *>(nodes);
-    const uint32_t off = info & 0x00FFFFFFu, cnt = info >> 24;
-    if (cnt > 0) {
-        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (uint32_t t = off; t < off + cnt; t++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const float *p = vdata + (size_t)tri_index[(size_t)t * 3 + k] * 8;
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    mn[a] = fminf(mn[a], p[a]);
-                    mx[a] = fmaxf(mx[a], p[a]);
-                }
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            nf[(size_t)i * 8 + a] = mn[a];
-            nf[(size_t)i * 8 + 3 + a] = mx[a];
-        }
-    }
-    int32_t p = parent[i];
-    while (p >= (int32_t)root) {
-        __threadfence();
-        if (atomicAdd(&arrive[p], 1u) == 0u) return;  // sibling not done yet: it will continue
-        __threadfence();
-        union_children(nf, (uint32_t)p, nodes, 8, 6, false);
-        p = parent[p];
-    }
-}
-
 // Bottom-up TLAS refit from per-slot instance world bounds (6 floats each).
 __global__ void k_refit_tlas(float *__restrict__ mats, uint32_t tlas_base, uint32_t tlas_count,
                              const float *__restrict__ inst_bounds, const int32_t *__restrict__ parent,
@@ -201,7 +151,67 @@ __global__ void k_oct_gather(uint32_t *__restrict__ oct, const uint32_t *__restr
     dst[6] = src[5];
 }
 
+// One height level of a bottom-up BLAS refit (refitFlatBlasRange, js/gi_refit.js:34-165):
+// order[0, n) lists nodes of equal height (leaves are height 0, an interior node is one
+// above its taller child), so every child was finished by an earlier launch and the
+// kernel boundary makes its bounds visible: no atomics and no fences. A leaf takes the
+// bounds of its triangles (an empty leaf keeps its bounds, gi_refit.js:77-85); an interior
+// node the union of its children (left = i + 1, right = the left child's miss).
+__global__ void k_refit_level(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
+                              const float *__restrict__ vdata, const uint32_t *__restrict__ order, uint32_t n) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    const uint32_t i = order[x];
+    float *nf = reinterpret_cast<float *>(nodes);
+    const uint32_t info = nodes[(size_t)i * 8 + 7];
+    float mn[3], mx[3];
+    if (info != 0xFFFFFFFFu) {
+        const uint32_t off = info & 0x00FFFFFFu, cnt = info >> 24;
+        if (cnt == 0) return;
+        for (int a = 0; a < 3; a++) {
+            mn[a] = INFINITY;
+            mx[a] = -INFINITY;
+        }
+        for (uint32_t t = off; t < off + cnt; t++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float *p = vdata + (size_t)tri_index[(size_t)t * 3 + k] * 8;
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    mn[a] = fminf(mn[a], p[a]);
+                    mx[a] = fmaxf(mx[a], p[a]);
+                }
+            }
+        }
+    } else {
+        const size_t l = (size_t)i + 1, r = nodes[l * 8 + 6];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(nf[l * 8 + a], nf[r * 8 + a]);
+            mx[a] = fmaxf(nf[l * 8 + 3 + a], nf[r * 8 + 3 + a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        nf[(size_t)i * 8 + a] = mn[a];
+        nf[(size_t)i * 8 + 3 + a] = mx[a];
+    }
+}
+
 }  // namespace
+
+hipError_t launch_refit_levels(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const uint32_t *order,
+                               const uint32_t *level_off, uint32_t levels, hipStream_t st) {
+    for (uint32_t h = 0; h < levels; h++) {
+        const uint32_t n = level_off[h + 1] - level_off[h];
+        if (!n) continue;
+        hipLaunchKernelGGL(k_refit_level, dim3((n + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata,
+                           order + level_off[h], n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
 
 hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_total,
                              hipStream_t st) {
@@ -247,23 +257,6 @@ hipError_t launch_pack_inst(const float *mats, uint32_t inst_base, uint32_t inst
     if (!inst_count) return hipSuccess;
     hipLaunchKernelGGL(k_pack_inst, dim3((inst_count + 255) / 256), dim3(256), 0, st, mats, inst_base, inst_count,
                        inst);
-    return hipGetLastError();
-}
-
-hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
-                             uint32_t *arrive, uint32_t root, uint32_t end, hipStream_t st) {
-    if (end <= root) return hipSuccess;
-    const uint32_t n = end - root;
-    hipLaunchKernelGGL(k_refit_blas, dim3((n + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata, parent, arrive,
-                       root, end, nullptr, 0u);
-    return hipGetLastError();
-}
-
-hipError_t launch_refit_leaves(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
-                               uint32_t *arrive, const uint32_t *leaves, uint32_t n_leaves, hipStream_t st) {
-    if (!n_leaves) return hipSuccess;
-    hipLaunchKernelGGL(k_refit_blas, dim3((n_leaves + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata, parent,
-                       arrive, 0u, 0u, leaves, n_leaves);
     return hipGetLastError();
 }
 

@@ -16,8 +16,8 @@
 //   over m slots has 2m − 1 nodes), so every node of a level is independent and a level
 //   is processed in parallel, one wave per node, with the slot ids partitioned stably
 //   from one ping-pong array into the other. Bin counts and bounds are LDS atomics (min
-//   and max through an order-preserving float→u32 map); the SAH sweep runs one lane per
-//   axis in double. Every decision (bin index, sweep order, strict "<" across axes and
+//   and max through an order-preserving float→u32 map); the SAH sweep is a segmented
+//   prefix/suffix scan across lanes with costs in double. Every decision (bin index, sweep order, strict "<" across axes and
 //   bins, stable partition, middle split when no axis has extent) is deterministic, so
 //   the tree does not depend on the schedule of the waves.
 #pragma clang fp contract(off)
@@ -118,8 +118,6 @@ __global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restri
                                                            uint4 *__restrict__ tlas) {
     __shared__ uint32_t s_count[2];
     __shared__ uint32_t s_bins[kTlasWaves][3 * kTlasBins * kBinWords];
-    __shared__ double s_ra[kTlasWaves][3 * kTlasBins];
-    __shared__ uint32_t s_rc[kTlasWaves][3 * kTlasBins];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     for (uint32_t i = tid; i < n; i += kTlasBlock) ids0[i] = i;
     if (tid == 0) {
@@ -129,8 +127,6 @@ __global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restri
     }
     __syncthreads();
     uint32_t *bins = s_bins[wave];
-    double *ra = s_ra[wave];
-    uint32_t *rc = s_rc[wave];
     for (uint32_t level = 0;; level++) {
         const uint32_t cur = level & 1u, nxt = cur ^ 1u;
         const uint32_t ntask = s_count[cur];
@@ -192,58 +188,81 @@ __global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restri
                 }
             }
             wave_sync();
-            // SAH sweep, one lane per axis: right-to-left prefix areas and counts, then the
-            // left-to-right scan keeping the first minimum
+            // SAH sweep across lanes: lane k of a 32-lane segment owns bin k of one axis
+            // (axes 0 and 1, then axis 2), an inclusive prefix scan gives the left side
+            // union(bins[0..k]) and a suffix scan the right side union(bins[k+1..31]); the
+            // cost of the split after bin k is area(L)|L| + area(R)|R|, and the wave arg-min
+            // over (cost, 32 axis + k) keeps the first minimum in axis, then bin, order.
             double best = INFINITY;
-            int best_bin = -1;
-            if (lane < 3) {
-                const int a = (int)lane;
+            int best_idx = 0x7FFFFFFF;
+            const int k = (int)(lane & 31u);
+            for (int pass = 0; pass < 2; pass++) {
+                const int a = pass == 0 ? (int)(lane >> 5) : 2;
                 const float e = a == 0 ? ext[0] : (a == 1 ? ext[1] : ext[2]);
-                if (e > 0) {
-                    const uint32_t *ab = bins + a * kTlasBins * kBinWords;
-                    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-                    uint32_t c = 0;
-                    for (int k = kTlasBins - 1; k > 0; k--) {
-                        const uint32_t *b = ab + k * kBinWords;
-                        for (int q = 0; q < 3; q++) {
-                            mn[q] = fmin_keep(mn[q], o2f(b[1 + q]));
-                            mx[q] = fmax_keep(mx[q], o2f(b[4 + q]));
-                        }
-                        c += b[0];
-                        ra[a * kTlasBins + k] = box_area(mn, mx);
-                        rc[a * kTlasBins + k] = c;
+                const bool axis_ok = (pass == 0 || lane < 32) && e > 0;
+                uint32_t pc = 0;
+                float pmn[3] = {INFINITY, INFINITY, INFINITY}, pmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+                if (axis_ok) {
+                    const uint32_t *b = bins + (a * kTlasBins + k) * kBinWords;
+                    pc = b[0];
+                    for (int q = 0; q < 3; q++) {
+                        pmn[q] = o2f(b[1 + q]);
+                        pmx[q] = o2f(b[4 + q]);
                     }
-                    float lmn[3] = {INFINITY, INFINITY, INFINITY}, lmx[3] = {-INFINITY, -INFINITY, -INFINITY};
-                    uint32_t lc = 0;
-                    for (int k = 0; k < kTlasBins - 1; k++) {
-                        const uint32_t *b = ab + k * kBinWords;
+                }
+                uint32_t sc = pc;
+                float smn[3] = {pmn[0], pmn[1], pmn[2]}, smx[3] = {pmx[0], pmx[1], pmx[2]};
+                for (int d = 1; d < kTlasBins; d <<= 1) {
+                    const uint32_t oc = (uint32_t)__shfl_up((int)pc, d, kTlasBins);
+                    const uint32_t uc = (uint32_t)__shfl_down((int)sc, d, kTlasBins);
+                    float omn[3], omx[3], umn[3], umx[3];
+                    for (int q = 0; q < 3; q++) {
+                        omn[q] = __shfl_up(pmn[q], d, kTlasBins);
+                        omx[q] = __shfl_up(pmx[q], d, kTlasBins);
+                        umn[q] = __shfl_down(smn[q], d, kTlasBins);
+                        umx[q] = __shfl_down(smx[q], d, kTlasBins);
+                    }
+                    if (k >= d) {
+                        pc += oc;
                         for (int q = 0; q < 3; q++) {
-                            lmn[q] = fmin_keep(lmn[q], o2f(b[1 + q]));
-                            lmx[q] = fmax_keep(lmx[q], o2f(b[4 + q]));
+                            pmn[q] = fmin_keep(pmn[q], omn[q]);
+                            pmx[q] = fmax_keep(pmx[q], omx[q]);
                         }
-                        lc += b[0];
-                        const uint32_t r = rc[a * kTlasBins + k + 1];
-                        if (!lc || !r) continue;
-                        const double cost = box_area(lmn, lmx) * lc + ra[a * kTlasBins + k + 1] * r;
-                        if (cost < best) {
-                            best = cost;
-                            best_bin = k;
+                    }
+                    if (k + d < kTlasBins) {
+                        sc += uc;
+                        for (int q = 0; q < 3; q++) {
+                            smn[q] = fmin_keep(smn[q], umn[q]);
+                            smx[q] = fmax_keep(smx[q], umx[q]);
                         }
                     }
                 }
-            }
-            // axes in order, strict "<": ties keep the lower axis, then the lower bin
-            int best_axis = -1, bbin = -1;
-            double bcost = INFINITY;
-            for (int a = 0; a < 3; a++) {
-                const double ca = __shfl(best, a);
-                const int ka = __shfl(best_bin, a);
-                if (ka >= 0 && ca < bcost) {
-                    bcost = ca;
-                    best_axis = a;
-                    bbin = ka;
+                // right side of the split after bin k = the suffix starting at bin k + 1
+                const uint32_t rc = (uint32_t)__shfl_down((int)sc, 1, kTlasBins);
+                float rmn[3], rmx[3];
+                for (int q = 0; q < 3; q++) {
+                    rmn[q] = __shfl_down(smn[q], 1, kTlasBins);
+                    rmx[q] = __shfl_down(smx[q], 1, kTlasBins);
+                }
+                if (axis_ok && k < kTlasBins - 1 && pc && rc) {
+                    const double cost = box_area(pmn, pmx) * pc + box_area(rmn, rmx) * rc;
+                    const int idx = a * kTlasBins + k;
+                    if (cost < best || (cost == best && idx < best_idx)) {
+                        best = cost;
+                        best_idx = idx;
+                    }
                 }
             }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double oc = __shfl_xor(best, o);
+                const int oi = __shfl_xor(best_idx, o);
+                if (oc < best || (oc == best && oi < best_idx)) {
+                    best = oc;
+                    best_idx = oi;
+                }
+            }
+            const int best_axis = best_idx == 0x7FFFFFFF ? -1 : best_idx / kTlasBins;
+            const int bbin = best_idx == 0x7FFFFFFF ? -1 : best_idx % kTlasBins;
             uint32_t mid;
             if (best_axis < 0) {
                 mid = lo + m / 2;

@@ -65,9 +65,6 @@ hipError_t launch_pack_tris(const uint32_t *tri_index, const float *vdata, const
                             uint32_t tri_count, float4 *tri, hipStream_t st);
 hipError_t launch_pack_tlas(const float *mats, uint32_t tlas_base, uint32_t tlas_count, uint4 *tlas, hipStream_t st);
 hipError_t launch_pack_inst(const float *mats, uint32_t inst_base, uint32_t inst_count, float4 *inst, hipStream_t st);
-// BLAS refit of the threaded range [root, end): bottom-up with per-node arrival counters.
-hipError_t launch_refit_blas(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
-                             uint32_t *arrive, uint32_t root, uint32_t end, hipStream_t st);
 // walk-format node copies: bounds of copy node i <- pool node perm[i] (after a refit)
 hipError_t launch_oct_gather(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_total,
                              hipStream_t st);
@@ -81,8 +78,10 @@ hipError_t launch_pack_tris_ranges(const uint32_t *tri_index, const float *vdata
 hipError_t launch_oct_gather_ranges(uint32_t *oct_nodes, const uint32_t *nodes, const uint32_t *perm, uint32_t n_nodes,
                                     uint32_t copies, const uint2 *ranges, uint32_t n_ranges, uint32_t max_len,
                                     hipStream_t st);
-hipError_t launch_refit_leaves(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const int32_t *parent,
-                               uint32_t *arrive, const uint32_t *leaves, uint32_t n_leaves, hipStream_t st);
+// level-synchronous BLAS refit: order lists nodes by height, level h is
+// order[level_off[h], level_off[h + 1]) (level_off on the host, levels + 1 entries)
+hipError_t launch_refit_levels(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const uint32_t *order,
+                               const uint32_t *level_off, uint32_t levels, hipStream_t st);
 
 // k_tlas.hip — instance world bounds (and the inflated copy the SAH build reads) and the
 // one-workgroup SAH TLAS build: tlas receives 2n - 1 walk-format nodes

@@ -356,6 +356,33 @@ int32_t sbgi_set_walk_tuning(sbgi_field *f, const sbgi_walk_tuning *t);
  * temporal state and atlases are not touched. */
 int32_t sbgi_autotune(sbgi_field *f, const sbgi_tick_params *p, sbgi_walk_tuning *best, float *best_ms);
 
+/* ---- batched solves ---------------------------------------------------------
+ * No reference counterpart: the reference updates one field per ordered computeAsync
+ * pass (js/gi_probes.js:3836). A batch groups fields of uneven size (own scenes,
+ * cascades, ray counts) on one device; a frame updates every cascade of every field with
+ * ONE persistent closest-hit walk and ONE persistent shadow walk over all (field, cascade)
+ * segments (a device-side work queue), per-segment emission, shading, blend and upload.
+ * A frame whose kernel arguments equal the last captured frame's replays one hipGraph;
+ * the first frame, frames with needs_clear and frames after parameter changes or scene
+ * edits that moved a buffer run eagerly and re-capture. While in a batch, fields are
+ * updated only through it (edits on their own streams are ordered before / after frames). */
+typedef struct sbgi_batch sbgi_batch;
+int32_t sbgi_batch_create(sbgi_field *const *fields, uint32_t n, sbgi_batch **out);
+/* params[i] for field i: cascade, probe_offset and updated_count are ignored (full
+ * updates); needs_clear clears field i's atlases and starts its blend from scratch. */
+int32_t sbgi_batch_tick(sbgi_batch *b, const sbgi_tick_params *params, uint64_t *ticket);
+int32_t sbgi_batch_wait(sbgi_batch *b);
+/* Device time of the last frame (ms) and whether it replayed the captured graph. */
+int32_t sbgi_batch_timing(sbgi_batch *b, float *ms, int32_t *replayed);
+/* Threads per block of the batched walks: 256 or 512. */
+int32_t sbgi_batch_set_walk_block(sbgi_batch *b, uint32_t block);
+/* Autotuner: coordinate descent over the walk schedule (shared by the batch's fields) and
+ * the walk block size, timing eager frames; keeps and returns the fastest. */
+int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_walk_tuning *best,
+                            uint32_t *best_block, float *best_ms);
+const char *sbgi_batch_last_error(const sbgi_batch *b);
+int32_t sbgi_batch_destroy(sbgi_batch *b);
+
 /* Last error message of this field (or of the last failed sbgi_create when f is NULL). */
 const char *sbgi_last_error(const sbgi_field *f);
 

@@ -38,6 +38,8 @@ EXPORTS = [
     "sbgi_sb_last_error", "sbgi_sb_destroy", "sbgi_diff_param_count", "sbgi_diff_albedo_map", "sbgi_diff_forward",
     "sbgi_diff_adjoint", "sbgi_diff_timing", "sbgi_get_walk_tuning", "sbgi_set_walk_tuning", "sbgi_autotune",
     "sbgi_set_skins", "sbgi_skin", "sbgi_edit_timing", "sbgi_sb_mesh_ranges",
+    "sbgi_batch_create", "sbgi_batch_tick", "sbgi_batch_wait", "sbgi_batch_timing", "sbgi_batch_set_walk_block",
+    "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy",
 ]
 
 
@@ -156,6 +158,15 @@ def lib() -> C.CDLL:
         "sbgi_set_skins": (i32, [vp, C.POINTER(SkinSet), u32, u32]),
         "sbgi_skin": (i32, [vp, vp, u32, i32]),
         "sbgi_edit_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "sbgi_batch_create": (i32, [C.POINTER(vp), u32, C.POINTER(vp)]),
+        "sbgi_batch_tick": (i32, [vp, C.POINTER(TickParams), C.POINTER(C.c_uint64)]),
+        "sbgi_batch_wait": (i32, [vp]),
+        "sbgi_batch_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(i32)]),
+        "sbgi_batch_set_walk_block": (i32, [vp, u32]),
+        "sbgi_batch_autotune": (i32, [vp, C.POINTER(TickParams), C.POINTER(WalkTuning), C.POINTER(u32),
+                                      C.POINTER(C.c_float)]),
+        "sbgi_batch_last_error": (C.c_char_p, [vp]),
+        "sbgi_batch_destroy": (i32, [vp]),
         "sbgi_last_error": (C.c_char_p, [vp]),
         "sbgi_destroy": (i32, [vp]),
         "sbgi_sb_create": (i32, [C.POINTER(vp)]),

@@ -18,153 +18,13 @@
 #include <thread>
 #include <vector>
 
+#include "field_impl.h"
 #include "kernels.h"
 #include "sbgi_host.h"
 
-namespace {
-
-thread_local std::string g_create_error;
-
-struct CascadeState {
-    sbgi_cascade_desc d{};
-    uint32_t total = 0, aw = 0, ah = 0, daw = 0, dah = 0, tile_d = 8;
-    float max_dist = 0.f, cell_min = 0.f;
-    float *irr = nullptr, *depth = nullptr, *meta = nullptr;
-    ushort4 *atlas = nullptr, *datlas = nullptr;
-};
-
-constexpr int kTicketRing = 16;
-
-// A level-synchronous refit of a set of threaded BLAS ranges: the ranges' nodes ordered by
-// height (leaves 0, interior = 1 + the taller child), level h = order[off[h], off[h + 1]).
-struct RefitPlan {
-    uint32_t *d_order = nullptr;
-    size_t cap = 0;
-    std::vector<uint32_t> off, key, h_order;
-};
-
-}  // namespace
-
-struct sbgi_field {
-    sbgi_field_desc desc{};
-    hipStream_t stream = nullptr;
-    CascadeState casc[SBGI_MAX_CASCADES];
-    uint32_t max_total = 0;
-    // scene (device)
-    uint32_t *d_nodes = nullptr, *d_tri_index = nullptr, *d_tri_mat = nullptr;
-    float *d_vdata = nullptr, *d_mats = nullptr, *d_lights = nullptr, *d_inst_bounds = nullptr;
-    float4 *d_tri = nullptr, *d_inst = nullptr;
-    uint4 *d_tlas = nullptr;
-    int32_t *d_tlas_parent = nullptr;
-    uint32_t *d_tlas_arrive = nullptr;
-    // walk layouts: the BLAS pool in the walk's node format (1 copy, or 8 near-first
-    // octant copies in MT_ORDERED mode) with the pool node behind each copy node, and
-    // the SAH TLAS of MT_ORDERED mode
-    uint32_t *d_walk_nodes = nullptr, *d_walk_perm = nullptr;
-    uint4 *d_tlas_sah = nullptr;
-    uint32_t tlas_sah_count = 0;
-    // per-frame TLAS on the device: world matrices (double, column-major 4x4 per slot),
-    // the inflated bounds the SAH build reads, its scratch, and the staging area of
-    // instance-record uploads {worlds f64[16 n], records f32[28 n], slots u32[n]}
-    double *d_world = nullptr;
-    float *d_inst_pad = nullptr;
-    void *d_tlas_scratch = nullptr;
-    void *d_inst_stage = nullptr;
-    // skinned vertex sets (sbgi_set_skins): pool vertex per skinned vertex, rest pose
-    // {pos, 0}, {normal, 0}, joints (4 x u16), weights, the bone palette, the leaves of the
-    // skinned BLASes and their node / triangle ranges
-    uint32_t *d_skin_vtx = nullptr;
-    float4 *d_skin_rest = nullptr, *d_skin_w = nullptr, *d_bones = nullptr;
-    uint2 *d_skin_joints = nullptr, *d_skin_nranges = nullptr, *d_skin_tranges = nullptr;
-    uint32_t skin_vtx_n = 0, skin_bones_n = 0, skin_ranges_n = 0, skin_tranges_n = 0;
-    uint32_t skin_nmax = 0, skin_tmax = 0;
-    // bounds of d_nodes are newer than h_nodes after a device refit (structure is equal)
-    bool nodes_dirty = false;
-    // pinned upload staging ring (sbgi_set_instance_transforms, sbgi_skin)
-    static constexpr int kStageSlots = 4;
-    void *stage_host[kStageSlots] = {};
-    size_t stage_cap[kStageSlots] = {};
-    hipEvent_t stage_done[kStageSlots] = {};
-    int stage_next = 0;
-    // device timing of the last scene edits: kev[0..2] before skinning, after it, after the
-    // BLAS refit and walk-copy update (sbgi_skin, sbgi_refit); kev[3..4] around the
-    // instance-record scatter, instance bounds and TLAS (sbgi_set_instance_transforms, and
-    // sbgi_skin / sbgi_refit when they update the TLAS)
-    hipEvent_t kev[5] = {};
-    bool kev_blas = false, kev_tlas = false;
-    // level-synchronous refit plans: the skinned BLASes, and the ranges of the last sbgi_refit
-    RefitPlan skin_plan, call_plan;
-    // scene (host mirror, reference layouts)
-    std::vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
-    std::vector<float> h_vdata, h_mats, h_lights;
-    std::vector<double> h_world;  // per TLAS slot, column-major 4x4
-    sbgi_scene_view view{};
-    bool has_scene = false;
-    // solve scratch
-    float4 *rays = nullptr, *dirs = nullptr;
-    int2 *hit_ids = nullptr;
-    TraceScratch tw{};
-    size_t queue_capacity = 0;
-    void *tables = nullptr;
-    BlendTables bt{};
-    Counters *counters = nullptr;
-    bool count_mode = false;
-    uint32_t last_updated = 0, last_cascade = 0;
-    // tickets and timing
-    hipEvent_t ring[kTicketRing] = {};
-    hipEvent_t ev[5] = {};
-    hipEvent_t tev[4] = {};  // trace sub-stages
-    sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk};
-    // differentiable mode (sbgi_diff_*): parameters, the first update's FP32 atlas, the
-    // gradients and the adjoint's scratch; the frozen hits and visibility masks are the
-    // trace scratch tw of the last diff forward (diff_valid until any other solve)
-    float *d_albedo = nullptr, *d_light_scale = nullptr, *d_gE = nullptr, *d_galb = nullptr, *d_glight = nullptr;
-    int2 *d_albedo_map = nullptr;
-    float4 *d_atlas1 = nullptr;
-    void *d_adj_scratch = nullptr;
-    uint32_t diff_albedo_n = 0;
-    std::vector<int32_t> h_albedo_map;
-    bool diff_ready = false, diff_valid = false;
-    sbgi_tick_params diff_p{};
-    hipEvent_t dev[4] = {};
-    float ms_diff_fwd = 0.f, ms_diff_adj = 0.f;
-    uint64_t ticket = 0;
-    bool stats_valid = false;
-    std::string err;
-};
+using namespace sbgi;
 
 namespace {
-
-int32_t fail(sbgi_field *f, int32_t code, const std::string &msg) {
-    if (f)
-        f->err = msg;
-    else
-        g_create_error = msg;
-    return code;
-}
-
-int32_t hip_fail(sbgi_field *f, hipError_t e, const char *what) {
-    return fail(f, SBGI_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-#define HIP_OK(f, expr)                                          \
-    do {                                                         \
-        hipError_t e_ = (expr);                                  \
-        if (e_ != hipSuccess) return hip_fail((f), e_, #expr);   \
-    } while (0)
-
-template <typename T>
-hipError_t dalloc(T **p, size_t bytes) {
-    *p = nullptr;
-    if (bytes == 0) return hipSuccess;
-    return hipMalloc(reinterpret_cast<void **>(p), bytes);
-}
-
-template <typename T>
-void dfree(T *&p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
 
 void free_plan(RefitPlan &p) {
     dfree(p.d_order);
@@ -268,52 +128,6 @@ void free_scene(sbgi_field *f) {
     f->nodes_dirty = false;
     f->kev_blas = f->kev_tlas = false;
     f->has_scene = false;
-}
-
-DevCascade make_cascade(const sbgi_field *f, const CascadeState &cs, uint32_t offset, uint32_t updated) {
-    DevCascade c{};
-    for (int a = 0; a < 3; a++) {
-        c.gmin[a] = cs.d.grid_min[a];
-        c.gsize[a] = cs.d.grid_size[a];
-        c.res[a] = cs.d.res[a];
-    }
-    c.total = cs.total;
-    c.offset = offset;
-    c.updated = updated;
-    c.rays = f->desc.rays_per_probe;
-    c.depth_oct = f->desc.depth_oct_res;
-    c.max_dist = cs.max_dist;
-    c.cell_min = cs.cell_min;
-    c.aw = cs.aw;
-    c.ah = cs.ah;
-    c.daw = cs.daw;
-    c.dah = cs.dah;
-    return c;
-}
-
-bool ordered(const sbgi_field *f) { return f->desc.traversal == SBGI_TRAV_MT_ORDERED; }
-
-DevScene make_scene(const sbgi_field *f) {
-    DevScene s{};
-    s.blas = reinterpret_cast<const uint4 *>(f->d_nodes);
-    s.nodes = reinterpret_cast<const uint4 *>(f->d_walk_nodes);
-    s.blas_octants = ordered(f) ? 8u : 1u;
-    s.tri = f->d_tri;
-    s.tlas = ordered(f) ? f->d_tlas_sah : f->d_tlas;
-    s.inst = f->d_inst;
-    s.mats = f->d_mats;
-    s.tri_mat = f->d_tri_mat;
-    s.lights = f->d_lights;
-    s.node_count = f->view.node_count;
-    s.tlas_count = ordered(f) ? f->tlas_sah_count : f->view.tlas_node_count;
-    s.light_count = f->view.light_count;
-    s.tri_count = f->view.tri_count;
-    s.inst_count = f->view.inst_count;
-    s.tri_ratio = f->tuning.tri_ratio;
-    s.inst_ratio = f->tuning.inst_ratio;
-    s.refill_min = f->tuning.refill_min;
-    s.fetch_chunk = f->tuning.fetch_chunk;
-    return s;
 }
 
 bool is_int(float v) { return std::isfinite(v) && v == std::floor(v); }

@@ -115,6 +115,43 @@ __global__ void __launch_bounds__(kWalkBlock) k_walk(DevScene s, Src src, const 
     }
 }
 
+// Batched walk: one segment per (field, cascade) solve. A wave starts on the segment that
+// holds its share of the frame's probe rays (prefix), walks it with walk_wave until the
+// segment's rays are all taken, marks it done and moves on to the next unfinished segment,
+// so waves that finish early take work from segments that are still running. The scene
+// changes only between segments, when every lane of the wave is idle.
+template <class Src>
+struct WalkSeg {
+    DevScene s;
+    Src src;
+    const uint32_t *n_dev;  // device ray count (shadow queue) or nullptr
+    uint32_t n_host;
+    uint32_t *heads;
+};
+
+template <int MODE, int BLOCK, class Src>
+__global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__restrict__ segs, uint32_t n_seg,
+                                                      const uint32_t *__restrict__ prefix, uint32_t total,
+                                                      uint32_t *__restrict__ done) {
+    const uint32_t waves = gridDim.x * (BLOCK / 64u), wave_id = blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6);
+    const uint32_t target = (uint32_t)(((uint64_t)total * wave_id) / waves);
+    uint32_t lo = 0, hi = n_seg;  // last segment whose prefix <= target
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (prefix[mid] <= target) lo = mid;
+        else hi = mid;
+    }
+    TravCounters<false> cnt;
+    for (uint32_t k = 0; k < n_seg; k++) {
+        const uint32_t j = lo + k < n_seg ? lo + k : lo + k - n_seg;
+        if (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+        const WalkSeg<Src> &g = segs[j];
+        const uint32_t n = g.n_dev ? min(*g.n_dev, g.n_host) : g.n_host;
+        walk_wave<MODE, false, false>(g.s, g.src, n, g.heads, wave_id, cnt);
+        if ((threadIdx.x & 63u) == 0) __hip_atomic_store(&done[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Shadow-ray emission: for every primary hit and every light that passes the reference's
 // gate (n.l > 0, in range, not band-gated), one record is appended to the queue. A block
 // counts its records (per wave, per light, by ballot), reserves them with ONE atomic, and
@@ -288,6 +325,62 @@ uint32_t persistent_blocks(K kernel, uint32_t block, size_t lds, uint32_t work_i
 }
 
 }  // namespace
+
+size_t walk_tables_bytes(uint32_t n_seg) {
+    return (size_t)n_seg * (sizeof(WalkSeg<PrimarySrc>) + sizeof(WalkSeg<ShadowSrc>));
+}
+
+void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char> &blob) {
+    blob.assign(walk_tables_bytes(n_seg), 0);
+    auto *pr = reinterpret_cast<WalkSeg<PrimarySrc> *>(blob.data());
+    auto *sh = reinterpret_cast<WalkSeg<ShadowSrc> *>(blob.data() + (size_t)n_seg * sizeof(WalkSeg<PrimarySrc>));
+    for (uint32_t j = 0; j < n_seg; j++) {
+        const TraceSeg &g = segs[j];
+        pr[j] = WalkSeg<PrimarySrc>{g.s, PrimarySrc{g.c, g.classify, g.dirs, g.meta, g.w.hits}, nullptr,
+                                    g.c.updated * g.c.rays, g.heads};
+        sh[j] = WalkSeg<ShadowSrc>{g.s, ShadowSrc{g.w.queue.rec, g.w.vis}, g.w.queue.count, g.w.queue.capacity,
+                                   g.heads + (size_t)kFetchHeads * kHeadStride};
+    }
+}
+
+hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const uint32_t *prefix, uint32_t total,
+                             uint32_t *done, uint32_t block, hipStream_t st) {
+    if (!n_seg || !total) return hipSuccess;
+    const auto *pr = static_cast<const WalkSeg<PrimarySrc> *>(tables);
+    const auto *sh = reinterpret_cast<const WalkSeg<ShadowSrc> *>(static_cast<const unsigned char *>(tables) +
+                                                                 (size_t)n_seg * sizeof(WalkSeg<PrimarySrc>));
+    auto go = [&](auto kern, uint32_t bs, auto table) {
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, bs, 0, total)), dim3(bs), 0, st, table, n_seg, prefix,
+                           total, done);
+    };
+    if (mode == 0) {
+        if (block == 256) go(k_walk_batch<WALK_CLOSEST, 256, PrimarySrc>, 256u, pr);
+        else go(k_walk_batch<WALK_CLOSEST, 512, PrimarySrc>, 512u, pr);
+    } else {
+        if (block == 256) go(k_walk_batch<WALK_ANY, 256, ShadowSrc>, 256u, sh);
+        else go(k_walk_batch<WALK_ANY, 512, ShadowSrc>, 512u, sh);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                       const float *meta, const float4 *hits, const ShadowQueue &q, hipStream_t st) {
+    const uint32_t n = c.updated * c.rays;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_emit, dim3((n + kEmitBlock - 1) / kEmitBlock), dim3(kEmitBlock), 0, st, s, c, p, dirs, meta,
+                       hits, q, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                          const float *meta, const ushort4 *prev_atlas, const float4 *hits,
+                          const unsigned long long *vis, float4 *rays, hipStream_t st) {
+    const uint32_t n = c.updated * c.rays;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
+                       prev_atlas, nullptr, hits, vis, rays, nullptr);
+    return hipGetLastError();
+}
 
 size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count) {
     return ((size_t)tlas_count * 2 + (size_t)inst_count * 4) * sizeof(uint4);

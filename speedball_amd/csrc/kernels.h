@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "sbgi_dev.h"
 
 // k_trace.hip
@@ -36,6 +38,34 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
                         const float *meta, const ushort4 *prev_atlas, const float4 *prev_atlas_f32,
                         const TraceScratch &w, float4 *rays, int2 *hit_ids, Counters *counters, const hipEvent_t *ev,
                         hipStream_t st);
+
+// Batched trace (batch.hip): one segment per (field, cascade) solve of a frame. The two
+// walks of a frame are single persistent launches over all segments; emission and
+// shading run per segment.
+struct TraceSeg {
+    DevScene s;
+    DevCascade c;
+    float classify;
+    const float4 *dirs;
+    const float *meta;
+    TraceScratch w;   // this segment's hits, visibility masks and shadow queue
+    uint32_t *heads;  // this segment's fetch counters: primary walk, then shadow walk
+};
+// device walk tables of n segments (primary, then shadow) serialised into blob, which the
+// caller uploads to a buffer of walk_tables_bytes(n) bytes
+size_t walk_tables_bytes(uint32_t n_seg);
+void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char> &blob);
+// mode 0: closest-hit walk over every segment's probe rays; 1: any-hit walk over every
+// segment's shadow queue. prefix[j] = probe rays before segment j (n_seg + 1 entries, on
+// the device) spreads the waves' first segments by work; done[n_seg] flags are cleared by
+// the caller. block: 256 or 512 threads.
+hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const uint32_t *prefix, uint32_t total,
+                             uint32_t *done, uint32_t block, hipStream_t st);
+hipError_t launch_emit(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                       const float *meta, const float4 *hits, const ShadowQueue &q, hipStream_t st);
+hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
+                          const float *meta, const ushort4 *prev_atlas, const float4 *hits,
+                          const unsigned long long *vis, float4 *rays, hipStream_t st);
 
 // k_blend.hip
 struct BlendTables {

@@ -249,3 +249,70 @@ class ProbeField:
         w = np.ascontiguousarray(np.stack([np.asarray(m, np.float64).T.reshape(-1) for m in worlds]))
         a = None if active is None else np.ascontiguousarray(np.asarray(active, np.uint8))
         self._chk(self.L.sbgi_set_instance_transforms(self.h, N.ptr(s), s.size, N.ptr(w), N.ptr(a)))
+
+
+class Batch:
+    """Batched solves of several ProbeFields (sbgi_batch_*): one frame updates every cascade
+    of every field with one persistent closest-hit walk and one persistent shadow walk over
+    all (field, cascade) segments, replayed from a hipGraph while the arguments repeat."""
+
+    def __init__(self, fields):
+        self.L = N.lib()
+        self.fields = list(fields)
+        arr = (C.c_void_p * len(self.fields))(*[f.h.value for f in self.fields])
+        h = C.c_void_p()
+        code = self.L.sbgi_batch_create(arr, len(self.fields), C.byref(h))
+        if code < 0:
+            raise N.SbgiError(code, "sbgi_batch_create failed")
+        self.h = h
+
+    def _chk(self, code):
+        if code < 0:
+            raise N.SbgiError(code, (self.L.sbgi_batch_last_error(self.h) or b"").decode(errors="replace"))
+        return code
+
+    def _params(self, params):
+        arr = (N.TickParams * len(self.fields))()
+        for i, (f, kw) in enumerate(zip(self.fields, params)):
+            arr[i] = N.default_tick_params(**kw)
+            arr[i].needs_clear = 1 if f.needs_clear[0] else 0
+        return arr
+
+    def tick(self, params, wait: bool = True) -> int:
+        """params: one keyword dict of tick parameters per field."""
+        arr = self._params(params)
+        t = C.c_uint64()
+        self._chk(self.L.sbgi_batch_tick(self.h, arr, C.byref(t)))
+        for f in self.fields:
+            f.needs_clear = [False] * len(f.needs_clear)
+        if wait:
+            self.wait()
+        return t.value
+
+    def wait(self):
+        self._chk(self.L.sbgi_batch_wait(self.h))
+
+    def timing(self):
+        """(device ms of the last frame, whether it replayed the captured graph)."""
+        ms, rp = C.c_float(), C.c_int32()
+        self._chk(self.L.sbgi_batch_timing(self.h, C.byref(ms), C.byref(rp)))
+        return ms.value, bool(rp.value)
+
+    def set_walk_block(self, block: int):
+        self._chk(self.L.sbgi_batch_set_walk_block(self.h, block))
+
+    def autotune(self, params):
+        arr = self._params(params)
+        for i in range(len(self.fields)):
+            arr[i].needs_clear = 0
+        t, blk, ms = N.WalkTuning(), C.c_uint32(), C.c_float()
+        self._chk(self.L.sbgi_batch_autotune(self.h, arr, C.byref(t), C.byref(blk), C.byref(ms)))
+        return {k: getattr(t, k) for k, _ in N.WalkTuning._fields_}, blk.value, ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbgi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

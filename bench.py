@@ -228,6 +228,91 @@ def dynamic_leg(frames=600, warmup=10):
                                            "probes_checked": 24, "cascade": big}}
 
 
+def batch_leg(frames=10, autotune=True):
+    """Batched uneven fields: 48 seeded room-and-boxes scenes (triangles log-uniform
+    [2k, 2M], probes per cascade log-uniform [512, 32768], rays from {32, 64, 128, 256},
+    1-4 cascades, bf16x3 blend, depth 16x16), every cascade of every field updated per frame
+    by one batch frame (one persistent closest-hit walk + one persistent shadow walk over
+    all segments, replayed from a hipGraph). Beside it, the same fields updated one cascade
+    at a time with sbgi_tick (device time summed)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    from speedball_amd import scenegen
+    from speedball_amd.field import Batch, ProbeField
+    from tests.parity import rel_err, interior_irr
+    t0 = time.perf_counter()
+    specs = scenegen.batch_specs()
+    with ThreadPoolExecutor(8) as ex:
+        built = list(ex.map(lambda s: scenegen.room_boxes_scaled(depth_oct=14, **s), specs))
+
+    def make(i):
+        scene, casc, _ = built[i]
+        f = ProbeField(casc[0][2], casc[0][0], casc[0][1], rays=specs[i]["rays"], depth_oct=14,
+                       precision=PRECISIONS["bf16x3"], cascades=casc, traversal=1)
+        f.set_scene(scene)
+        return f
+
+    with ThreadPoolExecutor(8) as ex:
+        fields = list(ex.map(make, range(len(specs))))
+    tris = sum(b[0].tri_count for b in built)
+    rays = sum(f.probes(c) * f.rays for f in fields for c in range(len(f.cascades)))
+    segs = sum(len(f.cascades) for f in fields)
+    log(f"batch: {len(fields)} fields, {segs} segments, {tris} triangles, {rays} probe rays/frame, "
+        f"setup {time.perf_counter() - t0:.1f} s")
+    params = [dict(frame_jitter=b[2], hysteresis=0.97) for b in built]
+    batch = Batch(fields)
+    batch.tick(params)  # cleared first frame (eager)
+    tuned = None
+    if autotune:
+        tuning, block, tms = batch.autotune(params)
+        tuned = {"walk_tuning": tuning, "walk_block": block, "frame_ms": tms}
+        log(f"batch: autotuned {tuning} block {block} ({tms:.2f} ms eager)")
+    batch.tick(params)  # capture
+    ms, wall, rep = [], [], []
+    for _ in range(frames):
+        ts = time.perf_counter()
+        batch.tick(params)
+        wall.append((time.perf_counter() - ts) * 1e3)
+        m, r = batch.timing()
+        ms.append(m)
+        rep.append(r)
+    batch.close()
+    # the same fields, one cascade at a time
+    single = []
+    for _ in range(3):
+        tot = 0.0
+        for f, kw in zip(fields, params):
+            for c in range(len(f.cascades)):
+                f.tick(cascade=c, **kw)
+                tot += f.stats().ms_total
+        single.append(tot)
+    for f in fields:
+        f.close()
+    # error statement: the first update of the smallest field's largest cascade vs FP64
+    i = min(range(len(specs)), key=lambda k: built[k][0].tri_count)
+    scene, casc, jit = built[i]
+    gm, gs, res = casc[-1]
+    f = ProbeField(res, gm, gs, rays=specs[i]["rays"], depth_oct=14, precision=PRECISIONS["bf16x3"], traversal=1)
+    f.set_scene(scene)
+    f.tick(frame_jitter=jit, hysteresis=0.97)
+    o = oracle.OracleField(scene, gm, gs, res, specs[i]["rays"], 14, prec="64")
+    p = o.params(frame_jitter=jit, hysteresis=0.97)
+    p.updated_count = 24
+    o.update(p)
+    e = rel_err(interior_irr(f.read_irr())[:24, :, :3], interior_irr(o.irr_view())[:24, :, :3])
+    f.close()
+    med = float(np.median(ms))
+    return {"workload": "batched: 48 room-and-boxes scenes, triangles log-uniform [2k, 2M], probes per cascade "
+                        "log-uniform [512, 32768], rays {32,64,128,256}, 1-4 cascades, bf16x3 blend, depth 16x16",
+            "fields": len(fields), "segments": segs, "triangles": int(tris), "probe_rays_per_frame": int(rays),
+            "frames": frames, "ms_per_frame": med, "ms_per_frame_p99": float(np.percentile(ms, 99)),
+            "wall_ms_per_frame": float(np.median(wall)), "graph_replayed": bool(all(rep)),
+            "probe_rays_per_s": rays / (med / 1e3), "unbatched_ms_per_frame": float(np.median(single)),
+            "speedup_vs_unbatched": float(np.median(single)) / med, "autotune": tuned,
+            "irradiance_rel_err_vs_fp64": {"p99": float(np.percentile(e, 99)), "max": float(e.max()),
+                                           "probes_checked": 24, "field": i}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +327,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint leg")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic-scene (per-frame refit) leg")
     ap.add_argument("--dynamic-frames", type=int, default=600, help="frames of the dynamic-scene leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-fields leg")
+    ap.add_argument("--batch-frames", type=int, default=10, help="timed frames of the batched-fields leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
@@ -333,6 +420,7 @@ def main():
     f.close()
     adj = None if args.no_adjoint else adjoint_leg()
     dyn = None if args.no_dynamic else dynamic_leg(frames=args.dynamic_frames)
+    bat = None if args.no_batch else batch_leg(frames=args.batch_frames)
     cpu = None if args.no_cpu else cpu_baseline(w)
 
     out = {
@@ -365,6 +453,7 @@ def main():
         "fwd_plus_adjoint_ms": adj["fwd_plus_adjoint_ms"] if adj else None,
         "adjoint": adj,
         "dynamic": dyn,
+        "batch": bat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

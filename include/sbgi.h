@@ -399,6 +399,8 @@ typedef struct sbgi_scene_builder sbgi_scene_builder;
 int32_t sbgi_sb_create(sbgi_scene_builder **out);
 /* Append one uber material record (28 floats); returns its index in *id. */
 int32_t sbgi_sb_add_material(sbgi_scene_builder *b, const float *rec28, uint32_t *id);
+/* n uber records at once (28 floats each); *first_id is the id of the first. */
+int32_t sbgi_sb_add_materials(sbgi_scene_builder *b, const float *recs28, uint32_t n, uint32_t *first_id);
 /* Add one local-space mesh and build its BLAS. normals/uvs may be NULL.
  * tri_material gives the uber index of every triangle (NULL = all 0). */
 int32_t sbgi_sb_add_mesh(sbgi_scene_builder *b, const float *positions, uint32_t vcount,

@@ -39,7 +39,7 @@ EXPORTS = [
     "sbgi_diff_adjoint", "sbgi_diff_timing", "sbgi_get_walk_tuning", "sbgi_set_walk_tuning", "sbgi_autotune",
     "sbgi_set_skins", "sbgi_skin", "sbgi_edit_timing", "sbgi_sb_mesh_ranges",
     "sbgi_batch_create", "sbgi_batch_tick", "sbgi_batch_wait", "sbgi_batch_timing", "sbgi_batch_set_walk_block",
-    "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy",
+    "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy", "sbgi_sb_add_materials",
 ]
 
 
@@ -171,6 +171,7 @@ def lib() -> C.CDLL:
         "sbgi_destroy": (i32, [vp]),
         "sbgi_sb_create": (i32, [C.POINTER(vp)]),
         "sbgi_sb_add_material": (i32, [vp, f32p, C.POINTER(u32)]),
+        "sbgi_sb_add_materials": (i32, [vp, vp, u32, C.POINTER(u32)]),
         "sbgi_sb_add_mesh": (i32, [vp, f32p, u32, f32p, f32p, vp, u32, vp, C.POINTER(u32)]),
         "sbgi_sb_add_instance": (i32, [vp, u32, vp, i32, C.POINTER(u32)]),
         "sbgi_sb_add_light": (i32, [vp, f32p]),

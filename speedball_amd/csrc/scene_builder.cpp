@@ -410,6 +410,14 @@ int32_t sbgi_sb_add_material(sbgi_scene_builder *b, const float *rec28, uint32_t
     return SBGI_OK;
 }
 
+int32_t sbgi_sb_add_materials(sbgi_scene_builder *b, const float *recs28, uint32_t n, uint32_t *first_id) {
+    if (!b || (n && !recs28)) return SBGI_E_INVALID;
+    if (first_id) *first_id = (uint32_t)b->materials.size();
+    for (uint32_t i = 0; i < n; i++) b->materials.emplace_back(recs28 + (size_t)i * 28, recs28 + (size_t)i * 28 + 28);
+    b->final = false;
+    return SBGI_OK;
+}
+
 int32_t sbgi_sb_add_mesh(sbgi_scene_builder *b, const float *positions, uint32_t vcount, const float *normals,
                          const float *uvs, const uint32_t *indices, uint32_t tcount, const uint32_t *tri_material,
                          uint32_t *mesh_id) {

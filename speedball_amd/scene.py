@@ -142,6 +142,13 @@ class SceneBuilder:
         self._chk(self.L.sbgi_sb_add_material(self.h, N.ptr(rec), C.byref(i)))
         return i.value
 
+    def add_materials(self, recs) -> int:
+        """n uber records [n, 28] at once; returns the id of the first."""
+        recs = np.ascontiguousarray(recs, np.float32).reshape(-1, MAT_STRIDE)
+        i = C.c_uint32()
+        self._chk(self.L.sbgi_sb_add_materials(self.h, N.ptr(recs), recs.shape[0], C.byref(i)))
+        return i.value
+
     def add_mesh(self, positions, indices, normals=None, uvs=None, tri_material=None) -> int:
         pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
         idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)

@@ -477,3 +477,79 @@ def dynamic(res=(24, 24, 24), n_cascades: int = 4, rays: int = 128, n_tubes: int
                            np.array(lens), np.array(axes), phase, slot_of[np.array(rigid_ids)].astype(np.uint32),
                            np.array(worlds)[scene.tlas_order], np.array(rpos), np.array(rrot), np.array(rscale), box,
                            np.random.default_rng(SEED_BASE + seed_id + 1))
+
+
+# ---------------------------------------------------------------- batched workload
+def boxes_vectorized(ctr, half, yaw):
+    """n yawed boxes at once: vertices (8n, 3) and outward triangles (12n, 3)."""
+    n = len(ctr)
+    sx = np.array([-1, 1, -1, 1, -1, 1, -1, 1], np.float64)
+    sy = np.array([-1, -1, 1, 1, -1, -1, 1, 1], np.float64)
+    sz = np.array([-1, -1, -1, -1, 1, 1, 1, 1], np.float64)
+    x = half[:, 0:1] * sx[None]
+    y = half[:, 1:2] * sy[None]
+    z = half[:, 2:3] * sz[None]
+    c, s = np.cos(yaw)[:, None], np.sin(yaw)[:, None]
+    v = np.stack([ctr[:, 0:1] + c * x + s * z, ctr[:, 1:2] + y, ctr[:, 2:3] - s * x + c * z], -1)
+    quads = np.array([(0, 4, 6, 2), (1, 3, 7, 5), (0, 1, 5, 4), (2, 6, 7, 3), (0, 2, 3, 1), (4, 5, 7, 6)])
+    tri = np.concatenate([quads[:, [0, 1, 2]], quads[:, [0, 2, 3]]], 0)  # 12 x 3 per box
+    t = tri[None, :, :] + (np.arange(n) * 8)[:, None, None]
+    return v.reshape(-1, 3).astype(np.float32), t.reshape(-1, 3).astype(np.uint32)
+
+
+def room_boxes_scaled(n_tris: int, res, rays: int, n_cascades: int = 1, seed_id: int = 100, depth_oct: int = 6,
+                      hysteresis: float = 0.97):
+    """The room-and-boxes recipe grown to about n_tris triangles: (n_tris - 12) / 12 boxes
+    (sizes U[0.3, 2] m, random yaw, albedo U[0.2, 0.8]^3 each) in a cubic room whose side
+    keeps 64 boxes per 1000 m^3, 1 point light (intensity 10) + 1 directional light;
+    n_cascades grids of res probes centred in the room, each twice the extent of the
+    previous, the largest inset 2.5 % from the walls. Returns (scene, cascades, jitter)."""
+    rng = np.random.default_rng(SEED_BASE + seed_id)
+    n_boxes = max(1, int(round((n_tris - 12) / 12)))
+    L = 10.0 * (n_boxes / 64.0) ** (1.0 / 3.0)
+    half = rng.uniform(0.3, 2.0, (n_boxes, 3)) / 2
+    yaw = rng.uniform(0.0, 2 * math.pi, n_boxes)
+    ex = np.abs(np.cos(yaw)) * half[:, 0] + np.abs(np.sin(yaw)) * half[:, 2]
+    ez = np.abs(np.sin(yaw)) * half[:, 0] + np.abs(np.cos(yaw)) * half[:, 2]
+    ctr = np.stack([rng.uniform(ex, L - ex), rng.uniform(half[:, 1], L - half[:, 1]), rng.uniform(ez, L - ez)], 1)
+    alb = rng.uniform(0.2, 0.8, (n_boxes, 3))
+    b = SceneBuilder()
+    wall = b.add_material(uber_material((0.5, 0.5, 0.5), side=2))
+    recs = np.tile(uber_material(), (n_boxes, 1))
+    recs[:, 0:3] = alb
+    first = b.add_materials(recs)
+    rv, rt = room_mesh(L)
+    bv, bt = boxes_vectorized(ctr, half, yaw)
+    v = np.concatenate([rv, bv])
+    t = np.concatenate([rt, bt + len(rv)])
+    tm = np.concatenate([np.full(len(rt), wall, np.uint32), (first + np.repeat(np.arange(n_boxes), 12)).astype(np.uint32)])
+    b.add_instance(b.add_mesh(v, t, tri_material=tm))
+    lp = (rng.uniform(0.1, 0.9) * L, rng.uniform(0.6, 0.9) * L, rng.uniform(0.1, 0.9) * L)
+    b.add_light(point_light(lp, intensity=10.0, distance=0.0, decay=2.0))
+    d = np.array([rng.uniform(-1, 1), -rng.uniform(0.2, 1.0), rng.uniform(-1, 1)])
+    top = np.array((L / 2, 2 * L, L / 2))
+    b.add_light(directional_light(tuple(top), top + d, intensity=1.0))
+    scene = b.finalize(workload="room_boxes_scaled", boxes=n_boxes, room=L)
+    centre = np.full(3, L / 2)
+    big = np.full(3, 0.95 * L)
+    cascades = []
+    for k in range(n_cascades):
+        size = big / (2 ** (n_cascades - 1 - k))
+        cascades.append((tuple((centre - size / 2).tolist()), tuple(size.tolist()), tuple(res)))
+    jitter = float(np.float32(((SEED_BASE + seed_id) * 0.6180339887) % 1.0))
+    return scene, cascades, jitter
+
+
+def batch_specs(n_fields: int = 48, seed_id: int = 3):
+    """Per field of the batched workload: (triangles log-uniform [2k, 2M], probes per cascade
+    log-uniform [512, 32768] as a cubic grid, rays from {32, 64, 128, 256}, cascades 1-4)."""
+    rng = SplitMix64(SEED_BASE + seed_id)
+    out = []
+    for i in range(n_fields):
+        tris = int(round(rng.log_uniform(2000, 2_000_000)))
+        probes = rng.log_uniform(512, 32768)
+        r = max(8, min(32, int(round(probes ** (1.0 / 3.0)))))
+        rays = (32, 64, 128, 256)[int(rng.uniform(0, 4)) % 4]
+        casc = 1 + int(rng.uniform(0, 4)) % 4
+        out.append(dict(n_tris=tris, res=(r, r, r), rays=rays, n_cascades=casc, seed_id=1000 + i))
+    return out

@@ -581,3 +581,36 @@ def test_diff_adjoint_matches_central_differences():
         lm[j] -= eps
         fd = (loss(alb, lp) - loss(alb, lm)) / (2 * eps)
         assert abs(fd - gl[j]) <= 1e-5 * max(abs(fd), 1e-6), (j, fd, gl[j])
+
+
+def test_diff_forward_first_update_equals_regular_update():
+    """With every albedo entry set to its triangle's material base colour and unit light
+    scales, the differentiable forward's first update E1 is the regular first update
+    (cleared state, hysteresis 0) of the same field: this pins the albedo map (slot-major,
+    then pool-triangle order) and the light-scale hook."""
+    w, _, _ = sg.differentiable(res=(3, 3, 3), rays=32, n_mesh=2, n_instances=3, torus_n=4, n_lights=2, box=6.0)
+    s = w.scene
+    amap, n = oracle.albedo_map(s)
+    alb = np.zeros((n, 3))
+    inst = s.materials[s.inst_base:s.inst_base + s.inst_count * 28].reshape(-1, 28)
+    nodes = s.bvh_nodes.reshape(-1, 8)
+    for slot in range(s.inst_count):
+        root, end = int(inst[slot, 12]), int(inst[slot, 13])
+        pay = nodes[root:end, 7]
+        pay = pay[pay != 0xFFFFFFFF]
+        for t in range(int((pay & 0xFFFFFF).min()), int((pay & 0xFFFFFF).min() + (pay >> 24).sum())):
+            m = int(s.tri_material[t])
+            alb[amap[slot, 0] + t - amap[slot, 1]] = s.materials[m * 28:m * 28 + 3]
+    o = oracle.OracleField(s, w.grid_min, w.grid_size, w.res, w.rays, 6, prec="64")
+    p = o.params(frame_jitter=w.frame_jitter, hysteresis=0.0)
+    E2, E1, A1 = oracle.diff_forward(s, w.grid_min, w.grid_size, w.res, w.rays, p, alb, amap, np.ones(2))
+    o.update(p)
+    idx = [(ly + 1) * 8 + (lx + 1) for ly in range(6) for lx in range(6)]
+    ref = o.irr_view()[:, idx, :3]
+    np.testing.assert_allclose(E1.reshape(ref.shape), ref, rtol=1e-12, atol=1e-12)
+    # and the light scales act linearly on the direct part: scaling every light by 2 with
+    # no bounce and no sky doubles E1
+    p2 = o.params(frame_jitter=w.frame_jitter, hysteresis=0.0, dbg_bounce=0.0, sky_intensity=0.0)
+    E1a = oracle.diff_forward(s, w.grid_min, w.grid_size, w.res, w.rays, p2, alb, amap, np.ones(2))[1]
+    E1b = oracle.diff_forward(s, w.grid_min, w.grid_size, w.res, w.rays, p2, alb, amap, 2 * np.ones(2))[1]
+    np.testing.assert_allclose(E1b, 2 * E1a, rtol=1e-12, atol=1e-14)

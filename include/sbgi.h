@@ -202,6 +202,10 @@ typedef struct sbgi_stats {
     uint64_t shadow_tlas_nodes;
     uint64_t shadow_tri_tests;
     uint64_t shadow_inst_visits;
+    /* wave steps of the walks' NODE, TRI and INST phases and of lane refills (counting
+       enabled): lane visits / (64 x wave steps) is the SIMD utilisation of a phase     */
+    uint64_t walk_waves[4];
+    uint64_t shadow_waves[4];
 } sbgi_stats;
 
 /* Wave schedule of the persistent walks (no reference counterpart: it is how this build

@@ -1005,6 +1005,10 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     out->shadow_tlas_nodes = c.shadow_walk[1];
     out->shadow_tri_tests = c.shadow_walk[2];
     out->shadow_inst_visits = c.shadow_walk[3];
+    for (int i = 0; i < 4; i++) {
+        out->walk_waves[i] = c.walk[4 + i];
+        out->shadow_waves[i] = c.shadow_walk[4 + i];
+    }
     float sub[4] = {0.f, 0.f, 0.f, 0.f};
     if (f->last_updated) {
         for (int i = 0; i < 3; i++) HIP_OK(f, hipEventElapsedTime(&sub[i], f->tev[i], f->tev[i + 1]));

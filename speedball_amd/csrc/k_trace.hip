@@ -66,7 +66,7 @@ struct ShadowSrc {
     unsigned long long *vis;
     __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
         const float4 a = rec[(size_t)i * 2], b = rec[(size_t)i * 2 + 1];
-        lane_begin(s, r, xyz(a), xyz(b), a.w);
+        lane_begin<true>(s, r, xyz(a), xyz(b), a.w);
         r.tag = __float_as_uint(b.w);
     }
     __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
@@ -112,6 +112,12 @@ __global__ void __launch_bounds__(kWalkBlock) k_walk(DevScene s, Src src, const 
         atomicAdd(&counters[1], (unsigned long long)cnt.tlas);
         atomicAdd(&counters[2], (unsigned long long)cnt.tris);
         atomicAdd(&counters[3], (unsigned long long)cnt.insts);
+        if ((threadIdx.x & 63u) == 0) {
+            atomicAdd(&counters[4], (unsigned long long)cnt.node_w);
+            atomicAdd(&counters[5], (unsigned long long)cnt.tri_w);
+            atomicAdd(&counters[6], (unsigned long long)cnt.inst_w);
+            atomicAdd(&counters[7], (unsigned long long)cnt.refill_w);
+        }
     }
 }
 

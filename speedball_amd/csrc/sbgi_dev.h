@@ -78,8 +78,10 @@ constexpr uint32_t kDefaultTriRatio = 4, kDefaultInstRatio = 4, kDefaultRefillMi
 // Traversal counters of counting ticks: [0..3] closest-hit walk {BLAS nodes, TLAS nodes,
 // triangle tests, instance entries}, [4] shadow rays emitted, [5..8] the shadow walk's
 // {BLAS nodes, TLAS nodes, triangle tests, instance entries}.
+// walk[0..3]: BLAS nodes, TLAS nodes, triangle tests, instance entries (lane visits);
+// walk[4..7]: wave steps of the NODE, TRI, INST phases and refills; same for shadow_walk
 struct Counters {
-    unsigned long long walk[4], shadow_rays, shadow_walk[4];
+    unsigned long long walk[8], shadow_rays, shadow_walk[8];
 };
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
@@ -114,13 +116,16 @@ __device__ __forceinline__ bool side_ok(uint32_t flags, float det) {
 // Moller-Trumbore with the reference's tests and epsilon (js/spectral_traverse.js:388-409).
 // Evaluated without FMA contraction and with a correctly rounded 1/det, the operation
 // order of the reference's graph, so edge decisions match the FP32 restatement.
+// RCP = true takes 1/det from the hardware reciprocal (1 ulp) instead: the any-hit walk,
+// whose answer can then differ only for a ray grazing a triangle edge.
+template <bool RCP = false>
 __device__ __forceinline__ bool mt_test(float4 A, float4 B, float4 C, float3 ro, float3 rd, float &t, float &det) {
 #pragma clang fp contract(off)
     float3 p0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
     float3 pv = cross3(rd, e2);
     det = dot3(e1, pv);
     if (!(fabsf(det) > 1e-12f)) return false;
-    float inv_det = 1.0f / det;
+    float inv_det = RCP ? __builtin_amdgcn_rcpf(det) : 1.0f / det;
     float3 tv = ro - p0;
     float u = dot3(tv, pv) * inv_det;
     if (!(u >= 0.f && u <= 1.f)) return false;

@@ -12,6 +12,8 @@ extern __shared__ uint4 g_lds[];
 template <bool COUNT>
 struct TravCounters {
     uint32_t blas = 0, tlas = 0, tris = 0, insts = 0;
+    // wave steps of the NODE, TRI and INST phases and of refills (counted by lane 0)
+    uint32_t node_w = 0, tri_w = 0, inst_w = 0, refill_w = 0;
     __device__ __forceinline__ void add_blas() {
         if (COUNT) blas++;
     }

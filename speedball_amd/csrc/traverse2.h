@@ -45,13 +45,23 @@ struct LaneRay {
     bool blocked;
 };
 
-__device__ __forceinline__ float3 ieee_inv(float3 d) { return f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z); }
+// Inverse direction of the slab test: correctly rounded (the closest-hit walk, so box
+// decisions are the FP32 restatement's), or the hardware reciprocal (RCP, 1 ulp: the
+// any-hit walk of shadow rays, where a box decision can then differ only for a ray that
+// grazes a box face, and a visibility answer only if it also grazes an occluder's edge;
+// the reference's own WGSL division is specified to 2.5 ulp).
+template <bool RCP = false>
+__device__ __forceinline__ float3 ray_inv(float3 d) {
+    if (RCP) return f3(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+    return f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+}
 
 // Start a walk at the TLAS root.
+template <bool RCP = false>
 __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3 ro, float3 rd, float tmax) {
     r.wo = ro;
     r.wd = rd;
-    r.wid = ieee_inv(rd);
+    r.wid = ray_inv<RCP>(rd);
     r.o = ro;
     r.d = rd;
     r.id = r.wid;
@@ -129,7 +139,7 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
     const float4 A = tp[0], B = tp[1], C = tp[2];
     cnt.add_tri();
     float th = 0.f, det = 0.f;
-    const bool geo = mt_test(A, B, C, r.o, r.d, th, det) && th > SBGI_RAY_EPS && th < r.tmax;
+    const bool geo = mt_test<MODE == WALK_ANY>(A, B, C, r.o, r.d, th, det) && th > SBGI_RAY_EPS && th < r.tmax;
     const uint32_t flags = __float_as_uint(A.w);
     const bool accept = geo && side_ok(flags, det * r.dsign) && (flags & (ANY ? 8u : 4u));
     const bool block = ANY && accept, better = !ANY && accept;
@@ -143,7 +153,7 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
 }
 
 // INST: enter the next instance of the pending TLAS leaf, or go back to the TLAS.
-template <bool COUNT, bool LDS>
+template <bool COUNT, bool LDS, bool RCP>
 __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     const bool enter = r.inext < r.iend;
     // default: back to the TLAS in world space
@@ -158,7 +168,7 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
         if (r3.w > 0.5f) {
             o = inst_point(r0, r1, r2, r.wo);
             d = inst_dir(r0, r1, r2, r.wd);
-            id = ieee_inv(d);
+            id = ray_inv<RCP>(d);
             dsign = r3.z;
             cur = (uint32_t)r3.x;
             end = (uint32_t)r3.y;
@@ -223,6 +233,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         const uint64_t done = __ballot(r.state == ST_DONE);
         uint64_t live = __ballot(r.state < ST_DONE);
         if (done && (__popcll(done) >= refill_min || !live)) {
+            if (COUNT && lane == 0) cnt.refill_w++;
             if (r.state == ST_DONE) {
                 if (have) src.store(idx, r);
                 have = false;
@@ -260,17 +271,20 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         if (!live) break;
         if (r.state < ST_DONE && ++r.steps > budget) r.state = ST_DONE;
         if (__ballot(r.state == ST_NODE)) {
+            if (COUNT && lane == 0) cnt.node_w++;
             if (r.state == ST_NODE) node_step<COUNT, LDS>(s, r, cnt);
         }
         uint64_t mn = __ballot(r.state == ST_NODE);
         const uint64_t mt = __ballot(r.state == ST_TRI);
         if (mt && __popcll(mt) * tri_ratio >= __popcll(mn)) {
+            if (COUNT && lane == 0) cnt.tri_w++;
             if (r.state == ST_TRI) tri_step<MODE, COUNT>(s, r, cnt);
         }
         mn = __ballot(r.state == ST_NODE);
         const uint64_t mi = __ballot(r.state == ST_INST);
         if (mi && __popcll(mi) * inst_ratio >= __popcll(mn)) {
-            if (r.state == ST_INST) inst_step<COUNT, LDS>(s, r, cnt);
+            if (COUNT && lane == 0) cnt.inst_w++;
+            if (r.state == ST_INST) inst_step<COUNT, LDS, MODE == WALK_ANY>(s, r, cnt);
         }
     }
 }

@@ -106,8 +106,13 @@ def test_room_radiance_fp32(room_run):
         g, r = gpu[step], refs["32"][step]
         same = np.all(g["hits"] == r["hits"], axis=-1)
         e = rel_err(g["rays"][..., :3][same], r["rays"][..., :3][same])
-        print(f"\nstep {step} ray radiance vs FP32 restatement", summary(e))
+        # the shadow walk takes 1/d from the hardware reciprocal: a light's visibility can
+        # flip only for a shadow ray grazing a box face and an occluder edge; such rays
+        # show as radiance outliers and are counted here (bar: at most 1 in 10,000 rays)
+        flipped = int((e > 1e-3).sum())
+        print(f"\nstep {step} ray radiance vs FP32 restatement", summary(e), "rays off by > 1e-3:", flipped)
         assert np.percentile(e, 99.9) <= 1e-4
+        assert flipped <= e.size // 10000
 
 
 def test_room_irradiance_fp32(room_run):

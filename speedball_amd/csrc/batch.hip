@@ -201,6 +201,20 @@ int32_t run_frame(sbgi_batch *b, const sbgi_tick_params *params, bool allow_grap
     return SBGI_OK;
 }
 
+// The segment scratch was sized at sbgi_batch_create: a field whose scene now has more
+// lights (shadow-queue capacity) or whose solve size changed needs a new batch.
+int32_t check_sizes(sbgi_batch *b) {
+    for (size_t j = 0; j < b->segs.size(); j++) {
+        const auto &g = b->segs[j];
+        const sbgi_field *f = b->fields[g.field];
+        if (!f->has_scene) return bfail(b, SBGI_E_STATE, "a field of the batch has no scene");
+        if (std::max(1u, f->view.light_count) > g.lights ||
+            (uint64_t)f->casc[g.cascade].total * f->desc.rays_per_probe != g.rays_total)
+            return bfail(b, SBGI_E_STATE, "a field's scene or size changed since sbgi_batch_create: create the batch again");
+    }
+    return SBGI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -289,8 +303,9 @@ int32_t sbgi_batch_create(sbgi_field *const *fields, uint32_t n, sbgi_batch **ou
 int32_t sbgi_batch_tick(sbgi_batch *b, const sbgi_tick_params *params, uint64_t *ticket) {
     if (!b || !params) return SBGI_E_INVALID;
     BHIP_OK(b, hipSetDevice(b->device));
+    int32_t rc0 = check_sizes(b);
+    if (rc0 != SBGI_OK) return rc0;
     for (sbgi_field *f : b->fields) {
-        if (!f->has_scene) return bfail(b, SBGI_E_STATE, "a field of the batch has no scene");
         // scene edits enqueued on the field's stream come first
         BHIP_OK(b, hipEventRecord(b->join, f->stream));
         BHIP_OK(b, hipStreamWaitEvent(b->stream, b->join, 0));
@@ -335,6 +350,8 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
                             uint32_t *best_block, float *best_ms) {
     if (!b || !params) return SBGI_E_INVALID;
     BHIP_OK(b, hipSetDevice(b->device));
+    int32_t rc0 = check_sizes(b);
+    if (rc0 != SBGI_OK) return rc0;
     for (sbgi_field *f : b->fields) {
         BHIP_OK(b, hipEventRecord(b->join, f->stream));
         BHIP_OK(b, hipStreamWaitEvent(b->stream, b->join, 0));

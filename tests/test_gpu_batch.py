@@ -142,3 +142,22 @@ def test_batch_rejects_bad_groups(specs):
     finally:
         for f in fs:
             f.close()
+
+
+def test_batch_refuses_a_field_whose_scene_outgrew_it(specs):
+    """The batch sizes each segment's shadow queue by the field's light count at creation:
+    after set_scene with more lights, a frame is refused instead of dropping records."""
+    fs = make_fields(specs[:1])
+    w = specs[0][0]
+    params = [dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)]
+    batch = Batch(fs)
+    try:
+        batch.tick(params)
+        bigger = sg.instanced(res=(4, 4, 4), rays=32, total_tris=6000, n_instances=8, n_lights=4).scene
+        fs[0].set_scene(bigger)
+        with pytest.raises(RuntimeError):
+            batch.tick(params)
+    finally:
+        batch.close()
+        for f in fs:
+            f.close()

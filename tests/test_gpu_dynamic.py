@@ -269,3 +269,42 @@ def test_set_skins_rejects_bad_sets(small_dynamic):
             f.skin(w.bones(0.0)[:-1])                               # palette size differs
     finally:
         f.close()
+
+
+@pytest.mark.parametrize("n,kind", [(1, "random"), (2, "random"), (3, "random"), (37, "coincident"),
+                                    (64, "inactive"), (3000, "random")])
+def test_device_sah_tlas_edge_cases(n, kind):
+    """The one-workgroup SAH build against the oracle's restatement, word for word, for a
+    single instance, tiny sets, every instance at one point (no axis has centroid extent:
+    middle splits), a quarter of the slots inactive (point boxes at their translation) and
+    a few thousand instances (many levels of many nodes)."""
+    rng = np.random.default_rng(100 + n)
+    b = sg.SceneBuilder()
+    v, t = sg.box_mesh((0.0, 0.0, 0.0), (0.5, 0.3, 0.2), 0.0)
+    mesh = b.add_mesh(v, t)
+    worlds, active = [], []
+    for i in range(n):
+        if kind == "coincident":
+            m = np.eye(4)
+            m[:3, 3] = (1.0, 2.0, 3.0)
+        else:
+            m = sg.trs(rng.uniform(-50, 50, 3), sg.random_rotation(sg.SplitMix64(i + 7)), rng.uniform(0.2, 3.0))
+        worlds.append(m)
+        active.append(not (kind == "inactive" and i % 4 == 0))
+        b.add_instance(mesh, m, active=active[-1])
+    b.add_light(sg.point_light((0.0, 10.0, 0.0), intensity=10.0))
+    scene = b.finalize(workload="tlas_edge")
+    f = ProbeField((2, 2, 2), (-1.0, -1.0, -1.0), (2.0, 2.0, 2.0), rays=32, traversal=N.TRAV_MT_ORDERED)
+    try:
+        f.set_scene(scene)
+        order = scene.tlas_order  # slot -> source instance
+        print(f"case: device TLAS {kind} n={n}", flush=True)
+        f.set_instance_transforms(np.arange(n), [worlds[order[s]] for s in range(n)],
+                                  active=[active[order[s]] for s in range(n)])
+        wd = np.stack([worlds[order[s]].T.reshape(-1) for s in range(n)])
+        b64, pad = oracle.inst_bounds(f.read(N.BUF_MATERIALS, np.float32), scene.inst_base, n,
+                                      f.read(N.BUF_BVH_NODES, np.uint32), wd)
+        assert np.array_equal(f.read(N.BUF_INST_BOUNDS, np.float32).reshape(-1, 6), b64)
+        assert np.array_equal(f.read(N.BUF_TLAS_SAH, np.uint32).reshape(-1, 8), oracle.build_sah_tlas(pad))
+    finally:
+        f.close()

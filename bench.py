@@ -183,6 +183,26 @@ def dynamic_leg(frames=600, warmup=10):
                 rec[k].append(v)
         if (i + 1) % 100 == 0:
             log(f"dynamic: frame {i + 1}/{total}, update {upd:.2f} ms, edits {sk + rf + tl:.3f} ms")
+    # device rebuild of every skinned BLAS on the final pose (the refit keeps the rest
+    # pose's tree): its time, and the update time before / after it
+    def update_ms():
+        tot = 0.0
+        for c in range(len(w.cascades)):
+            f.tick(cascade=c, **params)
+            tot += f.stats().ms_total
+        return tot
+
+    before = float(np.median([update_ms() for _ in range(3)]))
+    tube_ranges = [(int(br), int(be)) for (_, _, br, be, _, _) in w.skin_sets]
+    f.rebuild_blas(tube_ranges)  # warm-up (first-use allocations)
+    ts = time.perf_counter()
+    f.rebuild_blas(tube_ranges)
+    rebuild_wall = (time.perf_counter() - ts) * 1e3
+    _, rebuild_dev, rebuild_tlas = f.edit_timing()
+    after = float(np.median([update_ms() for _ in range(3)]))
+    rebuild = {"blases": len(tube_ranges), "triangles": int(w.scene.meta["skinned_triangles"]),
+               "device_ms": rebuild_dev + rebuild_tlas, "wall_ms": rebuild_wall,
+               "update_ms_refit_tree": before, "update_ms_rebuilt_tree": after}
     # error statement: first update (cleared state, empty bounce atlas) of the largest
     # cascade on the final frame's geometry, in a fresh field
     big = len(w.cascades) - 1
@@ -196,7 +216,8 @@ def dynamic_leg(frames=600, warmup=10):
     mats[s.tlas_base:s.tlas_base + s.tlas_node_count * 12] = tl
     import dataclasses
     kw = {fl.name: getattr(s, fl.name) for fl in dataclasses.fields(s) if fl.name != "_view"}
-    kw.update(vertex_data=vd, bvh_nodes=nodes, materials=mats)
+    kw.update(vertex_data=vd, bvh_nodes=nodes, materials=mats, tri_index=f.read(N.BUF_TRI_INDEX, np.uint32),
+              tri_material=f.read(N.BUF_TRI_MATERIAL, np.uint32))
     s2 = scenegen.Scene(**kw)
     f.close()
     gm, gs, res = w.cascades[big]
@@ -224,6 +245,7 @@ def dynamic_leg(frames=600, warmup=10):
             "ms_per_frame": q(frame), "refit_ms": q(edits), "update_ms": q(rec["update"]),
             "skin_ms": q(rec["skin"]), "blas_refit_ms": q(rec["refit"]), "tlas_ms": q(rec["tlas"]),
             "wall_ms_per_frame": q(rec["wall"]), "probe_rays_per_s": rays / (float(np.median(frame)) / 1e3),
+            "rebuild": rebuild,
             "irradiance_rel_err_vs_fp64": {"p99": float(np.percentile(e, 99)), "max": float(e.max()),
                                            "probes_checked": 24, "cascade": big}}
 

@@ -77,9 +77,9 @@ extern "C" {
 
 /* buffer ids for sbgi_read / sbgi_write / sbgi_update_ranges */
 #define SBGI_BUF_BVH_NODES 0    /* u32[8*N]                 (update_ranges)           */
-#define SBGI_BUF_TRI_INDEX 1    /* u32[3*T]                 (update_ranges)           */
+#define SBGI_BUF_TRI_INDEX 1    /* u32[3*T]                 (update_ranges, read)     */
 #define SBGI_BUF_VERTEX_DATA 2  /* f32[8*V]                 (update_ranges, read)     */
-#define SBGI_BUF_TRI_MATERIAL 3 /* u32[T]                   (update_ranges)           */
+#define SBGI_BUF_TRI_MATERIAL 3 /* u32[T]                   (update_ranges, read)     */
 #define SBGI_BUF_MATERIALS 4    /* f32 materials|inst|tlas  (update_ranges, read)     */
 #define SBGI_BUF_LIGHTS 5       /* f32[17*L]                (update_ranges, read)     */
 #define SBGI_BUF_IRR_STATE 10   /* f32[4*P*64]    per cascade (read, write)            */
@@ -283,8 +283,18 @@ int32_t sbgi_set_skins(sbgi_field *f, const sbgi_skin_set *sets, uint32_t n_sets
  * TLAS as sbgi_refit does. Enqueued on the field's stream. */
 int32_t sbgi_skin(sbgi_field *f, const float *bones, uint32_t n_bones, int32_t update_tlas);
 
-/* Device time (ms) of the last scene edits: skinning (or 0), BLAS refit with walk-copy
- * and triangle updates, and instance records + bounds + TLAS. Waits for them. */
+/* Rebuild the BLASes of the given [root, end) node ranges on the device from the current
+ * vertex positions (no reference counterpart: the reference rebuilds with three-mesh-bvh
+ * on the CPU, js/spectral_scene.js:898-966, and otherwise refits, js/gi_refit.js). For a
+ * deformation that a refit would leave with swollen boxes. Same node count N (odd) and
+ * pool range; L = (N + 1) / 2 leaves over Morton-ordered triangles; the ranges' triangles
+ * are reordered (tri_index, tri_material) within their span, so triangle ids of those
+ * BLASes change. Walk copies, packed triangles, instance bounds and the TLAS follow; the
+ * new structure is copied into the library's host mirrors before the call returns. */
+int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges);
+
+/* Device time (ms) of the last scene edits: skinning (or 0), BLAS refit or rebuild with
+ * walk-copy and triangle updates, and instance records + bounds + TLAS. Waits for them. */
 int32_t sbgi_edit_timing(sbgi_field *f, float *ms_skin, float *ms_blas_refit, float *ms_tlas);
 
 /* One solve of one cascade: trace -> blend -> upload, the ordered pass of

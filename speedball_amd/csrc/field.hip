@@ -906,6 +906,8 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
         case SBGI_BUF_BVH_NODES: return f->h_nodes.size() * 4;
         case SBGI_BUF_TRI_PACKED: return f->has_scene ? (size_t)f->view.tri_count * 48 : 0;
         case SBGI_BUF_VERTEX_DATA: return f->h_vdata.size() * 4;
+        case SBGI_BUF_TRI_INDEX: return f->h_tri_index.size() * 4;
+        case SBGI_BUF_TRI_MATERIAL: return f->h_tri_mat.size() * 4;
         case SBGI_BUF_INST_BOUNDS: return f->has_scene ? (size_t)f->view.inst_count * 24 : 0;
         case SBGI_BUF_TLAS_SAH: return f->has_scene ? (size_t)f->tlas_sah_count * 32 : 0;
     }
@@ -950,11 +952,13 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
         return SBGI_OK;
     }
     if (buf_id == SBGI_BUF_TRI_PACKED || buf_id == SBGI_BUF_VERTEX_DATA || buf_id == SBGI_BUF_INST_BOUNDS ||
-        buf_id == SBGI_BUF_TLAS_SAH) {
-        src = buf_id == SBGI_BUF_TRI_PACKED    ? (const void *)f->d_tri
-              : buf_id == SBGI_BUF_VERTEX_DATA ? (const void *)f->d_vdata
-              : buf_id == SBGI_BUF_INST_BOUNDS ? (const void *)f->d_inst_bounds
-                                               : (const void *)f->d_tlas_sah;
+        buf_id == SBGI_BUF_TLAS_SAH || buf_id == SBGI_BUF_TRI_INDEX || buf_id == SBGI_BUF_TRI_MATERIAL) {
+        src = buf_id == SBGI_BUF_TRI_PACKED     ? (const void *)f->d_tri
+              : buf_id == SBGI_BUF_VERTEX_DATA  ? (const void *)f->d_vdata
+              : buf_id == SBGI_BUF_INST_BOUNDS  ? (const void *)f->d_inst_bounds
+              : buf_id == SBGI_BUF_TRI_INDEX    ? (const void *)f->d_tri_index
+              : buf_id == SBGI_BUF_TRI_MATERIAL ? (const void *)f->d_tri_mat
+                                                : (const void *)f->d_tlas_sah;
         HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
@@ -1018,6 +1022,199 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     out->ms_emit = sub[1];
     out->ms_walk_shadow = sub[2];
     out->ms_resolve = sub[3];
+    return SBGI_OK;
+}
+
+int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n_ranges) {
+    if (!f || (n_ranges && !blas_ranges)) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    if (!n_ranges) return SBGI_OK;
+    const sbgi_scene_view &v = f->view;
+    // validate: threaded layout, odd node count, one contiguous triangle span per BLAS,
+    // disjoint ranges, at most 255 triangles per leaf
+    std::vector<uint4> desc(n_ranges);
+    std::vector<uint32_t> prefix(n_ranges + 1, 0u);
+    std::vector<std::pair<uint32_t, uint32_t>> nr, tr;
+    uint64_t total = 0, nodes_total = 0;
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        const uint32_t root = blas_ranges[2 * r], end = blas_ranges[2 * r + 1];
+        if (!(end > root) || end > v.node_count) return fail(f, SBGI_E_LAYOUT, "rebuild range out of bounds");
+        const uint32_t N = end - root;
+        if (!(N & 1u)) return fail(f, SBGI_E_LAYOUT, "rebuild range with an even node count");
+        uint32_t t0 = UINT32_MAX, t1 = 0;
+        uint64_t T = 0;
+        for (uint32_t i = root; i < end; i++) {
+            const uint32_t *nd = f->h_nodes.data() + (size_t)i * 8;
+            if (nd[7] == 0xFFFFFFFFu) {
+                const uint32_t l = i + 1;
+                if (l >= end) return fail(f, SBGI_E_LAYOUT, "interior node without a left child in range");
+                const uint32_t rr = f->h_nodes[(size_t)l * 8 + 6];
+                if (!(rr > l && rr < end)) return fail(f, SBGI_E_LAYOUT, "right child outside (left, end)");
+            } else {
+                const uint32_t off = nd[7] & 0x00FFFFFFu, cnt = nd[7] >> 24;
+                if (!cnt) continue;
+                t0 = std::min(t0, off);
+                t1 = std::max(t1, off + cnt);
+                T += cnt;
+            }
+        }
+        const uint32_t L = (N + 1) / 2;
+        if (!T || t1 - t0 != T) return fail(f, SBGI_E_LAYOUT, "BLAS leaves do not cover one contiguous triangle span");
+        if (T < L || (T + L - 1) / L > 255) return fail(f, SBGI_E_LAYOUT, "BLAS triangle count does not fit its node count");
+        desc[r] = make_uint4(root, N, t0, (uint32_t)T);
+        prefix[r + 1] = prefix[r] + (uint32_t)T;
+        nr.emplace_back(root, end);
+        tr.emplace_back(t0, t1);
+        total += T;
+        nodes_total += N;
+    }
+    std::sort(nr.begin(), nr.end());
+    std::sort(tr.begin(), tr.end());
+    for (size_t i = 1; i < nr.size(); i++)
+        if (nr[i].first < nr[i - 1].second || tr[i].first < tr[i - 1].second)
+            return fail(f, SBGI_E_INVALID, "rebuild ranges overlap");
+    if (total >= (1ull << 31)) return fail(f, SBGI_E_CAPACITY, "too many triangles to rebuild at once");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    hipStream_t st = f->stream;
+    const uint32_t n = (uint32_t)total, copies = ordered(f) ? 8u : 1u;
+    uint32_t range_bits = 0;
+    while ((1u << range_bits) < n_ranges) range_bits++;
+    const size_t sort_bytes = rebuild_sort_bytes(n);
+    // scratch (a rebuild is a structural edit: allocated per call)
+    void *scratch = nullptr;
+    const size_t sz_desc = (size_t)n_ranges * 16, sz_pref = ((size_t)n_ranges + 1) * 4, sz_cen = (size_t)n * 16,
+                 sz_cb = (size_t)n_ranges * 24, sz_keys = (size_t)n * 8, sz_vals = (size_t)n * 4,
+                 sz_old = (size_t)n * 16, sz_tasks = (size_t)n * 16, sz_order = nodes_total * 4,
+                 sz_q = ordered(f) ? (size_t)copies * v.node_count * 4 : 0;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t bytes = al(sz_desc) + al(sz_pref) + al(sz_cen) + al(sz_cb) + 2 * al(sz_keys) + 2 * al(sz_vals) +
+                         al(sz_old) + 2 * al(sz_tasks) + al(4) + al(sz_order) + al(sz_q) + al(sort_bytes) +
+                         al((size_t)n_ranges * 8);
+    if (hipMalloc(&scratch, bytes) != hipSuccess) return fail(f, SBGI_E_NOMEM, "rebuild scratch");
+    struct Guard {
+        void *p;
+        ~Guard() { (void)hipFree(p); }
+    } guard{nullptr};
+    char *p = static_cast<char *>(scratch);
+    auto take = [&](size_t sz) {
+        char *q = p;
+        p += al(sz);
+        return q;
+    };
+    uint4 *d_desc = reinterpret_cast<uint4 *>(take(sz_desc));
+    uint32_t *d_pref = reinterpret_cast<uint32_t *>(take(sz_pref));
+    float4 *d_cen = reinterpret_cast<float4 *>(take(sz_cen));
+    uint32_t *d_cb = reinterpret_cast<uint32_t *>(take(sz_cb));
+    unsigned long long *d_keys = reinterpret_cast<unsigned long long *>(take(sz_keys));
+    unsigned long long *d_keys_s = reinterpret_cast<unsigned long long *>(take(sz_keys));
+    uint32_t *d_vals = reinterpret_cast<uint32_t *>(take(sz_vals));
+    uint32_t *d_vals_s = reinterpret_cast<uint32_t *>(take(sz_vals));
+    uint32_t *d_old = reinterpret_cast<uint32_t *>(take(sz_old));  // 3 index words + material per triangle
+    uint4 *d_ta = reinterpret_cast<uint4 *>(take(sz_tasks));
+    uint4 *d_tb = reinterpret_cast<uint4 *>(take(sz_tasks));
+    uint32_t *d_nnext = reinterpret_cast<uint32_t *>(take(4));
+    uint32_t *d_order = reinterpret_cast<uint32_t *>(take(sz_order));
+    uint32_t *d_q = reinterpret_cast<uint32_t *>(take(sz_q));
+    void *d_sort = take(sort_bytes);
+    uint2 *d_tr = reinterpret_cast<uint2 *>(take((size_t)n_ranges * 8));
+    int32_t rc = SBGI_OK;
+    if ((rc = record_kev(f, 0)) != SBGI_OK || (rc = record_kev(f, 1)) != SBGI_OK) {
+        (void)hipFree(scratch);
+        return rc;
+    }
+    guard.p = scratch;
+    std::vector<uint32_t> cb_init((size_t)n_ranges * 6);
+    for (uint32_t r = 0; r < n_ranges; r++)
+        for (int q = 0; q < 3; q++) {
+            cb_init[r * 6 + q] = 0xFF800000u;      // order-mapped +inf
+            cb_init[r * 6 + 3 + q] = 0x007FFFFFu;  // order-mapped -inf
+        }
+    std::vector<uint2> trh(n_ranges);
+    uint32_t tmax = 0;
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        trh[r] = make_uint2(desc[r].z, desc[r].z + desc[r].w);
+        tmax = std::max(tmax, desc[r].w);
+    }
+    HIP_OK(f, hipMemcpyAsync(d_desc, desc.data(), sz_desc, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipMemcpyAsync(d_pref, prefix.data(), sz_pref, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipMemcpyAsync(d_cb, cb_init.data(), sz_cb, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipMemcpyAsync(d_tr, trh.data(), (size_t)n_ranges * 8, hipMemcpyHostToDevice, st));
+    // the ranges' triangle rows, concatenated (the permutation gathers from this copy)
+    uint32_t *d_old_idx = d_old, *d_old_mat = d_old + (size_t)n * 3;
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        HIP_OK(f, hipMemcpyAsync(d_old_idx + (size_t)prefix[r] * 3, f->d_tri_index + (size_t)desc[r].z * 3,
+                                 (size_t)desc[r].w * 12, hipMemcpyDeviceToDevice, st));
+        HIP_OK(f, hipMemcpyAsync(d_old_mat + prefix[r], f->d_tri_mat + desc[r].z, (size_t)desc[r].w * 4,
+                                 hipMemcpyDeviceToDevice, st));
+    }
+    HIP_OK(f, launch_rebuild_prepare(d_desc, d_pref, n_ranges, n, f->d_tri_index, f->d_vdata, d_cen, d_cb, d_keys,
+                                     d_vals, d_keys_s, d_vals_s, d_sort, sort_bytes, 30u + range_bits, st));
+    HIP_OK(f, launch_permute_tris(d_desc, d_pref, n_ranges, n, d_vals_s, d_old_idx, d_old_mat, f->d_tri_index,
+                                  f->d_tri_mat, st));
+    // top-down structure, one launch per depth
+    std::vector<uint4> roots(n_ranges);
+    for (uint32_t r = 0; r < n_ranges; r++) roots[r] = make_uint4(desc[r].x, r, 0u, (desc[r].y + 1) / 2);
+    HIP_OK(f, hipMemcpyAsync(d_ta, roots.data(), (size_t)n_ranges * 16, hipMemcpyHostToDevice, st));
+    std::vector<uint32_t> level_off{0u};
+    uint32_t ntask = n_ranges;
+    uint4 *cur = d_ta, *nxt = d_tb;
+    while (ntask) {
+        if (level_off.back() + (uint64_t)ntask > nodes_total)
+            return fail(f, SBGI_E_LAYOUT, "rebuild produced more nodes than the range holds");
+        HIP_OK(f, hipMemsetAsync(d_nnext, 0, 4, st));
+        HIP_OK(f, launch_build_level(d_desc, d_pref, d_keys_s, cur, ntask, nxt, d_nnext, f->d_nodes,
+                                     d_order + level_off.back(), st));
+        uint32_t nn = 0;
+        HIP_OK(f, hipMemcpyAsync(&nn, d_nnext, 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(f, hipStreamSynchronize(st));
+        level_off.push_back(level_off.back() + ntask);
+        ntask = nn;
+        std::swap(cur, nxt);
+    }
+    if (level_off.back() != nodes_total) return fail(f, SBGI_E_LAYOUT, "rebuild node count mismatch");
+    // bounds bottom-up (deepest depth first)
+    for (size_t d = level_off.size() - 1; d-- > 0;) {
+        const uint32_t lo[2] = {0u, level_off[d + 1] - level_off[d]};
+        HIP_OK(f, launch_refit_levels(f->d_nodes, f->d_tri_index, f->d_vdata, d_order + level_off[d], lo, 1, st));
+    }
+    // walk copies: positions top down (ordered), then every node of every copy
+    if (ordered(f)) {
+        std::vector<uint32_t> one(1);
+        for (uint32_t r = 0; r < n_ranges; r++)
+            for (uint32_t o = 0; o < copies; o++)
+                HIP_OK(f, hipMemcpyAsync(d_q + (size_t)o * v.node_count + desc[r].x, &desc[r].x, 4,
+                                         hipMemcpyHostToDevice, st));
+        for (size_t d = 0; d + 1 < level_off.size(); d++)
+            HIP_OK(f, launch_copy_positions(f->d_nodes, d_order + level_off[d], level_off[d + 1] - level_off[d], copies,
+                                            d_q, v.node_count, st));
+    }
+    HIP_OK(f, launch_write_copies(f->d_nodes, d_order, (uint32_t)nodes_total, copies, d_q, v.node_count, v.node_count,
+                                  f->d_walk_nodes, f->d_walk_perm, st));
+    HIP_OK(f, launch_pack_tris_ranges(f->d_tri_index, f->d_vdata, f->d_tri_mat, f->d_mats, v.tri_count, d_tr, n_ranges,
+                                      tmax, f->d_tri, st));
+    if ((rc = record_kev(f, 2)) != SBGI_OK) return rc;
+    f->kev_blas = true;
+    if ((rc = record_kev(f, 3)) != SBGI_OK || (rc = device_tlas(f, true)) != SBGI_OK || (rc = record_kev(f, 4)) != SBGI_OK)
+        return rc;
+    f->kev_tlas = true;
+    // the new structure into the host mirrors
+    for (uint32_t r = 0; r < n_ranges; r++) {
+        HIP_OK(f, hipMemcpyAsync(f->h_nodes.data() + (size_t)desc[r].x * 8, f->d_nodes + (size_t)desc[r].x * 8,
+                                 (size_t)desc[r].y * 32, hipMemcpyDeviceToHost, st));
+        HIP_OK(f, hipMemcpyAsync(f->h_tri_index.data() + (size_t)desc[r].z * 3, f->d_tri_index + (size_t)desc[r].z * 3,
+                                 (size_t)desc[r].w * 12, hipMemcpyDeviceToHost, st));
+        HIP_OK(f, hipMemcpyAsync(f->h_tri_mat.data() + desc[r].z, f->d_tri_mat + desc[r].z, (size_t)desc[r].w * 4,
+                                 hipMemcpyDeviceToHost, st));
+    }
+    HIP_OK(f, hipStreamSynchronize(st));
+    // cached refit plans and the differentiable mode's triangle map follow the structure
+    f->call_plan.key.clear();
+    if (f->skin_ranges_n) {
+        std::vector<uint32_t> kr(f->skin_plan.key);
+        f->skin_plan.key.clear();
+        if ((rc = build_plan(f, kr.data(), (uint32_t)kr.size() / 2, f->skin_plan)) != SBGI_OK) return rc;
+    }
+    f->diff_ready = f->diff_valid = false;
     return SBGI_OK;
 }
 

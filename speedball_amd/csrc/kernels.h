@@ -120,6 +120,26 @@ hipError_t launch_inst_bounds(const float *mats, uint32_t inst_base, uint32_t n,
                               const double *world, float *bounds, float *padded, hipStream_t st);
 hipError_t launch_build_tlas(const float *padded, uint32_t n, void *scratch, uint4 *tlas, hipStream_t st);
 
+// k_build.hip — device BLAS rebuild (see that file). ranges: {root, N, t0, T} per BLAS;
+// tri_prefix: triangles before each range (n_ranges + 1, on the device).
+size_t rebuild_sort_bytes(uint32_t total);
+hipError_t launch_rebuild_prepare(const uint4 *ranges, const uint32_t *tri_prefix, uint32_t n_ranges, uint32_t total,
+                                  const uint32_t *tri_index, const float *vdata, float4 *cen, uint32_t *cbounds,
+                                  unsigned long long *keys, uint32_t *vals, unsigned long long *keys_sorted,
+                                  uint32_t *vals_sorted, void *sort_tmp, size_t sort_bytes, uint32_t end_bit,
+                                  hipStream_t st);
+hipError_t launch_permute_tris(const uint4 *ranges, const uint32_t *tri_prefix, uint32_t n_ranges, uint32_t total,
+                               const uint32_t *sorted_vals, const uint32_t *old_index, const uint32_t *old_mat,
+                               uint32_t *tri_index, uint32_t *tri_mat, hipStream_t st);
+hipError_t launch_build_level(const uint4 *ranges, const uint32_t *tri_prefix, const unsigned long long *keys,
+                              const uint4 *tasks, uint32_t n_tasks, uint4 *next, uint32_t *n_next, uint32_t *nodes,
+                              uint32_t *order, hipStream_t st);
+hipError_t launch_copy_positions(const uint32_t *nodes, const uint32_t *order, uint32_t n, uint32_t copies,
+                                 uint32_t *qpos, uint32_t qstride, hipStream_t st);
+hipError_t launch_write_copies(const uint32_t *nodes, const uint32_t *order, uint32_t n, uint32_t copies,
+                               const uint32_t *qpos, uint32_t qstride, uint32_t node_count, uint32_t *walk,
+                               uint32_t *perm, hipStream_t st);
+
 // k_dynamic.hip — device skinning, staged instance-record scatter, fill
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);
 hipError_t launch_skin(const uint32_t *vtx, const float4 *rest, const uint2 *joints, const float4 *weights,

@@ -244,6 +244,11 @@ class ProbeField:
         r = np.ascontiguousarray(np.asarray(blas_ranges, np.uint32).reshape(-1))
         self._chk(self.L.sbgi_refit(self.h, N.ptr(r) if r.size else None, r.size // 2, int(refit_tlas)))
 
+    def rebuild_blas(self, blas_ranges):
+        """Rebuild the BLASes of (root, end) node ranges on the device (sbgi_rebuild_blas)."""
+        r = np.ascontiguousarray(np.asarray(blas_ranges, np.uint32).reshape(-1))
+        self._chk(self.L.sbgi_rebuild_blas(self.h, N.ptr(r), r.size // 2))
+
     def set_instance_transforms(self, slots, worlds, active=None):
         s = np.ascontiguousarray(np.asarray(slots, np.uint32))
         w = np.ascontiguousarray(np.stack([np.asarray(m, np.float64).T.reshape(-1) for m in worlds]))

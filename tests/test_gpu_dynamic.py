@@ -308,3 +308,66 @@ def test_device_sah_tlas_edge_cases(n, kind):
         assert np.array_equal(f.read(N.BUF_TLAS_SAH, np.uint32).reshape(-1, 8), oracle.build_sah_tlas(pad))
     finally:
         f.close()
+
+
+# ---------------------------------------------------------------- device BLAS rebuild
+@pytest.mark.parametrize("traversal", MODES, ids=IDS)
+def test_rebuild_blas_matches_oracle(instanced_small, traversal):
+    """After a deformation that a refit would serve badly (axes rotated, scaled), every
+    instanced BLAS is rebuilt on the device: same node count; each range's triangle rows are
+    a permutation of its old rows; the oracle's refit of the rebuilt tree reproduces the
+    device's bounds bit for bit (so the threaded layout is valid and the bounds tight);
+    hits on the rebuilt scene match the oracle's."""
+    w = instanced_small
+    s = w.scene
+    f = make_field(s, [(w.grid_min, w.grid_size, w.res)], w.rays, w.depth_oct, traversal)
+    try:
+        worlds = slot_worlds_from_rows(s)
+        f.set_instance_transforms(np.arange(s.inst_count), worlds)
+        vd = s.vertex_data.copy().reshape(-1, 8)
+        vd[:, :3] = (vd[:, [1, 2, 0]] * np.float32([1.6, 0.7, 1.2])).astype(np.float32)
+        vd = np.ascontiguousarray(vd.reshape(-1))
+        f.update_ranges(N.BUF_VERTEX_DATA, [0, vd.size], vd)
+        inst = s.materials[s.inst_base:s.inst_base + s.inst_count * 28].reshape(-1, 28)
+        ranges = sorted({(int(r[12]), int(r[13])) for r in inst})
+        print(f"case: rebuild {IDS[traversal]}, {len(ranges)} BLASes", flush=True)
+        f.rebuild_blas(ranges)
+        nodes = f.read(N.BUF_BVH_NODES, np.uint32)
+        ti = f.read(N.BUF_TRI_INDEX, np.uint32)
+        tm = f.read(N.BUF_TRI_MATERIAL, np.uint32)
+        assert nodes.size == s.bvh_nodes.size
+        old_rows = np.concatenate([s.tri_index.reshape(-1, 3), s.tri_material[:, None]], 1)
+        new_rows = np.concatenate([ti.reshape(-1, 3), tm[:, None]], 1)
+        s2 = scene_with(s, vertex_data=vd, tri_index=ti, tri_material=tm)
+        nu = nodes.reshape(-1, 8)
+        for root, end in ranges:
+            pay = nu[root:end, 7]
+            leaf = pay[pay != 0xFFFFFFFF]
+            t0, t1 = int((leaf & 0xFFFFFF).min()), int(((leaf & 0xFFFFFF) + (leaf >> 24)).max())
+            cover = np.zeros(len(old_rows), np.int32)
+            for p in leaf:
+                cover[(p & 0xFFFFFF):(p & 0xFFFFFF) + (p >> 24)] += 1
+            assert np.all(cover[t0:t1] == 1)
+            a = old_rows[t0:t1]
+            b_ = new_rows[t0:t1]
+            assert np.array_equal(a[np.lexsort(a.T[::-1])], b_[np.lexsort(b_.T[::-1])])
+            refit = nodes.copy()
+            assert oracle.refit_blas(refit, s2, root, end)
+            assert np.array_equal(refit.reshape(-1, 8)[root:end], nu[root:end])
+        wd = np.stack([m.T.reshape(-1) for m in worlds])
+        mats = f.read(N.BUF_MATERIALS, np.float32)
+        b64, pad = oracle.inst_bounds(mats, s.inst_base, s.inst_count, nodes, wd)
+        assert np.array_equal(f.read(N.BUF_INST_BOUNDS, np.float32).reshape(-1, 6), b64)
+        if traversal == N.TRAV_MT_ORDERED:
+            assert np.array_equal(f.read(N.BUF_TLAS_SAH, np.uint32).reshape(-1, 8), oracle.build_sah_tlas(pad))
+        f.tick(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+        scene2, _ = oracle_scene_after_edits(f, s2, vd, nodes, wd)
+        check_hits(f, scene2, [(w.grid_min, w.grid_size, w.res)], w.rays, w.depth_oct, w.frame_jitter,
+                   w.hysteresis, label=f"rebuild {IDS[traversal]}")
+        # a later refit of the rebuilt ranges (cached plans were invalidated) keeps the bounds
+        f.refit(ranges, refit_tlas=True)
+        assert np.array_equal(f.read(N.BUF_BVH_NODES, np.uint32), nodes)
+        sk, rb, tl = f.edit_timing()
+        print(f"rebuild timing: BLAS refit {rb:.4f} ms, TLAS {tl:.4f} ms")
+    finally:
+        f.close()

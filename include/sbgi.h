@@ -94,6 +94,10 @@ extern "C" {
 #define SBGI_BUF_INST_BOUNDS 19 /* f32[6*I] instance world AABBs per TLAS slot (read)  */
 #define SBGI_BUF_TLAS_SAH 20    /* u32[8*(2I-1)] MT_ORDERED SAH TLAS, walk format {min.xyz,
                                    miss},{max.xyz, payload} (read)                     */
+#define SBGI_BUF_EMITTER_VIS 21 /* f32[16*P] per cascade: retained emitter visibility,
+                                   probe-major, 16 = SBGI_MAX_EMITTERS (read, write)  */
+#define SBGI_BUF_LIGHT_GRID 22  /* u32[17*cells] clustered light lists of cascade 0 (read):
+                                   per cell {count, light ids[16]}, count 17 = overflow */
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {
@@ -117,6 +121,16 @@ typedef struct sbgi_field_desc {
 } sbgi_field_desc;
 
 #define SBGI_FIELD_KEEP_HIT_IDS 1u /* store per-ray (tri, inst) for parity checks */
+#define SBGI_FIELD_CLUSTERED 2u    /* clustered NEE: per-cell light lists over cascade 0's
+                                      cells (clusteredLighting, js/gi_probes.js:1097,
+                                      1888-1938, 2064-2127); needs cascade 0 res >= 2 on
+                                      every axis. Same radiance as the all-lights loop.  */
+
+/* Emitter lights (type 3 records, js/spectral_scene.js:815-875) injected as virtual rays
+   (js/gi_probes.js:133-134, 2216-2310): at most this many per scene. */
+#define SBGI_MAX_EMITTERS 16
+#define SBGI_EMITTER_RETENTION 0.8f /* GI_EMITTER_VIS_RETENTION                       */
+#define SBGI_LIGHTS_PER_CELL 16     /* GI_LIGHTS_PER_CELL                             */
 
 /* ---- scene view (reference layouts, borrowed host pointers) ---------------- */
 typedef struct sbgi_scene_view {
@@ -134,7 +148,9 @@ typedef struct sbgi_scene_view {
     uint32_t tlas_base;          /* float index of TLAS node 0                       */
     uint32_t tlas_node_count;
     const float *lights;         /* 17 floats per light                              */
-    uint32_t light_count;
+    uint32_t light_count;        /* U.lightCount: analytic lights and emitters       */
+    uint32_t emitter_base;       /* U.emitterBase: first emitter record (type 3)     */
+    uint32_t emitter_count;      /* U.emitterCount <= SBGI_MAX_EMITTERS              */
     uint32_t pad;
 } sbgi_scene_view;
 
@@ -175,6 +191,8 @@ typedef struct sbgi_tick_params {
     float dbg_filter_k;          /* U.debugFilterKScale (1)                          */
     float dbg_filter_rel;        /* U.debugFilterRelScale (1)                        */
     float dbg_filter_eps;        /* U.debugFilterEpsScale (1)                        */
+    float emitter_vis_seed;      /* U.emitterVisSeed (0): emitter sample jitter      */
+    uint32_t pad;
 } sbgi_tick_params;
 
 /* per-stage device time of the last completed tick, from HIP events */

@@ -139,6 +139,17 @@ class OracleField:
         self.datlas[3::4] = 1
         self.ray_data = None
         self.hit_ids = None
+        self.evis = np.zeros(P * 16, self.dt)  # retained emitter visibility
+        self.light_grid = None                  # clustered NEE lists (set_clustered)
+        self.c0 = None
+
+    def set_clustered(self, c0_grid_min=None, c0_grid_size=None, c0_res=None):
+        """Clustered NEE over the cells of cascade 0 (this field's grid by default)."""
+        if c0_res is None:
+            self.c0 = self.c
+        else:
+            self.c0 = cascade_desc(c0_grid_min, c0_grid_size, c0_res)
+        self.light_grid = light_grid(self.scene, self.c0)
 
     def params(self, **kw):
         from speedball_amd._native import default_tick_params
@@ -149,20 +160,36 @@ class OracleField:
         self.ray_data = np.zeros(n * 4, self.dt)
         self.hit_ids = np.zeros(n * 2, np.int32)
         cnt = np.zeros(5, np.uint64) if count else None
-        _fn(self.prec, "trace")(C.byref(self.scene.view()), C.byref(self.c), C.c_uint32(self.rays), C.byref(p),
-                                _p(self.atlas), _p(self.meta), _p(self.ray_data), _p(self.hit_ids), _p(cnt))
+        sg = _fn(self.prec, "set_light_grid")
+        if self.light_grid is not None:
+            sg(_p(self.light_grid), C.byref(self.c0))
+        try:
+            _fn(self.prec, "trace")(C.byref(self.scene.view()), C.byref(self.c), C.c_uint32(self.rays), C.byref(p),
+                                    _p(self.atlas), _p(self.meta), _p(self.ray_data), _p(self.hit_ids), _p(cnt))
+        finally:
+            sg(None, None)
         return cnt
 
+    def emitter_vis(self, p):
+        """emitterVisKernel over the updated probes (no-op without emitters)."""
+        if self.scene.emitter_count:
+            _fn(self.prec, "emitter_vis")(C.byref(self.scene.view()), C.byref(self.c), C.byref(p), _p(self.meta),
+                                          _p(self.evis))
+
     def blend(self, p):
-        _fn(self.prec, "blend")(C.byref(self.c), C.c_uint32(self.rays), C.c_uint32(self.depth_oct), C.byref(p),
-                                _p(self.ray_data), _p(self.irr), _p(self.depth))
+        _fn(self.prec, "blend_ex")(C.byref(self.c), C.c_uint32(self.rays), C.c_uint32(self.depth_oct), C.byref(p),
+                                   _p(self.ray_data), _p(self.irr), _p(self.depth), C.byref(self.scene.view()),
+                                   _p(self.meta), _p(self.evis))
 
     def upload(self, p):
         _fn(self.prec, "upload")(C.byref(self.c), C.c_uint32(self.depth_oct), C.byref(p), _p(self.irr),
                                  _p(self.depth), _p(self.atlas), _p(self.datlas))
 
     def update(self, p):
+        if p.needs_clear:
+            self.evis[:] = 0
         self.trace(p, p.updated_count or None)
+        self.emitter_vis(p)
         self.blend(p)
         self.upload(p)
 
@@ -184,6 +211,14 @@ class OracleField:
 
     def hits_view(self):
         return self.hit_ids.reshape(-1, self.rays, 2)
+
+
+def light_grid(scene, c0) -> np.ndarray:
+    """lightGridKernel lists of cascade 0 (a CascadeDesc): u32[cells * 17]."""
+    cells = (c0.res[0] - 1) * (c0.res[1] - 1) * (c0.res[2] - 1)
+    out = np.zeros(cells * 17, np.uint32)
+    lib().or32_light_grid(C.byref(scene.view()), C.byref(c0), _p(out))
+    return out
 
 
 def refit_blas(nodes: np.ndarray, scene, root: int, end: int) -> bool:

@@ -435,6 +435,49 @@ typedef struct {
 } P(diff_ctx);
 static P(diff_ctx) P(g_diff) = {NULL, NULL, NULL};
 
+/* Clustered NEE (js/gi_probes.js:2112-2127): cell lists {count, ids[16]} over cascade 0's
+ * cells. Set around a trace by P(set_light_grid); NULL visits every light. */
+typedef struct {
+    const uint32_t *grid;
+    float gmin[3], gsize[3];
+    uint32_t cells[3];
+} P(grid_ctx);
+static P(grid_ctx) P(g_grid) = {NULL, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+
+void P(set_light_grid)(const uint32_t *grid, const sbgi_cascade_desc *c0) {
+    P(g_grid).grid = grid;
+    if (grid && c0) {
+        for (int a = 0; a < 3; a++) {
+            P(g_grid).gmin[a] = c0->grid_min[a];
+            P(g_grid).gsize[a] = c0->grid_size[a];
+            P(g_grid).cells[a] = c0->res[a] - 1u;
+        }
+    }
+}
+
+/* Bit li set for the lights the NEE loop visits at hit point `hit`: the hit cell's list,
+ * or every light outside the grid / for an overflowed list. */
+static uint64_t P(light_members)(const sbgi_scene_view *s, const REAL hit[3]) {
+    uint64_t all = s->light_count >= 64 ? ~0ull : ((1ull << s->light_count) - 1ull);
+    if (!P(g_grid).grid) return all;
+    REAL cf[3];
+    int outside = 0;
+    for (int a = 0; a < 3; a++) {
+        REAL cells = (REAL)((float)(P(g_grid).cells[a] + 1u) - 1.0f);
+        REAL size = (REAL)P(g_grid).gsize[a] / cells;
+        cf[a] = RFLOOR((hit[a] - (REAL)P(g_grid).gmin[a]) / size);
+        outside = outside || cf[a] < 0 || cf[a] >= cells;
+        cf[a] = cf[a] < 0 ? 0 : (cf[a] > cells - 1 ? cells - 1 : cf[a]);
+    }
+    uint32_t cell = (uint32_t)cf[2] * (P(g_grid).cells[0] * P(g_grid).cells[1]) +
+                    (uint32_t)cf[1] * P(g_grid).cells[0] + (uint32_t)cf[0];
+    const uint32_t *list = P(g_grid).grid + (size_t)cell * (SBGI_LIGHTS_PER_CELL + 1);
+    if (outside || list[0] > SBGI_LIGHTS_PER_CELL) return all;
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < list[0]; i++) m |= 1ull << list[1 + i];
+    return m;
+}
+
 /* software bilinear, texel-centre convention, clamp-to-edge, of an RGBA atlas */
 static void P(bilinear)(const REAL *atlas, uint32_t w, uint32_t h, REAL u, REAL v, REAL out[3]) {
     REAL x = u * (REAL)w - RCONST(0.5);
@@ -546,8 +589,11 @@ void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t
             REAL v = base[a] * (REAL)p->dbg_albedo * (RCONST(1) - metal) * (RCONST(1) - transm);
             kd[a] = v < 0 ? 0 : (v > RCONST(0.95) ? RCONST(0.95) : v);
         }
-        /* NEE over all lights: js/gi_probes.js:2129-2176 */
+        /* NEE over all lights (js/gi_probes.js:2129-2176), or over the hit cell's list with
+         * clustered lighting (:2064-2127) */
+        uint64_t members = P(light_members)(s, hit);
         for (uint32_t li = 0; li < s->light_count; li++) {
+            if (!((members >> li) & 1ull)) continue;
             const float *L = s->lights + li * 17u;
             float ltype = L[0];
             if (!(ltype < 2.5f)) continue;
@@ -654,8 +700,21 @@ void P(texel_dir)(uint32_t tx, uint32_t ty, uint32_t n, REAL out[3]) {
 /* blendKernel js/gi_probes.js:2317-2532: cosine gather + adaptive temporal policy.
  * depth_oct = 6 keeps depth in the 8x8 irradiance tile (reference); 14 uses a 16x16
  * depth tile whose interior texels gather with their own directions. */
+void P(blend_ex)(const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct, const sbgi_tick_params *p,
+                 const REAL *ray_data, REAL *irr, REAL *depth, const sbgi_scene_view *s, const float *meta,
+                 const REAL *evis);
+
 void P(blend)(const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct, const sbgi_tick_params *p,
               const REAL *ray_data, REAL *irr, REAL *depth) {
+    P(blend_ex)(c, rays, depth_oct, p, ray_data, irr, depth, NULL, NULL, NULL);
+}
+
+/* blendKernel with the emitters of s injected as virtual rays after the ray loop
+ * (injectEmitterVirtualRays, js/gi_probes.js:2262-2292, called at :2407-2426); evis =
+ * retained visibility [probe * 16 + e]. s == NULL: no emitters. */
+void P(blend_ex)(const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct, const sbgi_tick_params *p,
+                 const REAL *ray_data, REAL *irr, REAL *depth, const sbgi_scene_view *s, const float *meta,
+                 const REAL *evis) {
     float max_dist, cell_min;
     P(cascade_derived)(c, &max_dist, &cell_min);
     uint32_t total = c->res[0] * c->res[1] * c->res[2];
@@ -694,6 +753,33 @@ void P(blend)(const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct, con
                 dacc += rdist * dw;
                 dacc2 += rdist * rdist * dw;
                 dwsum += dw;
+            }
+            if (s && s->emitter_count) {
+                REAL ro[3];
+                P(probe_pos)(c, probe, meta, p->classify_strength, ro);
+                for (uint32_t e = 0; e < s->emitter_count; e++) {
+                    const float *L = s->lights + (size_t)(s->emitter_base + e) * 17u;
+                    REAL axis[3] = {(REAL)L[1] - ro[0], (REAL)L[2] - ro[1], (REAL)L[3] - ro[2]};
+                    REAL dist = RSQRT(P(dot3)(axis, axis));
+                    REAL radius = L[14] > 0 ? (REAL)L[14] : 0;
+                    if (!(dist > (radius > RCONST(1e-3) ? radius : RCONST(1e-3)))) continue;
+                    REAL d[3] = {axis[0] / dist, axis[1] / dist, axis[2] / dist};
+                    REAL proj = L[15] > 0 ? (REAL)L[15] : 0;
+                    REAL omega = proj / (dist * dist);
+                    omega = omega < RCONST(6.283185307179586) ? omega : RCONST(6.283185307179586);
+                    REAL kv = ((REAL)rays * omega) / RCONST(12.566370614359172);
+                    kv = kv < (REAL)rays ? kv : (REAL)rays;
+                    REAL vis = evis[(size_t)probe * SBGI_MAX_EMITTERS + e];
+                    REAL Lc[3] = {(REAL)L[7] * vis, (REAL)L[8] * vis, (REAL)L[9] * vis};
+                    REAL cd = P(dot3)(dir, d);
+                    cd = cd > 0 ? cd : 0;
+                    REAL wk = RPOW(cd, cpow) * kv;
+                    for (int a = 0; a < 3; a++) acc[a] += Lc[a] * wk;
+                    wsum += wk;
+                    REAL lum = Lc[0] * LUMA[0] + Lc[1] * LUMA[1] + Lc[2] * LUMA[2];
+                    lacc += lum * wk;
+                    lacc2 += lum * lum * wk;
+                }
             }
             REAL ws = wsum > RCONST(1e-4) ? wsum : RCONST(1e-4);
             REAL mean[3] = {acc[0] / ws, acc[1] / ws, acc[2] / ws};
@@ -868,6 +954,107 @@ void P(upload)(const sbgi_cascade_desc *c, uint32_t depth_oct, const sbgi_tick_p
                 datlas[o + 3] = 1;
             }
         }
+    }
+}
+
+/* emitterVisKernel (js/gi_probes.js:2216-2260): for every updated probe and emitter, one
+ * any-hit ray from the probe's trace origin toward a hashed point of the emitter's disc
+ * facing the probe; evis[probe * 16 + e] = mix(sample, evis, 0.8). The hash
+ * fract(sin(seed) * 43758.5453) is the sampling basis and is evaluated in single precision
+ * in both oracles, with sin rounded once from double; the rest follows REAL. */
+void P(emitter_vis)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, const sbgi_tick_params *p,
+                    const float *meta, REAL *evis) {
+    float max_dist, cell_min;
+    P(cascade_derived)(c, &max_dist, &cell_min);
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+    REAL b1 = (REAL)cell_min * RCONST(0.005) * (REAL)p->dbg_trace_bias;
+    REAL b2 = RCONST(RAY_EPS) * (REAL)p->dbg_ray_eps;
+    REAL bias = b1 > b2 ? b1 : b2;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        REAL ro[3];
+        P(probe_pos)(c, probe, meta, p->classify_strength, ro);
+        for (uint32_t e = 0; e < s->emitter_count; e++) {
+            const float *L = s->lights + (size_t)(s->emitter_base + e) * 17u;
+            REAL center[3] = {L[1], L[2], L[3]};
+            REAL radius = L[14] > 0 ? (REAL)L[14] : 0;
+            REAL axis[3] = {center[0] - ro[0], center[1] - ro[1], center[2] - ro[2]};
+            REAL dist = RSQRT(P(dot3)(axis, axis));
+            REAL sample = 0;
+            if (dist > (radius > RCONST(1e-3) ? radius : RCONST(1e-3))) {
+                REAL d[3] = {axis[0] / dist, axis[1] / dist, axis[2] / dist};
+                REAL up[3] = {0, 1, 0};
+                if (!(RABS(d[1]) < RCONST(0.999))) {
+                    up[0] = 1;
+                    up[1] = 0;
+                }
+                REAL tg[3], bt[3];
+                P(cross3)(up, d, tg);
+                P(normalize3)(tg);
+                P(cross3)(d, tg, bt);
+                float es = ((float)probe * 12.9898f + (float)e * 78.233f) + p->emitter_vis_seed * 37.719f;
+                float h0 = (float)sin((double)es) * 43758.5453f;
+                float h1 = (float)sin((double)(es + 19.19f)) * 24634.6345f;
+                float u = h0 - floorf(h0), v = h1 - floorf(h1);
+                REAL rad = RSQRT((REAL)u) * radius;
+                REAL ang = (REAL)v * (RCONST(PI_D) * RCONST(2));
+                REAL ca = RCOSCR(ang), sa = RSINCR(ang);
+                REAL tgt[3], dir[3];
+                for (int a = 0; a < 3; a++) {
+                    tgt[a] = center[a] + (tg[a] * (ca * rad) + bt[a] * (sa * rad));
+                    dir[a] = tgt[a] - ro[a];
+                }
+                P(normalize3)(dir);
+                REAL tmax = (dist - radius) - bias;
+                tmax = tmax > RCONST(1e-4) ? tmax : RCONST(1e-4);
+                sample = P(any)(s, ro, dir, tmax, NULL) ? 0 : 1;
+            }
+            REAL *v = evis + (size_t)probe * SBGI_MAX_EMITTERS + e;
+            *v = sample * (RCONST(1) - (REAL)SBGI_EMITTER_RETENTION) + *v * (REAL)SBGI_EMITTER_RETENTION;
+        }
+    }
+}
+
+/* Light lists of cascade 0's cells (lightGridKernel js/gi_probes.js:1888-1938), in the
+ * reference's single precision whatever REAL is: cell box from gridMin + index x
+ * (gridSize / cells), membership when directional, unranged, or within the padded range of
+ * the box. out[cell * 17] = count (17 = overflow), then up to 16 ids; unused slots 0. */
+void P(light_grid)(const sbgi_scene_view *s, const sbgi_cascade_desc *c0, uint32_t *out) {
+    uint32_t cx = c0->res[0] - 1u, cy = c0->res[1] - 1u, cz = c0->res[2] - 1u;
+    for (uint32_t cell = 0; cell < cx * cy * cz; cell++) {
+        uint32_t ix = cell % cx, iy = (cell / cx) % cy, iz = cell / (cx * cy);
+        float size[3] = {c0->grid_size[0] / (float)cx, c0->grid_size[1] / (float)cy, c0->grid_size[2] / (float)cz};
+        uint32_t idx[3] = {ix, iy, iz};
+        float mn[3], mx[3];
+        for (int a = 0; a < 3; a++) {
+            mn[a] = c0->grid_min[a] + (float)idx[a] * size[a];
+            mx[a] = mn[a] + size[a];
+        }
+        uint32_t *list = out + (size_t)cell * (SBGI_LIGHTS_PER_CELL + 1);
+        memset(list, 0, sizeof(uint32_t) * (SBGI_LIGHTS_PER_CELL + 1));
+        uint32_t count = 0;
+        for (uint32_t li = 0; li < s->light_count; li++) {
+            const float *L = s->lights + li * 17u;
+            float dd[3];
+            for (int a = 0; a < 3; a++) {
+                float lo = mn[a] - L[1 + a];
+                lo = lo > 0.f ? lo : 0.f;
+                float hi = L[1 + a] - mx[a];
+                dd[a] = lo > hi ? lo : hi;
+            }
+            float range = L[10];
+            float pad = range * 1e-5f;
+            float padded = range + (pad > 1e-4f ? pad : 1e-4f);
+            int reaches = (dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2] <= padded * padded;
+            int member = L[0] < 2.5f && (L[0] < 0.5f || range <= 0.f || reaches);
+            if (member) {
+                if (count < SBGI_LIGHTS_PER_CELL) list[1 + count] = li;
+                count++;
+            }
+        }
+        list[0] = count > SBGI_LIGHTS_PER_CELL ? SBGI_LIGHTS_PER_CELL + 1 : count;
     }
 }
 

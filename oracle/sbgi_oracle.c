@@ -61,6 +61,8 @@ float half_round(float f) {
 #define RFLOOR floor
 #define RSIN sin
 #define RCOS cos
+#define RSINCR sin
+#define RCOSCR cos
 #define RPOW pow
 #define REXP exp
 #define RFMIN fmin
@@ -74,6 +76,8 @@ float half_round(float f) {
 #undef RFLOOR
 #undef RSIN
 #undef RCOS
+#undef RSINCR
+#undef RCOSCR
 #undef RPOW
 #undef REXP
 #undef RFMIN
@@ -89,6 +93,8 @@ float half_round(float f) {
 #define RFLOOR floorf
 #define RSIN sinf
 #define RCOS cosf
+#define RSINCR(x) ((float)sin((double)(x)))
+#define RCOSCR(x) ((float)cos((double)(x)))
 #define RPOW powf
 #define REXP expf
 #define RFMIN fminf

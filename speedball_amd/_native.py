@@ -23,10 +23,15 @@ SBGI_E_NOMEM = -7
 BLEND_FP32, BLEND_BF16, BLEND_BF16X3, BLEND_FP16 = 0, 1, 2, 3
 TRAV_MT_COMPAT, TRAV_MT_ORDERED, TRAV_WATERTIGHT = 0, 1, 2
 FIELD_KEEP_HIT_IDS = 1
+FIELD_CLUSTERED = 2
+MAX_EMITTERS = 16
+LIGHTS_PER_CELL = 16
+EMITTER_RETENTION = 0.8
 
 BUF_BVH_NODES, BUF_TRI_INDEX, BUF_VERTEX_DATA, BUF_TRI_MATERIAL, BUF_MATERIALS, BUF_LIGHTS = 0, 1, 2, 3, 4, 5
 BUF_IRR_STATE, BUF_DEPTH_STATE, BUF_META, BUF_IRR_ATLAS, BUF_DEPTH_ATLAS = 10, 11, 12, 13, 14
 BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT, BUF_TRI_PACKED, BUF_INST_BOUNDS, BUF_TLAS_SAH = 15, 16, 17, 18, 19, 20
+BUF_EMITTER_VIS, BUF_LIGHT_GRID = 21, 22
 
 # every symbol include/sbgi.h declares
 EXPORTS = [
@@ -61,7 +66,7 @@ class SceneView(C.Structure):
                 ("materials_len", C.c_uint32), ("tri_material", C.c_void_p), ("materials", C.c_void_p),
                 ("inst_base", C.c_uint32), ("inst_count", C.c_uint32), ("tlas_base", C.c_uint32),
                 ("tlas_node_count", C.c_uint32), ("lights", C.c_void_p), ("light_count", C.c_uint32),
-                ("pad", C.c_uint32)]
+                ("emitter_base", C.c_uint32), ("emitter_count", C.c_uint32), ("pad", C.c_uint32)]
 
 
 TICK_FLOATS = [
@@ -81,7 +86,8 @@ class TickParams(C.Structure):
                  ("needs_clear", C.c_uint32)]
                 + [(n, C.c_float) for n in TICK_FLOATS]
                 + [("sky_sh", C.c_float * 27)]
-                + [(n, C.c_float) for n in TICK_DEBUG])
+                + [(n, C.c_float) for n in TICK_DEBUG]
+                + [("emitter_vis_seed", C.c_float), ("pad", C.c_uint32)])
 
 
 class Stats(C.Structure):

@@ -114,6 +114,7 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
                 const CascadeState &cs = f->casc[c];
                 BHIP_OK(b, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
                 BHIP_OK(b, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
+                BHIP_OK(b, launch_fill_u32(reinterpret_cast<uint32_t *>(cs.evis), 0u, cs.total * SBGI_MAX_EMITTERS, st));
             }
         BHIP_OK(b, launch_raydirs(f->dirs, c0.rays, p.frame_jitter, st));
         BHIP_OK(b, launch_weights(c0, p, f->dirs, f->bt, f->desc.blend_precision, st));
@@ -137,7 +138,10 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
         const sbgi_tick_params &p = params[g.field];
         BHIP_OK(b, launch_resolve(ts[j].s, ts[j].c, p, ts[j].dirs, ts[j].meta, cs.atlas, ts[j].w.hits, ts[j].w.vis,
                                   b->rays[j], st));
-        BHIP_OK(b, launch_blend(ts[j].c, p, b->rays[j], f->bt, f->desc.blend_precision, cs.irr, cs.depth, st));
+        BHIP_OK(b, launch_emitter_vis(ts[j].s, ts[j].c, p, ts[j].meta, cs.evis, ts[j].w.queue.count + trace_heads(3),
+                                      st));
+        BHIP_OK(b, launch_blend(ts[j].c, p, b->rays[j], f->bt, f->desc.blend_precision, cs.irr, cs.depth,
+                                emitter_inject(f, cs, p), st));
         BHIP_OK(b, launch_upload(ts[j].c, p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st));
     }
     return SBGI_OK;

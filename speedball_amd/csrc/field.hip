@@ -138,6 +138,9 @@ int32_t validate_scene(sbgi_field *f) {
     if (v.node_count >= (1u << 24) || v.tri_count >= (1u << 24) || v.inst_count >= (1u << 24))
         return fail(f, SBGI_E_CAPACITY, "BVH pool or instance count exceeds the 24-bit index ceiling");
     if (v.light_count > SBGI_MAX_LIGHTS) return fail(f, SBGI_E_CAPACITY, "more than MAX_LIGHTS (64) lights");
+    if (v.emitter_count > SBGI_MAX_EMITTERS) return fail(f, SBGI_E_CAPACITY, "more than 16 emitters");
+    if ((uint64_t)v.emitter_base + v.emitter_count > v.light_count)
+        return fail(f, SBGI_E_INVALID, "emitter records lie outside the light records");
     if (v.node_count == 0 || v.tri_count == 0 || v.inst_count == 0 || v.tlas_node_count == 0)
         return fail(f, SBGI_E_INVALID, "empty scene");
     if ((size_t)v.inst_base + (size_t)v.inst_count * 28 > v.tlas_base ||
@@ -306,6 +309,15 @@ int32_t repack_all(sbgi_field *f) {
 
 extern "C" {
 
+// clustered lighting: the cell lists of the current light records, stream-ordered
+int32_t build_light_grid(sbgi_field *f) {
+    if (!f->d_light_grid) return SBGI_OK;
+    const sbgi_cascade_desc &c0 = f->casc[0].d;
+    HIP_OK(f, launch_light_grid(f->d_lights, f->view.light_count, c0.grid_min, c0.grid_size, f->lg_cells,
+                                f->d_light_grid, f->stream));
+    return SBGI_OK;
+}
+
 void sbgi_tick_params_default(sbgi_tick_params *p) {
     if (!p) return;
     memset(p, 0, sizeof(*p));
@@ -397,6 +409,9 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         double cm = std::min({gx / std::max(1u, cs.d.res[0] - 1), gy / std::max(1u, cs.d.res[1] - 1),
                               gz / std::max(1u, cs.d.res[2] - 1)});
         cs.cell_min = (float)std::max(1e-4, cm);
+        if (dalloc(&cs.evis, (size_t)cs.total * SBGI_MAX_EMITTERS * 4) != hipSuccess)
+            return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+        (void)hipMemsetAsync(cs.evis, 0, (size_t)cs.total * SBGI_MAX_EMITTERS * 4, f->stream);
         const size_t depth_texels = desc->depth_oct_res == 6 ? 64 : (size_t)tile_d * tile_d;
         if (dalloc(&cs.irr, (size_t)cs.total * 64 * 16) != hipSuccess ||
             dalloc(&cs.depth, (size_t)cs.total * depth_texels * 8) != hipSuccess ||
@@ -411,6 +426,15 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
             launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, f->stream) != hipSuccess)
             return bail(fail(f, SBGI_E_DEVICE, "atlas clear failed"));
         f->max_total = std::max(f->max_total, cs.total);
+    }
+    if (desc->flags & SBGI_FIELD_CLUSTERED) {
+        const uint32_t *r0 = desc->cascades[0].res;
+        if (r0[0] < 2 || r0[1] < 2 || r0[2] < 2)
+            return bail(fail(f, SBGI_E_INVALID, "clustered lighting needs cascade 0 res >= 2 on every axis"));
+        for (int a = 0; a < 3; a++) f->lg_cells[a] = r0[a] - 1;
+        if (dalloc(&f->d_light_grid, light_grid_cells(f) * (SBGI_LIGHTS_PER_CELL + 1) * 4) != hipSuccess)
+            return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+        (void)hipMemsetAsync(f->d_light_grid, 0, light_grid_cells(f) * (SBGI_LIGHTS_PER_CELL + 1) * 4, f->stream);
     }
     const uint32_t tdn = desc->depth_oct_res == 6 ? 36 : desc->depth_oct_res * desc->depth_oct_res;
     const size_t tb = blend_tables_bytes(R, tdn);
@@ -498,6 +522,7 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     HIP_OK(f, up(f->d_vdata, f->h_vdata.data(), f->h_vdata.size() * 4));
     HIP_OK(f, up(f->d_mats, f->h_mats.data(), f->h_mats.size() * 4));
     HIP_OK(f, up(f->d_lights, f->h_lights.data(), f->h_lights.size() * 4));
+    if ((rc = build_light_grid(f)) != SBGI_OK) return rc;
     HIP_OK(f, up(f->d_world, f->h_world.data(), f->h_world.size() * sizeof(double)));
     HIP_OK(f, up(f->d_tlas_parent, tparent.data(), tparent.size() * 4));
     rc = repack_all(f);
@@ -557,7 +582,7 @@ int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges
             HIP_OK(f, hipMemcpyAsync(static_cast<char *>(dev) + st * esz, static_cast<char *>(host) + st * esz,
                                      cnt * esz, hipMemcpyHostToDevice, f->stream));
     }
-    if (buf_id == SBGI_BUF_LIGHTS) return SBGI_OK;
+    if (buf_id == SBGI_BUF_LIGHTS) return build_light_grid(f);
     if ((rc = repack_all(f)) != SBGI_OK) return rc;
     if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_MATERIALS) {
         if ((rc = sync_nodes(f)) != SBGI_OK) return rc;
@@ -830,14 +855,17 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     if (p->needs_clear) {
         HIP_OK(f, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
         HIP_OK(f, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
+        HIP_OK(f, launch_fill_u32(reinterpret_cast<uint32_t *>(cs.evis), 0u, cs.total * SBGI_MAX_EMITTERS, st));
     }
     HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
     HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
     HIP_OK(f, hipEventRecord(f->ev[1], st));
     HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
                            f->count_mode ? f->counters : nullptr, f->tev, st));
+    HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.queue.count + trace_heads(3), st));
     HIP_OK(f, hipEventRecord(f->ev[2], st));
-    HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, st));
+    HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, emitter_inject(f, cs, *p),
+                           st));
     HIP_OK(f, hipEventRecord(f->ev[3], st));
     HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st));
     HIP_OK(f, hipEventRecord(f->ev[4], st));
@@ -910,6 +938,9 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
         case SBGI_BUF_TRI_MATERIAL: return f->h_tri_mat.size() * 4;
         case SBGI_BUF_INST_BOUNDS: return f->has_scene ? (size_t)f->view.inst_count * 24 : 0;
         case SBGI_BUF_TLAS_SAH: return f->has_scene ? (size_t)f->tlas_sah_count * 32 : 0;
+        case SBGI_BUF_EMITTER_VIS:
+            return cascade < f->desc.cascade_count ? (size_t)f->casc[cascade].total * SBGI_MAX_EMITTERS * 4 : 0;
+        case SBGI_BUF_LIGHT_GRID: return light_grid_cells(f) * (SBGI_LIGHTS_PER_CELL + 1) * 4;
     }
     return 0;
 }
@@ -966,20 +997,28 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
         HIP_OK(f, hipMemcpy(dst, f->d_nodes, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
+    if (buf_id == SBGI_BUF_EMITTER_VIS || buf_id == SBGI_BUF_LIGHT_GRID) {
+        src = buf_id == SBGI_BUF_EMITTER_VIS ? (const void *)f->casc[cascade].evis : (const void *)f->d_light_grid;
+        HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
     return fail(f, SBGI_E_INVALID, "buffer not readable");
 }
 
 int32_t sbgi_write(sbgi_field *f, int32_t buf_id, uint32_t cascade, const void *src, size_t bytes) {
     if (!f || !src) return SBGI_E_INVALID;
-    if (buf_id != SBGI_BUF_IRR_STATE && buf_id != SBGI_BUF_DEPTH_STATE && buf_id != SBGI_BUF_META)
-        return fail(f, SBGI_E_INVALID, "only irr/depth/meta state is writable");
+    if (buf_id != SBGI_BUF_IRR_STATE && buf_id != SBGI_BUF_DEPTH_STATE && buf_id != SBGI_BUF_META &&
+        buf_id != SBGI_BUF_EMITTER_VIS)
+        return fail(f, SBGI_E_INVALID, "only irr/depth/meta/emitter-visibility state is writable");
     const size_t need = sbgi_buffer_bytes(f, buf_id, cascade);
     if (need == 0 || bytes != need) return fail(f, SBGI_E_INVALID, "buffer id/cascade/size mismatch");
     HIP_OK(f, hipSetDevice(f->desc.device));
     HIP_OK(f, hipStreamSynchronize(f->stream));
     const CascadeState &cs = f->casc[cascade];
-    void *dst = buf_id == SBGI_BUF_IRR_STATE ? (void *)cs.irr : buf_id == SBGI_BUF_DEPTH_STATE ? (void *)cs.depth
-                                                                                             : (void *)cs.meta;
+    void *dst = buf_id == SBGI_BUF_IRR_STATE     ? (void *)cs.irr
+                : buf_id == SBGI_BUF_DEPTH_STATE ? (void *)cs.depth
+                : buf_id == SBGI_BUF_META        ? (void *)cs.meta
+                                                 : (void *)cs.evis;
     HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return SBGI_OK;
 }
@@ -1250,6 +1289,7 @@ int32_t sbgi_destroy(sbgi_field *f) {
         dfree(cs.meta);
         dfree(cs.atlas);
         dfree(cs.datlas);
+        dfree(cs.evis);
     }
     dfree(f->rays);
     dfree(f->dirs);
@@ -1259,6 +1299,7 @@ int32_t sbgi_destroy(sbgi_field *f) {
     dfree(f->hit_ids);
     dfree(f->tables);
     dfree(f->counters);
+    dfree(f->d_light_grid);
     for (auto &e : f->ring)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : f->ev)
@@ -1397,6 +1438,8 @@ int32_t sbgi_diff_forward(sbgi_field *f, const sbgi_tick_params *p, const float 
     if (!f || !p || !albedo || !irr_out) return SBGI_E_INVALID;
     if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
     if (f->desc.depth_oct_res != 6) return fail(f, SBGI_E_INVALID, "the differentiable mode uses the 8x8 depth tile");
+    if (f->has_scene && f->view.emitter_count)
+        return fail(f, SBGI_E_INVALID, "the differentiable mode takes no emitter lights");
     HIP_OK(f, hipSetDevice(f->desc.device));
     int32_t rc = diff_prepare(f);
     if (rc != SBGI_OK) return rc;
@@ -1426,14 +1469,14 @@ int32_t sbgi_diff_forward(sbgi_field *f, const sbgi_tick_params *p, const float 
     HIP_OK(f, hipMemsetAsync(f->d_atlas1, 0, (size_t)cs.aw * cs.ah * 16, st));
     HIP_OK(f, launch_trace(s, c, q, f->dirs, cs.meta, cs.atlas, f->d_atlas1, f->tw, f->rays, nullptr, nullptr, nullptr,
                            st));
-    HIP_OK(f, launch_blend(c, q, f->rays, f->bt, SBGI_BLEND_FP32, cs.irr, cs.depth, st));
+    HIP_OK(f, launch_blend(c, q, f->rays, f->bt, SBGI_BLEND_FP32, cs.irr, cs.depth, EmitterInject{}, st));
     HIP_OK(f, launch_upload(c, q, cs.irr, cs.depth, cs.atlas, cs.datlas, f->d_atlas1, st));
     // update 2: cleared state again, bouncing off A1
     HIP_OK(f, hipMemsetAsync(cs.irr, 0, irr_bytes, st));
     HIP_OK(f, hipMemsetAsync(cs.depth, 0, dep_bytes, st));
     HIP_OK(f, launch_trace(s, c, q, f->dirs, cs.meta, cs.atlas, f->d_atlas1, f->tw, f->rays, nullptr, nullptr, nullptr,
                            st));
-    HIP_OK(f, launch_blend(c, q, f->rays, f->bt, SBGI_BLEND_FP32, cs.irr, cs.depth, st));
+    HIP_OK(f, launch_blend(c, q, f->rays, f->bt, SBGI_BLEND_FP32, cs.irr, cs.depth, EmitterInject{}, st));
     HIP_OK(f, hipEventRecord(f->dev[1], st));
     std::vector<float> irr((size_t)cs.total * 64 * 4);
     HIP_OK(f, hipMemcpyAsync(irr.data(), cs.irr, irr_bytes, hipMemcpyDeviceToHost, st));

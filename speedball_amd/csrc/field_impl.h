@@ -22,6 +22,7 @@ struct CascadeState {
     float max_dist = 0.f, cell_min = 0.f;
     float *irr = nullptr, *depth = nullptr, *meta = nullptr;
     ushort4 *atlas = nullptr, *datlas = nullptr;
+    float *evis = nullptr;  // retained emitter visibility [probe * SBGI_MAX_EMITTERS + e]
 };
 
 constexpr int kTicketRing = 16;
@@ -93,6 +94,10 @@ struct sbgi_field {
     std::vector<double> h_world;  // per TLAS slot, column-major 4x4
     sbgi_scene_view view{};
     bool has_scene = false;
+    // clustered NEE (SBGI_FIELD_CLUSTERED): cell lists over cascade 0's cells, rebuilt
+    // whenever the light records change
+    uint32_t *d_light_grid = nullptr;
+    uint32_t lg_cells[3] = {0, 0, 0};
     // solve scratch
     float4 *rays = nullptr, *dirs = nullptr;
     int2 *hit_ids = nullptr;
@@ -205,7 +210,23 @@ inline DevScene make_scene(const sbgi_field *f) {
     s.inst_ratio = f->tuning.inst_ratio;
     s.refill_min = f->tuning.refill_min;
     s.fetch_chunk = f->tuning.fetch_chunk;
+    s.light_grid = f->d_light_grid;
+    for (int a = 0; a < 3; a++) {
+        s.lg_min[a] = f->casc[0].d.grid_min[a];
+        s.lg_size[a] = f->casc[0].d.grid_size[a];
+        s.lg_cells[a] = f->lg_cells[a];
+    }
+    s.emitter_base = f->view.emitter_base;
+    s.emitter_count = f->view.emitter_count;
     return s;
+}
+
+inline size_t light_grid_cells(const sbgi_field *f) { return (size_t)f->lg_cells[0] * f->lg_cells[1] * f->lg_cells[2]; }
+
+// the blend's emitter injection for one cascade solve
+inline EmitterInject emitter_inject(const sbgi_field *f, const CascadeState &cs, const sbgi_tick_params &p) {
+    return EmitterInject{f->d_lights, cs.meta, cs.evis, p.classify_strength, f->view.emitter_base,
+                         f->has_scene ? f->view.emitter_count : 0u};
 }
 
 }  // namespace sbgi

@@ -22,6 +22,7 @@
 #include <hip/hip_fp16.h>
 
 #include "kernels.h"
+#include "shade.h"
 
 namespace {
 
@@ -30,7 +31,7 @@ constexpr uint32_t kIrrTexels = 36;
 __device__ __forceinline__ uint32_t kb_count(uint32_t rays) { return (rays + 31) / 32; }
 __device__ __forceinline__ uint32_t mt_count(uint32_t texels) { return (texels + 15) / 16; }
 
-__device__ __forceinline__ float3 oct_decode(float u, float v) {
+__device__ __forceinline__ float3 oct_decode_uv(float u, float v) {
     float fx = u * 2.f - 1.f, fy = v * 2.f - 1.f;
     float nz = 1.f - fabsf(fx) - fabsf(fy);
     float t = fmaxf(-nz, 0.f);
@@ -66,7 +67,7 @@ __global__ void __launch_bounds__(256) k_weights(DevCascade c, sbgi_tick_params 
         tx = drow % n;
         ty = drow / n;
     }
-    const float3 dir = oct_decode(((float)tx + 0.5f) / (float)n, ((float)ty + 0.5f) / (float)n);
+    const float3 dir = oct_decode_uv(((float)tx + 0.5f) / (float)n, ((float)ty + 0.5f) / (float)n);
     const float cpow = fmaxf(p.dbg_cosine_power, 1e-4f);
     const uint32_t R = c.rays, KB = kb_count(R);
     const bool want_d = parity || !is_irr;
@@ -212,10 +213,41 @@ __device__ __forceinline__ float ray_dist(float ht, float max_dist) {
 
 constexpr int kGroup = 12;
 
-// FP32 blend. blockIdx.y < 3: irradiance texel group; else depth texel group.
+// direction of interior irradiance texel tt of the 6x6 octahedral interior
+__device__ __forceinline__ float3 irr_texel_dir(uint32_t tt) {
+    return oct_decode_uv(((float)(tt % 6) + 0.5f) / 6.f, ((float)(tt / 6) + 0.5f) / 6.f);
+}
+
+// Emitter e's virtual rays added to one texel's sums after the ray loop, as the reference
+// does (js/gi_probes.js:2262-2292, 2407-2426): w = pow(max(dir_t . d, 0), cosinePower),
+// wk = w * kVirtual; acc += L wk, wsum += wk, and the luminance moments.
+__device__ __forceinline__ void inject_tap(const EmitterTap &e, float3 tdir, float cpow, float &r, float &g, float &b,
+                                           float &l, float &l2, float &ws) {
+#pragma clang fp contract(off)
+    const float cd = fmaxf(dot3(tdir, e.d), 0.f);
+    const float wk = powf(cd, cpow) * e.kv;
+    r += e.L.x * wk;
+    g += e.L.y * wk;
+    b += e.L.z * wk;
+    ws += wk;
+    const float lum = e.L.x * 0.2126f + e.L.y * 0.7152f + e.L.z * 0.0722f;
+    l += lum * wk;
+    l2 += (lum * lum) * wk;
+}
+
+__device__ __forceinline__ EmitterTap emitter_of(const DevCascade &c, const EmitterInject &em, uint32_t probe, uint32_t e,
+                                                 float3 ro) {
+    return emitter_tap(em.lights + (size_t)(em.base + e) * SBGI_LIGHT_STRIDE, ro, (float)c.rays,
+                       em.vis[(size_t)probe * SBGI_MAX_EMITTERS + e]);
+}
+
+// FP32 blend. blockIdx.y < 3: irradiance texel group; else depth texel group. EMIT: the
+// variant with emitter injection (it holds more registers, so it runs only for scenes
+// with emitters).
+template <bool EMIT>
 __global__ void __launch_bounds__(64) k_blend_fp32(DevCascade c, sbgi_tick_params p, const float4 *__restrict__ rays,
-                                                   BlendTables t, float *__restrict__ irr,
-                                                   float *__restrict__ depth) {
+                                                   BlendTables t, float *__restrict__ irr, float *__restrict__ depth,
+                                                   EmitterInject em) {
     const uint32_t slot = blockIdx.x * 64 + threadIdx.x;
     if (slot >= c.updated) return;
     const uint32_t probe = (c.offset + slot) % c.total;
@@ -243,11 +275,26 @@ __global__ void __launch_bounds__(64) k_blend_fp32(DevCascade c, sbgi_tick_param
                 a[j][4] += L2 * cw;
             }
         }
+        float ws[kGroup];
+        if (EMIT) {
+#pragma unroll
+            for (int j = 0; j < kGroup; j++) ws[j] = t.norm_irr[gy * kGroup + j];
+            const float3 ro = probe_origin(c, em.classify, em.meta, probe);
+            const float cpow = fmaxf(p.dbg_cosine_power, 1e-4f);
+            for (uint32_t e = 0; e < em.count; e++) {
+                const EmitterTap tap = emitter_of(c, em, probe, e, ro);
+                if (!tap.valid) continue;
+#pragma unroll
+                for (int j = 0; j < kGroup; j++)
+                    inject_tap(tap, irr_texel_dir(gy * kGroup + j), cpow, a[j][0], a[j][1], a[j][2], a[j][3], a[j][4],
+                               ws[j]);
+            }
+        }
 #pragma unroll
         for (int j = 0; j < kGroup; j++) {
             const uint32_t tt = gy * kGroup + j;
             const uint32_t local = (tt / 6 + 1) * 8 + (tt % 6 + 1);
-            irr_policy(p, irr, probe, local, a[j][0], a[j][1], a[j][2], a[j][3], a[j][4], t.norm_irr[tt]);
+            irr_policy(p, irr, probe, local, a[j][0], a[j][1], a[j][2], a[j][3], a[j][4], EMIT ? ws[j] : t.norm_irr[tt]);
         }
     } else {
         const uint32_t g = gy - kIrrTexels / kGroup;
@@ -313,11 +360,13 @@ __device__ __forceinline__ void split(float v, uint16_t &hi, uint16_t &lo) {
     }
 }
 
-// MFMA blend: one wave per 16 probes. MTD = depth M-tiles (3 for 36 texels, 13 for 196).
-template <uint32_t PREC, int MTD>
+// MFMA blend: one wave per 16 probes. MTD = depth M-tiles (3 for 36 texels, 13 for 196);
+// EMIT as in k_blend_fp32.
+template <uint32_t PREC, int MTD, bool EMIT>
 __global__ void __launch_bounds__(64) k_blend_mfma(DevCascade c, sbgi_tick_params p,
                                                    const float4 *__restrict__ rays, BlendTables t,
-                                                   float *__restrict__ irr, float *__restrict__ depth) {
+                                                   float *__restrict__ irr, float *__restrict__ depth,
+                                                   EmitterInject em) {
     constexpr int MTI = 3;
     constexpr bool SPLIT = PREC == SBGI_BLEND_BF16X3;
     const uint32_t lane = threadIdx.x;
@@ -402,6 +451,37 @@ __global__ void __launch_bounds__(64) k_blend_mfma(DevCascade c, sbgi_tick_param
     if (!valid) return;
     const uint32_t probe = (c.offset + slot) % c.total;
     // D layout: lane holds rows 4*(lane/16) + jj of column lane%16
+    float ws[MTI][4];
+    if (EMIT) {
+#pragma unroll
+        for (int m = 0; m < MTI; m++)
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const uint32_t tt = m * 16 + g * 4 + jj;
+                ws[m][jj] = tt < kIrrTexels ? t.norm_irr_q[tt] : 0.f;
+            }
+        const float3 ro = probe_origin(c, em.classify, em.meta, probe);
+        const float cpow = fmaxf(p.dbg_cosine_power, 1e-4f);
+        for (uint32_t e = 0; e < em.count; e++) {
+            const EmitterTap tap = emitter_of(c, em, probe, e, ro);
+            if (!tap.valid) continue;
+#pragma unroll
+            for (int m = 0; m < MTI; m++)
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const uint32_t tt = m * 16 + g * 4 + jj;
+                    if (tt < kIrrTexels) {
+                        float r = ai[m][0][jj], gg = ai[m][1][jj], b = ai[m][2][jj], l = ai[m][3][jj], l2 = ai[m][4][jj];
+                        inject_tap(tap, irr_texel_dir(tt), cpow, r, gg, b, l, l2, ws[m][jj]);
+                        ai[m][0][jj] = r;
+                        ai[m][1][jj] = gg;
+                        ai[m][2][jj] = b;
+                        ai[m][3][jj] = l;
+                        ai[m][4][jj] = l2;
+                    }
+                }
+        }
+    }
 #pragma unroll
     for (int m = 0; m < MTI; m++) {
 #pragma unroll
@@ -410,7 +490,7 @@ __global__ void __launch_bounds__(64) k_blend_mfma(DevCascade c, sbgi_tick_param
             if (tt < kIrrTexels) {
                 const uint32_t local = (tt / 6 + 1) * 8 + (tt % 6 + 1);
                 irr_policy(p, irr, probe, local, ai[m][0][jj], ai[m][1][jj], ai[m][2][jj], ai[m][3][jj],
-                           ai[m][4][jj], t.norm_irr_q[tt]);
+                           ai[m][4][jj], EMIT ? ws[m][jj] : t.norm_irr_q[tt]);
             }
         }
     }
@@ -458,22 +538,29 @@ hipError_t launch_weights(const DevCascade &c, const sbgi_tick_params &p, const 
 }
 
 hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const BlendTables &t,
-                        uint32_t precision, float *irr, float *depth, hipStream_t st) {
+                        uint32_t precision, float *irr, float *depth, const EmitterInject &em, hipStream_t st) {
     if (c.updated == 0) return hipSuccess;
     const uint32_t tdn = c.depth_oct == 6 ? kIrrTexels : c.depth_oct * c.depth_oct;
     if (precision == SBGI_BLEND_FP32) {
         dim3 grid((c.updated + 63) / 64, kIrrTexels / kGroup + (tdn + kGroup - 1) / kGroup);
-        hipLaunchKernelGGL(k_blend_fp32, grid, dim3(64), 0, st, c, p, rays, t, irr, depth);
+        if (em.count)
+            hipLaunchKernelGGL(k_blend_fp32<true>, grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em);
+        else
+            hipLaunchKernelGGL(k_blend_fp32<false>, grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em);
         return hipGetLastError();
     }
     dim3 grid((c.updated + 15) / 16);
     const bool parity = c.depth_oct == 6;
     if (!parity && c.depth_oct != 14) return hipErrorInvalidValue;
-#define SBGI_LAUNCH(PREC)                                                                                       \
-    if (parity)                                                                                                 \
-        hipLaunchKernelGGL((k_blend_mfma<PREC, 3>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth);          \
-    else                                                                                                        \
-        hipLaunchKernelGGL((k_blend_mfma<PREC, 13>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth);
+#define SBGI_LAUNCH(PREC)                                                                                        \
+    if (parity && em.count)                                                                                      \
+        hipLaunchKernelGGL((k_blend_mfma<PREC, 3, true>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em);  \
+    else if (parity)                                                                                             \
+        hipLaunchKernelGGL((k_blend_mfma<PREC, 3, false>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em); \
+    else if (em.count)                                                                                           \
+        hipLaunchKernelGGL((k_blend_mfma<PREC, 13, true>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em); \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_blend_mfma<PREC, 13, false>), grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em);
     switch (precision) {
         case SBGI_BLEND_BF16:
             SBGI_LAUNCH(SBGI_BLEND_BF16);

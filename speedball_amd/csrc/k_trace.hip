@@ -74,6 +74,35 @@ struct ShadowSrc {
     }
 };
 
+// Emitter visibility (emitterVisKernel, js/gi_probes.js:2216-2260): ray i = slot * E + e
+// tests the probe's view of a hashed point on emitter e; the store blends the 0/1 sample
+// into the retained visibility, vis = mix(sample, vis, 0.8). A probe inside the emitter's
+// bounding sphere walks nothing (its lane starts past the TLAS end) and samples 0.
+struct EmitterSrc {
+    DevCascade c;
+    float classify, seed, bias;
+    uint32_t ebase, ecount;
+    const float *meta;
+    float *vis;  // [probe * SBGI_MAX_EMITTERS + e]
+    __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
+        const uint32_t slot = i / ecount, e = i - slot * ecount;
+        const uint32_t probe = (c.offset + slot) % c.total;
+        const float3 ro = probe_origin(c, classify, meta, probe);
+        float3 rd = f3(0.f, 0.f, 1.f);
+        float tmax = 0.f;
+        const bool valid = emitter_ray(s.lights + (size_t)(ebase + e) * SBGI_LIGHT_STRIDE, ro, probe, e, seed, bias,
+                                       rd, tmax);
+        lane_begin<true>(s, r, ro, rd, tmax);
+        if (!valid) r.cur = r.end;
+        r.tag = (probe * SBGI_MAX_EMITTERS + e) | (valid ? 0u : 0x80000000u);
+    }
+    __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
+        const float sample = (!(r.tag >> 31) && !r.blocked) ? 1.f : 0.f;
+        float *v = vis + (r.tag & 0x7FFFFFFFu);
+        *v = sample * (1.f - SBGI_EMITTER_RETENTION) + *v * SBGI_EMITTER_RETENTION;
+    }
+};
+
 // Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit
 // within tmax, 2 closest hit reported as blocked when before tmax.
 // out[i] = {t, tri, inst, blocked}.
@@ -179,18 +208,23 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
     const bool hit = active && tri >= 0;
     const bool wave_hit = __ballot(hit) != 0ull;
     HitFrame h;
+    uint64_t members = 0ull;
     if (hit) {
         float3 ro, rd;
         make_ray(c, p.classify_strength, dirs, meta, gid, ro, rd);
         h = hit_frame(s, c, p, ro, rd, hr.x, (uint32_t)tri, (uint32_t)inst);
+        members = light_members(s, h.hit);
     }
-    // pass 1: which lights pass, and how many records this wave appends
+    // pass 1: which lights pass, and how many records this wave appends. A light outside
+    // every lane's cell list is skipped by the whole wave.
     uint64_t pass_bits = 0;
     uint32_t count = 0;
     if (wave_hit) {
         for (uint32_t li = 0; li < s.light_count; li++) {
+            const bool member = (members >> li) & 1ull;
+            if (!__ballot(member)) continue;
             bool pass = false;
-            if (hit) pass = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p, light_scale(s, li)).pass;
+            if (member) pass = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p, light_scale(s, li)).pass;
             if (pass) pass_bits |= 1ull << li;
             count += (uint32_t)__popcll(__ballot(pass));
         }
@@ -282,12 +316,12 @@ __global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sb
     if (hit_ids) hit_ids[gid] = make_int2(tri, inst);
 }
 
-// Clears the visibility masks, the queue length and the fetch counters of the primary and
-// shadow walks (the solve counters of kernels.h).
+// Clears the visibility masks, the queue length and the fetch counters of every walk of
+// the solve (the solve counters of kernels.h).
 __global__ void k_clear_u64(unsigned long long *__restrict__ p, uint32_t n, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
-    if (i < 1u + 2u * kFetchHeads) counters[(size_t)i * kHeadStride] = 0u;
+    if (i < 1u + 4u * kFetchHeads) counters[(size_t)i * kHeadStride] = 0u;
 }
 
 // Direction table: normalize(rayDir(k, jitter)) (js/gi_probes.js:1692-1704, 1984).
@@ -434,7 +468,7 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
     if (n == 0) return hipSuccess;
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
     uint32_t *ctr = w.queue.count;
-    const uint32_t nclear = std::max(n, 1u + 2u * kFetchHeads);
+    const uint32_t nclear = std::max(n, 1u + 4u * kFetchHeads);
     hipLaunchKernelGGL(k_clear_u64, dim3((nclear + 255) / 256), dim3(256), 0, st, w.vis, n, ctr);
     const size_t lds = trace_lds_bytes(s.tlas_count, s.inst_count);
     hipError_t e = mark(0);
@@ -451,5 +485,15 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
                        prev_atlas, prev_atlas_f32, w.hits, w.vis, rays, hit_ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float *meta,
+                              float *vis, uint32_t *heads, hipStream_t st) {
+    const uint32_t n = c.updated * s.emitter_count;
+    if (!n) return hipSuccess;
+    const float bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
+    const EmitterSrc src{c, p.classify_strength, p.emitter_vis_seed, bias, s.emitter_base, s.emitter_count, meta, vis};
+    launch_walk<WALK_ANY>(s, src, nullptr, n, heads, nullptr, trace_lds_bytes(s.tlas_count, s.inst_count), st);
     return hipGetLastError();
 }

@@ -10,10 +10,11 @@
 // Dense queue of shadow rays: rec[2i] = {origin.xyz, tmax}, rec[2i+1] = {dir.xyz,
 // bits((ray << 6) | light)}. count points at the solve counters (kTraceCounterBytes):
 // word 0 is the number of records appended this solve; the fetch counters of the primary
-// (0), shadow (1) and query (2) walks start at word trace_heads(walk), one per 128 bytes.
-constexpr size_t kTraceCounterBytes = 32 * 1024;
+// (0), shadow (1), query (2) and emitter-visibility (3) walks start at word
+// trace_heads(walk), one per 128 bytes.
+constexpr size_t kTraceCounterBytes = 48 * 1024;
 constexpr size_t trace_heads(uint32_t walk) { return (1u + (size_t)walk * kFetchHeads) * kHeadStride; }
-static_assert((trace_heads(3)) * sizeof(uint32_t) <= kTraceCounterBytes, "solve counters overflow");
+static_assert((trace_heads(4)) * sizeof(uint32_t) <= kTraceCounterBytes, "solve counters overflow");
 struct ShadowQueue {
     float4 *rec;
     uint32_t *count;
@@ -66,6 +67,16 @@ hipError_t launch_emit(const DevScene &s, const DevCascade &c, const sbgi_tick_p
 hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                           const float *meta, const ushort4 *prev_atlas, const float4 *hits,
                           const unsigned long long *vis, float4 *rays, hipStream_t st);
+// emitterVisKernel: one any-hit ray per (updated probe, emitter of s); vis[probe * 16 + e]
+// <- mix(sample, vis, 0.8). heads: the walk's kFetchHeads fetch counters, zero on entry.
+hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float *meta,
+                              float *vis, uint32_t *heads, hipStream_t st);
+
+// k_lights.hip
+// lightGridKernel: the cell lists {count, ids[16]} of s's lights over cells[0] x cells[1]
+// x cells[2] cells spanning gmin + gsize (count 17 = overflow)
+hipError_t launch_light_grid(const float *lights, uint32_t light_count, const float *gmin, const float *gsize,
+                             const uint32_t *cells, uint32_t *grid, hipStream_t st);
 
 // k_blend.hip
 struct BlendTables {
@@ -81,8 +92,18 @@ size_t blend_tables_bytes(uint32_t rays, uint32_t depth_texels);
 void blend_tables_carve(void *base, uint32_t rays, uint32_t depth_texels, BlendTables &t);
 hipError_t launch_weights(const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs, const BlendTables &t,
                           uint32_t precision, hipStream_t st);
+// Emitters injected into the irradiance gather as virtual rays (injectEmitterVirtualRays):
+// lights[(base + e) * 17] for e < count, their retained visibility vis[probe * 16 + e], and
+// the probe trace origins (meta relocation x classify). count 0: no injection.
+struct EmitterInject {
+    const float *lights;
+    const float *meta;
+    const float *vis;
+    float classify;
+    uint32_t base, count;
+};
 hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const BlendTables &t,
-                        uint32_t precision, float *irr, float *depth, hipStream_t st);
+                        uint32_t precision, float *irr, float *depth, const EmitterInject &em, hipStream_t st);
 
 // k_upload.hip
 // atlas_f32 (optional): also store the irradiance tiles unrounded (differentiable mode)

@@ -46,6 +46,13 @@ struct DevScene {
     // waiting in it are >= 1/tri_ratio (1/inst_ratio) of those still in NODE; retired
     // lanes are refilled in batches of >= refill_min; fetch_chunk rays per fetch atomic
     uint32_t tri_ratio, inst_ratio, refill_min, fetch_chunk;
+    // clustered NEE (null otherwise): per-cell light lists {count, ids[16]} over the cells
+    // of cascade 0's probe grid, lg_cells = res - 1 per axis, spanning lg_min + lg_size
+    const uint32_t *__restrict__ light_grid;
+    float lg_min[3], lg_size[3];
+    uint32_t lg_cells[3];
+    // emitter records (type 3) injected as virtual rays: lights[emitter_base + e]
+    uint32_t emitter_base, emitter_count;
 };
 
 // albedo entry of (instance slot, pool triangle) in the differentiable mode

@@ -358,9 +358,11 @@ struct sbgi_scene_builder {
         bool active;
     };
     std::vector<Inst> insts;
-    std::vector<float> lights;
+    std::vector<float> lights;  // records in the order they were added
     // finalized arrays
     bool final = false;
+    std::vector<float> light_arena;  // analytic records, then emitters
+    uint32_t emitter_base = 0, emitter_count = 0;
     std::vector<uint32_t> nodes, tri_index, tri_material, order, slot_of;
     std::vector<float> vertex_data, mats, inst_bounds;
     uint32_t inst_base = 0, tlas_base = 0, tlas_count = 0;
@@ -593,8 +595,27 @@ int32_t sbgi_sb_finalize(sbgi_scene_builder *b, sbgi_scene_view *view) {
     view->inst_count = (uint32_t)ni;
     view->tlas_base = b->tlas_base;
     view->tlas_node_count = b->tlas_count;
-    view->lights = b->lights.empty() ? nullptr : b->lights.data();
-    view->light_count = (uint32_t)(b->lights.size() / 17);
+    // The light arena of selectGiLightRecords (js/gi_probes.js:4273-4289, the order-keeping
+    // selection of the non-clustered arena): analytic lights (type < 2.5) in the order added,
+    // up to 64 minus the emitters, then the emitters (type 3), at most 16; other records
+    // are dropped.
+    std::vector<const float *> analytic, emitters;
+    for (size_t i = 0; i + 17 <= b->lights.size(); i += 17) {
+        const float *r = b->lights.data() + i;
+        if (std::fabs(r[0] - 3.f) < 0.5f) emitters.push_back(r);
+        else if (r[0] < 2.5f) analytic.push_back(r);
+    }
+    b->emitter_count = (uint32_t)std::min<size_t>(emitters.size(), SBGI_MAX_EMITTERS);
+    const size_t n_analytic = std::min<size_t>(analytic.size(), SBGI_MAX_LIGHTS - b->emitter_count);
+    b->light_arena.clear();
+    for (size_t i = 0; i < n_analytic; i++) b->light_arena.insert(b->light_arena.end(), analytic[i], analytic[i] + 17);
+    for (size_t i = 0; i < b->emitter_count; i++)
+        b->light_arena.insert(b->light_arena.end(), emitters[i], emitters[i] + 17);
+    b->emitter_base = (uint32_t)n_analytic;
+    view->lights = b->light_arena.empty() ? nullptr : b->light_arena.data();
+    view->light_count = (uint32_t)(b->light_arena.size() / 17);
+    view->emitter_base = b->emitter_base;
+    view->emitter_count = b->emitter_count;
     view->pad = 0;
     return SBGI_OK;
 }

@@ -3,7 +3,8 @@
 // previous-atlas bounce lookup (js/gi_probes.js:1714-1726, 1876-1885, 1941-2188).
 //
 // Include it after `#pragma clang fp contract(off)`: the adjoint recomputes the forward's
-// values and must round them the same way.
+// values and must round them the same way. (The probe-origin, light-list and emitter
+// helpers turn contraction off themselves: the blend, which contracts, calls them too.)
 #pragma once
 #include <hip/hip_fp16.h>
 
@@ -99,21 +100,115 @@ __device__ float3 sample_atlas(const DevCascade &c, const ATLAS *__restrict__ at
     return acc * (1.0f / fmaxf(wsum, 1e-4f));
 }
 
-// ray origin and direction of ray gid (probeTraceOrigin js/gi_probes.js:1714-1726, 1876-1885)
-__device__ __forceinline__ void make_ray(const DevCascade &c, float classify_strength, const float4 *__restrict__ dirs,
-                                         const float *__restrict__ meta, uint32_t gid, float3 &ro, float3 &rd) {
-    const uint32_t k = gid / c.updated;
-    const uint32_t slot = gid - k * c.updated;
-    const uint32_t probe = (c.offset + slot) % c.total;
+// probeTraceOrigin of a probe: its grid position plus the relocation offset scaled by
+// classifyStrength (js/gi_probes.js:1714-1726, 1876-1885)
+__device__ __forceinline__ float3 probe_origin(const DevCascade &c, float classify_strength,
+                                               const float *__restrict__ meta, uint32_t probe) {
+#pragma clang fp contract(off)
     const uint32_t ix = probe % c.res[0], iy = (probe / c.res[0]) % c.res[1], iz = probe / (c.res[0] * c.res[1]);
-    ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
-            c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
-            c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
+    float3 ro = f3(c.gmin[0] + ((float)ix / fmaxf(1.f, (float)c.res[0] - 1.f)) * c.gsize[0],
+                   c.gmin[1] + ((float)iy / fmaxf(1.f, (float)c.res[1] - 1.f)) * c.gsize[1],
+                   c.gmin[2] + ((float)iz / fmaxf(1.f, (float)c.res[2] - 1.f)) * c.gsize[2]);
     if (classify_strength != 0.f) {
         const float *m = meta + (size_t)probe * 4;
         ro = ro + f3(m[1], m[2], m[3]) * classify_strength;
     }
+    return ro;
+}
+
+// ray origin and direction of ray gid (index k * updated + slot)
+__device__ __forceinline__ void make_ray(const DevCascade &c, float classify_strength, const float4 *__restrict__ dirs,
+                                         const float *__restrict__ meta, uint32_t gid, float3 &ro, float3 &rd) {
+    const uint32_t k = gid / c.updated;
+    const uint32_t slot = gid - k * c.updated;
+    ro = probe_origin(c, classify_strength, meta, (c.offset + slot) % c.total);
     rd = xyz(dirs[k]);
+}
+
+// Lights the NEE loop visits for a hit: every light, or with clustered lighting the list
+// of the cascade-0 cell that holds the hit (js/gi_probes.js:2112-2127): all lights when
+// the hit lies outside the grid or the cell's list overflowed. Bit li = light li.
+__device__ __forceinline__ uint64_t light_members(const DevScene &s, float3 hit) {
+#pragma clang fp contract(off)
+    const uint64_t all = s.light_count >= 64 ? ~0ull : ((1ull << s.light_count) - 1ull);
+    if (!s.light_grid) return all;
+    const float pp[3] = {hit.x, hit.y, hit.z};
+    float cf[3];
+    bool outside = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float cells = (float)(s.lg_cells[a] + 1u) - 1.f;
+        const float size = s.lg_size[a] / cells;
+        cf[a] = floorf((pp[a] - s.lg_min[a]) / size);
+        outside = outside || cf[a] < 0.f || cf[a] >= cells;
+        cf[a] = fminf(fmaxf(cf[a], 0.f), cells - 1.f);
+    }
+    const uint32_t cell = (uint32_t)cf[2] * (s.lg_cells[0] * s.lg_cells[1]) + (uint32_t)cf[1] * s.lg_cells[0] +
+                          (uint32_t)cf[0];
+    const uint32_t *list = s.light_grid + (size_t)cell * (SBGI_LIGHTS_PER_CELL + 1);
+    const uint32_t n = list[0];
+    if (outside || n > SBGI_LIGHTS_PER_CELL) return all;
+    uint64_t m = 0ull;
+    for (uint32_t i = 0; i < n; i++) m |= 1ull << list[1 + i];
+    return m;
+}
+
+// The per-probe part of an emitter's virtual rays (injectEmitterVirtualRays,
+// js/gi_probes.js:2262-2292): direction d from the probe's trace origin to the emitter
+// centre, the expected ray count kVirtual = min(R, R * Omega / 4 pi) with the Cauchy solid
+// angle Omega = min(2 pi, (A/4) / dist^2), and the radiance times the retained visibility.
+// valid is false when the probe lies inside the emitter's bounding sphere.
+struct EmitterTap {
+    float3 d, L;
+    float kv;
+    bool valid;
+};
+
+__device__ __forceinline__ EmitterTap emitter_tap(const float *__restrict__ L, float3 ro, float rays, float vis) {
+#pragma clang fp contract(off)
+    EmitterTap t;
+    const float3 axis = f3(L[1], L[2], L[3]) - ro;
+    const float dist = sqrtf(dot3(axis, axis));
+    const float radius = fmaxf(L[14], 0.f);
+    t.valid = dist > fmaxf(radius, 1e-3f);
+    t.d = f3(axis.x / dist, axis.y / dist, axis.z / dist);
+    const float proj = fmaxf(L[15], 0.f);
+    const float omega = fminf(6.2831853071795862f, proj / (dist * dist));
+    t.kv = fminf(rays, (rays * omega) / 12.566370614359172f);
+    t.L = f3(L[7], L[8], L[9]) * vis;
+    return t;
+}
+
+// sin / cos rounded once from double: the correctly rounded f32 values, so that the
+// emitter sample hash (which multiplies sin by ~4e4) is reproducible
+__device__ __forceinline__ float sin_cr(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cos_cr(float x) { return (float)cos((double)x); }
+
+// The visibility ray of emitterVisKernel (js/gi_probes.js:2216-2260): from the probe's
+// trace origin toward a hashed point of the emitter's disc facing the probe, ending
+// `bias` short of the emitter's bounding sphere. False when the probe lies inside it.
+__device__ __forceinline__ bool emitter_ray(const float *__restrict__ L, float3 ro, uint32_t probe, uint32_t e,
+                                            float seed, float bias, float3 &rd, float &tmax) {
+#pragma clang fp contract(off)
+    const float3 center = f3(L[1], L[2], L[3]);
+    const float radius = fmaxf(L[14], 0.f);
+    const float3 axis = center - ro;
+    const float dist = sqrtf(dot3(axis, axis));
+    if (!(dist > fmaxf(radius, 1e-3f))) return false;
+    const float3 d = f3(axis.x / dist, axis.y / dist, axis.z / dist);
+    const float3 up = fabsf(d.y) < 0.999f ? f3(0.f, 1.f, 0.f) : f3(1.f, 0.f, 0.f);
+    const float3 tg = normalize3(cross3(up, d));
+    const float3 bt = cross3(d, tg);
+    const float es = ((float)probe * 12.9898f + (float)e * 78.233f) + seed * 37.719f;
+    const float h0 = sin_cr(es) * 43758.5453f;
+    const float h1 = sin_cr(es + 19.19f) * 24634.6345f;
+    const float u = h0 - floorf(h0), v = h1 - floorf(h1);
+    const float rad = sqrtf(u) * radius;
+    const float ang = v * (SBGI_PI * 2.f);
+    const float3 target = center + (tg * (cos_cr(ang) * rad) + bt * (sin_cr(ang) * rad));
+    rd = normalize3(target - ro);
+    tmax = fmaxf((dist - radius) - bias, 1e-4f);
+    return true;
 }
 
 // Geometric shading frame of a hit (js/gi_probes.js:1993-2054).

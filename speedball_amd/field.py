@@ -31,7 +31,7 @@ def clamp_rays(n) -> int:
 class ProbeField:
     def __init__(self, res, grid_min, grid_size, rays: int = 64, depth_oct: int = 6,
                  precision: int = N.BLEND_FP32, keep_hit_ids: bool = False, device: int = 0, cascades=None,
-                 traversal: int = N.TRAV_MT_COMPAT):
+                 traversal: int = N.TRAV_MT_COMPAT, clustered: bool = False):
         self.L = N.lib()
         d = N.FieldDesc()
         d.device = device
@@ -41,7 +41,7 @@ class ProbeField:
         d.depth_oct_res = depth_oct
         d.blend_precision = precision
         d.traversal = traversal
-        d.flags = N.FIELD_KEEP_HIT_IDS if keep_hit_ids else 0
+        d.flags = (N.FIELD_KEEP_HIT_IDS if keep_hit_ids else 0) | (N.FIELD_CLUSTERED if clustered else 0)
         for i, (gm, gs, rs) in enumerate(casc):
             for a in range(3):
                 d.cascades[i].grid_min[a] = gm[a]
@@ -129,6 +129,14 @@ class ProbeField:
         res = self.cascades[cascade][2]
         tile = 8 if (not depth or self.depth_oct == 6) else self.depth_oct + 2
         return raw.view(np.float16).astype(np.float32).reshape(res[1] * res[2] * tile, res[0] * tile, 4)
+
+    def read_emitter_vis(self, cascade=0):
+        """Retained emitter visibility [probe, 16]."""
+        return self.read(N.BUF_EMITTER_VIS, np.float32, cascade).reshape(-1, N.MAX_EMITTERS)
+
+    def read_light_grid(self):
+        """Clustered light lists [cell, 17] of cascade 0."""
+        return self.read(N.BUF_LIGHT_GRID, np.uint32).reshape(-1, N.LIGHTS_PER_CELL + 1)
 
     def read_rays(self):
         return self.read(N.BUF_RAY_DATA, np.float32).reshape(-1, self.rays, 4)

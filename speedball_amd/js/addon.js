@@ -28,7 +28,7 @@ const STATUS = Object.freeze({
 const BUF = Object.freeze({
     BVH_NODES: 0, TRI_INDEX: 1, VERTEX_DATA: 2, TRI_MATERIAL: 3, MATERIALS: 4, LIGHTS: 5,
     IRR_STATE: 10, DEPTH_STATE: 11, META: 12, IRR_ATLAS: 13, DEPTH_ATLAS: 14, RAY_DATA: 15, HIT_IDS: 16,
-    NODES_REFIT: 17, TRI_PACKED: 18, INST_BOUNDS: 19, TLAS_SAH: 20,
+    NODES_REFIT: 17, TRI_PACKED: 18, INST_BOUNDS: 19, TLAS_SAH: 20, EMITTER_VIS: 21, LIGHT_GRID: 22,
 });
 const BLEND = Object.freeze({ fp32: 0, bf16: 1, bf16x3: 2, fp16: 3 });
 const TRAVERSAL = Object.freeze({ compat: 0, ordered: 1 });

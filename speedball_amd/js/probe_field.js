@@ -17,6 +17,7 @@ const TARGET_PROBES_LONG_AXIS = 12;
 const MAX_PROBES_PER_AXIS = 32;
 const RAYS_PER_TICK = 98304;
 const RAYS_PER_TICK_MIN = 2048;
+const FIELD_CLUSTERED = 2;  // SBGI_FIELD_CLUSTERED
 const MAX_PROBES_PER_TICK = 2048;
 const JITTER_HYSTERESIS_DEFAULTS = Object.freeze({ gated: 0.60, montecarlo: 0.90 });
 const GOLDEN_RATIO_FRACT = 0.6180339887498949;
@@ -75,6 +76,8 @@ function createProbeField(opts) {
     let manualVolumes = null;
     const blendPrecision = BLEND[opts.precision || 'bf16x3'];
     const traversal = TRAVERSAL[opts.traversal || 'ordered'];
+    // clusteredLighting (js/gi_probes.js:1097): per-cell light lists for the NEE loop
+    const clustered = opts.clusteredLighting === true;
     const depthOct = opts.depthOct === 14 ? 14 : OCT_RES;
     const device = opts.device | 0;
 
@@ -83,6 +86,7 @@ function createProbeField(opts) {
         filterStrength: 1, filterSmooth: 0.5, radianceClamp: 8, depthSharpness: 7, skyIntensity: 1,
         skySH: new Float32Array(27), nirGate: 0, nirGain: 1, classifyStrength: 0,
         tempChangeSigma1: 2.5, tempChangeHDrop: 0.30, tempClampSigma: 6, frameJitter: 0, hysteresisExponent: 1,
+        emitterVisSeed: 0,
     };
     // renderer-side knobs (state only)
     const R = {
@@ -99,6 +103,7 @@ function createProbeField(opts) {
     let buildRequestedFull = true;
     let inFlight = null;
     let frameCounter = 0;
+    let emitterSeedCounter = 0;
     let solveIndex = 0;
     let disposed = false;
     let lastStats = null;
@@ -155,6 +160,7 @@ function createProbeField(opts) {
             if (field) addon.fieldDestroy(field.h);
             const h = addon.fieldCreate({
                 device: device, rays: raysPerProbe, depthOct: depthOct, precision: blendPrecision, traversal: traversal,
+                flags: clustered ? FIELD_CLUSTERED : 0,
                 cascades: grids.map((g) => ({
                     gridMin: Float32Array.from(g.gridMin), gridSize: Float32Array.from(g.gridSize),
                     res: Uint32Array.from(g.res),
@@ -239,7 +245,7 @@ function createProbeField(opts) {
             depthSharpness: U.depthSharpness, radianceClamp: U.radianceClamp, classifyStrength: U.classifyStrength,
             filterStrength: U.filterStrength, filterSmooth: U.filterSmooth, tempChangeSigma1: U.tempChangeSigma1,
             tempChangeHDrop: U.tempChangeHDrop, tempClampSigma: U.tempClampSigma, nirGate: U.nirGate, nirGain: U.nirGain,
-            skyIntensity: U.skyIntensity, skySH: U.skySH,
+            skyIntensity: U.skyIntensity, skySH: U.skySH, emitterVisSeed: U.emitterVisSeed,
         };
         for (const k of Object.keys(debugKnobs)) p[k] = debugKnobs[k];
         return p;
@@ -268,6 +274,12 @@ function createProbeField(opts) {
             if (jitterMode === 'montecarlo') U.frameJitter = fract(frameCounter * GOLDEN_RATIO_FRACT);
             frameCounter++;
             const ci = casc.length > 1 ? solveIndex++ % casc.length : 0;
+            // the emitter sample target moves with the Monte Carlo epoch of cascade 0
+            // (js/gi_probes.js:3767-3777)
+            if (jitterMode === 'montecarlo' && ci === 0) {
+                emitterSeedCounter = (emitterSeedCounter + 1) >>> 0;
+                U.emitterVisSeed = fract(emitterSeedCounter * GOLDEN_RATIO_FRACT);
+            }
             const C = casc[ci];
             const updated = Math.min(updatedCap(), C.probeTotal);
             const ticket = addon.fieldTick(field.h, tickParams(C, ci, updated));

@@ -12,10 +12,12 @@
 //      materials?: per-triangle indices into `materials` }], materials?: [...],
 //      instances: [{ mesh, matrix (16, column-major like Matrix4.elements), active? }],
 //      lights: [{ type: 'point'|'directional'|'spot', position, target?, color, intensity,
-//      distance?, decay?, angle?, penumbra?, emitterClass? }] };
+//      distance?, decay?, angle?, penumbra?, emitterClass? }],
+//      emitters?: [{ center, radius, area, radiance } or 17-float type-3 records] };
 //   3. three-like scene graph: an object with traverse(cb) visiting objects that carry
 //      isMesh / geometry.attributes.position / geometry.index / matrixWorld.elements /
-//      material, and isPointLight / isDirectionalLight / isSpotLight lights.
+//      material, and isPointLight / isDirectionalLight / isSpotLight lights; meshes with
+//      userData.giEmitter also become emitter records (collectEmitterRecords).
 
 const MAT_STRIDE = 28;
 const LIGHT_STRIDE = 17;
@@ -89,6 +91,58 @@ function lightRecord(l) {
     return r;
 }
 
+// collectEmitterRecords (js/spectral_scene.js:815-875): an emissive mesh as a type-3
+// record {centre, radius (bounding sphere), surface radiance, range, A/4}. Accepts
+// { center, radius, area, radiance } directly, or a three-like mesh whose geometry gives
+// the area and the bounding sphere (box centre, farthest vertex) in world space.
+function emitterRecord(e) {
+    let center = e.center, radius = e.radius, area = e.area, rad = e.radiance;
+    if (center == null && e.geometry && e.geometry.attributes && e.geometry.attributes.position) {
+        const pos = e.geometry.attributes.position.array;
+        const vc = pos.length / 3;
+        const idx = e.geometry.index ? e.geometry.index.array : null;
+        const w = worldMatrix(e);
+        const tw = (i) => [w[0] * pos[3 * i] + w[4] * pos[3 * i + 1] + w[8] * pos[3 * i + 2] + w[12],
+                           w[1] * pos[3 * i] + w[5] * pos[3 * i + 1] + w[9] * pos[3 * i + 2] + w[13],
+                           w[2] * pos[3 * i] + w[6] * pos[3 * i + 1] + w[10] * pos[3 * i + 2] + w[14]];
+        const n = idx ? idx.length : vc;
+        area = 0;
+        for (let i = 0; i + 2 < n; i += 3) {
+            const a = tw(idx ? idx[i] : i), b = tw(idx ? idx[i + 1] : i + 1), c = tw(idx ? idx[i + 2] : i + 2);
+            const u = [b[0] - a[0], b[1] - a[1], b[2] - a[2]], v = [c[0] - a[0], c[1] - a[1], c[2] - a[2]];
+            area += 0.5 * Math.hypot(u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]);
+        }
+        const lo = [Infinity, Infinity, Infinity], hi = [-Infinity, -Infinity, -Infinity];
+        for (let i = 0; i < vc; i++)
+            for (let k = 0; k < 3; k++) { lo[k] = Math.min(lo[k], pos[3 * i + k]); hi[k] = Math.max(hi[k], pos[3 * i + k]); }
+        const c = [(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2];
+        let r2 = 0;
+        for (let i = 0; i < vc; i++)
+            r2 = Math.max(r2, (pos[3 * i] - c[0]) ** 2 + (pos[3 * i + 1] - c[1]) ** 2 + (pos[3 * i + 2] - c[2]) ** 2);
+        const sc = Math.max(Math.hypot(w[0], w[1], w[2]), Math.hypot(w[4], w[5], w[6]), Math.hypot(w[8], w[9], w[10]));
+        center = [w[0] * c[0] + w[4] * c[1] + w[8] * c[2] + w[12], w[1] * c[0] + w[5] * c[1] + w[9] * c[2] + w[13],
+                  w[2] * c[0] + w[6] * c[1] + w[10] * c[2] + w[14]];
+        radius = Math.sqrt(r2) * sc;
+        if (rad == null && e.material) {
+            const em = colorOf(e.material.emissive, [0, 0, 0]);
+            const ei = e.material.emissiveIntensity == null ? 1 : e.material.emissiveIntensity;
+            const scale = e.userData && Number.isFinite(e.userData.giEmitterScale) ? e.userData.giEmitterScale : 1;
+            rad = [em[0] * ei * scale, em[1] * ei * scale, em[2] * ei * scale];
+        }
+    }
+    rad = colorOf(rad, [1, 1, 1]);
+    const r = new Float32Array(LIGHT_STRIDE);
+    const proj = area / 4;
+    r[0] = 3;
+    r[1] = center[0]; r[2] = center[1]; r[3] = center[2];
+    r[4] = 0; r[5] = 0; r[6] = -1;
+    r[7] = rad[0]; r[8] = rad[1]; r[9] = rad[2];
+    r[10] = Math.sqrt(Math.max(0, rad[0], rad[1], rad[2]) * proj / 0.005);
+    r[11] = 2; r[12] = -1; r[13] = -1;
+    r[14] = radius; r[15] = proj; r[16] = 0;
+    return r;
+}
+
 function worldPosition(o) {
     if (Array.isArray(o.position) || ArrayBuffer.isView(o.position)) return [o.position[0], o.position[1], o.position[2]];
     if (o.matrixWorld && o.matrixWorld.elements) {
@@ -113,7 +167,7 @@ function isPrebuilt(s) {
 }
 
 function describeThreeScene(scene) {
-    const desc = { meshes: [], materials: [], instances: [], lights: [], objects: [] };
+    const desc = { meshes: [], materials: [], instances: [], lights: [], objects: [], emitters: [] };
     const meshOfGeometry = new Map();
     const matIndex = new Map();
     scene.traverse(function (o) {
@@ -142,6 +196,7 @@ function describeThreeScene(scene) {
             }
             desc.instances.push({ mesh: mesh, matrix: worldMatrix(o), active: true, object: o });
             desc.objects.push(o);
+            if (o.userData && o.userData.giEmitter === true) desc.emitters.push(o);
         } else if (o.isPointLight || o.isDirectionalLight || o.isSpotLight) {
             desc.lights.push(o);
         }
@@ -192,6 +247,12 @@ function buildScene(addon, input) {
     }
     const lights = desc.lights || [];
     for (let i = 0; i < lights.length; i++) addon.sbAddLight(b, lightRecord(lights[i]));
+    // emitters follow the analytic lights in the arena (the builder orders them)
+    const emitters = desc.emitters || [];
+    for (let i = 0; i < emitters.length; i++) {
+        const e = emitters[i];
+        addon.sbAddLight(b, e instanceof Float32Array && e.length === LIGHT_STRIDE ? e : emitterRecord(e));
+    }
     const arrays = addon.sbFinalize(b);
     // builder instance id -> TLAS slot
     const order = arrays.tlasOrder;
@@ -213,4 +274,6 @@ function sceneBounds(built) {
     return { min: lo, max: hi };
 }
 
-module.exports = { buildScene, uberRecord, lightRecord, worldMatrix, sceneBounds, MAT_STRIDE, LIGHT_STRIDE };
+module.exports = {
+    buildScene, uberRecord, lightRecord, emitterRecord, worldMatrix, sceneBounds, MAT_STRIDE, LIGHT_STRIDE,
+};

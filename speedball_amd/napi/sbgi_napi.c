@@ -173,6 +173,8 @@ static int read_scene(napi_env env, napi_value o, sbgi_scene_view *v) {
     v->inst_count = (uint32_t)get_num(env, o, "instCount", 0);
     v->tlas_base = (uint32_t)get_num(env, o, "tlasBase", 0);
     v->tlas_node_count = (uint32_t)get_num(env, o, "tlasNodeCount", 0);
+    v->emitter_base = (uint32_t)get_num(env, o, "emitterBase", 0);
+    v->emitter_count = (uint32_t)get_num(env, o, "emitterCount", 0);
     return v->bvh_nodes && v->tri_index && v->vertex_data && v->tri_material && v->materials;
 }
 
@@ -225,6 +227,7 @@ static void read_params(napi_env env, napi_value o, sbgi_tick_params *p) {
     F(dbg_filter_k, "debugFilterKScale");
     F(dbg_filter_rel, "debugFilterRelScale");
     F(dbg_filter_eps, "debugFilterEpsScale");
+    F(emitter_vis_seed, "emitterVisSeed");
 #undef F
     size_t l;
     napi_typedarray_type t;
@@ -486,6 +489,8 @@ static napi_value js_sb_finalize(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, o, "instCount", make_u32(env, v.inst_count));
     napi_set_named_property(env, o, "tlasBase", make_u32(env, v.tlas_base));
     napi_set_named_property(env, o, "tlasNodeCount", make_u32(env, v.tlas_node_count));
+    napi_set_named_property(env, o, "emitterBase", make_u32(env, v.emitter_base));
+    napi_set_named_property(env, o, "emitterCount", make_u32(env, v.emitter_count));
     uint32_t *order = malloc(sizeof(uint32_t) * (v.inst_count ? v.inst_count : 1));
     if (order && sbgi_sb_tlas_order(b, order, v.inst_count) == SBGI_OK)
         napi_set_named_property(env, o, "tlasOrder", copy_typed(env, order, v.inst_count, napi_uint32_array, 4));

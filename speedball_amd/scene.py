@@ -71,6 +71,43 @@ def directional_light(pos, target, color=(1.0, 1.0, 1.0), intensity=1.0) -> np.n
     return r
 
 
+def emitter_light(center, radius: float, area: float, radiance) -> np.ndarray:
+    """collectEmitterRecords record of an emissive mesh (js/spectral_scene.js:815-875):
+    bounding-sphere centre and radius, surface radiance (emissive x intensity x scale) in
+    the colour slots, the Cauchy projected area A/4 in [15] and the power-based range
+    sqrt(max(radiance) A/4 / 0.005)."""
+    rgb = np.asarray(radiance, np.float64)
+    proj = float(area) / 4.0
+    r = np.zeros(LIGHT_STRIDE, np.float32)
+    r[0] = 3
+    r[1:4] = center
+    r[4:7] = (0, 0, -1)
+    r[7:10] = rgb
+    r[10] = np.sqrt(max(0.0, float(rgb.max())) * proj / 0.005)
+    r[11] = 2
+    r[12] = -1
+    r[13] = -1
+    r[14] = radius
+    r[15] = proj
+    return r
+
+
+def emitter_from_mesh(positions, indices, world=None, radiance=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """The emitter record of an emissive mesh the way collectEmitterRecords derives it: the
+    world-space triangle area, the geometry's bounding sphere (box centre, farthest vertex)
+    moved by the world matrix and scaled by its largest axis scale."""
+    pos = np.asarray(positions, np.float64).reshape(-1, 3)
+    idx = np.asarray(indices, np.int64).reshape(-1, 3)
+    w = np.eye(4) if world is None else np.asarray(world, np.float64)
+    wp = pos @ w[:3, :3].T + w[:3, 3]
+    a, b, c = wp[idx[:, 0]], wp[idx[:, 1]], wp[idx[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1).sum()
+    centre = (pos.min(0) + pos.max(0)) / 2
+    radius = np.sqrt(((pos - centre) ** 2).sum(1).max())
+    scale = np.linalg.norm(w[:3, :3], axis=0).max()
+    return emitter_light(w[:3, :3] @ centre + w[:3, 3], radius * scale, area, radiance)
+
+
 @dataclass
 class Scene:
     bvh_nodes: np.ndarray
@@ -85,6 +122,8 @@ class Scene:
     tlas_node_count: int
     tlas_order: np.ndarray
     meta: dict = field(default_factory=dict)
+    emitter_base: int = 0
+    emitter_count: int = 0
     _view: N.SceneView | None = None
 
     @property
@@ -113,6 +152,8 @@ class Scene:
             v.tlas_node_count = self.tlas_node_count
             v.lights = self.lights.ctypes.data if self.lights.size else None
             v.light_count = self.lights.size // LIGHT_STRIDE
+            v.emitter_base = self.emitter_base
+            v.emitter_count = self.emitter_count
             self._view = v
         return self._view
 
@@ -170,6 +211,8 @@ class SceneBuilder:
         return i.value
 
     def add_light(self, rec):
+        """One light record; emitter records (type 3) are placed after the analytic lights
+        when the scene is finalized."""
         rec = np.ascontiguousarray(rec, np.float32)
         self._chk(self.L.sbgi_sb_add_light(self.h, N.ptr(rec)))
 
@@ -197,7 +240,8 @@ class SceneBuilder:
             materials=grab(v.materials, v.materials_len, np.float32),
             lights=grab(v.lights, v.light_count * LIGHT_STRIDE, np.float32),
             inst_base=v.inst_base, inst_count=v.inst_count, tlas_base=v.tlas_base,
-            tlas_node_count=v.tlas_node_count, tlas_order=order, meta=dict(meta))
+            tlas_node_count=v.tlas_node_count, tlas_order=order, meta=dict(meta),
+            emitter_base=v.emitter_base, emitter_count=v.emitter_count)
 
     def set_transform(self, instance_id: int, world) -> int:
         w = np.ascontiguousarray(np.asarray(world, np.float64).T.reshape(-1))

@@ -234,6 +234,69 @@ def room_boxes(res=(16, 16, 16), rays: int = 64, n_boxes: int = 64, depth_oct: i
                     float(np.float32(jitter)), hysteresis)
 
 
+def room_lights(res=(8, 8, 8), rays: int = 64, n_boxes: int = 32, n_point: int = 24, n_emitters: int = 4,
+                depth_oct: int = 6, seed_id: int = 21, hysteresis: float = 0.97) -> Workload:
+    """The room of room_boxes lit by ranged lights and emitters: n_point point lights with
+    range U[1.5, 5] m (and two spot lights), one directional light, and n_emitters
+    emissive ceiling panels that are both geometry and emitter records (type 3)."""
+    from .scene import emitter_from_mesh
+
+    rng = SplitMix64(SEED_BASE + seed_id)
+    b = SceneBuilder()
+    wall = b.add_material(uber_material((0.5, 0.5, 0.5), side=2))
+    rv, rt = room_mesh(10.0)
+    verts, tris, tmat = [rv], [rt], [np.full(len(rt), wall, np.uint32)]
+    base = len(rv)
+    for _ in range(n_boxes):
+        half = (rng.uniform(0.3, 1.5) / 2, rng.uniform(0.3, 1.5) / 2, rng.uniform(0.3, 1.5) / 2)
+        yaw = rng.uniform(0.0, 2 * math.pi)
+        ctr = (rng.uniform(1.0, 9.0), rng.uniform(half[1], 7.0), rng.uniform(1.0, 9.0))
+        alb = (rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8))
+        m = b.add_material(uber_material(alb))
+        bv, bt = box_mesh(ctr, half, yaw)
+        verts.append(bv)
+        tris.append(bt + base)
+        tmat.append(np.full(len(bt), m, np.uint32))
+        base += len(bv)
+    emitters = []
+    for i in range(n_emitters):
+        ctr = (rng.uniform(1.5, 8.5), 9.6, rng.uniform(1.5, 8.5))
+        half = (rng.uniform(0.2, 0.6), 0.02, rng.uniform(0.2, 0.6))
+        radiance = (rng.uniform(2, 8), rng.uniform(2, 8), rng.uniform(2, 8))
+        m = b.add_material(uber_material((0.8, 0.8, 0.8), emissive=radiance))
+        bv, bt = box_mesh(ctr, half, 0.0)
+        verts.append(bv)
+        tris.append(bt + base)
+        tmat.append(np.full(len(bt), m, np.uint32))
+        base += len(bv)
+        emitters.append(emitter_from_mesh(bv, bt, radiance=radiance))
+    mesh = b.add_mesh(np.concatenate(verts), np.concatenate(tris), tri_material=np.concatenate(tmat))
+    b.add_instance(mesh)
+    # emitter records first: the builder still places them after the analytic lights
+    for rec in emitters:
+        b.add_light(rec)
+    for i in range(n_point):
+        lp = (rng.uniform(0.5, 9.5), rng.uniform(0.5, 9.5), rng.uniform(0.5, 9.5))
+        col = (rng.uniform(0.5, 1.0), rng.uniform(0.5, 1.0), rng.uniform(0.5, 1.0))
+        b.add_light(point_light(lp, col, intensity=rng.log_uniform(2.0, 20.0), distance=rng.uniform(1.5, 5.0),
+                                decay=2.0))
+    for i in range(2):
+        lp = np.array((rng.uniform(2, 8), 9.0, rng.uniform(2, 8)))
+        r = point_light(lp, intensity=15.0, distance=8.0, decay=1.0)
+        r[0] = 2  # spot, pointing down
+        r[4:7] = (0.0, -1.0, 0.0)
+        r[12] = math.cos(0.6)
+        r[13] = math.cos(0.6 * 0.7)
+        b.add_light(r)
+    d = np.array([rng.uniform(-1, 1), -rng.uniform(0.2, 1.0), rng.uniform(-1, 1)])
+    b.add_light(directional_light((5.0, 20.0, 5.0), np.array((5.0, 20.0, 5.0)) + d, intensity=0.5))
+    scene = b.finalize(workload="room_lights")
+    seed = SEED_BASE + seed_id
+    jitter = (seed * 0.6180339887) % 1.0
+    return Workload("room_lights", scene, (0.25, 0.25, 0.25), (9.5, 9.5, 9.5), tuple(res), rays, depth_oct,
+                    float(np.float32(jitter)), hysteresis)
+
+
 def instanced(res=(32, 16, 24), rays: int = 256, total_tris: int = 262144, n_instances: int = 256,
               n_lights: int = 16, depth_oct: int = 14, seed_id: int = 2, hysteresis: float = 0.97) -> Workload:
     """8 procedural meshes x 256 random-TRS instances in a 40x20x25 m box, 16 point lights."""

@@ -335,6 +335,65 @@ def batch_leg(frames=10, autotune=True):
                                            "probes_checked": 24, "field": i}}
 
 
+def lights_leg(steps=10):
+    """Clustered NEE and emitters (the reference's clusteredLighting and emitter path): the
+    instanced scene with 48 point lights of range 8 m and 16 emissive-sphere emitters,
+    updated by a field that loops over every light and by a clustered field (per-cell
+    lists over the probe grid's cells); both bf16x3, ordered traversal. Reports ms per
+    update, the emission / shadow-walk / emitter-visibility / blend stages, shadow rays and
+    the mean list length, plus the FP64 error of the clustered field with emitters."""
+    import oracle
+    from speedball_amd import scenegen
+    from speedball_amd.field import ProbeField
+    from tests.parity import rel_err, interior_irr
+    w = scenegen.instanced(n_lights=48, light_range=8.0, n_emitters=16)
+    kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis, emitter_vis_seed=0.3)
+    out = {"workload": "instanced (261514 triangles, 32x16x24 probes x 256 rays) with 48 point lights of range "
+                       "8 m and 16 emitters; bf16x3 blend, ordered traversal", "steps": steps}
+    first = None
+    for name, clustered in (("all_lights", False), ("clustered", True)):
+        f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct,
+                       precision=PRECISIONS["bf16x3"], traversal=1, clustered=clustered)
+        f.set_scene(w.scene)
+        f.tick(**kw)
+        if clustered:
+            first = f.read_irr()
+            grid = f.read_light_grid()
+            out["mean_list_length"] = float(np.mean(np.minimum(grid[:, 0], 16)))
+            out["overflow_cells"] = int((grid[:, 0] > 16).sum())
+        f.tick(**kw)
+        st = {k: [] for k in ("total", "emit", "walk_shadow", "emitter_vis", "blend")}
+        for _ in range(steps):
+            f.tick(**kw)
+            x = f.stats()
+            st["total"].append(x.ms_total)
+            st["emit"].append(x.ms_emit)
+            st["walk_shadow"].append(x.ms_walk_shadow)
+            st["emitter_vis"].append(x.ms_emitter_vis)
+            st["blend"].append(x.ms_blend)
+        f.set_counting(True)
+        f.tick(**kw)
+        shadow = int(f.stats().shadow_rays)
+        f.set_counting(False)
+        f.close()
+        out[name] = dict({f"ms_{k}": float(np.median(v)) for k, v in st.items()}, shadow_rays=shadow)
+        log(f"lights: {name}: {out[name]}")
+    # the first update's irradiance on 24 probes against the FP32 restatement (the parity
+    # bar: same f32 rays) and the FP64 oracle (these corner probes are dim, so a primary
+    # ray that grazes an edge and shades the neighbouring facet moves a texel by percents)
+    for prec in ("32", "64"):
+        o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec=prec)
+        o.set_clustered()
+        p = o.params(**kw)
+        p.updated_count = 24
+        o.update(p)
+        e = rel_err(interior_irr(first)[:24, :, :3], interior_irr(o.irr_view())[:24, :, :3])
+        out[f"irradiance_rel_err_vs_fp{prec}"] = {"p99": float(np.percentile(e, 99)), "max": float(e.max()),
+                                                  "probes_checked": 24}
+    out["clustered_speedup"] = out["all_lights"]["ms_total"] / out["clustered"]["ms_total"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +410,7 @@ def main():
     ap.add_argument("--dynamic-frames", type=int, default=600, help="frames of the dynamic-scene leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-fields leg")
     ap.add_argument("--batch-frames", type=int, default=10, help="timed frames of the batched-fields leg")
+    ap.add_argument("--no-lights", action="store_true", help="skip the clustered-lights and emitters leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
@@ -443,6 +503,7 @@ def main():
     adj = None if args.no_adjoint else adjoint_leg()
     dyn = None if args.no_dynamic else dynamic_leg(frames=args.dynamic_frames)
     bat = None if args.no_batch else batch_leg(frames=args.batch_frames)
+    lights = None if args.no_lights else lights_leg()
     cpu = None if args.no_cpu else cpu_baseline(w)
 
     out = {
@@ -476,6 +537,7 @@ def main():
         "adjoint": adj,
         "dynamic": dyn,
         "batch": bat,
+        "lights": lights,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -224,6 +224,8 @@ typedef struct sbgi_stats {
        enabled): lane visits / (64 x wave steps) is the SIMD utilisation of a phase     */
     uint64_t walk_waves[4];
     uint64_t shadow_waves[4];
+    float ms_emitter_vis;        /* emitter visibility rays (scenes with emitters)    */
+    uint32_t pad;
 } sbgi_stats;
 
 /* Wave schedule of the persistent walks (no reference counterpart: it is how this build

@@ -98,7 +98,8 @@ class Stats(C.Structure):
                 ("ms_walk_primary", C.c_float), ("ms_emit", C.c_float), ("ms_walk_shadow", C.c_float),
                 ("ms_resolve", C.c_float), ("shadow_blas_nodes", C.c_uint64), ("shadow_tlas_nodes", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64),
-                ("walk_waves", C.c_uint64 * 4), ("shadow_waves", C.c_uint64 * 4)]
+                ("walk_waves", C.c_uint64 * 4), ("shadow_waves", C.c_uint64 * 4), ("ms_emitter_vis", C.c_float),
+                ("pad", C.c_uint32)]
 
 
 class SkinSet(C.Structure):

@@ -390,6 +390,7 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     for (auto &e : f->dev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
+    if (hipEventCreate(&f->eev) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     const uint32_t tile_d = desc->depth_oct_res + 2;
     for (uint32_t ci = 0; ci < desc->cascade_count; ci++) {
         CascadeState &cs = f->casc[ci];
@@ -862,6 +863,7 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     HIP_OK(f, hipEventRecord(f->ev[1], st));
     HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
                            f->count_mode ? f->counters : nullptr, f->tev, st));
+    HIP_OK(f, hipEventRecord(f->eev, st));
     HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.queue.count + trace_heads(3), st));
     HIP_OK(f, hipEventRecord(f->ev[2], st));
     HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, emitter_inject(f, cs, *p),
@@ -1055,7 +1057,8 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     float sub[4] = {0.f, 0.f, 0.f, 0.f};
     if (f->last_updated) {
         for (int i = 0; i < 3; i++) HIP_OK(f, hipEventElapsedTime(&sub[i], f->tev[i], f->tev[i + 1]));
-        HIP_OK(f, hipEventElapsedTime(&sub[3], f->tev[3], f->ev[2]));
+        HIP_OK(f, hipEventElapsedTime(&sub[3], f->tev[3], f->eev));
+        HIP_OK(f, hipEventElapsedTime(&out->ms_emitter_vis, f->eev, f->ev[2]));
     }
     out->ms_walk_primary = sub[0];
     out->ms_emit = sub[1];
@@ -1310,6 +1313,7 @@ int32_t sbgi_destroy(sbgi_field *f) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : f->kev)
         if (e) (void)hipEventDestroy(e);
+    if (f->eev) (void)hipEventDestroy(f->eev);
     for (int s = 0; s < sbgi_field::kStageSlots; s++) {
         if (f->stage_done[s]) (void)hipEventDestroy(f->stage_done[s]);
         if (f->stage_host[s]) (void)hipHostFree(f->stage_host[s]);

@@ -298,7 +298,8 @@ def room_lights(res=(8, 8, 8), rays: int = 64, n_boxes: int = 32, n_point: int =
 
 
 def instanced(res=(32, 16, 24), rays: int = 256, total_tris: int = 262144, n_instances: int = 256,
-              n_lights: int = 16, depth_oct: int = 14, seed_id: int = 2, hysteresis: float = 0.97) -> Workload:
+              n_lights: int = 16, depth_oct: int = 14, seed_id: int = 2, hysteresis: float = 0.97,
+              light_range: float = 0.0, n_emitters: int = 0) -> Workload:
     """8 procedural meshes x 256 random-TRS instances in a 40x20x25 m box, 16 point lights."""
     rng = SplitMix64(SEED_BASE + seed_id)
     box = np.array([40.0, 20.0, 25.0])
@@ -331,7 +332,14 @@ def instanced(res=(32, 16, 24), rays: int = 256, total_tris: int = 262144, n_ins
         b.add_instance(meshes[assign[i]], trs(t, R, s))
     for _ in range(n_lights):
         p = (rng.uniform(0, box[0]), rng.uniform(0, box[1]), rng.uniform(0, box[2]))
-        b.add_light(point_light(p, intensity=rng.log_uniform(1.0, 100.0), distance=0.0, decay=2.0))
+        b.add_light(point_light(p, intensity=rng.log_uniform(1.0, 100.0), distance=light_range, decay=2.0))
+    if n_emitters:
+        from .scene import emitter_light
+        for _ in range(n_emitters):  # emissive spheres, given as records only
+            c = (rng.uniform(0, box[0]), rng.uniform(0, box[1]), rng.uniform(0, box[2]))
+            r = rng.uniform(0.3, 1.0)
+            b.add_light(emitter_light(c, r, 4 * math.pi * r * r, (rng.uniform(2, 8), rng.uniform(2, 8),
+                                                                   rng.uniform(2, 8))))
     scene = b.finalize(workload="instanced", instanced_triangles=actual, unique_triangles=int(sum(mesh_tris)))
     seed = SEED_BASE + seed_id
     jitter = (seed * 0.6180339887) % 1.0

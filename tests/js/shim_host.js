@@ -200,6 +200,43 @@ tests.layout_errors_request_a_rebuild = async () => {
     assert.strictEqual(calls.filter((c) => c[0] === 'setScene').length, 2);
 };
 
+tests.emitters_and_clustered_lighting = async () => {
+    // a giEmitter mesh of a three-like graph and its plain { center, radius, area, radiance }
+    // description give the same type-3 record, placed after the analytic lights
+    const s = threeScene();
+    const panel = s.objs[1];                       // the 2 x 0.5 x 1 box at (3, 1, 0)
+    panel.userData = { giEmitter: true };
+    panel.material = Object.assign({}, panel.material, { emissive: [1, 0.5, 0.25], emissiveIntensity: 4 });
+    const three = shim.buildScene(real, s).arrays;
+    assert.strictEqual(three.emitterCount, 1);
+    assert.strictEqual(three.emitterBase, 1);
+    const rec = three.lights.subarray(17, 34);
+    const area = 2 * (2 * 0.5 + 2 * 1 + 0.5 * 1), radius = Math.hypot(1, 0.25, 0.5);
+    const near = (a, b) => Math.abs(a - b) <= 1e-5 * Math.max(1, Math.abs(b));
+    assert.strictEqual(rec[0], 3);
+    assert.ok(near(rec[1], 3) && near(rec[2], 1) && near(rec[3], 0));
+    assert.ok(near(rec[7], 4) && near(rec[8], 2) && near(rec[9], 1));
+    assert.ok(near(rec[14], radius) && near(rec[15], area / 4));
+    assert.ok(near(rec[10], Math.sqrt(4 * (area / 4) / 0.005)));
+    const plain = plainScene();
+    plain.emitters = [{ center: [3, 1, 0], radius: radius, area: area, radiance: [4, 2, 1] }];
+    const p = shim.buildScene(real, plain).arrays;
+    assert.deepStrictEqual(Array.from(p.lights), Array.from(three.lights));
+    // clusteredLighting sets the create flag; Monte Carlo ticks of cascade 0 move the
+    // emitter sample seed, gated ticks hold it
+    const { fake, calls } = recorder();
+    const gi = shim.createProbeField({ scene: plain, addon: fake, clusteredLighting: true, cascades: 1 });
+    gi.setEnabled(true);
+    await gi.tick({});
+    assert.strictEqual(calls.find((c) => c[0] === 'create')[1].flags, 2);
+    assert.strictEqual(calls.filter((c) => c[0] === 'tick')[0][1].emitterVisSeed, 0);
+    gi.setJitterMode('montecarlo');
+    await gi.tick({});
+    await gi.tick({});
+    const seeds = calls.filter((c) => c[0] === 'tick').map((c) => c[1].emitterVisSeed);
+    assert.ok(seeds[1] > 0 && seeds[2] > 0 && seeds[1] !== seeds[2]);
+};
+
 tests.install_handle_forwards_and_guards = async () => {
     const { fake } = recorder();
     let errs = 0;

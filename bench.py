@@ -335,6 +335,34 @@ def batch_leg(frames=10, autotune=True):
                                            "probes_checked": 24, "field": i}}
 
 
+def sample_leg(f, w, n=1 << 20, reps=5):
+    """Receiver sampling (sbgi_sample_irradiance) of the updated headline field at 2^20
+    random points with random normals, leak controls on (chebyStrength 1): kernel time
+    from the library's events (median of reps), and the points against the FP32
+    restatement over the same atlases on a 4096-point subset."""
+    import oracle
+    from speedball_amd import _native as N
+    from tests.parity import rel_err
+    rng = np.random.default_rng(11)
+    pos = (rng.uniform(0, 1, (n, 3)) * np.array(w.grid_size) + np.array(w.grid_min)).astype(np.float32)
+    nv = rng.normal(size=(n, 3)).astype(np.float32)
+    ms = []
+    out = None
+    for _ in range(reps):
+        out, m = f.sample(pos, nv, cheby_strength=1.0)
+        ms.append(m)
+    k = 4096
+    ref = oracle.sample_irradiance([(w.grid_min, w.grid_size, w.res)], w.depth_oct, [f.read_atlas()],
+                                   [f.read_atlas(depth=True)], [f.read(N.BUF_META, np.float32)],
+                                   N.default_sample_params(cheby_strength=1.0), pos[:k], nv[:k], prec="32")
+    e = rel_err(out[:k], ref, floor_frac=1e-3)
+    med = float(np.median(ms))
+    # per point: 8 taps x (16 B meta + 4 x 8 B depth texels + 4 x 8 B irradiance texels) + 36 B in/out
+    alg = n * (8 * (16 + 32 + 32) + 36)
+    return {"points": n, "ms": med, "points_per_s": n / (med / 1e3), "algorithmic_gbs": alg / (med / 1e3) / 1e9,
+            "rel_err_vs_fp32": {"max": float(e.max()), "points_checked": k}}
+
+
 def lights_leg(steps=10):
     """Clustered NEE and emitters (the reference's clusteredLighting and emitter path): the
     instanced scene with 48 point lights of range 8 m and 16 emissive-sphere emitters,
@@ -499,6 +527,7 @@ def main():
     err99 = errmax = None
     if not args.no_check:
         err99, errmax = fp64_check(w, first_irr, prec)
+    sampling = sample_leg(f, w)
     f.close()
     adj = None if args.no_adjoint else adjoint_leg()
     dyn = None if args.no_dynamic else dynamic_leg(frames=args.dynamic_frames)
@@ -538,6 +567,7 @@ def main():
         "dynamic": dyn,
         "batch": bat,
         "lights": lights,
+        "receiver_sampling": sampling,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

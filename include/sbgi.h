@@ -356,6 +356,32 @@ int32_t sbgi_set_counting(sbgi_field *f, int32_t on);
  * {t, tri (int32 bits), inst (int32 bits), blocked} per ray. Synchronous. */
 int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *out4);
 
+/* ---- receiver sampling: the probe field at N points ------------------------- */
+/* The receiver's irradiance gather (GiProbeNode.sampleIrradiance / _sampleCascade /
+ * _tapEW, js/gi_probes.js:480-614, 825-847) evaluated for a batch of points instead of
+ * per fragment: P = X + normalize(Nvis) * normal_bias; per cascade the 8 probes of P's
+ * cell with trilinear weights (+1e-4), Chebyshev visibility of the depth moments and the
+ * wrap weight (both scaled in by cheby_strength) and the probe state (classify_strength),
+ * irradiance fetched in the direction Ndir; finer cascades blended in over border_band
+ * of their box (cascade c+1 is taken as finer than cascade c). */
+typedef struct sbgi_sample_params {
+    float normal_bias;           /* normalBias[0] x sampleBiasScale (0.04)            */
+    float cheby_strength;        /* chebyStrength (0: pure trilinear)                 */
+    float classify_strength;     /* classifyStrength (0)                              */
+    float border_band;           /* borderBand (0.15)                                 */
+    float cheby_bias[SBGI_MAX_CASCADES]; /* chebyBias per cascade (0)                 */
+    uint32_t cascades;           /* cascades folded, 0 = all of the field's           */
+    uint32_t pad;
+} sbgi_sample_params;
+
+void sbgi_sample_params_default(sbgi_sample_params *p);
+
+/* pos, nvis: n x 3 floats (world position, visibility normal); ndir: n x 3 fetch
+ * normals or NULL (= nvis); out: n x 3 irradiance. Host pointers; synchronous, after
+ * the field's queued solves. ms (optional) receives the kernel's device time. */
+int32_t sbgi_sample_irradiance(sbgi_field *f, const sbgi_sample_params *p, const float *pos, const float *nvis,
+                               const float *ndir, uint32_t n, float *out, float *ms);
+
 /* ---- differentiable mode ---------------------------------------------------------
  * No counterpart in the reference (its probe update is not differentiable); the build
  * adds it for gradient-based callers (inverse rendering, light/albedo fitting).

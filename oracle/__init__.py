@@ -221,6 +221,30 @@ def light_grid(scene, c0) -> np.ndarray:
     return out
 
 
+def sample_irradiance(cascades, depth_oct, atlases, datlases, metas, params, pos, nvis, ndir=None, prec="64"):
+    """The receiver gather at n points. cascades: [(grid_min, grid_size, res)];
+    atlases / datlases: per cascade RGBA arrays (rows x cols x 4) as the upload writes them;
+    metas: per cascade float32 [P * 4]; params: an sbgi_sample_params."""
+    dt = _dt(prec)
+    nc = len(cascades)
+    desc = (type(cascade_desc((0, 0, 0), (1, 1, 1), (1, 1, 1))) * nc)()
+    for i, (gm, gs, rs) in enumerate(cascades):
+        desc[i] = cascade_desc(gm, gs, rs)
+    keep = [np.ascontiguousarray(a, dt) for a in atlases] + [np.ascontiguousarray(a, dt) for a in datlases]
+    ms = [np.ascontiguousarray(m, np.float32) for m in metas]
+    A = (C.c_void_p * nc)(*[k.ctypes.data for k in keep[:nc]])
+    D = (C.c_void_p * nc)(*[k.ctypes.data for k in keep[nc:]])
+    M = (C.c_void_p * nc)(*[m.ctypes.data for m in ms])
+    pos = np.ascontiguousarray(pos, dt).reshape(-1)
+    nvis = np.ascontiguousarray(nvis, dt).reshape(-1)
+    nd = None if ndir is None else np.ascontiguousarray(ndir, dt).reshape(-1)
+    n = pos.size // 3
+    out = np.zeros(n * 3, dt)
+    _fn(prec, "sample_irradiance")(desc, C.c_uint32(nc), C.c_uint32(depth_oct), A, D, M, C.byref(params), _p(pos),
+                                   _p(nvis), _p(nd), C.c_uint32(n), _p(out))
+    return out.reshape(n, 3)
+
+
 def refit_blas(nodes: np.ndarray, scene, root: int, end: int) -> bool:
     f = lib().or_refit_blas
     return bool(f(_p(nodes), C.c_uint32(nodes.size // 8), _p(scene.tri_index), C.c_uint32(scene.tri_count),

@@ -1058,6 +1058,127 @@ void P(light_grid)(const sbgi_scene_view *s, const sbgi_cascade_desc *c0, uint32
     }
 }
 
+/* The receiver's irradiance gather at n points (GiProbeNode.sampleIrradiance /
+ * _sampleCascade / _tapEW, js/gi_probes.js:480-614, 825-847) over the atlases and meta of
+ * n_casc cascades (atlas[c], datlas[c]: RGBA rows as the upload writes them; meta[c]:
+ * 4 floats per probe). ndir may be NULL (= nvis). out: n x 3. */
+static void P(sample_tile)(const REAL *atlas, uint32_t w, uint32_t h, uint32_t tile, uint32_t oct, REAL col, REAL row,
+                           const REAL uv[2], REAL out[3]) {
+    REAL ox = col * (REAL)tile + RCONST(1) + uv[0] * (REAL)oct;
+    REAL oy = row * (REAL)tile + RCONST(1) + uv[1] * (REAL)oct;
+    P(bilinear)(atlas, w, h, ox / (REAL)w, oy / (REAL)h, out);
+}
+
+static REAL P(mix)(REAL a, REAL b, REAL t) { return a * (RCONST(1) - t) + b * t; }
+
+static void P(sample_one)(const sbgi_cascade_desc *c, uint32_t depth_oct, const REAL *atlas, const REAL *datlas,
+                          const float *meta, const sbgi_sample_params *p, float cheby_bias, const REAL P_[3],
+                          const REAL Nn[3], const REAL octN[2], REAL E[3]) {
+    REAL gf[3], base[3], fr[3], res[3];
+    for (int k = 0; k < 3; k++) {
+        res[k] = (REAL)c->res[k];
+        REAL den = res[k] - RCONST(1);
+        REAL cell = (REAL)c->grid_size[k] / (den > 1 ? den : RCONST(1));
+        gf[k] = (P_[k] - (REAL)c->grid_min[k]) / (cell > RCONST(1e-6) ? cell : RCONST(1e-6));
+        REAL hi = res[k] - RCONST(2);
+        hi = hi > 0 ? hi : 0;
+        REAL b = RFLOOR(gf[k]);
+        b = b < 0 ? 0 : (b > hi ? hi : b);
+        base[k] = b;
+        REAL f = gf[k] - b;
+        fr[k] = f < 0 ? 0 : (f > 1 ? 1 : f);
+    }
+    uint32_t aw = c->res[0] * 8u, ah = c->res[1] * c->res[2] * 8u;
+    uint32_t tile_d = depth_oct == 6 ? 8u : depth_oct + 2u;
+    uint32_t daw = c->res[0] * tile_d, dah = c->res[1] * c->res[2] * tile_d;
+    REAL acc[3] = {0, 0, 0}, wsum = 0;
+    for (int i = 0; i < 8; i++) {
+        int d[3] = {i & 1, (i >> 1) & 1, (i >> 2) & 1};
+        REAL pc[3], wt = 1;
+        for (int k = 0; k < 3; k++) {
+            pc[k] = base[k] + (REAL)d[k];
+            if (pc[k] > res[k] - 1) pc[k] = res[k] - 1;
+            wt *= d[k] ? fr[k] : RCONST(1) - fr[k];
+        }
+        REAL wtri = wt + RCONST(1e-4);
+        REAL col = pc[0], row = pc[2] * res[1] + pc[1];
+        uint32_t probe = (uint32_t)pc[0] + (uint32_t)pc[1] * c->res[0] + (uint32_t)pc[2] * c->res[0] * c->res[1];
+        const float *m = meta + (size_t)probe * 4;
+        REAL ppos[3], ts[3];
+        for (int k = 0; k < 3; k++) {
+            REAL den = res[k] - RCONST(1);
+            ppos[k] = (REAL)c->grid_min[k] + (pc[k] / (den > 1 ? den : RCONST(1))) * (REAL)c->grid_size[k] +
+                      (REAL)m[1 + k] * (REAL)p->classify_strength;
+            ts[k] = P_[k] - ppos[k];
+        }
+        REAL dist = RSQRT(P(dot3)(ts, ts));
+        REAL dd = dist > RCONST(1e-6) ? dist : RCONST(1e-6);
+        REAL dir[3] = {ts[0] / dd, ts[1] / dd, ts[2] / dd}, ouv[2], mom[3];
+        P(oct_encode)(dir, ouv);
+        P(sample_tile)(datlas, daw, dah, tile_d, depth_oct, col, row, ouv, mom);
+        REAL m1 = mom[0], m2 = mom[1];
+        REAL var = RABS(m2 - m1 * m1);
+        REAL db = (REAL)cheby_bias;
+        REAL dm = (dist - m1) - db;
+        dm = dm > 0 ? dm : 0;
+        REAL den = var + dm * dm;
+        REAL cheby = dist <= m1 + db ? RCONST(1) : var / (den > RCONST(1e-6) ? den : RCONST(1e-6));
+        REAL c3 = cheby * cheby * cheby;
+        REAL vis_w = P(mix)(RCONST(1), c3 > RCONST(0.05) ? c3 : RCONST(0.05), (REAL)p->cheby_strength);
+        REAL tp[3] = {(ppos[0] - P_[0]) / dd, (ppos[1] - P_[1]) / dd, (ppos[2] - P_[2]) / dd};
+        REAL wrap = P(dot3)(tp, Nn) * RCONST(0.5) + RCONST(0.5);
+        REAL w2 = wrap * wrap;
+        REAL wrap_w = P(mix)(RCONST(1), w2 > RCONST(0.05) ? w2 : RCONST(0.05), (REAL)p->cheby_strength);
+        REAL state = P(mix)(RCONST(1), (REAL)m[0], (REAL)p->classify_strength);
+        REAL w = ((wtri * wrap_w) * vis_w) * state;
+        REAL e[3];
+        P(sample_tile)(atlas, aw, ah, 8u, 6u, col, row, octN, e);
+        for (int k = 0; k < 3; k++) acc[k] += e[k] * w;
+        wsum += w;
+    }
+    REAL ws = wsum > RCONST(1e-4) ? wsum : RCONST(1e-4);
+    for (int k = 0; k < 3; k++) E[k] = acc[k] / ws;
+}
+
+void P(sample_irradiance)(const sbgi_cascade_desc *casc, uint32_t n_casc, uint32_t depth_oct,
+                          const REAL *const *atlas, const REAL *const *datlas, const float *const *meta,
+                          const sbgi_sample_params *p, const REAL *pos, const REAL *nvis, const REAL *ndir,
+                          uint32_t n, REAL *out) {
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = 0; i < n; i++) {
+        REAL Nn[3] = {nvis[3 * i], nvis[3 * i + 1], nvis[3 * i + 2]};
+        P(normalize3)(Nn);
+        REAL Nd[3];
+        for (int k = 0; k < 3; k++) Nd[k] = ndir ? ndir[3 * i + k] : nvis[3 * i + k];
+        P(normalize3)(Nd);
+        REAL octN[2], Pp[3];
+        P(oct_encode)(Nd, octN);
+        for (int k = 0; k < 3; k++) Pp[k] = pos[3 * i + k] + Nn[k] * (REAL)p->normal_bias;
+        REAL wf[SBGI_MAX_CASCADES] = {0, 0, 0, 0};
+        for (uint32_t c = 1; c < n_casc; c++) {
+            REAL f[3];
+            for (int k = 0; k < 3; k++) {
+                REAL g = (REAL)casc[c].grid_size[k];
+                f[k] = (Pp[k] - (REAL)casc[c].grid_min[k]) / (g > RCONST(1e-6) ? g : RCONST(1e-6));
+            }
+            REAL lo = P(fmin)(P(fmin)(f[0], f[1]), f[2]);
+            REAL hi = P(fmin)(P(fmin)(RCONST(1) - f[0], RCONST(1) - f[1]), RCONST(1) - f[2]);
+            REAL band = (REAL)p->border_band > RCONST(1e-4) ? (REAL)p->border_band : RCONST(1e-4);
+            REAL x = P(fmin)(lo, hi) / band;
+            wf[c] = x < 0 ? 0 : (x > 1 ? 1 : x);
+        }
+        REAL E[3] = {0, 0, 0}, Ec[3];
+        if (n_casc < 2 || wf[1] < 1)
+            P(sample_one)(&casc[0], depth_oct, atlas[0], datlas[0], meta[0], p, p->cheby_bias[0], Pp, Nn, octN, E);
+        for (uint32_t c = 1; c < n_casc; c++) {
+            if (!(wf[c] > 0)) continue;
+            P(sample_one)(&casc[c], depth_oct, atlas[c], datlas[c], meta[c], p, p->cheby_bias[c], Pp, Nn, octN, Ec);
+            for (int k = 0; k < 3; k++) E[k] = P(mix)(E[k], Ec[k], wf[c]);
+        }
+        for (int k = 0; k < 3; k++) out[3 * i + k] = E[k];
+    }
+}
+
 /* One full update of one cascade (trace -> blend -> upload), as one solve of tick()
  * (js/gi_probes.js:3836). prev_atlas is read for the bounce, then atlas is rewritten:
  * pass the same array for both to chain updates. */

@@ -45,7 +45,7 @@ EXPORTS = [
     "sbgi_set_skins", "sbgi_skin", "sbgi_edit_timing", "sbgi_sb_mesh_ranges",
     "sbgi_batch_create", "sbgi_batch_tick", "sbgi_batch_wait", "sbgi_batch_timing", "sbgi_batch_set_walk_block",
     "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy", "sbgi_sb_add_materials",
-    "sbgi_rebuild_blas",
+    "sbgi_rebuild_blas", "sbgi_sample_params_default", "sbgi_sample_irradiance",
 ]
 
 
@@ -112,6 +112,24 @@ class WalkTuning(C.Structure):
                 ("fetch_chunk", C.c_uint32)]
 
 
+class SampleParams(C.Structure):
+    _fields_ = [("normal_bias", C.c_float), ("cheby_strength", C.c_float), ("classify_strength", C.c_float),
+                ("border_band", C.c_float), ("cheby_bias", C.c_float * 4), ("cascades", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
+def default_sample_params(**kw) -> SampleParams:
+    p = SampleParams()
+    lib().sbgi_sample_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "cheby_bias":
+            for i, x in enumerate(v):
+                p.cheby_bias[i] = float(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
 class SbgiError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"sbgi error {code}: {msg}")
@@ -156,6 +174,8 @@ def lib() -> C.CDLL:
         "sbgi_get_stats": (i32, [vp, C.POINTER(Stats)]),
         "sbgi_set_counting": (i32, [vp, i32]),
         "sbgi_query_rays": (i32, [vp, vp, u32, vp]),
+        "sbgi_sample_params_default": (None, [C.POINTER(SampleParams)]),
+        "sbgi_sample_irradiance": (i32, [vp, C.POINTER(SampleParams), vp, vp, vp, u32, vp, vp]),
         "sbgi_diff_param_count": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
         "sbgi_diff_albedo_map": (i32, [vp, vp, u32]),
         "sbgi_diff_forward": (i32, [vp, C.POINTER(TickParams), vp, vp, vp]),

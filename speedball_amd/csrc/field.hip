@@ -391,6 +391,8 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
     for (auto &e : f->dev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     if (hipEventCreate(&f->eev) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
+    for (auto &e : f->sev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipEventCreate failed"));
     const uint32_t tile_d = desc->depth_oct_res + 2;
     for (uint32_t ci = 0; ci < desc->cascade_count; ci++) {
         CascadeState &cs = f->casc[ci];
@@ -1314,6 +1316,9 @@ int32_t sbgi_destroy(sbgi_field *f) {
     for (auto &e : f->kev)
         if (e) (void)hipEventDestroy(e);
     if (f->eev) (void)hipEventDestroy(f->eev);
+    for (auto &e : f->sev)
+        if (e) (void)hipEventDestroy(e);
+    dfree(f->d_sample);
     for (int s = 0; s < sbgi_field::kStageSlots; s++) {
         if (f->stage_done[s]) (void)hipEventDestroy(f->stage_done[s]);
         if (f->stage_host[s]) (void)hipHostFree(f->stage_host[s]);
@@ -1340,6 +1345,50 @@ int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *ou
     dfree(din);
     dfree(dout);
     if (e != hipSuccess) return hip_fail(f, e, "sbgi_query_rays");
+    return SBGI_OK;
+}
+
+void sbgi_sample_params_default(sbgi_sample_params *p) {
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->normal_bias = 0.04f;  // normalBias (js/gi_probes.js:314, 1043) x sampleBiasScale 1
+    p->border_band = 0.15f;  // borderBand (js/gi_probes.js:350)
+}
+
+int32_t sbgi_sample_irradiance(sbgi_field *f, const sbgi_sample_params *p, const float *pos, const float *nvis,
+                               const float *ndir, uint32_t n, float *out, float *ms) {
+    if (!f || !p || (n && (!pos || !nvis || !out))) return SBGI_E_INVALID;
+    const uint32_t nc = p->cascades ? p->cascades : f->desc.cascade_count;
+    if (nc > f->desc.cascade_count) return fail(f, SBGI_E_INVALID, "more cascades than the field has");
+    if (ms) *ms = 0.f;
+    if (n == 0) return SBGI_OK;
+    if ((uint64_t)n * 3 >= (1ull << 32)) return fail(f, SBGI_E_CAPACITY, "too many points");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    const size_t vec = (size_t)n * 12;
+    const size_t need = vec * (ndir ? 4 : 3);
+    if (need > f->sample_cap) {
+        dfree(f->d_sample);
+        f->sample_cap = 0;
+        if (dalloc(&f->d_sample, need) != hipSuccess) return fail(f, SBGI_E_NOMEM, "sample buffer allocation failed");
+        f->sample_cap = need;
+    }
+    float *d_pos = f->d_sample, *d_nvis = d_pos + 3 * (size_t)n, *d_out = d_nvis + 3 * (size_t)n;
+    float *d_ndir = ndir ? d_out + 3 * (size_t)n : nullptr;
+    SampleCascadeIn sc[SBGI_MAX_CASCADES];
+    for (uint32_t c = 0; c < nc; c++) {
+        const CascadeState &cs = f->casc[c];
+        sc[c] = SampleCascadeIn{make_cascade(f, cs, 0, cs.total), cs.atlas, cs.datlas, cs.meta};
+    }
+    hipStream_t st = f->stream;
+    HIP_OK(f, hipMemcpyAsync(d_pos, pos, vec, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipMemcpyAsync(d_nvis, nvis, vec, hipMemcpyHostToDevice, st));
+    if (ndir) HIP_OK(f, hipMemcpyAsync(d_ndir, ndir, vec, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipEventRecord(f->sev[0], st));
+    HIP_OK(f, launch_sample(sc, nc, *p, d_pos, d_nvis, d_ndir, n, d_out, st));
+    HIP_OK(f, hipEventRecord(f->sev[1], st));
+    HIP_OK(f, hipMemcpyAsync(out, d_out, vec, hipMemcpyDeviceToHost, st));
+    HIP_OK(f, hipStreamSynchronize(st));
+    if (ms) HIP_OK(f, hipEventElapsedTime(ms, f->sev[0], f->sev[1]));
     return SBGI_OK;
 }
 

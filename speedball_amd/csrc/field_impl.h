@@ -113,6 +113,10 @@ struct sbgi_field {
     hipEvent_t ev[5] = {};
     hipEvent_t tev[4] = {};  // trace sub-stages
     hipEvent_t eev = nullptr;  // end of the trace, before the emitter visibility rays
+    // receiver sampling (sbgi_sample_irradiance): staging of points, normals and results
+    float *d_sample = nullptr;
+    size_t sample_cap = 0;
+    hipEvent_t sev[2] = {};
     sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk};
     // differentiable mode (sbgi_diff_*): parameters, the first update's FP32 atlas, the
     // gradients and the adjoint's scratch; the frozen hits and visibility masks are the

@@ -72,6 +72,15 @@ hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tic
 hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float *meta,
                               float *vis, uint32_t *heads, hipStream_t st);
 
+// k_sample.hip: the receiver gather at n points (sbgi_sample_irradiance)
+struct SampleCascadeIn {
+    DevCascade c;
+    const ushort4 *atlas, *datlas;
+    const float *meta;
+};
+hipError_t launch_sample(const SampleCascadeIn *casc, uint32_t n_casc, const sbgi_sample_params &p, const float *pos,
+                         const float *nvis, const float *ndir, uint32_t n, float *out, hipStream_t st);
+
 // k_lights.hip
 // lightGridKernel: the cell lists {count, ids[16]} of s's lights over cells[0] x cells[1]
 // x cells[2] cells spanning gmin + gsize (count 17 = overflow)

@@ -243,6 +243,19 @@ class ProbeField:
         self._chk(self.L.sbgi_query_rays(self.h, N.ptr(rec), n, N.ptr(out)))
         return out[:, 0], out[:, 1].view(np.int32), out[:, 2].view(np.int32), out[:, 3] > 0.5
 
+    def sample(self, pos, nvis, ndir=None, **params):
+        """Irradiance of the field at n points (sbgi_sample_irradiance): pos, nvis (and
+        ndir) are (n, 3); returns ((n, 3) float32, device ms)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nvis = np.ascontiguousarray(nvis, np.float32).reshape(-1, 3)
+        nd = None if ndir is None else np.ascontiguousarray(ndir, np.float32).reshape(-1, 3)
+        out = np.empty_like(pos)
+        ms = C.c_float()
+        p = N.default_sample_params(**params)
+        self._chk(self.L.sbgi_sample_irradiance(self.h, C.byref(p), N.ptr(pos), N.ptr(nvis), N.ptr(nd), pos.shape[0],
+                                                N.ptr(out), C.byref(ms)))
+        return out, ms.value
+
     def update_ranges(self, buf: int, ranges, src: np.ndarray):
         r = np.ascontiguousarray(np.asarray(ranges, np.uint32).reshape(-1))
         src = np.ascontiguousarray(src)

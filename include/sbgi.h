@@ -75,6 +75,14 @@ extern "C" {
 #define SBGI_TRAV_WATERTIGHT 2  /* reserved (watertight triangle test, Woop et al. 2013);
                                    sbgi_create rejects it in this build */
 
+/* local reflection lobes from the same rays (reflectionQuality, REFLECTION_QUALITY_TIERS
+   js/gi_probes.js:60-73): the rough power-8 lobe in the irradiance tiles, and the glossy
+   power-64 cache at glossyOct x glossyOct (+ border) per probe */
+#define SBGI_REFLECT_OFF 0
+#define SBGI_REFLECT_ROUGH 1    /* rough lobe only                                      */
+#define SBGI_REFLECT_HIGH 2     /* + glossy 8x8, every texel solved on alternate updates */
+#define SBGI_REFLECT_ULTRA 3    /* + glossy 16x16, every texel solved every update       */
+
 /* buffer ids for sbgi_read / sbgi_write / sbgi_update_ranges */
 #define SBGI_BUF_BVH_NODES 0    /* u32[8*N]                 (update_ranges)           */
 #define SBGI_BUF_TRI_INDEX 1    /* u32[3*T]                 (update_ranges, read)     */
@@ -98,6 +106,13 @@ extern "C" {
                                    probe-major, 16 = SBGI_MAX_EMITTERS (read, write)  */
 #define SBGI_BUF_LIGHT_GRID 22  /* u32[17*cells] clustered light lists of cascade 0 (read):
                                    per cell {count, light ids[16]}, count 17 = overflow */
+#define SBGI_BUF_ROUGH_STATE 23 /* f32[4*P*64] rough lobe {rgb x coverage, coverage} per
+                                   irradiance texel, per cascade (read, write)          */
+#define SBGI_BUF_ROUGH_ATLAS 24 /* u16 RGBA16F, the irradiance atlas's layout (read)     */
+#define SBGI_BUF_GLOSSY_STATE 25 /* f32[4*P*G*G] glossy numerator, G = glossy tile      */
+#define SBGI_BUF_GLOSSY_WEIGHT 26 /* f32[P*G*G] glossy support                           */
+#define SBGI_BUF_GLOSSY_ATLAS 27 /* u16 RGBA16F [(tx*G) x (ty*G)], tx = ceil(sqrt(P)),
+                                   ty = ceil(P / tx), probe p at (p % tx, p / tx) (read) */
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {
@@ -116,7 +131,7 @@ typedef struct sbgi_field_desc {
     uint32_t blend_precision;    /* SBGI_BLEND_*                                     */
     uint32_t traversal;          /* SBGI_TRAV_*                                      */
     uint32_t flags;              /* SBGI_FIELD_* bits                                */
-    uint32_t pad;
+    uint32_t reflections;        /* SBGI_REFLECT_* (reflectionQuality)               */
     sbgi_cascade_desc cascades[SBGI_MAX_CASCADES];
 } sbgi_field_desc;
 
@@ -192,6 +207,10 @@ typedef struct sbgi_tick_params {
     float dbg_filter_rel;        /* U.debugFilterRelScale (1)                        */
     float dbg_filter_eps;        /* U.debugFilterEpsScale (1)                        */
     float emitter_vis_seed;      /* U.emitterVisSeed (0): emitter sample jitter      */
+    float sky_configured;        /* U.skyConfigured (0): a sky SH was supplied        */
+    float reflection_sky_fallback; /* U.reflectionSkyFallback (0): misses count as
+                                    reflection coverage when the sky is on            */
+    uint32_t glossy_phase;       /* U.glossyPhase: texels solved this update (HIGH)  */
     uint32_t pad;
 } sbgi_tick_params;
 

@@ -117,7 +117,7 @@ def brute_closest(scene, ro, rd, prec="64"):
 class OracleField:
     """FP64 (or FP32) oracle state of one cascade, mirroring the device buffers."""
 
-    def __init__(self, scene, grid_min, grid_size, res, rays, depth_oct=6, prec="64"):
+    def __init__(self, scene, grid_min, grid_size, res, rays, depth_oct=6, prec="64", reflections=0):
         self.scene = scene
         self.prec = prec
         self.dt = _dt(prec)
@@ -140,6 +140,19 @@ class OracleField:
         self.ray_data = None
         self.hit_ids = None
         self.evis = np.zeros(P * 16, self.dt)  # retained emitter visibility
+        # reflection lobes (0 off, 1 rough, 2 high: glossy 10x10 tiles every other texel,
+        # 3 ultra: glossy 18x18 tiles)
+        self.reflections = reflections
+        self.G = {2: 10, 3: 18}.get(reflections, 0)
+        self.interval = 2 if reflections == 2 else 1
+        self.rough = np.zeros(P * 64 * 4, self.dt) if reflections else None
+        self.ratlas = np.zeros(aw * ah * 4, self.dt) if reflections else None
+        if self.G:
+            self.gtiles_x = max(1, int(np.ceil(np.sqrt(P))))
+            gty = -(-P // self.gtiles_x)
+            self.gnum = np.zeros(P * self.G * self.G * 4, self.dt)
+            self.gden = np.zeros(P * self.G * self.G, self.dt)
+            self.gatlas = np.zeros(self.gtiles_x * self.G * gty * self.G * 4, self.dt)
         self.light_grid = None                  # clustered NEE lists (set_clustered)
         self.c0 = None
 
@@ -163,11 +176,13 @@ class OracleField:
         sg = _fn(self.prec, "set_light_grid")
         if self.light_grid is not None:
             sg(_p(self.light_grid), C.byref(self.c0))
+        _fn(self.prec, "set_rough")(C.c_int(1 if self.reflections else 0))
         try:
             _fn(self.prec, "trace")(C.byref(self.scene.view()), C.byref(self.c), C.c_uint32(self.rays), C.byref(p),
                                     _p(self.atlas), _p(self.meta), _p(self.ray_data), _p(self.hit_ids), _p(cnt))
         finally:
             sg(None, None)
+            _fn(self.prec, "set_rough")(C.c_int(0))
         return cnt
 
     def emitter_vis(self, p):
@@ -185,13 +200,31 @@ class OracleField:
         _fn(self.prec, "upload")(C.byref(self.c), C.c_uint32(self.depth_oct), C.byref(p), _p(self.irr),
                                  _p(self.depth), _p(self.atlas), _p(self.datlas))
 
+    def reflect_rough(self, p):
+        """The rough lobe (before the blend: it reads the depth moments as its sentinel)."""
+        _fn(self.prec, "rough")(C.byref(self.c), C.c_uint32(self.rays), C.c_uint32(self.depth_oct), C.byref(p),
+                                _p(self.ray_data), _p(self.depth), _p(self.rough), C.byref(self.scene.view()),
+                                _p(self.meta), _p(self.evis))
+
+    def reflect_glossy(self, p):
+        _fn(self.prec, "glossy")(C.byref(self.c), C.c_uint32(self.rays), C.byref(p), _p(self.ray_data),
+                                 C.c_uint32(self.G), C.c_uint32(self.interval), _p(self.gnum), _p(self.gden),
+                                 _p(self.gatlas), C.c_uint32(self.gtiles_x), C.byref(self.scene.view()),
+                                 _p(self.meta), _p(self.evis))
+
     def update(self, p):
         if p.needs_clear:
             self.evis[:] = 0
         self.trace(p, p.updated_count or None)
         self.emitter_vis(p)
+        if self.reflections:
+            self.reflect_rough(p)
         self.blend(p)
+        if self.G:
+            self.reflect_glossy(p)
         self.upload(p)
+        if self.reflections:
+            _fn(self.prec, "upload_rough")(C.byref(self.c), C.byref(p), _p(self.rough), _p(self.ratlas))
 
     # reshaped views
     def irr_view(self):

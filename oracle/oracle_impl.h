@@ -444,6 +444,11 @@ typedef struct {
 } P(grid_ctx);
 static P(grid_ctx) P(g_grid) = {NULL, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 
+/* Reflection lobes on (roughReflectionsEnabled): hits without Lambert or emissive radiance
+ * record -(t + 2) as their distance (js/gi_probes.js:2055-2062). */
+static int P(g_rough) = 0;
+void P(set_rough)(int on) { P(g_rough) = on; }
+
 void P(set_light_grid)(const uint32_t *grid, const sbgi_cascade_desc *c0) {
     P(g_grid).grid = grid;
     if (grid && c0) {
@@ -651,6 +656,13 @@ void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t
         out[1] = rad[1];
         out[2] = rad[2];
         out[3] = best_t;
+        if (P(g_rough)) {
+            float energy = (1.f - P(mat_f)(s, m, 4)) * (1.f - P(mat_f)(s, m, 5));
+            float e0 = P(mat_f)(s, m, 7), e1 = P(mat_f)(s, m, 8), e2 = P(mat_f)(s, m, 9);
+            float peak = e0 > e1 ? e0 : e1;
+            peak = peak > e2 ? peak : e2;
+            if (!(energy > 0.02f || peak > 1e-6f)) out[3] = -(best_t + RCONST(2));
+        }
     } else {
         /* miss -> SH-9 sky js/gi_probes.js:2189-2207 */
         REAL x = rd[0], y = rd[1], z = rd[2];
@@ -953,6 +965,181 @@ void P(upload)(const sbgi_cascade_desc *c, uint32_t depth_oct, const sbgi_tick_p
                 datlas[o + 2] = 0;
                 datlas[o + 3] = 1;
             }
+        }
+    }
+}
+
+/* ---- reflection lobes (js/gi_probes.js:2358-2426, 2504-2522, 2538-2706) ---- */
+static REAL P(pow8)(REAL cd) {
+    REAL c2 = cd * cd, c4 = c2 * c2;
+    return c4 * c4;
+}
+static REAL P(pow64)(REAL cd) {
+    REAL c8 = P(pow8)(cd), c16 = c8 * c8, c32 = c16 * c16;
+    return c32 * c32;
+}
+static REAL P(ray_valid)(REAL ht, REAL sky_valid) {
+    return ht >= 0 ? RCONST(1) : (ht == RCONST(-1) ? sky_valid : RCONST(0));
+}
+static REAL P(sky_valid)(const sbgi_tick_params *p) {
+    return (REAL)(p->reflection_sky_fallback * p->sky_configured * (p->sky_intensity > 0.f ? 1.f : 0.f));
+}
+
+/* emitter e's virtual rays in a lobe of power POW (8 or 64) for texel direction dir:
+ * acc += L wk, hit += wk, wsum += wk (injectEmitterVirtualRays) */
+static void P(emitter_lobe)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays, const float *meta,
+                            float classify, const REAL *evis, uint32_t probe, const REAL dir[3], int pow64,
+                            REAL acc[3], REAL *hit, REAL *wsum) {
+    if (!s || !s->emitter_count) return;
+    REAL ro[3];
+    P(probe_pos)(c, probe, meta, classify, ro);
+    for (uint32_t e = 0; e < s->emitter_count; e++) {
+        const float *L = s->lights + (size_t)(s->emitter_base + e) * 17u;
+        REAL axis[3] = {(REAL)L[1] - ro[0], (REAL)L[2] - ro[1], (REAL)L[3] - ro[2]};
+        REAL dist = RSQRT(P(dot3)(axis, axis));
+        REAL radius = L[14] > 0 ? (REAL)L[14] : 0;
+        if (!(dist > (radius > RCONST(1e-3) ? radius : RCONST(1e-3)))) continue;
+        REAL d[3] = {axis[0] / dist, axis[1] / dist, axis[2] / dist};
+        REAL proj = L[15] > 0 ? (REAL)L[15] : 0;
+        REAL omega = proj / (dist * dist);
+        omega = omega < RCONST(6.283185307179586) ? omega : RCONST(6.283185307179586);
+        REAL kv = ((REAL)rays * omega) / RCONST(12.566370614359172);
+        kv = kv < (REAL)rays ? kv : (REAL)rays;
+        REAL vis = evis[(size_t)probe * SBGI_MAX_EMITTERS + e];
+        REAL cd = P(dot3)(dir, d);
+        cd = cd > 0 ? cd : 0;
+        REAL wk = (pow64 ? P(pow64)(cd) : P(pow8)(cd)) * kv;
+        for (int a = 0; a < 3; a++) acc[a] += ((REAL)L[7 + a] * vis) * wk;
+        *hit += wk;
+        *wsum += wk;
+    }
+}
+
+/* The rough power-8 lobe of every updated probe's 36 irradiance texels into rough (4
+ * floats per 8x8 tile texel). depth is the depth state BEFORE this update's blend: a probe
+ * whose texel moment is still < 1e-6 takes the new value outright. */
+void P(rough)(const sbgi_cascade_desc *c, uint32_t rays, uint32_t depth_oct, const sbgi_tick_params *p,
+              const REAL *ray_data, const REAL *depth, REAL *rough, const sbgi_scene_view *s, const float *meta,
+              const REAL *evis) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+    REAL *dirs = (REAL *)malloc(sizeof(REAL) * 3 * rays);
+    for (uint32_t k = 0; k < rays; k++) P(ray_dir)(k, rays, p->frame_jitter, dirs + 3 * k);
+    REAL sv = P(sky_valid)(p);
+    REAL raw = (REAL)p->hysteresis + (RCONST(1) - (REAL)p->hysteresis) * (REAL)p->dbg_noise_h_boost;
+    raw = raw < RCONST(1e-6) ? RCONST(1e-6) : (raw > 1 ? 1 : raw);
+    REAL steady = RPOW(raw, (REAL)p->hysteresis_exponent);
+    uint32_t td = depth_oct == 6 ? 8u : depth_oct + 2u;
+#pragma omp parallel for schedule(static)
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        const REAL *rd = ray_data + (size_t)slot * rays * 4;
+        for (uint32_t t = 0; t < 36; t++) {
+            uint32_t tx = t % 6, ty = t / 6;
+            REAL dir[3];
+            P(texel_dir)(tx, ty, 6, dir);
+            REAL acc[3] = {0, 0, 0}, hit = 0, wsum = 0;
+            for (uint32_t k = 0; k < rays; k++) {
+                const REAL *r = rd + k * 4;
+                REAL cd = P(dot3)(dir, dirs + 3 * k);
+                cd = cd > 0 ? cd : 0;
+                REAL sw = P(pow8)(cd);
+                REAL v = P(ray_valid)(r[3], sv);
+                for (int a = 0; a < 3; a++) acc[a] += (r[a] * sw) * v;
+                hit += sw * v;
+                wsum += sw;
+            }
+            P(emitter_lobe)(s, c, rays, meta, p->classify_strength, evis, probe, dir, 0, acc, &hit, &wsum);
+            REAL den = wsum > RCONST(1e-4) ? wsum : RCONST(1e-4);
+            REAL cur[4] = {acc[0] / den, acc[1] / den, acc[2] / den, hit / den};
+            cur[3] = cur[3] < 0 ? 0 : (cur[3] > 1 ? 1 : cur[3]);
+            REAL dprev = depth[((size_t)probe * td * td + (ty + 1) * td + (tx + 1)) * 2];
+            REAL sh = dprev < RCONST(1e-6) ? RCONST(0) : steady;
+            REAL *S = rough + ((size_t)probe * 64 + (ty + 1) * 8 + (tx + 1)) * 4;
+            for (int a = 0; a < 4; a++) S[a] = cur[a] * (RCONST(1) - sh) + S[a] * sh;
+        }
+    }
+    free(dirs);
+}
+
+/* glossyKernel: every tile texel of G x G (its direction that of its mirrored gutter
+ * source), texels with index % interval == glossy_phase; numerator and support history
+ * gnum (4 per texel) / gden, resolved into gatlas (RGBA rows, probe p at tile
+ * (p % tiles_x, p / tiles_x), values rounded to FP16 as the RGBA16F store does). */
+void P(glossy)(const sbgi_cascade_desc *c, uint32_t rays, const sbgi_tick_params *p, const REAL *ray_data,
+               uint32_t G, uint32_t interval, REAL *gnum, REAL *gden, REAL *gatlas, uint32_t tiles_x,
+               const sbgi_scene_view *s, const float *meta, const REAL *evis) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+    REAL *dirs = (REAL *)malloc(sizeof(REAL) * 3 * rays);
+    for (uint32_t k = 0; k < rays; k++) P(ray_dir)(k, rays, p->frame_jitter, dirs + 3 * k);
+    REAL sv = P(sky_valid)(p);
+    REAL inv = (REAL)(float)(1.0 / (double)rays);
+    REAL ref = (REAL)p->hysteresis + (RCONST(1) - (REAL)p->hysteresis) * (REAL)p->dbg_noise_h_boost;
+    ref = ref < RCONST(1e-6) ? RCONST(1e-6) : (ref > 1 ? 1 : ref);
+    REAL gh_full = RPOW(ref, (REAL)p->hysteresis_exponent * (REAL)interval);
+    uint32_t gaw = tiles_x * G;
+#pragma omp parallel for schedule(static)
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        const REAL *rd = ray_data + (size_t)slot * rays * 4;
+        for (uint32_t t = 0; t < G * G; t++) {
+            if (interval > 1 && t % interval != p->glossy_phase) continue;
+            uint32_t sx, sy;
+            P(gutter_src)(t % G, t / G, G, &sx, &sy);
+            REAL oct = (REAL)(G - 2);
+            REAL dir[3];
+            P(oct_decode)(((REAL)sx - RCONST(1) + RCONST(0.5)) / oct, ((REAL)sy - RCONST(1) + RCONST(0.5)) / oct, dir);
+            REAL acc[3] = {0, 0, 0}, hit = 0, wsum = 0;
+            for (uint32_t k = 0; k < rays; k++) {
+                const REAL *r = rd + k * 4;
+                REAL cd = P(dot3)(dir, dirs + 3 * k);
+                cd = cd > 0 ? cd : 0;
+                REAL gw = P(pow64)(cd);
+                REAL v = P(ray_valid)(r[3], sv);
+                for (int a = 0; a < 3; a++) acc[a] += (r[a] * gw) * v;
+                hit += gw * v;
+                wsum += gw;
+            }
+            P(emitter_lobe)(s, c, rays, meta, p->classify_strength, evis, probe, dir, 1, acc, &hit, &wsum);
+            size_t gt = (size_t)probe * G * G + t;
+            REAL *N = gnum + gt * 4;
+            REAL pd = gden[gt];
+            REAL cur[4] = {acc[0] * inv, acc[1] * inv, acc[2] * inv, hit * inv};
+            REAL cd_ = wsum * inv;
+            REAL gh = pd < RCONST(1e-6) ? RCONST(0) : gh_full;
+            for (int a = 0; a < 4; a++) N[a] = cur[a] * (RCONST(1) - gh) + N[a] * gh;
+            REAL den = cd_ * (RCONST(1) - gh) + pd * gh;
+            den = den > RCONST(1e-6) ? den : RCONST(1e-6);
+            gden[gt] = den;
+            uint32_t col = probe % tiles_x, row = probe / tiles_x;
+            size_t o = ((size_t)(row * G + t / G) * gaw + col * G + t % G) * 4;
+            REAL cov = N[3] / den;
+            cov = cov < 0 ? 0 : (cov > 1 ? 1 : cov);
+            gatlas[o + 0] = (REAL)half_round((float)(N[0] / den));
+            gatlas[o + 1] = (REAL)half_round((float)(N[1] / den));
+            gatlas[o + 2] = (REAL)half_round((float)(N[2] / den));
+            gatlas[o + 3] = (REAL)half_round((float)cov);
+        }
+    }
+    free(dirs);
+}
+
+/* the rough atlas: the irradiance atlas's tiles, unfiltered, mirrored gutters, FP16 */
+void P(upload_rough)(const sbgi_cascade_desc *c, const sbgi_tick_params *p, const REAL *rough, REAL *ratlas) {
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+    uint32_t updated = p->updated_count ? p->updated_count : total;
+    uint32_t aw = c->res[0] * 8u;
+    for (uint32_t slot = 0; slot < updated; slot++) {
+        uint32_t probe = (p->probe_offset + slot) % total;
+        uint32_t col = probe % c->res[0];
+        uint32_t row = (probe / (c->res[0] * c->res[1])) * c->res[1] + (probe / c->res[0]) % c->res[1];
+        for (uint32_t local = 0; local < 64; local++) {
+            uint32_t lx = local % 8, ly = local / 8, sx, sy;
+            P(gutter_src)(lx, ly, 8, &sx, &sy);
+            const REAL *S = rough + ((size_t)probe * 64 + sy * 8 + sx) * 4;
+            size_t o = ((size_t)(row * 8 + ly) * aw + col * 8 + lx) * 4;
+            for (int a = 0; a < 4; a++) ratlas[o + a] = (REAL)half_round((float)S[a]);
         }
     }
 }

@@ -32,6 +32,8 @@ BUF_BVH_NODES, BUF_TRI_INDEX, BUF_VERTEX_DATA, BUF_TRI_MATERIAL, BUF_MATERIALS, 
 BUF_IRR_STATE, BUF_DEPTH_STATE, BUF_META, BUF_IRR_ATLAS, BUF_DEPTH_ATLAS = 10, 11, 12, 13, 14
 BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT, BUF_TRI_PACKED, BUF_INST_BOUNDS, BUF_TLAS_SAH = 15, 16, 17, 18, 19, 20
 BUF_EMITTER_VIS, BUF_LIGHT_GRID = 21, 22
+BUF_ROUGH_STATE, BUF_ROUGH_ATLAS, BUF_GLOSSY_STATE, BUF_GLOSSY_WEIGHT, BUF_GLOSSY_ATLAS = 23, 24, 25, 26, 27
+REFLECT_OFF, REFLECT_ROUGH, REFLECT_HIGH, REFLECT_ULTRA = 0, 1, 2, 3
 
 # every symbol include/sbgi.h declares
 EXPORTS = [
@@ -57,7 +59,7 @@ class CascadeDesc(C.Structure):
 class FieldDesc(C.Structure):
     _fields_ = [("device", C.c_int32), ("rays_per_probe", C.c_uint32), ("cascade_count", C.c_uint32),
                 ("depth_oct_res", C.c_uint32), ("blend_precision", C.c_uint32), ("traversal", C.c_uint32),
-                ("flags", C.c_uint32), ("pad", C.c_uint32), ("cascades", CascadeDesc * 4)]
+                ("flags", C.c_uint32), ("reflections", C.c_uint32), ("cascades", CascadeDesc * 4)]
 
 
 class SceneView(C.Structure):
@@ -87,7 +89,8 @@ class TickParams(C.Structure):
                 + [(n, C.c_float) for n in TICK_FLOATS]
                 + [("sky_sh", C.c_float * 27)]
                 + [(n, C.c_float) for n in TICK_DEBUG]
-                + [("emitter_vis_seed", C.c_float), ("pad", C.c_uint32)])
+                + [("emitter_vis_seed", C.c_float), ("sky_configured", C.c_float),
+                   ("reflection_sky_fallback", C.c_float), ("glossy_phase", C.c_uint32), ("pad", C.c_uint32)])
 
 
 class Stats(C.Structure):

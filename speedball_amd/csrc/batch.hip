@@ -115,9 +115,11 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
                 BHIP_OK(b, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
                 BHIP_OK(b, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
                 BHIP_OK(b, launch_fill_u32(reinterpret_cast<uint32_t *>(cs.evis), 0u, cs.total * SBGI_MAX_EMITTERS, st));
+                BHIP_OK(b, clear_refl_atlases(cs, st));
             }
         BHIP_OK(b, launch_raydirs(f->dirs, c0.rays, p.frame_jitter, st));
         BHIP_OK(b, launch_weights(c0, p, f->dirs, f->bt, f->desc.blend_precision, st));
+        if (f->rtables) BHIP_OK(b, launch_weights_refl(c0.rays, f->dirs, f->rt, f->desc.blend_precision, st));
     }
     const uint32_t n_seg = (uint32_t)b->segs.size();
     // visibility masks, queue counts, fetch heads and done flags of every segment
@@ -140,9 +142,8 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
                                   b->rays[j], st));
         BHIP_OK(b, launch_emitter_vis(ts[j].s, ts[j].c, p, ts[j].meta, cs.evis, ts[j].w.queue.count + trace_heads(3),
                                       st));
-        BHIP_OK(b, launch_blend(ts[j].c, p, b->rays[j], f->bt, f->desc.blend_precision, cs.irr, cs.depth,
-                                emitter_inject(f, cs, p), st));
-        BHIP_OK(b, launch_upload(ts[j].c, p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st));
+        BHIP_OK(b, launch_reflect_and_blend(f, cs, ts[j].c, p, b->rays[j], st));
+        BHIP_OK(b, launch_upload(ts[j].c, p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st, cs.rough, cs.ratlas));
     }
     return SBGI_OK;
 }

@@ -368,6 +368,7 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
     if (desc->blend_precision > SBGI_BLEND_FP16) return fail(nullptr, SBGI_E_INVALID, "unknown blend precision");
     if (desc->traversal != SBGI_TRAV_MT_COMPAT && desc->traversal != SBGI_TRAV_MT_ORDERED)
         return fail(nullptr, SBGI_E_INVALID, "traversal must be MT_COMPAT or MT_ORDERED");
+    if (desc->reflections > SBGI_REFLECT_ULTRA) return fail(nullptr, SBGI_E_INVALID, "unknown reflection quality");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || desc->device < 0 || desc->device >= ndev)
         return fail(nullptr, SBGI_E_DEVICE, "HIP device not available");
@@ -415,6 +416,25 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         if (dalloc(&cs.evis, (size_t)cs.total * SBGI_MAX_EMITTERS * 4) != hipSuccess)
             return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
         (void)hipMemsetAsync(cs.evis, 0, (size_t)cs.total * SBGI_MAX_EMITTERS * 4, f->stream);
+        if (desc->reflections != SBGI_REFLECT_OFF) {
+            const size_t G = glossy_tile(desc->reflections), gt = (size_t)cs.total * G * G;
+            cs.gtiles_x = G ? std::max(1u, (uint32_t)std::ceil(std::sqrt((double)cs.total))) : 0;
+            cs.gtiles_y = G ? (cs.total + cs.gtiles_x - 1) / cs.gtiles_x : 0;
+            cs.gaw = cs.gtiles_x * (uint32_t)G;
+            cs.gah = cs.gtiles_y * (uint32_t)G;
+            const size_t ra = (size_t)cs.d.res[0] * 8 * cs.d.res[1] * cs.d.res[2] * 8;
+            if (dalloc(&cs.rough, (size_t)cs.total * 64 * 16) != hipSuccess || dalloc(&cs.ratlas, ra * 8) != hipSuccess ||
+                (G && (dalloc(&cs.gnum, gt * 16) != hipSuccess || dalloc(&cs.gden, gt * 4) != hipSuccess ||
+                       dalloc(&cs.gatlas, (size_t)cs.gaw * cs.gah * 8) != hipSuccess)))
+                return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+            (void)hipMemsetAsync(cs.rough, 0, (size_t)cs.total * 64 * 16, f->stream);
+            (void)hipMemsetAsync(cs.ratlas, 0, ra * 8, f->stream);
+            if (G) {
+                (void)hipMemsetAsync(cs.gnum, 0, gt * 16, f->stream);
+                (void)hipMemsetAsync(cs.gden, 0, gt * 4, f->stream);
+                (void)hipMemsetAsync(cs.gatlas, 0, (size_t)cs.gaw * cs.gah * 8, f->stream);
+            }
+        }
         const size_t depth_texels = desc->depth_oct_res == 6 ? 64 : (size_t)tile_d * tile_d;
         if (dalloc(&cs.irr, (size_t)cs.total * 64 * 16) != hipSuccess ||
             dalloc(&cs.depth, (size_t)cs.total * depth_texels * 8) != hipSuccess ||
@@ -457,6 +477,12 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
     (void)hipMemsetAsync(f->tables, 0, tb, f->stream);
     (void)hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream);
     blend_tables_carve(f->tables, R, tdn, f->bt);
+    if (desc->reflections != SBGI_REFLECT_OFF) {
+        const size_t rb = refl_tables_bytes(R, glossy_tile(desc->reflections));
+        if (dalloc(&f->rtables, rb) != hipSuccess) return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
+        (void)hipMemsetAsync(f->rtables, 0, rb, f->stream);
+        refl_tables_carve(f->rtables, R, glossy_tile(desc->reflections), f->rt);
+    }
     if (hipStreamSynchronize(f->stream) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "stream sync failed"));
     *out = f;
     return SBGI_OK;
@@ -859,19 +885,20 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
         HIP_OK(f, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
         HIP_OK(f, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
         HIP_OK(f, launch_fill_u32(reinterpret_cast<uint32_t *>(cs.evis), 0u, cs.total * SBGI_MAX_EMITTERS, st));
+        HIP_OK(f, clear_refl_atlases(cs, st));
     }
     HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
     HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
+    if (f->rtables) HIP_OK(f, launch_weights_refl(c.rays, f->dirs, f->rt, f->desc.blend_precision, st));
     HIP_OK(f, hipEventRecord(f->ev[1], st));
     HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
                            f->count_mode ? f->counters : nullptr, f->tev, st));
     HIP_OK(f, hipEventRecord(f->eev, st));
     HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.queue.count + trace_heads(3), st));
     HIP_OK(f, hipEventRecord(f->ev[2], st));
-    HIP_OK(f, launch_blend(c, *p, f->rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, emitter_inject(f, cs, *p),
-                           st));
+    HIP_OK(f, launch_reflect_and_blend(f, cs, c, *p, f->rays, st));
     HIP_OK(f, hipEventRecord(f->ev[3], st));
-    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st));
+    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st, cs.rough, cs.ratlas));
     HIP_OK(f, hipEventRecord(f->ev[4], st));
     f->ticket++;
     HIP_OK(f, hipEventRecord(f->ring[f->ticket % kTicketRing], st));
@@ -946,6 +973,18 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
             return cascade < f->desc.cascade_count ? (size_t)f->casc[cascade].total * SBGI_MAX_EMITTERS * 4 : 0;
         case SBGI_BUF_LIGHT_GRID: return light_grid_cells(f) * (SBGI_LIGHTS_PER_CELL + 1) * 4;
     }
+    if (buf_id >= SBGI_BUF_ROUGH_STATE && buf_id <= SBGI_BUF_GLOSSY_ATLAS) {
+        if (cascade >= f->desc.cascade_count || !f->casc[cascade].rough) return 0;
+        const CascadeState &cs = f->casc[cascade];
+        const size_t G = glossy_tile(f->desc.reflections);
+        switch (buf_id) {
+            case SBGI_BUF_ROUGH_STATE: return (size_t)cs.total * 64 * 16;
+            case SBGI_BUF_ROUGH_ATLAS: return (size_t)cs.aw * cs.ah * 8;
+            case SBGI_BUF_GLOSSY_STATE: return (size_t)cs.total * G * G * 16;
+            case SBGI_BUF_GLOSSY_WEIGHT: return (size_t)cs.total * G * G * 4;
+            case SBGI_BUF_GLOSSY_ATLAS: return (size_t)cs.gaw * cs.gah * 8;
+        }
+    }
     return 0;
 }
 
@@ -1001,6 +1040,17 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
         HIP_OK(f, hipMemcpy(dst, f->d_nodes, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
+    if (buf_id >= SBGI_BUF_ROUGH_STATE && buf_id <= SBGI_BUF_GLOSSY_ATLAS) {
+        const CascadeState &cs = f->casc[cascade];
+        src = buf_id == SBGI_BUF_ROUGH_STATE    ? (const void *)cs.rough
+              : buf_id == SBGI_BUF_ROUGH_ATLAS  ? (const void *)cs.ratlas
+              : buf_id == SBGI_BUF_GLOSSY_STATE ? (const void *)cs.gnum
+              : buf_id == SBGI_BUF_GLOSSY_WEIGHT ? (const void *)cs.gden
+                                                 : (const void *)cs.gatlas;
+        if (!src) return fail(f, SBGI_E_INVALID, "buffer not present at this reflection quality");
+        HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
     if (buf_id == SBGI_BUF_EMITTER_VIS || buf_id == SBGI_BUF_LIGHT_GRID) {
         src = buf_id == SBGI_BUF_EMITTER_VIS ? (const void *)f->casc[cascade].evis : (const void *)f->d_light_grid;
         HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
@@ -1012,17 +1062,21 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
 int32_t sbgi_write(sbgi_field *f, int32_t buf_id, uint32_t cascade, const void *src, size_t bytes) {
     if (!f || !src) return SBGI_E_INVALID;
     if (buf_id != SBGI_BUF_IRR_STATE && buf_id != SBGI_BUF_DEPTH_STATE && buf_id != SBGI_BUF_META &&
-        buf_id != SBGI_BUF_EMITTER_VIS)
-        return fail(f, SBGI_E_INVALID, "only irr/depth/meta/emitter-visibility state is writable");
+        buf_id != SBGI_BUF_EMITTER_VIS && buf_id != SBGI_BUF_ROUGH_STATE && buf_id != SBGI_BUF_GLOSSY_STATE &&
+        buf_id != SBGI_BUF_GLOSSY_WEIGHT)
+        return fail(f, SBGI_E_INVALID, "only the per-probe state buffers are writable");
     const size_t need = sbgi_buffer_bytes(f, buf_id, cascade);
     if (need == 0 || bytes != need) return fail(f, SBGI_E_INVALID, "buffer id/cascade/size mismatch");
     HIP_OK(f, hipSetDevice(f->desc.device));
     HIP_OK(f, hipStreamSynchronize(f->stream));
     const CascadeState &cs = f->casc[cascade];
-    void *dst = buf_id == SBGI_BUF_IRR_STATE     ? (void *)cs.irr
-                : buf_id == SBGI_BUF_DEPTH_STATE ? (void *)cs.depth
-                : buf_id == SBGI_BUF_META        ? (void *)cs.meta
-                                                 : (void *)cs.evis;
+    void *dst = buf_id == SBGI_BUF_IRR_STATE      ? (void *)cs.irr
+                : buf_id == SBGI_BUF_DEPTH_STATE  ? (void *)cs.depth
+                : buf_id == SBGI_BUF_META         ? (void *)cs.meta
+                : buf_id == SBGI_BUF_ROUGH_STATE  ? (void *)cs.rough
+                : buf_id == SBGI_BUF_GLOSSY_STATE ? (void *)cs.gnum
+                : buf_id == SBGI_BUF_GLOSSY_WEIGHT ? (void *)cs.gden
+                                                  : (void *)cs.evis;
     HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
     return SBGI_OK;
 }
@@ -1295,6 +1349,11 @@ int32_t sbgi_destroy(sbgi_field *f) {
         dfree(cs.atlas);
         dfree(cs.datlas);
         dfree(cs.evis);
+        dfree(cs.rough);
+        dfree(cs.ratlas);
+        dfree(cs.gnum);
+        dfree(cs.gden);
+        dfree(cs.gatlas);
     }
     dfree(f->rays);
     dfree(f->dirs);
@@ -1305,6 +1364,7 @@ int32_t sbgi_destroy(sbgi_field *f) {
     dfree(f->tables);
     dfree(f->counters);
     dfree(f->d_light_grid);
+    dfree(f->rtables);
     for (auto &e : f->ring)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : f->ev)

@@ -23,6 +23,10 @@ struct CascadeState {
     float *irr = nullptr, *depth = nullptr, *meta = nullptr;
     ushort4 *atlas = nullptr, *datlas = nullptr;
     float *evis = nullptr;  // retained emitter visibility [probe * SBGI_MAX_EMITTERS + e]
+    // reflection lobes (desc.reflections): rough state + atlas, glossy history + atlas
+    float *rough = nullptr, *gnum = nullptr, *gden = nullptr;
+    ushort4 *ratlas = nullptr, *gatlas = nullptr;
+    uint32_t gtiles_x = 0, gtiles_y = 0, gaw = 0, gah = 0;
 };
 
 constexpr int kTicketRing = 16;
@@ -113,6 +117,9 @@ struct sbgi_field {
     hipEvent_t ev[5] = {};
     hipEvent_t tev[4] = {};  // trace sub-stages
     hipEvent_t eev = nullptr;  // end of the trace, before the emitter visibility rays
+    // reflection lobe weight tables (desc.reflections != OFF)
+    void *rtables = nullptr;
+    ReflTables rt{};
     // receiver sampling (sbgi_sample_irradiance): staging of points, normals and results
     float *d_sample = nullptr;
     size_t sample_cap = 0;
@@ -189,7 +196,16 @@ inline DevCascade make_cascade(const sbgi_field *f, const CascadeState &cs, uint
     c.ah = cs.ah;
     c.daw = cs.daw;
     c.dah = cs.dah;
+    c.rough = f->desc.reflections != SBGI_REFLECT_OFF;
     return c;
+}
+
+// glossy tile (oct + 2 border) and update interval of the reflection quality
+inline uint32_t glossy_tile(uint32_t q) { return q == SBGI_REFLECT_ULTRA ? 18u : (q == SBGI_REFLECT_HIGH ? 10u : 0u); }
+inline uint32_t glossy_interval(uint32_t q) { return q == SBGI_REFLECT_HIGH ? 2u : 1u; }
+
+inline ReflState refl_state(const CascadeState &cs) {
+    return ReflState{cs.rough, cs.ratlas, cs.gnum, cs.gden, cs.gatlas, cs.gtiles_x, cs.gaw, cs.gah};
 }
 
 inline bool ordered(const sbgi_field *f) { return f->desc.traversal == SBGI_TRAV_MT_ORDERED; }
@@ -232,6 +248,29 @@ inline size_t light_grid_cells(const sbgi_field *f) { return (size_t)f->lg_cells
 inline EmitterInject emitter_inject(const sbgi_field *f, const CascadeState &cs, const sbgi_tick_params &p) {
     return EmitterInject{f->d_lights, cs.meta, cs.evis, p.classify_strength, f->view.emitter_base,
                          f->has_scene ? f->view.emitter_count : 0u};
+}
+
+// the reflection atlases' clear (the rough and glossy clears of clearAtlasKernel /
+// clearGlossyAtlasKernel, js/gi_probes.js:2690-2724): transparent black
+inline hipError_t clear_refl_atlases(const CascadeState &cs, hipStream_t st) {
+    hipError_t e = hipSuccess;
+    if (cs.ratlas) e = launch_fill_atlas(cs.ratlas, (size_t)cs.aw * cs.ah, make_ushort4(0, 0, 0, 0), st);
+    if (e == hipSuccess && cs.gatlas) e = launch_fill_atlas(cs.gatlas, (size_t)cs.gaw * cs.gah, make_ushort4(0, 0, 0, 0), st);
+    return e;
+}
+
+// rough lobe (reads the depth moments before the blend rewrites them), the irradiance and
+// depth blend, then the glossy cache
+inline hipError_t launch_reflect_and_blend(sbgi_field *f, const CascadeState &cs, const DevCascade &c,
+                                           const sbgi_tick_params &p, const float4 *rays, hipStream_t st) {
+    const EmitterInject em = emitter_inject(f, cs, p);
+    const ReflState rs = refl_state(cs);
+    hipError_t e = hipSuccess;
+    if (f->rtables) e = launch_rough(c, p, rays, f->rt, f->desc.blend_precision, cs.depth, rs, em, st);
+    if (e == hipSuccess) e = launch_blend(c, p, rays, f->bt, f->desc.blend_precision, cs.irr, cs.depth, em, st);
+    if (e == hipSuccess && f->rtables && f->rt.gtile)
+        e = launch_glossy(c, p, rays, f->rt, f->desc.blend_precision, glossy_interval(f->desc.reflections), rs, em, st);
+    return e;
 }
 
 }  // namespace sbgi

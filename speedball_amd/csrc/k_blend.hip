@@ -22,6 +22,7 @@
 #include <hip/hip_fp16.h>
 
 #include "kernels.h"
+#include "mfma_util.h"
 #include "shade.h"
 
 namespace {
@@ -30,20 +31,6 @@ constexpr uint32_t kIrrTexels = 36;
 
 __device__ __forceinline__ uint32_t kb_count(uint32_t rays) { return (rays + 31) / 32; }
 __device__ __forceinline__ uint32_t mt_count(uint32_t texels) { return (texels + 15) / 16; }
-
-__device__ __forceinline__ float3 oct_decode_uv(float u, float v) {
-    float fx = u * 2.f - 1.f, fy = v * 2.f - 1.f;
-    float nz = 1.f - fabsf(fx) - fabsf(fy);
-    float t = fmaxf(-nz, 0.f);
-    float nx = fx + (fx >= 0.f ? -t : t);
-    float ny = fy + (fy >= 0.f ? -t : t);
-    return normalize3(f3(nx, ny, nz));
-}
-
-__device__ __forceinline__ uint16_t to_bf16(float x) { return __bfloat16_as_ushort(__float2bfloat16(x)); }
-__device__ __forceinline__ float from_bf16(uint16_t h) { return __bfloat162float(__ushort_as_bfloat16(h)); }
-__device__ __forceinline__ uint16_t to_f16(float x) { return __half_as_ushort(__float2half_rn(x)); }
-__device__ __forceinline__ float from_f16(uint16_t h) { return __half2float(__ushort_as_half(h)); }
 
 // Weight tables. Block b < 36 owns irradiance texel b (and, with the shared 8x8 depth
 // tile, its depth weights); blocks 36.. own the 14x14 depth texels.
@@ -206,11 +193,6 @@ __device__ __forceinline__ size_t depth_index(const DevCascade &c, uint32_t prob
     return (size_t)probe * td * td + (t / n + 1) * td + (t % n + 1);
 }
 
-__device__ __forceinline__ float ray_dist(float ht, float max_dist) {
-    const float dec = ht < -1.5f ? -ht - 2.f : ht;
-    return ht == -1.f ? max_dist : dec;
-}
-
 constexpr int kGroup = 12;
 
 // direction of interior irradiance texel tt of the 6x6 octahedral interior
@@ -330,33 +312,6 @@ __global__ void __launch_bounds__(64) k_blend_fp32(DevCascade c, sbgi_tick_param
             const uint32_t tt = t0 + j;
             if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), a[j][0], a[j][1], t.norm_d[tt]);
         }
-    }
-}
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
-
-template <uint32_t PREC>
-__device__ __forceinline__ f32x4 mfma(u16x8 a, u16x8 b, f32x4 acc) {
-    if constexpr (PREC == SBGI_BLEND_FP16) {
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a), __builtin_bit_cast(f16x8, b), acc,
-                                                      0, 0, 0);
-    } else {
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
-                                                       acc, 0, 0, 0);
-    }
-}
-
-template <uint32_t PREC>
-__device__ __forceinline__ void split(float v, uint16_t &hi, uint16_t &lo) {
-    if constexpr (PREC == SBGI_BLEND_FP16) {
-        hi = to_f16(v);
-        lo = 0;
-    } else {
-        hi = to_bf16(v);
-        lo = PREC == SBGI_BLEND_BF16X3 ? to_bf16(v - from_bf16(hi)) : (uint16_t)0;
     }
 }
 

@@ -290,6 +290,16 @@ __global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sb
             const LightSample ls = light_gate(L, h, p, light_scale(s, li));
             if (ls.pass) rad = rad + light_contrib(L, h, ls, p);
         }
+        // with reflection lobes, a hit without Lambert or emissive radiance (pure metal or
+        // glass) keeps occluding depth but is marked as carrying no reflection radiance
+        // (js/gi_probes.js:2055-2062)
+        if (c.rough) {
+            const float4 *tp = s.tri + (size_t)tri * 3;
+            const float *mat = s.mats + (size_t)__float_as_uint(tp[1].w) * SBGI_MAT_STRIDE;
+            const float energy = (1.f - mat[4]) * (1.f - mat[5]);
+            const float peak = fmaxf(fmaxf(mat[7], mat[8]), mat[9]);
+            if (!(energy > 0.02f || peak > 1e-6f)) hit_t = -(hr.x + 2.f);
+        }
         // multibounce from the previous filtered atlas (js/gi_probes.js:2181-2188)
         const float3 smp = prev_atlas_f32 ? sample_atlas(c, prev_atlas_f32, h.hit, h.ng)
                                           : sample_atlas(c, prev_atlas, h.hit, h.ng);

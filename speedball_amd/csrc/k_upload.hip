@@ -26,7 +26,8 @@ __device__ __forceinline__ void gutter_src(uint32_t lx, uint32_t ly, uint32_t ti
 
 __global__ void __launch_bounds__(64) k_upload(DevCascade c, sbgi_tick_params p, const float *__restrict__ irr,
                                                const float *__restrict__ depth, ushort4 *__restrict__ atlas,
-                                               ushort4 *__restrict__ datlas, float4 *__restrict__ atlas_f32) {
+                                               ushort4 *__restrict__ datlas, float4 *__restrict__ atlas_f32,
+                                               const float *__restrict__ rough, ushort4 *__restrict__ ratlas) {
     __shared__ float4 cache[36];
     __shared__ float2 dcache[36];
     const uint32_t slot = blockIdx.x;
@@ -83,6 +84,11 @@ __global__ void __launch_bounds__(64) k_upload(DevCascade c, sbgi_tick_params p,
             const float2 d = dcache[src];
             datlas[o] = pack4(d.x, d.y, 0.f, 1.f);
         }
+        // the rough lobe: unfiltered, same mirrored gutter (js/gi_probes.js:2792-2799, 2841-2843)
+        if (ratlas) {
+            const float4 r = *reinterpret_cast<const float4 *>(rough + ((size_t)probe * 64 + sy * 8 + sx) * 4);
+            ratlas[o] = pack4(r.x, r.y, r.z, r.w);
+        }
     }
     if (!parity) {
         const uint32_t td = c.depth_oct + 2;
@@ -104,9 +110,11 @@ __global__ void k_clear_atlas(ushort4 *__restrict__ atlas, size_t texels) {
 }  // namespace
 
 hipError_t launch_upload(const DevCascade &c, const sbgi_tick_params &p, const float *irr, const float *depth,
-                         ushort4 *atlas, ushort4 *datlas, float4 *atlas_f32, hipStream_t st) {
+                         ushort4 *atlas, ushort4 *datlas, float4 *atlas_f32, hipStream_t st, const float *rough,
+                         ushort4 *ratlas) {
     if (c.updated == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_upload, dim3(c.updated), dim3(64), 0, st, c, p, irr, depth, atlas, datlas, atlas_f32);
+    hipLaunchKernelGGL(k_upload, dim3(c.updated), dim3(64), 0, st, c, p, irr, depth, atlas, datlas, atlas_f32, rough,
+                       ratlas);
     return hipGetLastError();
 }
 

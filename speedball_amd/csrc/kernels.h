@@ -114,10 +114,44 @@ struct EmitterInject {
 hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const BlendTables &t,
                         uint32_t precision, float *irr, float *depth, const EmitterInject &em, hipStream_t st);
 
+// k_reflect.hip: the rough and glossy reflection lobes
+struct ReflTables {
+    float *w_rough_t;                   // [R][36] max(dir_t . dir_k, 0)^8, k-major
+    float *norm_rough, *norm_rough_q;   // [36] sum_k W (exact, and of the quantized weights)
+    uint16_t *frag_rough;               // MFMA A fragments, hi then lo
+    float *w_gloss_t;                   // [R][G*G] ^64 over the glossy tile's texels
+    float *norm_gloss, *norm_gloss_q;   // [G*G]
+    uint16_t *frag_gloss;
+    uint32_t gtile;                     // glossy tile G (oct + 2), 0 without the glossy cache
+};
+// one cascade's reflection state: rough {rgb x coverage, coverage} per irradiance texel
+// and its atlas; glossy numerator / support per tile texel and the glossy atlas
+struct ReflState {
+    float *rough;
+    ushort4 *ratlas;
+    float *gnum, *gden;
+    ushort4 *gatlas;
+    uint32_t gtiles_x, gaw, gah;
+};
+size_t refl_tables_bytes(uint32_t rays, uint32_t gtile);
+void refl_tables_carve(void *base, uint32_t rays, uint32_t gtile, ReflTables &t);
+hipError_t launch_weights_refl(uint32_t rays, const float4 *dirs, const ReflTables &t, uint32_t precision,
+                               hipStream_t st);
+// before the irradiance blend: it reads the depth moments as the probe's seeded sentinel
+hipError_t launch_rough(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const ReflTables &t,
+                        uint32_t precision, const float *depth, const ReflState &s, const EmitterInject &em,
+                        hipStream_t st);
+// interval: 1 solves every glossy texel; 2 the texels with index % 2 == p.glossy_phase
+hipError_t launch_glossy(const DevCascade &c, const sbgi_tick_params &p, const float4 *rays, const ReflTables &t,
+                         uint32_t precision, uint32_t interval, const ReflState &s, const EmitterInject &em,
+                         hipStream_t st);
+hipError_t launch_fill_atlas(ushort4 *atlas, size_t texels, ushort4 v, hipStream_t st);
+
 // k_upload.hip
 // atlas_f32 (optional): also store the irradiance tiles unrounded (differentiable mode)
 hipError_t launch_upload(const DevCascade &c, const sbgi_tick_params &p, const float *irr, const float *depth,
-                         ushort4 *atlas, ushort4 *datlas, float4 *atlas_f32, hipStream_t st);
+                         ushort4 *atlas, ushort4 *datlas, float4 *atlas_f32, hipStream_t st,
+                         const float *rough = nullptr, ushort4 *ratlas = nullptr);
 hipError_t launch_clear_atlas(ushort4 *atlas, size_t texels, hipStream_t st);
 
 // k_scene.hip

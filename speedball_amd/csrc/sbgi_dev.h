@@ -73,6 +73,7 @@ struct DevCascade {
     float max_dist, cell_min;
     uint32_t aw, ah;      // irradiance atlas size in texels
     uint32_t daw, dah;    // depth atlas size
+    uint32_t rough;       // reflection lobes on: unshaded hits record -(t + 2) as their distance
 };
 
 // Fetch counters of the persistent walks: kFetchHeads per walk, kHeadStride uint32 words

@@ -363,6 +363,41 @@ def sample_leg(f, w, n=1 << 20, reps=5):
             "rel_err_vs_fp32": {"max": float(e.max()), "points_checked": k}}
 
 
+def reflect_leg(steps=10):
+    """Reflection lobes on the headline field (BF16x3 contractions): ms per update and the
+    blend stage (irradiance + depth blend, plus the rough lobe and the glossy cache when on)
+    for reflectionQuality off / rough / high / ultra, and the lobes' work per update."""
+    from speedball_amd import scenegen
+    from speedball_amd import _native as N
+    from speedball_amd.field import ProbeField
+    w = scenegen.instanced()
+    out = {"workload": "instanced (261514 triangles, 32x16x24 probes x 256 rays), bf16x3, ordered", "steps": steps}
+    for name, q in (("off", N.REFLECT_OFF), ("rough", N.REFLECT_ROUGH), ("high", N.REFLECT_HIGH),
+                    ("ultra", N.REFLECT_ULTRA)):
+        f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct,
+                       precision=PRECISIONS["bf16x3"], traversal=1, reflections=q)
+        f.set_scene(w.scene)
+        total, blend = [], []
+        for i in range(steps + 2):
+            f.tick(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis, glossy_phase=i % 2)
+            if i >= 2:
+                x = f.stats()
+                total.append(x.ms_total)
+                blend.append(x.ms_blend)
+        f.close()
+        G = {N.REFLECT_HIGH: 10, N.REFLECT_ULTRA: 18}.get(q, 0)
+        texels = 36 * (q != N.REFLECT_OFF) + G * G // (2 if q == N.REFLECT_HIGH else 1)
+        out[name] = {"ms_total": float(np.median(total)), "ms_blend_stage": float(np.median(blend)),
+                     "lobe_texels_per_probe": texels,
+                     "lobe_gflop": 2.0 * w.probes * w.rays * 4 * texels / 1e9}
+        log(f"reflections {name}: {out[name]}")
+    for name in ("rough", "high", "ultra"):
+        extra = out[name]["ms_blend_stage"] - out["off"]["ms_blend_stage"]
+        out[name]["lobe_ms"] = extra
+        out[name]["lobe_tflops"] = out[name]["lobe_gflop"] / max(extra, 1e-6)
+    return out
+
+
 def lights_leg(steps=10):
     """Clustered NEE and emitters (the reference's clusteredLighting and emitter path): the
     instanced scene with 48 point lights of range 8 m and 16 emissive-sphere emitters,
@@ -439,6 +474,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-fields leg")
     ap.add_argument("--batch-frames", type=int, default=10, help="timed frames of the batched-fields leg")
     ap.add_argument("--no-lights", action="store_true", help="skip the clustered-lights and emitters leg")
+    ap.add_argument("--no-reflect", action="store_true", help="skip the reflection-lobes leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
@@ -533,6 +569,7 @@ def main():
     dyn = None if args.no_dynamic else dynamic_leg(frames=args.dynamic_frames)
     bat = None if args.no_batch else batch_leg(frames=args.batch_frames)
     lights = None if args.no_lights else lights_leg()
+    reflect = None if args.no_reflect else reflect_leg()
     cpu = None if args.no_cpu else cpu_baseline(w)
 
     out = {
@@ -567,6 +604,7 @@ def main():
         "dynamic": dyn,
         "batch": bat,
         "lights": lights,
+        "reflections": reflect,
         "receiver_sampling": sampling,
         "cpu_baseline": cpu,
     }

@@ -31,6 +31,8 @@ function installSpeedballGI(opts) {
         continuous: opts.continuous !== false,
         autoDetectChanges: opts.autoDetectChanges !== false,
         precision: opts.precision, traversal: opts.traversal, device: opts.device,
+        roughReflections: opts.roughReflections, reflectionQuality: opts.reflectionQuality,
+        reflectionSkyFallback: opts.reflectionSkyFallback, clusteredLighting: opts.clusteredLighting,
         addon: opts.addon, addonPath: opts.addonPath,
     });
     if (opts.enabled !== false) gi.setEnabled(true);

@@ -5,7 +5,7 @@
 // tick() with an in-flight guard. Knobs that only affect the renderer's sampling
 // (reflection, intensity, Chebyshev strength, normal detail) are kept as state so
 // callers keep working; the renderer is outside this build.
-const { loadAddon, STATUS, BUF, BLEND, TRAVERSAL } = require('./addon');
+const { loadAddon, STATUS, BUF, BLEND, TRAVERSAL, REFLECTION } = require('./addon');
 const { buildScene, worldMatrix, sceneBounds } = require('./scene_ingest');
 
 // js/gi_probes.js:53-191 constants that shape the update
@@ -78,6 +78,15 @@ function createProbeField(opts) {
     const traversal = TRAVERSAL[opts.traversal || 'ordered'];
     // clusteredLighting (js/gi_probes.js:1097): per-cell light lists for the NEE loop
     const clustered = opts.clusteredLighting === true;
+    // reflectionQuality / roughReflections (resolveReflectionQuality, js/gi_probes.js:67-73)
+    const reflectionName = opts.reflectionQuality == null
+        ? (opts.roughReflections === true ? 'ultra' : 'off')
+        : String(opts.reflectionQuality).trim().toLowerCase();
+    if (!(reflectionName in REFLECTION)) {
+        throw new RangeError(`Unknown reflectionQuality "${opts.reflectionQuality}". Expected off, rough, high, or ultra.`);
+    }
+    const reflections = REFLECTION[reflectionName];
+    const glossyInterval = reflections === REFLECTION.high ? 2 : 1;
     const depthOct = opts.depthOct === 14 ? 14 : OCT_RES;
     const device = opts.device | 0;
 
@@ -86,7 +95,8 @@ function createProbeField(opts) {
         filterStrength: 1, filterSmooth: 0.5, radianceClamp: 8, depthSharpness: 7, skyIntensity: 1,
         skySH: new Float32Array(27), nirGate: 0, nirGain: 1, classifyStrength: 0,
         tempChangeSigma1: 2.5, tempChangeHDrop: 0.30, tempClampSigma: 6, frameJitter: 0, hysteresisExponent: 1,
-        emitterVisSeed: 0,
+        emitterVisSeed: 0, skyConfigured: 0,
+        reflectionSkyFallback: opts.reflectionSkyFallback === true ? 1 : 0,
     };
     // renderer-side knobs (state only)
     const R = {
@@ -160,7 +170,7 @@ function createProbeField(opts) {
             if (field) addon.fieldDestroy(field.h);
             const h = addon.fieldCreate({
                 device: device, rays: raysPerProbe, depthOct: depthOct, precision: blendPrecision, traversal: traversal,
-                flags: clustered ? FIELD_CLUSTERED : 0,
+                flags: clustered ? FIELD_CLUSTERED : 0, reflections: reflections,
                 cascades: grids.map((g) => ({
                     gridMin: Float32Array.from(g.gridMin), gridSize: Float32Array.from(g.gridSize),
                     res: Uint32Array.from(g.res),
@@ -169,7 +179,7 @@ function createProbeField(opts) {
             field = { h: h, rays: raysPerProbe };
             casc = grids.map((g) => ({
                 gridMin: g.gridMin, gridSize: g.gridSize, res: g.res, probeTotal: g.res[0] * g.res[1] * g.res[2],
-                probeCursor: 0, needsClear: true,
+                probeCursor: 0, needsClear: true, glossyPhase: 0,
             }));
         }
         addon.fieldSetScene(field.h, built.arrays);
@@ -246,6 +256,8 @@ function createProbeField(opts) {
             filterStrength: U.filterStrength, filterSmooth: U.filterSmooth, tempChangeSigma1: U.tempChangeSigma1,
             tempChangeHDrop: U.tempChangeHDrop, tempClampSigma: U.tempClampSigma, nirGate: U.nirGate, nirGain: U.nirGain,
             skyIntensity: U.skyIntensity, skySH: U.skySH, emitterVisSeed: U.emitterVisSeed,
+            skyConfigured: U.skyConfigured, reflectionSkyFallback: U.reflectionSkyFallback,
+            glossyPhase: C.glossyPhase || 0,
         };
         for (const k of Object.keys(debugKnobs)) p[k] = debugKnobs[k];
         return p;
@@ -283,6 +295,7 @@ function createProbeField(opts) {
             const C = casc[ci];
             const updated = Math.min(updatedCap(), C.probeTotal);
             const ticket = addon.fieldTick(field.h, tickParams(C, ci, updated));
+            C.glossyPhase = ((C.glossyPhase || 0) + 1) % glossyInterval;  // js/gi_probes.js:3831-3835
             C.needsClear = false;
             C.probeCursor = (C.probeCursor + updated) % C.probeTotal;
             inFlight = waitTicket(ticket).then(function (t) {
@@ -354,11 +367,7 @@ function createProbeField(opts) {
         getReflectionIntensity: function () { return R.reflectionIntensity; },
         setRoughnessLimit: function (v) { if (Number.isFinite(v)) R.roughnessLimit = v; },
         getRoughnessLimit: function () { return R.roughnessLimit; },
-        setReflectionSkyFallback: function (on) { R.reflectionSkyFallback = on === true; },
-        getReflectionSkyFallback: function () { return R.reflectionSkyFallback; },
-        hasRoughReflections: function () { return false; },
-        hasGlossyReflections: function () { return false; },
-        getReflectionQuality: function () { return 'off'; },
+        getReflectionQuality: function () { return reflectionName; },
         setChebyStrength: function (v) { if (Number.isFinite(v)) R.chebyStrength = v; },
         setNormalDetail: function (v) { if (Number.isFinite(v)) R.normalDetail = v; },
         setClassifyStrength: function (v) { if (Number.isFinite(v)) U.classifyStrength = clamp(v, 0, 1); },
@@ -413,6 +422,7 @@ function createProbeField(opts) {
         // coefficient) | { coefficients: [9 x {x,y,z}] } | a uniform colour
         setSky: function (sky) {
             const sh = new Float32Array(27);
+            U.skyConfigured = sky == null ? 0 : 1;  // js/gi_probes.js:1316
             if (sky == null) { U.skySH = sh; return; }
             if (ArrayBuffer.isView(sky) || Array.isArray(sky)) {
                 if (sky.length === 27) { for (let i = 0; i < 27; i++) sh[i] = sky[i]; U.skySH = sh; return; }
@@ -430,6 +440,10 @@ function createProbeField(opts) {
             }
         },
         setSkyIntensity: function (v) { if (Number.isFinite(v)) U.skyIntensity = Math.max(0, v); },
+        setReflectionSkyFallback: function (on) { U.reflectionSkyFallback = on === true ? 1 : 0; },
+        getReflectionSkyFallback: function () { return U.reflectionSkyFallback > 0.5; },
+        hasRoughReflections: function () { return reflections !== REFLECTION.off; },
+        hasGlossyReflections: function () { return reflections === REFLECTION.high || reflections === REFLECTION.ultra; },
         getSkyIntensity: function () { return U.skyIntensity; },
         setNirSensing: function (on) { U.nirGate = on ? 1 : 0; },
         getNirSensing: function () { return U.nirGate > 0.5; },

@@ -120,6 +120,7 @@ static napi_value js_field_create(napi_env env, napi_callback_info info) {
     d.blend_precision = (uint32_t)get_num(env, argv[0], "precision", SBGI_BLEND_FP32);
     d.traversal = (uint32_t)get_num(env, argv[0], "traversal", SBGI_TRAV_MT_COMPAT);
     d.flags = (uint32_t)get_num(env, argv[0], "flags", 0);
+    d.reflections = (uint32_t)get_num(env, argv[0], "reflections", SBGI_REFLECT_OFF);
     napi_value casc = get_prop(env, argv[0], "cascades");
     uint32_t n = 0;
     napi_get_array_length(env, casc, &n);
@@ -228,7 +229,10 @@ static void read_params(napi_env env, napi_value o, sbgi_tick_params *p) {
     F(dbg_filter_rel, "debugFilterRelScale");
     F(dbg_filter_eps, "debugFilterEpsScale");
     F(emitter_vis_seed, "emitterVisSeed");
+    F(sky_configured, "skyConfigured");
+    F(reflection_sky_fallback, "reflectionSkyFallback");
 #undef F
+    p->glossy_phase = (uint32_t)get_num(env, o, "glossyPhase", 0);
     size_t l;
     napi_typedarray_type t;
     const float *sh = get_typed(env, get_prop(env, o, "skySH"), &l, &t);

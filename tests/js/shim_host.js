@@ -237,6 +237,32 @@ tests.emitters_and_clustered_lighting = async () => {
     assert.ok(seeds[1] > 0 && seeds[2] > 0 && seeds[1] !== seeds[2]);
 };
 
+tests.reflection_quality_and_glossy_phase = async () => {
+    // reflectionQuality resolves like the reference; HIGH alternates the glossy phase per
+    // solve of a cascade; setSky marks the sky configured for the reflection fallback
+    assert.throws(() => shim.createProbeField({ scene: plainScene(), addon: recorder().fake, reflectionQuality: 'max' }),
+        RangeError);
+    const { fake, calls } = recorder();
+    const gi = shim.createProbeField({ scene: plainScene(), addon: fake, reflectionQuality: 'High', cascades: 1,
+        reflectionSkyFallback: true });
+    assert.strictEqual(gi.getReflectionQuality(), 'high');
+    assert.ok(gi.hasRoughReflections() && gi.hasGlossyReflections() && gi.getReflectionSkyFallback());
+    gi.setEnabled(true);
+    await gi.tick({});
+    gi.setSky([1, 1, 1]);
+    await gi.tick({});
+    await gi.tick({});
+    assert.strictEqual(calls.find((c) => c[0] === 'create')[1].reflections, 2);
+    const ticks = calls.filter((c) => c[0] === 'tick').map((c) => c[1]);
+    assert.deepStrictEqual(ticks.map((p) => p.glossyPhase), [0, 1, 0]);
+    assert.deepStrictEqual(ticks.map((p) => p.skyConfigured), [0, 1, 1]);
+    assert.ok(ticks.every((p) => p.reflectionSkyFallback === 1));
+    const off = shim.createProbeField({ scene: plainScene(), addon: recorder().fake, roughReflections: false });
+    assert.ok(!off.hasRoughReflections());
+    const ultra = shim.createProbeField({ scene: plainScene(), addon: recorder().fake, roughReflections: true });
+    assert.strictEqual(ultra.getReflectionQuality(), 'ultra');
+};
+
 tests.install_handle_forwards_and_guards = async () => {
     const { fake } = recorder();
     let errs = 0;

@@ -25,4 +25,4 @@ def test_js_shim_host_logic():
                        timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stderr
-    assert "passed 8" in r.stdout
+    assert "passed 9" in r.stdout

@@ -207,7 +207,8 @@ __device__ __forceinline__ void inject_tap(const EmitterTap &e, float3 tdir, flo
                                            float &l, float &l2, float &ws) {
 #pragma clang fp contract(off)
     const float cd = fmaxf(dot3(tdir, e.d), 0.f);
-    const float wk = powf(cd, cpow) * e.kv;
+    // pow(x, 1) is x exactly: the default cosine power skips the general pow
+    const float wk = (cpow == 1.f ? cd : powf(cd, cpow)) * e.kv;
     r += e.L.x * wk;
     g += e.L.y * wk;
     b += e.L.z * wk;
@@ -406,6 +407,15 @@ __global__ void __launch_bounds__(64) k_blend_mfma(DevCascade c, sbgi_tick_param
     if (!valid) return;
     const uint32_t probe = (c.offset + slot) % c.total;
     // D layout: lane holds rows 4*(lane/16) + jj of column lane%16
+    // depth first: its accumulators are then free for the irradiance epilogue
+#pragma unroll
+    for (int m = 0; m < MTD; m++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const uint32_t tt = m * 16 + g * 4 + jj;
+            if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), ad[m][0][jj], ad[m][1][jj], t.norm_d_q[tt]);
+        }
+    }
     float ws[MTI][4];
     if (EMIT) {
 #pragma unroll
@@ -447,14 +457,6 @@ __global__ void __launch_bounds__(64) k_blend_mfma(DevCascade c, sbgi_tick_param
                 irr_policy(p, irr, probe, local, ai[m][0][jj], ai[m][1][jj], ai[m][2][jj], ai[m][3][jj],
                            ai[m][4][jj], EMIT ? ws[m][jj] : t.norm_irr_q[tt]);
             }
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < MTD; m++) {
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            const uint32_t tt = m * 16 + g * 4 + jj;
-            if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), ad[m][0][jj], ad[m][1][jj], t.norm_d_q[tt]);
         }
     }
 }

@@ -287,7 +287,7 @@ def batch_leg(frames=10, autotune=True):
     tuned = None
     if autotune:
         tuning, block, tms = batch.autotune(params)
-        tuned = {"walk_tuning": tuning, "walk_block": block, "frame_ms": tms}
+        tuned = {"walk_tuning": tuning, "walk_block": block, "walk_waves_per_cu": batch.walk_waves, "frame_ms": tms}
         log(f"batch: autotuned {tuning} block {block} ({tms:.2f} ms eager)")
     batch.tick(params)  # capture
     ms, wall, rep = [], [], []

@@ -455,8 +455,13 @@ int32_t sbgi_batch_wait(sbgi_batch *b);
 int32_t sbgi_batch_timing(sbgi_batch *b, float *ms, int32_t *replayed);
 /* Threads per block of the batched walks: 256 or 512. */
 int32_t sbgi_batch_set_walk_block(sbgi_batch *b, uint32_t block);
-/* Autotuner: coordinate descent over the walk schedule (shared by the batch's fields) and
- * the walk block size, timing eager frames; keeps and returns the fastest. */
+/* Resident waves per CU of the batched walks: 0 (as many as registers and LDS allow) or
+ * 4..64. Fewer waves keep a larger share of L2 per wave on scenes whose nodes do not fit. */
+int32_t sbgi_batch_set_walk_waves(sbgi_batch *b, uint32_t waves_per_cu);
+int32_t sbgi_batch_get_walk_waves(const sbgi_batch *b, uint32_t *waves_per_cu);
+/* Autotuner: coordinate descent over the walk schedule (shared by the batch's fields), the
+ * walk block size and the resident waves, timing eager frames; keeps and returns the
+ * fastest (the waves through sbgi_batch_get_walk_waves). */
 int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_walk_tuning *best,
                             uint32_t *best_block, float *best_ms);
 const char *sbgi_batch_last_error(const sbgi_batch *b);

@@ -46,7 +46,7 @@ EXPORTS = [
     "sbgi_diff_adjoint", "sbgi_diff_timing", "sbgi_get_walk_tuning", "sbgi_set_walk_tuning", "sbgi_autotune",
     "sbgi_set_skins", "sbgi_skin", "sbgi_edit_timing", "sbgi_sb_mesh_ranges",
     "sbgi_batch_create", "sbgi_batch_tick", "sbgi_batch_wait", "sbgi_batch_timing", "sbgi_batch_set_walk_block",
-    "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy", "sbgi_sb_add_materials",
+    "sbgi_batch_set_walk_waves", "sbgi_batch_get_walk_waves", "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy", "sbgi_sb_add_materials",
     "sbgi_rebuild_blas", "sbgi_sample_params_default", "sbgi_sample_irradiance",
 ]
 
@@ -196,6 +196,8 @@ def lib() -> C.CDLL:
         "sbgi_batch_wait": (i32, [vp]),
         "sbgi_batch_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(i32)]),
         "sbgi_batch_set_walk_block": (i32, [vp, u32]),
+        "sbgi_batch_set_walk_waves": (i32, [vp, u32]),
+        "sbgi_batch_get_walk_waves": (i32, [vp, C.POINTER(u32)]),
         "sbgi_batch_autotune": (i32, [vp, C.POINTER(TickParams), C.POINTER(WalkTuning), C.POINTER(u32),
                                       C.POINTER(C.c_float)]),
         "sbgi_batch_last_error": (C.c_char_p, [vp]),

@@ -46,6 +46,7 @@ struct sbgi_batch {
     std::vector<uint32_t> qcap;
     uint32_t total_rays = 0;
     uint32_t walk_block = 512;
+    uint32_t walk_waves = 0;  // resident waves per CU of the batched walks (0: occupancy limit)
     // capture state: the argument blob of the captured frame
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -126,13 +127,14 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
     BHIP_OK(b, launch_fill_u32(reinterpret_cast<uint32_t *>(b->vis[0]), 0u, 2u * b->total_rays, st));
     BHIP_OK(b, launch_fill_u32(b->d_ctr, 0u, n_seg * (uint32_t)(kTraceCounterBytes / 4), st));
     BHIP_OK(b, launch_fill_u32(b->d_done, 0u, 2u * n_seg, st));
-    BHIP_OK(b, launch_walk_batch(0, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done, b->walk_block, st));
+    BHIP_OK(b, launch_walk_batch(0, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done, b->walk_block,
+                                 b->walk_waves, st));
     for (uint32_t j = 0; j < n_seg; j++) {
         const sbgi_tick_params &p = params[b->segs[j].field];
         BHIP_OK(b, launch_emit(ts[j].s, ts[j].c, p, ts[j].dirs, ts[j].meta, ts[j].w.hits, ts[j].w.queue, st));
     }
     BHIP_OK(b, launch_walk_batch(1, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done + n_seg, b->walk_block,
-                                 st));
+                                 b->walk_waves, st));
     for (uint32_t j = 0; j < n_seg; j++) {
         const auto &g = b->segs[j];
         sbgi_field *f = b->fields[g.field];
@@ -160,6 +162,7 @@ void frame_key(const sbgi_batch *b, const sbgi_tick_params *params, const std::v
     put(params, sizeof(sbgi_tick_params) * b->fields.size());
     put(ts.data(), sizeof(TraceSeg) * ts.size());
     put(&b->walk_block, sizeof(b->walk_block));
+    put(&b->walk_waves, sizeof(b->walk_waves));
     for (const sbgi_field *f : b->fields) put(&f->bt, sizeof(f->bt));
 }
 
@@ -351,6 +354,18 @@ int32_t sbgi_batch_set_walk_block(sbgi_batch *b, uint32_t block) {
     return SBGI_OK;
 }
 
+int32_t sbgi_batch_set_walk_waves(sbgi_batch *b, uint32_t waves_per_cu) {
+    if (!b || (waves_per_cu != 0 && (waves_per_cu < 4 || waves_per_cu > 64))) return SBGI_E_INVALID;
+    b->walk_waves = waves_per_cu;
+    return SBGI_OK;
+}
+
+int32_t sbgi_batch_get_walk_waves(const sbgi_batch *b, uint32_t *waves_per_cu) {
+    if (!b || !waves_per_cu) return SBGI_E_INVALID;
+    *waves_per_cu = b->walk_waves;
+    return SBGI_OK;
+}
+
 int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_walk_tuning *best,
                             uint32_t *best_block, float *best_ms) {
     if (!b || !params) return SBGI_E_INVALID;
@@ -361,16 +376,18 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
         BHIP_OK(b, hipEventRecord(b->join, f->stream));
         BHIP_OK(b, hipStreamWaitEvent(b->stream, b->join, 0));
     }
-    // one schedule for every field of the batch, and the walk block ("tile") size; each
-    // candidate is timed on eager frames (the temporal state advances, as with ticks)
+    // one schedule for every field of the batch, the walk block ("tile") size and the
+    // resident waves per CU; each candidate is timed on eager frames (the temporal state
+    // advances, as with ticks)
     std::vector<sbgi_tick_params> p(params, params + b->fields.size());
     for (auto &q : p) q.needs_clear = 0;
-    auto apply = [&](const sbgi_walk_tuning &t, uint32_t block) {
+    auto apply = [&](const sbgi_walk_tuning &t, uint32_t block, uint32_t waves) {
         for (sbgi_field *f : b->fields) f->tuning = t;
         b->walk_block = block;
+        b->walk_waves = waves;
     };
-    auto time_it = [&](const sbgi_walk_tuning &t, uint32_t block, float &ms) -> int32_t {
-        apply(t, block);
+    auto time_it = [&](const sbgi_walk_tuning &t, uint32_t block, uint32_t waves, float &ms) -> int32_t {
+        apply(t, block, waves);
         float best_run = 1e30f;
         for (int rep = 0; rep < 2; rep++) {
             int32_t rc = run_frame(b, p.data(), false);
@@ -384,34 +401,37 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
         return SBGI_OK;
     };
     sbgi_walk_tuning cur = b->fields[0]->tuning;
-    uint32_t block = b->walk_block;
+    uint32_t block = b->walk_block, waves = b->walk_waves;
     float cur_ms = 0.f;
-    int32_t rc = time_it(cur, block, cur_ms);
+    int32_t rc = time_it(cur, block, waves, cur_ms);
     if (rc != SBGI_OK) return rc;
-    static const uint32_t ratios[] = {2, 4, 8}, refills[] = {4, 8, 16}, chunks[] = {32, 64, 128}, blocks[] = {256, 512};
-    for (int param = 0; param < 5; param++) {
+    static const uint32_t ratios[] = {2, 4, 8}, refills[] = {4, 8, 16}, chunks[] = {32, 64, 128}, blocks[] = {256, 512},
+                          wave_caps[] = {0, 16, 8};
+    for (int param = 0; param < 6; param++) {
         const int nc = param == 4 ? 2 : 3;
         for (int i = 0; i < nc; i++) {
             sbgi_walk_tuning t = cur;
-            uint32_t bl = block;
+            uint32_t bl = block, wv = waves;
             uint32_t v = 0;
             switch (param) {
                 case 0: v = t.tri_ratio = ratios[i]; if (cur.tri_ratio == v) continue; break;
                 case 1: v = t.inst_ratio = ratios[i]; if (cur.inst_ratio == v) continue; break;
                 case 2: v = t.refill_min = refills[i]; if (cur.refill_min == v) continue; break;
                 case 3: v = t.fetch_chunk = chunks[i]; if (cur.fetch_chunk == v) continue; break;
-                default: bl = blocks[i]; if (bl == block) continue; break;
+                case 4: bl = blocks[i]; if (bl == block) continue; break;
+                default: wv = wave_caps[i]; if (wv == waves) continue; break;
             }
             float ms = 0.f;
-            if ((rc = time_it(t, bl, ms)) != SBGI_OK) return rc;
+            if ((rc = time_it(t, bl, wv, ms)) != SBGI_OK) return rc;
             if (ms < cur_ms * 0.99f) {
                 cur = t;
                 block = bl;
+                waves = wv;
                 cur_ms = ms;
             }
         }
     }
-    apply(cur, block);
+    apply(cur, block, waves);
     BHIP_OK(b, hipEventRecord(b->join, b->stream));
     for (sbgi_field *f : b->fields) BHIP_OK(b, hipStreamWaitEvent(f->stream, b->join, 0));
     if (best) *best = cur;

@@ -77,7 +77,7 @@ struct ShadowSrc {
 // Emitter visibility (emitterVisKernel, js/gi_probes.js:2216-2260): ray i = slot * E + e
 // tests the probe's view of a hashed point on emitter e; the store blends the 0/1 sample
 // into the retained visibility, vis = mix(sample, vis, 0.8). A probe inside the emitter's
-// bounding sphere walks nothing (its lane starts past the TLAS end) and samples 0.
+// bounding sphere walks nothing (its extent is NaN, so no box test passes) and samples 0.
 struct EmitterSrc {
     DevCascade c;
     float classify, seed, bias;
@@ -92,8 +92,8 @@ struct EmitterSrc {
         float tmax = 0.f;
         const bool valid = emitter_ray(s.lights + (size_t)(ebase + e) * SBGI_LIGHT_STRIDE, ro, probe, e, seed, bias,
                                        rd, tmax);
-        lane_begin<true>(s, r, ro, rd, tmax);
-        if (!valid) r.cur = r.end;
+        // an invalid ray's extent is NaN: its root box test fails, so it retires after one step
+        lane_begin<true>(s, r, ro, rd, valid ? tmax : __builtin_nanf(""));
         r.tag = (probe * SBGI_MAX_EMITTERS + e) | (valid ? 0u : 0x80000000u);
     }
     __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
@@ -168,7 +168,10 @@ template <int MODE, int BLOCK, class Src>
 __global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__restrict__ segs, uint32_t n_seg,
                                                       const uint32_t *__restrict__ prefix, uint32_t total,
                                                       uint32_t *__restrict__ done) {
-    const uint32_t waves = gridDim.x * (BLOCK / 64u), wave_id = blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6);
+    // wave-uniform by construction; readfirstlane tells the compiler, so the segment table
+    // is read with scalar loads into SGPRs instead of per lane on every step
+    const uint32_t waves = gridDim.x * (BLOCK / 64u);
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6));
     const uint32_t target = (uint32_t)(((uint64_t)total * wave_id) / waves);
     uint32_t lo = 0, hi = n_seg;  // last segment whose prefix <= target
     while (hi - lo > 1) {
@@ -182,7 +185,11 @@ __global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__rest
         if (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
         const WalkSeg<Src> &g = segs[j];
         const uint32_t n = g.n_dev ? min(*g.n_dev, g.n_host) : g.n_host;
-        walk_wave<MODE, false, false>(g.s, g.src, n, g.heads, wave_id, cnt);
+        // copies read once per segment: the walk's stores cannot then force a reload of the
+        // scene and source fields on every step
+        const DevScene sc = g.s;
+        const Src src = g.src;
+        walk_wave<MODE, false, false>(sc, src, n, g.heads, wave_id, cnt);
         if ((threadIdx.x & 63u) == 0) __hip_atomic_store(&done[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -364,11 +371,13 @@ int device_cus() {
 
 // Persistent grid: as many blocks as can be resident at once (occupancy API), capped by
 // the work. Every block then strides over the work, so no block waits for a slot.
+// waves_cap > 0 limits the resident waves per CU (at least one block).
 template <typename K>
-uint32_t persistent_blocks(K kernel, uint32_t block, size_t lds, uint32_t work_items) {
+uint32_t persistent_blocks(K kernel, uint32_t block, size_t lds, uint32_t work_items, uint32_t waves_cap = 0) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, lds) != hipSuccess || per_cu <= 0)
         per_cu = 1;
+    if (waves_cap) per_cu = min(per_cu, max(1, (int)(waves_cap / (block / 64u))));
     const uint32_t cap = (uint32_t)device_cus() * (uint32_t)per_cu;
     const uint32_t want = (work_items + block - 1) / block;
     return want < cap ? (want ? want : 1u) : cap;
@@ -394,14 +403,14 @@ void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char
 }
 
 hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const uint32_t *prefix, uint32_t total,
-                             uint32_t *done, uint32_t block, hipStream_t st) {
+                             uint32_t *done, uint32_t block, uint32_t waves, hipStream_t st) {
     if (!n_seg || !total) return hipSuccess;
     const auto *pr = static_cast<const WalkSeg<PrimarySrc> *>(tables);
     const auto *sh = reinterpret_cast<const WalkSeg<ShadowSrc> *>(static_cast<const unsigned char *>(tables) +
                                                                  (size_t)n_seg * sizeof(WalkSeg<PrimarySrc>));
     auto go = [&](auto kern, uint32_t bs, auto table) {
-        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, bs, 0, total)), dim3(bs), 0, st, table, n_seg, prefix,
-                           total, done);
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, bs, 0, total, waves)), dim3(bs), 0, st, table, n_seg,
+                           prefix, total, done);
     };
     if (mode == 0) {
         if (block == 256) go(k_walk_batch<WALK_CLOSEST, 256, PrimarySrc>, 256u, pr);

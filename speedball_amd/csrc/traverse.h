@@ -1,6 +1,6 @@
 // traverse.h — shared pieces of the two-level walk: traversal counters, TLAS node and
 // instance-record access (LDS or global), the instance transform of the ray, the node
-// copy a ray walks, the walk budget, and the LDS staging of the TLAS.
+// copy a ray walks, and the LDS staging of the TLAS.
 //
 // LDS=true reads the TLAS nodes and instance records from the block's LDS copy
 // (g_lds: 2 uint4 per TLAS node followed by 4 uint4 per instance).
@@ -77,10 +77,6 @@ __device__ __forceinline__ float3 inst_dir(float4 r0, float4 r1, float4 r2, floa
 __device__ __forceinline__ uint32_t octant_of(float3 rd, uint32_t octants) {
     if (octants < 8) return 0u;
     return (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
-}
-
-__device__ __forceinline__ uint64_t walk_budget(const DevScene &s) {
-    return (uint64_t)s.tlas_count * 2 + (uint64_t)s.node_count * 256 + 1024;
 }
 
 // Copy the TLAS nodes and instance records into the block's LDS (LDS=true walks).

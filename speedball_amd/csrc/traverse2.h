@@ -16,6 +16,10 @@
 //   INST  enter the next active instance of the pending TLAS leaf (local ray, inverse
 //         direction, BLAS range), or return to the TLAS when the leaf is exhausted.
 //   DONE  the ray has its answer and waits to be retired; IDLE: no ray.
+// A lane is in NODE only while cur < end: the step that moves the cursor past the end of
+// its level takes the level exit itself (BLAS: INST, TLAS: DONE), so no step is spent on
+// an exhausted level. The cursor only moves forward (a threaded layout's i+1 and miss links
+// are larger than i, and the step clamps them so), which bounds every walk.
 // A wave runs the NODE phase whenever a lane needs it and the TRI / INST phases only
 // when enough lanes are waiting in them (or nothing else is left), so the expensive
 // phases run with most lanes busy instead of once per straggler. Finished lanes are
@@ -37,7 +41,6 @@ struct LaneRay {
     uint32_t inst;        // instance being walked
     uint32_t level;       // 0 TLAS, 1 BLAS
     uint32_t state;
-    uint32_t steps;       // walk budget guard
     float dsign;
     int hit_tri, hit_inst;
     uint32_t tag;         // caller's payload (shadow rays: (ray << 6) | light)
@@ -56,7 +59,7 @@ __device__ __forceinline__ float3 ray_inv(float3 d) {
     return f3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
 }
 
-// Start a walk at the TLAS root.
+// Start a walk at the TLAS root (a scene has at least one TLAS node: the upload checks it).
 template <bool RCP = false>
 __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3 ro, float3 rd, float tmax) {
     r.wo = ro;
@@ -75,7 +78,6 @@ __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3
     r.inst = 0u;
     r.level = 0u;
     r.state = ST_NODE;
-    r.steps = 0u;
     r.dsign = 1.f;
     r.hit_tri = -1;
     r.hit_inst = -1;
@@ -90,44 +92,45 @@ enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
 // merged by the compiler into one store through a pointer phi, which forces the lane
 // state out of registers into scratch memory.
 
-// NODE: one box test at the current level. Both levels use the 32-byte node
+// NODE: one box test at the current level (cur < end). Both levels use the 32-byte node
 // {min.xyz, miss}, {max.xyz, payload} with payload 0xFFFFFFFF for interior nodes, else
 // (count << 24) | offset (triangles for a BLAS leaf, instance slots for a TLAS leaf).
+// BLAS nodes are addressed by a 32-bit byte offset from the pool: the upload limits a
+// copy to fewer than 2^24 nodes, so 8 copies stay below 4 GiB.
 template <bool COUNT, bool LDS>
 __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
-    const bool in_range = r.cur < r.end;
-    uint4 a = make_uint4(0u, 0u, 0u, r.cur), b = make_uint4(0u, 0u, 0u, 0xFFFFFFFFu);
-    if (in_range) {
-        if (r.level) {
-            const uint4 *n = s.nodes + ((size_t)r.base + r.cur) * 2;
-            a = n[0];
-            b = n[1];
-            cnt.add_blas();
-        } else {
-            tlas_node<LDS>(s, r.cur, a, b);
-            cnt.add_tlas();
-        }
+    uint4 a, b;
+    if (r.level) {
+        const uint4 *n = reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(s.nodes) +
+                                                         ((r.base + r.cur) << 5));
+        a = n[0];
+        b = n[1];
+        cnt.add_blas();
+    } else {
+        tlas_node<LDS>(s, r.cur, a, b);
+        cnt.add_tlas();
     }
     const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
     const float3 bmax = f3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
     float tn, tf;
     slab(bmin, bmax, r.o, r.id, tn, tf);
-    const bool hit = in_range && tf >= fmaxf(tn, 0.f) && tn < r.tmax;
-    const bool interior = b.w == 0xFFFFFFFFu;
-    const bool leaf = hit && !interior;
+    const bool hit = tf >= fmaxf(tn, 0.f) && tn < r.tmax;
+    const bool descend = hit && b.w == 0xFFFFFFFFu;
     const uint32_t off = b.w & 0x00FFFFFFu, end = off + (b.w >> 24);
     // an empty leaf (count 0) is passed over like a miss: TRI / INST never see an empty range
-    const bool blas_leaf = leaf && r.level && end > off, tlas_leaf = leaf && !r.level && end > off;
-    uint32_t state = r.state;
-    if (!in_range) state = r.level ? ST_INST : ST_DONE;
-    else if (blas_leaf) state = ST_TRI;
-    else if (tlas_leaf) state = ST_INST;
-    r.cur = in_range ? ((hit && interior) ? r.cur + 1u : a.w) : r.cur;
+    const bool leaf = hit && !descend && end > off;
+    // i+1 on a descent, else the miss link; masked rather than selected, so the compiler
+    // cannot sink the miss link's load into a branch of its own (a second round trip)
+    const uint32_t next = max(a.w & (descend ? 0u : ~0u), r.cur + 1u);
+    // leaf: TRI (BLAS) or INST (TLAS); exhausted level: INST (BLAS) or DONE (TLAS)
+    const uint32_t state = leaf ? ST_INST - r.level : (next < r.end ? ST_NODE : ST_DONE - r.level);
+    const bool blas_leaf = leaf && r.level, tlas_leaf = leaf && !r.level;
+    r.cur = next;
     r.toff = blas_leaf ? off : r.toff;
     r.tend = blas_leaf ? end : r.tend;
     r.inext = tlas_leaf ? off : r.inext;
     r.iend = tlas_leaf ? end : r.iend;
-    r.tres = tlas_leaf ? a.w : r.tres;
+    r.tres = tlas_leaf ? next : r.tres;
     r.state = state;
 }
 
@@ -148,7 +151,8 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
     r.hit_tri = better ? (int)r.toff : r.hit_tri;
     r.hit_inst = better ? (int)r.inst : r.hit_inst;
     r.blocked = r.blocked || block;
-    r.state = block ? ST_DONE : (next >= r.tend ? ST_NODE : ST_TRI);
+    // leaf exhausted: on to the node after it, or back to INST when the BLAS is done
+    r.state = block ? ST_DONE : (next < r.tend ? ST_TRI : (r.cur < r.end ? ST_NODE : ST_INST));
     r.toff = next;
 }
 
@@ -159,7 +163,8 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
     // default: back to the TLAS in world space
     float3 o = r.wo, d = r.wd, id = r.wid;
     float dsign = r.dsign;
-    uint32_t cur = r.tres, end = s.tlas_count, base = r.base, inst = r.inst, level = 0u, state = ST_NODE;
+    uint32_t cur = r.tres, end = s.tlas_count, base = r.base, inst = r.inst, level = 0u;
+    uint32_t state = r.tres < s.tlas_count ? ST_NODE : ST_DONE;
     if (enter) {
         inst = r.inext;
         cnt.add_inst();
@@ -174,6 +179,7 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
             end = (uint32_t)r3.y;
             base = octant_of(d, s.blas_octants) * s.node_count;
             level = 1u;
+            state = cur < end ? ST_NODE : ST_INST;  // an empty range: on to the next slot
         } else {  // inactive slot: stay in INST for the next one
             o = r.o;
             d = r.d;
@@ -218,8 +224,6 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
                                           uint32_t wave_id, TravCounters<COUNT> &cnt) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = lane ? (~0ull >> (64u - lane)) : 0ull;
-    const uint64_t wb = walk_budget(s);
-    const uint32_t budget = wb > 0x7FFFFFFFull ? 0x7FFFFFFFu : (uint32_t)wb;
     LaneRay r;
     r.state = ST_DONE;
     uint32_t idx = 0;
@@ -269,7 +273,6 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
             live = __ballot(r.state < ST_DONE);
         }
         if (!live) break;
-        if (r.state < ST_DONE && ++r.steps > budget) r.state = ST_DONE;
         if (__ballot(r.state == ST_NODE)) {
             if (COUNT && lane == 0) cnt.node_w++;
             if (r.state == ST_NODE) node_step<COUNT, LDS>(s, r, cnt);

@@ -328,6 +328,17 @@ class Batch:
     def set_walk_block(self, block: int):
         self._chk(self.L.sbgi_batch_set_walk_block(self.h, block))
 
+    @property
+    def walk_waves(self) -> int:
+        """Resident waves per CU of the batched walks (0: as many as fit)."""
+        w = C.c_uint32()
+        self._chk(self.L.sbgi_batch_get_walk_waves(self.h, C.byref(w)))
+        return w.value
+
+    @walk_waves.setter
+    def walk_waves(self, waves: int):
+        self._chk(self.L.sbgi_batch_set_walk_waves(self.h, waves))
+
     def autotune(self, params):
         arr = self._params(params)
         for i in range(len(self.fields)):

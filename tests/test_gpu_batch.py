@@ -90,8 +90,8 @@ def test_batch_frames_equal_individual_ticks(specs):
 
 
 def test_batch_autotune_keeps_results(specs):
-    """The autotuner's schedule (walk tuning + block size), applied to a fresh batch, must
-    leave every frame equal to the individually ticked twin fields."""
+    """The autotuner's schedule (walk tuning, block size, resident waves), applied to a
+    fresh batch, must leave every frame equal to the individually ticked twin fields."""
     params = [dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis) for w, *_ in specs]
     tuned = make_fields(specs)
     batch = Batch(tuned)
@@ -102,20 +102,25 @@ def test_batch_autotune_keeps_results(specs):
         print("batch autotune:", tuning, block, f"{ms:.3f} ms")
         assert block in (256, 512) and ms > 0 and all(v >= 1 for v in tuning.values())
         assert all(f.walk_tuning() == tuning for f in tuned)
+        waves = batch.walk_waves
+        assert waves in (0, 8, 16)
+        with pytest.raises(RuntimeError):
+            batch.walk_waves = 2
     finally:
         batch.close()
         for f in tuned:
             f.close()
     # a schedule far from the defaults as well
-    for tun, blk in ((tuning, block), (dict(tri_ratio=1, inst_ratio=16, refill_min=32, fetch_chunk=8), 256)):
+    for tun, blk, wv in ((tuning, block, waves), (dict(tri_ratio=1, inst_ratio=16, refill_min=32, fetch_chunk=8), 256, 8)):
         ref, bat = make_fields(specs), make_fields(specs)
         for f in bat:
             f.set_walk_tuning(**tun)
         batch = Batch(bat)
         batch.set_walk_block(blk)
+        batch.walk_waves = wv
         try:
             for frame in range(3):
-                print(f"case: tuned batch {tun} block {blk} frame {frame}", flush=True)
+                print(f"case: tuned batch {tun} block {blk} waves {wv} frame {frame}", flush=True)
                 tick_individually(ref, params)
                 batch.tick(params)
                 for fr, fb in zip(ref, bat):

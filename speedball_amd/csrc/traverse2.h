@@ -156,52 +156,40 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
     r.toff = next;
 }
 
-// INST: enter the next instance of the pending TLAS leaf, or go back to the TLAS.
+// INST: enter the next instance slot of the pending TLAS leaf, or go back to the TLAS.
+// Straight-line with selects: every INST lane reads a slot's rows (slot 0 when returning)
+// and computes its local ray, so the compiler has no nested joins whose register copies
+// would cost more than the selects. An inactive slot is entered with an empty range, which
+// sends the lane straight back to INST for the next slot.
+// Component-wise select: a select of whole float3 values can become a select of their
+// addresses, which moves the lane state out of registers into scratch memory.
+__device__ __forceinline__ float3 sel3(bool c, float3 a, float3 b) {
+    return f3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
+}
+
 template <bool COUNT, bool LDS, bool RCP>
 __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     const bool enter = r.inext < r.iend;
-    // default: back to the TLAS in world space
-    float3 o = r.wo, d = r.wd, id = r.wid;
-    float dsign = r.dsign;
-    uint32_t cur = r.tres, end = s.tlas_count, base = r.base, inst = r.inst, level = 0u;
-    uint32_t state = r.tres < s.tlas_count ? ST_NODE : ST_DONE;
-    if (enter) {
-        inst = r.inext;
-        cnt.add_inst();
-        float4 r0, r1, r2, r3;
-        inst_rows<LDS>(s, inst, r0, r1, r2, r3);
-        if (r3.w > 0.5f) {
-            o = inst_point(r0, r1, r2, r.wo);
-            d = inst_dir(r0, r1, r2, r.wd);
-            id = ray_inv<RCP>(d);
-            dsign = r3.z;
-            cur = (uint32_t)r3.x;
-            end = (uint32_t)r3.y;
-            base = octant_of(d, s.blas_octants) * s.node_count;
-            level = 1u;
-            state = cur < end ? ST_NODE : ST_INST;  // an empty range: on to the next slot
-        } else {  // inactive slot: stay in INST for the next one
-            o = r.o;
-            d = r.d;
-            id = r.id;
-            cur = r.cur;
-            end = r.end;
-            inst = r.inst;
-            level = r.level;
-            state = ST_INST;
-        }
-    }
-    r.inext = enter ? r.inext + 1u : r.inext;
-    r.o = o;
-    r.d = d;
-    r.id = id;
-    r.dsign = dsign;
-    r.cur = cur;
-    r.end = end;
-    r.base = base;
-    r.inst = inst;
-    r.level = level;
-    r.state = state;
+    const uint32_t slot = enter ? r.inext : 0u;
+    if (enter) cnt.add_inst();
+    float4 r0, r1, r2, r3;
+    inst_rows<LDS>(s, slot, r0, r1, r2, r3);
+    const bool active = r3.w > 0.5f;
+    const float3 lo = inst_point(r0, r1, r2, r.wo), ld = inst_dir(r0, r1, r2, r.wd);
+    const float3 lid = ray_inv<RCP>(ld);
+    const uint32_t c0 = active ? (uint32_t)r3.x : 0u, c1 = active ? (uint32_t)r3.y : 0u;
+    r.o = sel3(enter, lo, r.wo);
+    r.d = sel3(enter, ld, r.wd);
+    r.id = sel3(enter, lid, r.wid);
+    r.dsign = enter ? r3.z : r.dsign;
+    r.cur = enter ? c0 : r.tres;
+    r.end = enter ? c1 : s.tlas_count;
+    r.base = enter ? octant_of(ld, s.blas_octants) * s.node_count : r.base;
+    r.inst = enter ? slot : r.inst;
+    r.level = enter ? 1u : 0u;
+    // an empty range (or an inactive slot): on to the next slot
+    r.state = enter ? (c0 < c1 ? ST_NODE : ST_INST) : (r.tres < s.tlas_count ? ST_NODE : ST_DONE);
+    r.inext = enter ? slot + 1u : r.inext;
 }
 
 // Wave schedule (DevScene::tri_ratio, inst_ratio, refill_min, fetch_chunk; the defaults

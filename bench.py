@@ -45,6 +45,19 @@ def algorithmic_bytes(stats, hits: int) -> float:
     return walk_bytes(stats.blas_nodes, stats.tlas_nodes, stats.inst_visits, stats.tri_tests) + (16 + 112 + 256) * hits
 
 
+def shadow_phases(stats) -> dict:
+    """Wave steps of the shadow walk's NODE, TRI, INST phases and refills (counting run),
+    and the share of the 64 lanes that did work in each: node visits, triangle tests and
+    instance entries per lane-step (INST steps that return to the TLAS count as idle)."""
+    w = [int(x) for x in stats.shadow_waves]
+    work = (stats.shadow_blas_nodes + stats.shadow_tlas_nodes, stats.shadow_tri_tests, stats.shadow_inst_visits)
+    out = {}
+    for name, steps, done in zip(("node", "tri", "inst"), w[:3], work):
+        out[name] = {"wave_steps": steps, "lanes_busy": float(done) / (64.0 * steps) if steps else 0.0}
+    out["refill"] = {"wave_steps": w[3]}
+    return out
+
+
 def shadow_walk_bytes(stats) -> float:
     """The shadow any-hit walk alone (the dominant kernel)."""
     return walk_bytes(stats.shadow_blas_nodes, stats.shadow_tlas_nodes, stats.shadow_inst_visits,
@@ -596,7 +609,8 @@ def main():
         "trace": {"algorithmic_bytes": bytes_alg, "ms": trace_ms, "gbs": bytes_alg / (trace_ms / 1000.0) / 1e9,
                   "counts": {"blas_nodes": int(cs.blas_nodes), "tlas_nodes": int(cs.tlas_nodes),
                              "tri_tests": int(cs.tri_tests), "inst_visits": int(cs.inst_visits),
-                             "primary_hits": hits}},
+                             "primary_hits": hits},
+                  "shadow_walk_phases": shadow_phases(cs)},
         "blend": {"kernel": "k_blend", "ms": blend_ms, "tflops": blend_flops / (blend_ms / 1000.0) / 1e12,
                   "gbs": blend_bytes / (blend_ms / 1000.0) / 1e9, "flops": blend_flops, "bytes": blend_bytes},
         "fwd_plus_adjoint_ms": adj["fwd_plus_adjoint_ms"] if adj else None,

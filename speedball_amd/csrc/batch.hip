@@ -91,6 +91,7 @@ void frame_segments(const sbgi_batch *b, const sbgi_tick_params *params, std::ve
         t.s = make_scene(f);
         t.c = make_cascade(f, cs, 0, cs.total);
         t.classify = params[g.field].classify_strength;
+        t.bias = hit_bias(t.c.cell_min, params[g.field].dbg_trace_bias, params[g.field].dbg_ray_eps);
         t.dirs = f->dirs;
         t.meta = cs.meta;
         t.w.hits = b->hits[j];
@@ -260,7 +261,7 @@ int32_t sbgi_batch_create(sbgi_field *const *fields, uint32_t n, sbgi_batch **ou
             }
             b->segs.push_back({i, c, (uint32_t)r, L});
             total += r;
-            bytes += r * (16 + 16) + r * L * 32;
+            bytes += r * (16 + 16) + r * L * 16;
         }
     }
     if (total >= (1ull << 31)) {
@@ -299,7 +300,7 @@ int32_t sbgi_batch_create(sbgi_field *const *fields, uint32_t n, sbgi_batch **ou
         p += (size_t)g.rays_total * 16;
         b->qrec.push_back(reinterpret_cast<float4 *>(p));
         b->qcap.push_back(g.rays_total * g.lights);
-        p += (size_t)g.rays_total * g.lights * 32;
+        p += (size_t)g.rays_total * g.lights * 16;
         prefix[j + 1] = prefix[j] + g.rays_total;
     }
     if (hipMemcpy(b->d_prefix, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice) != hipSuccess)

@@ -537,7 +537,7 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     }
     // shadow queue: at most one record per (ray, light)
     const size_t cap = (size_t)f->max_total * f->desc.rays_per_probe * std::max(1u, v.light_count);
-    if (cap >= (1ull << 32) || dalloc(&f->tw.queue.rec, cap * 32) != hipSuccess) {
+    if (cap >= (1ull << 32) || dalloc(&f->tw.queue.rec, cap * 16) != hipSuccess) {
         free_scene(f);
         return fail(f, SBGI_E_NOMEM, "device allocation for the shadow-ray queue failed");
     }

@@ -60,14 +60,22 @@ struct PrimarySrc {
     }
 };
 
-// Shadow rays of the dense queue; an unblocked ray sets bit `light` of its ray's mask.
+// Shadow rays of the dense queue, 16 bytes each: {origin.xyz, (ray << 6) | light}. The
+// direction and extent are re-derived from the light record exactly as the emission did
+// (light_vector, and the cascade's hit bias). An unblocked ray sets bit `light` of its
+// ray's mask.
 struct ShadowSrc {
     const float4 *rec;
     unsigned long long *vis;
+    float bias;
     __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
-        const float4 a = rec[(size_t)i * 2], b = rec[(size_t)i * 2 + 1];
-        lane_begin<true>(s, r, xyz(a), xyz(b), a.w);
-        r.tag = __float_as_uint(b.w);
+        const float4 a = rec[i];
+        const uint32_t tag = __float_as_uint(a.w);
+        float3 wi;
+        float dist;
+        light_vector(s.lights + (size_t)(tag & 63u) * SBGI_LIGHT_STRIDE, xyz(a), wi, dist);
+        lane_begin<true>(s, r, xyz(a), wi, dist - bias);
+        r.tag = tag;
     }
     __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
         if (!r.blocked) atomicOr(&vis[r.tag >> 6], 1ull << (r.tag & 63u));
@@ -261,8 +269,7 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
             const LightSample ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p, light_scale(s, li));
             const uint32_t idx = pos + (uint32_t)__popcll(m & lt_mask);
             if (idx < q.capacity) {
-                q.rec[(size_t)idx * 2] = make_float4(h.hit.x, h.hit.y, h.hit.z, ls.dist - h.bias);
-                q.rec[(size_t)idx * 2 + 1] = make_float4(ls.wi.x, ls.wi.y, ls.wi.z, __uint_as_float((gid << 6) | li));
+                q.rec[idx] = make_float4(h.hit.x, h.hit.y, h.hit.z, __uint_as_float((gid << 6) | li));
             }
         }
         pos += (uint32_t)__popcll(m);
@@ -397,7 +404,7 @@ void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char
         const TraceSeg &g = segs[j];
         pr[j] = WalkSeg<PrimarySrc>{g.s, PrimarySrc{g.c, g.classify, g.dirs, g.meta, g.w.hits}, nullptr,
                                     g.c.updated * g.c.rays, g.heads};
-        sh[j] = WalkSeg<ShadowSrc>{g.s, ShadowSrc{g.w.queue.rec, g.w.vis}, g.w.queue.count, g.w.queue.capacity,
+        sh[j] = WalkSeg<ShadowSrc>{g.s, ShadowSrc{g.w.queue.rec, g.w.vis, g.bias}, g.w.queue.count, g.w.queue.capacity,
                                    g.heads + (size_t)kFetchHeads * kHeadStride};
     }
 }
@@ -499,7 +506,8 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
                        w.hits, w.queue, counters);
     if ((e = mark(2)) != hipSuccess) return e;
     // the queue length is only known on the device: the kernel reads it, bounded by the capacity
-    launch_walk<WALK_ANY>(s, ShadowSrc{w.queue.rec, w.vis}, w.queue.count, w.queue.capacity, ctr + trace_heads(1),
+    const float bias = hit_bias(c.cell_min, p.dbg_trace_bias, p.dbg_ray_eps);
+    launch_walk<WALK_ANY>(s, ShadowSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity, ctr + trace_heads(1),
                           counters ? counters->shadow_walk : nullptr, lds, st);
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
@@ -511,7 +519,7 @@ hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi
                               float *vis, uint32_t *heads, hipStream_t st) {
     const uint32_t n = c.updated * s.emitter_count;
     if (!n) return hipSuccess;
-    const float bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
+    const float bias = hit_bias(c.cell_min, p.dbg_trace_bias, p.dbg_ray_eps);
     const EmitterSrc src{c, p.classify_strength, p.emitter_vis_seed, bias, s.emitter_base, s.emitter_count, meta, vis};
     launch_walk<WALK_ANY>(s, src, nullptr, n, heads, nullptr, trace_lds_bytes(s.tlas_count, s.inst_count), st);
     return hipGetLastError();

@@ -15,6 +15,8 @@
 constexpr size_t kTraceCounterBytes = 48 * 1024;
 constexpr size_t trace_heads(uint32_t walk) { return (1u + (size_t)walk * kFetchHeads) * kHeadStride; }
 static_assert((trace_heads(4)) * sizeof(uint32_t) <= kTraceCounterBytes, "solve counters overflow");
+// Shadow-ray queue: 16-byte records {origin.xyz, (ray << 6) | light}; the walk re-derives
+// direction and extent from the light record.
 struct ShadowQueue {
     float4 *rec;
     uint32_t *count;
@@ -51,6 +53,7 @@ struct TraceSeg {
     const float *meta;
     TraceScratch w;   // this segment's hits, visibility masks and shadow queue
     uint32_t *heads;  // this segment's fetch counters: primary walk, then shadow walk
+    float bias;       // the cascade's hit bias (hit_bias), for the shadow walk's rays
 };
 // device walk tables of n segments (primary, then shadow) serialised into blob, which the
 // caller uploads to a buffer of walk_tables_bytes(n) bytes

@@ -22,6 +22,13 @@
 #define SBGI_RAY_EPS 1e-3f
 #define SBGI_T_MAX 1e30f
 #define SBGI_PI 3.141592653589793f
+
+// Offset of a hit along its normal, and so of its shadow rays' origin and extent: 0.5 % of
+// the cascade's smallest cell, at least the ray epsilon (js/gi_probes.js:1993-2054). Host
+// and device evaluate it alike (two products and a max).
+__host__ __device__ __forceinline__ float hit_bias(float cell_min, float trace_bias, float ray_eps) {
+    return fmaxf(cell_min * 0.005f * trace_bias, SBGI_RAY_EPS * ray_eps);
+}
 #define SBGI_GOLDEN_ANGLE 2.39996322972865332f
 #define SBGI_TWO_PI 6.283185307179586f
 

@@ -211,6 +211,17 @@ __device__ __forceinline__ bool emitter_ray(const float *__restrict__ L, float3 
     return true;
 }
 
+// Unit vector and distance from pos to light record L (a directional light: -dir at 1e4).
+// Evaluated without FMA contraction, so the shadow walk re-derives a queued ray's direction
+// and extent from its origin word for word as the emission computed them.
+__device__ __forceinline__ void light_vector(const float *__restrict__ L, float3 pos, float3 &wi, float &dist) {
+#pragma clang fp contract(off)
+    const bool is_dir = L[0] < 0.5f;
+    const float3 tl = is_dir ? f3(L[4], L[5], L[6]) * -1.f : f3(L[1], L[2], L[3]) - pos;
+    dist = is_dir ? 1e4f : fmaxf(sqrtf(dot3(tl, tl)), 1e-4f);
+    wi = normalize3(tl);
+}
+
 // Geometric shading frame of a hit (js/gi_probes.js:1993-2054).
 struct HitFrame {
     float3 hit, ng, kd, emis;
@@ -228,7 +239,7 @@ __device__ __forceinline__ HitFrame hit_frame(const DevScene &s, const DevCascad
     float3 nl = normalize3(cross3(xyz(B), xyz(C)));
     float3 ng = normalize3(xyz(r0) * nl.x + xyz(r1) * nl.y + xyz(r2) * nl.z);
     h.ng = ng * (dot3(ng, rd) < 0.f ? 1.f : -1.f);
-    h.bias = fmaxf(c.cell_min * 0.005f * p.dbg_trace_bias, SBGI_RAY_EPS * p.dbg_ray_eps);
+    h.bias = hit_bias(c.cell_min, p.dbg_trace_bias, p.dbg_ray_eps);
     h.hit = (ro + rd * t) + h.ng * h.bias;
     const float *mat = s.mats + (size_t)m * SBGI_MAT_STRIDE;
     const float metal = mat[4], transm = mat[5];
@@ -263,10 +274,7 @@ __device__ __forceinline__ LightSample light_gate(const float *__restrict__ L, c
     ls.lcol = f3(L[7], L[8], L[9]) * gate;
     const float lrange = L[10];
     const bool is_dir = ltype < 0.5f;
-    const float3 ldir = f3(L[4], L[5], L[6]);
-    const float3 tl = is_dir ? ldir * -1.f : f3(L[1], L[2], L[3]) - h.hit;
-    ls.dist = is_dir ? 1e4f : fmaxf(sqrtf(dot3(tl, tl)), 1e-4f);
-    ls.wi = normalize3(tl);
+    light_vector(L, h.hit, ls.wi, ls.dist);
     ls.ndl = fmaxf(dot3(h.ng, ls.wi), 0.f);
     const float lpeak = fmaxf(fmaxf(ls.lcol.x, ls.lcol.y), ls.lcol.z);
     ls.pass = ls.ndl * lpeak > 0.f && (is_dir || lrange <= 0.f || ls.dist < lrange);

@@ -276,8 +276,10 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
     }
 }
 
-// Stage 3: shading in the reference's summation order.
-__global__ void __launch_bounds__(kBlock) k_resolve(DevScene s, DevCascade c, sbgi_tick_params p,
+// Stage 3: shading in the reference's summation order. Asked for at least 5 waves per
+// SIMD, the compiler fits it in 80 VGPRs (6 waves) without spilling; left alone it takes
+// 101 (4 waves).
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_resolve(DevScene s, DevCascade c, sbgi_tick_params p,
                                                      const float4 *__restrict__ dirs, const float *__restrict__ meta,
                                                      const ushort4 *__restrict__ prev_atlas,
                                                      const float4 *__restrict__ prev_atlas_f32,

@@ -288,7 +288,10 @@ __device__ __forceinline__ float3 light_contrib(const float *__restrict__ L, con
     const bool is_dir = ltype < 0.5f;
     const bool is_spot = fabsf(ltype - 2.0f) < 0.5f;
     const float3 ldir = f3(L[4], L[5], L[6]);
-    const float falloff = 1.0f / fmaxf(powf(ls.dist, ldecay), 0.01f);
+    // pow(d, 2) is d * d exactly (one rounding either way), so the reference's default
+    // decay skips the general pow
+    const float pw = ldecay == 2.f ? ls.dist * ls.dist : powf(ls.dist, ldecay);
+    const float falloff = 1.0f / fmaxf(pw, 0.01f);
     const float rr = ls.dist / fmaxf(lrange, 1e-4f);
     const float rr2 = rr * rr;
     const float win = fminf(fmaxf(1.f - rr2 * rr2, 0.f), 1.f);

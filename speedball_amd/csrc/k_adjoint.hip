@@ -91,7 +91,8 @@ __device__ __forceinline__ uint32_t bounce_cell(const DevCascade &c, float3 P) {
 // Bounce term of update 2 (js/gi_probes.js:2181-2188 with the luminance roll-off):
 // u = B * kd / pi, out = u * roll(u) * dbg_bounce. Writes g albedo (atomics), the lookup
 // gradient gB and the hit's lookup point, and counts the hit into its cell.
-__global__ void __launch_bounds__(kAdjBlock) k_adj_bounce(DevScene s, DevCascade c, sbgi_tick_params p,
+// 6 waves per SIMD: 79 VGPRs, none spilled (115 and 4 waves without the hint).
+__global__ void __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(6))) k_adj_bounce(DevScene s, DevCascade c, sbgi_tick_params p,
                                                           const float4 *__restrict__ dirs, const float4 *__restrict__ hits,
                                                           const float4 *__restrict__ atlas1, const float4 *__restrict__ gL,
                                                           float *__restrict__ g_albedo, float4 *__restrict__ gB,

@@ -123,6 +123,19 @@ __device__ __forceinline__ void slab(float3 bmin, float3 bmax, float3 ro, float3
     tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
 }
 
+// The same slab test with each plane distance fused, t = fma(b, 1/d, -o/d): one rounding
+// instead of two, so a box decision can differ from slab()'s only for a ray that grazes the
+// box face. Where 1/d is infinite and b/d and o/d are infinities of one sign the fused form
+// gives NaN, which fminf/fmaxf pass over, so the box test only grows more permissive there.
+// For any-hit walks, whose answer no extra box visit can change.
+__device__ __forceinline__ void slab_fma(float3 bmin, float3 bmax, float3 noid, float3 invd, float &tn, float &tf) {
+    const float t0x = fmaf(bmin.x, invd.x, noid.x), t1x = fmaf(bmax.x, invd.x, noid.x);
+    const float t0y = fmaf(bmin.y, invd.y, noid.y), t1y = fmaf(bmax.y, invd.y, noid.y);
+    const float t0z = fmaf(bmin.z, invd.z, noid.z), t1z = fmaf(bmax.z, invd.z, noid.z);
+    tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+}
+
 __device__ __forceinline__ bool side_ok(uint32_t flags, float det) {
     uint32_t side = flags & 3u;
     return side == 2u || (side == 0u && det > 0.f) || (side == 1u && det < 0.f);

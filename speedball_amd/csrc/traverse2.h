@@ -32,6 +32,7 @@ enum : uint32_t { ST_NODE = 0u, ST_TRI = 1u, ST_INST = 2u, ST_DONE = 3u, ST_IDLE
 struct LaneRay {
     float3 wo, wd, wid;   // world ray and its inverse direction
     float3 o, d, id;      // ray of the current level (world or instance-local)
+    float3 noid;          // -o * id of the current level (the any-hit walk's fused slab test)
     float tmax;           // closest: best t so far; any: the shadow ray's extent
     uint32_t cur, end;    // node cursor and end of the current level
     uint32_t tres;        // TLAS node to resume at after the pending instances
@@ -68,6 +69,7 @@ __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3
     r.o = ro;
     r.d = rd;
     r.id = r.wid;
+    r.noid = f3(-(ro.x * r.wid.x), -(ro.y * r.wid.y), -(ro.z * r.wid.z));
     r.tmax = tmax;
     r.cur = 0u;
     r.end = s.tlas_count;
@@ -97,7 +99,7 @@ enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
 // (count << 24) | offset (triangles for a BLAS leaf, instance slots for a TLAS leaf).
 // BLAS nodes are addressed by a 32-bit byte offset from the pool: the upload limits a
 // copy to fewer than 2^24 nodes, so 8 copies stay below 4 GiB.
-template <bool COUNT, bool LDS>
+template <bool COUNT, bool LDS, bool FMA>
 __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     uint4 a, b;
     if (r.level) {
@@ -113,7 +115,8 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     const float3 bmin = f3(__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z));
     const float3 bmax = f3(__uint_as_float(b.x), __uint_as_float(b.y), __uint_as_float(b.z));
     float tn, tf;
-    slab(bmin, bmax, r.o, r.id, tn, tf);
+    if (FMA) slab_fma(bmin, bmax, r.noid, r.id, tn, tf);
+    else slab(bmin, bmax, r.o, r.id, tn, tf);
     const bool hit = tf >= fmaxf(tn, 0.f) && tn < r.tmax;
     const bool descend = hit && b.w == 0xFFFFFFFFu;
     const uint32_t off = b.w & 0x00FFFFFFu, end = off + (b.w >> 24);
@@ -181,6 +184,7 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
     r.o = sel3(enter, lo, r.wo);
     r.d = sel3(enter, ld, r.wd);
     r.id = sel3(enter, lid, r.wid);
+    r.noid = f3(-(r.o.x * r.id.x), -(r.o.y * r.id.y), -(r.o.z * r.id.z));
     r.dsign = enter ? r3.z : r.dsign;
     r.cur = enter ? c0 : r.tres;
     r.end = enter ? c1 : s.tlas_count;
@@ -263,7 +267,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         if (!live) break;
         if (__ballot(r.state == ST_NODE)) {
             if (COUNT && lane == 0) cnt.node_w++;
-            if (r.state == ST_NODE) node_step<COUNT, LDS>(s, r, cnt);
+            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY>(s, r, cnt);
         }
         uint64_t mn = __ballot(r.state == ST_NODE);
         const uint64_t mt = __ballot(r.state == ST_TRI);

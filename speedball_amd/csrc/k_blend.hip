@@ -316,65 +316,127 @@ __global__ void __launch_bounds__(64) k_blend_fp32(DevCascade c, sbgi_tick_param
     }
 }
 
-// MFMA blend: one wave per 16 probes. MTD = depth M-tiles (3 for 36 texels, 13 for 196);
-// EMIT as in k_blend_fp32. Asked for 2 waves per SIMD, the compiler keeps the MFMA
-// accumulators in ordinary VGPRs (the 16x16 depth variant: 248 VGPRs, none spilled)
-// instead of 190 VGPRs + 164 accumulation registers at 1 wave.
-template <uint32_t PREC, int MTD, bool EMIT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_blend_mfma(DevCascade c, sbgi_tick_params p,
-                                                   const float4 *__restrict__ rays, BlendTables t,
-                                                   float *__restrict__ irr, float *__restrict__ depth,
-                                                   EmitterInject em) {
-    constexpr int MTI = 3;
+// MFMA blend: one wave per 16 probes and part. Part 0 (blockIdx.y) contracts the
+// irradiance texels (MTI M-tiles, five channels) and runs their epilogue, with the emitter
+// injection when EMIT; parts 1.. contract a range of the depth M-tiles (MTD in all: 3 for
+// 36 texels, 13 for 196), two channels each. Splitting the texels over waves gives the
+// grid three waves per 16 probes instead of one (12,288 probes: 2,304 waves for 1,024
+// SIMDs), and each wave holds only its own accumulators.
+constexpr int kBlendIrrTiles = 3;
+
+template <uint32_t PREC, int NCH>
+__device__ __forceinline__ void blend_b_frags(const float4 *__restrict__ rays, const DevCascade &c, bool valid,
+                                              uint32_t slot, uint32_t kb, uint32_t g, u16x8 (&bh)[NCH],
+                                              u16x8 (&bl)[NCH]) {
+    // channels: NCH == 5: r, g, b, L, L^2; NCH == 2: d, d^2
+    const uint32_t R = c.rays, U = c.updated;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const uint32_t k = kb * 32 + g * 8 + j;
+        float v[NCH];
+#pragma unroll
+        for (int q = 0; q < NCH; q++) v[q] = 0.f;
+        if (valid && k < R) {
+            const float4 x = rays[(size_t)k * U + slot];
+            if (NCH == 5) {
+                const float L = x.x * 0.2126f + x.y * 0.7152f + x.z * 0.0722f;
+                v[0] = x.x;
+                v[1] = x.y;
+                v[2] = x.z;
+                v[3] = L;
+                v[NCH - 1] = L * L;
+            } else {
+                const float d = ray_dist(x.w, c.max_dist);
+                v[0] = d;
+                v[NCH - 1] = d * d;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NCH; q++) {
+            uint16_t hi, lo;
+            split<PREC>(v[q], hi, lo);
+            bh[q][j] = hi;
+            bl[q][j] = lo;
+        }
+    }
+}
+
+// Depth M-tiles [M0, M1) of one wave's 16 probes, contraction and epilogue.
+template <uint32_t PREC, int MTD, int M0, int M1>
+__device__ __forceinline__ void blend_depth_part(const DevCascade &c, const sbgi_tick_params &p,
+                                                 const float4 *__restrict__ rays, const BlendTables &t,
+                                                 float *__restrict__ depth) {
     constexpr bool SPLIT = PREC == SBGI_BLEND_BF16X3;
     const uint32_t lane = threadIdx.x;
     const uint32_t g = lane >> 4, n = lane & 15;
     const uint32_t slot = blockIdx.x * 16 + n;
     const bool valid = slot < c.updated;
-    const uint32_t R = c.rays, U = c.updated, KB = (R + 31) / 32;
+    const uint32_t KB = (c.rays + 31) / 32;
     const uint32_t tdn = c.depth_oct == 6 ? kIrrTexels : c.depth_oct * c.depth_oct;
     const size_t tile_elems = 64 * 8;
-    const size_t irr_frag = (size_t)MTI * KB * tile_elems;
+    const size_t irr_frag = (size_t)kBlendIrrTiles * KB * tile_elems;
     const size_t d_frag = (size_t)MTD * KB * tile_elems;
-    const u16x8 *f_irr_hi = reinterpret_cast<const u16x8 *>(t.frag);
-    const u16x8 *f_irr_lo = reinterpret_cast<const u16x8 *>(t.frag + irr_frag);
     const u16x8 *f_d_hi = reinterpret_cast<const u16x8 *>(t.frag + 2 * irr_frag);
     const u16x8 *f_d_lo = reinterpret_cast<const u16x8 *>(t.frag + 2 * irr_frag + d_frag);
+    f32x4 ad[M1 - M0][2];
+#pragma unroll
+    for (int m = 0; m < M1 - M0; m++) ad[m][0] = ad[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (uint32_t kb = 0; kb < KB; kb++) {
+        u16x8 bh[2], bl[2];
+        blend_b_frags<PREC, 2>(rays, c, valid, slot, kb, g, bh, bl);
+#pragma unroll
+        for (int m = 0; m < M1 - M0; m++) {
+            const size_t fo = ((size_t)(M0 + m) * KB + kb) * 64 + lane;
+            const u16x8 a_hi = f_d_hi[fo];
+            u16x8 a_lo;
+            if (SPLIT) a_lo = f_d_lo[fo];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                ad[m][q] = mfma<PREC>(a_hi, bh[q], ad[m][q]);
+                if (SPLIT) {
+                    ad[m][q] = mfma<PREC>(a_hi, bl[q], ad[m][q]);
+                    ad[m][q] = mfma<PREC>(a_lo, bh[q], ad[m][q]);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const uint32_t probe = (c.offset + slot) % c.total;
+    // D layout: lane holds rows 4*(lane/16) + jj of column lane%16
+#pragma unroll
+    for (int m = 0; m < M1 - M0; m++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            const uint32_t tt = (M0 + m) * 16 + g * 4 + jj;
+            if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), ad[m][0][jj], ad[m][1][jj], t.norm_d_q[tt]);
+        }
+    }
+}
 
-    f32x4 ai[MTI][5], ad[MTD][2];
+// Irradiance M-tiles of one wave's 16 probes: contraction, emitter injection, epilogue.
+template <uint32_t PREC, int MTD, bool EMIT>
+__device__ __forceinline__ void blend_irr_part(const DevCascade &c, const sbgi_tick_params &p,
+                                               const float4 *__restrict__ rays, const BlendTables &t,
+                                               float *__restrict__ irr, const EmitterInject &em) {
+    constexpr int MTI = kBlendIrrTiles;
+    constexpr bool SPLIT = PREC == SBGI_BLEND_BF16X3;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane >> 4, n = lane & 15;
+    const uint32_t slot = blockIdx.x * 16 + n;
+    const bool valid = slot < c.updated;
+    const uint32_t KB = (c.rays + 31) / 32;
+    const size_t tile_elems = 64 * 8;
+    const size_t irr_frag = (size_t)MTI * KB * tile_elems;
+    const u16x8 *f_irr_hi = reinterpret_cast<const u16x8 *>(t.frag);
+    const u16x8 *f_irr_lo = reinterpret_cast<const u16x8 *>(t.frag + irr_frag);
+    f32x4 ai[MTI][5];
 #pragma unroll
     for (int m = 0; m < MTI; m++)
 #pragma unroll
         for (int q = 0; q < 5; q++) ai[m][q] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int m = 0; m < MTD; m++) ad[m][0] = ad[m][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-
     for (uint32_t kb = 0; kb < KB; kb++) {
-        u16x8 bh[7], bl[7];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint32_t k = kb * 32 + g * 8 + j;
-            float v[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (valid && k < R) {
-                const float4 x = rays[(size_t)k * U + slot];
-                const float L = x.x * 0.2126f + x.y * 0.7152f + x.z * 0.0722f;
-                const float d = ray_dist(x.w, c.max_dist);
-                v[0] = x.x;
-                v[1] = x.y;
-                v[2] = x.z;
-                v[3] = L;
-                v[4] = L * L;
-                v[5] = d;
-                v[6] = d * d;
-            }
-#pragma unroll
-            for (int q = 0; q < 7; q++) {
-                uint16_t hi, lo;
-                split<PREC>(v[q], hi, lo);
-                bh[q][j] = hi;
-                bl[q][j] = lo;
-            }
-        }
+        u16x8 bh[5], bl[5];
+        blend_b_frags<PREC, 5>(rays, c, valid, slot, kb, g, bh, bl);
 #pragma unroll
         for (int m = 0; m < MTI; m++) {
             const size_t fo = ((size_t)m * KB + kb) * 64 + lane;
@@ -390,34 +452,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
                 }
             }
         }
-#pragma unroll
-        for (int m = 0; m < MTD; m++) {
-            const size_t fo = ((size_t)m * KB + kb) * 64 + lane;
-            const u16x8 a_hi = f_d_hi[fo];
-            u16x8 a_lo;
-            if (SPLIT) a_lo = f_d_lo[fo];
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                ad[m][q] = mfma<PREC>(a_hi, bh[5 + q], ad[m][q]);
-                if (SPLIT) {
-                    ad[m][q] = mfma<PREC>(a_hi, bl[5 + q], ad[m][q]);
-                    ad[m][q] = mfma<PREC>(a_lo, bh[5 + q], ad[m][q]);
-                }
-            }
-        }
     }
     if (!valid) return;
     const uint32_t probe = (c.offset + slot) % c.total;
-    // D layout: lane holds rows 4*(lane/16) + jj of column lane%16
-    // depth first: its accumulators are then free for the irradiance epilogue
-#pragma unroll
-    for (int m = 0; m < MTD; m++) {
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-            const uint32_t tt = m * 16 + g * 4 + jj;
-            if (tt < tdn) depth_policy(p, depth, depth_index(c, probe, tt), ad[m][0][jj], ad[m][1][jj], t.norm_d_q[tt]);
-        }
-    }
     float ws[MTI][4];
     if (EMIT) {
 #pragma unroll
@@ -461,6 +498,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_
             }
         }
     }
+}
+
+// Parts per 16 probes: irradiance, then the depth tiles in one part (3 tiles) or two halves.
+template <int MTD>
+constexpr uint32_t blend_parts() { return MTD > 3 ? 3u : 2u; }
+
+template <uint32_t PREC, int MTD, bool EMIT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) k_blend_mfma(DevCascade c, sbgi_tick_params p,
+                                                   const float4 *__restrict__ rays, BlendTables t,
+                                                   float *__restrict__ irr, float *__restrict__ depth,
+                                                   EmitterInject em) {
+    constexpr int H = (MTD + 1) / 2;
+    if (blockIdx.y == 0) blend_irr_part<PREC, MTD, EMIT>(c, p, rays, t, irr, em);
+    else if (MTD <= 3) blend_depth_part<PREC, MTD, 0, MTD>(c, p, rays, t, depth);
+    else if (blockIdx.y == 1) blend_depth_part<PREC, MTD, 0, H>(c, p, rays, t, depth);
+    else blend_depth_part<PREC, MTD, H, MTD>(c, p, rays, t, depth);
 }
 
 }  // namespace
@@ -508,8 +561,8 @@ hipError_t launch_blend(const DevCascade &c, const sbgi_tick_params &p, const fl
             hipLaunchKernelGGL(k_blend_fp32<false>, grid, dim3(64), 0, st, c, p, rays, t, irr, depth, em);
         return hipGetLastError();
     }
-    dim3 grid((c.updated + 15) / 16);
     const bool parity = c.depth_oct == 6;
+    dim3 grid((c.updated + 15) / 16, parity ? blend_parts<3>() : blend_parts<13>());
     if (!parity && c.depth_oct != 14) return hipErrorInvalidValue;
 #define SBGI_LAUNCH(PREC)                                                                                        \
     if (parity && em.count)                                                                                      \

@@ -164,7 +164,9 @@ __device__ __forceinline__ void irr_policy(const sbgi_tick_params &p, float *__r
     const float noise_h = hy + (1.f - hy) * p.dbg_noise_h_boost;
     const float change_h = fmaxf(hy - p.temp_change_h_drop, p.dbg_min_change_h);
     const float raw = fminf(fmaxf(noise_h * (1.f - cw) + change_h * cw, 1e-6f), 1.f);
-    const float h = was_black ? 0.f : powf(raw, p.hysteresis_exponent);
+    // pow(x, 1) is x exactly: the default exponent skips the general pow
+    const float hexp = p.hysteresis_exponent;
+    const float h = was_black ? 0.f : (hexp == 1.f ? raw : powf(raw, hexp));
     const float band = sqrtf(prev_var + var_floor) * p.temp_clamp_sigma;
     const float cs = fminf(1.f, band / fmaxf(ad, 1e-6f));
     const float cr = was_black ? mr : prev.x + (mr - prev.x) * cs;
@@ -183,7 +185,8 @@ __device__ __forceinline__ void depth_policy(const sbgi_tick_params &p, float *_
     const float2 prev = *D;
     const bool dzero = prev.x < 1e-6f;
     const float rdh = fminf(fmaxf(p.hysteresis * p.dbg_depth_history, 0.f), 0.999f);
-    const float dh = dzero ? 0.f : powf(fmaxf(rdh, 1e-6f), p.hysteresis_exponent);
+    const float dhb = fmaxf(rdh, 1e-6f);
+    const float dh = dzero ? 0.f : (p.hysteresis_exponent == 1.f ? dhb : powf(dhb, p.hysteresis_exponent));
     *D = make_float2(mr * (1.f - dh) + prev.x * dh, mr2 * (1.f - dh) + prev.y * dh);
 }
 

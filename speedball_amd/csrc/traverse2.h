@@ -36,8 +36,10 @@ struct LaneRay {
     float tmax;           // closest: best t so far; any: the shadow ray's extent
     uint32_t cur, end;    // node cursor and end of the current level
     uint32_t tres;        // TLAS node to resume at after the pending instances
-    uint32_t inext, iend; // pending instance slots of the current TLAS leaf
-    uint32_t toff, tend;  // pending triangle range of the current BLAS leaf
+    // pending ranges as leaf payload words, (count << 24) | offset: the instance slots of
+    // the current TLAS leaf and the triangles of the current BLAS leaf still to take; a
+    // step takes the offset and moves the word to (count - 1, offset + 1)
+    uint32_t ipend, tpend;
     uint32_t base;        // node offset of the BLAS copy being walked (octant copy * nodes)
     uint32_t inst;        // instance being walked
     uint32_t level;       // 0 TLAS, 1 BLAS
@@ -74,8 +76,7 @@ __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3
     r.cur = 0u;
     r.end = s.tlas_count;
     r.tres = 0u;
-    r.inext = r.iend = 0u;
-    r.toff = r.tend = 0u;
+    r.ipend = r.tpend = 0u;
     r.base = 0u;
     r.inst = 0u;
     r.level = 0u;
@@ -119,9 +120,8 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     else slab(bmin, bmax, r.o, r.id, tn, tf);
     const bool hit = tf >= fmaxf(tn, 0.f) && tn < r.tmax;
     const bool descend = hit && b.w == 0xFFFFFFFFu;
-    const uint32_t off = b.w & 0x00FFFFFFu, end = off + (b.w >> 24);
     // an empty leaf (count 0) is passed over like a miss: TRI / INST never see an empty range
-    const bool leaf = hit && !descend && end > off;
+    const bool leaf = hit && !descend && (b.w >> 24) != 0u;
     // i+1 on a descent, else the miss link; masked rather than selected, so the compiler
     // cannot sink the miss link's load into a branch of its own (a second round trip)
     const uint32_t next = max(a.w & (descend ? 0u : ~0u), r.cur + 1u);
@@ -129,10 +129,8 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     const uint32_t state = leaf ? ST_INST - r.level : (next < r.end ? ST_NODE : ST_DONE - r.level);
     const bool blas_leaf = leaf && r.level, tlas_leaf = leaf && !r.level;
     r.cur = next;
-    r.toff = blas_leaf ? off : r.toff;
-    r.tend = blas_leaf ? end : r.tend;
-    r.inext = tlas_leaf ? off : r.inext;
-    r.iend = tlas_leaf ? end : r.iend;
+    r.tpend = blas_leaf ? b.w : r.tpend;
+    r.ipend = tlas_leaf ? b.w : r.ipend;
     r.tres = tlas_leaf ? next : r.tres;
     r.state = state;
 }
@@ -141,7 +139,8 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
 template <int MODE, bool COUNT>
 __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     const bool ANY = MODE == WALK_MIXED ? r.any : MODE == WALK_ANY;
-    const float4 *tp = s.tri + (size_t)r.toff * 3;
+    const uint32_t tri = r.tpend & 0x00FFFFFFu;
+    const float4 *tp = s.tri + (size_t)tri * 3;
     const float4 A = tp[0], B = tp[1], C = tp[2];
     cnt.add_tri();
     float th = 0.f, det = 0.f;
@@ -149,14 +148,14 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
     const uint32_t flags = __float_as_uint(A.w);
     const bool accept = geo && side_ok(flags, det * r.dsign) && (flags & (ANY ? 8u : 4u));
     const bool block = ANY && accept, better = !ANY && accept;
-    const uint32_t next = r.toff + 1u;
+    const uint32_t next = r.tpend + 1u - (1u << 24);  // offsets stay below 2^24
     r.tmax = better ? th : r.tmax;
-    r.hit_tri = better ? (int)r.toff : r.hit_tri;
+    r.hit_tri = better ? (int)tri : r.hit_tri;
     r.hit_inst = better ? (int)r.inst : r.hit_inst;
     r.blocked = r.blocked || block;
     // leaf exhausted: on to the node after it, or back to INST when the BLAS is done
-    r.state = block ? ST_DONE : (next < r.tend ? ST_TRI : (r.cur < r.end ? ST_NODE : ST_INST));
-    r.toff = next;
+    r.state = block ? ST_DONE : ((next >> 24) ? ST_TRI : (r.cur < r.end ? ST_NODE : ST_INST));
+    r.tpend = next;
 }
 
 // INST: enter the next instance slot of the pending TLAS leaf, or go back to the TLAS.
@@ -172,8 +171,8 @@ __device__ __forceinline__ float3 sel3(bool c, float3 a, float3 b) {
 
 template <bool COUNT, bool LDS, bool RCP>
 __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
-    const bool enter = r.inext < r.iend;
-    const uint32_t slot = enter ? r.inext : 0u;
+    const bool enter = (r.ipend >> 24) != 0u;
+    const uint32_t slot = enter ? r.ipend & 0x00FFFFFFu : 0u;
     if (enter) cnt.add_inst();
     float4 r0, r1, r2, r3;
     inst_rows<LDS>(s, slot, r0, r1, r2, r3);
@@ -193,7 +192,7 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
     r.level = enter ? 1u : 0u;
     // an empty range (or an inactive slot): on to the next slot
     r.state = enter ? (c0 < c1 ? ST_NODE : ST_INST) : (r.tres < s.tlas_count ? ST_NODE : ST_DONE);
-    r.inext = enter ? slot + 1u : r.inext;
+    r.ipend = enter ? r.ipend + 1u - (1u << 24) : r.ipend;
 }
 
 // Wave schedule (DevScene::tri_ratio, inst_ratio, refill_min, fetch_chunk; the defaults

@@ -118,7 +118,9 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     float tn, tf;
     if (FMA) slab_fma(bmin, bmax, r.noid, r.id, tn, tf);
     else slab(bmin, bmax, r.o, r.id, tn, tf);
-    const bool hit = tf >= fmaxf(tn, 0.f) && tn < r.tmax;
+    // tf >= max(tn, 0) as two compares: tn and tf are never NaN (a unit direction has a
+    // finite plane distance on some axis, and fminf/fmaxf pass over NaN ones)
+    const bool hit = tf >= tn && tf >= 0.f && tn < r.tmax;
     const bool descend = hit && b.w == 0xFFFFFFFFu;
     // an empty leaf (count 0) is passed over like a miss: TRI / INST never see an empty range
     const bool leaf = hit && !descend && (b.w >> 24) != 0u;

@@ -38,6 +38,8 @@ __global__ void __launch_bounds__(256) k_weights(DevCascade c, sbgi_tick_params 
                                                  BlendTables t, uint32_t precision) {
     __shared__ float w_s[256];
     __shared__ float d_s[256];
+    __shared__ float qw_s[256];  // the weights as the MFMA operands round them
+    __shared__ float qd_s[256];
     const uint32_t b = blockIdx.x;
     const bool parity = c.depth_oct == 6;
     const uint32_t td_count = parity ? kIrrTexels : c.depth_oct * c.depth_oct;
@@ -64,6 +66,8 @@ __global__ void __launch_bounds__(256) k_weights(DevCascade c, sbgi_tick_params 
         float dw = powf(cw, p.depth_sharpness);
         w_s[k] = cw;
         d_s[k] = dw;
+        qw_s[k] = quantized(cw, precision);
+        qd_s[k] = quantized(dw, precision);
         if (is_irr) t.w_irr_t[(size_t)k * kIrrTexels + b] = cw;
         if (want_d) t.w_d_t[(size_t)k * td_count + drow] = dw;
     }
@@ -105,28 +109,13 @@ __global__ void __launch_bounds__(256) k_weights(DevCascade c, sbgi_tick_params 
     }
     if (threadIdx.x == 0) {
         // normalisers, summed in ray order as the reference loop does (js/gi_probes.js:2368-2377)
+        // (the rounded weights come from every thread above; one lane only adds)
         float s = 0.f, sq = 0.f, ds = 0.f, dsq = 0.f;
         for (uint32_t k = 0; k < R; k++) {
             s += w_s[k];
             ds += d_s[k];
-            float qw, qd;
-            if (precision == SBGI_BLEND_FP16) {
-                qw = from_f16(to_f16(w_s[k]));
-                qd = from_f16(to_f16(d_s[k]));
-            } else if (precision == SBGI_BLEND_BF16) {
-                qw = from_bf16(to_bf16(w_s[k]));
-                qd = from_bf16(to_bf16(d_s[k]));
-            } else if (precision == SBGI_BLEND_BF16X3) {
-                uint16_t h = to_bf16(w_s[k]);
-                qw = from_bf16(h) + from_bf16(to_bf16(w_s[k] - from_bf16(h)));
-                h = to_bf16(d_s[k]);
-                qd = from_bf16(h) + from_bf16(to_bf16(d_s[k] - from_bf16(h)));
-            } else {
-                qw = w_s[k];
-                qd = d_s[k];
-            }
-            sq += qw;
-            dsq += qd;
+            sq += qw_s[k];
+            dsq += qd_s[k];
         }
         if (is_irr) {
             t.norm_irr[b] = s;

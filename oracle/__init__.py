@@ -51,6 +51,12 @@ def half_round(x: float) -> float:
     return lib().half_round(C.c_float(x))
 
 
+def set_f32_dirs(on: bool) -> None:
+    """FP64 oracle only: take the ray directions from the FP32 restatement (phi in f32, as
+    the reference evaluates it), widened to double; everything downstream stays FP64."""
+    lib().or64_set_f32_dirs(C.c_int(1 if on else 0))
+
+
 def ray_dir(k: int, rays: int, jitter: float, prec="64") -> np.ndarray:
     out = np.zeros(3, _dt(prec))
     f = _fn(prec, "ray_dir")

@@ -62,6 +62,9 @@ static void P(cross3)(const REAL a[3], const REAL b[3], REAL o[3]) {
 
 /* ---- rayDir: js/gi_probes.js:1692-1704, normalized at :1984 and :2331 ---- */
 void P(ray_dir)(uint32_t k, uint32_t rays, float jitter, REAL out[3]) {
+#ifdef RAY_DIR_HOOK
+    if (RAY_DIR_HOOK(k, rays, jitter, out)) return;
+#endif
     REAL j = (REAL)jitter;
     REAL sk = ((REAL)k + RCONST(0.5)) / (REAL)rays;
     REAL u = sk + j;

@@ -54,6 +54,24 @@ float half_round(float f) {
 }
 
 /* ---------------- FP64 oracle ---------------- */
+/* Direction mode of the FP64 oracle: with f32 directions on, rayDir returns the FP32
+ * restatement's direction (phi = k GA evaluated in f32, as the reference's WGSL does),
+ * widened to double, and everything downstream stays FP64. Comparing the FP64 oracle in
+ * both modes splits the error of an FP32 path into the part the f32 direction definition
+ * causes and the part its arithmetic causes. */
+void or32_ray_dir(uint32_t k, uint32_t rays, float jitter, float out[3]);
+static int g_f32_dirs = 0;
+void or64_set_f32_dirs(int on) { g_f32_dirs = on; }
+static int ray_dir_f32_hook(uint32_t k, uint32_t rays, float jitter, double out[3]) {
+    if (!g_f32_dirs) return 0;
+    float f[3];
+    or32_ray_dir(k, rays, jitter, f);
+    out[0] = f[0];
+    out[1] = f[1];
+    out[2] = f[2];
+    return 1;
+}
+#define RAY_DIR_HOOK ray_dir_f32_hook
 #define REAL double
 #define P(n) or64_##n
 #define RABS fabs
@@ -84,6 +102,8 @@ float half_round(float f) {
 #undef RFMAX
 #undef RGAUSS
 #undef VSTRIDE
+
+#undef RAY_DIR_HOOK
 
 /* ---------------- FP32 restatement ---------------- */
 #define REAL float

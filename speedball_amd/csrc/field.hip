@@ -474,13 +474,15 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         if (dalloc(&f->hit_ids, (size_t)f->max_total * R * 8) != hipSuccess)
             return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
     }
-    (void)hipMemsetAsync(f->tables, 0, tb, f->stream);
-    (void)hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream);
+    if (hipMemsetAsync(f->tables, 0, tb, f->stream) != hipSuccess ||
+        hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream) != hipSuccess)
+        return bail(fail(f, SBGI_E_DEVICE, "clearing the weight tables failed"));
     blend_tables_carve(f->tables, R, tdn, f->bt);
     if (desc->reflections != SBGI_REFLECT_OFF) {
         const size_t rb = refl_tables_bytes(R, glossy_tile(desc->reflections));
         if (dalloc(&f->rtables, rb) != hipSuccess) return bail(fail(f, SBGI_E_NOMEM, "device allocation failed"));
-        (void)hipMemsetAsync(f->rtables, 0, rb, f->stream);
+        if (hipMemsetAsync(f->rtables, 0, rb, f->stream) != hipSuccess)
+            return bail(fail(f, SBGI_E_DEVICE, "clearing the reflection tables failed"));
         refl_tables_carve(f->rtables, R, glossy_tile(desc->reflections), f->rt);
     }
     if (hipStreamSynchronize(f->stream) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "stream sync failed"));
@@ -913,9 +915,11 @@ int32_t sbgi_poll(sbgi_field *f, uint64_t ticket) {
     if (!f) return SBGI_E_INVALID;
     if (ticket > f->ticket) return fail(f, SBGI_E_INVALID, "unknown ticket");
     if (ticket == 0) return 1;
-    hipEvent_t e = (f->ticket - ticket < kTicketRing) ? f->ring[ticket % kTicketRing] : nullptr;
-    if (!e) return 1;  // older than the ring: a later ticket's event already covers it
-    hipError_t q = hipEventQuery(e);
+    // A ticket older than the ring is covered by the oldest event still in it: that event
+    // was recorded later on the same in-order stream, so once it has completed the older
+    // ticket has too. Until then the older ticket reads as pending.
+    const uint64_t slot = (f->ticket - ticket < kTicketRing) ? ticket : f->ticket + 1u;
+    hipError_t q = hipEventQuery(f->ring[slot % kTicketRing]);
     if (q == hipSuccess) return 1;
     if (q == hipErrorNotReady) return 0;
     return hip_fail(f, q, "hipEventQuery");

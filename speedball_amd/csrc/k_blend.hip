@@ -80,13 +80,16 @@ __global__ void __launch_bounds__(256) k_weights(DevCascade c, sbgi_tick_params 
     uint16_t *f_irr_hi = t.frag, *f_irr_lo = t.frag + irr_frag;
     uint16_t *f_d_hi = t.frag + 2 * irr_frag, *f_d_lo = f_d_hi + d_frag;
     if (precision != SBGI_BLEND_FP32) {
-        for (uint32_t k = threadIdx.x; k < R; k += blockDim.x) {
+        // the K padding of the last block (rays R..32 KB - 1, when R is not a multiple of 32)
+        // is written as zeros here, so the contraction never depends on the table's
+        // initial contents
+        for (uint32_t k = threadIdx.x; k < 32 * KB; k += blockDim.x) {
             const uint32_t kb = k / 32, g = (k % 32) / 8, j = k % 8;
             for (int which = 0; which < 2; which++) {
                 if (which == 0 && !is_irr) continue;
                 if (which == 1 && !want_d) continue;
                 const uint32_t row = which == 0 ? b : drow;
-                const float w = which == 0 ? w_s[k] : d_s[k];
+                const float w = k < R ? (which == 0 ? w_s[k] : d_s[k]) : 0.f;
                 const uint32_t mt = row / 16, lane = g * 16 + row % 16;
                 const size_t o = (((size_t)mt * KB + kb) * 64 + lane) * 8 + j;
                 uint16_t hi, lo;

@@ -72,11 +72,16 @@ __global__ void __launch_bounds__(256) k_weights_refl(uint32_t rays, const float
     float *wt = rough ? t.w_rough_t : t.w_gloss_t;
     uint16_t *fhi = rough ? t.frag_rough : t.frag_gloss;
     const size_t frag = (size_t)mt_count(T) * kb_count(rays) * 64 * 8;
-    for (uint32_t k = threadIdx.x; k < rays; k += blockDim.x) {
-        const float cd = fmaxf(dot3(dir, xyz(dirs[k])), 0.f);
-        const float w = rough ? pow8(cd) : pow64(cd);
-        w_s[k] = w;
-        wt[(size_t)k * T + row] = w;
+    // k runs over the K padding of the last block too (rays..32 KB - 1): zero weights there,
+    // so the MFMA contraction never depends on the table's initial contents
+    for (uint32_t k = threadIdx.x; k < 32 * kb_count(rays); k += blockDim.x) {
+        float w = 0.f;
+        if (k < rays) {
+            const float cd = fmaxf(dot3(dir, xyz(dirs[k])), 0.f);
+            w = rough ? pow8(cd) : pow64(cd);
+            w_s[k] = w;
+            wt[(size_t)k * T + row] = w;
+        }
         if (precision != SBGI_BLEND_FP32) {
             uint16_t hi, lo;
             if (precision == SBGI_BLEND_FP16) {

@@ -155,28 +155,58 @@ static napi_value js_field_destroy(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
-static int read_scene(napi_env env, napi_value o, sbgi_scene_view *v) {
-    size_t l;
-    napi_typedarray_type t;
+/* The scene's typed arrays with their element types and lengths checked against each
+ * other, so that no count taken from one array can make sbgi_set_scene read past the end
+ * of another: bvhNodes Uint32Array[8 N], triIndex Uint32Array[3 T], vertexData
+ * Float32Array[8 V], triMaterial Uint32Array[T], materials Float32Array, lights
+ * Float32Array[17 L] (optional). Returns NULL when the scene is well formed, else the
+ * reason. */
+static const char *typed_of(napi_env env, napi_value o, const char *key, napi_typedarray_type want, size_t stride,
+                            int required, const void **data, size_t *count) {
+    size_t l = 0;
+    napi_typedarray_type t = napi_int8_array;
+    *data = get_typed(env, get_prop(env, o, key), &l, &t);
+    *count = 0;
+    if (!*data) return required ? key : NULL;
+    if (t != want || l % stride != 0) return key;
+    *count = l / stride;
+    return NULL;
+}
+
+static const char *read_scene(napi_env env, napi_value o, sbgi_scene_view *v) {
+    const void *d;
+    size_t n, tris = 0, mats = 0;
+    const char *bad;
     memset(v, 0, sizeof(*v));
-    v->bvh_nodes = get_typed(env, get_prop(env, o, "bvhNodes"), &l, &t);
-    v->node_count = (uint32_t)(l / 8);
-    v->tri_index = get_typed(env, get_prop(env, o, "triIndex"), &l, &t);
-    v->tri_count = (uint32_t)(l / 3);
-    v->vertex_data = get_typed(env, get_prop(env, o, "vertexData"), &l, &t);
-    v->vertex_count = (uint32_t)(l / 8);
-    v->tri_material = get_typed(env, get_prop(env, o, "triMaterial"), &l, &t);
-    v->materials = get_typed(env, get_prop(env, o, "materials"), &l, &t);
-    v->materials_len = (uint32_t)l;
-    v->lights = get_typed(env, get_prop(env, o, "lights"), &l, &t);
-    v->light_count = (uint32_t)(l / 17);
+    if ((bad = typed_of(env, o, "bvhNodes", napi_uint32_array, 8, 1, &d, &n))) return bad;
+    v->bvh_nodes = d;
+    v->node_count = (uint32_t)n;
+    if ((bad = typed_of(env, o, "triIndex", napi_uint32_array, 3, 1, &d, &tris))) return bad;
+    v->tri_index = d;
+    v->tri_count = (uint32_t)tris;
+    if ((bad = typed_of(env, o, "vertexData", napi_float32_array, 8, 1, &d, &n))) return bad;
+    v->vertex_data = d;
+    v->vertex_count = (uint32_t)n;
+    if ((bad = typed_of(env, o, "triMaterial", napi_uint32_array, 1, 1, &d, &n))) return bad;
+    if (n != tris) return "triMaterial (one entry per triIndex triangle)";
+    v->tri_material = d;
+    if ((bad = typed_of(env, o, "materials", napi_float32_array, 1, 1, &d, &mats))) return bad;
+    v->materials = d;
+    v->materials_len = (uint32_t)mats;
+    if ((bad = typed_of(env, o, "lights", napi_float32_array, SBGI_LIGHT_STRIDE, 0, &d, &n))) return bad;
+    v->lights = d;
+    v->light_count = (uint32_t)n;
     v->inst_base = (uint32_t)get_num(env, o, "instBase", 0);
     v->inst_count = (uint32_t)get_num(env, o, "instCount", 0);
     v->tlas_base = (uint32_t)get_num(env, o, "tlasBase", 0);
     v->tlas_node_count = (uint32_t)get_num(env, o, "tlasNodeCount", 0);
     v->emitter_base = (uint32_t)get_num(env, o, "emitterBase", 0);
     v->emitter_count = (uint32_t)get_num(env, o, "emitterCount", 0);
-    return v->bvh_nodes && v->tri_index && v->vertex_data && v->tri_material && v->materials;
+    if ((uint64_t)v->inst_base + (uint64_t)v->inst_count * SBGI_MAT_STRIDE > mats ||
+        (uint64_t)v->tlas_base + (uint64_t)v->tlas_node_count * SBGI_TLAS_STRIDE > mats)
+        return "materials (instance or TLAS records past its end)";
+    if ((uint64_t)v->emitter_base + v->emitter_count > v->light_count) return "lights (emitter range past its end)";
+    return NULL;
 }
 
 static napi_value js_field_set_scene(napi_env env, napi_callback_info info) {
@@ -184,7 +214,14 @@ static napi_value js_field_set_scene(napi_env env, napi_callback_info info) {
     if (!get_args(env, info, 2, argv)) return NULL;
     sbgi_field *f = get_ptr(env, argv[0]);
     sbgi_scene_view v;
-    if (!f || !read_scene(env, argv[1], &v)) return throw_status(env, SBGI_E_INVALID, "bad field or scene");
+    /* the scene is checked before the field, so its checks need no device */
+    const char *bad = read_scene(env, argv[1], &v);
+    if (bad) {
+        char msg[160];
+        snprintf(msg, sizeof(msg), "malformed scene: %s has the wrong type or length", bad);
+        return throw_status(env, SBGI_E_INVALID, msg);
+    }
+    if (!f) return throw_status(env, SBGI_E_INVALID, "bad field");
     int32_t rc = sbgi_set_scene(f, &v);
     if (rc < 0) return throw_status(env, rc, sbgi_last_error(f));
     return NULL;

@@ -7,6 +7,7 @@ copies plus a `view()` that points at them for sbgi_set_scene and for the oracle
 from __future__ import annotations
 
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -68,6 +69,19 @@ def directional_light(pos, target, color=(1.0, 1.0, 1.0), intensity=1.0) -> np.n
     r[11] = 2
     r[12] = -1
     r[13] = -1
+    return r
+
+
+def spot_light(pos, target, color=(1.0, 1.0, 1.0), intensity=1.0, distance=0.0, angle=math.pi / 4, penumbra=0.0,
+               decay=2.0) -> np.ndarray:
+    """collectLights record for a SpotLight (js/spectral_scene.js:785-797): the axis from
+    the light to its target, cosAngle = cos(angle), cosPenumbra = cos(angle (1 - penumbra))."""
+    r = directional_light(pos, target, color, intensity)
+    r[0] = 2
+    r[10] = distance
+    r[11] = decay
+    r[12] = math.cos(angle)
+    r[13] = math.cos(angle * (1.0 - penumbra))
     return r
 
 

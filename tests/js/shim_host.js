@@ -280,6 +280,39 @@ tests.install_handle_forwards_and_guards = async () => {
     assert.throws(() => shim.installSpeedballGI({}), /scene/);
 };
 
+tests.addon_rejects_malformed_scene_arrays = () => {
+    // the real addon's fieldSetScene checks the scene's element types and lengths before it
+    // looks at the field, so a malformed scene is refused without a device
+    const valid = () => ({
+        bvhNodes: new Uint32Array(8), triIndex: new Uint32Array(3), vertexData: new Float32Array(24),
+        triMaterial: new Uint32Array(1), materials: new Float32Array(28 + 28 + 12), lights: new Float32Array(17),
+        instBase: 28, instCount: 1, tlasBase: 56, tlasNodeCount: 1,
+    });
+    assert.throws(() => real.fieldSetScene(null, valid()), /bad field/);
+    const bad = [
+        ['bvhNodes', (s) => { s.bvhNodes = new Uint8Array(32); }],
+        ['bvhNodes', (s) => { s.bvhNodes = new Uint32Array(12); }],
+        ['triIndex', (s) => { s.triIndex = new Uint16Array(3); }],
+        ['vertexData', (s) => { s.vertexData = new Uint32Array(24); }],
+        ['triMaterial', (s) => { s.triMaterial = new Uint32Array(2); }],
+        ['triMaterial', (s) => { s.triMaterial = new Uint8Array(1); }],
+        ['materials', (s) => { s.instCount = 2; }],
+        ['materials', (s) => { s.tlasNodeCount = 2; }],
+        ['materials', (s) => { s.materials = new Float64Array(68); }],
+        ['lights', (s) => { s.lights = new Float32Array(16); }],
+        ['lights', (s) => { s.emitterBase = 0; s.emitterCount = 2; }],
+        ['triIndex', (s) => { delete s.triIndex; }],
+    ];
+    for (const [name, edit] of bad) {
+        const s = valid();
+        edit(s);
+        assert.throws(() => real.fieldSetScene(null, s), new RegExp('malformed scene: ' + name));
+    }
+    const noLights = valid();
+    delete noLights.lights;
+    assert.throws(() => real.fieldSetScene(null, noLights), /bad field/);   // lights are optional
+};
+
 (async () => {
     const only = process.argv[2];
     let n = 0;

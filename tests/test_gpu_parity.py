@@ -48,16 +48,22 @@ def room_run(room, request):
     stats = f.stats()
     f.close()
     refs = {}
-    for prec in ("32", "64"):
+    # "64f": the FP64 oracle fed the f32 ray directions of the reference's rayDir (phi in
+    # f32), so that its difference from "64" is the direction definition's share of the error
+    for key, prec in (("32", "32"), ("64", "64"), ("64f", "64")):
         o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec=prec)
         p = o.params(**params(w))
         steps = []
-        for _ in range(2):
-            o.update(p)
-            steps.append(dict(rays=o.rays_view().copy(), hits=o.hits_view().copy(), irr=o.irr_view().copy(),
-                              depth=o.depth_view().copy(), atlas=o.atlas_view().copy(),
-                              datlas=o.atlas_view(depth=True).copy()))
-        refs[prec] = steps
+        oracle.set_f32_dirs(key == "64f")
+        try:
+            for _ in range(2):
+                o.update(p)
+                steps.append(dict(rays=o.rays_view().copy(), hits=o.hits_view().copy(), irr=o.irr_view().copy(),
+                                  depth=o.depth_view().copy(), atlas=o.atlas_view().copy(),
+                                  datlas=o.atlas_view(depth=True).copy()))
+        finally:
+            oracle.set_f32_dirs(False)
+        refs[key] = steps
     return w, gpu, refs, stats
 
 
@@ -116,16 +122,29 @@ def test_room_radiance_fp32(room_run):
 
 
 def test_room_irradiance_fp32(room_run):
+    """Irradiance of both updates against the FP32 restatement and the FP64 oracle, with
+    max bars. Every texel off the FP64 oracle by more than 1e-3 is classified: the FP64
+    oracle fed the reference's f32 ray directions must bring it within 1e-3, i.e. the
+    direction definition (phi = k GA rounded in f32, up to ~3e-5 rad at k ~ 60), not the
+    FP32 arithmetic, accounts for it. Any other outlier fails the test."""
     w, gpu, refs, _ = room_run
     for step in range(2):
         g = interior_irr(gpu[step]["irr"])
         r32 = interior_irr(refs["32"][step]["irr"])
         r64 = interior_irr(refs["64"][step]["irr"])
+        r64f = interior_irr(refs["64f"][step]["irr"])
         e32 = rel_err(g[..., :3], r32[..., :3])
         e64 = rel_err(g[..., :3], r64[..., :3])
-        print(f"\nstep {step} irradiance rel err vs FP32 {summary(e32)} vs FP64 {summary(e64)}")
-        assert np.percentile(e32, 99) <= 1e-4
+        e64f = rel_err(g[..., :3], r64f[..., :3])
+        out = e64.max(axis=-1) > 1e-3
+        explained = out & (e64f.max(axis=-1) <= 1e-3)
+        print(f"\nstep {step} irradiance rel err vs FP32 {summary(e32)} vs FP64 {summary(e64)} "
+              f"vs FP64 with f32 directions {summary(e64f)}; texels off FP64 by > 1e-3: {int(out.sum())} "
+              f"of {out.size}, explained by the f32 directions: {int(explained.sum())}")
+        assert np.percentile(e32, 99) <= 1e-4 and e32.max() <= 1e-3
         assert np.percentile(e64, 99.9) <= 1e-3
+        assert np.percentile(e64f, 99.9) <= 1e-4 and e64f.max() <= 1e-3
+        assert int((out & ~explained).sum()) == 0
 
 
 def test_room_depth_and_atlas(room_run):

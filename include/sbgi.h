@@ -375,6 +375,16 @@ int32_t sbgi_set_counting(sbgi_field *f, int32_t on);
  * {t, tri (int32 bits), inst (int32 bits), blocked} per ray. Synchronous. */
 int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *out4);
 
+/* sbgi_query_rays without device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory once running (per-frame query loops, hosts that
+ * forbid allocation after start-up). Same rays, modes and results as sbgi_query_rays; the
+ * staging and the walk's counters live in a library-owned static device array of 776 KiB
+ * per GPU (held until the process ends), so n is limited to SBGI_NOALLOC_MAX_RAYS
+ * (SBGI_E_CAPACITY above it). Every call on one GPU shares that array: calls that overlap
+ * on one GPU (from several threads or fields) race. Synchronous. */
+#define SBGI_NOALLOC_MAX_RAYS 16384
+int32_t sbgi_query_rays_noalloc(sbgi_field *f, const float *rays8, uint32_t n, float *out4);
+
 /* ---- receiver sampling: the probe field at N points ------------------------- */
 /* The receiver's irradiance gather (GiProbeNode.sampleIrradiance / _sampleCascade /
  * _tapEW, js/gi_probes.js:480-614, 825-847) evaluated for a batch of points instead of

@@ -39,7 +39,7 @@ REFLECT_OFF, REFLECT_ROUGH, REFLECT_HIGH, REFLECT_ULTRA = 0, 1, 2, 3
 EXPORTS = [
     "sbgi_tick_params_default", "sbgi_version", "sbgi_create", "sbgi_set_scene", "sbgi_update_ranges",
     "sbgi_refit", "sbgi_set_instance_transforms", "sbgi_tick", "sbgi_wait", "sbgi_poll", "sbgi_read",
-    "sbgi_write", "sbgi_buffer_bytes", "sbgi_query_rays", "sbgi_get_stats", "sbgi_set_counting", "sbgi_last_error",
+    "sbgi_write", "sbgi_buffer_bytes", "sbgi_query_rays", "sbgi_query_rays_noalloc", "sbgi_get_stats", "sbgi_set_counting", "sbgi_last_error",
     "sbgi_destroy", "sbgi_sb_create", "sbgi_sb_add_material", "sbgi_sb_add_mesh", "sbgi_sb_add_instance",
     "sbgi_sb_add_light", "sbgi_sb_finalize", "sbgi_sb_tlas_order", "sbgi_sb_set_transform",
     "sbgi_sb_last_error", "sbgi_sb_destroy", "sbgi_diff_param_count", "sbgi_diff_albedo_map", "sbgi_diff_forward",
@@ -177,6 +177,7 @@ def lib() -> C.CDLL:
         "sbgi_get_stats": (i32, [vp, C.POINTER(Stats)]),
         "sbgi_set_counting": (i32, [vp, i32]),
         "sbgi_query_rays": (i32, [vp, vp, u32, vp]),
+        "sbgi_query_rays_noalloc": (i32, [vp, vp, u32, vp]),
         "sbgi_sample_params_default": (None, [C.POINTER(SampleParams)]),
         "sbgi_sample_irradiance": (i32, [vp, C.POINTER(SampleParams), vp, vp, vp, u32, vp, vp]),
         "sbgi_diff_param_count": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -23,6 +24,40 @@
 #include "sbgi_host.h"
 
 using namespace sbgi;
+
+// ---- device scratch of sbgi_query_rays_noalloc ---------------------------------------
+// The walk's fetch counters, then the staged query records (32 B per ray) and results
+// (16 B per ray) of at most SBGI_NOALLOC_MAX_RAYS rays: 776 KiB per GPU, held until the
+// process ends. Host code reaches it only through the address that hipGetSymbolAddress
+// returns on each device (g_scratch_dev): the array's own name is a host-side stand-in.
+constexpr size_t kNoAllocHeadBytes = (size_t)kFetchHeads * kHeadStride * sizeof(uint32_t);
+constexpr size_t kNoAllocBytes = kNoAllocHeadBytes + (size_t)SBGI_NOALLOC_MAX_RAYS * (32 + 16);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kScratchDevices] = {};
+
+// Once per device, with that device current: the device address of its scratch copy.
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_dev[device] = static_cast<unsigned char *>(p);
+    return hipSuccess;
+}
+
+unsigned char *scratch_of(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+}  // namespace
 
 namespace {
 
@@ -381,6 +416,8 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
         return code;
     };
     if (hipSetDevice(desc->device) != hipSuccess) return bail(fail(f, SBGI_E_DEVICE, "hipSetDevice failed"));
+    if (scratch_setup(desc->device) != hipSuccess)
+        return bail(fail(f, SBGI_E_DEVICE, "device address of the query scratch not available"));
     if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(f, SBGI_E_DEVICE, "hipStreamCreate failed"));
     for (auto &e : f->ring)
@@ -1409,6 +1446,28 @@ int32_t sbgi_query_rays(sbgi_field *f, const float *rays8, uint32_t n, float *ou
     dfree(din);
     dfree(dout);
     if (e != hipSuccess) return hip_fail(f, e, "sbgi_query_rays");
+    return SBGI_OK;
+}
+
+int32_t sbgi_query_rays_noalloc(sbgi_field *f, const float *rays8, uint32_t n, float *out4) {
+    if (!f || (n && (!rays8 || !out4))) return SBGI_E_INVALID;
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    if (n > SBGI_NOALLOC_MAX_RAYS)
+        return fail(f, SBGI_E_CAPACITY, "sbgi_query_rays_noalloc takes at most SBGI_NOALLOC_MAX_RAYS (16384) rays per call");
+    if (n == 0) return SBGI_OK;
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    unsigned char *base = scratch_of(f->desc.device);
+    if (!base) return fail(f, SBGI_E_STATE, "query scratch not set up on this device");
+    uint32_t *heads = reinterpret_cast<uint32_t *>(base);
+    float4 *din = reinterpret_cast<float4 *>(base + kNoAllocHeadBytes);
+    float4 *dout = din + 2 * (size_t)SBGI_NOALLOC_MAX_RAYS;
+    // the same kernels as sbgi_query_rays (launch_query clears the fetch counters first);
+    // only where the staging and counters live differs
+    hipError_t e = hipMemcpyAsync(din, rays8, (size_t)n * 32, hipMemcpyHostToDevice, f->stream);
+    if (e == hipSuccess) e = launch_query(make_scene(f), din, dout, n, heads, f->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out4, dout, (size_t)n * 16, hipMemcpyDeviceToHost, f->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+    if (e != hipSuccess) return hip_fail(f, e, "sbgi_query_rays_noalloc");
     return SBGI_OK;
 }
 

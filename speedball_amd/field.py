@@ -229,9 +229,10 @@ class ProbeField:
         self._chk(self.L.sbgi_diff_timing(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def query(self, ro, rd, tmax=None, mode=0):
-        """Closest-hit (mode 0) or any-hit (mode 1) queries of the resident scene.
-        Returns (t, tri, inst, blocked) arrays."""
+    def query(self, ro, rd, tmax=None, mode=0, noalloc=False):
+        """Closest-hit (mode 0) or any-hit (mode 1) queries of the resident scene, through
+        sbgi_query_rays or (noalloc) sbgi_query_rays_noalloc. Returns (t, tri, inst, blocked)
+        arrays."""
         ro = np.asarray(ro, np.float32).reshape(-1, 3)
         rd = np.asarray(rd, np.float32).reshape(-1, 3)
         n = ro.shape[0]
@@ -241,7 +242,8 @@ class ProbeField:
         rec[:, 4:7] = rd
         rec[:, 7] = mode
         out = np.zeros((n, 4), np.float32)
-        self._chk(self.L.sbgi_query_rays(self.h, N.ptr(rec), n, N.ptr(out)))
+        fn = self.L.sbgi_query_rays_noalloc if noalloc else self.L.sbgi_query_rays
+        self._chk(fn(self.h, N.ptr(rec), n, N.ptr(out)))
         return out[:, 0], out[:, 1].view(np.int32), out[:, 2].view(np.int32), out[:, 3] > 0.5
 
     def sample(self, pos, nvis, ndir=None, **params):

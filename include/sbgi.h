@@ -140,6 +140,8 @@ typedef struct sbgi_field_desc {
                                       cells (clusteredLighting, js/gi_probes.js:1097,
                                       1888-1938, 2064-2127); needs cascade 0 res >= 2 on
                                       every axis. Same radiance as the all-lights loop.  */
+#define SBGI_FIELD_NO_GRAPH 4u     /* every tick launches its kernels eagerly instead of
+                                      replaying the tick's captured hipGraph             */
 
 /* Emitter lights (type 3 records, js/spectral_scene.js:815-875) injected as virtual rays
    (js/gi_probes.js:133-134, 2216-2310): at most this many per scene. */
@@ -244,7 +246,7 @@ typedef struct sbgi_stats {
     uint64_t walk_waves[4];
     uint64_t shadow_waves[4];
     float ms_emitter_vis;        /* emitter visibility rays (scenes with emitters)    */
-    uint32_t pad;
+    uint32_t tick_replayed;      /* 1 when the last tick replayed its captured hipGraph */
 } sbgi_stats;
 
 /* Wave schedule of the persistent walks (no reference counterpart: it is how this build
@@ -338,7 +340,10 @@ int32_t sbgi_edit_timing(sbgi_field *f, float *ms_skin, float *ms_blas_refit, fl
 
 /* One solve of one cascade: trace -> blend -> upload, the ordered pass of
  * renderer.computeAsync(gpu.solveKernels) (js/gi_probes.js:3729-3836). Enqueues on the
- * field's stream and returns; *ticket identifies the submission for sbgi_wait/poll. */
+ * field's stream and returns; *ticket identifies the submission for sbgi_wait/poll. The
+ * solve is one hipGraph per frame: a tick whose kernel arguments (scene layout, cascade,
+ * parameters) equal the last captured tick's replays its graph, another tick is captured
+ * and launched; needs_clear ticks and SBGI_FIELD_NO_GRAPH fields launch eagerly. */
 int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket);
 
 /* Block until the ticket has completed (timeout_ms < 0 waits forever). */

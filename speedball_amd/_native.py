@@ -24,6 +24,7 @@ BLEND_FP32, BLEND_BF16, BLEND_BF16X3, BLEND_FP16 = 0, 1, 2, 3
 TRAV_MT_COMPAT, TRAV_MT_ORDERED, TRAV_WATERTIGHT = 0, 1, 2
 FIELD_KEEP_HIT_IDS = 1
 FIELD_CLUSTERED = 2
+FIELD_NO_GRAPH = 4
 MAX_EMITTERS = 16
 LIGHTS_PER_CELL = 16
 EMITTER_RETENTION = 0.8
@@ -102,7 +103,7 @@ class Stats(C.Structure):
                 ("ms_resolve", C.c_float), ("shadow_blas_nodes", C.c_uint64), ("shadow_tlas_nodes", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64),
                 ("walk_waves", C.c_uint64 * 4), ("shadow_waves", C.c_uint64 * 4), ("ms_emitter_vis", C.c_float),
-                ("pad", C.c_uint32)]
+                ("tick_replayed", C.c_uint32)]
 
 
 class SkinSet(C.Structure):

@@ -907,6 +907,126 @@ int32_t sbgi_skin(sbgi_field *f, const float *bones, uint32_t n_bones, int32_t u
     return SBGI_OK;
 }
 
+// Every by-value kernel argument and launch choice of one tick: a captured tick graph may
+// be replayed only while this is unchanged, byte for byte.
+static void tick_key(const sbgi_field *f, const CascadeState &cs, const DevScene &s, const DevCascade &c,
+                     const sbgi_tick_params &p, std::vector<unsigned char> &key) {
+    key.clear();
+    auto put = [&](const void *v, size_t n) {
+        const unsigned char *b = static_cast<const unsigned char *>(v);
+        key.insert(key.end(), b, b + n);
+    };
+    put(&s, sizeof(s));
+    put(&c, sizeof(c));
+    put(&p, sizeof(p));
+    const void *ptrs[] = {f->dirs, f->rays, f->hit_ids, f->counters, f->tw.hits, f->tw.vis, f->tw.queue.rec,
+                          f->tw.queue.count, cs.irr, cs.depth, cs.meta, cs.atlas, cs.datlas, cs.evis, cs.rough,
+                          cs.gnum, cs.gden, cs.ratlas, cs.gatlas, f->rtables};
+    put(ptrs, sizeof(ptrs));
+    const uint32_t words[] = {f->tw.queue.capacity, f->count_mode ? 1u : 0u, f->desc.blend_precision,
+                              f->desc.reflections};
+    put(words, sizeof(words));
+}
+
+static void drop_tick_graph(sbgi_field *f) {
+    if (f->tick_exec) (void)hipGraphExecDestroy(f->tick_exec);
+    if (f->tick_graph) (void)hipGraphDestroy(f->tick_graph);
+    f->tick_exec = nullptr;
+    f->tick_graph = nullptr;
+    f->tick_key.clear();
+}
+
+// The ordered solve of one tick (trace -> emitter visibility -> reflections + blend ->
+// upload) with its stage events, marked through mk.
+static int32_t enqueue_tick(sbgi_field *f, CascadeState &cs, const DevScene &s, const DevCascade &c,
+                            const sbgi_tick_params *p, hipStream_t st, const StageMarker &mk) {
+    HIP_OK(f, mk(f->ev[0], st));
+    if (p->needs_clear) {
+        HIP_OK(f, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
+        HIP_OK(f, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
+        HIP_OK(f, launch_fill_u32(reinterpret_cast<uint32_t *>(cs.evis), 0u, cs.total * SBGI_MAX_EMITTERS, st));
+        HIP_OK(f, clear_refl_atlases(cs, st));
+    }
+    HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
+    HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
+    if (f->rtables) HIP_OK(f, launch_weights_refl(c.rays, f->dirs, f->rt, f->desc.blend_precision, st));
+    HIP_OK(f, mk(f->ev[1], st));
+    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
+                           f->count_mode ? f->counters : nullptr, f->tev, st, mk));
+    HIP_OK(f, mk(f->eev, st));
+    HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.queue.count + trace_heads(3), st));
+    HIP_OK(f, mk(f->ev[2], st));
+    HIP_OK(f, launch_reflect_and_blend(f, cs, c, *p, f->rays, st));
+    HIP_OK(f, mk(f->ev[3], st));
+    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st, cs.rough, cs.ratlas));
+    HIP_OK(f, mk(f->ev[4], st));
+    return SBGI_OK;
+}
+
+// Builds a tick graph as captured pieces separated by event-record nodes: each stage
+// boundary closes the piece captured so far and starts the next one. (Event records made
+// during a stream capture are not graph nodes on every HIP runtime; explicit event nodes
+// between child graphs are, so replays time their stages like eager ticks.)
+struct TickGraphBuild {
+    std::vector<hipGraph_t> pieces;
+    std::vector<hipEvent_t> events;  // events[i] follows pieces[i]
+    static hipError_t mark(void *ctx, hipEvent_t e, hipStream_t st) {
+        TickGraphBuild *b = static_cast<TickGraphBuild *>(ctx);
+        hipGraph_t g = nullptr;
+        hipError_t err = hipStreamEndCapture(st, &g);
+        if (err != hipSuccess) return err;
+        b->pieces.push_back(g);
+        b->events.push_back(e);
+        return hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    }
+    ~TickGraphBuild() {
+        for (hipGraph_t g : pieces)
+            if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+static hipError_t build_tick_graph(sbgi_field *f, CascadeState &cs, const DevScene &s, const DevCascade &c,
+                                   const sbgi_tick_params *p, hipStream_t st, hipGraph_t *out) {
+    *out = nullptr;
+    TickGraphBuild b;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const StageMarker mk{&TickGraphBuild::mark, &b};
+    const int32_t rc = enqueue_tick(f, cs, s, c, p, st, mk);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(st, &cap);
+    hipGraph_t last = nullptr;
+    if (cap != hipStreamCaptureStatusNone) {
+        const hipError_t ee = hipStreamEndCapture(st, &last);
+        if (e == hipSuccess) e = ee;
+    }
+    b.pieces.push_back(last);
+    if (rc != SBGI_OK) return hipErrorUnknown;
+    if (e != hipSuccess) return e;
+    hipGraph_t g = nullptr;
+    if ((e = hipGraphCreate(&g, 0)) != hipSuccess) return e;
+    hipGraphNode_t prev = nullptr;
+    for (size_t i = 0; i < b.pieces.size() && e == hipSuccess; i++) {
+        size_t nodes = 0;
+        if (b.pieces[i] && hipGraphGetNodes(b.pieces[i], nullptr, &nodes) == hipSuccess && nodes) {
+            hipGraphNode_t n = nullptr;
+            e = hipGraphAddChildGraphNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, b.pieces[i]);
+            prev = n;
+        }
+        if (e == hipSuccess && i < b.events.size()) {
+            hipGraphNode_t n = nullptr;
+            e = hipGraphAddEventRecordNode(&n, g, prev ? &prev : nullptr, prev ? 1 : 0, b.events[i]);
+            prev = n;
+        }
+    }
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return e;
+    }
+    *out = g;
+    return hipSuccess;
+}
+
 int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     if (!f || !p) return SBGI_E_INVALID;
     if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene: call sbgi_set_scene first");
@@ -919,30 +1039,47 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     const DevScene s = make_scene(f);
     hipStream_t st = f->stream;
     f->diff_valid = false;
-    HIP_OK(f, hipEventRecord(f->ev[0], st));
-    if (p->needs_clear) {
-        HIP_OK(f, launch_clear_atlas(cs.atlas, (size_t)cs.aw * cs.ah, st));
-        HIP_OK(f, launch_clear_atlas(cs.datlas, (size_t)cs.daw * cs.dah, st));
-        HIP_OK(f, launch_fill_u32(reinterpret_cast<uint32_t *>(cs.evis), 0u, cs.total * SBGI_MAX_EMITTERS, st));
-        HIP_OK(f, clear_refl_atlases(cs, st));
+    // One hipGraph per frame: a tick whose kernel arguments equal the captured tick's
+    // replays it; another is captured and launched (the first tick, needs_clear ticks and
+    // fields created with SBGI_FIELD_NO_GRAPH run eagerly). Device data (the scene, the
+    // temporal state) is read at replay time, so edits between ticks need no recapture.
+    const bool graph = !(f->desc.flags & SBGI_FIELD_NO_GRAPH) && !p->needs_clear && !f->tick_graph_failed;
+    bool replayed = false, eager = !graph;
+    if (graph) {
+        tick_key(f, cs, s, c, *p, f->scratch_key);
+        if (f->tick_exec && f->scratch_key == f->tick_key) {
+            HIP_OK(f, hipGraphLaunch(f->tick_exec, st));
+            f->tick_replays++;
+            replayed = true;
+        } else {
+            drop_tick_graph(f);
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            hipError_t e = build_tick_graph(f, cs, s, c, p, st, &g);
+            if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            if (e == hipSuccess) {
+                f->tick_graph = g;
+                f->tick_exec = x;
+                f->tick_key = f->scratch_key;
+                HIP_OK(f, hipGraphLaunch(f->tick_exec, st));
+            } else {
+                // graphs not available here: this field ticks eagerly from now on
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                f->tick_graph_failed = true;
+                eager = true;
+            }
+        }
     }
-    HIP_OK(f, launch_raydirs(f->dirs, c.rays, p->frame_jitter, st));
-    HIP_OK(f, launch_weights(c, *p, f->dirs, f->bt, f->desc.blend_precision, st));
-    if (f->rtables) HIP_OK(f, launch_weights_refl(c.rays, f->dirs, f->rt, f->desc.blend_precision, st));
-    HIP_OK(f, hipEventRecord(f->ev[1], st));
-    HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
-                           f->count_mode ? f->counters : nullptr, f->tev, st));
-    HIP_OK(f, hipEventRecord(f->eev, st));
-    HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.queue.count + trace_heads(3), st));
-    HIP_OK(f, hipEventRecord(f->ev[2], st));
-    HIP_OK(f, launch_reflect_and_blend(f, cs, c, *p, f->rays, st));
-    HIP_OK(f, hipEventRecord(f->ev[3], st));
-    HIP_OK(f, launch_upload(c, *p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st, cs.rough, cs.ratlas));
-    HIP_OK(f, hipEventRecord(f->ev[4], st));
+    if (eager) {
+        const int32_t rc = enqueue_tick(f, cs, s, c, p, st, StageMarker{});
+        if (rc != SBGI_OK) return rc;
+    }
     f->ticket++;
     HIP_OK(f, hipEventRecord(f->ring[f->ticket % kTicketRing], st));
     f->last_updated = updated;
     f->last_cascade = p->cascade;
+    f->last_replayed = replayed;
     f->stats_valid = true;
     if (ticket) *ticket = f->ticket;
     return SBGI_OK;
@@ -1161,6 +1298,7 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     out->ms_emit = sub[1];
     out->ms_walk_shadow = sub[2];
     out->ms_resolve = sub[3];
+    out->tick_replayed = f->last_replayed ? 1u : 0u;
     return SBGI_OK;
 }
 
@@ -1382,6 +1520,7 @@ int32_t sbgi_destroy(sbgi_field *f) {
     if (!f) return SBGI_OK;
     (void)hipSetDevice(f->desc.device);
     if (f->stream) (void)hipStreamSynchronize(f->stream);
+    drop_tick_graph(f);
     free_scene(f);
     for (auto &cs : f->casc) {
         dfree(cs.irr);

@@ -117,6 +117,12 @@ struct sbgi_field {
     hipEvent_t ev[5] = {};
     hipEvent_t tev[4] = {};  // trace sub-stages
     hipEvent_t eev = nullptr;  // end of the trace, before the emitter visibility rays
+    // the captured tick (one hipGraph per frame) and the kernel arguments it was captured with
+    hipGraph_t tick_graph = nullptr;
+    hipGraphExec_t tick_exec = nullptr;
+    std::vector<unsigned char> tick_key, scratch_key;
+    uint64_t tick_replays = 0;
+    bool tick_graph_failed = false, last_replayed = false;
     // reflection lobe weight tables (desc.reflections != OFF)
     void *rtables = nullptr;
     ReflTables rt{};

@@ -491,10 +491,10 @@ void launch_walk(const DevScene &s, const Src &src, const uint32_t *n_dev, uint3
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                         const float *meta, const ushort4 *prev_atlas, const float4 *prev_atlas_f32,
                         const TraceScratch &w, float4 *rays, int2 *hit_ids, Counters *counters, const hipEvent_t *ev,
-                        hipStream_t st) {
+                        hipStream_t st, const StageMarker &marker) {
     const uint32_t n = c.updated * c.rays;
     if (n == 0) return hipSuccess;
-    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
+    auto mark = [&](int i) { return ev ? marker(ev[i], st) : hipSuccess; };
     uint32_t *ctr = w.queue.count;
     const uint32_t nclear = std::max(n, 1u + 4u * kFetchHeads);
     hipLaunchKernelGGL(k_clear_u64, dim3((nclear + 255) / 256), dim3(256), 0, st, w.vis, n, ctr);

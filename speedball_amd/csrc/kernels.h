@@ -33,14 +33,23 @@ constexpr size_t kTraceLdsBudget = 64 * 1024;  // TLAS + instances staged in LDS
 size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count);
 hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, uint32_t *head, hipStream_t st);
 hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
-// ev (optional): 4 events recorded before the closest-hit walk, after it, after the
+// How a solve marks a stage boundary with an event: recorded on the stream (fn null), or
+// handed to fn, which the tick-graph builder uses to close the captured piece and place the
+// event as a graph node between pieces.
+struct StageMarker {
+    hipError_t (*fn)(void *ctx, hipEvent_t e, hipStream_t st) = nullptr;
+    void *ctx = nullptr;
+    hipError_t operator()(hipEvent_t e, hipStream_t st) const { return fn ? fn(ctx, e, st) : hipEventRecord(e, st); }
+};
+
+// ev (optional): 4 events marked before the closest-hit walk, after it, after the
 // shadow-ray emission and after the shadow walk.
 // prev_atlas_f32 (optional): an FP32 bounce source used instead of the RGBA16F prev_atlas
 // (the differentiable mode).
 hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                         const float *meta, const ushort4 *prev_atlas, const float4 *prev_atlas_f32,
                         const TraceScratch &w, float4 *rays, int2 *hit_ids, Counters *counters, const hipEvent_t *ev,
-                        hipStream_t st);
+                        hipStream_t st, const StageMarker &mark = StageMarker{});
 
 // Batched trace (batch.hip): one segment per (field, cascade) solve of a frame. The two
 // walks of a frame are single persistent launches over all segments; emission and

@@ -418,6 +418,7 @@ static napi_value js_field_stats(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, o, "msWalkShadow", make_f64(env, s.ms_walk_shadow));
     napi_set_named_property(env, o, "msResolve", make_f64(env, s.ms_resolve));
     napi_set_named_property(env, o, "msEmitterVis", make_f64(env, s.ms_emitter_vis));
+    napi_set_named_property(env, o, "tickReplayed", make_f64(env, (double)s.tick_replayed));
     return o;
 }
 

@@ -72,8 +72,13 @@ extern "C" {
                                    octant copies of the BLAS pool; same triangle test, so
                                    results differ only in which of two exactly equal-t
                                    hits wins */
-#define SBGI_TRAV_WATERTIGHT 2  /* reserved (watertight triangle test, Woop et al. 2013);
-                                   sbgi_create rejects it in this build */
+#define SBGI_TRAV_WATERTIGHT 2  /* the MT_ORDERED walk with the watertight triangle test
+                                   (Woop, Benthin and Wald 2013) on the stored vertices in
+                                   place of Moller-Trumbore: a ray through an edge or vertex
+                                   shared by two triangles hits one of them; hits differ from
+                                   MT's only at edges and near ties (no reference
+                                   counterpart: the reference has Moller-Trumbore only,
+                                   js/spectral_traverse.js:388-409) */
 
 /* local reflection lobes from the same rays (reflectionQuality, REFLECTION_QUALITY_TIERS
    js/gi_probes.js:60-73): the rough power-8 lobe in the irradiance tiles, and the glossy

@@ -57,6 +57,12 @@ def set_f32_dirs(on: bool) -> None:
     lib().or64_set_f32_dirs(C.c_int(1 if on else 0))
 
 
+def set_watertight(on: bool, prec="32") -> None:
+    """Replace the Moller-Trumbore test by the watertight test (Woop et al. 2013) in every
+    walk of the given precision until switched off."""
+    _fn(prec, "set_watertight")(C.c_int(1 if on else 0))
+
+
 def ray_dir(k: int, rays: int, jitter: float, prec="64") -> np.ndarray:
     out = np.zeros(3, _dt(prec))
     f = _fn(prec, "ray_dir")

@@ -153,7 +153,16 @@ static int P(alpha_accepts)(const sbgi_scene_view *s, uint32_t m) {
 static REAL P(fmin)(REAL a, REAL b) { return RFMIN(a, b); }
 static REAL P(fmax)(REAL a, REAL b) { return RFMAX(a, b); }
 
-/* slab test shared by both levels: spectral_traverse.js:372-377, 446-451 */
+/* watertight mode (set_watertight): the watertight triangle test below, and box tests made
+ * conservative, so that a ray meeting a triangle on the face of its box is not culled */
+static int P(g_wt) = 0;
+void P(set_watertight)(int on) { P(g_wt) = on; }
+
+/* slab test shared by both levels: spectral_traverse.js:372-377, 446-451. In watertight
+ * mode the exit distance is scaled by 1 + 4 gamma_3 (= 1 + 6 f32 ulps, gamma_n = n u /
+ * (1 - n u)), twice the margin that covers the roundings of (b - o) * (1/d) (Ize 2013,
+ * "Robust BVH ray traversal", JCGT 2(2)). */
+#define ROBUST_FAR 1.0000007152557373
 static void P(slab)(const REAL bmin[3], const REAL bmax[3], const REAL ro[3], const REAL invd[3], REAL *tn,
                     REAL *tf) {
     REAL t0[3], t1[3];
@@ -163,12 +172,62 @@ static void P(slab)(const REAL bmin[3], const REAL bmax[3], const REAL ro[3], co
     }
     *tn = P(fmax)(P(fmax)(P(fmin)(t0[0], t1[0]), P(fmin)(t0[1], t1[1])), P(fmin)(t0[2], t1[2]));
     *tf = P(fmin)(P(fmin)(P(fmax)(t0[0], t1[0]), P(fmax)(t0[1], t1[1])), P(fmax)(t0[2], t1[2]));
+    if (P(g_wt)) *tf = *tf * RCONST(ROBUST_FAR);
+}
+
+/* Watertight ray/triangle test (Woop, Benthin and Wald 2013, JCGT 2(1)): the build's
+ * SBGI_TRAV_WATERTIGHT mode in place of Moller-Trumbore. The triangle is translated to
+ * the ray origin and sheared so the ray runs along +z (axis kz = the largest |d|, kx, ky
+ * the next two, swapped when d[kz] < 0 to keep the winding); the three 2-D edge functions
+ * U, V, W of the sheared vertices decide the hit, recomputed in double precision when one
+ * is exactly zero, so two triangles that share an edge cannot both miss a ray through it.
+ * t = (U z0 + V z1 + W z2) / (U + V + W). U + V + W has the sign of the Moller-Trumbore
+ * determinant (positive when the ray meets the counter-clockwise front), so the side rules
+ * read it unchanged. Vertices are read exactly as stored (no edge vectors). */
+static int P(wt)(const sbgi_scene_view *s, uint32_t tri, const REAL ro[3], const REAL rd[3], REAL *t_out,
+                 REAL *det_out) {
+    REAL p[3][3];
+    for (int i = 0; i < 3; i++) P(vert)(s, s->tri_index[tri * 3 + i], p[i]);
+    REAL ad[3] = {RABS(rd[0]), RABS(rd[1]), RABS(rd[2])};
+    int kz = ad[0] > ad[1] ? (ad[0] > ad[2] ? 0 : 2) : (ad[1] > ad[2] ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    if (rd[kz] < 0) {
+        int tmp = kx;
+        kx = ky;
+        ky = tmp;
+    }
+    REAL sx = rd[kx] / rd[kz], sy = rd[ky] / rd[kz], sz = RCONST(1) / rd[kz];
+    REAL X[3], Y[3], Z[3];
+    for (int i = 0; i < 3; i++) {
+        REAL a[3] = {p[i][0] - ro[0], p[i][1] - ro[1], p[i][2] - ro[2]};
+        X[i] = a[kx] - sx * a[kz];
+        Y[i] = a[ky] - sy * a[kz];
+        Z[i] = sz * a[kz];
+    }
+    /* U: edge (B, C), V: edge (C, A), W: edge (A, B) for A, B, C = vertices 0, 1, 2 */
+    REAL U = X[2] * Y[1] - Y[2] * X[1];
+    REAL V = X[0] * Y[2] - Y[0] * X[2];
+    REAL W = X[1] * Y[0] - Y[1] * X[0];
+    if (U == 0 || V == 0 || W == 0) {
+        U = (REAL)((double)X[2] * (double)Y[1] - (double)Y[2] * (double)X[1]);
+        V = (REAL)((double)X[0] * (double)Y[2] - (double)Y[0] * (double)X[2]);
+        W = (REAL)((double)X[1] * (double)Y[0] - (double)Y[1] * (double)X[0]);
+    }
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return 0;
+    REAL det = U + V + W;
+    if (det == 0) return 0;
+    REAL T = U * Z[0] + V * Z[1] + W * Z[2];
+    *t_out = T / det;
+    *det_out = det;
+    return 1;
 }
 
 /* Moller-Trumbore as in spectral_traverse.js:388-409. Returns 1 and fills t/det when the
- * barycentric tests pass. */
+ * barycentric tests pass (the watertight test instead while watertight mode is set). */
 static int P(mt)(const sbgi_scene_view *s, uint32_t tri, const REAL ro[3], const REAL rd[3], REAL *t_out,
                  REAL *det_out) {
+    if (P(g_wt)) return P(wt)(s, tri, ro, rd, t_out, det_out);
     REAL p0[3], p1[3], p2[3], e1[3], e2[3], pv[3], tv[3], qv[3];
     P(vert)(s, s->tri_index[tri * 3 + 0], p0);
     P(vert)(s, s->tri_index[tri * 3 + 1], p1);

@@ -104,6 +104,7 @@ static int ray_dir_f32_hook(uint32_t k, uint32_t rays, float jitter, double out[
 #undef VSTRIDE
 
 #undef RAY_DIR_HOOK
+#undef ROBUST_FAR
 
 /* ---------------- FP32 restatement ---------------- */
 #define REAL float

@@ -128,14 +128,16 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
     BHIP_OK(b, launch_fill_u32(reinterpret_cast<uint32_t *>(b->vis[0]), 0u, 2u * b->total_rays, st));
     BHIP_OK(b, launch_fill_u32(b->d_ctr, 0u, n_seg * (uint32_t)(kTraceCounterBytes / 4), st));
     BHIP_OK(b, launch_fill_u32(b->d_done, 0u, 2u * n_seg, st));
+    bool wt = false;
+    for (const sbgi_field *f : b->fields) wt = wt || f->desc.traversal == SBGI_TRAV_WATERTIGHT;
     BHIP_OK(b, launch_walk_batch(0, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done, b->walk_block,
-                                 b->walk_waves, st));
+                                 b->walk_waves, wt, st));
     for (uint32_t j = 0; j < n_seg; j++) {
         const sbgi_tick_params &p = params[b->segs[j].field];
         BHIP_OK(b, launch_emit(ts[j].s, ts[j].c, p, ts[j].dirs, ts[j].meta, ts[j].w.hits, ts[j].w.queue, st));
     }
     BHIP_OK(b, launch_walk_batch(1, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done + n_seg, b->walk_block,
-                                 b->walk_waves, st));
+                                 b->walk_waves, wt, st));
     for (uint32_t j = 0; j < n_seg; j++) {
         const auto &g = b->segs[j];
         sbgi_field *f = b->fields[g.field];

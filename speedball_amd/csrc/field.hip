@@ -401,8 +401,8 @@ int32_t sbgi_create(const sbgi_field_desc *desc, sbgi_field **out) {
     if (desc->depth_oct_res != 6 && desc->depth_oct_res != 14)
         return fail(nullptr, SBGI_E_INVALID, "depth_oct_res must be 6 or 14");
     if (desc->blend_precision > SBGI_BLEND_FP16) return fail(nullptr, SBGI_E_INVALID, "unknown blend precision");
-    if (desc->traversal != SBGI_TRAV_MT_COMPAT && desc->traversal != SBGI_TRAV_MT_ORDERED)
-        return fail(nullptr, SBGI_E_INVALID, "traversal must be MT_COMPAT or MT_ORDERED");
+    if (desc->traversal > SBGI_TRAV_WATERTIGHT)
+        return fail(nullptr, SBGI_E_INVALID, "traversal must be MT_COMPAT, MT_ORDERED or WATERTIGHT");
     if (desc->reflections > SBGI_REFLECT_ULTRA) return fail(nullptr, SBGI_E_INVALID, "unknown reflection quality");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || desc->device < 0 || desc->device >= ndev)

@@ -214,7 +214,10 @@ inline ReflState refl_state(const CascadeState &cs) {
     return ReflState{cs.rough, cs.ratlas, cs.gnum, cs.gden, cs.gatlas, cs.gtiles_x, cs.gaw, cs.gah};
 }
 
-inline bool ordered(const sbgi_field *f) { return f->desc.traversal == SBGI_TRAV_MT_ORDERED; }
+// MT_ORDERED and WATERTIGHT walk the SAH TLAS and the near-first octant copies
+inline bool ordered(const sbgi_field *f) {
+    return f->desc.traversal == SBGI_TRAV_MT_ORDERED || f->desc.traversal == SBGI_TRAV_WATERTIGHT;
+}
 
 inline DevScene make_scene(const sbgi_field *f) {
     DevScene s;
@@ -245,6 +248,9 @@ inline DevScene make_scene(const sbgi_field *f) {
     }
     s.emitter_base = f->view.emitter_base;
     s.emitter_count = f->view.emitter_count;
+    s.watertight = f->desc.traversal == SBGI_TRAV_WATERTIGHT ? 1u : 0u;
+    s.tri_index = f->d_tri_index;
+    s.vdata = f->d_vdata;
     return s;
 }
 

@@ -136,14 +136,14 @@ struct QuerySrc {
 
 // Persistent walk: every wave pulls rays from the source until it is drained. The number
 // of rays is n_host, or min(*n_dev, n_host) when the count is only known on the device.
-template <int MODE, bool COUNT, bool LDS, class Src>
+template <int MODE, bool COUNT, bool LDS, class Src, bool WT = false>
 __global__ void __launch_bounds__(kWalkBlock) k_walk(DevScene s, Src src, const uint32_t *__restrict__ n_dev,
                                                       uint32_t n_host, uint32_t *heads,
                                                       unsigned long long *__restrict__ counters) {
     if (LDS) stage_tlas_lds(s);
     const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     TravCounters<COUNT> cnt;
-    walk_wave<MODE, COUNT, LDS>(s, src, n, heads, blockIdx.x * (kWalkBlock / 64u) + (threadIdx.x >> 6), cnt);
+    walk_wave<MODE, COUNT, LDS, WT>(s, src, n, heads, blockIdx.x * (kWalkBlock / 64u) + (threadIdx.x >> 6), cnt);
     if (COUNT) {
         atomicAdd(&counters[0], (unsigned long long)cnt.blas);
         atomicAdd(&counters[1], (unsigned long long)cnt.tlas);
@@ -172,7 +172,7 @@ struct WalkSeg {
     uint32_t *heads;
 };
 
-template <int MODE, int BLOCK, class Src>
+template <int MODE, int BLOCK, class Src, bool WT = false>
 __global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__restrict__ segs, uint32_t n_seg,
                                                       const uint32_t *__restrict__ prefix, uint32_t total,
                                                       uint32_t *__restrict__ done) {
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__rest
         // scene and source fields on every step
         const DevScene sc = g.s;
         const Src src = g.src;
-        walk_wave<MODE, false, false>(sc, src, n, g.heads, wave_id, cnt);
+        walk_wave<MODE, false, false, WT>(sc, src, n, g.heads, wave_id, cnt);
         if ((threadIdx.x & 63u) == 0) __hip_atomic_store(&done[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -412,7 +412,7 @@ void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char
 }
 
 hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const uint32_t *prefix, uint32_t total,
-                             uint32_t *done, uint32_t block, uint32_t waves, hipStream_t st) {
+                             uint32_t *done, uint32_t block, uint32_t waves, bool wt, hipStream_t st) {
     if (!n_seg || !total) return hipSuccess;
     const auto *pr = static_cast<const WalkSeg<PrimarySrc> *>(tables);
     const auto *sh = reinterpret_cast<const WalkSeg<ShadowSrc> *>(static_cast<const unsigned char *>(tables) +
@@ -421,12 +421,20 @@ hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const
         hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, bs, 0, total, waves)), dim3(bs), 0, st, table, n_seg,
                            prefix, total, done);
     };
-    if (mode == 0) {
+    // the watertight variants only when a segment's scene asks for it: the watertight test
+    // costs registers that the Moller-Trumbore walks do not pay for
+    if (mode == 0 && !wt) {
         if (block == 256) go(k_walk_batch<WALK_CLOSEST, 256, PrimarySrc>, 256u, pr);
         else go(k_walk_batch<WALK_CLOSEST, 512, PrimarySrc>, 512u, pr);
-    } else {
+    } else if (mode == 0) {
+        if (block == 256) go(k_walk_batch<WALK_CLOSEST, 256, PrimarySrc, true>, 256u, pr);
+        else go(k_walk_batch<WALK_CLOSEST, 512, PrimarySrc, true>, 512u, pr);
+    } else if (!wt) {
         if (block == 256) go(k_walk_batch<WALK_ANY, 256, ShadowSrc>, 256u, sh);
         else go(k_walk_batch<WALK_ANY, 512, ShadowSrc>, 512u, sh);
+    } else {
+        if (block == 256) go(k_walk_batch<WALK_ANY, 256, ShadowSrc, true>, 256u, sh);
+        else go(k_walk_batch<WALK_ANY, 512, ShadowSrc, true>, 512u, sh);
     }
     return hipGetLastError();
 }
@@ -459,9 +467,12 @@ hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32
     hipError_t e = hipMemsetAsync(head, 0, (size_t)kFetchHeads * kHeadStride * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
     const QuerySrc src{in, out};
-    auto kern = k_walk<WALK_MIXED, false, false, QuerySrc>;
-    hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, 0, n)), dim3(kWalkBlock), 0, st, s, src, nullptr,
-                       n, head, nullptr);
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, 0, n)), dim3(kWalkBlock), 0, st, s, src,
+                           nullptr, n, head, nullptr);
+    };
+    if (s.watertight) go(k_walk<WALK_MIXED, false, false, QuerySrc, true>);
+    else go(k_walk<WALK_MIXED, false, false, QuerySrc>);
     return hipGetLastError();
 }
 
@@ -479,7 +490,15 @@ void launch_walk(const DevScene &s, const Src &src, const uint32_t *n_dev, uint3
         hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, l, n_host)), dim3(kWalkBlock), l, st, s, src,
                            n_dev, n_host, heads, cnt);
     };
-    if (cnt) {
+    if (s.watertight) {
+        if (cnt) {
+            if (use_lds) go(k_walk<MODE, true, true, Src, true>, lds);
+            else go(k_walk<MODE, true, false, Src, true>, 0);
+        } else {
+            if (use_lds) go(k_walk<MODE, false, true, Src, true>, lds);
+            else go(k_walk<MODE, false, false, Src, true>, 0);
+        }
+    } else if (cnt) {
         if (use_lds) go(k_walk<MODE, true, true, Src>, lds);
         else go(k_walk<MODE, true, false, Src>, 0);
     } else {

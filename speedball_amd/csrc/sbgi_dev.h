@@ -60,6 +60,11 @@ struct DevScene {
     uint32_t lg_cells[3];
     // emitter records (type 3) injected as virtual rays: lights[emitter_base + e]
     uint32_t emitter_base, emitter_count;
+    // SBGI_TRAV_WATERTIGHT: the walks test triangles with wt_test on the stored vertices
+    // (tri_index into vdata, 8 floats per vertex) instead of Moller-Trumbore on tri
+    uint32_t watertight;
+    const uint32_t *__restrict__ tri_index;
+    const float *__restrict__ vdata;
 };
 
 // albedo entry of (instance slot, pool triangle) in the differentiable mode
@@ -164,22 +169,29 @@ __device__ __forceinline__ bool mt_test(float4 A, float4 B, float4 C, float3 ro,
     return true;
 }
 
-// Watertight ray/triangle test (Woop, Benthin, Wald 2013, JCGT 2(1)): shear the
-// triangle into ray space, signed edge functions with a double-precision fallback
-// when one is exactly zero. Returns det with the sign convention of the
-// Moller-Trumbore det (positive for a counter-clockwise front face seen by the ray).
+// Watertight ray/triangle test (Woop, Benthin and Wald 2013, JCGT 2(1)): the
+// SBGI_TRAV_WATERTIGHT mode's triangle test, in the same operation order as the oracle's
+// restatement so the two agree bit for bit. The triangle is translated to the ray origin and
+// sheared so the ray runs along +z (kz = the axis of the largest |d|, kx and ky the next
+// two, swapped when d[kz] < 0 to keep the winding); the 2-D edge functions U, V, W of the
+// sheared vertices decide the hit, recomputed in double precision when one is exactly zero,
+// so two triangles that share an edge cannot both miss a ray through it. The vertices are
+// the stored ones (not p0 + edge), which the shared-edge argument needs.
+// det (= U + V + W) has the sign of the Moller-Trumbore determinant, so the side rules read
+// it unchanged; t = (U z0 + V z1 + W z2) / det.
 struct WtRay {
     int kx, ky, kz;
     float sx, sy, sz;
 };
+__device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 __device__ __forceinline__ WtRay wt_setup(float3 rd) {
-    float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
-    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+#pragma clang fp contract(off)
+    const float ax = fabsf(rd.x), ay = fabsf(rd.y), az = fabsf(rd.z);
+    const int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
     int ky = kx == 2 ? 0 : kx + 1;
-    float d[3] = {rd.x, rd.y, rd.z};
-    if (d[kz] < 0.f) {
-        int tmp = kx;
+    if (comp(rd, kz) < 0.f) {
+        const int tmp = kx;
         kx = ky;
         ky = tmp;
     }
@@ -187,36 +199,30 @@ __device__ __forceinline__ WtRay wt_setup(float3 rd) {
     w.kx = kx;
     w.ky = ky;
     w.kz = kz;
-    w.sx = d[kx] / d[kz];
-    w.sy = d[ky] / d[kz];
-    w.sz = 1.0f / d[kz];
+    const float dz = comp(rd, kz);
+    w.sx = comp(rd, kx) / dz;
+    w.sy = comp(rd, ky) / dz;
+    w.sz = 1.0f / dz;
     return w;
 }
-__device__ __forceinline__ float comp(float3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
-__device__ __forceinline__ bool wt_test(float4 A, float4 B, float4 C, float3 ro, const WtRay &w, float &t,
-                                        float &det_mt) {
-    float3 p0 = xyz(A);
-    float3 p1 = p0 + xyz(B), p2 = p0 + xyz(C);
-    float3 a = p0 - ro, b = p1 - ro, c = p2 - ro;
-    float ax = comp(a, w.kx) - w.sx * comp(a, w.kz), ay = comp(a, w.ky) - w.sy * comp(a, w.kz);
-    float bx = comp(b, w.kx) - w.sx * comp(b, w.kz), by = comp(b, w.ky) - w.sy * comp(b, w.kz);
-    float cx = comp(c, w.kx) - w.sx * comp(c, w.kz), cy = comp(c, w.ky) - w.sy * comp(c, w.kz);
-    float U = cx * by - cy * bx, V = ax * cy - ay * cx, W = bx * ay - by * ax;
+__device__ __forceinline__ bool wt_test(float3 p0, float3 p1, float3 p2, float3 ro, const WtRay &w, float &t,
+                                        float &det) {
+#pragma clang fp contract(off)
+    const float3 a = p0 - ro, b = p1 - ro, c = p2 - ro;
+    const float x0 = comp(a, w.kx) - w.sx * comp(a, w.kz), y0 = comp(a, w.ky) - w.sy * comp(a, w.kz);
+    const float x1 = comp(b, w.kx) - w.sx * comp(b, w.kz), y1 = comp(b, w.ky) - w.sy * comp(b, w.kz);
+    const float x2 = comp(c, w.kx) - w.sx * comp(c, w.kz), y2 = comp(c, w.ky) - w.sy * comp(c, w.kz);
+    float U = x2 * y1 - y2 * x1, V = x0 * y2 - y0 * x2, W = x1 * y0 - y1 * x0;
     if (U == 0.f || V == 0.f || W == 0.f) {
-        double CxBy = (double)cx * by, CyBx = (double)cy * bx;
-        U = (float)(CxBy - CyBx);
-        double AxCy = (double)ax * cy, AyCx = (double)ay * cx;
-        V = (float)(AxCy - AyCx);
-        double BxAy = (double)bx * ay, ByAx = (double)by * ax;
-        W = (float)(BxAy - ByAx);
+        U = (float)((double)x2 * (double)y1 - (double)y2 * (double)x1);
+        V = (float)((double)x0 * (double)y2 - (double)y0 * (double)x2);
+        W = (float)((double)x1 * (double)y0 - (double)y1 * (double)x0);
     }
     if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
-    float det = U + V + W;
+    det = U + V + W;
     if (det == 0.f) return false;
-    float az = w.sz * comp(a, w.kz), bz = w.sz * comp(b, w.kz), cz = w.sz * comp(c, w.kz);
-    float T = U * az + V * bz + W * cz;
+    const float z0 = w.sz * comp(a, w.kz), z1 = w.sz * comp(b, w.kz), z2 = w.sz * comp(c, w.kz);
+    const float T = U * z0 + V * z1 + W * z2;
     t = T / det;
-    // MT det = dot(e1, cross(rd, e2)) has the sign of -(edge-function sum) in this frame
-    det_mt = -det;
     return true;
 }

@@ -100,7 +100,13 @@ enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
 // (count << 24) | offset (triangles for a BLAS leaf, instance slots for a TLAS leaf).
 // BLAS nodes are addressed by a 32-bit byte offset from the pool: the upload limits a
 // copy to fewer than 2^24 nodes, so 8 copies stay below 4 GiB.
-template <bool COUNT, bool LDS, bool FMA>
+// Exit-distance scale of the watertight walks' box tests: 1 + 4 gamma_3 (= 1 + 6 ulps,
+// gamma_n = n u / (1 - n u)), twice the margin that covers the roundings of (b - o) * (1/d)
+// (Ize 2013, "Robust BVH ray traversal", JCGT 2(2)), so a ray meeting a triangle on the
+// face of its box is never culled.
+constexpr float kRobustFar = 1.0000007152557373f;
+
+template <bool COUNT, bool LDS, bool FMA, bool WT = false>
 __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     uint4 a, b;
     if (r.level) {
@@ -118,6 +124,7 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     float tn, tf;
     if (FMA) slab_fma(bmin, bmax, r.noid, r.id, tn, tf);
     else slab(bmin, bmax, r.o, r.id, tn, tf);
+    if (WT && s.watertight) tf = tf * kRobustFar;
     // tf >= max(tn, 0) as two compares: tn and tf are never NaN (a unit direction has a
     // finite plane distance on some axis, and fminf/fmaxf pass over NaN ones)
     const bool hit = tf >= tn && tf >= 0.f && tn < r.tmax;
@@ -137,8 +144,9 @@ __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCou
     r.state = state;
 }
 
-// TRI: one Moller-Trumbore test of the pending leaf range.
-template <int MODE, bool COUNT>
+// TRI: one Moller-Trumbore test of the pending leaf range (or, in walks compiled with WT
+// for a watertight scene, one watertight test of its stored vertices).
+template <int MODE, bool COUNT, bool WT>
 __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     const bool ANY = MODE == WALK_MIXED ? r.any : MODE == WALK_ANY;
     const uint32_t tri = r.tpend & 0x00FFFFFFu;
@@ -146,7 +154,16 @@ __device__ __forceinline__ void tri_step(const DevScene &s, LaneRay &r, TravCoun
     const float4 A = tp[0], B = tp[1], C = tp[2];
     cnt.add_tri();
     float th = 0.f, det = 0.f;
-    const bool geo = mt_test<MODE == WALK_ANY>(A, B, C, r.o, r.d, th, det) && th > SBGI_RAY_EPS && th < r.tmax;
+    bool hit;
+    if (WT && s.watertight) {  // uniform over the launch
+        const uint32_t *ti = s.tri_index + (size_t)tri * 3;
+        const float4 *vd = reinterpret_cast<const float4 *>(s.vdata);  // 2 float4 per vertex row
+        const float4 v0 = vd[(size_t)ti[0] * 2], v1 = vd[(size_t)ti[1] * 2], v2 = vd[(size_t)ti[2] * 2];
+        hit = wt_test(xyz(v0), xyz(v1), xyz(v2), r.o, wt_setup(r.d), th, det);
+    } else {
+        hit = mt_test<MODE == WALK_ANY>(A, B, C, r.o, r.d, th, det);
+    }
+    const bool geo = hit && th > SBGI_RAY_EPS && th < r.tmax;
     const uint32_t flags = __float_as_uint(A.w);
     const bool accept = geo && side_ok(flags, det * r.dsign) && (flags & (ANY ? 8u : 4u));
     const bool block = ANY && accept, better = !ANY && accept;
@@ -212,7 +229,7 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
 // walk_wave runs one wave until all n rays of the source are taken and walked; `heads`
 // are the source's kFetchHeads fetch counters (zero before the launch), shared by every
 // wave of the grid.
-template <int MODE, bool COUNT, bool LDS, class Src>
+template <int MODE, bool COUNT, bool LDS, bool WT, class Src>
 __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uint32_t n, uint32_t *heads,
                                           uint32_t wave_id, TravCounters<COUNT> &cnt) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -268,13 +285,13 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         if (!live) break;
         if (__ballot(r.state == ST_NODE)) {
             if (COUNT && lane == 0) cnt.node_w++;
-            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY>(s, r, cnt);
+            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY, WT>(s, r, cnt);
         }
         uint64_t mn = __ballot(r.state == ST_NODE);
         const uint64_t mt = __ballot(r.state == ST_TRI);
         if (mt && __popcll(mt) * tri_ratio >= __popcll(mn)) {
             if (COUNT && lane == 0) cnt.tri_w++;
-            if (r.state == ST_TRI) tri_step<MODE, COUNT>(s, r, cnt);
+            if (r.state == ST_TRI) tri_step<MODE, COUNT, WT>(s, r, cnt);
         }
         mn = __ballot(r.state == ST_NODE);
         const uint64_t mi = __ballot(r.state == ST_INST);

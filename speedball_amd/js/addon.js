@@ -34,6 +34,6 @@ const BUF = Object.freeze({
 // reflectionQuality tiers (js/gi_probes.js:60-65) -> SBGI_REFLECT_*
 const REFLECTION = Object.freeze({ off: 0, rough: 1, high: 2, ultra: 3 });
 const BLEND = Object.freeze({ fp32: 0, bf16: 1, bf16x3: 2, fp16: 3 });
-const TRAVERSAL = Object.freeze({ compat: 0, ordered: 1 });
+const TRAVERSAL = Object.freeze({ compat: 0, ordered: 1, watertight: 2 });
 
 module.exports = { loadAddon, STATUS, BUF, BLEND, TRAVERSAL, REFLECTION };

@@ -166,3 +166,45 @@ def test_batch_refuses_a_field_whose_scene_outgrew_it(specs):
         batch.close()
         for f in fs:
             f.close()
+
+
+def test_batch_frames_match_the_oracle(specs):
+    """Two batch frames (from cleared state, then bouncing off the first) against the CPU
+    oracle, per field and cascade (each cascade solve reads only its own atlas): the FP32
+    field against the FP32 restatement at the FP32 bars (p99 1e-4, max 1e-3), the MFMA
+    fields against the FP64 oracle fed the reference's f32 ray directions at the blend bars
+    (p99.9 1e-3; BF16x3 and FP16), over both chained updates."""
+    import oracle
+    from tests.parity import interior_irr, rel_err, summary
+    params = [dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis) for w, *_ in specs]
+    fields = make_fields(specs)
+    batch = Batch(fields)
+    try:
+        got = []
+        for frame in range(2):
+            print(f"case: batch frame {frame} for the oracle comparison", flush=True)
+            batch.tick(params)
+            batch.wait()
+            got.append([[f.read_irr(c) for c in range(len(f.cascades))] for f in fields])
+    finally:
+        batch.close()
+        for f in fields:
+            f.close()
+    for i, ((w, casc, prec, trav), kw) in enumerate(zip(specs, params)):
+        fp32 = prec == N.BLEND_FP32
+        for c, (gm, gs, res) in enumerate(casc):
+            oracle.set_f32_dirs(not fp32)
+            try:
+                o = oracle.OracleField(w.scene, gm, gs, res, w.rays, w.depth_oct, prec="32" if fp32 else "64")
+                p = o.params(**kw)
+                for frame in range(2):
+                    o.update(p)
+                    e = rel_err(interior_irr(got[frame][i][c])[..., :3], interior_irr(o.irr_view())[..., :3])
+                    print(f"field {i} (precision {prec}) cascade {c} frame {frame}: irradiance vs "
+                          f"{'FP32' if fp32 else 'FP64'} {summary(e)}")
+                    if fp32:
+                        assert np.percentile(e, 99) <= 1e-4 and e.max() <= 1e-3
+                    else:
+                        assert np.percentile(e, 99.9) <= 1e-3
+            finally:
+                oracle.set_f32_dirs(False)

@@ -548,3 +548,95 @@ def test_nee_ranged_point_light_window_and_gate():
     inside = dist < 2.9
     alt = 0.5 / math.pi * (ndl / np.maximum(dist ** 2, 0.01))[:, None] * rec[7:10].astype(np.float64)
     assert np.all(np.abs(alt[inside] - rad[inside]).max(axis=1) > 1e-6)
+
+
+# ------------------------------------------------------------- watertight triangle test
+def _mesh_scene(v, t, side=2):
+    b = SceneBuilder()
+    m = b.add_material(uber_material((0.5, 0.5, 0.5), side=side))
+    b.add_instance(b.add_mesh(v, t, tri_material=np.full(len(t), m, np.uint32)))
+    return b.finalize()
+
+
+def _closest(scene, ro, rd, prec, wt):
+    oracle.set_watertight(wt, prec)
+    try:
+        return oracle.brute_closest(scene, ro, rd, prec)
+    finally:
+        oracle.set_watertight(False, prec)
+
+
+@pytest.mark.parametrize("prec", ["64", "32"])
+def test_watertight_agrees_with_moller_trumbore_away_from_edges(prec):
+    """Rays through a triangle's interior (barycentrics >= 0.05 from every edge) and rays
+    missing it by the same margin: the watertight test and Moller-Trumbore agree on hit or
+    miss, on t (1e-12 in FP64, 5e-5 in FP32; rays within 3 degrees of the plane, where t is
+    ill-conditioned in either form, are left out) and on the sign of the determinant that the
+    side rules read (front faces of a counter-clockwise triangle positive)."""
+    rng = np.random.default_rng(51)
+    n_hit = 0
+    for trial in range(300):
+        tri = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        n = np.cross(tri[1] - tri[0], tri[2] - tri[0])
+        if np.linalg.norm(n) < 0.2:
+            continue
+        scene = _mesh_scene(tri, np.array([[0, 1, 2]], np.uint32), side=2 if trial % 2 else 0)
+        bary = rng.dirichlet((1.0, 1.0, 1.0))
+        inside = bary.min() >= 0.05
+        if not inside and bary.min() > -0.05:
+            continue
+        target = bary @ tri.astype(np.float64)
+        if trial % 3 == 0:  # a point beyond an edge: b0 < -0.05
+            target = (-0.1) * tri[0] + 0.55 * tri[1] + 0.55 * tri[2]
+            inside = False
+        ro = target + rng.normal(size=3) * 2.0
+        rd = (target - ro) / np.linalg.norm(target - ro)
+        if abs(rd @ n) < 0.05 * np.linalg.norm(n):
+            continue  # grazing: t is ill-conditioned in either form
+        a = _closest(scene, ro, rd, prec, False)
+        b = _closest(scene, ro, rd, prec, True)
+        assert (a[1] >= 0) == (b[1] >= 0), trial
+        if a[1] >= 0:
+            np.testing.assert_allclose(b[0], a[0], rtol=1e-12 if prec == "64" else 5e-5)
+            n_hit += 1
+        # single-sided triangles (even trials) are seen only from the front, by both tests
+        if trial % 2 == 0 and inside:
+            assert (a[1] >= 0) == (rd @ n < 0)
+    assert n_hit > 80
+
+
+def _edge_and_vertex_rays(v, t, centre):
+    """Rays from an interior point aimed at every vertex and at the f32 midpoint of every
+    edge of a closed mesh."""
+    pts = [v[i] for i in range(len(v))]
+    edges = set()
+    for a, b, c in t:
+        for x, y in ((a, b), (b, c), (c, a)):
+            edges.add((min(x, y), max(x, y)))
+    for x, y in sorted(edges):
+        pts.append((v[x] + (v[y] - v[x]) * np.float32(0.5)).astype(np.float32))
+    dirs = np.array(pts, np.float64) - centre
+    return dirs / np.linalg.norm(dirs, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("shape", ["sphere", "box"])
+def test_watertight_closed_mesh_never_leaks(shape):
+    """From inside a closed mesh, a ray aimed exactly at a shared vertex or at a shared
+    edge must hit the surface: the watertight test (FP32, as the GPU evaluates it) leaks no
+    ray. Moller-Trumbore's count of leaking rays on the same set is reported beside it."""
+    if shape == "sphere":
+        v, t = sg.uv_sphere(24, 48, radius=1.3)
+        centre = np.array([0.013, -0.021, 0.007])
+    else:
+        v, t = sg.box_mesh((0.0, 0.0, 0.0), (1.1, 0.7, 0.9), 0.37)
+        centre = np.array([0.05, -0.02, 0.03])
+    v, t = np.asarray(v, np.float32), np.asarray(t, np.uint32)
+    scene = _mesh_scene(v, t)
+    dirs = _edge_and_vertex_rays(v, t, centre)
+    leaks_wt = leaks_mt = 0
+    for rd in dirs:
+        leaks_wt += int(_closest(scene, centre, rd, "32", True)[1] < 0)
+        leaks_mt += int(_closest(scene, centre, rd, "32", False)[1] < 0)
+    print(f"\n{shape}: {len(dirs)} vertex and edge rays, leaks: watertight {leaks_wt}, Moller-Trumbore {leaks_mt}")
+    assert len(dirs) > 20
+    assert leaks_wt == 0

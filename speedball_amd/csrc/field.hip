@@ -148,6 +148,8 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_inst_pad);
     dfree(f->d_tlas_scratch);
     dfree(f->d_inst_stage);
+    dfree(f->d_rebuild);
+    f->rebuild_bytes = 0;
     free_skins(f);
     dfree(f->d_albedo);
     dfree(f->d_light_scale);
@@ -1357,22 +1359,37 @@ int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n
     uint32_t range_bits = 0;
     while ((1u << range_bits) < n_ranges) range_bits++;
     const size_t sort_bytes = rebuild_sort_bytes(n);
-    // scratch (a rebuild is a structural edit: allocated per call)
-    void *scratch = nullptr;
+    // depth bound of the top-down build and each depth's grid (nodes at depth d: at most
+    // n_ranges 2^d, and at most the leaf count)
+    uint32_t max_leaves = 0;
+    for (uint32_t r = 0; r < n_ranges; r++) max_leaves = std::max(max_leaves, (desc[r].y + 1) / 2);
+    const uint32_t depths = rebuild_depth_bound(max_leaves), cap = (uint32_t)nodes_total;
+    const uint64_t leaves_total = (nodes_total + n_ranges) / 2;
+    std::vector<uint32_t> blocks(depths);
+    for (uint32_t d = 0; d < depths; d++) {
+        const uint64_t at_d = d < 32 ? std::min<uint64_t>((uint64_t)n_ranges << d, leaves_total) : leaves_total;
+        blocks[d] = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((at_d + 255) / 256, 1024));
+    }
+    // scratch, kept on the field and grown on demand (stream order separates rebuilds)
     const size_t sz_desc = (size_t)n_ranges * 16, sz_pref = ((size_t)n_ranges + 1) * 4, sz_cen = (size_t)n * 16,
                  sz_cb = (size_t)n_ranges * 24, sz_keys = (size_t)n * 8, sz_vals = (size_t)n * 4,
-                 sz_old = (size_t)n * 16, sz_tasks = (size_t)n * 16, sz_order = nodes_total * 4,
-                 sz_q = ordered(f) ? (size_t)copies * v.node_count * 4 : 0;
+                 sz_old = (size_t)n * 16, sz_tasks = nodes_total * 16, sz_lv = ((size_t)depths * 2 + 3) * 4,
+                 sz_order = nodes_total * 4, sz_q = ordered(f) ? (size_t)copies * v.node_count * 4 : 0;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t bytes = al(sz_desc) + al(sz_pref) + al(sz_cen) + al(sz_cb) + 2 * al(sz_keys) + 2 * al(sz_vals) +
-                         al(sz_old) + 2 * al(sz_tasks) + al(4) + al(sz_order) + al(sz_q) + al(sort_bytes) +
+                         al(sz_old) + al(sz_tasks) + al(sz_lv) + al(sz_order) + al(sz_q) + al(sort_bytes) +
                          al((size_t)n_ranges * 8);
-    if (hipMalloc(&scratch, bytes) != hipSuccess) return fail(f, SBGI_E_NOMEM, "rebuild scratch");
-    struct Guard {
-        void *p;
-        ~Guard() { (void)hipFree(p); }
-    } guard{nullptr};
-    char *p = static_cast<char *>(scratch);
+    if (bytes > f->rebuild_bytes) {
+        HIP_OK(f, hipStreamSynchronize(st));
+        dfree(f->d_rebuild);
+        f->rebuild_bytes = 0;
+        if (hipMalloc(&f->d_rebuild, bytes) != hipSuccess) {
+            f->d_rebuild = nullptr;
+            return fail(f, SBGI_E_NOMEM, "rebuild scratch");
+        }
+        f->rebuild_bytes = bytes;
+    }
+    char *p = static_cast<char *>(f->d_rebuild);
     auto take = [&](size_t sz) {
         char *q = p;
         p += al(sz);
@@ -1387,19 +1404,14 @@ int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n
     uint32_t *d_vals = reinterpret_cast<uint32_t *>(take(sz_vals));
     uint32_t *d_vals_s = reinterpret_cast<uint32_t *>(take(sz_vals));
     uint32_t *d_old = reinterpret_cast<uint32_t *>(take(sz_old));  // 3 index words + material per triangle
-    uint4 *d_ta = reinterpret_cast<uint4 *>(take(sz_tasks));
-    uint4 *d_tb = reinterpret_cast<uint4 *>(take(sz_tasks));
-    uint32_t *d_nnext = reinterpret_cast<uint32_t *>(take(4));
+    uint4 *d_tasks = reinterpret_cast<uint4 *>(take(sz_tasks));
+    uint32_t *d_lv = reinterpret_cast<uint32_t *>(take(sz_lv));
     uint32_t *d_order = reinterpret_cast<uint32_t *>(take(sz_order));
     uint32_t *d_q = reinterpret_cast<uint32_t *>(take(sz_q));
     void *d_sort = take(sort_bytes);
     uint2 *d_tr = reinterpret_cast<uint2 *>(take((size_t)n_ranges * 8));
     int32_t rc = SBGI_OK;
-    if ((rc = record_kev(f, 0)) != SBGI_OK || (rc = record_kev(f, 1)) != SBGI_OK) {
-        (void)hipFree(scratch);
-        return rc;
-    }
-    guard.p = scratch;
+    if ((rc = record_kev(f, 0)) != SBGI_OK || (rc = record_kev(f, 1)) != SBGI_OK) return rc;
     std::vector<uint32_t> cb_init((size_t)n_ranges * 6);
     for (uint32_t r = 0; r < n_ranges; r++)
         for (int q = 0; q < 3; q++) {
@@ -1428,45 +1440,27 @@ int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n
                                      d_vals, d_keys_s, d_vals_s, d_sort, sort_bytes, 30u + range_bits, st));
     HIP_OK(f, launch_permute_tris(d_desc, d_pref, n_ranges, n, d_vals_s, d_old_idx, d_old_mat, f->d_tri_index,
                                   f->d_tri_mat, st));
-    // top-down structure, one launch per depth
+    // top-down structure, one launch per depth over the device depth table: the roots are
+    // depth 0's tasks, every later count is appended on the device
     std::vector<uint4> roots(n_ranges);
     for (uint32_t r = 0; r < n_ranges; r++) roots[r] = make_uint4(desc[r].x, r, 0u, (desc[r].y + 1) / 2);
-    HIP_OK(f, hipMemcpyAsync(d_ta, roots.data(), (size_t)n_ranges * 16, hipMemcpyHostToDevice, st));
-    std::vector<uint32_t> level_off{0u};
-    uint32_t ntask = n_ranges;
-    uint4 *cur = d_ta, *nxt = d_tb;
-    while (ntask) {
-        if (level_off.back() + (uint64_t)ntask > nodes_total)
-            return fail(f, SBGI_E_LAYOUT, "rebuild produced more nodes than the range holds");
-        HIP_OK(f, hipMemsetAsync(d_nnext, 0, 4, st));
-        HIP_OK(f, launch_build_level(d_desc, d_pref, d_keys_s, cur, ntask, nxt, d_nnext, f->d_nodes,
-                                     d_order + level_off.back(), st));
-        uint32_t nn = 0;
-        HIP_OK(f, hipMemcpyAsync(&nn, d_nnext, 4, hipMemcpyDeviceToHost, st));
-        HIP_OK(f, hipStreamSynchronize(st));
-        level_off.push_back(level_off.back() + ntask);
-        ntask = nn;
-        std::swap(cur, nxt);
-    }
-    if (level_off.back() != nodes_total) return fail(f, SBGI_E_LAYOUT, "rebuild node count mismatch");
+    std::vector<uint32_t> lv_init((size_t)depths * 2 + 3, 0u);
+    lv_init[1] = n_ranges;
+    HIP_OK(f, hipMemcpyAsync(d_tasks, roots.data(), (size_t)n_ranges * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(f, hipMemcpyAsync(d_lv, lv_init.data(), sz_lv, hipMemcpyHostToDevice, st));
+    HIP_OK(f, launch_build_depths(d_desc, d_pref, d_keys_s, d_tasks, cap, d_lv, depths, blocks.data(), f->d_nodes,
+                                  d_order, st));
     // bounds bottom-up (deepest depth first)
-    for (size_t d = level_off.size() - 1; d-- > 0;) {
-        const uint32_t lo[2] = {0u, level_off[d + 1] - level_off[d]};
-        HIP_OK(f, launch_refit_levels(f->d_nodes, f->d_tri_index, f->d_vdata, d_order + level_off[d], lo, 1, st));
-    }
+    HIP_OK(f, launch_refit_depths(f->d_nodes, f->d_tri_index, f->d_vdata, d_order, d_lv, depths, cap, blocks.data(),
+                                  st));
     // walk copies: positions top down (ordered), then every node of every copy
     if (ordered(f)) {
-        std::vector<uint32_t> one(1);
-        for (uint32_t r = 0; r < n_ranges; r++)
-            for (uint32_t o = 0; o < copies; o++)
-                HIP_OK(f, hipMemcpyAsync(d_q + (size_t)o * v.node_count + desc[r].x, &desc[r].x, 4,
-                                         hipMemcpyHostToDevice, st));
-        for (size_t d = 0; d + 1 < level_off.size(); d++)
-            HIP_OK(f, launch_copy_positions(f->d_nodes, d_order + level_off[d], level_off[d + 1] - level_off[d], copies,
-                                            d_q, v.node_count, st));
+        HIP_OK(f, launch_copy_roots(d_desc, n_ranges, copies, d_q, v.node_count, st));
+        HIP_OK(f, launch_copy_positions(f->d_nodes, d_order, d_lv, depths, cap, blocks.data(), copies, d_q,
+                                        v.node_count, st));
     }
-    HIP_OK(f, launch_write_copies(f->d_nodes, d_order, (uint32_t)nodes_total, copies, d_q, v.node_count, v.node_count,
-                                  f->d_walk_nodes, f->d_walk_perm, st));
+    HIP_OK(f, launch_write_copies(f->d_nodes, d_order, cap, d_lv + (size_t)depths * 2, copies, d_q, v.node_count,
+                                  v.node_count, f->d_walk_nodes, f->d_walk_perm, st));
     HIP_OK(f, launch_pack_tris_ranges(f->d_tri_index, f->d_vdata, f->d_tri_mat, f->d_mats, v.tri_count, d_tr, n_ranges,
                                       tmax, f->d_tri, st));
     if ((rc = record_kev(f, 2)) != SBGI_OK) return rc;
@@ -1483,7 +1477,12 @@ int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n
         HIP_OK(f, hipMemcpyAsync(f->h_tri_mat.data() + desc[r].z, f->d_tri_mat + desc[r].z, (size_t)desc[r].w * 4,
                                  hipMemcpyDeviceToHost, st));
     }
+    // the depth table's end: {nodes built, tasks past the bound, overflow flag}
+    uint32_t built[3] = {0u, 0u, 0u};
+    HIP_OK(f, hipMemcpyAsync(built, d_lv + (size_t)depths * 2, 12, hipMemcpyDeviceToHost, st));
     HIP_OK(f, hipStreamSynchronize(st));
+    if (built[0] != cap || built[1] || built[2])
+        return fail(f, SBGI_E_LAYOUT, "rebuild node count mismatch");
     // cached refit plans and the differentiable mode's triangle map follow the structure
     f->call_plan.key.clear();
     if (f->skin_ranges_n) {

@@ -68,6 +68,10 @@ struct sbgi_field {
     float *d_inst_pad = nullptr;
     void *d_tlas_scratch = nullptr;
     void *d_inst_stage = nullptr;
+    // BLAS rebuild scratch (sort keys, depth queue, walk-copy positions), kept between
+    // rebuilds and grown when a larger rebuild arrives
+    void *d_rebuild = nullptr;
+    size_t rebuild_bytes = 0;
     // skinned vertex sets (sbgi_set_skins): pool vertex per skinned vertex, rest pose
     // {pos, 0}, {normal, 0}, joints (4 x u16), weights, the bone palette, the leaves of the
     // skinned BLASes and their node / triangle ranges

@@ -14,11 +14,14 @@
 //      first and last leaf differ (the middle when they are equal), and its children go
 //      to p + 1 and p + 2 (m - a) (a subtree over k leaves has 2k - 1 nodes), so every
 //      task of a depth is independent; the tasks, kept by depth, order the bottom-up
-//      bounds pass (k_refit_level, deepest first);
+//      bounds pass (k_refit_depth, deepest first). Task counts stay on the device (a
+//      depth table the launches read and extend), and the host launches a bound on the
+//      depth (rebuild_depth_bound) instead of reading each depth's count back;
 //   5. the walk copies: pool order (one copy), or the eight near-first octant copies,
 //      placed top down from the subtree sizes and the child-box centres.
 // No reference counterpart: the reference rebuilds on the CPU with three-mesh-bvh
 // (js/spectral_scene.js:898-966) and keeps a refit otherwise (js/gi_refit.js).
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -147,90 +150,122 @@ __device__ __forceinline__ void leaf_run(uint32_t T, uint32_t L, uint32_t j, uin
     count = base + (j < rem ? 1u : 0u);
 }
 
+// Depth table on the device: lv[2 d] is the first task of depth d in the task queue,
+// lv[2 d + 1] its task count; lv[2 D + 2] is set when a child would fall past the queue.
+// The launch for depth d writes lv[2 d + 2] and appends the tasks of depth d + 1 right
+// after its own, so the queue index of a task is also its slot in `order`, and no count
+// returns to the host between depths. Counts are clamped to the queue capacity.
+__device__ __forceinline__ void depth_span(const uint32_t *lv, uint32_t d, uint32_t cap, uint32_t &off, uint32_t &n) {
+    off = min(lv[2 * d], cap);
+    n = min(lv[2 * d + 1], cap - off);
+}
+
 // One depth of the top-down build. Task = {node position, range, first leaf, end leaf}.
 // codes: the sorted keys; a leaf's code is its first triangle's (30 low bits).
 __global__ void k_build_level(const uint4 *__restrict__ ranges, const uint32_t *__restrict__ tri_prefix,
-                              const unsigned long long *__restrict__ keys, const uint4 *__restrict__ tasks,
-                              uint32_t n_tasks, uint4 *__restrict__ next, uint32_t *__restrict__ n_next,
-                              uint32_t *__restrict__ nodes, uint32_t *__restrict__ order) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_tasks) return;
-    const uint4 tk = tasks[i];
-    const uint32_t pos = tk.x, r = tk.y, a = tk.z, b = tk.w;
-    const uint4 rg = ranges[r];
-    const uint32_t T = rg.w, L = (rg.y + 1) / 2;
-    uint32_t *nd = nodes + (size_t)pos * 8;
-    order[i] = pos;
-    if (b - a == 1) {
-        uint32_t st, cnt;
-        leaf_run(T, L, a, st, cnt);
-        nd[6] = pos + 1;
-        nd[7] = (cnt << 24) | ((rg.z + st) & 0x00FFFFFFu);
-        return;
-    }
-    auto code = [&](uint32_t leaf) {
-        uint32_t st, cnt;
-        leaf_run(T, L, leaf, st, cnt);
-        return (uint32_t)(keys[tri_prefix[r] + st] & 0x3FFFFFFFull);
-    };
-    const uint32_t ca = code(a), cb = code(b - 1);
-    uint32_t m = a + (b - a) / 2;
-    if (ca != cb) {
-        // first leaf in (a, b) whose code has the highest differing bit set
-        const uint32_t bit = 31u - __clz(ca ^ cb);
-        uint32_t lo = a, hi = b - 1;  // code(lo) has the bit clear, code(hi) set
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if ((code(mid) >> bit) & 1u) hi = mid;
-            else lo = mid;
+                              const unsigned long long *__restrict__ keys, uint4 *__restrict__ tasks, uint32_t cap,
+                              uint32_t *__restrict__ lv, uint32_t d, uint32_t depths, uint32_t *__restrict__ nodes,
+                              uint32_t *__restrict__ order) {
+    uint32_t off, n;
+    depth_span(lv, d, cap, off, n);
+    const uint32_t noff = off + n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) lv[2 * d + 2] = noff;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint4 tk = tasks[off + i];
+        const uint32_t pos = tk.x, r = tk.y, a = tk.z, b = tk.w;
+        const uint4 rg = ranges[r];
+        const uint32_t T = rg.w, L = (rg.y + 1) / 2;
+        uint32_t *nd = nodes + (size_t)pos * 8;
+        order[off + i] = pos;
+        if (b - a == 1) {
+            uint32_t st, cnt;
+            leaf_run(T, L, a, st, cnt);
+            nd[6] = pos + 1;
+            nd[7] = (cnt << 24) | ((rg.z + st) & 0x00FFFFFFu);
+            continue;
         }
-        m = hi;
+        auto code = [&](uint32_t leaf) {
+            uint32_t st, cnt;
+            leaf_run(T, L, leaf, st, cnt);
+            return (uint32_t)(keys[tri_prefix[r] + st] & 0x3FFFFFFFull);
+        };
+        const uint32_t ca = code(a), cb = code(b - 1);
+        uint32_t m = a + (b - a) / 2;
+        if (ca != cb) {
+            // first leaf in (a, b) whose code has the highest differing bit set
+            const uint32_t bit = 31u - __clz(ca ^ cb);
+            uint32_t lo = a, hi = b - 1;  // code(lo) has the bit clear, code(hi) set
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if ((code(mid) >> bit) & 1u) hi = mid;
+                else lo = mid;
+            }
+            m = hi;
+        }
+        nd[6] = pos + 2 * (b - a) - 1;
+        nd[7] = 0xFFFFFFFFu;
+        const uint32_t slot = noff + atomicAdd(&lv[2 * d + 3], 2u);
+        if (d + 1 >= depths || slot + 2 > cap) {
+            atomicOr(&lv[2 * depths + 2], 1u);
+            continue;
+        }
+        tasks[slot] = make_uint4(pos + 1, r, a, m);
+        tasks[slot + 1] = make_uint4(pos + 2 * (m - a), r, m, b);
     }
-    nd[6] = pos + 2 * (b - a) - 1;
-    nd[7] = 0xFFFFFFFFu;
-    const uint32_t slot = atomicAdd(n_next, 2u);
-    next[slot] = make_uint4(pos + 1, r, a, m);
-    next[slot + 1] = make_uint4(pos + 2 * (m - a), r, m, b);
 }
 
-// Walk copies, top down by depth (order[off[d], off[d+1]) lists the pool nodes of depth
-// d): qpos[o][i - base] is node i's position in copy o. Near-first children as in the
+// Walk copies, top down by depth (the depth table's span of order lists the pool nodes of
+// depth d): qpos[o][i] is pool node i's position in copy o. Near-first children as in the
 // host layout: the child whose box centre projects lower on the octant direction.
-__global__ void k_copy_positions(const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ order, uint32_t n,
-                                 uint32_t copies, uint32_t *__restrict__ qpos, uint32_t qstride) {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
-    const uint32_t i = order[x];
-    const uint32_t *nd = nodes + (size_t)i * 8;
-    if (nd[7] != 0xFFFFFFFFu) return;
-    const float *nf = reinterpret_cast<const float *>(nodes);
-    const uint32_t l = i + 1, r = nodes[(size_t)l * 8 + 6];
-    const uint32_t size_l = r - l;
-    for (uint32_t o = 0; o < copies; o++) {
-        const uint32_t q = qpos[(size_t)o * qstride + i];
-        bool left_near = true;
-        if (copies == 8) {
-            const float d[3] = {(o & 1) ? -1.f : 1.f, (o & 2) ? -1.f : 1.f, (o & 4) ? -1.f : 1.f};
-            float cl = 0.f, cr = 0.f;
-            for (int a = 0; a < 3; a++) {
-                cl += (nf[(size_t)l * 8 + a] + nf[(size_t)l * 8 + 3 + a]) * d[a];
-                cr += (nf[(size_t)r * 8 + a] + nf[(size_t)r * 8 + 3 + a]) * d[a];
+__global__ void k_copy_positions(const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ order,
+                                 const uint32_t *__restrict__ lv, uint32_t d, uint32_t cap, uint32_t copies,
+                                 uint32_t *__restrict__ qpos, uint32_t qstride) {
+    uint32_t off, n;
+    depth_span(lv, d, cap, off, n);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) {
+        const uint32_t i = order[off + x];
+        const uint32_t *nd = nodes + (size_t)i * 8;
+        if (nd[7] != 0xFFFFFFFFu) continue;
+        const float *nf = reinterpret_cast<const float *>(nodes);
+        const uint32_t l = i + 1, r = nodes[(size_t)l * 8 + 6];
+        const uint32_t size_l = r - l;
+        for (uint32_t o = 0; o < copies; o++) {
+            const uint32_t q = qpos[(size_t)o * qstride + i];
+            bool left_near = true;
+            if (copies == 8) {
+                const float dir[3] = {(o & 1) ? -1.f : 1.f, (o & 2) ? -1.f : 1.f, (o & 4) ? -1.f : 1.f};
+                float cl = 0.f, cr = 0.f;
+                for (int a = 0; a < 3; a++) {
+                    cl += (nf[(size_t)l * 8 + a] + nf[(size_t)l * 8 + 3 + a]) * dir[a];
+                    cr += (nf[(size_t)r * 8 + a] + nf[(size_t)r * 8 + 3 + a]) * dir[a];
+                }
+                left_near = cl <= cr;
             }
-            left_near = cl <= cr;
+            const uint32_t size_r = nd[6] - r;
+            qpos[(size_t)o * qstride + l] = left_near ? q + 1 : q + 1 + size_r;
+            qpos[(size_t)o * qstride + r] = left_near ? q + 1 + size_l : q + 1;
         }
-        const uint32_t size_r = nd[6] - r;
-        qpos[(size_t)o * qstride + l] = left_near ? q + 1 : q + 1 + size_r;
-        qpos[(size_t)o * qstride + r] = left_near ? q + 1 + size_l : q + 1;
     }
+}
+
+// Each range's root keeps its pool position in every copy.
+__global__ void k_copy_roots(const uint4 *__restrict__ ranges, uint32_t n_ranges, uint32_t copies,
+                             uint32_t *__restrict__ qpos, uint32_t qstride) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_ranges) return;
+    const uint32_t root = ranges[r].x;
+    for (uint32_t o = 0; o < copies; o++) qpos[(size_t)o * qstride + root] = root;
 }
 
 // Write copy nodes {min.xyz, miss}, {max.xyz, payload} and perm for every pool node of the
-// ranges (qpos indexed by pool node; miss in copy o = position + subtree size).
+// ranges (qpos indexed by pool node; miss in copy o = position + subtree size). n_built,
+// when given, is the node count the build reached (the entries of order written).
 __global__ void k_write_copies(const uint32_t *__restrict__ nodes, const uint32_t *__restrict__ order, uint32_t n,
-                               uint32_t copies, const uint32_t *__restrict__ qpos, uint32_t qstride,
-                               uint32_t node_count, uint32_t *__restrict__ walk, uint32_t *__restrict__ perm) {
+                               const uint32_t *__restrict__ n_built, uint32_t copies,
+                               const uint32_t *__restrict__ qpos, uint32_t qstride, uint32_t node_count,
+                               uint32_t *__restrict__ walk, uint32_t *__restrict__ perm) {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
+    if (x >= (n_built ? min(n, *n_built) : n)) return;
     const uint32_t i = order[x];
     const uint32_t *src = nodes + (size_t)i * 8;
     const uint32_t size = src[6] - i;
@@ -283,28 +318,51 @@ hipError_t launch_permute_tris(const uint4 *ranges, const uint32_t *tri_prefix, 
     return hipGetLastError();
 }
 
-hipError_t launch_build_level(const uint4 *ranges, const uint32_t *tri_prefix, const unsigned long long *keys,
-                              const uint4 *tasks, uint32_t n_tasks, uint4 *next, uint32_t *n_next, uint32_t *nodes,
-                              uint32_t *order, hipStream_t st) {
-    if (!n_tasks) return hipSuccess;
-    hipLaunchKernelGGL(k_build_level, dim3((n_tasks + 255) / 256), dim3(256), 0, st, ranges, tri_prefix, keys, tasks,
-                       n_tasks, next, n_next, nodes, order);
-    return hipGetLastError();
+uint32_t rebuild_depth_bound(uint32_t max_leaves) {
+    // a radix split lowers the highest differing Morton bit (30 bits), the middle splits
+    // of equal codes halve the leaf count, and a chain over L leaves is L deep
+    uint32_t k = 0;
+    while (k < 32 && (1ull << k) < max_leaves) k++;
+    return std::max(1u, std::min(31u + k, max_leaves));
 }
 
-hipError_t launch_copy_positions(const uint32_t *nodes, const uint32_t *order, uint32_t n, uint32_t copies,
-                                 uint32_t *qpos, uint32_t qstride, hipStream_t st) {
-    if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_copy_positions, dim3((n + 255) / 256), dim3(256), 0, st, nodes, order, n, copies, qpos,
+hipError_t launch_build_depths(const uint4 *ranges, const uint32_t *tri_prefix, const unsigned long long *keys,
+                               uint4 *tasks, uint32_t cap, uint32_t *lv, uint32_t depths, const uint32_t *blocks,
+                               uint32_t *nodes, uint32_t *order, hipStream_t st) {
+    for (uint32_t d = 0; d < depths; d++) {
+        hipLaunchKernelGGL(k_build_level, dim3(blocks[d]), dim3(256), 0, st, ranges, tri_prefix, keys, tasks, cap, lv,
+                           d, depths, nodes, order);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_copy_roots(const uint4 *ranges, uint32_t n_ranges, uint32_t copies, uint32_t *qpos, uint32_t qstride,
+                             hipStream_t st) {
+    if (!n_ranges) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_roots, dim3((n_ranges + 255) / 256), dim3(256), 0, st, ranges, n_ranges, copies, qpos,
                        qstride);
     return hipGetLastError();
 }
 
-hipError_t launch_write_copies(const uint32_t *nodes, const uint32_t *order, uint32_t n, uint32_t copies,
-                               const uint32_t *qpos, uint32_t qstride, uint32_t node_count, uint32_t *walk,
-                               uint32_t *perm, hipStream_t st) {
+hipError_t launch_copy_positions(const uint32_t *nodes, const uint32_t *order, const uint32_t *lv, uint32_t depths,
+                                 uint32_t cap, const uint32_t *blocks, uint32_t copies, uint32_t *qpos,
+                                 uint32_t qstride, hipStream_t st) {
+    for (uint32_t d = 0; d < depths; d++) {
+        hipLaunchKernelGGL(k_copy_positions, dim3(blocks[d]), dim3(256), 0, st, nodes, order, lv, d, cap, copies, qpos,
+                           qstride);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_write_copies(const uint32_t *nodes, const uint32_t *order, uint32_t n, const uint32_t *n_built,
+                               uint32_t copies, const uint32_t *qpos, uint32_t qstride, uint32_t node_count,
+                               uint32_t *walk, uint32_t *perm, hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_write_copies, dim3((n + 255) / 256), dim3(256), 0, st, nodes, order, n, copies, qpos, qstride,
-                       node_count, walk, perm);
+    hipLaunchKernelGGL(k_write_copies, dim3((n + 255) / 256), dim3(256), 0, st, nodes, order, n, n_built, copies, qpos,
+                       qstride, node_count, walk, perm);
     return hipGetLastError();
 }

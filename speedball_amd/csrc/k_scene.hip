@@ -157,11 +157,8 @@ __global__ void k_oct_gather(uint32_t *__restrict__ oct, const uint32_t *__restr
 // kernel boundary makes its bounds visible: no atomics and no fences. A leaf takes the
 // bounds of its triangles (an empty leaf keeps its bounds, gi_refit.js:77-85); an interior
 // node the union of its children (left = i + 1, right = the left child's miss).
-__global__ void k_refit_level(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
-                              const float *__restrict__ vdata, const uint32_t *__restrict__ order, uint32_t n) {
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
-    const uint32_t i = order[x];
+__device__ __forceinline__ void refit_node(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
+                                           const float *__restrict__ vdata, uint32_t i) {
     float *nf = reinterpret_cast<float *>(nodes);
     const uint32_t info = nodes[(size_t)i * 8 + 7];
     float mn[3], mx[3];
@@ -198,6 +195,23 @@ __global__ void k_refit_level(uint32_t *__restrict__ nodes, const uint32_t *__re
     }
 }
 
+__global__ void k_refit_level(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
+                              const float *__restrict__ vdata, const uint32_t *__restrict__ order, uint32_t n) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    refit_node(nodes, tri_index, vdata, order[x]);
+}
+
+// One depth of a rebuilt tree: lv[2 d], lv[2 d + 1] are the depth's first slot in order
+// and its node count (on the device, clamped to cap).
+__global__ void k_refit_depth(uint32_t *__restrict__ nodes, const uint32_t *__restrict__ tri_index,
+                              const float *__restrict__ vdata, const uint32_t *__restrict__ order,
+                              const uint32_t *__restrict__ lv, uint32_t d, uint32_t cap) {
+    const uint32_t off = min(lv[2 * d], cap), n = min(lv[2 * d + 1], cap - off);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x)
+        refit_node(nodes, tri_index, vdata, order[off + x]);
+}
+
 }  // namespace
 
 hipError_t launch_refit_levels(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const uint32_t *order,
@@ -207,6 +221,18 @@ hipError_t launch_refit_levels(uint32_t *nodes, const uint32_t *tri_index, const
         if (!n) continue;
         hipLaunchKernelGGL(k_refit_level, dim3((n + 255) / 256), dim3(256), 0, st, nodes, tri_index, vdata,
                            order + level_off[h], n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_refit_depths(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const uint32_t *order,
+                               const uint32_t *lv, uint32_t depths, uint32_t cap, const uint32_t *blocks,
+                               hipStream_t st) {
+    for (uint32_t d = depths; d-- > 0;) {
+        hipLaunchKernelGGL(k_refit_depth, dim3(blocks[d]), dim3(256), 0, st, nodes, tri_index, vdata, order, lv, d,
+                           cap);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

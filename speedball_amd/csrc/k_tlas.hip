@@ -112,13 +112,34 @@ __device__ __forceinline__ void wave_sync() {
 // bins of one wave: per axis and bin {count, min.xyz, max.xyz} (min/max as f2o words)
 constexpr int kBinWords = 7;
 
-__global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restrict__ bnd, uint32_t n,
-                                                           uint32_t *__restrict__ ids0, uint32_t *__restrict__ ids1,
+// Up to kTlasLdsSlots slots, the boxes and both id arrays live in LDS for the whole build
+// (32 bytes a slot in dynamic LDS): every level reads each box three times (bounds,
+// binning, partition) and the ids twice, so the chain of dependent loads per task runs at
+// LDS latency instead of L2 latency. Larger TLASes read them from global memory.
+constexpr uint32_t kTlasLdsSlots = 2048;
+
+template <bool LDS>
+__global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restrict__ bnd_g, uint32_t n,
+                                                           uint32_t *__restrict__ ids0_g, uint32_t *__restrict__ ids1_g,
                                                            uint4 *__restrict__ task0, uint4 *__restrict__ task1,
                                                            uint4 *__restrict__ tlas) {
     __shared__ uint32_t s_count[2];
     __shared__ uint32_t s_bins[kTlasWaves][3 * kTlasBins * kBinWords];
+    extern __shared__ __align__(16) uint32_t s_slots[];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const float *bnd;
+    uint32_t *ids0, *ids1;
+    if constexpr (LDS) {
+        float *sb = reinterpret_cast<float *>(s_slots);
+        for (uint32_t i = tid; i < n * 6; i += kTlasBlock) sb[i] = bnd_g[i];
+        bnd = sb;
+        ids0 = s_slots + (size_t)n * 6;
+        ids1 = ids0 + n;
+    } else {
+        bnd = bnd_g;
+        ids0 = ids0_g;
+        ids1 = ids1_g;
+    }
     for (uint32_t i = tid; i < n; i += kTlasBlock) ids0[i] = i;
     if (tid == 0) {
         task0[0] = make_uint4(0u, 0u, n, 0u);
@@ -332,6 +353,11 @@ hipError_t launch_build_tlas(const float *padded, uint32_t n, void *scratch, uin
     // tasks first: scratch is 256-byte aligned and the uint4 lists need 16
     uint4 *task0 = static_cast<uint4 *>(scratch), *task1 = task0 + n;
     uint32_t *ids0 = reinterpret_cast<uint32_t *>(task1 + n), *ids1 = ids0 + n;
-    hipLaunchKernelGGL(k_build_tlas, dim3(1), dim3(kTlasBlock), 0, st, padded, n, ids0, ids1, task0, task1, tlas);
+    if (n <= kTlasLdsSlots)
+        hipLaunchKernelGGL(k_build_tlas<true>, dim3(1), dim3(kTlasBlock), (size_t)n * 32, st, padded, n, ids0, ids1,
+                           task0, task1, tlas);
+    else
+        hipLaunchKernelGGL(k_build_tlas<false>, dim3(1), dim3(kTlasBlock), 0, st, padded, n, ids0, ids1, task0, task1,
+                           tlas);
     return hipGetLastError();
 }

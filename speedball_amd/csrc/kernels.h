@@ -188,6 +188,10 @@ hipError_t launch_oct_gather_ranges(uint32_t *oct_nodes, const uint32_t *nodes, 
 // order[level_off[h], level_off[h + 1]) (level_off on the host, levels + 1 entries)
 hipError_t launch_refit_levels(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const uint32_t *order,
                                const uint32_t *level_off, uint32_t levels, hipStream_t st);
+// the same over the rebuild's device depth table (deepest depth first)
+hipError_t launch_refit_depths(uint32_t *nodes, const uint32_t *tri_index, const float *vdata, const uint32_t *order,
+                               const uint32_t *lv, uint32_t depths, uint32_t cap, const uint32_t *blocks,
+                               hipStream_t st);
 
 // k_tlas.hip — instance world bounds (and the inflated copy the SAH build reads) and the
 // one-workgroup SAH TLAS build: tlas receives 2n - 1 walk-format nodes
@@ -207,14 +211,20 @@ hipError_t launch_rebuild_prepare(const uint4 *ranges, const uint32_t *tri_prefi
 hipError_t launch_permute_tris(const uint4 *ranges, const uint32_t *tri_prefix, uint32_t n_ranges, uint32_t total,
                                const uint32_t *sorted_vals, const uint32_t *old_index, const uint32_t *old_mat,
                                uint32_t *tri_index, uint32_t *tri_mat, hipStream_t st);
-hipError_t launch_build_level(const uint4 *ranges, const uint32_t *tri_prefix, const unsigned long long *keys,
-                              const uint4 *tasks, uint32_t n_tasks, uint4 *next, uint32_t *n_next, uint32_t *nodes,
-                              uint32_t *order, hipStream_t st);
-hipError_t launch_copy_positions(const uint32_t *nodes, const uint32_t *order, uint32_t n, uint32_t copies,
-                                 uint32_t *qpos, uint32_t qstride, hipStream_t st);
-hipError_t launch_write_copies(const uint32_t *nodes, const uint32_t *order, uint32_t n, uint32_t copies,
-                               const uint32_t *qpos, uint32_t qstride, uint32_t node_count, uint32_t *walk,
-                               uint32_t *perm, hipStream_t st);
+// top-down build by depth over a device depth table lv (see k_build.hip): tasks holds cap
+// entries with the roots first; blocks[d] is the grid of depth d's launch
+uint32_t rebuild_depth_bound(uint32_t max_leaves);
+hipError_t launch_build_depths(const uint4 *ranges, const uint32_t *tri_prefix, const unsigned long long *keys,
+                               uint4 *tasks, uint32_t cap, uint32_t *lv, uint32_t depths, const uint32_t *blocks,
+                               uint32_t *nodes, uint32_t *order, hipStream_t st);
+hipError_t launch_copy_roots(const uint4 *ranges, uint32_t n_ranges, uint32_t copies, uint32_t *qpos, uint32_t qstride,
+                             hipStream_t st);
+hipError_t launch_copy_positions(const uint32_t *nodes, const uint32_t *order, const uint32_t *lv, uint32_t depths,
+                                 uint32_t cap, const uint32_t *blocks, uint32_t copies, uint32_t *qpos,
+                                 uint32_t qstride, hipStream_t st);
+hipError_t launch_write_copies(const uint32_t *nodes, const uint32_t *order, uint32_t n, const uint32_t *n_built,
+                               uint32_t copies, const uint32_t *qpos, uint32_t qstride, uint32_t node_count,
+                               uint32_t *walk, uint32_t *perm, hipStream_t st);
 
 // k_dynamic.hip — device skinning, staged instance-record scatter, fill
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, uint32_t n, hipStream_t st);

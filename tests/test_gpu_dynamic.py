@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import time
 from types import SimpleNamespace
 
 import numpy as np
@@ -272,12 +273,13 @@ def test_set_skins_rejects_bad_sets(small_dynamic):
 
 
 @pytest.mark.parametrize("n,kind", [(1, "random"), (2, "random"), (3, "random"), (37, "coincident"),
-                                    (64, "inactive"), (3000, "random")])
+                                    (64, "inactive"), (2048, "random"), (3000, "random")])
 def test_device_sah_tlas_edge_cases(n, kind):
     """The one-workgroup SAH build against the oracle's restatement, word for word, for a
     single instance, tiny sets, every instance at one point (no axis has centroid extent:
     middle splits), a quarter of the slots inactive (point boxes at their translation) and
-    a few thousand instances (many levels of many nodes)."""
+    a few thousand instances (many levels of many nodes); 2048 slots is the largest build
+    whose boxes and ids sit in LDS, 3000 reads them from global memory."""
     rng = np.random.default_rng(100 + n)
     b = sg.SceneBuilder()
     v, t = sg.box_mesh((0.0, 0.0, 0.0), (0.5, 0.3, 0.2), 0.0)
@@ -369,5 +371,18 @@ def test_rebuild_blas_matches_oracle(instanced_small, traversal):
         assert np.array_equal(f.read(N.BUF_BVH_NODES, np.uint32), nodes)
         sk, rb, tl = f.edit_timing()
         print(f"rebuild timing: BLAS refit {rb:.4f} ms, TLAS {tl:.4f} ms")
+        # the scratch is kept between rebuilds: a smaller rebuild, then the full one again,
+        # reproduce the tree (the triangles are already in Morton order and the sort is
+        # stable, so a rebuild of unchanged vertices is idempotent)
+        f.rebuild_blas(ranges[:1])
+        assert np.array_equal(f.read(N.BUF_BVH_NODES, np.uint32), nodes)
+        assert np.array_equal(f.read(N.BUF_TRI_INDEX, np.uint32), ti)
+        t0 = time.perf_counter()
+        f.rebuild_blas(ranges)
+        wall = (time.perf_counter() - t0) * 1e3
+        assert np.array_equal(f.read(N.BUF_BVH_NODES, np.uint32), nodes)
+        assert np.array_equal(f.read(N.BUF_TRI_INDEX, np.uint32), ti)
+        assert np.array_equal(f.read(N.BUF_TRI_MATERIAL, np.uint32), tm)
+        print(f"rebuild of {len(ranges)} BLASes with kept scratch: {wall:.3f} ms host wall")
     finally:
         f.close()

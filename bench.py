@@ -70,15 +70,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_instanced_pmc.md")
 def measured_traffic(kernel_prefix: str):
     """HBM bytes per dispatch (calibrated FETCH_SIZE + WRITE_SIZE) of a kernel, from the
     committed rocprofv3 --pmc summary of this workload; None when it is not there."""
-    try:
-        with open(TRAFFIC_FILE) as fh:
-            for line in fh:
-                parts = line.split()
-                if len(parts) == 4 and parts[0] == "traffic" and parts[1].startswith(kernel_prefix):
-                    return float(parts[2]) + float(parts[3])
-    except OSError:
-        return None
-    return None
+    v = _summary_lines(TRAFFIC_FILE, "traffic", kernel_prefix)
+    return float(v[0][0]) + float(v[0][1]) if v else None
 
 
 def cpu_baseline(w, seconds_target=12.0):
@@ -113,19 +106,176 @@ def cpu_baseline(w, seconds_target=12.0):
                       f"probes x {w.rays} rays of the same field from cleared state, {total:.2f} s"}
 
 
-def fp64_check(w, field, precision):
-    """Relative irradiance error of the first GPU update vs the FP64 oracle on a probe subset."""
+def sample_ranges(total: int, n_ranges: int = 8, per: int = 48):
+    """Stratified probe sample: n_ranges runs of `per` consecutive probes spread evenly over
+    the field (each run covers one stretch of x rows at its own y and z)."""
+    return [(int(i * total / n_ranges), per) for i in range(n_ranges)]
+
+
+def fp64_check(w, states, depth_oct, ranges):
+    """Irradiance of both GPU updates against the FP64 oracle on a stratified probe sample.
+
+    Update 1 starts from cleared state; update 2 is recomputed by the oracle from the GPU's
+    own update-1 atlas and temporal state (states[0]), so each update is checked on its
+    own. Each update is compared twice: with the oracle's exact ray directions, and with the
+    reference's f32 directions (phi = k GA rounded in f32, js/gi_probes.js:1702) widened to
+    double. The second isolates the arithmetic of the GPU path; the difference between the
+    two is the f32 direction definition's share. Texels off by more than 1e-3 are counted,
+    and those that the f32 directions bring within 1e-3 are counted as explained."""
     import oracle
     from tests.parity import rel_err, interior_irr  # shared metric definition
-    n = 24
-    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec="64")
-    p = o.params(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
-    p.updated_count = n
-    o.update(p)
-    g = interior_irr(field)[:n, :, :3]
-    r = interior_irr(o.irr_view())[:n, :, :3]
-    e = rel_err(g, r)
-    return float(np.percentile(e, 99)), float(e.max())
+    out = []
+    for u in range(2):
+        errs = {}
+        for mode in ("exact", "f32"):
+            oracle.set_f32_dirs(mode == "f32")
+            try:
+                o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, depth_oct, prec="64")
+                if u == 1:
+                    o.atlas[:] = states[0]["atlas"].reshape(-1)
+                    o.irr[:] = states[0]["irr"].reshape(-1)
+                    o.depth[:] = states[0]["depth"].reshape(-1)
+                # every range's bounce reads the atlas the update started from (the GPU
+                # traces all probes before any upload), so each range starts from that atlas
+                atlas0 = o.atlas.copy()
+                e = []
+                for off, n in ranges:
+                    o.atlas[:] = atlas0
+                    p = o.params(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+                    p.probe_offset, p.updated_count = off, n
+                    o.update(p)
+                    g = interior_irr(states[u]["irr"][off:off + n])[..., :3]
+                    r = interior_irr(o.irr_view()[off:off + n])[..., :3]
+                    e.append(rel_err(g, r).max(axis=-1).reshape(-1))
+                errs[mode] = np.concatenate(e)
+            finally:
+                oracle.set_f32_dirs(False)
+        ex, f3 = errs["exact"], errs["f32"]
+        over = ex > 1e-3
+        out.append({"update": u + 1, "p99": float(np.percentile(ex, 99)), "max": float(ex.max()),
+                    "p99_f32_dirs": float(np.percentile(f3, 99)), "max_f32_dirs": float(f3.max()),
+                    "texels": int(ex.size), "texels_over_1e-3": int(over.sum()),
+                    "explained_by_f32_dirs": int((over & (f3 <= 1e-3)).sum())})
+    return {"probes_checked": int(sum(n for _, n in ranges)), "sample": f"{len(ranges)} runs of "
+            f"{ranges[0][1]} consecutive probes spread over the field", "updates": out}
+
+
+def read_state(f):
+    return {"irr": f.read_irr(), "depth": f.read_depth(), "atlas": f.read_atlas()}
+
+
+def room_leg(steps=20, warmup=3):
+    """The room workload: room + 64 boxes (780 triangles), a point
+    and a directional light, 16^3 probes x 64 rays, plain FP32 blend, in the configuration
+    depth mode (16x16 tile) and the reference-parity mode (depth in the 8x8 tile). Per mode:
+    ms per update (wall clock over `steps` queued updates), the stage split, and the FP64
+    check of both updates on the whole field."""
+    from speedball_amd import scenegen
+    from speedball_amd.field import ProbeField
+    import torch
+    out = {"workload": "room: 10 m room + 64 boxes (780 triangles), 1 point + 1 directional light, 16^3 probes x "
+                       "64 rays, FP32 blend"}
+    for depth_oct in (14, 6):
+        w = scenegen.room_boxes(depth_oct=depth_oct)
+        f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=depth_oct, precision=0, traversal=1)
+        f.set_scene(w.scene)
+        kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+        states = []
+        for _ in range(2):
+            f.tick(**kw)
+            states.append(read_state(f))
+        for _ in range(warmup):
+            f.tick(**kw)
+        f.wait()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f.tick(wait=False, **kw)
+        f.wait()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        s = f.stats()
+        f.close()
+        chk = fp64_check(w, states, depth_oct, [(0, w.probes)])
+        out["depth_16x16" if depth_oct == 14 else "depth_8x8_reference"] = {
+            "ms_per_update": ms, "probe_rays_per_s": w.probes * w.rays / (ms / 1000.0),
+            "stage_ms": {"trace": s.ms_trace, "blend": s.ms_blend, "upload": s.ms_upload, "total": s.ms_total},
+            "irradiance_rel_err_vs_fp64": chk}
+    return out
+
+
+# ---------------------------------------------------------------- cache and issue ceilings
+VALU_CLASSES = ("add_f32", "mul_f32", "fma_f32", "trans_f32", "int32", "cvt", "int64")
+CEILINGS_FILE = os.path.join(ROOT, "profiles", "roofline_ceilings.md")
+
+
+def _summary_lines(path: str, tag: str, kernel_prefix: str = None):
+    """Fields of the machine-readable lines `tag [kernel] values...` of a committed summary."""
+    out = []
+    try:
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if not parts or parts[0] != tag:
+                    continue
+                if kernel_prefix is None:
+                    out.append(parts[1:])
+                elif len(parts) > 2 and parts[1].startswith(kernel_prefix):
+                    out.append(parts[2:])
+    except OSError:
+        return []
+    return out
+
+
+def ceilings():
+    """Measured ceilings of this GPU (speedball_amd/calib/ceilings, summarised in
+    profiles/roofline_ceilings.md): VALU class rates and peak (wave-instr/s), the L2 read
+    ceiling (bytes/s of 128-byte requests) and the L1 ceiling (tag accesses/s)."""
+    out = {}
+    for f in _summary_lines(CEILINGS_FILE, "rate"):
+        out["rate_" + f[0]] = float(f[1]) * 1e9
+    for tag in ("l2_ceiling", "l1_ceiling"):
+        v = _summary_lines(CEILINGS_FILE, tag)
+        if v:
+            out[tag] = float(v[0][0]) * 1e9
+    return out
+
+
+def valu_issue(kernel_prefix: str, launch_ms: float, ceil: dict):
+    """VALU issue of a kernel from the committed counter summary: wave-instructions per
+    launch (SQ_INSTS_VALU) over the launch time, against the fastest measured issue rate
+    (a lower bound on the share of the issue capacity used, whatever the mix), and the
+    bracket of the additive class model (each counted class at its own measured rate, the
+    unclassified ones at the fast or the slow rate)."""
+    mix = _summary_lines(TRAFFIC_FILE, "valu_mix", kernel_prefix)
+    if not mix or "rate_peak" not in ceil:
+        return None
+    v = [float(x) for x in mix[0]]
+    total, by = v[0], dict(zip(VALU_CLASSES + ("other",), v[1:]))
+    s = launch_ms / 1000.0
+    base = sum(by[c] / ceil["rate_" + c] for c in VALU_CLASSES)
+    lo = (base + by["other"] / ceil["rate_other_fast"]) / s
+    hi = (base + by["other"] / ceil["rate_other_slow"]) / s
+    return {"achieved": total / s / 1e9, "peak": ceil["rate_peak"] / 1e9, "unit": "G wave-instr/s",
+            "frac": total / s / ceil["rate_peak"], "mix_model_frac": [lo, hi],
+            "instructions_per_launch": total, "mix": by}
+
+
+def l2_requests(kernel_prefix: str, launch_ms: float, ceil: dict):
+    """L2->L1 request bytes per launch (TCP_TCC_READ_REQ x 128 B) over the L2 read ceiling."""
+    v = _summary_lines(TRAFFIC_FILE, "l2_req", kernel_prefix)
+    if not v or "l2_ceiling" not in ceil:
+        return None
+    b = float(v[0][0])
+    gbs = b / (launch_ms / 1000.0) / 1e9
+    return {"bytes_per_launch": b, "gbs": gbs, "frac": gbs * 1e9 / ceil["l2_ceiling"]}
+
+
+def mfma_util(kernel_prefix: str):
+    """MFMA busy fraction of a kernel from the committed counter summary:
+    SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE per XCD x 1,024 SIMDs)."""
+    v = _summary_lines(TRAFFIC_FILE, "mfma_busy", kernel_prefix)
+    return float(v[0][0]) if v else None
 
 
 def adjoint_leg(reps=5):
@@ -488,6 +638,7 @@ def main():
     ap.add_argument("--batch-frames", type=int, default=10, help="timed frames of the batched-fields leg")
     ap.add_argument("--no-lights", action="store_true", help="skip the clustered-lights and emitters leg")
     ap.add_argument("--no-reflect", action="store_true", help="skip the reflection-lobes leg")
+    ap.add_argument("--no-room", action="store_true", help="skip the room leg (16^3 probes x 64 rays)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
@@ -518,10 +669,13 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # first update from cleared state: kept for the FP64 subset check
-    f.tick(**kw)
-    first_irr = f.read_irr()
-    for _ in range(max(0, args.warmup - 1)):
+    # the first two updates (from cleared state, then bouncing off the first): kept for the
+    # FP64 check
+    states = []
+    for _ in range(2):
+        f.tick(**kw)
+        states.append(read_state(f))
+    for _ in range(max(0, args.warmup - 2)):
         f.tick(**kw)
 
     import torch  # device-wide synchronisation around the timed region
@@ -563,25 +717,53 @@ def main():
     hits = int(np.count_nonzero(f.read_rays()[..., 3] >= 0))
     bytes_alg = algorithmic_bytes(cs, hits)
     trace_ms = float(np.mean(stage["trace"]))
-    # roofline of the dominant kernel, the shadow any-hit walk, timed by its own events
+    # roofline of the dominant kernel, the shadow any-hit walk, timed by its own events; the
+    # primary walk beside it
+    ceil = ceilings()
+    ordered = args.traversal == "ordered"
     sw_bytes = shadow_walk_bytes(cs)
     sw_ms = float(np.mean(stage["walk_shadow"]))
-    achieved = sw_bytes / (sw_ms / 1000.0) / 1e9
-    traffic = measured_traffic("k_walk<1,false,true,ShadowSrc>") if args.traversal == "ordered" else None
+    pw_bytes = walk_bytes(cs.blas_nodes, cs.tlas_nodes, cs.inst_visits, cs.tri_tests)
+    pw_ms = float(np.mean(stage["walk_primary"]))
+
+    def walk_roofline(kernel, ref_bytes, ms):
+        """The walk against the measured L2 read ceiling: the bytes it pulls from L2 (its
+        TCP->TCC requests from the committed counter summary) over its live launch time;
+        VALU issue, HBM and the reference-layout bytes beside it."""
+        l2 = l2_requests(kernel, ms, ceil) if ordered else None
+        l2_peak = ceil.get("l2_ceiling", 0.0) / 1e9 or None
+        traffic = measured_traffic(kernel) if ordered else None
+        ref_gbs = ref_bytes / (ms / 1000.0) / 1e9
+        return {"bound": "l2", "kernel": kernel, "achieved": l2["gbs"] if l2 else None, "peak": l2_peak,
+                "unit": "GB/s", "frac": l2["frac"] if l2 else None, "traffic": traffic, "avg_launch_ms": ms,
+                "l2_request_bytes_per_launch": l2["bytes_per_launch"] if l2 else None,
+                "valu_issue": valu_issue(kernel, ms, ceil) if ordered else None,
+                "hbm": {"peak": HBM_PEAK_GBS, "traffic_gbs": traffic / (ms / 1000.0) / 1e9 if traffic else None,
+                        "frac_traffic": traffic / (ms / 1000.0) / 1e9 / HBM_PEAK_GBS if traffic else None},
+                # 32 B a BLAS node, 48 B a TLAS node, 64 B an instance entry, 36 B a triangle test:
+                # the reference's layout; this walk reads the TLAS and instances from LDS and
+                # re-reads nodes from L1, so these bytes are a work count, not traffic
+                "reference_layout_bytes": {"per_launch": ref_bytes, "gbs": ref_gbs},
+                "sources": ["profiles/roofline_ceilings.md", "profiles/walk_instanced_pmc.md"]}
+
     blend_ms = float(np.mean(stage["blend"]))
     T_irr, T_d = 36, w.depth_oct * w.depth_oct
     blend_flops = 2.0 * w.probes * w.rays * (5 * T_irr + 2 * T_d)
-    blend_bytes = w.probes * w.rays * (16 if prec == 0 else 8) + w.probes * (T_irr * 32 + T_d * 16)
+    # the FP32 ray records k_resolve writes (16 B a ray), and the previous and new interior texels
+    blend_bytes = w.probes * w.rays * 16 + w.probes * (T_irr * 32 + T_d * 16)
+    blend_kernel = {0: "k_blend_fp32", 1: "k_blend_mfma<1u,13,false>", 2: "k_blend_mfma<2u,13,false>",
+                    3: "k_blend_mfma<3u,13,false>"}[prec]
+    blend_traffic = measured_traffic(blend_kernel)
 
-    err99 = errmax = None
-    if not args.no_check:
-        err99, errmax = fp64_check(w, first_irr, prec)
+    fp64 = None if args.no_check else fp64_check(w, states, w.depth_oct, sample_ranges(w.probes))
+    blend_mfma = mfma_util(blend_kernel) if prec else None
     sampling = sample_leg(f, w)
     f.close()
     adj = None if args.no_adjoint else adjoint_leg()
     dyn = None if args.no_dynamic else dynamic_leg(frames=args.dynamic_frames)
     bat = None if args.no_batch else batch_leg(frames=args.batch_frames)
     lights = None if args.no_lights else lights_leg()
+    room = None if args.no_room else room_leg()
     reflect = None if args.no_reflect else reflect_leg()
     cpu = None if args.no_cpu else cpu_baseline(w)
 
@@ -598,21 +780,26 @@ def main():
         "stage_ms": {k: float(np.mean(v)) for k, v in stage.items()},
         "shadow_rays_per_update": int(cs.shadow_rays),
         "total_rays_per_s": (rays + cs.shadow_rays) / (ms_step / 1000.0),
-        "irradiance_rel_err_vs_fp64": {"p99": err99, "max": errmax, "probes_checked": 24},
-        "roofline": {"bound": "hbm", "kernel": "k_walk<ANY> (shadow any-hit walk)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": sw_bytes, "avg_launch_ms": sw_ms,
-                     "traffic_source": "profiles/walk_instanced_pmc.md" if traffic else None,
-                     "counts": {"blas_nodes": int(cs.shadow_blas_nodes), "tlas_nodes": int(cs.shadow_tlas_nodes),
-                                "tri_tests": int(cs.shadow_tri_tests), "inst_visits": int(cs.shadow_inst_visits),
-                                "rays": int(cs.shadow_rays)}},
+        "irradiance_rel_err_vs_fp64": fp64,
+        # roofline of record: the shadow walk's L2 read bytes against the measured L2 read
+        # ceiling (its nodes come from L2 and L1; HBM carries ~1 % of its reads); VALU issue
+        # and HBM beside it
+        "roofline": dict(walk_roofline("k_walk<1,false,true,ShadowSrc,false>", sw_bytes, sw_ms),
+                         counts={"blas_nodes": int(cs.shadow_blas_nodes), "tlas_nodes": int(cs.shadow_tlas_nodes),
+                                 "tri_tests": int(cs.shadow_tri_tests), "inst_visits": int(cs.shadow_inst_visits),
+                                 "rays": int(cs.shadow_rays)}),
+        "primary_walk_roofline": walk_roofline("k_walk<0,false,true,PrimarySrc,false>", pw_bytes, pw_ms),
         "trace": {"algorithmic_bytes": bytes_alg, "ms": trace_ms, "gbs": bytes_alg / (trace_ms / 1000.0) / 1e9,
                   "counts": {"blas_nodes": int(cs.blas_nodes), "tlas_nodes": int(cs.tlas_nodes),
                              "tri_tests": int(cs.tri_tests), "inst_visits": int(cs.inst_visits),
                              "primary_hits": hits},
                   "shadow_walk_phases": shadow_phases(cs)},
-        "blend": {"kernel": "k_blend", "ms": blend_ms, "tflops": blend_flops / (blend_ms / 1000.0) / 1e12,
-                  "gbs": blend_bytes / (blend_ms / 1000.0) / 1e9, "flops": blend_flops, "bytes": blend_bytes},
+        "blend": {"kernel": blend_kernel, "ms": blend_ms, "tflops": blend_flops / (blend_ms / 1000.0) / 1e12,
+                  "gbs": blend_bytes / (blend_ms / 1000.0) / 1e9, "flops": blend_flops, "bytes": blend_bytes,
+                  "hbm_traffic": blend_traffic,
+                  "hbm_traffic_frac": blend_traffic / (blend_ms / 1000.0) / 1e9 / HBM_PEAK_GBS if blend_traffic else None,
+                  "mfma_busy": blend_mfma},
+        "room": room,
         "fwd_plus_adjoint_ms": adj["fwd_plus_adjoint_ms"] if adj else None,
         "adjoint": adj,
         "dynamic": dyn,

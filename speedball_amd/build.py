@@ -74,7 +74,24 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     if verbose:
         print("built", LIB)
     build_addon(verbose)
+    build_calib(verbose)
     return LIB
+
+
+CALIB_DIR = os.path.join(PKG, "calib")
+
+
+def build_calib(verbose: bool = False):
+    """The roofline-ceiling measurement tool (calib/ceilings: VALU issue rates by
+    instruction class, L2 and L1 read bandwidth), run by the profiling commands that
+    calibrate the walk kernels' rooflines; not part of the library."""
+    src = os.path.join(CALIB_DIR, "ceilings.hip")
+    exe = os.path.join(CALIB_DIR, "ceilings")
+    _run([HIPCC, *HIP_FLAGS, src, "-o", exe + ".tmp"])
+    shutil.move(exe + ".tmp", exe)
+    if verbose:
+        print("built", exe)
+    return exe
 
 
 NAPI_DIR = os.path.join(PKG, "napi")

@@ -1477,18 +1477,46 @@ int32_t sbgi_rebuild_blas(sbgi_field *f, const uint32_t *blas_ranges, uint32_t n
         HIP_OK(f, hipMemcpyAsync(f->h_tri_mat.data() + desc[r].z, f->d_tri_mat + desc[r].z, (size_t)desc[r].w * 4,
                                  hipMemcpyDeviceToHost, st));
     }
-    // the depth table's end: {nodes built, tasks past the bound, overflow flag}
-    uint32_t built[3] = {0u, 0u, 0u};
-    HIP_OK(f, hipMemcpyAsync(built, d_lv + (size_t)depths * 2, 12, hipMemcpyDeviceToHost, st));
+    // the depth table (its end: {nodes built, tasks past the bound, overflow flag})
+    std::vector<uint32_t> lvh((size_t)depths * 2 + 3);
+    HIP_OK(f, hipMemcpyAsync(lvh.data(), d_lv, lvh.size() * 4, hipMemcpyDeviceToHost, st));
     HIP_OK(f, hipStreamSynchronize(st));
-    if (built[0] != cap || built[1] || built[2])
+    if (lvh[(size_t)depths * 2] != cap || lvh[(size_t)depths * 2 + 1] || lvh[(size_t)depths * 2 + 2])
         return fail(f, SBGI_E_LAYOUT, "rebuild node count mismatch");
+    // the rebuilt ranges' refit plan straight from the depth table, deepest depth first (a
+    // node's children are one depth deeper, so each level finds its children finished)
+    auto depth_plan = [&](RefitPlan &pl, const uint32_t *key, size_t key_words) -> int32_t {
+        if (pl.cap < nodes_total) {
+            dfree(pl.d_order);
+            pl.cap = 0;
+            if (dalloc(&pl.d_order, nodes_total * 4) != hipSuccess) return fail(f, SBGI_E_NOMEM, "refit plan allocation");
+            pl.cap = nodes_total;
+        }
+        pl.off.assign(1, 0u);
+        for (uint32_t d = depths; d-- > 0;) {
+            const uint32_t cnt = lvh[(size_t)d * 2 + 1];
+            if (cnt)
+                HIP_OK(f, hipMemcpyAsync(pl.d_order + pl.off.back(), d_order + lvh[(size_t)d * 2], (size_t)cnt * 4,
+                                         hipMemcpyDeviceToDevice, st));
+            pl.off.push_back(pl.off.back() + cnt);
+        }
+        pl.h_order.clear();
+        pl.key.assign(key, key + key_words);
+        return SBGI_OK;
+    };
     // cached refit plans and the differentiable mode's triangle map follow the structure
-    f->call_plan.key.clear();
+    if ((rc = depth_plan(f->call_plan, blas_ranges, (size_t)n_ranges * 2)) != SBGI_OK) return rc;
     if (f->skin_ranges_n) {
         std::vector<uint32_t> kr(f->skin_plan.key);
+        std::vector<std::pair<uint32_t, uint32_t>> skin_set;
+        for (size_t i = 0; i + 1 < kr.size(); i += 2) skin_set.emplace_back(kr[i], kr[i + 1]);
+        std::sort(skin_set.begin(), skin_set.end());
         f->skin_plan.key.clear();
-        if ((rc = build_plan(f, kr.data(), (uint32_t)kr.size() / 2, f->skin_plan)) != SBGI_OK) return rc;
+        if (skin_set == nr)
+            rc = depth_plan(f->skin_plan, kr.data(), kr.size());
+        else
+            rc = build_plan(f, kr.data(), (uint32_t)kr.size() / 2, f->skin_plan);
+        if (rc != SBGI_OK) return rc;
     }
     f->diff_ready = f->diff_valid = false;
     return SBGI_OK;

@@ -6,12 +6,35 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
 #include "kernels.h"
 
 namespace sbgi {
+
+// Page-locked host storage for the mirrors that device edits copy back (BVH nodes,
+// triangle rows): copies into it run at DMA rate instead of through a staging buffer.
+template <class T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <class U>
+    PinnedAlloc(const PinnedAlloc<U> &) {}
+    T *allocate(size_t n) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t) { (void)hipHostFree(p); }
+};
+template <class T, class U>
+bool operator==(const PinnedAlloc<T> &, const PinnedAlloc<U> &) { return true; }
+template <class T, class U>
+bool operator!=(const PinnedAlloc<T> &, const PinnedAlloc<U> &) { return false; }
+template <class T>
+using pinned_vector = std::vector<T, PinnedAlloc<T>>;
 
 // message of the last failed sbgi_create on this thread
 inline thread_local std::string g_create_error;
@@ -97,7 +120,7 @@ struct sbgi_field {
     // level-synchronous refit plans: the skinned BLASes, and the ranges of the last sbgi_refit
     RefitPlan skin_plan, call_plan;
     // scene (host mirror, reference layouts)
-    std::vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
+    sbgi::pinned_vector<uint32_t> h_nodes, h_tri_index, h_tri_mat;
     std::vector<float> h_vdata, h_mats, h_lights;
     std::vector<double> h_world;  // per TLAS slot, column-major 4x4
     sbgi_scene_view view{};

@@ -247,6 +247,44 @@ def test_skinning_frames_match_oracle(small_dynamic, traversal):
         f.close()
 
 
+@pytest.mark.parametrize("traversal", MODES, ids=IDS)
+def test_skin_after_rebuild_matches_oracle(small_dynamic, traversal):
+    """Skin, rebuild the skinned BLASes on the device, skin again: the second refit runs on
+    the plan the rebuild leaves behind (its depth table, deepest depth first), and must equal
+    the oracle's refit of the rebuilt tree over the new pose bit for bit; the packed
+    triangles follow the rebuilt triangle order."""
+    w = small_dynamic()
+    s = w.scene
+    f = make_field(s, w.cascades, w.rays, w.depth_oct, traversal)
+    try:
+        f.set_instance_transforms(np.arange(s.inst_count), [np.array(m) for m in w.slot_worlds])
+        f.set_skins(w.skin_sets, w.n_bones)
+        f.skin(w.bones(0.61), update_tlas=False)
+        ranges = [(int(br), int(be)) for _, _, br, be, _, _ in w.skin_sets]
+        print(f"case: skin after rebuild {IDS[traversal]}, {len(ranges)} BLASes", flush=True)
+        f.rebuild_blas(ranges)
+        n1 = f.read(N.BUF_BVH_NODES, np.uint32)
+        t1 = f.read(N.BUF_TRI_INDEX, np.uint32)
+        m1 = f.read(N.BUF_TRI_MATERIAL, np.uint32)
+        bones = w.bones(1.13)
+        f.skin(bones, update_tlas=True)
+        vd = s.vertex_data.copy().reshape(-1, 8)
+        for vs, vc, br, be, j, wt in w.skin_sets:
+            vd[vs:vs + vc, :6] = oracle.skin(s.vertex_data.reshape(-1, 8)[vs:vs + vc, :6], j, wt, bones)
+        s2 = scene_with(s, vertex_data=np.ascontiguousarray(vd.reshape(-1)), tri_index=t1, tri_material=m1)
+        nodes = n1.copy()
+        for root, end in ranges:
+            assert oracle.refit_blas(nodes, s2, root, end)
+        assert np.array_equal(f.read(N.BUF_BVH_NODES, np.uint32), nodes)
+        pk = f.read(N.BUF_TRI_PACKED, np.float32).reshape(-1, 3, 4)
+        tri = t1.reshape(-1, 3)
+        p = vd[:, :3]
+        assert np.array_equal(pk[:, 0, :3], p[tri[:, 0]])
+        assert np.array_equal(pk[:, 1, :3], p[tri[:, 1]] - p[tri[:, 0]])
+    finally:
+        f.close()
+
+
 def test_set_skins_rejects_bad_sets(small_dynamic):
     w = small_dynamic()
     f = make_field(w.scene, w.cascades, w.rays, w.depth_oct, N.TRAV_MT_ORDERED)

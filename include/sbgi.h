@@ -371,6 +371,17 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade);
 /* Device time of each stage of the last waited tick. */
 int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out);
 
+/* Probe classification (classifyKernel, js/gi_probes.js:2850-2899; the tick dispatches it
+ * at :3803-3806 when a cascade needs it and classifyStrength > 0): 32 fixed full-sphere
+ * closest-hit rays per probe of the cascade from its grid position; a probe that sees back
+ * faces in more than a quarter of its hits is buried (state 0, else 1), and one whose
+ * nearest hit is a back face gets the relocation offset along that ray past the surface,
+ * clamped to 0.45 x the smallest cell side. Writes every probe's meta {state, offset.xyz}
+ * (SBGI_BUF_META), which the trace, the blend's emitter rays and the receiver use scaled
+ * by classify_strength. Call it when the reference would set needsClassify: after a scene
+ * change, and when classify_strength becomes positive. Stream-ordered with the ticks. */
+int32_t sbgi_classify(sbgi_field *f, uint32_t cascade);
+
 /* Counting mode (no reference counterpart; the measurement hook for the algorithmic
  * byte accounting): while on, ticks run an instrumented trace that accumulates node
  * visits, triangle tests, instance entries and shadow rays into sbgi_stats. Turning it

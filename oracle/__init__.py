@@ -197,6 +197,14 @@ class OracleField:
             _fn(self.prec, "set_rough")(C.c_int(0))
         return cnt
 
+    def classify(self):
+        """classifyKernel over every probe: returns meta (P x 4: state, offset.xyz), also kept
+        in self.meta so later traces can use the relocation."""
+        out = np.zeros(self.P * 4, np.float32)
+        _fn(self.prec, "classify")(C.byref(self.scene.view()), C.byref(self.c), _p(out))
+        self.meta[:] = out
+        return out.reshape(self.P, 4)
+
     def emitter_vis(self, p):
         """emitterVisKernel over the updated probes (no-op without emitters)."""
         if self.scene.emitter_count:

@@ -1211,6 +1211,73 @@ void P(upload_rough)(const sbgi_cascade_desc *c, const sbgi_tick_params *p, cons
  * facing the probe; evis[probe * 16 + e] = mix(sample, evis, 0.8). The hash
  * fract(sin(seed) * 43758.5453) is the sampling basis and is evaluated in single precision
  * in both oracles, with sin rounded once from double; the rest follows REAL. */
+/* classifyKernel (js/gi_probes.js:2850-2899): for every probe of the cascade, 32 fixed
+ * closest-hit rays from its grid position (classifyRayDir :1707-1712 is rayDir with 32 rays
+ * and no jitter); a hit whose world geometric normal faces along the ray is a back face.
+ * state = 0 when back / max(hits, 1) > BACKFACE_FRACTION (0.25), else 1; when the nearest
+ * hit is a back face, offset = d (step) min(|d step|, relocClamp) / max(|d step|, 1e-6) with
+ * step = back distance + 0.5 min(front distance, cellMin) and relocClamp = 0.45 minCell
+ * (:3153, :3364). meta_out[p] = {state, offset.xyz}. */
+void P(classify)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, float *meta_out) {
+    float max_dist, cell_min;
+    P(cascade_derived)(c, &max_dist, &cell_min);
+    double cx = c->grid_size[0] / (double)(c->res[0] > 1 ? c->res[0] - 1 : 1);
+    double cy = c->grid_size[1] / (double)(c->res[1] > 1 ? c->res[1] - 1 : 1);
+    double cz = c->grid_size[2] / (double)(c->res[2] > 1 ? c->res[2] - 1 : 1);
+    double mc = cx < cy ? cx : cy;
+    mc = mc < cz ? mc : cz;
+    mc = mc > 1e-4 ? mc : 1e-4;
+    const REAL reloc = (REAL)(float)(0.45 * mc);
+    uint32_t total = c->res[0] * c->res[1] * c->res[2];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint32_t p = 0; p < total; p++) {
+        REAL ro[3];
+        P(probe_pos)(c, p, NULL, 0.0f, ro);
+        REAL back = 0, hits = 0, close_back = RCONST(1e30), close_front = RCONST(1e30);
+        REAL back_dir[3] = {0, 0, 0};
+        for (uint32_t k = 0; k < 32; k++) {
+            REAL rd[3], t;
+            int32_t tri, inst;
+            P(ray_dir)(k, 32, 0.0f, rd);
+            P(closest)(s, ro, rd, &t, &tri, &inst, NULL);
+            if (tri < 0) continue;
+            hits += 1;
+            REAL p0[3], p1[3], p2[3], e1[3], e2[3], nl[3], ng[3];
+            P(vert)(s, s->tri_index[(uint32_t)tri * 3 + 0], p0);
+            P(vert)(s, s->tri_index[(uint32_t)tri * 3 + 1], p1);
+            P(vert)(s, s->tri_index[(uint32_t)tri * 3 + 2], p2);
+            for (int a = 0; a < 3; a++) {
+                e1[a] = p1[a] - p0[a];
+                e2[a] = p2[a] - p0[a];
+            }
+            P(cross3)(e1, e2, nl);
+            P(normalize3)(nl);
+            P(inst_normal)(s, (uint32_t)inst, nl, ng);
+            if (P(dot3)(rd, ng) > 0) {
+                back += 1;
+                if (t < close_back) {
+                    close_back = t;
+                    for (int a = 0; a < 3; a++) back_dir[a] = rd[a];
+                }
+            } else if (t < close_front) {
+                close_front = t;
+            }
+        }
+        REAL frac = back / (hits > 1 ? hits : RCONST(1));
+        REAL off[3] = {0, 0, 0};
+        if (back > RCONST(0.5) && close_back < close_front) {
+            REAL near = close_front < (REAL)cell_min ? close_front : (REAL)cell_min;
+            REAL step = close_back + RCONST(0.5) * near;
+            REAL raw[3] = {back_dir[0] * step, back_dir[1] * step, back_dir[2] * step};
+            REAL len = RSQRT(P(dot3)(raw, raw));
+            REAL sc = (len < reloc ? len : reloc) / (len > RCONST(1e-6) ? len : RCONST(1e-6));
+            for (int a = 0; a < 3; a++) off[a] = raw[a] * sc;
+        }
+        meta_out[(size_t)p * 4 + 0] = frac > RCONST(0.25) ? 0.0f : 1.0f;
+        for (int a = 0; a < 3; a++) meta_out[(size_t)p * 4 + 1 + a] = (float)off[a];
+    }
+}
+
 void P(emitter_vis)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, const sbgi_tick_params *p,
                     const float *meta, REAL *evis) {
     float max_dist, cell_min;

@@ -48,7 +48,7 @@ EXPORTS = [
     "sbgi_set_skins", "sbgi_skin", "sbgi_edit_timing", "sbgi_sb_mesh_ranges",
     "sbgi_batch_create", "sbgi_batch_tick", "sbgi_batch_wait", "sbgi_batch_timing", "sbgi_batch_set_walk_block",
     "sbgi_batch_set_walk_waves", "sbgi_batch_get_walk_waves", "sbgi_batch_autotune", "sbgi_batch_last_error", "sbgi_batch_destroy", "sbgi_sb_add_materials",
-    "sbgi_rebuild_blas", "sbgi_sample_params_default", "sbgi_sample_irradiance",
+    "sbgi_rebuild_blas", "sbgi_sample_params_default", "sbgi_sample_irradiance", "sbgi_classify",
 ]
 
 
@@ -191,6 +191,7 @@ def lib() -> C.CDLL:
         "sbgi_autotune": (i32, [vp, C.POINTER(TickParams), C.POINTER(WalkTuning), C.POINTER(C.c_float)]),
         "sbgi_set_skins": (i32, [vp, C.POINTER(SkinSet), u32, u32]),
         "sbgi_rebuild_blas": (i32, [vp, vp, u32]),
+        "sbgi_classify": (i32, [vp, u32]),
         "sbgi_skin": (i32, [vp, vp, u32, i32]),
         "sbgi_edit_timing": (i32, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "sbgi_batch_create": (i32, [C.POINTER(vp), u32, C.POINTER(vp)]),

@@ -150,6 +150,8 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_inst_stage);
     dfree(f->d_rebuild);
     f->rebuild_bytes = 0;
+    dfree(f->d_classify);
+    f->classify_cap = 0;
     free_skins(f);
     dfree(f->d_albedo);
     dfree(f->d_light_scale);
@@ -1592,6 +1594,31 @@ int32_t sbgi_destroy(sbgi_field *f) {
     }
     if (f->stream) (void)hipStreamDestroy(f->stream);
     delete f;
+    return SBGI_OK;
+}
+
+int32_t sbgi_classify(sbgi_field *f, uint32_t cascade) {
+    if (!f) return SBGI_E_INVALID;
+    if (cascade >= f->desc.cascade_count) return fail(f, SBGI_E_INVALID, "cascade out of range");
+    if (!f->has_scene) return fail(f, SBGI_E_STATE, "no scene");
+    HIP_OK(f, hipSetDevice(f->desc.device));
+    const CascadeState &cs = f->casc[cascade];
+    const size_t need = (size_t)kClassifyRays * (1 + (size_t)cs.total);
+    if (need > f->classify_cap) {
+        HIP_OK(f, hipStreamSynchronize(f->stream));
+        dfree(f->d_classify);
+        f->classify_cap = 0;
+        if (dalloc(&f->d_classify, need * sizeof(float4)) != hipSuccess)
+            return fail(f, SBGI_E_NOMEM, "classification scratch");
+        f->classify_cap = need;
+    }
+    // relocClamp = 0.45 x minCell, the smallest cell side (js/gi_probes.js:3153, 3364)
+    const double gx = cs.d.grid_size[0], gy = cs.d.grid_size[1], gz = cs.d.grid_size[2];
+    const double cm = std::max(1e-4, std::min({gx / std::max(1u, cs.d.res[0] - 1), gy / std::max(1u, cs.d.res[1] - 1),
+                                               gz / std::max(1u, cs.d.res[2] - 1)}));
+    const DevCascade c = make_cascade(f, cs, 0, cs.total);
+    HIP_OK(f, launch_classify(make_scene(f), c, (float)(0.45 * cm), f->d_classify, f->d_classify + kClassifyRays,
+                              cs.meta, f->tw.queue.count + trace_heads(2), f->stream));
     return SBGI_OK;
 }
 

@@ -91,6 +91,9 @@ struct sbgi_field {
     float *d_inst_pad = nullptr;
     void *d_tlas_scratch = nullptr;
     void *d_inst_stage = nullptr;
+    // probe classification scratch: 32 directions, then 32 hit records per probe
+    float4 *d_classify = nullptr;
+    size_t classify_cap = 0;
     // BLAS rebuild scratch (sort keys, depth queue, walk-copy positions), kept between
     // rebuilds and grown when a larger rebuild arrives
     void *d_rebuild = nullptr;

@@ -111,6 +111,23 @@ struct EmitterSrc {
     }
 };
 
+// Classification rays (classifyKernel, js/gi_probes.js:2850-2899): ray i = probe * 32 + k of
+// every probe of the cascade, from the probe's grid position (no relocation) along fixed
+// direction k (classifyRayDir, :1707-1712: rayDir with 32 rays and no jitter); result
+// hits[i] = {t, tri, inst, 0}.
+struct ClassifySrc {
+    DevCascade c;
+    const float4 *dirs;
+    float4 *hits;
+    __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
+        const uint32_t probe = i / kClassifyRays, k = i - probe * kClassifyRays;
+        lane_begin(s, r, probe_origin(c, 0.f, nullptr, probe), xyz(dirs[k]), SBGI_T_MAX);
+    }
+    __device__ __forceinline__ void store(uint32_t i, const LaneRay &r) const {
+        hits[i] = make_float4(r.tmax, __int_as_float(r.hit_tri), __int_as_float(r.hit_inst), 0.f);
+    }
+};
+
 // Ray queries: in[i] = {ro.xyz, tmax}, {rd.xyz, mode}; mode 0 closest hit, 1 any hit
 // within tmax, 2 closest hit reported as blocked when before tmax.
 // out[i] = {t, tri, inst, blocked}.
@@ -367,6 +384,54 @@ __global__ void k_raydirs(float4 *__restrict__ dirs, uint32_t rays, float jitter
     dirs[k] = make_float4(d.x, d.y, d.z, 0.f);
 }
 
+// classifyKernel's reduction per probe (js/gi_probes.js:2866-2898): a hit whose world
+// geometric normal faces along the ray (dot(rd, ng) > 0) sees a back face; above
+// BACKFACE_FRACTION (0.25) of the hits the probe is buried (state 0, else 1). A probe whose
+// nearest hit is a back face moves along that ray past the surface by the back distance plus
+// half of min(front distance, cellMin), the move clamped to relocClamp. meta[p] = {state,
+// offset.xyz}. The geometric normal is the one the shading uses (hit_frame), before any
+// face-forwarding.
+__global__ void k_classify(DevScene s, DevCascade c, const float4 *__restrict__ dirs, const float4 *__restrict__ hits,
+                           float reloc_clamp, float *__restrict__ meta) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= c.total) return;
+    float back = 0.f, nhit = 0.f, close_back = 1e30f, close_front = 1e30f;
+    float3 back_dir = f3(0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < kClassifyRays; k++) {
+        const float4 h = hits[(size_t)p * kClassifyRays + k];
+        const int tri = __float_as_int(h.y);
+        if (tri < 0) continue;
+        nhit += 1.f;
+        const float3 rd = xyz(dirs[k]);
+        const float4 *tp = s.tri + (size_t)tri * 3;
+        const float4 *ip = s.inst + (size_t)__float_as_uint(h.z) * 4;
+        const float3 nl = normalize3(cross3(xyz(tp[1]), xyz(tp[2])));
+        const float3 ng = normalize3(xyz(ip[0]) * nl.x + xyz(ip[1]) * nl.y + xyz(ip[2]) * nl.z);
+        if (dot3(rd, ng) > 0.f) {
+            back += 1.f;
+            if (h.x < close_back) {
+                close_back = h.x;
+                back_dir = rd;
+            }
+        } else if (h.x < close_front) {
+            close_front = h.x;
+        }
+    }
+    const float state = back / fmaxf(nhit, 1.f) > kBackfaceFraction ? 0.f : 1.f;
+    float3 off = f3(0.f, 0.f, 0.f);
+    if (back > 0.5f && close_back < close_front) {
+        const float step = close_back + 0.5f * fminf(close_front, c.cell_min);
+        const float3 raw = back_dir * step;
+        const float len = sqrtf(dot3(raw, raw));
+        off = raw * (fminf(len, reloc_clamp) / fmaxf(len, 1e-6f));
+    }
+    float *m = meta + (size_t)p * 4;
+    m[0] = state;
+    m[1] = off.x;
+    m[2] = off.y;
+    m[3] = off.z;
+}
+
 int device_cus() {
     static int cus = 0;
     if (!cus) {
@@ -533,6 +598,19 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
                        prev_atlas, prev_atlas_f32, w.hits, w.vis, rays, hit_ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_classify(const DevScene &s, const DevCascade &c, float reloc_clamp, float4 *dirs, float4 *hits,
+                           float *meta, uint32_t *heads, hipStream_t st) {
+    if (!c.total) return hipSuccess;
+    hipError_t e = launch_raydirs(dirs, kClassifyRays, 0.f, st);
+    if (e != hipSuccess) return e;
+    if ((e = hipMemsetAsync(heads, 0, (size_t)kFetchHeads * kHeadStride * sizeof(uint32_t), st)) != hipSuccess) return e;
+    launch_walk<WALK_CLOSEST>(s, ClassifySrc{c, dirs, hits}, nullptr, c.total * kClassifyRays, heads, nullptr,
+                              trace_lds_bytes(s.tlas_count, s.inst_count), st);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_classify, dim3((c.total + 255) / 256), dim3(256), 0, st, s, c, dirs, hits, reloc_clamp, meta);
     return hipGetLastError();
 }
 

@@ -31,6 +31,10 @@ struct TraceScratch {
 };
 constexpr size_t kTraceLdsBudget = 64 * 1024;  // TLAS + instances staged in LDS up to this size
 size_t trace_lds_bytes(uint32_t tlas_count, uint32_t inst_count);
+// probe classification (classifyKernel): CLASSIFY_RAYS fixed rays per probe and the
+// buried fraction above which a probe is inactive (js/gi_probes.js:104-105)
+constexpr uint32_t kClassifyRays = 32;
+constexpr float kBackfaceFraction = 0.25f;
 hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32_t n, uint32_t *head, hipStream_t st);
 hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t st);
 // How a solve marks a stage boundary with an event: recorded on the stream (fn null), or
@@ -83,6 +87,13 @@ hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tic
 // <- mix(sample, vis, 0.8). heads: the walk's kFetchHeads fetch counters, zero on entry.
 hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float *meta,
                               float *vis, uint32_t *heads, hipStream_t st);
+
+// classifyKernel over every probe of the cascade: kClassifyRays fixed closest-hit rays per
+// probe from its grid position (dirs: kClassifyRays float4, written here; hits: total x
+// kClassifyRays float4 of scratch), then meta[p] = {state, relocation offset.xyz}. heads:
+// kFetchHeads fetch counters (cleared here).
+hipError_t launch_classify(const DevScene &s, const DevCascade &c, float reloc_clamp, float4 *dirs, float4 *hits,
+                           float *meta, uint32_t *heads, hipStream_t st);
 
 // k_sample.hip: the receiver gather at n points (sbgi_sample_irradiance)
 struct SampleCascadeIn {

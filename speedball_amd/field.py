@@ -270,6 +270,11 @@ class ProbeField:
         r = np.ascontiguousarray(np.asarray(blas_ranges, np.uint32).reshape(-1))
         self._chk(self.L.sbgi_refit(self.h, N.ptr(r) if r.size else None, r.size // 2, int(refit_tlas)))
 
+    def classify(self, cascade: int = 0):
+        """Probe classification of one cascade (sbgi_classify): writes every probe's meta
+        {state, relocation offset}, read back with read(BUF_META)."""
+        self._chk(self.L.sbgi_classify(self.h, cascade))
+
     def rebuild_blas(self, blas_ranges):
         """Rebuild the BLASes of (root, end) node ranges on the device (sbgi_rebuild_blas)."""
         r = np.ascontiguousarray(np.asarray(blas_ranges, np.uint32).reshape(-1))

@@ -179,7 +179,7 @@ function createProbeField(opts) {
             field = { h: h, rays: raysPerProbe };
             casc = grids.map((g) => ({
                 gridMin: g.gridMin, gridSize: g.gridSize, res: g.res, probeTotal: g.res[0] * g.res[1] * g.res[2],
-                probeCursor: 0, needsClear: true, glossyPhase: 0,
+                probeCursor: 0, needsClear: true, needsClassify: true, glossyPhase: 0,
             }));
         }
         addon.fieldSetScene(field.h, built.arrays);
@@ -294,6 +294,12 @@ function createProbeField(opts) {
             }
             const C = casc[ci];
             const updated = Math.min(updatedCap(), C.probeTotal);
+            // classifyKernel runs once per rebuild while classifyStrength > 0
+            // (js/gi_probes.js:3803-3806)
+            if (C.needsClassify) {
+                if (U.classifyStrength > 0) addon.fieldClassify(field.h, ci);
+                C.needsClassify = false;
+            }
             const ticket = addon.fieldTick(field.h, tickParams(C, ci, updated));
             C.glossyPhase = ((C.glossyPhase || 0) + 1) % glossyInterval;  // js/gi_probes.js:3831-3835
             C.needsClear = false;
@@ -370,7 +376,12 @@ function createProbeField(opts) {
         getReflectionQuality: function () { return reflectionName; },
         setChebyStrength: function (v) { if (Number.isFinite(v)) R.chebyStrength = v; },
         setNormalDetail: function (v) { if (Number.isFinite(v)) R.normalDetail = v; },
-        setClassifyStrength: function (v) { if (Number.isFinite(v)) U.classifyStrength = clamp(v, 0, 1); },
+        setClassifyStrength: function (v) {
+            if (!Number.isFinite(v)) return;
+            U.classifyStrength = clamp(v, 0, 1);
+            // the state buffer is only written while the strength is positive (js/gi_probes.js:4570-4577)
+            if (U.classifyStrength > 0) for (const C of casc) C.needsClassify = true;
+        },
         setDivisions: function (n) {
             const v = clamp(Math.round(Number(n)) || TARGET_PROBES_LONG_AXIS, 2, MAX_PROBES_PER_AXIS);
             if (v === targetLongAxis) return;

@@ -316,6 +316,18 @@ static napi_value js_field_poll(napi_env env, napi_callback_info info) {
     return b;
 }
 
+/* fieldClassify(field, cascade): the classifyKernel pass of one cascade (sbgi_classify) */
+static napi_value js_field_classify(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return NULL;
+    sbgi_field *f = get_ptr(env, argv[0]);
+    uint32_t casc = 0;
+    napi_get_value_uint32(env, argv[1], &casc);
+    int32_t rc = sbgi_classify(f, casc);
+    if (rc < 0) return throw_status(env, rc, sbgi_last_error(f));
+    return NULL;
+}
+
 static napi_value js_field_read(napi_env env, napi_callback_info info) {
     napi_value argv[3];
     if (!get_args(env, info, 3, argv)) return NULL;
@@ -570,6 +582,7 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("fieldDestroy", js_field_destroy);
     EXPORT("fieldSetScene", js_field_set_scene);
     EXPORT("fieldTick", js_field_tick);
+    EXPORT("fieldClassify", js_field_classify);
     EXPORT("fieldWait", js_field_wait);
     EXPORT("fieldPoll", js_field_poll);
     EXPORT("fieldRead", js_field_read);

@@ -21,6 +21,7 @@ function recorder() {
         fieldSetTransforms: (h, slots, worlds, active) => calls.push(['transforms', slots, worlds, active]),
         fieldUpdateRanges: (h, id, ranges, src) => calls.push(['ranges', id, ranges, src.length]),
         fieldRefit: (h, ranges, tl) => calls.push(['refit', ranges, tl]),
+        fieldClassify: (h, c) => calls.push(['classify', c]),
         fieldRead: () => new ArrayBuffer(16),
     });
     return { fake, calls };
@@ -183,6 +184,31 @@ tests.transform_and_deform_edits_reach_the_library = async () => {
     const rf = calls.filter((c) => c[0] === 'refit');
     assert.strictEqual(rf.length, 1);
     assert.strictEqual(rf[0][1].length, 4);        // two unique BLAS ranges
+};
+
+tests.classification_runs_once_per_rebuild_while_the_strength_is_positive = async () => {
+    // js/gi_probes.js:3803-3806 (run when a cascade needs it and classifyStrength > 0),
+    // :4570-4577 (a positive strength asks every cascade again)
+    const { fake, calls } = recorder();
+    const gi = shim.createProbeField({ scene: plainScene(), addon: fake, cascades: 1 });
+    gi.setEnabled(true);
+    await gi.tick({});
+    await gi.tick({});
+    assert.strictEqual(calls.filter((c) => c[0] === 'classify').length, 0);  // strength 0: never
+    gi.setClassifyStrength(0.5);
+    await gi.tick({});
+    await gi.tick({});
+    const cl = calls.filter((c) => c[0] === 'classify');
+    assert.strictEqual(cl.length, 1);
+    assert.strictEqual(cl[0][1], 0);
+    const order = calls.map((c) => c[0]);
+    assert.ok(order.lastIndexOf('classify') < order.lastIndexOf('tick'));
+    gi.setClassifyStrength(0.75);
+    await gi.tick({});
+    assert.strictEqual(calls.filter((c) => c[0] === 'classify').length, 2);
+    gi.setClassifyStrength(0);
+    await gi.tick({});
+    assert.strictEqual(calls.filter((c) => c[0] === 'classify').length, 2);
 };
 
 tests.layout_errors_request_a_rebuild = async () => {

@@ -5,8 +5,9 @@ place of the GPU-side field calls: the reference's argument normalisation
 (js/gi_probes.js:4548-4844), scene-change routing (js/gi_probes.js:3869-3937), that a
 plain description and a three-like scene graph build the same buffers, the calls one
 tick() makes (grid placement, solve range, clear flag, in-flight guard), transform and
-deform edits, layout-error routing to a rebuild, the install handle's guards, and that the
-addon refuses scene arrays of the wrong element type or inconsistent lengths.
+deform edits, layout-error routing to a rebuild, when the classification pass runs, the
+install handle's guards, and that the addon refuses scene arrays of the wrong element type
+or inconsistent lengths.
 """
 import os
 import shutil
@@ -26,4 +27,4 @@ def test_js_shim_host_logic():
                        timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stderr
-    assert "passed 10" in r.stdout
+    assert "passed 11" in r.stdout

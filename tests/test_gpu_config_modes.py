@@ -91,8 +91,8 @@ def test_config2_trace_vs_fp32(config2, config2_run):
             if mism.any():
                 slots, ks = np.nonzero(mism)
                 ties, grazes = classify_mismatches(oracle, w.scene, w, slots + off, ks, gh[slots, ks], rh[slots, ks],
-                                                   gr[slots, ks, 3], edge=1e-5)
-                assert int((~(ties | grazes)).sum()) == 0
+                                                   gr[slots, ks, 3], edge=0.0)
+                assert int((~ties).sum()) == 0  # only near ties may differ from the FP32 restatement
             same = ~mism
             hit = same & (rh[..., 0] >= 0)
             assert rel_err(gr[..., 3][hit], rr[..., 3][hit], 0).max() <= 1e-5

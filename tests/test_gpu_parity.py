@@ -68,28 +68,33 @@ def room_run(room, request):
 
 
 def test_room_hit_ids_and_t(room_run):
-    """Hit ids: identical except near ties (two nearest hits within 1e-6 relative) and edge
-    grazes, both counted and reported.
+    """Hit ids: identical except near ties (two nearest hits within 1e-6 relative), counted
+    and reported.
 
-    Edge grazes are rays within `edge` barycentric units of an edge of either triangle,
-    where Moller-Trumbore's accept decision turns on the last bits. Against the FP32
-    restatement (the same f32 ray directions) the band is 1e-5. Against the FP64 oracle it
-    is 5e-4: the reference evaluates phi = k * GA in f32 (js/gi_probes.js:1702), and at
-    phi ~ 150 rad one f32 ulp is ~8e-6 rad, so the f32 ray itself differs from the exact
-    one by that angle, i.e. up to ~3e-4 of a box face at the field's hit distances."""
+    Against the FP32 restatement (the reference's f32 ray directions and f32 arithmetic)
+    that is the whole bar: no other difference is allowed. Against the FP64 oracle fed the
+    same f32 ray directions, the walk's f32 arithmetic may also decide a ray that passes
+    within 1e-6 barycentric units of a triangle edge the other way (an "edge graze").
+    Against the FP64 oracle with exact directions the graze band is 5e-4: the reference
+    evaluates phi = k * GA in f32 (js/gi_probes.js:1702), and at phi ~ 150 rad one f32 ulp
+    is ~8e-6 rad, so the f32 ray itself differs from the exact one by that angle, i.e. up to
+    ~3e-4 of a box face at the field's hit distances; that comparison is reported."""
     w, gpu, refs, _ = room_run
-    for prec, edge in (("32", 1e-5), ("64", 5e-4)):
+    for prec, edge in (("32", 0.0), ("64f", 1e-6), ("64", 5e-4)):
         g, r = gpu[0], refs[prec][0]
         mism = np.any(g["hits"] != r["hits"], axis=-1)
         slots, ks = np.nonzero(mism)
         ties, grazes = classify_mismatches(oracle, w.scene, w, slots, ks, g["hits"][slots, ks], r["hits"][slots, ks],
                                            g["rays"][slots, ks, 3], edge=edge)
         print(f"\nvs FP{prec}: rays {mism.size}, id mismatches {int(mism.sum())}: near ties {int(ties.sum())}, "
-              f"edge grazes {int(grazes.sum())}")
+              f"edge grazes (band {edge:g}) {int(grazes.sum())}")
         for s_, k_ in zip(slots[:6], ks[:6]):
             print(f"  slot {s_} k {k_}: gpu {g['hits'][s_, k_]} t {g['rays'][s_, k_, 3]:.7f} | "
                   f"ref {r['hits'][s_, k_]} t {r['rays'][s_, k_, 3]:.7f}")
-        assert int((~(ties | grazes)).sum()) == 0, "hit ids differ on rays that are neither near ties nor edge grazes"
+        if prec == "32":
+            assert int((~ties).sum()) == 0, "hit ids differ from the FP32 restatement on rays that are not near ties"
+        else:
+            assert int((~(ties | grazes)).sum()) == 0, "hit ids differ on rays that are neither near ties nor edge grazes"
         assert mism.sum() <= max(8, mism.size // 20000)
     # hit t: the 1e-5 bar is against the FP32 path (same f32 ray directions). Against the
     # FP64 oracle a grazing hit's t moves with the ~8e-6 rad direction difference, so that
@@ -208,10 +213,9 @@ def test_instanced_subset(instanced_small, traversal):
     mism = np.any(g_hits != r_hits, axis=-1)
     slots, ks = np.nonzero(mism)
     ties, grazes = classify_mismatches(oracle, w.scene, w, slots, ks, g_hits[slots, ks], r_hits[slots, ks],
-                                       g_rays[slots, ks, 3], edge=1e-5)
-    print(f"\ninstanced: rays {mism.size} id mismatches {int(mism.sum())} near ties {int(ties.sum())} "
-          f"edge grazes {int(grazes.sum())}")
-    assert int((~(ties | grazes)).sum()) == 0
+                                       g_rays[slots, ks, 3], edge=0.0)
+    print(f"\ninstanced: rays {mism.size} id mismatches {int(mism.sum())} near ties {int(ties.sum())}")
+    assert int((~ties).sum()) == 0  # only near ties may differ from the FP32 restatement
     same = ~mism
     et = rel_err(g_rays[..., 3][same & (r_hits[..., 0] >= 0)], r_rays[..., 3][same & (r_hits[..., 0] >= 0)], 0)
     assert et.max() <= 1e-5

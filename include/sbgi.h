@@ -118,6 +118,10 @@ extern "C" {
 #define SBGI_BUF_GLOSSY_WEIGHT 26 /* f32[P*G*G] glossy support                           */
 #define SBGI_BUF_GLOSSY_ATLAS 27 /* u16 RGBA16F [(tx*G) x (ty*G)], tx = ceil(sqrt(P)),
                                    ty = ceil(P / tx), probe p at (p % tx, p / tx) (read) */
+#define SBGI_BUF_TLAS_OCT 28    /* u32[8*8*(2I-1)] the SAH TLAS's near-first copies, one per
+                                   ray direction octant (bit 0: -x, 1: -y, 2: -z), that the
+                                   closest-hit walk reads; MT_ORDERED / WATERTIGHT scenes of
+                                   at most 512 instances, else 0 bytes (read)           */
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {

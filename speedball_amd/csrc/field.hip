@@ -144,6 +144,7 @@ void free_scene(sbgi_field *f) {
     dfree(f->d_walk_nodes);
     dfree(f->d_walk_perm);
     dfree(f->d_tlas_sah);
+    dfree(f->d_tlas_oct);
     dfree(f->d_world);
     dfree(f->d_inst_pad);
     dfree(f->d_tlas_scratch);
@@ -261,6 +262,7 @@ int32_t device_tlas(sbgi_field *f, bool refit_compat) {
                                  f->d_inst_pad, f->stream));
     if (ordered(f)) {
         HIP_OK(f, launch_build_tlas(f->d_inst_pad, v.inst_count, f->d_tlas_scratch, f->d_tlas_sah, f->stream));
+        if (f->d_tlas_oct) HIP_OK(f, launch_tlas_octants(f->d_tlas_sah, f->tlas_sah_count, f->d_tlas_oct, f->stream));
     } else if (refit_compat) {
         HIP_OK(f, launch_fill_u32(f->d_tlas_arrive, 0u, v.tlas_node_count, f->stream));
         HIP_OK(f, launch_refit_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_inst_bounds, f->d_tlas_parent,
@@ -570,6 +572,8 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
         dalloc(&f->d_inst_stage, stage_bytes) != hipSuccess ||
         (ordered(f) && (dalloc(&f->d_tlas_sah, (2 * (size_t)v.inst_count - 1) * 32) != hipSuccess ||
                         dalloc(&f->d_tlas_scratch, tlas_build_scratch_bytes(v.inst_count)) != hipSuccess)) ||
+        (ordered(f) && 2 * (size_t)v.inst_count - 1 <= kTlasOctMax &&
+         dalloc(&f->d_tlas_oct, 8 * (2 * (size_t)v.inst_count - 1) * 32) != hipSuccess) ||
         dalloc(&f->d_tri, (size_t)v.tri_count * 48) != hipSuccess ||
         dalloc(&f->d_inst, (size_t)v.inst_count * 64) != hipSuccess ||
         dalloc(&f->d_tlas, (size_t)v.tlas_node_count * 32) != hipSuccess ||
@@ -1151,6 +1155,7 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
         case SBGI_BUF_TRI_MATERIAL: return f->h_tri_mat.size() * 4;
         case SBGI_BUF_INST_BOUNDS: return f->has_scene ? (size_t)f->view.inst_count * 24 : 0;
         case SBGI_BUF_TLAS_SAH: return f->has_scene ? (size_t)f->tlas_sah_count * 32 : 0;
+        case SBGI_BUF_TLAS_OCT: return f->has_scene && f->d_tlas_oct ? (size_t)f->tlas_sah_count * 8 * 32 : 0;
         case SBGI_BUF_EMITTER_VIS:
             return cascade < f->desc.cascade_count ? (size_t)f->casc[cascade].total * SBGI_MAX_EMITTERS * 4 : 0;
         case SBGI_BUF_LIGHT_GRID: return light_grid_cells(f) * (SBGI_LIGHTS_PER_CELL + 1) * 4;
@@ -1208,12 +1213,14 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
         return SBGI_OK;
     }
     if (buf_id == SBGI_BUF_TRI_PACKED || buf_id == SBGI_BUF_VERTEX_DATA || buf_id == SBGI_BUF_INST_BOUNDS ||
-        buf_id == SBGI_BUF_TLAS_SAH || buf_id == SBGI_BUF_TRI_INDEX || buf_id == SBGI_BUF_TRI_MATERIAL) {
+        buf_id == SBGI_BUF_TLAS_SAH || buf_id == SBGI_BUF_TLAS_OCT || buf_id == SBGI_BUF_TRI_INDEX ||
+        buf_id == SBGI_BUF_TRI_MATERIAL) {
         src = buf_id == SBGI_BUF_TRI_PACKED     ? (const void *)f->d_tri
               : buf_id == SBGI_BUF_VERTEX_DATA  ? (const void *)f->d_vdata
               : buf_id == SBGI_BUF_INST_BOUNDS  ? (const void *)f->d_inst_bounds
               : buf_id == SBGI_BUF_TRI_INDEX    ? (const void *)f->d_tri_index
               : buf_id == SBGI_BUF_TRI_MATERIAL ? (const void *)f->d_tri_mat
+              : buf_id == SBGI_BUF_TLAS_OCT     ? (const void *)f->d_tlas_oct
                                                 : (const void *)f->d_tlas_sah;
         HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;

@@ -83,6 +83,7 @@ struct sbgi_field {
     // the SAH TLAS of MT_ORDERED mode
     uint32_t *d_walk_nodes = nullptr, *d_walk_perm = nullptr;
     uint4 *d_tlas_sah = nullptr;
+    uint4 *d_tlas_oct = nullptr;  // its near-first copies per ray octant (closest-hit walk)
     uint32_t tlas_sah_count = 0;
     // per-frame TLAS on the device: world matrices (double, column-major 4x4 per slot),
     // the inflated bounds the SAH build reads, its scratch, and the staging area of
@@ -257,6 +258,7 @@ inline DevScene make_scene(const sbgi_field *f) {
     s.blas_octants = ordered(f) ? 8u : 1u;
     s.tri = f->d_tri;
     s.tlas = ordered(f) ? f->d_tlas_sah : f->d_tlas;
+    s.tlas_oct = ordered(f) ? f->d_tlas_oct : nullptr;
     s.inst = f->d_inst;
     s.mats = f->d_mats;
     s.tri_mat = f->d_tri_mat;

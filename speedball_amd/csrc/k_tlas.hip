@@ -336,7 +336,61 @@ __global__ __launch_bounds__(kTlasBlock) void k_build_tlas(const float *__restri
     }
 }
 
+// Near-first copies of the SAH TLAS, one per ray direction octant o (sign bits of x, y, z):
+// in copy o every interior node's children are ordered by the projection of their box
+// centres on the octant direction, nearer first, so a closest-hit walk meets near instances
+// first and culls far ones against its shrinking t. Positions follow the pre-order of the
+// source (a node's position is known before its children's): thread o walks the nodes in
+// index order in LDS, then every thread writes copy nodes {min.xyz, position + subtree size},
+// {max.xyz, payload}. Only TLASes of at most kTlasOctMax nodes get copies.
+__global__ __launch_bounds__(256) void k_tlas_octants(const uint4 *__restrict__ tlas, uint32_t m, uint4 *__restrict__ out) {
+    __shared__ uint4 s_nodes[2 * kTlasOctMax];
+    __shared__ uint16_t s_pos[8][kTlasOctMax];
+    for (uint32_t i = threadIdx.x; i < 2 * m; i += blockDim.x) s_nodes[i] = tlas[i];
+    for (uint32_t i = threadIdx.x; i < 8 * kTlasOctMax; i += blockDim.x) s_pos[i / kTlasOctMax][i % kTlasOctMax] = 0xFFFFu;
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const uint32_t o = threadIdx.x;
+        const float sx = (o & 1) ? -1.f : 1.f, sy = (o & 2) ? -1.f : 1.f, sz = (o & 4) ? -1.f : 1.f;
+        uint16_t *q = s_pos[o];
+        q[0] = 0;
+        for (uint32_t i = 0; i < m; i++) {
+            const uint4 a = s_nodes[2 * i], b = s_nodes[2 * i + 1];
+            if (b.w != 0xFFFFFFFFu) continue;
+            const uint32_t l = i + 1, r = l < m ? s_nodes[2 * l].w : 0u;  // left child's miss = right child
+            if (!(r > l && r < m)) continue;  // not a valid threaded layout: left as placed
+            const uint4 la = s_nodes[2 * l], lb = s_nodes[2 * l + 1], ra = s_nodes[2 * r], rb = s_nodes[2 * r + 1];
+            const float cl = (__uint_as_float(la.x) + __uint_as_float(lb.x)) * sx +
+                             (__uint_as_float(la.y) + __uint_as_float(lb.y)) * sy +
+                             (__uint_as_float(la.z) + __uint_as_float(lb.z)) * sz;
+            const float cr = (__uint_as_float(ra.x) + __uint_as_float(rb.x)) * sx +
+                             (__uint_as_float(ra.y) + __uint_as_float(rb.y)) * sy +
+                             (__uint_as_float(ra.z) + __uint_as_float(rb.z)) * sz;
+            const uint32_t size_l = r - l, size_r = a.w - r;
+            const bool left_near = cl <= cr;
+            q[l] = (uint16_t)(left_near ? q[i] + 1u : q[i] + 1u + size_r);
+            q[r] = (uint16_t)(left_near ? q[i] + 1u + size_l : q[i] + 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < 8 * m; x += blockDim.x) {
+        const uint32_t o = x / m, i = x - o * m;
+        const uint32_t q = s_pos[o][i];
+        if (q >= m) continue;  // unreached (only in an invalid layout)
+        const uint4 a = s_nodes[2 * i], b = s_nodes[2 * i + 1];
+        uint4 *d = out + 2 * ((size_t)o * m + q);
+        d[0] = make_uint4(a.x, a.y, a.z, q + (a.w - i));
+        d[1] = b;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_tlas_octants(const uint4 *tlas, uint32_t m, uint4 *out, hipStream_t st) {
+    if (!m || m > kTlasOctMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tlas_octants, dim3(1), dim3(256), 0, st, tlas, m, out);
+    return hipGetLastError();
+}
 
 size_t tlas_build_scratch_bytes(uint32_t n) { return (size_t)n * (2 * sizeof(uint32_t) + 2 * sizeof(uint4)); }
 

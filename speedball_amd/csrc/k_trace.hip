@@ -532,8 +532,10 @@ hipError_t launch_query(const DevScene &s, const float4 *in, float4 *out, uint32
     hipError_t e = hipMemsetAsync(head, 0, (size_t)kFetchHeads * kHeadStride * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
     const QuerySrc src{in, out};
+    DevScene sq = s;
+    sq.tlas_oct = nullptr;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, 0, n)), dim3(kWalkBlock), 0, st, s, src,
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, 0, n)), dim3(kWalkBlock), 0, st, sq, src,
                            nullptr, n, head, nullptr);
     };
     if (s.watertight) go(k_walk<WALK_MIXED, false, false, QuerySrc, true>);
@@ -548,9 +550,12 @@ hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t
 
 // One persistent walk launch: counting and LDS variants picked at run time.
 template <int MODE, class Src>
-void launch_walk(const DevScene &s, const Src &src, const uint32_t *n_dev, uint32_t n_host, uint32_t *heads,
+void launch_walk(const DevScene &scene, const Src &src, const uint32_t *n_dev, uint32_t n_host, uint32_t *heads,
                  unsigned long long *cnt, size_t lds, hipStream_t st) {
     const bool use_lds = lds <= kTraceLdsBudget;
+    // only the closest-hit walk takes the near-first TLAS copies
+    DevScene s = scene;
+    if (MODE != WALK_CLOSEST) s.tlas_oct = nullptr;
     auto go = [&](auto kern, size_t l) {
         hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, l, n_host)), dim3(kWalkBlock), l, st, s, src,
                            n_dev, n_host, heads, cnt);

@@ -210,6 +210,10 @@ size_t tlas_build_scratch_bytes(uint32_t n);
 hipError_t launch_inst_bounds(const float *mats, uint32_t inst_base, uint32_t n, const uint32_t *nodes,
                               const double *world, float *bounds, float *padded, hipStream_t st);
 hipError_t launch_build_tlas(const float *padded, uint32_t n, void *scratch, uint4 *tlas, hipStream_t st);
+// 8 near-first copies (one per ray direction octant) of a walk-format TLAS of m nodes,
+// out: 8 m nodes; TLASes of more than kTlasOctMax nodes get none
+constexpr uint32_t kTlasOctMax = 1023;
+hipError_t launch_tlas_octants(const uint4 *tlas, uint32_t m, uint4 *out, hipStream_t st);
 
 // k_build.hip — device BLAS rebuild (see that file). ranges: {root, N, t0, T} per BLAS;
 // tri_prefix: triangles before each range (n_ranges + 1, on the device).

@@ -37,6 +37,9 @@ struct DevScene {
     const uint4 *__restrict__ nodes;  // BLAS in the TLAS node format {min.xyz, miss}, {max.xyz, payload}
     const float4 *__restrict__ tri;
     const uint4 *__restrict__ tlas;
+    // near-first copies of the TLAS, one per ray direction octant (tlas_count nodes each;
+    // null: the closest-hit walk reads the single TLAS)
+    const uint4 *__restrict__ tlas_oct;
     const float4 *__restrict__ inst;
     const float *__restrict__ mats;
     const uint32_t *__restrict__ tri_mat;

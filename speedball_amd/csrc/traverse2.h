@@ -77,7 +77,7 @@ __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3
     r.end = s.tlas_count;
     r.tres = 0u;
     r.ipend = r.tpend = 0u;
-    r.base = 0u;
+    r.base = s.tlas_oct ? octant_of(rd, 8u) * s.tlas_count : 0u;
     r.inst = 0u;
     r.level = 0u;
     r.state = ST_NODE;
@@ -106,15 +106,18 @@ enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
 // face of its box is never culled.
 constexpr float kRobustFar = 1.0000007152557373f;
 
-template <bool COUNT, bool LDS, bool FMA, bool WT = false>
+template <bool COUNT, bool LDS, bool FMA, bool WT = false, bool TOCT = false>
 __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     uint4 a, b;
-    if (r.level) {
-        const uint4 *n = reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(s.nodes) +
+    if (r.level || (TOCT && s.tlas_oct)) {
+        // BLAS copy, or (closest-hit walk) the near-first TLAS copy of the ray's octant
+        const uint4 *pool = r.level ? s.nodes : s.tlas_oct;
+        const uint4 *n = reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) +
                                                          ((r.base + r.cur) << 5));
         a = n[0];
         b = n[1];
-        cnt.add_blas();
+        if (r.level) cnt.add_blas();
+        else cnt.add_tlas();
     } else {
         tlas_node<LDS>(s, r.cur, a, b);
         cnt.add_tlas();
@@ -188,7 +191,7 @@ __device__ __forceinline__ float3 sel3(bool c, float3 a, float3 b) {
     return f3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
 }
 
-template <bool COUNT, bool LDS, bool RCP>
+template <bool COUNT, bool LDS, bool RCP, bool TOCT = false>
 __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     const bool enter = (r.ipend >> 24) != 0u;
     const uint32_t slot = enter ? r.ipend & 0x00FFFFFFu : 0u;
@@ -206,7 +209,9 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
     r.dsign = enter ? r3.z : r.dsign;
     r.cur = enter ? c0 : r.tres;
     r.end = enter ? c1 : s.tlas_count;
-    r.base = enter ? octant_of(ld, s.blas_octants) * s.node_count : r.base;
+    // back on the TLAS: the near-first copy of the world ray's octant (closest-hit walk)
+    const uint32_t tbase = TOCT && s.tlas_oct ? octant_of(r.wd, 8u) * s.tlas_count : r.base;
+    r.base = enter ? octant_of(ld, s.blas_octants) * s.node_count : tbase;
     r.inst = enter ? slot : r.inst;
     r.level = enter ? 1u : 0u;
     // an empty range (or an inactive slot): on to the next slot
@@ -285,7 +290,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         if (!live) break;
         if (__ballot(r.state == ST_NODE)) {
             if (COUNT && lane == 0) cnt.node_w++;
-            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY, WT>(s, r, cnt);
+            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY, WT, MODE == WALK_CLOSEST>(s, r, cnt);
         }
         uint64_t mn = __ballot(r.state == ST_NODE);
         const uint64_t mt = __ballot(r.state == ST_TRI);
@@ -297,7 +302,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         const uint64_t mi = __ballot(r.state == ST_INST);
         if (mi && __popcll(mi) * inst_ratio >= __popcll(mn)) {
             if (COUNT && lane == 0) cnt.inst_w++;
-            if (r.state == ST_INST) inst_step<COUNT, LDS, MODE == WALK_ANY>(s, r, cnt);
+            if (r.state == ST_INST) inst_step<COUNT, LDS, MODE == WALK_ANY, MODE == WALK_CLOSEST>(s, r, cnt);
         }
     }
 }

@@ -346,9 +346,56 @@ def test_device_sah_tlas_edge_cases(n, kind):
         b64, pad = oracle.inst_bounds(f.read(N.BUF_MATERIALS, np.float32), scene.inst_base, n,
                                       f.read(N.BUF_BVH_NODES, np.uint32), wd)
         assert np.array_equal(f.read(N.BUF_INST_BOUNDS, np.float32).reshape(-1, 6), b64)
-        assert np.array_equal(f.read(N.BUF_TLAS_SAH, np.uint32).reshape(-1, 8), oracle.build_sah_tlas(pad))
+        sah = f.read(N.BUF_TLAS_SAH, np.uint32).reshape(-1, 8)
+        assert np.array_equal(sah, oracle.build_sah_tlas(pad))
+        oct_ = f.read(N.BUF_TLAS_OCT, np.uint32) if len(sah) <= 1023 else np.zeros(0, np.uint32)
+        if len(sah) <= 1023:
+            check_tlas_octants(sah, oct_.reshape(8, -1, 8))
+        else:
+            assert oct_.size == 0  # too large for copies: the closest-hit walk reads the SAH TLAS
     finally:
         f.close()
+
+
+def tlas_octant_copies(sah):
+    """The near-first copies of a walk-format TLAS: copy o orders each interior node's
+    children by the projection of their box centres (min + max, in f32) on the octant
+    direction (-1 on the axes whose bit is set), nearer first; positions follow the source's
+    pre-order, miss = position + subtree size."""
+    m = len(sah)
+    f = sah.view(np.float32)
+    out = np.zeros((8, m, 8), np.uint32)
+    for o in range(8):
+        sgn = np.float32([-1.0 if o & (1 << a) else 1.0 for a in range(3)])
+        q = np.zeros(m, np.int64)
+        for i in range(m):
+            if sah[i, 7] != 0xFFFFFFFF:
+                continue
+            l, r = i + 1, int(sah[i + 1, 3])
+            cl = (f[l, 0] + f[l, 4]) * sgn[0] + (f[l, 1] + f[l, 5]) * sgn[1]
+            cl = np.float32(cl) + np.float32((f[l, 2] + f[l, 6]) * sgn[2])
+            cr = (f[r, 0] + f[r, 4]) * sgn[0] + (f[r, 1] + f[r, 5]) * sgn[1]
+            cr = np.float32(cr) + np.float32((f[r, 2] + f[r, 6]) * sgn[2])
+            size_l, size_r = r - l, int(sah[i, 3]) - r
+            near_l = cl <= cr
+            q[l] = q[i] + 1 if near_l else q[i] + 1 + size_r
+            q[r] = q[i] + 1 + size_l if near_l else q[i] + 1
+        for i in range(m):
+            row = sah[i].copy()
+            row[3] = q[i] + (int(sah[i, 3]) - i)
+            out[o, q[i]] = row
+    return out
+
+
+def check_tlas_octants(sah, got):
+    want = tlas_octant_copies(sah)
+    assert np.array_equal(got, want)
+    # each copy is a valid threaded layout over the same leaves
+    for o in range(8):
+        c = got[o]
+        m = len(c)
+        assert sorted(c[c[:, 7] != 0xFFFFFFFF, 7].tolist()) == sorted(sah[sah[:, 7] != 0xFFFFFFFF, 7].tolist())
+        assert np.all(c[:, 3] > np.arange(m)) and np.all(c[:, 3] <= m)
 
 
 # ---------------------------------------------------------------- device BLAS rebuild

@@ -122,6 +122,10 @@ extern "C" {
                                    ray direction octant (bit 0: -x, 1: -y, 2: -z), that the
                                    closest-hit walk reads; MT_ORDERED / WATERTIGHT scenes of
                                    at most 512 instances, else 0 bytes (read)           */
+#define SBGI_BUF_SHADOW_VIS 29  /* u64[updated*R] the last solve's light visibility per
+                                   ray, same order as RAY_DATA: bit l set when the ray's
+                                   hit passed light l's NEE gate and the shadow ray toward
+                                   l is unblocked (read)                               */
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {

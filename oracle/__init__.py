@@ -180,21 +180,27 @@ class OracleField:
         from speedball_amd._native import default_tick_params
         return default_tick_params(**kw)
 
-    def trace(self, p, updated=None, count=False):
+    def trace(self, p, updated=None, count=False, vis=False):
+        """vis: also keep the light visibility of every ray in self.vis (uint64 per ray,
+        reference order: bit l set when the hit passed light l's gate and is unshadowed)."""
         n = (updated or self.P) * self.rays
         self.ray_data = np.zeros(n * 4, self.dt)
         self.hit_ids = np.zeros(n * 2, np.int32)
+        self.vis = np.zeros(n, np.uint64) if vis else None
         cnt = np.zeros(5, np.uint64) if count else None
         sg = _fn(self.prec, "set_light_grid")
         if self.light_grid is not None:
             sg(_p(self.light_grid), C.byref(self.c0))
         _fn(self.prec, "set_rough")(C.c_int(1 if self.reflections else 0))
+        if vis:
+            _fn(self.prec, "set_vis_out")(_p(self.vis))
         try:
             _fn(self.prec, "trace")(C.byref(self.scene.view()), C.byref(self.c), C.c_uint32(self.rays), C.byref(p),
                                     _p(self.atlas), _p(self.meta), _p(self.ray_data), _p(self.hit_ids), _p(cnt))
         finally:
             sg(None, None)
             _fn(self.prec, "set_rough")(C.c_int(0))
+            _fn(self.prec, "set_vis_out")(None)
         return cnt
 
     def classify(self):

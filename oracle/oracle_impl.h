@@ -610,6 +610,13 @@ static void P(sample_atlas)(const sbgi_cascade_desc *c, const REAL *atlas, const
     for (int a = 0; a < 3; a++) out[a] = acc[a] / den;
 }
 
+/* Light visibility of the ray this thread traced last: bit l set when its hit passed light
+ * l's NEE gate and the shadow ray toward l is unblocked. P(trace) stores it per ray in
+ * P(g_vis) when a caller set one (P(set_vis_out)). */
+static __thread uint64_t P(t_vis) = 0;
+static uint64_t *P(g_vis) = NULL;
+void P(set_vis_out)(uint64_t *vis) { P(g_vis) = vis; }
+
 /* One ray of traceKernel js/gi_probes.js:1977-2214 (non-clustered NEE, no texture maps,
  * rough reflections off). out = (r, g, b, hitT). prev_atlas may be NULL (zero atlas). */
 void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t rays,
@@ -622,6 +629,7 @@ void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t
     P(ray_dir)(k, rays, p->frame_jitter, rd);
     REAL best_t;
     int32_t best_tri, best_inst;
+    P(t_vis) = 0;
     P(closest)(s, ro, rd, &best_t, &best_tri, &best_inst, cnt);
     ids[0] = best_tri;
     ids[1] = best_inst;
@@ -683,6 +691,7 @@ void P(trace_ray)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t
             REAL lpeak = P(fmax)(P(fmax)(lcol[0], lcol[1]), lcol[2]);
             if (!(ndl * lpeak > 0 && (is_dir || lrange <= 0 || dist < lrange))) continue;
             if (P(any)(s, hit, wi, dist - bias, cnt)) continue;
+            P(t_vis) |= 1ull << li;
             REAL pw = RPOW(dist, ldecay);
             REAL falloff = RCONST(1) / (pw > RCONST(0.01) ? pw : RCONST(0.01));
             REAL rr = dist / (lrange > RCONST(1e-4) ? lrange : RCONST(1e-4));
@@ -755,6 +764,7 @@ void P(trace)(const sbgi_scene_view *s, const sbgi_cascade_desc *c, uint32_t ray
             int32_t ids[2];
             size_t o = (size_t)slot * rays + k;
             P(trace_ray)(s, c, rays, p, prev_atlas, meta, probe, k, ray_data + o * 4, ids, cnt);
+            if (P(g_vis)) P(g_vis)[o] = P(t_vis);
             if (hit_ids) {
                 hit_ids[o * 2] = ids[0];
                 hit_ids[o * 2 + 1] = ids[1];

@@ -35,6 +35,7 @@ BUF_RAY_DATA, BUF_HIT_IDS, BUF_NODES_REFIT, BUF_TRI_PACKED, BUF_INST_BOUNDS, BUF
 BUF_EMITTER_VIS, BUF_LIGHT_GRID = 21, 22
 BUF_ROUGH_STATE, BUF_ROUGH_ATLAS, BUF_GLOSSY_STATE, BUF_GLOSSY_WEIGHT, BUF_GLOSSY_ATLAS = 23, 24, 25, 26, 27
 BUF_TLAS_OCT = 28
+BUF_SHADOW_VIS = 29
 REFLECT_OFF, REFLECT_ROUGH, REFLECT_HIGH, REFLECT_ULTRA = 0, 1, 2, 3
 
 # every symbol include/sbgi.h declares

@@ -1145,6 +1145,7 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
     switch (buf_id) {
         case SBGI_BUF_RAY_DATA: return (size_t)f->last_updated * R * 16;
         case SBGI_BUF_HIT_IDS: return f->hit_ids ? (size_t)f->last_updated * R * 8 : 0;
+        case SBGI_BUF_SHADOW_VIS: return (size_t)f->last_updated * R * 8;
         case SBGI_BUF_MATERIALS: return f->h_mats.size() * 4;
         case SBGI_BUF_LIGHTS: return f->h_lights.size() * 4;
         case SBGI_BUF_NODES_REFIT:
@@ -1192,13 +1193,15 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
         HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }
-    if (buf_id == SBGI_BUF_RAY_DATA || buf_id == SBGI_BUF_HIT_IDS) {
+    if (buf_id == SBGI_BUF_RAY_DATA || buf_id == SBGI_BUF_HIT_IDS || buf_id == SBGI_BUF_SHADOW_VIS) {
         // device order is k-major (k * updated + slot); the reference order is slot * R + k
         const uint32_t R = f->desc.rays_per_probe, U = f->last_updated;
         const size_t es = buf_id == SBGI_BUF_RAY_DATA ? 16 : 8;
         std::vector<char> tmp(bytes);
-        HIP_OK(f, hipMemcpy(tmp.data(), buf_id == SBGI_BUF_RAY_DATA ? (const void *)f->rays : (const void *)f->hit_ids,
-                            bytes, hipMemcpyDeviceToHost));
+        src = buf_id == SBGI_BUF_RAY_DATA  ? (const void *)f->rays
+              : buf_id == SBGI_BUF_HIT_IDS ? (const void *)f->hit_ids
+                                           : (const void *)f->tw.vis;
+        HIP_OK(f, hipMemcpy(tmp.data(), src, bytes, hipMemcpyDeviceToHost));
         char *o = static_cast<char *>(dst);
         for (uint32_t k = 0; k < R; k++)
             for (uint32_t sl = 0; sl < U; sl++) memcpy(o + ((size_t)sl * R + k) * es, tmp.data() + ((size_t)k * U + sl) * es, es);

@@ -453,7 +453,7 @@ uint32_t persistent_blocks(K kernel, uint32_t block, size_t lds, uint32_t work_i
         per_cu = 1;
     if (waves_cap) per_cu = min(per_cu, max(1, (int)(waves_cap / (block / 64u))));
     const uint32_t cap = (uint32_t)device_cus() * (uint32_t)per_cu;
-    const uint32_t want = (work_items + block - 1) / block;
+    const uint32_t want = (uint32_t)(((uint64_t)work_items + block - 1) / block);
     return want < cap ? (want ? want : 1u) : cap;
 }
 

@@ -30,6 +30,7 @@ const BUF = Object.freeze({
     IRR_STATE: 10, DEPTH_STATE: 11, META: 12, IRR_ATLAS: 13, DEPTH_ATLAS: 14, RAY_DATA: 15, HIT_IDS: 16,
     NODES_REFIT: 17, TRI_PACKED: 18, INST_BOUNDS: 19, TLAS_SAH: 20, EMITTER_VIS: 21, LIGHT_GRID: 22,
     ROUGH_STATE: 23, ROUGH_ATLAS: 24, GLOSSY_STATE: 25, GLOSSY_WEIGHT: 26, GLOSSY_ATLAS: 27, TLAS_OCT: 28,
+    SHADOW_VIS: 29,
 });
 // reflectionQuality tiers (js/gi_probes.js:60-65) -> SBGI_REFLECT_*
 const REFLECTION = Object.freeze({ off: 0, rough: 1, high: 2, ultra: 3 });

@@ -227,6 +227,69 @@ def test_instanced_subset(instanced_small, traversal):
     assert np.percentile(ei, 99) <= 1e-3
 
 
+def _popcount(x):
+    return int(np.unpackbits(np.ascontiguousarray(x).view(np.uint8)).sum())
+
+
+@pytest.mark.parametrize("traversal", [N.TRAV_MT_COMPAT, N.TRAV_MT_ORDERED], ids=["compat", "ordered"])
+@pytest.mark.parametrize("name", ["room", "instanced_small"])
+def test_shadow_visibility_matches_oracle(name, traversal, request):
+    """The any-hit walk on its own: per ray, the light-visibility mask of the solve
+    (SBGI_BUF_SHADOW_VIS, bit l: the hit passed light l's gate and its shadow ray is
+    unblocked) against the FP32 restatement's. Rays whose hit ids differ (near ties) are
+    left out; on the others the gate is the same, and a shadow answer may differ only where
+    the walk's 1-ulp hardware reciprocal decides a ray grazing an occluder's edge the other
+    way: at most 1 shadow ray in 10,000, and some of the emitted shadow rays must be
+    blocked and some visible."""
+    w = request.getfixturevalue(name)
+    print(f"case: shadow visibility {name} traversal {traversal}", flush=True)
+    f = make_field(w, traversal=traversal)
+    try:
+        f.set_counting(True)  # for the count of emitted shadow rays
+        f.tick(**params(w))
+        g_vis = f.read(N.BUF_SHADOW_VIS, np.uint64).reshape(-1, w.rays)
+        g_hits = f.read_hit_ids()
+        shadow_rays = int(f.stats().shadow_rays)
+    finally:
+        f.close()
+    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec="32")
+    o.trace(o.params(**params(w)), vis=True)
+    r_vis = o.vis.reshape(-1, w.rays)
+    same = np.all(g_hits == o.hits_view(), axis=-1)
+    diff = _popcount((g_vis ^ r_vis)[same])
+    visible = _popcount(r_vis[same])
+    print(f"rays {same.size}, same hit {int(same.sum())}, shadow rays {shadow_rays}, visible {visible}, "
+          f"bits that differ {diff}")
+    assert 0 < visible < shadow_rays
+    assert diff <= max(1, shadow_rays // 10000)
+
+
+def test_shadow_visibility_headline_sample():
+    """The same bar on the headline instanced field (32x16x24 probes x 256 rays, 16
+    lights, ordered traversal): the GPU updates every probe, the FP32 restatement traces
+    probes 6,000-6,511 (131,072 rays) and their masks are compared."""
+    w = sg.instanced()
+    off, n = 6000, 512
+    print("case: shadow visibility, headline field sample", flush=True)
+    f = make_field(w, traversal=N.TRAV_MT_ORDERED)
+    try:
+        f.tick(**params(w))
+        g_vis = f.read(N.BUF_SHADOW_VIS, np.uint64).reshape(-1, w.rays)[off:off + n]
+        g_hits = f.read_hit_ids()[off:off + n]
+    finally:
+        f.close()
+    o = oracle.OracleField(w.scene, w.grid_min, w.grid_size, w.res, w.rays, w.depth_oct, prec="32")
+    o.trace(o.params(**params(w), probe_offset=off, updated_count=n), updated=n, vis=True)
+    r_vis = o.vis.reshape(-1, w.rays)
+    same = np.all(g_hits == o.hits_view(), axis=-1)
+    diff = _popcount((g_vis ^ r_vis)[same])
+    visible, any_bit = _popcount(r_vis[same]), int((r_vis[same] != 0).sum())
+    print(f"rays {same.size}, same hit {int(same.sum())}, visible shadow rays {visible}, rays with a visible "
+          f"light {any_bit}, bits that differ {diff}")
+    assert same.mean() > 0.999 and visible > 10000
+    assert diff <= max(1, visible // 10000)
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(shutil.which("node") is None, reason="node is not installed")
 def test_js_shim_matches_ctypes_path(tmp_path):

@@ -550,6 +550,57 @@ def test_nee_ranged_point_light_window_and_gate():
     assert np.all(np.abs(alt[inside] - rad[inside]).max(axis=1) > 1e-6)
 
 
+def test_nee_visibility_mask_follows_an_occluder():
+    """Per-ray light visibility (OracleField.trace(vis=True)): a point light above a
+    double-sided floor and a square occluder between them. A floor hit's bit is set exactly
+    when the segment from its offset hit point to the light misses the square (computed
+    here analytically); a hit on the square's underside sees the light from behind
+    (n . l < 0), so its bit is clear, and a miss has no bit."""
+    b = SceneBuilder()
+    m = b.add_material(uber_material((0.5, 0.5, 0.5), side=2))
+    fv = np.array([[-50, 0, -50], [50, 0, -50], [50, 0, 50], [-50, 0, 50]], np.float32)
+    a = 0.6
+    sv = np.array([[-a, 1.5, -a], [a, 1.5, -a], [a, 1.5, a], [-a, 1.5, a]], np.float32)
+    quad = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
+    b.add_instance(b.add_mesh(np.concatenate([fv, sv]), np.concatenate([quad, quad + 4]),
+                              tri_material=np.full(4, m, np.uint32)))
+    lp = np.array([0.2, 3.0, -0.1])
+    b.add_light(point_light(tuple(lp), intensity=4.0))
+    scene = b.finalize()
+    res, gmin, gsize, rays = (3, 2, 3), (-2.0, 0.5, -2.0), (4.0, 0.5, 4.0), 256
+    o = oracle.OracleField(scene, gmin, gsize, res, rays, 6, prec="64")
+    o.trace(o.params(frame_jitter=0.15), vis=True)
+    vis = o.vis.reshape(-1, rays)
+    hits, rv = o.hits_view(), o.rays_view()
+    res_a = np.asarray(res)
+    bias = max(min(np.asarray(gsize) / np.maximum(res_a - 1, 1)) * 0.005, 1e-3)
+    n_shadow = n_lit = n_under = 0
+    for slot in range(int(np.prod(res))):
+        ix, iy, iz = slot % res[0], (slot // res[0]) % res[1], slot // (res[0] * res[1])
+        ro = np.asarray(gmin) + np.array([ix, iy, iz]) / np.maximum(res_a - 1, 1) * np.asarray(gsize)
+        for k in range(rays):
+            rd = oracle.ray_dir(k, rays, 0.15)
+            if hits[slot, k, 0] < 0:
+                assert vis[slot, k] == 0
+                continue
+            if rd[1] > 0:  # the square, from below
+                assert vis[slot, k] == 0
+                n_under += 1
+                continue
+            h = ro + rd * rv[slot, k, 3] + np.array([0.0, bias, 0.0])
+            s = (1.5 - h[1]) / (lp[1] - h[1])
+            q = h + s * (lp - h)
+            margin = min(abs(q[0]) - a, abs(q[2]) - a) if max(abs(q[0]), abs(q[2])) < a else \
+                max(abs(q[0]) - a, abs(q[2]) - a)
+            if abs(margin) < 1e-6:
+                continue
+            blocked = max(abs(q[0]), abs(q[2])) < a
+            assert vis[slot, k] == (0 if blocked else 1), (slot, k, q)
+            n_shadow += int(blocked)
+            n_lit += int(not blocked)
+    assert n_shadow > 50 and n_lit > 200 and n_under > 50
+
+
 # ------------------------------------------------------------- watertight triangle test
 def _mesh_scene(v, t, side=2):
     b = SceneBuilder()

@@ -283,6 +283,7 @@ inline DevScene make_scene(const sbgi_field *f) {
     s.watertight = f->desc.traversal == SBGI_TRAV_WATERTIGHT ? 1u : 0u;
     s.tri_index = f->d_tri_index;
     s.vdata = f->d_vdata;
+    s.lds_inst = s.tlas_count * 2;
     return s;
 }
 

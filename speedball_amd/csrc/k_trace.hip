@@ -40,6 +40,7 @@ constexpr uint32_t kBlock = 256;
 // Walk blocks of 8 waves share one LDS copy of the TLAS, so the copy does not limit how
 // many waves a CU holds.
 constexpr uint32_t kWalkBlock = 512;
+constexpr uint32_t kWalkBlockSmall = 256;
 
 // Ray sources of the walk kernels (see walk_wave).
 
@@ -153,14 +154,14 @@ struct QuerySrc {
 
 // Persistent walk: every wave pulls rays from the source until it is drained. The number
 // of rays is n_host, or min(*n_dev, n_host) when the count is only known on the device.
-template <int MODE, bool COUNT, bool LDS, class Src, bool WT = false>
-__global__ void __launch_bounds__(kWalkBlock) k_walk(DevScene s, Src src, const uint32_t *__restrict__ n_dev,
-                                                      uint32_t n_host, uint32_t *heads,
-                                                      unsigned long long *__restrict__ counters) {
+template <int MODE, bool COUNT, bool LDS, class Src, bool WT = false, int BLOCK = kWalkBlock>
+__global__ void __launch_bounds__(BLOCK) k_walk(DevScene s, Src src, const uint32_t *__restrict__ n_dev,
+                                                 uint32_t n_host, uint32_t *heads,
+                                                 unsigned long long *__restrict__ counters) {
     if (LDS) stage_tlas_lds(s);
     const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     TravCounters<COUNT> cnt;
-    walk_wave<MODE, COUNT, LDS, WT>(s, src, n, heads, blockIdx.x * (kWalkBlock / 64u) + (threadIdx.x >> 6), cnt);
+    walk_wave<MODE, COUNT, LDS, WT>(s, src, n, heads, blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6), cnt);
     if (COUNT) {
         atomicAdd(&counters[0], (unsigned long long)cnt.blas);
         atomicAdd(&counters[1], (unsigned long long)cnt.tlas);
@@ -553,9 +554,20 @@ template <int MODE, class Src>
 void launch_walk(const DevScene &scene, const Src &src, const uint32_t *n_dev, uint32_t n_host, uint32_t *heads,
                  unsigned long long *cnt, size_t lds, hipStream_t st) {
     const bool use_lds = lds <= kTraceLdsBudget;
-    // only the closest-hit walk takes the near-first TLAS copies
+    // only the closest-hit walk takes the near-first TLAS copies; it reads them from global
+    // memory and stages only the instance records, in blocks of 4 waves (its VGPRs allow 7
+    // waves per SIMD, which blocks of 8 waves would round down to 6)
     DevScene s = scene;
     if (MODE != WALK_CLOSEST) s.tlas_oct = nullptr;
+    if (MODE == WALK_CLOSEST && s.tlas_oct && use_lds && !s.watertight) {
+        s.lds_inst = 0;
+        const size_t l = (size_t)s.inst_count * 64;
+        auto k = cnt ? k_walk<MODE, true, true, Src, false, kWalkBlockSmall>
+                     : k_walk<MODE, false, true, Src, false, kWalkBlockSmall>;
+        hipLaunchKernelGGL(k, dim3(persistent_blocks(k, kWalkBlockSmall, l, n_host)), dim3(kWalkBlockSmall), l, st, s,
+                           src, n_dev, n_host, heads, cnt);
+        return;
+    }
     auto go = [&](auto kern, size_t l) {
         hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, kWalkBlock, l, n_host)), dim3(kWalkBlock), l, st, s, src,
                            n_dev, n_host, heads, cnt);

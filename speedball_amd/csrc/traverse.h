@@ -47,7 +47,7 @@ template <bool LDS>
 __device__ __forceinline__ void inst_rows(const DevScene &s, uint32_t inst, float4 &r0, float4 &r1, float4 &r2,
                                           float4 &r3) {
     if (LDS) {
-        const uint4 *p = g_lds + (size_t)s.tlas_count * 2 + (size_t)inst * 4;
+        const uint4 *p = g_lds + (size_t)s.lds_inst + (size_t)inst * 4;
         r0 = as_f4(p[0]);
         r1 = as_f4(p[1]);
         r2 = as_f4(p[2]);
@@ -79,9 +79,10 @@ __device__ __forceinline__ uint32_t octant_of(float3 rd, uint32_t octants) {
     return (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
 }
 
-// Copy the TLAS nodes and instance records into the block's LDS (LDS=true walks).
+// Copy the TLAS nodes (unless lds_inst is 0) and the instance records into the block's
+// LDS (LDS=true walks).
 __device__ __forceinline__ void stage_tlas_lds(const DevScene &s) {
-    const uint32_t nt = s.tlas_count * 2, ni = s.inst_count * 4;
+    const uint32_t nt = s.lds_inst, ni = s.inst_count * 4;
     const float4 *inst = s.inst;
     for (uint32_t i = threadIdx.x; i < nt + ni; i += blockDim.x) {
         if (i < nt) {

@@ -194,6 +194,8 @@ def room_leg(steps=20, warmup=3):
         f.wait()
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1000.0 / steps
+        f.set_stage_timing(True)  # stage events for one more tick (they cost ~0.1 ms a tick)
+        f.tick(**kw)
         s = f.stats()
         f.close()
         chk = fp64_check(w, states, depth_oct, [(0, w.probes)])
@@ -321,6 +323,7 @@ def dynamic_leg(frames=600, warmup=10):
     w = scenegen.dynamic()
     f = ProbeField(w.cascades[0][2], w.cascades[0][0], w.cascades[0][1], rays=w.rays, depth_oct=w.depth_oct,
                    precision=PRECISIONS["bf16x3"], cascades=w.cascades, traversal=1)
+    f.set_stage_timing(True)  # the frames are timed by the field's stage events
     f.set_scene(w.scene)
     f.set_instance_transforms(np.arange(w.scene.inst_count), list(w.slot_worlds))
     f.set_skins(w.skin_sets, w.n_bones)
@@ -462,7 +465,9 @@ def batch_leg(frames=10, autotune=True):
         ms.append(m)
         rep.append(r)
     batch.close()
-    # the same fields, one cascade at a time
+    # the same fields, one cascade at a time (timed by their stage events)
+    for f in fields:
+        f.set_stage_timing(True)
     single = []
     for _ in range(3):
         tot = 0.0
@@ -539,6 +544,7 @@ def reflect_leg(steps=10):
                     ("ultra", N.REFLECT_ULTRA)):
         f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct,
                        precision=PRECISIONS["bf16x3"], traversal=1, reflections=q)
+        f.set_stage_timing(True)
         f.set_scene(w.scene)
         total, blend = [], []
         for i in range(steps + 2):
@@ -580,6 +586,7 @@ def lights_leg(steps=10):
     for name, clustered in (("all_lights", False), ("clustered", True)):
         f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct,
                        precision=PRECISIONS["bf16x3"], traversal=1, clustered=clustered)
+        f.set_stage_timing(True)
         f.set_scene(w.scene)
         f.tick(**kw)
         if clustered:
@@ -691,7 +698,9 @@ def main():
     wall = time.perf_counter() - t_start
     ms_step = wall * 1000.0 / args.steps
 
-    # per-stage device time from the library's HIP events (recorded on the field's stream)
+    # per-stage device time from the library's HIP events (recorded on the field's stream;
+    # the timed region above runs without them: they cost ~0.1 ms a tick)
+    f.set_stage_timing(True)
     stage = {"trace": [], "blend": [], "upload": [], "total": [], "raydir": [], "walk_primary": [], "emit": [],
              "walk_shadow": [], "resolve": []}
     for _ in range(args.steps):

@@ -260,6 +260,9 @@ typedef struct sbgi_stats {
     uint64_t shadow_waves[4];
     float ms_emitter_vis;        /* emitter visibility rays (scenes with emitters)    */
     uint32_t tick_replayed;      /* 1 when the last tick replayed its captured hipGraph */
+    uint32_t stage_timed;        /* 1 when the ms_* stage times above belong to the last
+                                    tick: an eager tick, or a graph tick with stage timing
+                                    on (sbgi_set_stage_timing); else they are 0          */
 } sbgi_stats;
 
 /* Wave schedule of the persistent walks (no reference counterpart: it is how this build
@@ -395,6 +398,13 @@ int32_t sbgi_classify(sbgi_field *f, uint32_t cascade);
  * visits, triangle tests, instance entries and shadow rays into sbgi_stats. Turning it
  * on or off resets the counters. */
 int32_t sbgi_set_counting(sbgi_field *f, int32_t on);
+
+/* Stage timing (no reference counterpart): while on, a tick replayed from its hipGraph
+ * records the HIP events of its stages (ray directions, the two walks, emission, shading,
+ * blend, upload), which sbgi_stats reports as ms_*. Off (the default), the tick's graph
+ * is one captured piece without event nodes, which saves ~0.1 ms per tick on MI355X;
+ * eager ticks record the events either way. Changing it recaptures the next tick. */
+int32_t sbgi_set_stage_timing(sbgi_field *f, int32_t on);
 
 /* Ray queries against the resident scene: the closest-hit / any-hit walks that the
  * trace uses (traverseClosest / traverseAny, js/spectral_traverse.js:441-474, 539-569),

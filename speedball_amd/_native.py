@@ -42,7 +42,7 @@ REFLECT_OFF, REFLECT_ROUGH, REFLECT_HIGH, REFLECT_ULTRA = 0, 1, 2, 3
 EXPORTS = [
     "sbgi_tick_params_default", "sbgi_version", "sbgi_create", "sbgi_set_scene", "sbgi_update_ranges",
     "sbgi_refit", "sbgi_set_instance_transforms", "sbgi_tick", "sbgi_wait", "sbgi_poll", "sbgi_read",
-    "sbgi_write", "sbgi_buffer_bytes", "sbgi_query_rays", "sbgi_query_rays_noalloc", "sbgi_get_stats", "sbgi_set_counting", "sbgi_last_error",
+    "sbgi_write", "sbgi_buffer_bytes", "sbgi_query_rays", "sbgi_query_rays_noalloc", "sbgi_get_stats", "sbgi_set_counting", "sbgi_set_stage_timing", "sbgi_last_error",
     "sbgi_destroy", "sbgi_sb_create", "sbgi_sb_add_material", "sbgi_sb_add_mesh", "sbgi_sb_add_instance",
     "sbgi_sb_add_light", "sbgi_sb_finalize", "sbgi_sb_tlas_order", "sbgi_sb_set_transform",
     "sbgi_sb_last_error", "sbgi_sb_destroy", "sbgi_diff_param_count", "sbgi_diff_albedo_map", "sbgi_diff_forward",
@@ -105,7 +105,7 @@ class Stats(C.Structure):
                 ("ms_resolve", C.c_float), ("shadow_blas_nodes", C.c_uint64), ("shadow_tlas_nodes", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64),
                 ("walk_waves", C.c_uint64 * 4), ("shadow_waves", C.c_uint64 * 4), ("ms_emitter_vis", C.c_float),
-                ("tick_replayed", C.c_uint32)]
+                ("tick_replayed", C.c_uint32), ("stage_timed", C.c_uint32)]
 
 
 class SkinSet(C.Structure):
@@ -179,6 +179,7 @@ def lib() -> C.CDLL:
         "sbgi_buffer_bytes": (C.c_size_t, [vp, i32, u32]),
         "sbgi_get_stats": (i32, [vp, C.POINTER(Stats)]),
         "sbgi_set_counting": (i32, [vp, i32]),
+        "sbgi_set_stage_timing": (i32, [vp, i32]),
         "sbgi_query_rays": (i32, [vp, vp, u32, vp]),
         "sbgi_query_rays_noalloc": (i32, [vp, vp, u32, vp]),
         "sbgi_sample_params_default": (None, [C.POINTER(SampleParams)]),

@@ -932,7 +932,7 @@ static void tick_key(const sbgi_field *f, const CascadeState &cs, const DevScene
                           cs.gnum, cs.gden, cs.ratlas, cs.gatlas, f->rtables};
     put(ptrs, sizeof(ptrs));
     const uint32_t words[] = {f->tw.queue.capacity, f->count_mode ? 1u : 0u, f->desc.blend_precision,
-                              f->desc.reflections};
+                              f->desc.reflections, f->stage_timing ? 1u : 0u};
     put(words, sizeof(words));
 }
 
@@ -993,13 +993,16 @@ struct TickGraphBuild {
     }
 };
 
+// Without stage timing the tick is captured as one piece: no stage boundary is marked.
+static hipError_t no_mark(void *, hipEvent_t, hipStream_t) { return hipSuccess; }
+
 static hipError_t build_tick_graph(sbgi_field *f, CascadeState &cs, const DevScene &s, const DevCascade &c,
                                    const sbgi_tick_params *p, hipStream_t st, hipGraph_t *out) {
     *out = nullptr;
     TickGraphBuild b;
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    const StageMarker mk{&TickGraphBuild::mark, &b};
+    const StageMarker mk = f->stage_timing ? StageMarker{&TickGraphBuild::mark, &b} : StageMarker{&no_mark, nullptr};
     const int32_t rc = enqueue_tick(f, cs, s, c, p, st, mk);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cap);
@@ -1088,6 +1091,7 @@ int32_t sbgi_tick(sbgi_field *f, const sbgi_tick_params *p, uint64_t *ticket) {
     f->last_updated = updated;
     f->last_cascade = p->cascade;
     f->last_replayed = replayed;
+    f->last_timed = eager || f->stage_timing;
     f->stats_valid = true;
     if (ticket) *ticket = f->ticket;
     return SBGI_OK;
@@ -1277,14 +1281,18 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     if (!f || !out) return SBGI_E_INVALID;
     memset(out, 0, sizeof(*out));
     if (!f->stats_valid) return fail(f, SBGI_E_STATE, "no tick yet");
-    HIP_OK(f, hipEventSynchronize(f->ev[4]));
-    float ms[4];
-    for (int i = 0; i < 4; i++) HIP_OK(f, hipEventElapsedTime(&ms[i], f->ev[i], f->ev[i + 1]));
-    out->ms_raydir = ms[0];
-    out->ms_trace = ms[1];
-    out->ms_blend = ms[2];
-    out->ms_upload = ms[3];
-    HIP_OK(f, hipEventElapsedTime(&out->ms_total, f->ev[0], f->ev[4]));
+    if (f->last_timed) {
+        HIP_OK(f, hipEventSynchronize(f->ev[4]));
+        float ms[4];
+        for (int i = 0; i < 4; i++) HIP_OK(f, hipEventElapsedTime(&ms[i], f->ev[i], f->ev[i + 1]));
+        out->ms_raydir = ms[0];
+        out->ms_trace = ms[1];
+        out->ms_blend = ms[2];
+        out->ms_upload = ms[3];
+        HIP_OK(f, hipEventElapsedTime(&out->ms_total, f->ev[0], f->ev[4]));
+    } else {
+        HIP_OK(f, hipStreamSynchronize(f->stream));
+    }
     out->updated = f->last_updated;
     out->rays = f->last_updated * f->desc.rays_per_probe;
     Counters c{};
@@ -1303,7 +1311,7 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
         out->shadow_waves[i] = c.shadow_walk[4 + i];
     }
     float sub[4] = {0.f, 0.f, 0.f, 0.f};
-    if (f->last_updated) {
+    if (f->last_updated && f->last_timed) {
         for (int i = 0; i < 3; i++) HIP_OK(f, hipEventElapsedTime(&sub[i], f->tev[i], f->tev[i + 1]));
         HIP_OK(f, hipEventElapsedTime(&sub[3], f->tev[3], f->eev));
         HIP_OK(f, hipEventElapsedTime(&out->ms_emitter_vis, f->eev, f->ev[2]));
@@ -1313,6 +1321,7 @@ int32_t sbgi_get_stats(sbgi_field *f, sbgi_stats *out) {
     out->ms_walk_shadow = sub[2];
     out->ms_resolve = sub[3];
     out->tick_replayed = f->last_replayed ? 1u : 0u;
+    out->stage_timed = f->last_timed ? 1u : 0u;
     return SBGI_OK;
 }
 
@@ -1725,6 +1734,14 @@ int32_t sbgi_set_counting(sbgi_field *f, int32_t on) {
     f->count_mode = on != 0;
     HIP_OK(f, hipMemsetAsync(f->counters, 0, sizeof(Counters), f->stream));
     HIP_OK(f, hipStreamSynchronize(f->stream));
+    return SBGI_OK;
+}
+
+// Stage timing: captured ticks record their stage events (the tick key holds the flag,
+// so a change recaptures).
+int32_t sbgi_set_stage_timing(sbgi_field *f, int32_t on) {
+    if (!f) return SBGI_E_INVALID;
+    f->stage_timing = on != 0;
     return SBGI_OK;
 }
 

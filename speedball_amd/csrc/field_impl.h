@@ -154,6 +154,9 @@ struct sbgi_field {
     std::vector<unsigned char> tick_key, scratch_key;
     uint64_t tick_replays = 0;
     bool tick_graph_failed = false, last_replayed = false;
+    // stage events in captured ticks (sbgi_set_stage_timing), and whether the last tick's
+    // stage events are current
+    bool stage_timing = false, last_timed = false;
     // reflection lobe weight tables (desc.reflections != OFF)
     void *rtables = nullptr;
     ReflTables rt{};

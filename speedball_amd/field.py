@@ -107,6 +107,10 @@ class ProbeField:
     def set_counting(self, on: bool):
         self._chk(self.L.sbgi_set_counting(self.h, int(on)))
 
+    def set_stage_timing(self, on: bool):
+        """Record per-stage HIP events in graph-replayed ticks (stats().ms_*); off by default."""
+        self._chk(self.L.sbgi_set_stage_timing(self.h, int(on)))
+
     def nbytes(self, buf: int, cascade: int = 0) -> int:
         return int(self.L.sbgi_buffer_bytes(self.h, buf, cascade))
 

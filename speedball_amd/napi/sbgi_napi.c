@@ -328,6 +328,18 @@ static napi_value js_field_classify(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+/* fieldSetStageTiming(f, on): per-stage events in graph-replayed ticks (stats ms*). */
+static napi_value js_field_set_stage_timing(napi_env env, napi_callback_info info) {
+    napi_value argv[2];
+    if (!get_args(env, info, 2, argv)) return NULL;
+    sbgi_field *f = get_ptr(env, argv[0]);
+    bool on = false;
+    napi_get_value_bool(env, argv[1], &on);
+    int32_t rc = sbgi_set_stage_timing(f, on ? 1 : 0);
+    if (rc < 0) return throw_status(env, rc, sbgi_last_error(f));
+    return NULL;
+}
+
 static napi_value js_field_read(napi_env env, napi_callback_info info) {
     napi_value argv[3];
     if (!get_args(env, info, 3, argv)) return NULL;
@@ -431,6 +443,7 @@ static napi_value js_field_stats(napi_env env, napi_callback_info info) {
     napi_set_named_property(env, o, "msResolve", make_f64(env, s.ms_resolve));
     napi_set_named_property(env, o, "msEmitterVis", make_f64(env, s.ms_emitter_vis));
     napi_set_named_property(env, o, "tickReplayed", make_f64(env, (double)s.tick_replayed));
+    napi_set_named_property(env, o, "stageTimed", make_f64(env, (double)s.stage_timed));
     return o;
 }
 
@@ -591,6 +604,7 @@ static napi_value init(napi_env env, napi_value exports) {
     EXPORT("fieldRefit", js_field_refit);
     EXPORT("fieldSetTransforms", js_field_set_transforms);
     EXPORT("fieldStats", js_field_stats);
+    EXPORT("fieldSetStageTiming", js_field_set_stage_timing);
     EXPORT("sbCreate", js_sb_create);
     EXPORT("sbDestroy", js_sb_destroy);
     EXPORT("sbAddMaterial", js_sb_add_material);

@@ -793,11 +793,11 @@ def main():
         # roofline of record: the shadow walk's L2 read bytes against the measured L2 read
         # ceiling (its nodes come from L2 and L1; HBM carries ~1 % of its reads); VALU issue
         # and HBM beside it
-        "roofline": dict(walk_roofline("k_walk<1,false,true,ShadowSrc,false>", sw_bytes, sw_ms),
+        "roofline": dict(walk_roofline("k_walk<1,false,true,ShadowSrc,false,512>", sw_bytes, sw_ms),
                          counts={"blas_nodes": int(cs.shadow_blas_nodes), "tlas_nodes": int(cs.shadow_tlas_nodes),
                                  "tri_tests": int(cs.shadow_tri_tests), "inst_visits": int(cs.shadow_inst_visits),
                                  "rays": int(cs.shadow_rays)}),
-        "primary_walk_roofline": walk_roofline("k_walk<0,false,true,PrimarySrc,false>", pw_bytes, pw_ms),
+        "primary_walk_roofline": walk_roofline("k_walk<0,false,true,PrimarySrc,false,256>", pw_bytes, pw_ms),
         "trace": {"algorithmic_bytes": bytes_alg, "ms": trace_ms, "gbs": bytes_alg / (trace_ms / 1000.0) / 1e9,
                   "counts": {"blas_nodes": int(cs.blas_nodes), "tlas_nodes": int(cs.tlas_nodes),
                              "tri_tests": int(cs.tri_tests), "inst_visits": int(cs.inst_visits),

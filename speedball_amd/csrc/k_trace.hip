@@ -284,7 +284,6 @@ __global__ void __launch_bounds__(kEmitBlock) k_emit(DevScene s, DevCascade c, s
         const uint64_t m = __ballot(pass);
         if (m == 0ull) continue;
         if (pass) {
-            const LightSample ls = light_gate(s.lights + li * SBGI_LIGHT_STRIDE, h, p, light_scale(s, li));
             const uint32_t idx = pos + (uint32_t)__popcll(m & lt_mask);
             if (idx < q.capacity) {
                 q.rec[idx] = make_float4(h.hit.x, h.hit.y, h.hit.z, __uint_as_float((gid << 6) | li));

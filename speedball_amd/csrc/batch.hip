@@ -130,14 +130,18 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
     BHIP_OK(b, launch_fill_u32(b->d_done, 0u, 2u * n_seg, st));
     bool wt = false;
     for (const sbgi_field *f : b->fields) wt = wt || f->desc.traversal == SBGI_TRAV_WATERTIGHT;
+    // one scene for every segment (the cascades of one field): the walks stage it in LDS
+    const DevScene *shared = n_seg ? &ts[0].s : nullptr;
+    for (uint32_t j = 1; j < n_seg && shared; j++)
+        if (memcmp(&ts[j].s, &ts[0].s, sizeof(DevScene)) != 0) shared = nullptr;
     BHIP_OK(b, launch_walk_batch(0, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done, b->walk_block,
-                                 b->walk_waves, wt, st));
+                                 b->walk_waves, wt, shared, st));
     for (uint32_t j = 0; j < n_seg; j++) {
         const sbgi_tick_params &p = params[b->segs[j].field];
         BHIP_OK(b, launch_emit(ts[j].s, ts[j].c, p, ts[j].dirs, ts[j].meta, ts[j].w.hits, ts[j].w.queue, st));
     }
     BHIP_OK(b, launch_walk_batch(1, b->d_tables, n_seg, b->d_prefix, b->total_rays, b->d_done + n_seg, b->walk_block,
-                                 b->walk_waves, wt, st));
+                                 b->walk_waves, wt, shared, st));
     for (uint32_t j = 0; j < n_seg; j++) {
         const auto &g = b->segs[j];
         sbgi_field *f = b->fields[g.field];

@@ -190,10 +190,13 @@ struct WalkSeg {
     uint32_t *heads;
 };
 
-template <int MODE, int BLOCK, class Src, bool WT = false>
+// LDS: every segment walks the same scene (the cascades of one field), `shared`, whose TLAS
+// and instance records each block stages in its LDS as the single-field walk does.
+template <int MODE, int BLOCK, class Src, bool WT = false, bool LDS = false>
 __global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__restrict__ segs, uint32_t n_seg,
                                                       const uint32_t *__restrict__ prefix, uint32_t total,
-                                                      uint32_t *__restrict__ done) {
+                                                      uint32_t *__restrict__ done, DevScene shared) {
+    if (LDS) stage_tlas_lds(shared);
     // wave-uniform by construction; readfirstlane tells the compiler, so the segment table
     // is read with scalar loads into SGPRs instead of per lane on every step
     const uint32_t waves = gridDim.x * (BLOCK / 64u);
@@ -215,7 +218,7 @@ __global__ void __launch_bounds__(BLOCK) k_walk_batch(const WalkSeg<Src> *__rest
         // scene and source fields on every step
         const DevScene sc = g.s;
         const Src src = g.src;
-        walk_wave<MODE, false, false, WT>(sc, src, n, g.heads, wave_id, cnt);
+        walk_wave<MODE, false, LDS, WT>(sc, src, n, g.heads, wave_id, cnt);
         if ((threadIdx.x & 63u) == 0) __hip_atomic_store(&done[j], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -477,15 +480,31 @@ void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char
 }
 
 hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const uint32_t *prefix, uint32_t total,
-                             uint32_t *done, uint32_t block, uint32_t waves, bool wt, hipStream_t st) {
+                             uint32_t *done, uint32_t block, uint32_t waves, bool wt, const DevScene *shared,
+                             hipStream_t st) {
     if (!n_seg || !total) return hipSuccess;
     const auto *pr = static_cast<const WalkSeg<PrimarySrc> *>(tables);
     const auto *sh = reinterpret_cast<const WalkSeg<ShadowSrc> *>(static_cast<const unsigned char *>(tables) +
                                                                  (size_t)n_seg * sizeof(WalkSeg<PrimarySrc>));
+    DevScene one{};
+    const size_t lds = shared ? trace_lds_bytes(shared->tlas_count, shared->inst_count) : 0;
+    const bool use_lds = shared && !wt && lds <= kTraceLdsBudget;
+    if (use_lds) one = *shared;
     auto go = [&](auto kern, uint32_t bs, auto table) {
-        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, bs, 0, total, waves)), dim3(bs), 0, st, table, n_seg,
-                           prefix, total, done);
+        const size_t l = use_lds ? lds : 0;
+        hipLaunchKernelGGL(kern, dim3(persistent_blocks(kern, bs, l, total, waves)), dim3(bs), l, st, table, n_seg,
+                           prefix, total, done, one);
     };
+    if (use_lds) {
+        if (mode == 0) {
+            if (block == 256) go(k_walk_batch<WALK_CLOSEST, 256, PrimarySrc, false, true>, 256u, pr);
+            else go(k_walk_batch<WALK_CLOSEST, 512, PrimarySrc, false, true>, 512u, pr);
+        } else {
+            if (block == 256) go(k_walk_batch<WALK_ANY, 256, ShadowSrc, false, true>, 256u, sh);
+            else go(k_walk_batch<WALK_ANY, 512, ShadowSrc, false, true>, 512u, sh);
+        }
+        return hipGetLastError();
+    }
     // the watertight variants only when a segment's scene asks for it: the watertight test
     // costs registers that the Moller-Trumbore walks do not pay for
     if (mode == 0 && !wt) {

@@ -76,8 +76,11 @@ void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char
 // segment's shadow queue. prefix[j] = probe rays before segment j (n_seg + 1 entries, on
 // the device) spreads the waves' first segments by work; done[n_seg] flags are cleared by
 // the caller. block: 256 or 512 threads; waves: resident waves per CU (0: occupancy limit).
+// shared: the scene of every segment when they all walk one (the cascades of one field):
+// its TLAS and instance records are then staged in each block's LDS; else null.
 hipError_t launch_walk_batch(int mode, const void *tables, uint32_t n_seg, const uint32_t *prefix, uint32_t total,
-                             uint32_t *done, uint32_t block, uint32_t waves, bool wt, hipStream_t st);
+                             uint32_t *done, uint32_t block, uint32_t waves, bool wt, const DevScene *shared,
+                             hipStream_t st);
 hipError_t launch_emit(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,
                        const float *meta, const float4 *hits, const ShadowQueue &q, hipStream_t st);
 hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float4 *dirs,

@@ -89,6 +89,32 @@ def test_batch_frames_equal_individual_ticks(specs):
             f.close()
 
 
+@pytest.mark.parametrize("block", [512, 256])
+def test_one_field_batch_equals_individual_ticks(specs, block):
+    """A batch of one field (the 3-cascade instanced field): every segment walks the same
+    scene, so the batched walks stage its TLAS and instance records in LDS; the frames must
+    still equal the field's per-cascade ticks bit for bit, eager, captured and replayed."""
+    one = [specs[2]]
+    params = [dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis) for w, *_ in one]
+    ref, bat = make_fields(one), make_fields(one)
+    batch = Batch(bat)
+    batch.set_walk_block(block)
+    try:
+        flags = []
+        for frame in range(4):
+            print(f"case: one-field batch, block {block}, frame {frame}", flush=True)
+            tick_individually(ref, params)
+            batch.tick(params)
+            flags.append(batch.timing()[1])
+            for a, b in zip(state(ref[0]), state(bat[0])):
+                assert np.array_equal(a, b), frame
+        assert flags == [False, False, True, True]
+    finally:
+        batch.close()
+        for f in ref + bat:
+            f.close()
+
+
 def test_batch_autotune_keeps_results(specs):
     """The autotuner's schedule (walk tuning, block size, resident waves), applied to a
     fresh batch, must leave every frame equal to the individually ticked twin fields."""

@@ -646,6 +646,8 @@ def main():
     ap.add_argument("--no-lights", action="store_true", help="skip the clustered-lights and emitters leg")
     ap.add_argument("--no-reflect", action="store_true", help="skip the reflection-lobes leg")
     ap.add_argument("--no-room", action="store_true", help="skip the room leg (16^3 probes x 64 rays)")
+    ap.add_argument("--no-shadow-lists", action="store_true",
+                    help="walk the headline field's shadow rays over the TLAS instead of the light-space lists")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
@@ -664,7 +666,8 @@ def main():
         f"built in {time.time() - t0:.1f}s")
     prec = PRECISIONS[args.precision]
     trav = N.TRAV_MT_ORDERED if args.traversal == "ordered" else N.TRAV_MT_COMPAT
-    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=prec, traversal=trav)
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=prec, traversal=trav,
+                   shadow_lists=not args.no_shadow_lists)
     f.set_scene(w.scene)
     kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
 
@@ -732,6 +735,9 @@ def main():
     ordered = args.traversal == "ordered"
     sw_bytes = shadow_walk_bytes(cs)
     sw_ms = float(np.mean(stage["walk_shadow"]))
+    # the shadow walk over the light-space candidate lists (default) or over the TLAS
+    shadow_kernel = ("k_walk<1,false,true,ShadowSrc,false,512>" if args.no_shadow_lists
+                     else "k_walk<1,false,true,ShadowListSrc,false,512>")
     pw_bytes = walk_bytes(cs.blas_nodes, cs.tlas_nodes, cs.inst_visits, cs.tri_tests)
     pw_ms = float(np.mean(stage["walk_primary"]))
 
@@ -793,7 +799,7 @@ def main():
         # roofline of record: the shadow walk's L2 read bytes against the measured L2 read
         # ceiling (its nodes come from L2 and L1; HBM carries ~1 % of its reads); VALU issue
         # and HBM beside it
-        "roofline": dict(walk_roofline("k_walk<1,false,true,ShadowSrc,false,512>", sw_bytes, sw_ms),
+        "roofline": dict(walk_roofline(shadow_kernel, sw_bytes, sw_ms),
                          counts={"blas_nodes": int(cs.shadow_blas_nodes), "tlas_nodes": int(cs.shadow_tlas_nodes),
                                  "tri_tests": int(cs.shadow_tri_tests), "inst_visits": int(cs.shadow_inst_visits),
                                  "rays": int(cs.shadow_rays)}),

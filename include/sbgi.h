@@ -126,6 +126,15 @@ extern "C" {
                                    ray, same order as RAY_DATA: bit l set when the ray's
                                    hit passed light l's NEE gate and the shadow ray toward
                                    l is unblocked (read)                               */
+/* The shadow walk's light-space candidate lists (no reference counterpart; 0 bytes when
+   the field walks shadow rays over the TLAS). PROJ: f32[16*L] per light {kind (0 point /
+   spot, 1 directional, 2 none), near radius, range cut, 0}, then {position, 0} or {e1,
+   u0}, {e2, v0}, {su, sv, margin, 0}; CELLS: u32[2*L*6146] {first, end} per cell record
+   (6 x 32 x 32 cube-map cells, or a 64 x 64 grid for directional lights, then {0, I} and
+   {0, 0}); LIST: u16[capacity] instance slots, the identity list first (read). */
+#define SBGI_BUF_SHADOW_PROJ 30
+#define SBGI_BUF_SHADOW_CELLS 31
+#define SBGI_BUF_SHADOW_LIST 32
 
 /* ---- field description ---------------------------------------------------- */
 typedef struct sbgi_cascade_desc {
@@ -155,6 +164,10 @@ typedef struct sbgi_field_desc {
                                       every axis. Same radiance as the all-lights loop.  */
 #define SBGI_FIELD_NO_GRAPH 4u     /* every tick launches its kernels eagerly instead of
                                       replaying the tick's captured hipGraph             */
+#define SBGI_FIELD_NO_SHADOW_LISTS 8u /* walk shadow rays over the TLAS instead of the
+                                      light-space candidate lists (no reference
+                                      counterpart: the lists are this build's
+                                      acceleration of traverseAny; same answers)         */
 
 /* Emitter lights (type 3 records, js/spectral_scene.js:815-875) injected as virtual rays
    (js/gi_probes.js:133-134, 2216-2310): at most this many per scene. */

@@ -25,6 +25,7 @@ TRAV_MT_COMPAT, TRAV_MT_ORDERED, TRAV_WATERTIGHT = 0, 1, 2
 FIELD_KEEP_HIT_IDS = 1
 FIELD_CLUSTERED = 2
 FIELD_NO_GRAPH = 4
+FIELD_NO_SHADOW_LISTS = 8
 MAX_EMITTERS = 16
 LIGHTS_PER_CELL = 16
 EMITTER_RETENTION = 0.8
@@ -36,6 +37,7 @@ BUF_EMITTER_VIS, BUF_LIGHT_GRID = 21, 22
 BUF_ROUGH_STATE, BUF_ROUGH_ATLAS, BUF_GLOSSY_STATE, BUF_GLOSSY_WEIGHT, BUF_GLOSSY_ATLAS = 23, 24, 25, 26, 27
 BUF_TLAS_OCT = 28
 BUF_SHADOW_VIS = 29
+BUF_SHADOW_PROJ, BUF_SHADOW_CELLS, BUF_SHADOW_LIST = 30, 31, 32
 REFLECT_OFF, REFLECT_ROUGH, REFLECT_HIGH, REFLECT_ULTRA = 0, 1, 2, 3
 
 # every symbol include/sbgi.h declares

@@ -126,7 +126,19 @@ void free_skins(sbgi_field *f) {
     f->skin_nmax = f->skin_tmax = 0;
 }
 
+void free_shadow_lists(sbgi_field *f) {
+    dfree(f->sl.proj);
+    dfree(f->sl.boxes);
+    dfree(f->sl.rects);
+    dfree(f->sl.offsets);
+    dfree(f->sl.total);
+    dfree(f->sl.cells);
+    dfree(f->sl.list);
+    f->sl.capacity = 0;
+}
+
 void free_scene(sbgi_field *f) {
+    free_shadow_lists(f);
     dfree(f->d_nodes);
     dfree(f->d_tri_index);
     dfree(f->d_tri_mat);
@@ -252,6 +264,58 @@ void invert_affine_rows(const float *rows12, double *world16) {
         world16[12 + r] = -(inv[r * 3 + 0] * m[12] + inv[r * 3 + 1] * m[13] + inv[r * 3 + 2] * m[14]);
 }
 
+// The shadow walk's candidate lists from the current instance boxes and light records
+// (stream-ordered; a build that outgrows the list capacity leaves identity lists, which the
+// next sized build fixes).
+int32_t build_shadow_lists(sbgi_field *f) {
+    if (!f->sl.list) return SBGI_OK;
+    HIP_OK(f, launch_shadow_lists(f->d_lights, f->view.light_count, f->d_inst_pad, f->view.inst_count, f->sl, f->stream));
+    return SBGI_OK;
+}
+
+// Allocate the candidate lists of a new scene and build them with a capacity of twice the
+// entries the scene needs now (one host sync), so that later edits rarely outgrow it. A
+// scene whose build would cost more than kShadowListWork rectangle tests, or whose slots
+// do not fit 16 bits, walks its shadow rays over the TLAS.
+constexpr uint64_t kShadowListWork = 1ull << 28;
+int32_t size_shadow_lists(sbgi_field *f) {
+    free_shadow_lists(f);
+    const sbgi_scene_view &v = f->view;
+    const uint64_t work = (uint64_t)v.light_count * kShadowCells * v.inst_count;
+    if ((f->desc.flags & SBGI_FIELD_NO_SHADOW_LISTS) || v.light_count == 0 || v.inst_count == 0 ||
+        v.inst_count > 65535u || work > kShadowListWork)
+        return SBGI_OK;
+    const size_t nl = v.light_count, ni = v.inst_count;
+    auto alloc_list = [&](size_t entries) {
+        entries = (entries + 3) & ~(size_t)3;  // the walk reads entries four per 64-bit word
+        dfree(f->sl.list);
+        f->sl.capacity = (uint32_t)entries;
+        return dalloc(&f->sl.list, entries * sizeof(uint16_t));
+    };
+    if (dalloc(&f->sl.proj, nl * kShadowProj * sizeof(float4)) != hipSuccess ||
+        dalloc(&f->sl.boxes, (ni + 1) * 6 * sizeof(float)) != hipSuccess ||
+        dalloc(&f->sl.rects, nl * ni * 6 * sizeof(uint32_t)) != hipSuccess ||
+        dalloc(&f->sl.offsets, nl * kShadowCells * sizeof(uint32_t)) != hipSuccess ||
+        dalloc(&f->sl.total, sizeof(uint32_t)) != hipSuccess ||
+        dalloc(&f->sl.cells, nl * kShadowRecs * sizeof(uint2)) != hipSuccess || alloc_list(ni + (1u << 20)) != hipSuccess) {
+        free_shadow_lists(f);
+        return fail(f, SBGI_E_NOMEM, "shadow list allocation");
+    }
+    HIP_OK(f, launch_shadow_lists(f->d_lights, v.light_count, f->d_inst_pad, v.inst_count, f->sl, f->stream));
+    uint32_t total = 0;
+    HIP_OK(f, hipMemcpyAsync(&total, f->sl.total, sizeof(total), hipMemcpyDeviceToHost, f->stream));
+    HIP_OK(f, hipStreamSynchronize(f->stream));
+    const uint64_t want = ni + 1 + 2 * ((uint64_t)total - ni - 1) + 4096;
+    if (want > f->sl.capacity) {
+        if (want >= (1ull << 32) || alloc_list((size_t)want) != hipSuccess) {
+            free_shadow_lists(f);
+            return fail(f, SBGI_E_NOMEM, "shadow list allocation");
+        }
+        HIP_OK(f, launch_shadow_lists(f->d_lights, v.light_count, f->d_inst_pad, v.inst_count, f->sl, f->stream));
+    }
+    return SBGI_OK;
+}
+
 // Instance world bounds from the device node pool and world matrices, then the TLAS: in
 // MT_ORDERED mode the SAH TLAS is rebuilt; in MT_COMPAT mode the reference TLAS is refit
 // bottom-up when refit_compat is set (updateTransforms' refit, js/spectral_scene.js:
@@ -269,7 +333,7 @@ int32_t device_tlas(sbgi_field *f, bool refit_compat) {
                                     f->d_tlas_arrive, f->stream));
         HIP_OK(f, launch_pack_tlas(f->d_mats, v.tlas_base, v.tlas_node_count, f->d_tlas, f->stream));
     }
-    return SBGI_OK;
+    return build_shadow_lists(f);
 }
 
 // Bring the host mirror of the node pool up to date after device refits (before host code
@@ -607,6 +671,7 @@ int32_t sbgi_set_scene(sbgi_field *f, const sbgi_scene_view *s) {
     // reference TLAS of MT_COMPAT mode is used as given.
     f->tlas_sah_count = ordered(f) ? 2 * v.inst_count - 1 : 0;
     if ((rc = device_tlas(f, false)) != SBGI_OK) return rc;
+    if ((rc = size_shadow_lists(f)) != SBGI_OK) return rc;
     HIP_OK(f, hipStreamSynchronize(f->stream));
     if ((rc = upload_walk_nodes(f)) != SBGI_OK) {
         free_scene(f);
@@ -658,7 +723,10 @@ int32_t sbgi_update_ranges(sbgi_field *f, int32_t buf_id, const uint32_t *ranges
             HIP_OK(f, hipMemcpyAsync(static_cast<char *>(dev) + st * esz, static_cast<char *>(host) + st * esz,
                                      cnt * esz, hipMemcpyHostToDevice, f->stream));
     }
-    if (buf_id == SBGI_BUF_LIGHTS) return build_light_grid(f);
+    if (buf_id == SBGI_BUF_LIGHTS) {
+        if ((rc = build_light_grid(f)) != SBGI_OK) return rc;
+        return build_shadow_lists(f);
+    }
     if ((rc = repack_all(f)) != SBGI_OK) return rc;
     if (buf_id == SBGI_BUF_BVH_NODES || buf_id == SBGI_BUF_MATERIALS) {
         if ((rc = sync_nodes(f)) != SBGI_OK) return rc;
@@ -1164,6 +1232,9 @@ size_t sbgi_buffer_bytes(const sbgi_field *f, int32_t buf_id, uint32_t cascade) 
         case SBGI_BUF_EMITTER_VIS:
             return cascade < f->desc.cascade_count ? (size_t)f->casc[cascade].total * SBGI_MAX_EMITTERS * 4 : 0;
         case SBGI_BUF_LIGHT_GRID: return light_grid_cells(f) * (SBGI_LIGHTS_PER_CELL + 1) * 4;
+        case SBGI_BUF_SHADOW_PROJ: return f->sl.list ? (size_t)f->view.light_count * kShadowProj * 16 : 0;
+        case SBGI_BUF_SHADOW_CELLS: return f->sl.list ? (size_t)f->view.light_count * kShadowRecs * 8 : 0;
+        case SBGI_BUF_SHADOW_LIST: return f->sl.list ? (size_t)f->sl.capacity * 2 : 0;
     }
     if (buf_id >= SBGI_BUF_ROUGH_STATE && buf_id <= SBGI_BUF_GLOSSY_ATLAS) {
         if (cascade >= f->desc.cascade_count || !f->casc[cascade].rough) return 0;
@@ -1229,6 +1300,13 @@ int32_t sbgi_read(sbgi_field *f, int32_t buf_id, uint32_t cascade, void *dst, si
               : buf_id == SBGI_BUF_TRI_MATERIAL ? (const void *)f->d_tri_mat
               : buf_id == SBGI_BUF_TLAS_OCT     ? (const void *)f->d_tlas_oct
                                                 : (const void *)f->d_tlas_sah;
+        HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return SBGI_OK;
+    }
+    if (buf_id == SBGI_BUF_SHADOW_PROJ || buf_id == SBGI_BUF_SHADOW_CELLS || buf_id == SBGI_BUF_SHADOW_LIST) {
+        src = buf_id == SBGI_BUF_SHADOW_PROJ    ? (const void *)f->sl.proj
+              : buf_id == SBGI_BUF_SHADOW_CELLS ? (const void *)f->sl.cells
+                                                : (const void *)f->sl.list;
         HIP_OK(f, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         return SBGI_OK;
     }

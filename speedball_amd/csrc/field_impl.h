@@ -133,6 +133,10 @@ struct sbgi_field {
     // whenever the light records change
     uint32_t *d_light_grid = nullptr;
     uint32_t lg_cells[3] = {0, 0, 0};
+    // light-space candidate lists of the shadow walk (k_shadow.hip), rebuilt on the stream
+    // whenever instance boxes or light records change; sl.list is null when the field walks
+    // its shadow rays over the TLAS (SBGI_FIELD_NO_SHADOW_LISTS, or too large a build)
+    ShadowLists sl{};
     // solve scratch
     float4 *rays = nullptr, *dirs = nullptr;
     int2 *hit_ids = nullptr;
@@ -287,6 +291,12 @@ inline DevScene make_scene(const sbgi_field *f) {
     s.tri_index = f->d_tri_index;
     s.vdata = f->d_vdata;
     s.lds_inst = s.tlas_count * 2;
+    if (f->sl.list) {
+        s.sl_proj = f->sl.proj;
+        s.sl_cells = f->sl.cells;
+        s.sl_list = f->sl.list;
+        s.inst_box = f->sl.boxes;
+    }
     return s;
 }
 

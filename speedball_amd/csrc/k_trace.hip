@@ -46,6 +46,7 @@ constexpr uint32_t kWalkBlockSmall = 256;
 
 // Probe rays of this solve, index = k * updated + slot; result hits[i] = {t, tri, inst, 0}.
 struct PrimarySrc {
+    static constexpr bool kList = false;
     DevCascade c;
     float classify;
     const float4 *dirs;
@@ -66,6 +67,7 @@ struct PrimarySrc {
 // (light_vector, and the cascade's hit bias). An unblocked ray sets bit `light` of its
 // ray's mask.
 struct ShadowSrc {
+    static constexpr bool kList = false;
     const float4 *rec;
     unsigned long long *vis;
     float bias;
@@ -83,11 +85,39 @@ struct ShadowSrc {
     }
 };
 
+// The same shadow rays walked over their light's candidate list (k_shadow.hip) instead of
+// the TLAS: the load finds the cell of the ray's origin for its light and walks that cell's
+// list (never empty: a cell that no box reaches lists one box that no ray hits).
+struct ShadowListSrc {
+    static constexpr bool kList = true;
+    const float4 *rec;
+    unsigned long long *vis;
+    float bias;
+    __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
+        const float4 a = rec[i];
+        const uint32_t tag = __float_as_uint(a.w), l = tag & 63u;
+        float3 wi;
+        float dist;
+        light_vector(s.lights + (size_t)l * SBGI_LIGHT_STRIDE, xyz(a), wi, dist);
+        lane_begin<true>(s, r, xyz(a), wi, dist - bias);
+        r.tag = tag;
+        const uint2 ce = s.sl_cells[(size_t)l * kShadowRecs + shadow_cell(s.sl_proj + (size_t)l * kShadowProj, xyz(a))];
+        r.cur = ce.x;
+        r.end = ce.y;
+        r.lend = ce.y;
+        r.lquad = reinterpret_cast<const uint2 *>(s.sl_list)[ce.x >> 2];
+    }
+    __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
+        if (!r.blocked) atomicOr(&vis[r.tag >> 6], 1ull << (r.tag & 63u));
+    }
+};
+
 // Emitter visibility (emitterVisKernel, js/gi_probes.js:2216-2260): ray i = slot * E + e
 // tests the probe's view of a hashed point on emitter e; the store blends the 0/1 sample
 // into the retained visibility, vis = mix(sample, vis, 0.8). A probe inside the emitter's
 // bounding sphere walks nothing (its extent is NaN, so no box test passes) and samples 0.
 struct EmitterSrc {
+    static constexpr bool kList = false;
     DevCascade c;
     float classify, seed, bias;
     uint32_t ebase, ecount;
@@ -117,6 +147,7 @@ struct EmitterSrc {
 // direction k (classifyRayDir, :1707-1712: rayDir with 32 rays and no jitter); result
 // hits[i] = {t, tri, inst, 0}.
 struct ClassifySrc {
+    static constexpr bool kList = false;
     DevCascade c;
     const float4 *dirs;
     float4 *hits;
@@ -133,6 +164,7 @@ struct ClassifySrc {
 // within tmax, 2 closest hit reported as blocked when before tmax.
 // out[i] = {t, tri, inst, blocked}.
 struct QuerySrc {
+    static constexpr bool kList = false;
     const float4 *in;
     float4 *out;
     __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
@@ -158,7 +190,10 @@ template <int MODE, bool COUNT, bool LDS, class Src, bool WT = false, int BLOCK 
 __global__ void __launch_bounds__(BLOCK) k_walk(DevScene s, Src src, const uint32_t *__restrict__ n_dev,
                                                  uint32_t n_host, uint32_t *heads,
                                                  unsigned long long *__restrict__ counters) {
-    if (LDS) stage_tlas_lds(s);
+    if (LDS) {
+        if constexpr (Src::kList) stage_list_lds(s);
+        else stage_tlas_lds(s);
+    }
     const uint32_t n = n_dev ? min(*n_dev, n_host) : n_host;
     TravCounters<COUNT> cnt;
     walk_wave<MODE, COUNT, LDS, WT>(s, src, n, heads, blockIdx.x * (BLOCK / 64u) + (threadIdx.x >> 6), cnt);
@@ -571,12 +606,17 @@ hipError_t launch_raydirs(float4 *dirs, uint32_t rays, float jitter, hipStream_t
 template <int MODE, class Src>
 void launch_walk(const DevScene &scene, const Src &src, const uint32_t *n_dev, uint32_t n_host, uint32_t *heads,
                  unsigned long long *cnt, size_t lds, hipStream_t st) {
-    const bool use_lds = lds <= kTraceLdsBudget;
     // only the closest-hit walk takes the near-first TLAS copies; it reads them from global
     // memory and stages only the instance records, in blocks of 4 waves (its VGPRs allow 7
     // waves per SIMD, which blocks of 8 waves would round down to 6)
     DevScene s = scene;
     if (MODE != WALK_CLOSEST) s.tlas_oct = nullptr;
+    if constexpr (Src::kList) {  // LDS: instance records, then the boxes the list steps test
+        s.lds_inst = 0;
+        s.lds_box = s.inst_count * 4;
+        lds = ((size_t)s.inst_count * 6 + 2) * sizeof(uint4);
+    }
+    const bool use_lds = lds <= kTraceLdsBudget;
     if (MODE == WALK_CLOSEST && s.tlas_oct && use_lds && !s.watertight) {
         s.lds_inst = 0;
         const size_t l = (size_t)s.inst_count * 64;
@@ -628,8 +668,12 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
     if ((e = mark(2)) != hipSuccess) return e;
     // the queue length is only known on the device: the kernel reads it, bounded by the capacity
     const float bias = hit_bias(c.cell_min, p.dbg_trace_bias, p.dbg_ray_eps);
-    launch_walk<WALK_ANY>(s, ShadowSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity, ctr + trace_heads(1),
-                          counters ? counters->shadow_walk : nullptr, lds, st);
+    if (s.sl_cells)
+        launch_walk<WALK_ANY>(s, ShadowListSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity,
+                              ctr + trace_heads(1), counters ? counters->shadow_walk : nullptr, lds, st);
+    else
+        launch_walk<WALK_ANY>(s, ShadowSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity,
+                              ctr + trace_heads(1), counters ? counters->shadow_walk : nullptr, lds, st);
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,
                        prev_atlas, prev_atlas_f32, w.hits, w.vis, rays, hit_ids);

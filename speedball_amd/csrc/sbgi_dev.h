@@ -71,7 +71,72 @@ struct DevScene {
     uint32_t watertight;
     const uint32_t *__restrict__ tri_index;
     const float *__restrict__ vdata;
+    // light-space candidate lists of the shadow-ray walk (k_shadow.hip; null: the shadow
+    // walk walks the TLAS): per light kShadowProj float4 of projection, kShadowRecs cell
+    // records {first, end} into sl_list (instance slots), and the boxes the list steps test
+    // in place of TLAS nodes (6 floats: the padded world box of every slot, then at slot =
+    // inst_count a NaN box that no ray hits, the one entry of a cell no box reaches)
+    const float4 *__restrict__ sl_proj;
+    const uint2 *__restrict__ sl_cells;
+    const uint16_t *__restrict__ sl_list;
+    const float *__restrict__ inst_box;
+    // LDS walks of the lists: uint4 offset of the boxes (2 per slot, after the records)
+    uint32_t lds_box;
 };
+
+// Light-space candidate lists. A shadow ray runs from a hit toward its light, so every
+// point of it lies in the cone of directions from the light that contains the hit (point
+// and spot lights), or on the line through the hit along the light direction (directional
+// lights). Cells of those directions (a cube map of kShadowFace^2 cells per face around the
+// light) or of that plane (a kShadowDirGrid^2 grid across the light direction) list, in
+// slot order, the instances whose padded world box reaches the cell; the walk tests that
+// list instead of the TLAS.
+constexpr uint32_t kShadowFace = 32;
+constexpr uint32_t kShadowDirGrid = 64;
+constexpr uint32_t kShadowCells = 6 * kShadowFace * kShadowFace;
+static_assert(kShadowDirGrid * kShadowDirGrid <= kShadowCells, "directional grid exceeds the cell records");
+constexpr uint32_t kShadowAll = kShadowCells;       // record {0, instances}: every slot
+constexpr uint32_t kShadowNone = kShadowCells + 1;  // record {instances, instances + 1}: the NaN box
+constexpr uint32_t kShadowRecs = kShadowCells + 2;  // cell records per light
+constexpr uint32_t kShadowProj = 4;                 // float4 of projection per light
+// projection kinds (x of a light's first float4)
+constexpr float kShadowKindPoint = 0.f, kShadowKindDir = 1.f, kShadowKindAll = 2.f;
+
+// Cell record index of a shadow ray from o for the light whose projection is pj:
+//   point / spot {kind, ...}, {position, 0}: the cube-map cell of o - position (face by the
+//     largest |component|, +x -x +y -y +z -z; u, v = the two others over it, in [-1, 1]);
+//   directional {kind, ...}, {e1, u0}, {e2, v0}, {su, sv, ...}: the grid cell of
+//     ((o.e1 - u0) su, (o.e2 - v0) sv), or kShadowNone off the grid (no box reaches it);
+//   anything else: kShadowAll.
+// The lists are built with margins that cover this function's rounding.
+__device__ __forceinline__ uint32_t shadow_cell(const float4 *__restrict__ pj, float3 o) {
+    const float kind = pj[0].x;
+    if (kind == kShadowKindPoint) {
+        const float4 lp = pj[1];
+        const float vx = o.x - lp.x, vy = o.y - lp.y, vz = o.z - lp.z;
+        const float ax = fabsf(vx), ay = fabsf(vy), az = fabsf(vz);
+        const bool fx = ax >= ay && ax >= az, fy = !fx && ay >= az;
+        const float m = fx ? ax : (fy ? ay : az), major = fx ? vx : (fy ? vy : vz);
+        const float a = fx ? vy : vx, b = fx || fy ? vz : vy;
+        const uint32_t face = (fx ? 0u : (fy ? 2u : 4u)) + (major < 0.f ? 1u : 0u);
+        // (a / m + 1) N / 2 with the hardware reciprocal; m = 0 gives NaN, which the clamps
+        // send to cell 0
+        const float h = 0.5f * (float)kShadowFace, sc = h * __builtin_amdgcn_rcpf(m);
+        const float top = (float)(kShadowFace - 1);
+        const uint32_t iu = (uint32_t)fminf(fmaxf(a * sc + h, 0.f), top);
+        const uint32_t iv = (uint32_t)fminf(fmaxf(b * sc + h, 0.f), top);
+        return (face * kShadowFace + iv) * kShadowFace + iu;
+    }
+    if (kind == kShadowKindDir) {
+        const float4 e1 = pj[1], e2 = pj[2], g = pj[3];
+        const float fu = (o.x * e1.x + o.y * e1.y + o.z * e1.z - e1.w) * g.x;
+        const float fv = (o.x * e2.x + o.y * e2.y + o.z * e2.z - e2.w) * g.y;
+        const float n = (float)kShadowDirGrid;
+        if (!(fu >= 0.f && fu < n && fv >= 0.f && fv < n)) return kShadowNone;
+        return (uint32_t)fv * kShadowDirGrid + (uint32_t)fu;
+    }
+    return kShadowAll;
+}
 
 // albedo entry of (instance slot, pool triangle) in the differentiable mode
 __device__ __forceinline__ uint32_t albedo_index(const DevScene &s, uint32_t inst, uint32_t tri) {

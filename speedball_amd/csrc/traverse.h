@@ -79,6 +79,40 @@ __device__ __forceinline__ uint32_t octant_of(float3 rd, uint32_t octants) {
     return (rd.x < 0.f ? 1u : 0u) | (rd.y < 0.f ? 2u : 0u) | (rd.z < 0.f ? 4u : 0u);
 }
 
+// Box of instance slot `slot` as a one-instance TLAS leaf of a candidate list (the
+// shadow walk's lists, k_shadow.hip): {min.xyz, -}, {max.xyz, -} from the block's LDS copy
+// (after the instance records) or from the list boxes (the padded world box of every slot,
+// then at slot = instances a NaN box, which no ray hits).
+template <bool LDS>
+__device__ __forceinline__ void list_box(const DevScene &s, uint32_t slot, uint4 &a, uint4 &b) {
+    if (LDS) {
+        a = g_lds[(size_t)s.lds_box + (size_t)slot * 2];
+        b = g_lds[(size_t)s.lds_box + (size_t)slot * 2 + 1];
+    } else {
+        const float *p = s.inst_box + (size_t)slot * 6;
+        a = make_uint4(__float_as_uint(p[0]), __float_as_uint(p[1]), __float_as_uint(p[2]), 0u);
+        b = make_uint4(__float_as_uint(p[3]), __float_as_uint(p[4]), __float_as_uint(p[5]), 0u);
+    }
+}
+
+// Copy the instance records and the list boxes (one per slot, then the never-hit box)
+// into the block's LDS (walks of candidate lists): records at lds_inst, boxes at lds_box.
+__device__ __forceinline__ void stage_list_lds(const DevScene &s) {
+    const uint32_t ni = s.inst_count * 4, nb = (s.inst_count + 1) * 2;
+    for (uint32_t i = threadIdx.x; i < ni + nb; i += blockDim.x) {
+        if (i < ni) {
+            const float4 v = s.inst[i];
+            g_lds[s.lds_inst + i] = make_uint4(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z),
+                                               __float_as_uint(v.w));
+        } else {
+            const uint32_t j = i - ni, slot = j >> 1, hi = j & 1u;
+            const float *p = s.inst_box + (size_t)slot * 6 + hi * 3;
+            g_lds[s.lds_box + j] = make_uint4(__float_as_uint(p[0]), __float_as_uint(p[1]), __float_as_uint(p[2]), 0u);
+        }
+    }
+    __syncthreads();
+}
+
 // Copy the TLAS nodes (unless lds_inst is 0) and the instance records into the block's
 // LDS (LDS=true walks).
 __device__ __forceinline__ void stage_tlas_lds(const DevScene &s) {

@@ -35,6 +35,9 @@ struct LaneRay {
     float3 noid;          // -o * id of the current level (the any-hit walk's fused slab test)
     float tmax;           // closest: best t so far; any: the shadow ray's extent
     uint32_t cur, end;    // node cursor and end of the current level
+    uint32_t lend;        // end of the top level: TLAS nodes, or the ray's candidate list
+    uint2 lquad;          // candidate lists: the 64-bit word of list entries holding entry
+                          // `cur` (entries 4j .. 4j + 3, lowest first)
     uint32_t tres;        // TLAS node to resume at after the pending instances
     // pending ranges as leaf payload words, (count << 24) | offset: the instance slots of
     // the current TLAS leaf and the triangles of the current BLAS leaf still to take; a
@@ -75,6 +78,7 @@ __device__ __forceinline__ void lane_begin(const DevScene &s, LaneRay &r, float3
     r.tmax = tmax;
     r.cur = 0u;
     r.end = s.tlas_count;
+    r.lend = s.tlas_count;
     r.tres = 0u;
     r.ipend = r.tpend = 0u;
     r.base = s.tlas_oct ? octant_of(rd, 8u) * s.tlas_count : 0u;
@@ -106,10 +110,22 @@ enum : int { WALK_CLOSEST = 0, WALK_ANY = 1, WALK_MIXED = 2 };
 // face of its box is never culled.
 constexpr float kRobustFar = 1.0000007152557373f;
 
-template <bool COUNT, bool LDS, bool FMA, bool WT = false, bool TOCT = false>
+// LIST: the top level is the ray's candidate list (k_shadow.hip): entry `cur` names an
+// instance slot, tested as a one-instance leaf whose miss link is the next entry.
+template <bool COUNT, bool LDS, bool FMA, bool WT = false, bool TOCT = false, bool LIST = false>
 __device__ __forceinline__ void node_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     uint4 a, b;
-    if (r.level || (TOCT && s.tlas_oct)) {
+    if (LIST && !r.level) {
+        // entries are read four per 64-bit word: the step of a word's last entry loads the
+        // next word, so a list costs a quarter as many reads, each issued a step ahead
+        const uint32_t q = r.cur & 3u, w = (q & 2u) ? r.lquad.y : r.lquad.x;
+        const uint32_t slot = (q & 1u) ? (w >> 16) : (w & 0xFFFFu);
+        if (q == 3u) r.lquad = reinterpret_cast<const uint2 *>(s.sl_list)[min(r.cur + 1u, r.lend - 1u) >> 2];
+        list_box<LDS>(s, slot, a, b);
+        a.w = r.cur + 1u;
+        b.w = (1u << 24) | slot;
+        cnt.add_tlas();
+    } else if (r.level || (TOCT && s.tlas_oct)) {
         // BLAS copy, or (closest-hit walk) the near-first TLAS copy of the ray's octant
         const uint4 *pool = r.level ? s.nodes : s.tlas_oct;
         const uint4 *n = reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(pool) +
@@ -191,7 +207,7 @@ __device__ __forceinline__ float3 sel3(bool c, float3 a, float3 b) {
     return f3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z);
 }
 
-template <bool COUNT, bool LDS, bool RCP, bool TOCT = false>
+template <bool COUNT, bool LDS, bool RCP, bool TOCT = false, bool LIST = false>
 __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCounters<COUNT> &cnt) {
     const bool enter = (r.ipend >> 24) != 0u;
     const uint32_t slot = enter ? r.ipend & 0x00FFFFFFu : 0u;
@@ -207,15 +223,16 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
     r.id = sel3(enter, lid, r.wid);
     r.noid = f3(-(r.o.x * r.id.x), -(r.o.y * r.id.y), -(r.o.z * r.id.z));
     r.dsign = enter ? r3.z : r.dsign;
+    const uint32_t top_end = LIST ? r.lend : s.tlas_count;
     r.cur = enter ? c0 : r.tres;
-    r.end = enter ? c1 : s.tlas_count;
+    r.end = enter ? c1 : top_end;
     // back on the TLAS: the near-first copy of the world ray's octant (closest-hit walk)
     const uint32_t tbase = TOCT && s.tlas_oct ? octant_of(r.wd, 8u) * s.tlas_count : r.base;
     r.base = enter ? octant_of(ld, s.blas_octants) * s.node_count : tbase;
     r.inst = enter ? slot : r.inst;
     r.level = enter ? 1u : 0u;
     // an empty range (or an inactive slot): on to the next slot
-    r.state = enter ? (c0 < c1 ? ST_NODE : ST_INST) : (r.tres < s.tlas_count ? ST_NODE : ST_DONE);
+    r.state = enter ? (c0 < c1 ? ST_NODE : ST_INST) : (r.tres < top_end ? ST_NODE : ST_DONE);
     r.ipend = enter ? r.ipend + 1u - (1u << 24) : r.ipend;
 }
 
@@ -230,13 +247,16 @@ __device__ __forceinline__ void inst_step(const DevScene &s, LaneRay &r, TravCou
 // (128 bytes) apart, and a wave takes fetch_chunk rays per atomic. A wave starts on slice
 // (wave id % kFetchHeads) and moves to the next slice when its slice is drained.
 
-// Ray sources of walk_wave: load(scene, index, lane ray), store(index, lane ray).
+// Ray sources of walk_wave: load(scene, index, lane ray), store(index, lane ray), and
+// kList (the top level is the ray's candidate list, which is never empty: a ray that can
+// reach no instance gets a list of one box that no ray hits).
 // walk_wave runs one wave until all n rays of the source are taken and walked; `heads`
 // are the source's kFetchHeads fetch counters (zero before the launch), shared by every
 // wave of the grid.
 template <int MODE, bool COUNT, bool LDS, bool WT, class Src>
 __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uint32_t n, uint32_t *heads,
                                           uint32_t wave_id, TravCounters<COUNT> &cnt) {
+    constexpr bool LIST = Src::kList;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = lane ? (~0ull >> (64u - lane)) : 0ull;
     LaneRay r;
@@ -260,6 +280,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
             const uint32_t want = (uint32_t)__popcll(done), rank = (uint32_t)__popcll(done & lt);
             const int leader = __ffsll((unsigned long long)done) - 1;
             uint32_t got = 0;
+            bool take_ray = false;  // this lane is given ray idx in this refill
             while (got < want && !drained) {
                 if (cnext >= cend) {
                     const uint32_t s0 = (uint32_t)(((uint64_t)n * h) / kFetchHeads);
@@ -278,11 +299,15 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
                 const uint32_t take = min(want - got, cend - cnext);
                 if (r.state == ST_DONE && rank >= got && rank < got + take) {
                     idx = cnext + (rank - got);
-                    src.load(s, idx, r);
-                    have = true;
+                    take_ray = true;
                 }
                 cnext += take;
                 got += take;
+            }
+            // one load after the fetch loop, so none of its counters is live across it
+            if (take_ray) {
+                src.load(s, idx, r);
+                have = true;
             }
             if (r.state == ST_DONE) r.state = ST_IDLE;
             live = __ballot(r.state < ST_DONE);
@@ -290,7 +315,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         if (!live) break;
         if (__ballot(r.state == ST_NODE)) {
             if (COUNT && lane == 0) cnt.node_w++;
-            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY, WT, MODE == WALK_CLOSEST>(s, r, cnt);
+            if (r.state == ST_NODE) node_step<COUNT, LDS, MODE == WALK_ANY, WT, MODE == WALK_CLOSEST, LIST>(s, r, cnt);
         }
         uint64_t mn = __ballot(r.state == ST_NODE);
         const uint64_t mt = __ballot(r.state == ST_TRI);
@@ -302,7 +327,7 @@ __device__ __forceinline__ void walk_wave(const DevScene &s, const Src &src, uin
         const uint64_t mi = __ballot(r.state == ST_INST);
         if (mi && __popcll(mi) * inst_ratio >= __popcll(mn)) {
             if (COUNT && lane == 0) cnt.inst_w++;
-            if (r.state == ST_INST) inst_step<COUNT, LDS, MODE == WALK_ANY, MODE == WALK_CLOSEST>(s, r, cnt);
+            if (r.state == ST_INST) inst_step<COUNT, LDS, MODE == WALK_ANY, MODE == WALK_CLOSEST, LIST>(s, r, cnt);
         }
     }
 }

@@ -32,7 +32,7 @@ class ProbeField:
     def __init__(self, res, grid_min, grid_size, rays: int = 64, depth_oct: int = 6,
                  precision: int = N.BLEND_FP32, keep_hit_ids: bool = False, device: int = 0, cascades=None,
                  traversal: int = N.TRAV_MT_COMPAT, clustered: bool = False, reflections: int = N.REFLECT_OFF,
-                 graph: bool = True):
+                 graph: bool = True, shadow_lists: bool = True):
         self.L = N.lib()
         d = N.FieldDesc()
         d.device = device
@@ -43,7 +43,7 @@ class ProbeField:
         d.blend_precision = precision
         d.traversal = traversal
         d.flags = ((N.FIELD_KEEP_HIT_IDS if keep_hit_ids else 0) | (N.FIELD_CLUSTERED if clustered else 0) |
-                   (0 if graph else N.FIELD_NO_GRAPH))
+                   (0 if graph else N.FIELD_NO_GRAPH) | (0 if shadow_lists else N.FIELD_NO_SHADOW_LISTS))
         d.reflections = reflections
         for i, (gm, gs, rs) in enumerate(casc):
             for a in range(3):

@@ -1,0 +1,208 @@
+"""The shadow walk's light-space candidate lists (run on an MI355X with -m gpu).
+
+The lists replace the TLAS walk of NEE shadow rays (traverseAny, js/spectral_traverse.js:
+480-569): per light, a cube map of cells around a point or spot light (a grid across the
+direction of a directional light) lists the instances whose padded box reaches the cell,
+and a shadow ray walks its cell's list. Two properties are checked:
+
+- conservative: for segments from random origins toward every light (within a ranged
+  light's range), every instance whose world box the segment meets in FP64 is in the list
+  of the segment's cell, computed here by a float32 restatement of the walk's cell lookup;
+  the lists are also selective (far shorter than the instance count);
+- same answers: the per-ray light visibility masks of a field that walks the lists equal,
+  bit for bit, those of the same field walking the TLAS, on the instanced workload (also
+  after instance edits, which rebuild the lists on the device), the room workload
+  (directional light, one instance) and seeded random scenes with mirrored and inactive
+  instances and point, ranged, spot and directional lights.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from speedball_amd import _native as N
+from speedball_amd import scenegen as sg
+from speedball_amd.field import ProbeField
+from tests.test_gpu_random_scenes import random_workload
+
+pytestmark = pytest.mark.gpu
+
+FACE = 32        # cube-map cells per face edge
+DIR_GRID = 64    # directional grid cells per axis
+CELLS = 6 * FACE * FACE
+RECS = CELLS + 2
+
+
+def host_cell(pj, o):
+    """Cell record index of shadow rays from origins o (float32 [n, 3]) for a light whose
+    projection record is pj (float32 [4, 4]), restating the walk's lookup in float32."""
+    f32 = np.float32
+    kind = pj[0, 0]
+    if kind == 0.0:
+        v = (o - pj[1, :3]).astype(f32)
+        a = np.abs(v)
+        fx = (a[:, 0] >= a[:, 1]) & (a[:, 0] >= a[:, 2])
+        fy = ~fx & (a[:, 1] >= a[:, 2])
+        m = np.where(fx, a[:, 0], np.where(fy, a[:, 1], a[:, 2]))
+        major = np.where(fx, v[:, 0], np.where(fy, v[:, 1], v[:, 2]))
+        aa = np.where(fx, v[:, 1], v[:, 0])
+        bb = np.where(fx | fy, v[:, 2], v[:, 1])
+        face = np.where(fx, 0, np.where(fy, 2, 4)) + (major < 0)
+        h = f32(FACE / 2)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sc = h / m
+            iu = np.nan_to_num(np.clip(aa * sc + h, 0, FACE - 1), nan=0.0).astype(np.int64)
+            iv = np.nan_to_num(np.clip(bb * sc + h, 0, FACE - 1), nan=0.0).astype(np.int64)
+        return (face * FACE + iv) * FACE + iu
+    if kind == 1.0:
+        fu = ((o @ pj[1, :3]).astype(f32) - pj[1, 3]) * pj[3, 0]
+        fv = ((o @ pj[2, :3]).astype(f32) - pj[2, 3]) * pj[3, 1]
+        inside = (fu >= 0) & (fu < DIR_GRID) & (fv >= 0) & (fv < DIR_GRID)
+        c = np.clip(fv, 0, DIR_GRID - 1).astype(np.int64) * DIR_GRID + np.clip(fu, 0, DIR_GRID - 1).astype(np.int64)
+        return np.where(inside, c, CELLS + 1)
+    return np.full(len(o), CELLS, np.int64)
+
+
+def segment_meets_boxes(p, q, boxes):
+    """[n_boxes] bool: the segment p -> q (float64) meets each box (slab test, closed)."""
+    d = q - p
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / d
+        t0 = (boxes[:, :3] - p) * inv
+        t1 = (boxes[:, 3:] - p) * inv
+    lo = np.where(np.isnan(t0), -np.inf, np.minimum(t0, t1))
+    hi = np.where(np.isnan(t0), np.inf, np.maximum(t0, t1))
+    # an axis along which the segment does not move: inside the slab or not at all
+    flat = d == 0
+    inside = (p >= boxes[:, :3]) & (p <= boxes[:, 3:])
+    lo = np.where(flat, np.where(inside, -np.inf, np.inf), lo)
+    hi = np.where(flat, np.where(inside, np.inf, -np.inf), hi)
+    tn = np.maximum(lo.max(1), 0.0)
+    tf = np.minimum(hi.min(1), 1.0)
+    return tn <= tf
+
+
+def check_lists(f, scene, rng, n_origins=3000):
+    lights = scene.lights.view(np.float32).reshape(-1, 17)
+    n_inst = scene.inst_count
+    proj = f.read(N.BUF_SHADOW_PROJ, np.float32).reshape(-1, 4, 4)
+    cells = f.read(N.BUF_SHADOW_CELLS, np.uint32).reshape(-1, RECS, 2)
+    lst = f.read(N.BUF_SHADOW_LIST, np.uint16)
+    boxes = f.read(N.BUF_INST_BOUNDS, np.float32).reshape(-1, 6).astype(np.float64)
+    assert proj.shape[0] == len(lights) and boxes.shape[0] == n_inst
+    lo, hi = boxes[:, :3].min(0), boxes[:, 3:].max(0)
+    span = hi - lo
+    checked, lengths = 0, []
+    for li, L in enumerate(lights):
+        kind = proj[li, 0, 0]
+        if L[0] > 2.5:
+            assert kind == 2.0
+            continue
+        # origins: uniform over the scene, and on the faces of random instance boxes
+        u = rng.uniform(-0.05, 1.05, (n_origins, 3)) * span + lo
+        bi = rng.integers(0, n_inst, n_origins)
+        on = boxes[bi, :3] + rng.uniform(0, 1, (n_origins, 3)) * (boxes[bi, 3:] - boxes[bi, :3])
+        ax = rng.integers(0, 3, n_origins)
+        side = rng.integers(0, 2, n_origins)
+        on[np.arange(n_origins), ax] = np.where(side == 1, boxes[bi, 3 + ax], boxes[bi, ax])
+        origins = np.concatenate([u, on]).astype(np.float32)
+        if kind == 0.0:
+            pos = L[1:4].astype(np.float64)
+            ends = np.repeat(pos[None], len(origins), 0)
+            if L[10] > 0:  # ranged: only origins within range queue shadow rays
+                keep = np.linalg.norm(origins - pos, axis=1) < L[10]
+                origins, ends = origins[keep], ends[keep]
+        else:
+            w = -L[4:7].astype(np.float64)
+            w /= np.linalg.norm(w)
+            ends = origins.astype(np.float64) + 1e4 * w
+        c = host_cell(proj[li], origins)
+        for j in range(len(origins)):
+            first, end = cells[li, c[j]]
+            listed = set(lst[first:end].tolist())
+            lengths.append(end - first)
+            need = np.nonzero(segment_meets_boxes(origins[j].astype(np.float64), ends[j], boxes))[0]
+            missing = [int(i) for i in need if int(i) not in listed]
+            assert not missing, (f"light {li} (type {L[0]:.0f}) origin {origins[j]} cell {c[j]}: "
+                                 f"instances {missing} met but not listed")
+            checked += 1
+    return checked, float(np.mean(lengths)) if lengths else 0.0
+
+
+def test_lists_hold_every_reachable_instance():
+    rng = np.random.default_rng(7)
+    w = sg.instanced(res=(8, 4, 6), rays=64)
+    print(f"case: instanced scene, {w.scene.inst_count} instances, {len(w.scene.lights) // 17} lights", flush=True)
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, traversal=N.TRAV_MT_ORDERED)
+    try:
+        f.set_scene(w.scene)
+        n, mean_len = check_lists(f, w.scene, rng, 1500)
+    finally:
+        f.close()
+    print(f"instanced: {n} segments checked, mean list length {mean_len:.2f} of {w.scene.inst_count} instances")
+    assert n > 10000
+    assert mean_len < w.scene.inst_count / 8  # selective, not the every-instance fallback
+    for seed in (11, 59, 73):
+        w = random_workload(seed)
+        types = sorted(set(int(t) for t in w.scene.lights.view(np.float32).reshape(-1, 17)[:, 0]))
+        print(f"case: random scene seed {seed}, light types {types}", flush=True)
+        f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct)
+        try:
+            f.set_scene(w.scene)
+            n, mean_len = check_lists(f, w.scene, rng, 800)
+        finally:
+            f.close()
+        print(f"random {seed}: {n} segments checked, mean list length {mean_len:.2f} of {w.scene.inst_count}")
+        assert n > 1000
+
+
+def _masks(w, lists, traversal, edit=None):
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, traversal=traversal,
+                   shadow_lists=lists)
+    try:
+        f.set_scene(w.scene)
+        assert (f.nbytes(N.BUF_SHADOW_LIST) > 0) == lists
+        out = []
+        f.tick(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+        out.append(f.read(N.BUF_SHADOW_VIS, np.uint64))
+        if edit is not None:
+            edit(f)
+            f.tick(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+            out.append(f.read(N.BUF_SHADOW_VIS, np.uint64))
+        return out
+    finally:
+        f.close()
+
+
+def _move_instances(w):
+    rng = np.random.default_rng(3)
+    mats = w.scene.materials.view(np.float32)
+    slots = rng.choice(w.scene.inst_count, 32, replace=False)
+    worlds = []
+    for s in slots:
+        rec = mats[w.scene.inst_base + 28 * s: w.scene.inst_base + 28 * s + 12].reshape(3, 4).astype(np.float64)
+        M = np.linalg.inv(np.vstack([rec, [0, 0, 0, 1]]))
+        M[:3, 3] += rng.uniform(-3, 3, 3)
+        worlds.append(M)
+    return lambda f: f.set_instance_transforms(slots, worlds)
+
+
+@pytest.mark.parametrize("case", ["instanced", "room", "random11", "random37", "random73"])
+def test_lists_give_the_tlas_walks_masks(case):
+    if case == "instanced":
+        w = sg.instanced(res=(16, 8, 12), rays=128)
+        edit = _move_instances(w)
+    elif case == "room":
+        w, edit = sg.room_boxes(res=(12, 12, 12), rays=64), None
+    else:
+        w, edit = random_workload(int(case[6:])), None
+    for trav in (N.TRAV_MT_ORDERED, N.TRAV_MT_COMPAT):
+        print(f"case: {case}, traversal {trav}", flush=True)
+        a = _masks(w, True, trav, edit)
+        b = _masks(w, False, trav, edit)
+        for i, (x, y) in enumerate(zip(a, b)):
+            bits = int(np.unpackbits((x ^ y).view(np.uint8)).sum())
+            ones = int(np.unpackbits(y.view(np.uint8)).sum())
+            print(f"{case} traversal {trav} update {i}: {ones} unblocked shadow rays, {bits} bits differ")
+            assert ones > 0
+            assert bits == 0

@@ -129,9 +129,10 @@ extern "C" {
 /* The shadow walk's light-space candidate lists (no reference counterpart; 0 bytes when
    the field walks shadow rays over the TLAS). PROJ: f32[16*L] per light {kind (0 point /
    spot, 1 directional, 2 none), near radius, range cut, 0}, then {position, 0} or {e1,
-   u0}, {e2, v0}, {su, sv, margin, 0}; CELLS: u32[2*L*6146] {first, end} per cell record
-   (6 x 32 x 32 cube-map cells, or a 64 x 64 grid for directional lights, then {0, I} and
-   {0, 0}); LIST: u16[capacity] instance slots, the identity list first (read). */
+   u0}, {e2, v0}, {su, sv, margin, 0}; CELLS: u32[2*L*1538] {first, end} per cell record
+   (6 x 16 x 16 cube-map cells, or a 32 x 32 grid for directional lights, then {0, I} and
+   {I, I+1}); LIST: u16[capacity] instance slots: the identity list, then entry I (the
+   never-hit box of cells no instance reaches), then the cells' lists (read). */
 #define SBGI_BUF_SHADOW_PROJ 30
 #define SBGI_BUF_SHADOW_CELLS 31
 #define SBGI_BUF_SHADOW_LIST 32

@@ -131,6 +131,7 @@ void free_shadow_lists(sbgi_field *f) {
     dfree(f->sl.boxes);
     dfree(f->sl.rects);
     dfree(f->sl.offsets);
+    dfree(f->sl.bsum);
     dfree(f->sl.total);
     dfree(f->sl.cells);
     dfree(f->sl.list);
@@ -296,6 +297,7 @@ int32_t size_shadow_lists(sbgi_field *f) {
         dalloc(&f->sl.boxes, (ni + 1) * 6 * sizeof(float)) != hipSuccess ||
         dalloc(&f->sl.rects, nl * ni * 6 * sizeof(uint32_t)) != hipSuccess ||
         dalloc(&f->sl.offsets, nl * kShadowCells * sizeof(uint32_t)) != hipSuccess ||
+        dalloc(&f->sl.bsum, shadow_list_blocks((uint32_t)nl) * sizeof(uint32_t)) != hipSuccess ||
         dalloc(&f->sl.total, sizeof(uint32_t)) != hipSuccess ||
         dalloc(&f->sl.cells, nl * kShadowRecs * sizeof(uint2)) != hipSuccess || alloc_list(ni + (1u << 20)) != hipSuccess) {
         free_shadow_lists(f);

@@ -7,8 +7,8 @@
 // the BLAS walks inside them. Every shadow ray ends at its light, so it lies in the cone
 // of directions from a point or spot light that contains its origin, or on the line
 // through its origin along a directional light. Per light, this file lists for every cell
-// of those directions (a 6 x 32 x 32 cube map around the light) or of the plane across the
-// light direction (a 64 x 64 grid) the instance slots whose padded world box reaches the
+// of those directions (a 6 x 16 x 16 cube map around the light) or of the plane across the
+// light direction (a 32 x 32 grid) the instance slots whose padded world box reaches the
 // cell. The walk (traverse2.h, ShadowListSrc in k_trace.hip) takes its ray's list and tests
 // each listed box as a one-instance TLAS leaf. The answer is the any-hit walk's: a list
 // holds every instance the ray can reach, and which instances are tested first cannot
@@ -31,10 +31,11 @@
 //   k_sl_proj   one block per light: projection record (cube-map centre, or the plane
 //               basis and grid extent over every box corner) and the near radius;
 //   k_sl_rects  per (light, slot): the box's cell rectangle on each face;
-//   k_sl_count  per (light, cell): how many rectangles hold the cell;
+//   k_sl_count  per (light, cell): how many rectangles hold the cell, as an offset within
+//               its block of cells, and the block's total;
 //   k_sl_boxes  the list boxes: every slot's padded world box, then the NaN box;
-//   k_sl_scan   one block: exclusive offsets after the identity list [0, slots) and the NaN
-//               box's entry, and the total;
+//   k_sl_scan   one block: the blocks' exclusive offsets after the identity list [0, slots)
+//               and the NaN box's entry, and the total;
 //   k_sl_fill   per (light, cell): the slots in slot order and the record {first, end}. When
 //               the lists exceed the capacity every record becomes the identity list (every
 //               slot): slower, still exact.
@@ -273,43 +274,59 @@ __device__ __forceinline__ bool cell_coords(float kind, uint32_t c, uint32_t &fa
     return false;
 }
 
-__global__ void __launch_bounds__(kSlBlock) k_sl_count(const float4 *__restrict__ proj, const uint32_t *__restrict__ rects,
-                                                       uint32_t n_inst, uint32_t n_lights, uint32_t *__restrict__ counts) {
-    const uint32_t gid = blockIdx.x * kSlBlock + threadIdx.x;
-    if (gid >= n_lights * kShadowCells) return;
-    const uint32_t l = gid / kShadowCells, c = gid - l * kShadowCells;
-    uint32_t face, u, v, n = 0;
-    if (cell_coords(proj[(size_t)l * kShadowProj].x, c, face, u, v)) {
-        const uint32_t *r = rects + (size_t)l * n_inst * 6 + face;
-        for (uint32_t i = 0; i < n_inst; i++) n += in_rect(r[(size_t)i * 6], u, v) ? 1u : 0u;
-    }
-    counts[gid] = n;
-}
-
-// exclusive offsets of counts[0..n) in place, starting at `base`; total[0] = base + sum
-constexpr uint32_t kScanBlock = 1024;
-__global__ void __launch_bounds__(kScanBlock) k_sl_scan(uint32_t *__restrict__ counts, uint32_t n, uint32_t base,
-                                                        uint32_t *__restrict__ total) {
-    __shared__ uint32_t sh[kScanBlock];
-    const uint32_t per = (n + kScanBlock - 1) / kScanBlock;
-    const uint32_t b0 = min(threadIdx.x * per, n), b1 = min(b0 + per, n);
-    uint32_t s = 0;
-    for (uint32_t i = b0; i < b1; i++) s += counts[i];
-    sh[threadIdx.x] = s;
+// exclusive scan of one value per thread across the block (kSlBlock threads); *sum = the
+// block's total
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh, uint32_t *sum) {
+    sh[threadIdx.x] = v;
     __syncthreads();
-    for (uint32_t d = 1; d < kScanBlock; d <<= 1) {  // inclusive Hillis-Steele scan
+    for (uint32_t d = 1; d < kSlBlock; d <<= 1) {  // inclusive Hillis-Steele scan
         const uint32_t t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0u;
         __syncthreads();
         sh[threadIdx.x] += t;
         __syncthreads();
     }
-    uint32_t run = base + sh[threadIdx.x] - s;
-    for (uint32_t i = b0; i < b1; i++) {
-        const uint32_t c = counts[i];
-        counts[i] = run;
-        run += c;
+    const uint32_t incl = sh[threadIdx.x];
+    *sum = sh[kSlBlock - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// per (light, cell): offsets[cell] = the cell's offset within its block of kSlBlock cells,
+// bsum[block] = the block's entries
+__global__ void __launch_bounds__(kSlBlock) k_sl_count(const float4 *__restrict__ proj, const uint32_t *__restrict__ rects,
+                                                       uint32_t n_inst, uint32_t n_lights, uint32_t *__restrict__ offsets,
+                                                       uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t sh[kSlBlock];
+    const uint32_t gid = blockIdx.x * kSlBlock + threadIdx.x;
+    uint32_t n = 0;
+    if (gid < n_lights * kShadowCells) {
+        const uint32_t l = gid / kShadowCells, c = gid - l * kShadowCells;
+        uint32_t face, u, v;
+        if (cell_coords(proj[(size_t)l * kShadowProj].x, c, face, u, v)) {
+            const uint32_t *r = rects + (size_t)l * n_inst * 6 + face;
+            for (uint32_t i = 0; i < n_inst; i++) n += in_rect(r[(size_t)i * 6], u, v) ? 1u : 0u;
+        }
     }
-    if (threadIdx.x == kScanBlock - 1) total[0] = base + sh[kScanBlock - 1];
+    uint32_t total;
+    const uint32_t off = block_exclusive_scan(n, sh, &total);
+    if (gid < n_lights * kShadowCells) offsets[gid] = off;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// exclusive offsets of the n block sums in place, starting at `base`; total[0] = base + sum
+__global__ void __launch_bounds__(kSlBlock) k_sl_scan(uint32_t *__restrict__ bsum, uint32_t n, uint32_t base,
+                                                      uint32_t *__restrict__ total) {
+    __shared__ uint32_t sh[kSlBlock];
+    uint32_t run = base;
+    for (uint32_t c0 = 0; c0 < n; c0 += kSlBlock) {  // chunks of kSlBlock sums
+        const uint32_t i = c0 + threadIdx.x;
+        const uint32_t v = i < n ? bsum[i] : 0u;
+        uint32_t sum;
+        const uint32_t off = block_exclusive_scan(v, sh, &sum);
+        if (i < n) bsum[i] = run + off;
+        run += sum;
+    }
+    if (threadIdx.x == 0) total[0] = run;
 }
 
 // boxes[0, n) = the padded world boxes, boxes[n] = NaN (no slab test passes it)
@@ -322,6 +339,7 @@ __global__ void __launch_bounds__(kSlBlock) k_sl_boxes(const float *__restrict__
 
 __global__ void __launch_bounds__(kSlBlock) k_sl_fill(const float4 *__restrict__ proj, const uint32_t *__restrict__ rects,
                                                       const uint32_t *__restrict__ offsets,
+                                                      const uint32_t *__restrict__ bsum,
                                                       const uint32_t *__restrict__ total, uint32_t n_inst,
                                                       uint32_t n_lights, uint32_t capacity, uint2 *__restrict__ cells,
                                                       uint16_t *__restrict__ list) {
@@ -343,7 +361,8 @@ __global__ void __launch_bounds__(kSlBlock) k_sl_fill(const float4 *__restrict__
         return;
     }
     uint32_t face, u, v, k = 0;
-    const uint32_t first = offsets[(size_t)l * kShadowCells + c];
+    const uint32_t cell = l * kShadowCells + c;
+    const uint32_t first = bsum[cell / kSlBlock] + offsets[cell];
     if (cell_coords(proj[(size_t)l * kShadowProj].x, c, face, u, v)) {
         const uint32_t *r = rects + (size_t)l * n_inst * 6 + face;
         for (uint32_t i = 0; i < n_inst; i++)
@@ -361,14 +380,15 @@ hipError_t launch_shadow_lists(const float *lights, uint32_t n_lights, const flo
     const uint32_t pairs = n_lights * n_inst, cells = n_lights * kShadowCells, recs = n_lights * kShadowRecs;
     hipLaunchKernelGGL(k_sl_rects, dim3((pairs + kSlBlock - 1) / kSlBlock), dim3(kSlBlock), 0, st, sl.proj, inst_box,
                        n_inst, n_lights, sl.rects);
-    hipLaunchKernelGGL(k_sl_count, dim3((cells + kSlBlock - 1) / kSlBlock), dim3(kSlBlock), 0, st, sl.proj, sl.rects,
-                       n_inst, n_lights, sl.offsets);
+    const uint32_t cell_blocks = (cells + kSlBlock - 1) / kSlBlock;
+    hipLaunchKernelGGL(k_sl_count, dim3(cell_blocks), dim3(kSlBlock), 0, st, sl.proj, sl.rects, n_inst, n_lights,
+                       sl.offsets, sl.bsum);
     const uint32_t nb = (n_inst + 1) * 6;
     hipLaunchKernelGGL(k_sl_boxes, dim3((nb + kSlBlock - 1) / kSlBlock), dim3(kSlBlock), 0, st, inst_box, n_inst,
                        sl.boxes);
-    hipLaunchKernelGGL(k_sl_scan, dim3(1), dim3(kScanBlock), 0, st, sl.offsets, cells, n_inst + 1u, sl.total);
+    hipLaunchKernelGGL(k_sl_scan, dim3(1), dim3(kSlBlock), 0, st, sl.bsum, cell_blocks, n_inst + 1u, sl.total);
     const uint32_t fill = std::max(recs, n_inst + 1u);
     hipLaunchKernelGGL(k_sl_fill, dim3((fill + kSlBlock - 1) / kSlBlock), dim3(kSlBlock), 0, st, sl.proj, sl.rects,
-                       sl.offsets, sl.total, n_inst, n_lights, sl.capacity, sl.cells, sl.list);
+                       sl.offsets, sl.bsum, sl.total, n_inst, n_lights, sl.capacity, sl.cells, sl.list);
     return hipGetLastError();
 }

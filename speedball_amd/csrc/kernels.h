@@ -110,18 +110,21 @@ hipError_t launch_sample(const SampleCascadeIn *casc, uint32_t n_casc, const sbg
 
 // k_shadow.hip: light-space candidate lists of the shadow-ray walk (sbgi_dev.h
 // shadow_cell). Device buffers: proj float4[lights * kShadowProj]; rects u32[lights *
-// slots * 6] and offsets u32[lights * kShadowCells] (build scratch); total u32[1] (the
+// slots * 6], offsets u32[lights * kShadowCells] and bsum u32[shadow_list_blocks(lights)]
+// (build scratch); total u32[1] (the
 // entries written, identity list included); cells uint2[lights * kShadowRecs]; list
 // u16[capacity >= slots + 1]: the identity list [0, slots), the NaN box's entry `slots`,
 // then the cells' lists; boxes f32[(slots + 1) * 6], the boxes the list steps test.
 struct ShadowLists {
     float4 *proj;
     float *boxes;
-    uint32_t *rects, *offsets, *total;
+    uint32_t *rects, *offsets, *bsum, *total;
     uint2 *cells;
     uint16_t *list;
     uint32_t capacity;
 };
+// blocks of 256 cells that the build scans (the length of ShadowLists::bsum)
+constexpr uint32_t shadow_list_blocks(uint32_t n_lights) { return (n_lights * kShadowCells + 255) / 256; }
 // inst_box: the padded world box of every slot (6 floats), as the SAH TLAS build reads them
 hipError_t launch_shadow_lists(const float *lights, uint32_t n_lights, const float *inst_box, uint32_t n_inst,
                                const ShadowLists &sl, hipStream_t st);

@@ -91,8 +91,8 @@ struct DevScene {
 // light) or of that plane (a kShadowDirGrid^2 grid across the light direction) list, in
 // slot order, the instances whose padded world box reaches the cell; the walk tests that
 // list instead of the TLAS.
-constexpr uint32_t kShadowFace = 32;
-constexpr uint32_t kShadowDirGrid = 64;
+constexpr uint32_t kShadowFace = 16;
+constexpr uint32_t kShadowDirGrid = 32;
 constexpr uint32_t kShadowCells = 6 * kShadowFace * kShadowFace;
 static_assert(kShadowDirGrid * kShadowDirGrid <= kShadowCells, "directional grid exceeds the cell records");
 constexpr uint32_t kShadowAll = kShadowCells;       // record {0, instances}: every slot

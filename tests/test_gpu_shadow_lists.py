@@ -27,8 +27,8 @@ from tests.test_gpu_random_scenes import random_workload
 
 pytestmark = pytest.mark.gpu
 
-FACE = 32        # cube-map cells per face edge
-DIR_GRID = 64    # directional grid cells per axis
+FACE = 16        # cube-map cells per face edge
+DIR_GRID = 32    # directional grid cells per axis
 CELLS = 6 * FACE * FACE
 RECS = CELLS + 2
 

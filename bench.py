@@ -65,6 +65,8 @@ def shadow_walk_bytes(stats) -> float:
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_instanced_pmc.md")
+# shadow rays walk the light-space candidate lists (False: --no-shadow-lists, the TLAS)
+SHADOW_LISTS = True
 
 
 def measured_traffic(kernel_prefix: str):
@@ -177,7 +179,8 @@ def room_leg(steps=20, warmup=3):
                        "64 rays, FP32 blend"}
     for depth_oct in (14, 6):
         w = scenegen.room_boxes(depth_oct=depth_oct)
-        f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=depth_oct, precision=0, traversal=1)
+        f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=depth_oct, precision=0, traversal=1,
+                       shadow_lists=SHADOW_LISTS)
         f.set_scene(w.scene)
         kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
         states = []
@@ -287,7 +290,8 @@ def adjoint_leg(reps=5):
     from speedball_amd import scenegen
     from speedball_amd.field import ProbeField
     w, albedo, target = scenegen.differentiable()
-    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=6, precision=0, traversal=1)
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=6, precision=0, traversal=1,
+                   shadow_lists=SHADOW_LISTS)
     f.set_scene(w.scene)
     params = dict(frame_jitter=w.frame_jitter, hysteresis=0.0)
     g = 2 * (f.diff_forward(albedo, **params) - f.diff_forward(target, **params))
@@ -322,7 +326,7 @@ def dynamic_leg(frames=600, warmup=10):
     t0 = time.perf_counter()
     w = scenegen.dynamic()
     f = ProbeField(w.cascades[0][2], w.cascades[0][0], w.cascades[0][1], rays=w.rays, depth_oct=w.depth_oct,
-                   precision=PRECISIONS["bf16x3"], cascades=w.cascades, traversal=1)
+                   precision=PRECISIONS["bf16x3"], cascades=w.cascades, traversal=1, shadow_lists=SHADOW_LISTS)
     f.set_stage_timing(True)  # the frames are timed by the field's stage events
     f.set_scene(w.scene)
     f.set_instance_transforms(np.arange(w.scene.inst_count), list(w.slot_worlds))
@@ -387,7 +391,8 @@ def dynamic_leg(frames=600, warmup=10):
     s2 = scenegen.Scene(**kw)
     f.close()
     gm, gs, res = w.cascades[big]
-    f = ProbeField(res, gm, gs, rays=w.rays, depth_oct=w.depth_oct, precision=PRECISIONS["bf16x3"], traversal=1)
+    f = ProbeField(res, gm, gs, rays=w.rays, depth_oct=w.depth_oct, precision=PRECISIONS["bf16x3"], traversal=1,
+                   shadow_lists=SHADOW_LISTS)
     f.set_scene(s2)
     f.tick(**params)
     o = oracle.OracleField(s2, gm, gs, res, w.rays, w.depth_oct, prec="64")
@@ -436,7 +441,7 @@ def batch_leg(frames=10, autotune=True):
     def make(i):
         scene, casc, _ = built[i]
         f = ProbeField(casc[0][2], casc[0][0], casc[0][1], rays=specs[i]["rays"], depth_oct=14,
-                       precision=PRECISIONS["bf16x3"], cascades=casc, traversal=1)
+                       precision=PRECISIONS["bf16x3"], cascades=casc, traversal=1, shadow_lists=SHADOW_LISTS)
         f.set_scene(scene)
         return f
 
@@ -482,7 +487,8 @@ def batch_leg(frames=10, autotune=True):
     i = min(range(len(specs)), key=lambda k: built[k][0].tri_count)
     scene, casc, jit = built[i]
     gm, gs, res = casc[-1]
-    f = ProbeField(res, gm, gs, rays=specs[i]["rays"], depth_oct=14, precision=PRECISIONS["bf16x3"], traversal=1)
+    f = ProbeField(res, gm, gs, rays=specs[i]["rays"], depth_oct=14, precision=PRECISIONS["bf16x3"], traversal=1,
+                   shadow_lists=SHADOW_LISTS)
     f.set_scene(scene)
     f.tick(frame_jitter=jit, hysteresis=0.97)
     o = oracle.OracleField(scene, gm, gs, res, specs[i]["rays"], 14, prec="64")
@@ -543,7 +549,7 @@ def reflect_leg(steps=10):
     for name, q in (("off", N.REFLECT_OFF), ("rough", N.REFLECT_ROUGH), ("high", N.REFLECT_HIGH),
                     ("ultra", N.REFLECT_ULTRA)):
         f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct,
-                       precision=PRECISIONS["bf16x3"], traversal=1, reflections=q)
+                       precision=PRECISIONS["bf16x3"], traversal=1, reflections=q, shadow_lists=SHADOW_LISTS)
         f.set_stage_timing(True)
         f.set_scene(w.scene)
         total, blend = [], []
@@ -585,7 +591,7 @@ def lights_leg(steps=10):
     first = None
     for name, clustered in (("all_lights", False), ("clustered", True)):
         f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct,
-                       precision=PRECISIONS["bf16x3"], traversal=1, clustered=clustered)
+                       precision=PRECISIONS["bf16x3"], traversal=1, clustered=clustered, shadow_lists=SHADOW_LISTS)
         f.set_stage_timing(True)
         f.set_scene(w.scene)
         f.tick(**kw)
@@ -660,6 +666,8 @@ def main():
     from speedball_amd.field import ProbeField
     from speedball_amd import _native as N
 
+    global SHADOW_LISTS
+    SHADOW_LISTS = not args.no_shadow_lists
     t0 = time.time()
     w = scenegen.instanced()
     log(f"scene: {w.scene.meta}, {w.scene.node_count} BLAS nodes, {w.scene.tlas_node_count} TLAS nodes, "
@@ -667,7 +675,7 @@ def main():
     prec = PRECISIONS[args.precision]
     trav = N.TRAV_MT_ORDERED if args.traversal == "ordered" else N.TRAV_MT_COMPAT
     f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, precision=prec, traversal=trav,
-                   shadow_lists=not args.no_shadow_lists)
+                   shadow_lists=SHADOW_LISTS)
     f.set_scene(w.scene)
     kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
 
@@ -735,9 +743,11 @@ def main():
     ordered = args.traversal == "ordered"
     sw_bytes = shadow_walk_bytes(cs)
     sw_ms = float(np.mean(stage["walk_shadow"]))
-    # the shadow walk over the light-space candidate lists (default) or over the TLAS
-    shadow_kernel = ("k_walk<1,false,true,ShadowSrc,false,512>" if args.no_shadow_lists
-                     else "k_walk<1,false,true,ShadowListSrc,false,512>")
+    # the shadow walk over the light-space candidate lists (the field's default on this
+    # cache-resident scene) or over the TLAS
+    lists_on = f.nbytes(N.BUF_SHADOW_LIST) > 0 and f.walk_tuning()["shadow_lists"] == 1
+    shadow_kernel = ("k_walk<1,false,true,ShadowListSrc,false,512>" if lists_on
+                     else "k_walk<1,false,true,ShadowSrc,false,512>")
     pw_bytes = walk_bytes(cs.blas_nodes, cs.tlas_nodes, cs.inst_visits, cs.tri_tests)
     pw_ms = float(np.mean(stage["walk_primary"]))
 
@@ -800,6 +810,8 @@ def main():
         # ceiling (its nodes come from L2 and L1; HBM carries ~1 % of its reads); VALU issue
         # and HBM beside it
         "roofline": dict(walk_roofline(shadow_kernel, sw_bytes, sw_ms),
+                         # top-level steps: TLAS nodes, or candidate-list entries with the lists
+                         top_level="candidate lists" if lists_on else "tlas",
                          counts={"blas_nodes": int(cs.shadow_blas_nodes), "tlas_nodes": int(cs.shadow_tlas_nodes),
                                  "tri_tests": int(cs.shadow_tri_tests), "inst_visits": int(cs.shadow_inst_visits),
                                  "rays": int(cs.shadow_rays)}),

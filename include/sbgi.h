@@ -289,6 +289,12 @@ typedef struct sbgi_walk_tuning {
     uint32_t inst_ratio;  /* 1..64, default 4 */
     uint32_t refill_min;  /* 1..64, default 8 */
     uint32_t fetch_chunk; /* 1..1024, default 64 */
+    uint32_t shadow_lists; /* 0 or 1: single-field ticks walk shadow rays over the
+                              light-space candidate lists (SBGI_BUF_SHADOW_LIST) instead
+                              of the TLAS; ignored when the field has no lists. Default
+                              (set by sbgi_set_scene): 1 when the scene's walk data
+                              (node copies and triangles) is at most 16 MiB, so that it
+                              stays cache-resident, else 0. Batches walk the TLAS.     */
 } sbgi_walk_tuning;
 
 typedef struct sbgi_field sbgi_field;
@@ -493,7 +499,7 @@ int32_t sbgi_diff_timing(sbgi_field *f, float *ms_forward, float *ms_adjoint);
 int32_t sbgi_get_walk_tuning(sbgi_field *f, sbgi_walk_tuning *out);
 int32_t sbgi_set_walk_tuning(sbgi_field *f, const sbgi_walk_tuning *t);
 /* Autotuner: times the trace of cascade p->cascade (probe range of p) under candidate
- * schedules by coordinate descent over the four parameters, keeps the fastest in the
+ * schedules by coordinate descent over the five parameters, keeps the fastest in the
  * field and returns it with its trace time. Traces into the ray scratch only: the
  * temporal state and atlases are not touched. */
 int32_t sbgi_autotune(sbgi_field *f, const sbgi_tick_params *p, sbgi_walk_tuning *best, float *best_ms);
@@ -524,7 +530,8 @@ int32_t sbgi_batch_set_walk_waves(sbgi_batch *b, uint32_t waves_per_cu);
 int32_t sbgi_batch_get_walk_waves(const sbgi_batch *b, uint32_t *waves_per_cu);
 /* Autotuner: coordinate descent over the walk schedule (shared by the batch's fields), the
  * walk block size and the resident waves, timing eager frames; keeps and returns the
- * fastest (the waves through sbgi_batch_get_walk_waves). */
+ * fastest (the waves through sbgi_batch_get_walk_waves). Each field keeps its own
+ * shadow_lists choice for its own ticks; *best reports 0, the batch's TLAS walk. */
 int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_walk_tuning *best,
                             uint32_t *best_block, float *best_ms);
 const char *sbgi_batch_last_error(const sbgi_batch *b);

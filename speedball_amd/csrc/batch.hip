@@ -389,7 +389,11 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
     std::vector<sbgi_tick_params> p(params, params + b->fields.size());
     for (auto &q : p) q.needs_clear = 0;
     auto apply = [&](const sbgi_walk_tuning &t, uint32_t block, uint32_t waves) {
-        for (sbgi_field *f : b->fields) f->tuning = t;
+        for (sbgi_field *f : b->fields) {  // each field keeps its own choice of shadow walk
+            const uint32_t lists = f->tuning.shadow_lists;
+            f->tuning = t;
+            f->tuning.shadow_lists = lists;
+        }
         b->walk_block = block;
         b->walk_waves = waves;
     };
@@ -441,6 +445,7 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
     apply(cur, block, waves);
     BHIP_OK(b, hipEventRecord(b->join, b->stream));
     for (sbgi_field *f : b->fields) BHIP_OK(b, hipStreamWaitEvent(f->stream, b->join, 0));
+    cur.shadow_lists = 0;  // batch frames walk shadow rays over the TLAS
     if (best) *best = cur;
     if (best_block) *best_block = block;
     if (best_ms) *best_ms = cur_ms;

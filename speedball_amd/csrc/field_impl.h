@@ -137,6 +137,9 @@ struct sbgi_field {
     // whenever instance boxes or light records change; sl.list is null when the field walks
     // its shadow rays over the TLAS (SBGI_FIELD_NO_SHADOW_LISTS, or too large a build)
     ShadowLists sl{};
+    // the lists predate the last change of boxes or lights: builds are skipped while the
+    // walk tuning has them off and caught up when it turns them on
+    bool sl_stale = false;
     // solve scratch
     float4 *rays = nullptr, *dirs = nullptr;
     int2 *hit_ids = nullptr;
@@ -168,7 +171,7 @@ struct sbgi_field {
     float *d_sample = nullptr;
     size_t sample_cap = 0;
     hipEvent_t sev[2] = {};
-    sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk};
+    sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk, 1u};
     // differentiable mode (sbgi_diff_*): parameters, the first update's FP32 atlas, the
     // gradients and the adjoint's scratch; the frozen hits and visibility masks are the
     // trace scratch tw of the last diff forward (diff_valid until any other solve)
@@ -291,7 +294,7 @@ inline DevScene make_scene(const sbgi_field *f) {
     s.tri_index = f->d_tri_index;
     s.vdata = f->d_vdata;
     s.lds_inst = s.tlas_count * 2;
-    if (f->sl.list) {
+    if (f->sl.list && f->tuning.shadow_lists) {
         s.sl_proj = f->sl.proj;
         s.sl_cells = f->sl.cells;
         s.sl_list = f->sl.list;

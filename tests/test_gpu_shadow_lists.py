@@ -206,3 +206,34 @@ def test_lists_give_the_tlas_walks_masks(case):
             print(f"{case} traversal {trav} update {i}: {ones} unblocked shadow rays, {bits} bits differ")
             assert ones > 0
             assert bits == 0
+
+
+def test_walk_tuning_switches_the_lists():
+    """The lists are a walk-tuning choice (default on for a cache-resident scene): switching
+    them off, editing instances while they are off (the builds are skipped) and on again
+    (the stale lists are rebuilt) keeps the masks equal to the TLAS walk's."""
+    w = sg.instanced(res=(12, 6, 8), rays=64)
+    edit = _move_instances(w)
+    kw = dict(frame_jitter=w.frame_jitter, hysteresis=w.hysteresis)
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct, traversal=N.TRAV_MT_ORDERED)
+    try:
+        f.set_scene(w.scene)
+        assert f.walk_tuning()["shadow_lists"] == 1
+        f.tick(**kw)
+        on = f.read(N.BUF_SHADOW_VIS, np.uint64)
+        f.set_walk_tuning(shadow_lists=0)
+        f.tick(**kw)
+        off = f.read(N.BUF_SHADOW_VIS, np.uint64)
+        edit(f)  # lists not rebuilt while off
+        f.tick(**kw)
+        off_edited = f.read(N.BUF_SHADOW_VIS, np.uint64)
+        f.set_walk_tuning(shadow_lists=1)  # stale lists rebuilt
+        f.tick(**kw)
+        on_edited = f.read(N.BUF_SHADOW_VIS, np.uint64)
+    finally:
+        f.close()
+    for name, a, b in (("before the edit", on, off), ("after the edit", on_edited, off_edited)):
+        bits = int(np.unpackbits((a ^ b).view(np.uint8)).sum())
+        print(f"{name}: {int(np.unpackbits(b.view(np.uint8)).sum())} unblocked shadow rays, {bits} bits differ")
+        assert bits == 0
+    assert int(np.unpackbits((on ^ on_edited).view(np.uint8)).sum()) > 0  # the edit moved shadows

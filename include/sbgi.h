@@ -283,7 +283,9 @@ typedef struct sbgi_stats {
  * schedules the reference's walk on 64-lane waves). A triangle / instance phase runs when
  * the lanes waiting in it are >= 1/tri_ratio (1/inst_ratio) of the lanes still walking
  * nodes; finished lanes are refilled in batches of >= refill_min; rays are fetched
- * fetch_chunk at a time. Results do not depend on it. */
+ * fetch_chunk at a time. Results do not depend on the schedule. shadow_lists changes which
+ * boxes cull a shadow ray (its list's instance boxes instead of TLAS nodes), so only a ray
+ * that grazes one of those boxes could get another answer; the GPU tests find none. */
 typedef struct sbgi_walk_tuning {
     uint32_t tri_ratio;   /* 1..64, default 4 */
     uint32_t inst_ratio;  /* 1..64, default 4 */

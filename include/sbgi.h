@@ -297,6 +297,12 @@ typedef struct sbgi_walk_tuning {
                               (set by sbgi_set_scene): 1 when the scene's walk data
                               (node copies and triangles) is at most 16 MiB, so that it
                               stays cache-resident, else 0. Batches walk the TLAS.     */
+    uint32_t shadow_copies; /* 8 or 1: BLAS walk copies of shadow rays in MT_ORDERED /
+                              WATERTIGHT fields: 8, the near-first copy of the ray's
+                              direction octant (as closest-hit rays); 1, one copy for
+                              every shadow ray, a smaller working set. Default (set by
+                              sbgi_set_scene): 8 when the walk data fits 16 MiB, else 1.
+                              Used by ticks and batches alike.                        */
 } sbgi_walk_tuning;
 
 typedef struct sbgi_field sbgi_field;
@@ -501,7 +507,7 @@ int32_t sbgi_diff_timing(sbgi_field *f, float *ms_forward, float *ms_adjoint);
 int32_t sbgi_get_walk_tuning(sbgi_field *f, sbgi_walk_tuning *out);
 int32_t sbgi_set_walk_tuning(sbgi_field *f, const sbgi_walk_tuning *t);
 /* Autotuner: times the trace of cascade p->cascade (probe range of p) under candidate
- * schedules by coordinate descent over the five parameters, keeps the fastest in the
+ * schedules by coordinate descent over the six parameters, keeps the fastest in the
  * field and returns it with its trace time. Traces into the ray scratch only: the
  * temporal state and atlases are not touched. */
 int32_t sbgi_autotune(sbgi_field *f, const sbgi_tick_params *p, sbgi_walk_tuning *best, float *best_ms);

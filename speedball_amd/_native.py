@@ -117,7 +117,7 @@ class SkinSet(C.Structure):
 
 class WalkTuning(C.Structure):
     _fields_ = [("tri_ratio", C.c_uint32), ("inst_ratio", C.c_uint32), ("refill_min", C.c_uint32),
-                ("fetch_chunk", C.c_uint32), ("shadow_lists", C.c_uint32)]
+                ("fetch_chunk", C.c_uint32), ("shadow_lists", C.c_uint32), ("shadow_copies", C.c_uint32)]
 
 
 class SampleParams(C.Structure):

@@ -390,9 +390,10 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
     for (auto &q : p) q.needs_clear = 0;
     auto apply = [&](const sbgi_walk_tuning &t, uint32_t block, uint32_t waves) {
         for (sbgi_field *f : b->fields) {  // each field keeps its own choice of shadow walk
-            const uint32_t lists = f->tuning.shadow_lists;
+            const uint32_t lists = f->tuning.shadow_lists, copies = f->tuning.shadow_copies;
             f->tuning = t;
             f->tuning.shadow_lists = lists;
+            f->tuning.shadow_copies = copies;
         }
         b->walk_block = block;
         b->walk_waves = waves;
@@ -446,6 +447,7 @@ int32_t sbgi_batch_autotune(sbgi_batch *b, const sbgi_tick_params *params, sbgi_
     BHIP_OK(b, hipEventRecord(b->join, b->stream));
     for (sbgi_field *f : b->fields) BHIP_OK(b, hipStreamWaitEvent(f->stream, b->join, 0));
     cur.shadow_lists = 0;  // batch frames walk shadow rays over the TLAS
+    cur.shadow_copies = b->fields[0]->tuning.shadow_copies;  // the fields keep their own
     if (best) *best = cur;
     if (best_block) *best_block = block;
     if (best_ms) *best_ms = cur_ms;

@@ -295,6 +295,11 @@ int32_t size_shadow_lists(sbgi_field *f) {
     const sbgi_scene_view &v = f->view;
     const size_t walk_bytes = (size_t)v.node_count * 32 * (ordered(f) ? 8 : 1) + (size_t)v.tri_count * 48;
     f->tuning.shadow_lists = walk_bytes <= kShadowListScene ? 1u : 0u;
+    // past it, shadow rays also walk one BLAS copy instead of their octant's near-first copy:
+    // a shadow ray needs any occluder, and on the dynamic workload (8 copies: 269 MiB) its
+    // walks take 8.49 instead of 8.82 ms a frame; on the cache-resident instanced workload
+    // one copy costs the shadow walk 6 %
+    f->tuning.shadow_copies = walk_bytes <= kShadowListScene ? 8u : 1u;
     const uint64_t work = (uint64_t)v.light_count * kShadowCells * v.inst_count;
     if ((f->desc.flags & SBGI_FIELD_NO_SHADOW_LISTS) || v.light_count == 0 || v.inst_count == 0 ||
         v.inst_count > 65535u || work > kShadowListWork)
@@ -2025,7 +2030,8 @@ int32_t sbgi_get_walk_tuning(sbgi_field *f, sbgi_walk_tuning *out) {
 int32_t sbgi_set_walk_tuning(sbgi_field *f, const sbgi_walk_tuning *t) {
     if (!f || !t) return SBGI_E_INVALID;
     if (t->tri_ratio < 1 || t->tri_ratio > 64 || t->inst_ratio < 1 || t->inst_ratio > 64 || t->refill_min < 1 ||
-        t->refill_min > 64 || t->fetch_chunk < 1 || t->fetch_chunk > 1024 || t->shadow_lists > 1)
+        t->refill_min > 64 || t->fetch_chunk < 1 || t->fetch_chunk > 1024 || t->shadow_lists > 1 ||
+        (t->shadow_copies != 1 && t->shadow_copies != 8))
         return fail(f, SBGI_E_INVALID, "walk tuning out of range");
     HIP_OK(f, hipSetDevice(f->desc.device));
     f->tuning = *t;
@@ -2072,18 +2078,25 @@ int32_t sbgi_autotune(sbgi_field *f, const sbgi_tick_params *p, sbgi_walk_tuning
     int32_t rc = time_trace(cur, cur_ms);
     if (rc != SBGI_OK) return rc;
     static const uint32_t ratios[] = {2, 4, 8, 16}, refills[] = {4, 8, 16, 32}, chunks[] = {32, 64, 128, 256};
-    static const uint32_t onoff[] = {0, 1};
+    static const uint32_t onoff[] = {0, 1}, copies[] = {1, 8};
     for (int pass = 0; pass < 2; pass++) {
-        // the shadow lists are a candidate only where the field has them
-        for (int param = 0; param < (f->sl.list ? 5 : 4); param++) {
-            const uint32_t *cand = param < 2 ? ratios : (param == 2 ? refills : (param == 3 ? chunks : onoff));
-            for (int i = 0; i < (param == 4 ? 2 : 4); i++) {
+        for (int param = 0; param < 6; param++) {
+            // the shadow lists are a candidate only where the field has them, one shadow copy
+            // only where there are eight
+            if ((param == 4 && !f->sl.list) || (param == 5 && !ordered(f))) continue;
+            const uint32_t *cand = param < 2    ? ratios
+                                   : param == 2 ? refills
+                                   : param == 3 ? chunks
+                                   : param == 4 ? onoff
+                                                : copies;
+            for (int i = 0; i < (param >= 4 ? 2 : 4); i++) {
                 sbgi_walk_tuning t = cur;
                 uint32_t *slot = param == 0   ? &t.tri_ratio
                                  : param == 1 ? &t.inst_ratio
                                  : param == 2 ? &t.refill_min
                                  : param == 3 ? &t.fetch_chunk
-                                              : &t.shadow_lists;
+                                 : param == 4 ? &t.shadow_lists
+                                              : &t.shadow_copies;
                 if (*slot == cand[i]) continue;
                 *slot = cand[i];
                 float ms = 0.f;

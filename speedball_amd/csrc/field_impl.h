@@ -171,7 +171,7 @@ struct sbgi_field {
     float *d_sample = nullptr;
     size_t sample_cap = 0;
     hipEvent_t sev[2] = {};
-    sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk, 1u};
+    sbgi_walk_tuning tuning{kDefaultTriRatio, kDefaultInstRatio, kDefaultRefillMin, kDefaultFetchChunk, 1u, 8u};
     // differentiable mode (sbgi_diff_*): parameters, the first update's FP32 atlas, the
     // gradients and the adjoint's scratch; the frozen hits and visibility masks are the
     // trace scratch tw of the last diff forward (diff_valid until any other solve)
@@ -266,6 +266,7 @@ inline DevScene make_scene(const sbgi_field *f) {
     s.blas = reinterpret_cast<const uint4 *>(f->d_nodes);
     s.nodes = reinterpret_cast<const uint4 *>(f->d_walk_nodes);
     s.blas_octants = ordered(f) ? 8u : 1u;
+    s.shadow_octants = std::min(s.blas_octants, f->tuning.shadow_copies);
     s.tri = f->d_tri;
     s.tlas = ordered(f) ? f->d_tlas_sah : f->d_tlas;
     s.tlas_oct = ordered(f) ? f->d_tlas_oct : nullptr;

@@ -509,7 +509,9 @@ void walk_tables(const TraceSeg *segs, uint32_t n_seg, std::vector<unsigned char
         const TraceSeg &g = segs[j];
         pr[j] = WalkSeg<PrimarySrc>{g.s, PrimarySrc{g.c, g.classify, g.dirs, g.meta, g.w.hits}, nullptr,
                                     g.c.updated * g.c.rays, g.heads};
-        sh[j] = WalkSeg<ShadowSrc>{g.s, ShadowSrc{g.w.queue.rec, g.w.vis, g.bias}, g.w.queue.count, g.w.queue.capacity,
+        DevScene gs = g.s;
+        gs.blas_octants = gs.shadow_octants;  // the shadow walk's BLAS copies
+        sh[j] = WalkSeg<ShadowSrc>{gs, ShadowSrc{g.w.queue.rec, g.w.vis, g.bias}, g.w.queue.count, g.w.queue.capacity,
                                    g.heads + (size_t)kFetchHeads * kHeadStride};
     }
 }
@@ -668,11 +670,13 @@ hipError_t launch_trace(const DevScene &s, const DevCascade &c, const sbgi_tick_
     if ((e = mark(2)) != hipSuccess) return e;
     // the queue length is only known on the device: the kernel reads it, bounded by the capacity
     const float bias = hit_bias(c.cell_min, p.dbg_trace_bias, p.dbg_ray_eps);
+    DevScene ss = s;
+    ss.blas_octants = s.shadow_octants;  // the shadow walk's BLAS copies
     if (s.sl_cells)
-        launch_walk<WALK_ANY>(s, ShadowListSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity,
+        launch_walk<WALK_ANY>(ss, ShadowListSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity,
                               ctr + trace_heads(1), counters ? counters->shadow_walk : nullptr, lds, st);
     else
-        launch_walk<WALK_ANY>(s, ShadowSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity,
+        launch_walk<WALK_ANY>(ss, ShadowSrc{w.queue.rec, w.vis, bias}, w.queue.count, w.queue.capacity,
                               ctr + trace_heads(1), counters ? counters->shadow_walk : nullptr, lds, st);
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_resolve, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p, dirs, meta,

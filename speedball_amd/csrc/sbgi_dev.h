@@ -46,6 +46,7 @@ struct DevScene {
     const float *__restrict__ lights;
     uint32_t node_count, tlas_count, light_count, tri_count, inst_count;
     uint32_t blas_octants;  // 1: reference node order only; 8: + near-first copy per ray octant
+    uint32_t shadow_octants;  // copies the shadow walks take: blas_octants, or 1 (one copy for all)
     // LDS walks: uint4 offset of the instance records in the block's LDS (after the TLAS
     // nodes, or 0 when the walk reads its TLAS from global memory and stages only them)
     uint32_t lds_inst;

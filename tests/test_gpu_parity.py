@@ -329,11 +329,15 @@ def test_walk_schedule_does_not_change_results(instanced_small):
     ray still walks its own nodes in its own order, so state after two updates is
     bit-identical under extreme schedules, and the autotuner (which traces into the ray
     scratch only) leaves the temporal state as it found it. The shadow walk's top level
-    (candidate lists or TLAS) changes which instances a shadow ray tests, never its answer."""
+    (candidate lists or TLAS) and its BLAS copies (its octant's near-first copy or one copy)
+    change which nodes a shadow ray tests, never its answer."""
     w = instanced_small
     out = []
-    for tuning in (None, dict(tri_ratio=1, inst_ratio=64, refill_min=1, fetch_chunk=1, shadow_lists=0),
-                   dict(tri_ratio=64, inst_ratio=1, refill_min=64, fetch_chunk=1024, shadow_lists=1), "auto"):
+    for tuning in (None, dict(tri_ratio=1, inst_ratio=64, refill_min=1, fetch_chunk=1, shadow_lists=0,
+                              shadow_copies=1),
+                   dict(tri_ratio=64, inst_ratio=1, refill_min=64, fetch_chunk=1024, shadow_lists=1, shadow_copies=8),
+                   dict(tri_ratio=4, inst_ratio=4, refill_min=8, fetch_chunk=64, shadow_lists=1, shadow_copies=1),
+                   "auto"):
         f = make_field(w, traversal=N.TRAV_MT_ORDERED)
         print(f"case: walk tuning {tuning}", flush=True)
         f.tick(**params(w))
@@ -341,7 +345,7 @@ def test_walk_schedule_does_not_change_results(instanced_small):
             t, ms = f.autotune(**params(w))
             print("autotuned", t, f"{ms:.3f} ms")
             assert all(1 <= v <= 1024 for k, v in t.items() if k != "shadow_lists") and ms > 0
-            assert t["shadow_lists"] in (0, 1)
+            assert t["shadow_lists"] in (0, 1) and t["shadow_copies"] in (1, 8)
             assert f.walk_tuning() == t
         elif tuning:
             f.set_walk_tuning(**tuning)
@@ -358,11 +362,12 @@ def test_walk_tuning_rejects_out_of_range(instanced_small):
     f = make_field(instanced_small, traversal=N.TRAV_MT_ORDERED)
     try:
         for bad in (dict(tri_ratio=0), dict(inst_ratio=65), dict(refill_min=0), dict(fetch_chunk=2048),
-                    dict(shadow_lists=2)):
+                    dict(shadow_lists=2), dict(shadow_copies=2)):
             with pytest.raises(RuntimeError):
                 f.set_walk_tuning(**bad)
         # defaults; the lists are on for this cache-resident scene
-        assert f.walk_tuning() == dict(tri_ratio=4, inst_ratio=4, refill_min=8, fetch_chunk=64, shadow_lists=1)
+        assert f.walk_tuning() == dict(tri_ratio=4, inst_ratio=4, refill_min=8, fetch_chunk=64, shadow_lists=1,
+                                       shadow_copies=8)
     finally:
         f.close()
 

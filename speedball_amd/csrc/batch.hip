@@ -149,8 +149,8 @@ int32_t enqueue_frame(sbgi_batch *b, const sbgi_tick_params *params, const std::
         const sbgi_tick_params &p = params[g.field];
         BHIP_OK(b, launch_resolve(ts[j].s, ts[j].c, p, ts[j].dirs, ts[j].meta, cs.atlas, ts[j].w.hits, ts[j].w.vis,
                                   b->rays[j], st));
-        BHIP_OK(b, launch_emitter_vis(ts[j].s, ts[j].c, p, ts[j].meta, cs.evis, ts[j].w.queue.count + trace_heads(3),
-                                      st));
+        BHIP_OK(b, launch_emitter_vis(ts[j].s, ts[j].c, p, ts[j].meta, cs.evis, ts[j].w.hits,
+                                      ts[j].w.queue.count + trace_heads(3), st));
         BHIP_OK(b, launch_reflect_and_blend(f, cs, ts[j].c, p, b->rays[j], st));
         BHIP_OK(b, launch_upload(ts[j].c, p, cs.irr, cs.depth, cs.atlas, cs.datlas, nullptr, st, cs.rough, cs.ratlas));
     }

@@ -1051,7 +1051,8 @@ static int32_t enqueue_tick(sbgi_field *f, CascadeState &cs, const DevScene &s, 
     HIP_OK(f, launch_trace(s, c, *p, f->dirs, cs.meta, cs.atlas, nullptr, f->tw, f->rays, f->hit_ids,
                            f->count_mode ? f->counters : nullptr, f->tev, st, mk));
     HIP_OK(f, mk(f->eev, st));
-    HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.queue.count + trace_heads(3), st));
+    // the emitter rays' table takes the hit records, which the shading has consumed
+    HIP_OK(f, launch_emitter_vis(s, c, *p, cs.meta, cs.evis, f->tw.hits, f->tw.queue.count + trace_heads(3), st));
     HIP_OK(f, mk(f->ev[2], st));
     HIP_OK(f, launch_reflect_and_blend(f, cs, c, *p, f->rays, st));
     HIP_OK(f, mk(f->ev[3], st));

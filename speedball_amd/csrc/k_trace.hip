@@ -114,26 +114,19 @@ struct ShadowListSrc {
 
 // Emitter visibility (emitterVisKernel, js/gi_probes.js:2216-2260): ray i = slot * E + e
 // tests the probe's view of a hashed point on emitter e; the store blends the 0/1 sample
-// into the retained visibility, vis = mix(sample, vis, 0.8). A probe inside the emitter's
+// into the retained visibility, vis = mix(sample, vis, 0.8). The rays come from a table that
+// k_emitter_rays writes just before the walk, {origin, extent}, {direction, tag} per ray:
+// the hash's double-precision sines and the sample frame cost the walk's refill ~300
+// instructions and 40 VGPRs per ray if evaluated there. A probe inside the emitter's
 // bounding sphere walks nothing (its extent is NaN, so no box test passes) and samples 0.
 struct EmitterSrc {
     static constexpr bool kList = false;
-    DevCascade c;
-    float classify, seed, bias;
-    uint32_t ebase, ecount;
-    const float *meta;
-    float *vis;  // [probe * SBGI_MAX_EMITTERS + e]
+    const float4 *rays;  // 2 per ray
+    float *vis;          // [probe * SBGI_MAX_EMITTERS + e]
     __device__ __forceinline__ void load(const DevScene &s, uint32_t i, LaneRay &r) const {
-        const uint32_t slot = i / ecount, e = i - slot * ecount;
-        const uint32_t probe = (c.offset + slot) % c.total;
-        const float3 ro = probe_origin(c, classify, meta, probe);
-        float3 rd = f3(0.f, 0.f, 1.f);
-        float tmax = 0.f;
-        const bool valid = emitter_ray(s.lights + (size_t)(ebase + e) * SBGI_LIGHT_STRIDE, ro, probe, e, seed, bias,
-                                       rd, tmax);
-        // an invalid ray's extent is NaN: its root box test fails, so it retires after one step
-        lane_begin<true>(s, r, ro, rd, valid ? tmax : __builtin_nanf(""));
-        r.tag = (probe * SBGI_MAX_EMITTERS + e) | (valid ? 0u : 0x80000000u);
+        const float4 a = rays[(size_t)i * 2], b = rays[(size_t)i * 2 + 1];
+        lane_begin<true>(s, r, xyz(a), xyz(b), a.w);
+        r.tag = __float_as_uint(b.w);
     }
     __device__ __forceinline__ void store(uint32_t, const LaneRay &r) const {
         const float sample = (!(r.tag >> 31) && !r.blocked) ? 1.f : 0.f;
@@ -399,6 +392,27 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
 
 // Clears the visibility masks, the queue length and the fetch counters of every walk of
 // the solve (the solve counters of kernels.h).
+static_assert(2 * SBGI_MAX_EMITTERS <= SBGI_RAYS_MIN, "the emitter ray table must fit the hit records");
+// The emitter-visibility rays of a solve (EmitterSrc): ray i = slot * E + e, from the
+// probe's traced position toward a hashed point of emitter e (emitter_ray), as
+// {origin, extent (NaN when the probe is inside the emitter)}, {direction, tag}.
+__global__ void __launch_bounds__(kBlock) k_emitter_rays(DevScene s, DevCascade c, float classify, float seed,
+                                                          float bias, const float *__restrict__ meta,
+                                                          float4 *__restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.updated * s.emitter_count) return;
+    const uint32_t slot = i / s.emitter_count, e = i - slot * s.emitter_count;
+    const uint32_t probe = (c.offset + slot) % c.total;
+    const float3 ro = probe_origin(c, classify, meta, probe);
+    float3 rd = f3(0.f, 0.f, 1.f);
+    float tmax = 0.f;
+    const bool valid = emitter_ray(s.lights + (size_t)(s.emitter_base + e) * SBGI_LIGHT_STRIDE, ro, probe, e, seed,
+                                   bias, rd, tmax);
+    const uint32_t tag = (probe * SBGI_MAX_EMITTERS + e) | (valid ? 0u : 0x80000000u);
+    out[(size_t)i * 2] = make_float4(ro.x, ro.y, ro.z, valid ? tmax : __builtin_nanf(""));
+    out[(size_t)i * 2 + 1] = make_float4(rd.x, rd.y, rd.z, __uint_as_float(tag));
+}
+
 __global__ void k_clear_u64(unsigned long long *__restrict__ p, uint32_t n, uint32_t *__restrict__ counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
@@ -698,11 +712,13 @@ hipError_t launch_classify(const DevScene &s, const DevCascade &c, float reloc_c
 }
 
 hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float *meta,
-                              float *vis, uint32_t *heads, hipStream_t st) {
+                              float *vis, float4 *scratch, uint32_t *heads, hipStream_t st) {
     const uint32_t n = c.updated * s.emitter_count;
     if (!n) return hipSuccess;
     const float bias = hit_bias(c.cell_min, p.dbg_trace_bias, p.dbg_ray_eps);
-    const EmitterSrc src{c, p.classify_strength, p.emitter_vis_seed, bias, s.emitter_base, s.emitter_count, meta, vis};
-    launch_walk<WALK_ANY>(s, src, nullptr, n, heads, nullptr, trace_lds_bytes(s.tlas_count, s.inst_count), st);
+    hipLaunchKernelGGL(k_emitter_rays, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s, c, p.classify_strength,
+                       p.emitter_vis_seed, bias, meta, scratch);
+    launch_walk<WALK_ANY>(s, EmitterSrc{scratch, vis}, nullptr, n, heads, nullptr,
+                          trace_lds_bytes(s.tlas_count, s.inst_count), st);
     return hipGetLastError();
 }

@@ -88,9 +88,11 @@ hipError_t launch_resolve(const DevScene &s, const DevCascade &c, const sbgi_tic
                           const float *meta, const ushort4 *prev_atlas, const float4 *hits,
                           const unsigned long long *vis, float4 *rays, hipStream_t st);
 // emitterVisKernel: one any-hit ray per (updated probe, emitter of s); vis[probe * 16 + e]
-// <- mix(sample, vis, 0.8). heads: the walk's kFetchHeads fetch counters, zero on entry.
+// <- mix(sample, vis, 0.8). scratch: 2 float4 per ray (updated x emitters x 32 bytes, free
+// between the shading and the next solve: the trace's hit records). heads: the walk's
+// kFetchHeads fetch counters, zero on entry.
 hipError_t launch_emitter_vis(const DevScene &s, const DevCascade &c, const sbgi_tick_params &p, const float *meta,
-                              float *vis, uint32_t *heads, hipStream_t st);
+                              float *vis, float4 *scratch, uint32_t *heads, hipStream_t st);
 
 // classifyKernel over every probe of the cascade: kClassifyRays fixed closest-hit rays per
 // probe from its grid position (dirs: kClassifyRays float4, written here; hits: total x

@@ -237,3 +237,53 @@ def test_walk_tuning_switches_the_lists():
         print(f"{name}: {int(np.unpackbits(b.view(np.uint8)).sum())} unblocked shadow rays, {bits} bits differ")
         assert bits == 0
     assert int(np.unpackbits((on ^ on_edited).view(np.uint8)).sum()) > 0  # the edit moved shadows
+
+
+def many_instances(n=900, seed=5):
+    """n instances of three small meshes (a box, a sphere, a torus) over a 30 x 10 x 20 m
+    box: more slots than the walk stages in LDS, so the list walk reads its boxes and
+    instance records from global memory. Point, ranged, spot and directional lights."""
+    from speedball_amd.scene import SceneBuilder, directional_light, point_light, spot_light, uber_material
+    rng = np.random.default_rng(seed)
+    box = np.array([30.0, 10.0, 20.0])
+    b = SceneBuilder()
+    meshes = []
+    for kind, (v, t) in enumerate((sg.box_mesh((0, 0, 0), (0.4, 0.3, 0.5), 0.3), sg.uv_sphere(6, 12, 0.5),
+                                   sg.torus(16, 8, 0.5, 0.15))):
+        mat = b.add_material(uber_material((0.6, 0.5, 0.4), side=2 if kind == 2 else 0))
+        meshes.append(b.add_mesh(np.asarray(v, np.float32), np.asarray(t, np.uint32),
+                                 tri_material=np.full(len(t), mat, np.uint32)))
+    for i in range(n):
+        M = np.eye(4)
+        a = rng.uniform(0, 2 * np.pi)
+        M[:3, :3] = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]]) * rng.uniform(0.5, 1.5)
+        M[:3, 3] = rng.uniform(0, 1, 3) * box
+        b.add_instance(meshes[i % 3], M, active=bool(rng.uniform() > 0.05))
+    b.add_light(point_light((15.0, 9.0, 10.0), intensity=40.0))
+    b.add_light(point_light((5.0, 5.0, 5.0), intensity=20.0, distance=6.0, decay=1.5))
+    b.add_light(spot_light((25.0, 8.0, 15.0), (20.0, 0.0, 10.0), intensity=60.0, angle=0.7, penumbra=0.3))
+    b.add_light(directional_light((15.0, 30.0, 10.0), (14.0, 0.0, 12.0), intensity=1.0))
+    scene = b.finalize(workload="many_instances")
+    return sg.Workload("many_instances", scene, (0.0, 0.0, 0.0), tuple(box.tolist()), (10, 4, 7), 64, 6, 0.37)
+
+
+def test_lists_from_global_memory_give_the_tlas_walks_masks():
+    w = many_instances()
+    print(f"case: {w.scene.inst_count} instances (list boxes and records from global memory)", flush=True)
+    assert w.scene.inst_count * 96 > 64 * 1024
+    for trav in (N.TRAV_MT_ORDERED, N.TRAV_MT_COMPAT):
+        a = _masks(w, True, trav)[0]
+        b = _masks(w, False, trav)[0]
+        bits = int(np.unpackbits((a ^ b).view(np.uint8)).sum())
+        ones = int(np.unpackbits(b.view(np.uint8)).sum())
+        print(f"traversal {trav}: {ones} unblocked shadow rays, {bits} bits differ")
+        assert ones > 0 and bits == 0
+    rng = np.random.default_rng(11)
+    f = ProbeField(w.res, w.grid_min, w.grid_size, rays=w.rays, depth_oct=w.depth_oct)
+    try:
+        f.set_scene(w.scene)
+        n, mean_len = check_lists(f, w.scene, rng, 400)
+    finally:
+        f.close()
+    print(f"{n} segments checked, mean list length {mean_len:.2f} of {w.scene.inst_count}")
+    assert n > 1000

@@ -132,7 +132,8 @@ extern "C" {
    u0}, {e2, v0}, {su, sv, margin, 0}; CELLS: u32[2*L*1538] {first, end} per cell record
    (6 x 16 x 16 cube-map cells, or a 32 x 32 grid for directional lights, then {0, I} and
    {I, I+1}); LIST: u16[capacity] instance slots: the identity list, then entry I (the
-   never-hit box of cells no instance reaches), then the cells' lists (read). */
+   never-hit box of cells no instance reaches), then the cells' lists, each nearest its
+   light first (ties by slot) (read). */
 #define SBGI_BUF_SHADOW_PROJ 30
 #define SBGI_BUF_SHADOW_CELLS 31
 #define SBGI_BUF_SHADOW_LIST 32

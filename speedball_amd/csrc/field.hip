@@ -135,6 +135,7 @@ void free_shadow_lists(sbgi_field *f) {
     dfree(f->sl.total);
     dfree(f->sl.cells);
     dfree(f->sl.list);
+    dfree(f->sl.order);
     f->sl.capacity = 0;
 }
 
@@ -285,9 +286,10 @@ int32_t build_shadow_lists(sbgi_field *f) {
 // do not fit 16 bits, walks its shadow rays over the TLAS.
 // Whether ticks take the lists (sbgi_walk_tuning::shadow_lists) defaults by the size of the
 // walk data: on the cache-resident instanced workload (2.5 MiB of node copies and
-// triangles) the lists cut the top-level steps of a shadow ray from 37 to 7 and the walk
-// time by ~2 %; on the dynamic workload (320 MiB, BLAS reads from HBM) they cut them from
-// 31 to 5 but the update takes 6.5 % longer, the list reads missing L2.
+// triangles) the lists, nearest the light first, cut the top-level steps of a shadow ray
+// from 37 to 5 and the walk time by a fifth; on the dynamic workload (320 MiB, BLAS reads
+// from HBM) they cut them from 31 to 5 but the update takes 6.5 % longer, the list reads
+// missing L2.
 constexpr uint64_t kShadowListWork = 1ull << 28;
 constexpr size_t kShadowListScene = 16ull << 20;
 int32_t size_shadow_lists(sbgi_field *f) {
@@ -317,7 +319,8 @@ int32_t size_shadow_lists(sbgi_field *f) {
         dalloc(&f->sl.offsets, nl * kShadowCells * sizeof(uint32_t)) != hipSuccess ||
         dalloc(&f->sl.bsum, shadow_list_blocks((uint32_t)nl) * sizeof(uint32_t)) != hipSuccess ||
         dalloc(&f->sl.total, sizeof(uint32_t)) != hipSuccess ||
-        dalloc(&f->sl.cells, nl * kShadowRecs * sizeof(uint2)) != hipSuccess || alloc_list(ni + (1u << 20)) != hipSuccess) {
+        dalloc(&f->sl.cells, nl * kShadowRecs * sizeof(uint2)) != hipSuccess ||
+        dalloc(&f->sl.order, nl * ni * sizeof(uint16_t)) != hipSuccess || alloc_list(ni + (1u << 20)) != hipSuccess) {
         free_shadow_lists(f);
         return fail(f, SBGI_E_NOMEM, "shadow list allocation");
     }

@@ -15,6 +15,14 @@
 // change whether an occluder exists. A cell that no box reaches lists a NaN box, which no
 // ray hits, so every list has an entry.
 //
+// The order does change how soon a blocked ray stops: each list holds its slots nearest the
+// light first (a point or spot light: by the squared distance from the light to the box;
+// directional: by the box centre's height along the direction toward the light; ties by
+// slot). A box near the light lies between the light and most origins of its cell, so it
+// is the likeliest occluder: on the instanced workload (256 instances, 16 point lights)
+// this order cuts the shadow walk's BLAS node visits from 425 to 336 M and its time by a
+// fifth against slot order (farthest first: 526 M).
+//
 // Conservative by construction:
 // - a box's cells are widened by kFaceMargin in face coordinates (directional: kDirMargin
 //   of the scene's largest coordinate), far more than the rounding of a ray's direction
@@ -36,11 +44,14 @@
 //   k_sl_boxes  the list boxes: every slot's padded world box, then the NaN box;
 //   k_sl_scan   one block: the blocks' exclusive offsets after the identity list [0, slots)
 //               and the NaN box's entry, and the total;
-//   k_sl_fill   per (light, cell): the slots in slot order and the record {first, end}. When
-//               the lists exceed the capacity every record becomes the identity list (every
-//               slot): slower, still exact.
-// The lists depend only on boxes and lights, are deterministic (slot order, no atomics),
-// and cost ~2 x lights x cells x slots rectangle tests to build.
+//   k_sl_order  per (light, slot): the slot's rank in the light's order (a rank sort over
+//               the light's slots, staged through LDS), written as order[light][rank];
+//   k_sl_fill   per (light, cell): the slots in the light's order and the record
+//               {first, end}. When the lists exceed the capacity every record becomes the
+//               identity list (every slot): slower, still exact.
+// The lists depend only on boxes and lights, are deterministic (a total order, no atomics),
+// and cost ~2 x lights x cells x slots rectangle tests and lights x slots^2 comparisons to
+// build.
 #include "kernels.h"
 
 namespace {
@@ -337,11 +348,53 @@ __global__ void __launch_bounds__(kSlBlock) k_sl_boxes(const float *__restrict__
     else if (gid < (n_inst + 1) * 6) boxes[gid] = __builtin_nanf("");
 }
 
+// Sort key of a slot's box for light record pj (lower = walked first): the squared distance
+// from a point or spot light to the box, or minus the box centre's height along the
+// direction toward a directional light (w = e1 x e2); NaN sorts last.
+__device__ __forceinline__ float order_key(const float4 *pj, const float *b) {
+    float k = 0.f;
+    if (pj[0].x == kShadowKindPoint) {
+        const float3 lp = xyz(pj[1]);
+        const float dx = fmaxf(fmaxf(b[0] - lp.x, lp.x - b[3]), 0.f);
+        const float dy = fmaxf(fmaxf(b[1] - lp.y, lp.y - b[4]), 0.f);
+        const float dz = fmaxf(fmaxf(b[2] - lp.z, lp.z - b[5]), 0.f);
+        k = dx * dx + dy * dy + dz * dz;
+    } else if (pj[0].x == kShadowKindDir) {
+        const float3 w = cross3(xyz(pj[1]), xyz(pj[2]));
+        k = -(w.x * 0.5f * (b[0] + b[3]) + w.y * 0.5f * (b[1] + b[4]) + w.z * 0.5f * (b[2] + b[5]));
+    }
+    return k == k ? k : INFINITY;
+}
+
+// grid (slot blocks, lights): rank of slot i = how many slots come before it in the total
+// order (key, slot); order[l][rank] = i
+__global__ void __launch_bounds__(kSlBlock) k_sl_order(const float4 *__restrict__ proj, const float *__restrict__ box,
+                                                       uint32_t n_inst, uint16_t *__restrict__ order) {
+    __shared__ float keys[kSlBlock];
+    const uint32_t l = blockIdx.y, i = blockIdx.x * kSlBlock + threadIdx.x;
+    const float4 *pj = proj + (size_t)l * kShadowProj;
+    const float ki = i < n_inst ? order_key(pj, box + (size_t)i * 6) : 0.f;
+    uint32_t rank = 0;
+    for (uint32_t j0 = 0; j0 < n_inst; j0 += kSlBlock) {
+        const uint32_t j = j0 + threadIdx.x;
+        keys[threadIdx.x] = j < n_inst ? order_key(pj, box + (size_t)j * 6) : INFINITY;
+        __syncthreads();
+        const uint32_t m = min(kSlBlock, n_inst - j0);
+        for (uint32_t t = 0; t < m; t++) {
+            const float kj = keys[t];
+            rank += (kj < ki || (kj == ki && j0 + t < i)) ? 1u : 0u;
+        }
+        __syncthreads();
+    }
+    if (i < n_inst) order[(size_t)l * n_inst + rank] = (uint16_t)i;
+}
+
 __global__ void __launch_bounds__(kSlBlock) k_sl_fill(const float4 *__restrict__ proj, const uint32_t *__restrict__ rects,
                                                       const uint32_t *__restrict__ offsets,
                                                       const uint32_t *__restrict__ bsum,
                                                       const uint32_t *__restrict__ total, uint32_t n_inst,
-                                                      uint32_t n_lights, uint32_t capacity, uint2 *__restrict__ cells,
+                                                      uint32_t n_lights, uint32_t capacity,
+                                                      const uint16_t *__restrict__ order, uint2 *__restrict__ cells,
                                                       uint16_t *__restrict__ list) {
     const uint32_t gid = blockIdx.x * kSlBlock + threadIdx.x;
     if (gid <= n_inst) list[gid] = (uint16_t)gid;  // the identity list [0, n_inst), then the NaN box
@@ -365,8 +418,11 @@ __global__ void __launch_bounds__(kSlBlock) k_sl_fill(const float4 *__restrict__
     const uint32_t first = bsum[cell / kSlBlock] + offsets[cell];
     if (cell_coords(proj[(size_t)l * kShadowProj].x, c, face, u, v)) {
         const uint32_t *r = rects + (size_t)l * n_inst * 6 + face;
-        for (uint32_t i = 0; i < n_inst; i++)
+        const uint16_t *ord = order + (size_t)l * n_inst;
+        for (uint32_t t = 0; t < n_inst; t++) {
+            const uint32_t i = ord[t];
             if (in_rect(r[(size_t)i * 6], u, v)) list[first + k++] = (uint16_t)i;
+        }
     }
     cells[gid] = k ? make_uint2(first, first + k) : none;
 }
@@ -387,8 +443,10 @@ hipError_t launch_shadow_lists(const float *lights, uint32_t n_lights, const flo
     hipLaunchKernelGGL(k_sl_boxes, dim3((nb + kSlBlock - 1) / kSlBlock), dim3(kSlBlock), 0, st, inst_box, n_inst,
                        sl.boxes);
     hipLaunchKernelGGL(k_sl_scan, dim3(1), dim3(kSlBlock), 0, st, sl.bsum, cell_blocks, n_inst + 1u, sl.total);
+    hipLaunchKernelGGL(k_sl_order, dim3((n_inst + kSlBlock - 1) / kSlBlock, n_lights), dim3(kSlBlock), 0, st, sl.proj,
+                       inst_box, n_inst, sl.order);
     const uint32_t fill = std::max(recs, n_inst + 1u);
     hipLaunchKernelGGL(k_sl_fill, dim3((fill + kSlBlock - 1) / kSlBlock), dim3(kSlBlock), 0, st, sl.proj, sl.rects,
-                       sl.offsets, sl.bsum, sl.total, n_inst, n_lights, sl.capacity, sl.cells, sl.list);
+                       sl.offsets, sl.bsum, sl.total, n_inst, n_lights, sl.capacity, sl.order, sl.cells, sl.list);
     return hipGetLastError();
 }

@@ -123,6 +123,7 @@ struct ShadowLists {
     uint32_t *rects, *offsets, *bsum, *total;
     uint2 *cells;
     uint16_t *list;
+    uint16_t *order;  // per light, the slots nearest the light first ([lights][slots])
     uint32_t capacity;
 };
 // blocks of 256 cells that the build scans (the length of ShadowLists::bsum)

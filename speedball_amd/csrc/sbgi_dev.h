@@ -89,9 +89,9 @@ struct DevScene {
 // point of it lies in the cone of directions from the light that contains the hit (point
 // and spot lights), or on the line through the hit along the light direction (directional
 // lights). Cells of those directions (a cube map of kShadowFace^2 cells per face around the
-// light) or of that plane (a kShadowDirGrid^2 grid across the light direction) list, in
-// slot order, the instances whose padded world box reaches the cell; the walk tests that
-// list instead of the TLAS.
+// light) or of that plane (a kShadowDirGrid^2 grid across the light direction) list,
+// nearest the light first, the instances whose padded world box reaches the cell; the walk
+// tests that list instead of the TLAS.
 constexpr uint32_t kShadowFace = 16;
 constexpr uint32_t kShadowDirGrid = 32;
 constexpr uint32_t kShadowCells = 6 * kShadowFace * kShadowFace;

@@ -295,9 +295,7 @@ typedef struct sbgi_walk_tuning {
     uint32_t shadow_lists; /* 0 or 1: single-field ticks walk shadow rays over the
                               light-space candidate lists (SBGI_BUF_SHADOW_LIST) instead
                               of the TLAS; ignored when the field has no lists. Default
-                              (set by sbgi_set_scene): 1 when the scene's walk data
-                              (node copies and triangles) is at most 16 MiB, so that it
-                              stays cache-resident, else 0. Batches walk the TLAS.     */
+                              (set by sbgi_set_scene): 1. Batches walk the TLAS.       */
     uint32_t shadow_copies; /* 8 or 1: BLAS walk copies of shadow rays in MT_ORDERED /
                               WATERTIGHT fields: 8, the near-first copy of the ray's
                               direction octant (as closest-hit rays); 1, one copy for

@@ -284,24 +284,23 @@ int32_t build_shadow_lists(sbgi_field *f) {
 // entries the scene needs now (one host sync), so that later edits rarely outgrow it. A
 // scene whose build would cost more than kShadowListWork rectangle tests, or whose slots
 // do not fit 16 bits, walks its shadow rays over the TLAS.
-// Whether ticks take the lists (sbgi_walk_tuning::shadow_lists) defaults by the size of the
-// walk data: on the cache-resident instanced workload (2.5 MiB of node copies and
-// triangles) the lists, nearest the light first, cut the top-level steps of a shadow ray
-// from 37 to 5 and the walk time by a fifth; on the dynamic workload (320 MiB, BLAS reads
-// from HBM) they cut them from 31 to 5 but the update takes 6.5 % longer, the list reads
-// missing L2.
+// Ticks take the lists by default (sbgi_walk_tuning::shadow_lists): nearest the light
+// first, they cut the top-level steps of a shadow ray from 37 to 5 and the walk time by a
+// fifth on the cache-resident instanced workload (2.5 MiB of node copies and triangles),
+// and the frame by 3.5 % on the dynamic workload (320 MiB, BLAS reads from HBM; their
+// rebuild every frame included), where in slot order they had cost 6.5 %.
 constexpr uint64_t kShadowListWork = 1ull << 28;
-constexpr size_t kShadowListScene = 16ull << 20;
+constexpr size_t kShadowCopyScene = 16ull << 20;
 int32_t size_shadow_lists(sbgi_field *f) {
     free_shadow_lists(f);
     const sbgi_scene_view &v = f->view;
     const size_t walk_bytes = (size_t)v.node_count * 32 * (ordered(f) ? 8 : 1) + (size_t)v.tri_count * 48;
-    f->tuning.shadow_lists = walk_bytes <= kShadowListScene ? 1u : 0u;
+    f->tuning.shadow_lists = 1u;
     // past it, shadow rays also walk one BLAS copy instead of their octant's near-first copy:
     // a shadow ray needs any occluder, and on the dynamic workload (8 copies: 269 MiB) its
     // walks take 8.49 instead of 8.82 ms a frame; on the cache-resident instanced workload
     // one copy costs the shadow walk 6 %
-    f->tuning.shadow_copies = walk_bytes <= kShadowListScene ? 8u : 1u;
+    f->tuning.shadow_copies = walk_bytes <= kShadowCopyScene ? 8u : 1u;
     const uint64_t work = (uint64_t)v.light_count * kShadowCells * v.inst_count;
     if ((f->desc.flags & SBGI_FIELD_NO_SHADOW_LISTS) || v.light_count == 0 || v.inst_count == 0 ||
         v.inst_count > 65535u || work > kShadowListWork)

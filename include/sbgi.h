@@ -288,9 +288,9 @@ typedef struct sbgi_stats {
  * boxes cull a shadow ray (its list's instance boxes instead of TLAS nodes), so only a ray
  * that grazes one of those boxes could get another answer; the GPU tests find none. */
 typedef struct sbgi_walk_tuning {
-    uint32_t tri_ratio;   /* 1..64, default 4 */
+    uint32_t tri_ratio;   /* 1..64, default 2 */
     uint32_t inst_ratio;  /* 1..64, default 4 */
-    uint32_t refill_min;  /* 1..64, default 8 */
+    uint32_t refill_min;  /* 1..64, default 16 */
     uint32_t fetch_chunk; /* 1..1024, default 64 */
     uint32_t shadow_lists; /* 0 or 1: single-field ticks walk shadow rays over the
                               light-space candidate lists (SBGI_BUF_SHADOW_LIST) instead

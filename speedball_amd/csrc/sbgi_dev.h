@@ -165,7 +165,7 @@ struct DevCascade {
 constexpr uint32_t kFetchHeads = 64;
 constexpr uint32_t kHeadStride = 32;
 // default wave schedule of the walks (DevScene::tri_ratio ... fetch_chunk)
-constexpr uint32_t kDefaultTriRatio = 4, kDefaultInstRatio = 4, kDefaultRefillMin = 8, kDefaultFetchChunk = 64;
+constexpr uint32_t kDefaultTriRatio = 2, kDefaultInstRatio = 4, kDefaultRefillMin = 16, kDefaultFetchChunk = 64;
 
 // Traversal counters of counting ticks: [0..3] closest-hit walk {BLAS nodes, TLAS nodes,
 // triangle tests, instance entries}, [4] shadow rays emitted, [5..8] the shadow walk's

@@ -366,7 +366,7 @@ def test_walk_tuning_rejects_out_of_range(instanced_small):
             with pytest.raises(RuntimeError):
                 f.set_walk_tuning(**bad)
         # defaults; the lists are on for this cache-resident scene
-        assert f.walk_tuning() == dict(tri_ratio=4, inst_ratio=4, refill_min=8, fetch_chunk=64, shadow_lists=1,
+        assert f.walk_tuning() == dict(tri_ratio=2, inst_ratio=4, refill_min=16, fetch_chunk=64, shadow_lists=1,
                                        shadow_copies=8)
     finally:
         f.close()
